@@ -1,0 +1,180 @@
+/*
+ * magudi_hip.h -- C ABI of the MI355X implementation of magudi's right-hand-side
+ * hot path (forward RHS, discrete-adjoint RHS, RK4, HBM checkpointing).
+ *
+ * Plain C: pointers, sizes and an int status.  0 = success, < 0 = error
+ * (mgd_last_error() gives the message), > 0 = solution-limit style conditions.
+ * Host arrays use magudi's layout x(nGridPoints, nComponents): component-major,
+ * grid index i fastest.  Device buffers are owned by the region and stay
+ * resident in HBM across RK4 stages; host copies happen only in put/get calls.
+ *
+ * Each entry point names the magudi interface it replaces (paths relative to
+ * the magudi source tree).  Fortran hosts bind these with ISO_C_BINDING.
+ * A region is not thread-safe; use one per host thread.
+ */
+#ifndef MAGUDI_HIP_H
+#define MAGUDI_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGD_OK 0
+#define MGD_ERR_ARG (-1)
+#define MGD_ERR_HIP (-2)
+#define MGD_ERR_UNSUPPORTED (-3)
+#define MGD_ERR_STATE (-4)
+
+/* Region%computeRhs modes (include/Region.f90:3-12) */
+#define MGD_FORWARD 1
+#define MGD_ADJOINT (-1)
+
+/* Patch types: bc.dat "Type" column (src/PatchFactoryImpl.f90:41-82) */
+#define MGD_PATCH_FAR_FIELD 1
+#define MGD_PATCH_SPONGE 2
+#define MGD_PATCH_ACTUATOR 3
+#define MGD_PATCH_COST_TARGET 4
+#define MGD_PATCH_KOLMOGOROV 5
+
+/* Grid periodicity (Grid_enum NONE / PLANE / OVERLAP) */
+#define MGD_PERIODIC_NONE 0
+#define MGD_PERIODIC_PLANE 1
+#define MGD_PERIODIC_OVERLAP 2
+
+/* State arrays for put/get */
+#define MGD_CONSERVED 0
+#define MGD_ADJOINT_VARS 1
+#define MGD_RHS 2
+#define MGD_TARGET 3
+
+/* Fields for mgd_region_set_field */
+#define MGD_FIELD_TARGET_STATE 0
+#define MGD_FIELD_CONTROL_MOLLIFIER 1
+#define MGD_FIELD_TARGET_MOLLIFIER 2
+
+/* Stencil operators for mgd_stencil_apply (t_Grid members, include/Grid.f90) */
+#define MGD_OP_FIRST_DERIVATIVE 0
+#define MGD_OP_ADJOINT_FIRST_DERIVATIVE 1
+#define MGD_OP_DISSIPATION 2
+#define MGD_OP_DISSIPATION_TRANSPOSE 3
+
+/* Solver options and simulation flags a region is built from
+ * (src/SolverOptionsImpl.f90:3-140, src/SimulationFlagsImpl.f90:3-70, grid options
+ * of src/GridImpl.f90:142-291). */
+typedef struct {
+  int nd;                       /* 1, 2 or 3 */
+  int n[3];                     /* global grid size */
+  int periodicity[3];           /* MGD_PERIODIC_* */
+  double periodic_length[3];
+  int scheme;                   /* 0: "SBP 4-8" (defaults/discretization_scheme) */
+  int viscous;                  /* include_viscous_terms */
+  int dissipation;              /* add_dissipation */
+  int composite_dissipation;    /* composite_dissipation */
+  double gamma;                 /* ratio_of_specific_heats */
+  double reynolds;              /* Reynolds_number (acoustic) */
+  double prandtl;               /* Prandtl_number */
+  double power_law;             /* viscosity_power_law_exponent */
+  double bulk_ratio;            /* bulk_viscosity_ratio */
+  double dissipation_amount;    /* dissipation_amount */
+  int n_instances;              /* independent states sharing grid and patches (batching) */
+} mgd_region_desc;
+
+/* One bc.dat row plus its patch options.  p[]: far field (inviscid_penalty_amount,
+ * viscous_penalty_amount); sponge (sponge_amount, sponge_exponent); Kolmogorov
+ * (wavenumber, amplitude). */
+typedef struct {
+  int type;
+  int normal_dir;
+  int extent[6];                /* iMin iMax jMin jMax kMin kMax, 1-based, < 0 from end */
+  double p[4];
+} mgd_patch_desc;
+
+typedef struct mgd_region mgd_region;
+
+/* Last error message of the calling thread. */
+const char* mgd_last_error(void);
+
+/* Number of HIP devices visible (0 when no GPU). Calls no kernel. */
+int mgd_device_count(int* count);
+
+/* Region%setup + Grid%setupSpatialDiscretization + Grid%update
+ * (src/RegionImpl.f90:1225-1339, src/GridImpl.f90:487-1065).  coords: nd x N
+ * host array of a rectilinear grid.  Replaces t_Region%setup. */
+int mgd_region_create(const mgd_region_desc* desc, const double* coords, mgd_region** out);
+int mgd_region_destroy(mgd_region* r);
+
+/* t_PatchFactory%connect + t_Patch%setup for one bc.dat row (src/PatchFactoryImpl.f90). */
+int mgd_region_add_patch(mgd_region* r, const mgd_patch_desc* pd);
+
+/* Target state (N x nv) and control / target mollifiers (N): Solver%setup inputs
+ * (src/SolverImpl.f90:505-560). */
+int mgd_region_set_field(mgd_region* r, int which, const double* host);
+
+/* t_AcousticSource%setup (src/AcousticSourceImpl.f90:3-32) */
+int mgd_region_add_source(mgd_region* r, const double* location, double amplitude,
+                          double frequency, double radius, double phase);
+
+/* Remainder of Solver%setup: sponge strengths (computeSpongeStrengths,
+ * src/PatchFactoryImpl.f90:161-374), far-field penalties and A+ blocks
+ * (src/FarFieldPatchImpl.f90:3-74), mean pressure (src/AcousticNoiseImpl.f90:3-71). */
+int mgd_region_finalize(mgd_region* r);
+
+/* Copies a state array of one instance (MGD_CONSERVED / MGD_ADJOINT_VARS / MGD_RHS)
+ * between host and HBM.  Replaces Region%loadData / saveData for in-memory states. */
+int mgd_state_put(mgd_region* r, int which, int instance, const double* host);
+int mgd_state_get(mgd_region* r, int which, int instance, double* host);
+
+/* Region%computeRhs(mode) (src/RegionImpl.f90:1877-2027) for all instances into the
+ * region's rhs buffer.  time: State%time for acoustic sources.  control: actuator
+ * forcing per instance per actuator patch point (host, may be NULL).  adjoint
+ * forcing of the cost target is taken from the last mgd_update_adjoint_forcing,
+ * scaled by forcing_factor (State%adjointForcingFactor). */
+int mgd_compute_rhs(mgd_region* r, int mode, double time, const double* control,
+                    double forcing_factor);
+
+/* Functional%updateAdjointForcing (src/FunctionalImpl.f90:37-91) on the conserved state;
+ * zero=1 clears it (the final adjoint step). */
+int mgd_update_adjoint_forcing(mgd_region* r, int zero);
+
+/* AcousticNoise%compute (src/AcousticNoiseImpl.f90:123-206) per instance. */
+int mgd_functional(mgd_region* r, double* out);
+
+/* Solver%runForward with RK4 (src/SolverImpl.f90:672-912, src/RK4IntegratorImpl.f90:65-162)
+ * from the current conserved state.  control: per instance, per stage (nsteps*4,
+ * forward order), per actuator point, or NULL.  J: per instance cost functional.
+ * use_graph: replay one captured RK4 step per time step. */
+int mgd_run_forward(mgd_region* r, double t0, double dt, int nsteps, const double* control,
+                    double* J, int use_graph);
+
+/* Solver%runAdjoint with uniform checkpointing held in HBM
+ * (src/SolverImpl.f90:914-1245, src/UniformCheckpointerImpl.f90:78-208,
+ * src/RK4IntegratorImpl.f90:164-270) starting from the current conserved state as
+ * the initial condition: runs the forward pass (storing a snapshot every
+ * save_interval steps), then marches the adjoint back with window recompute.
+ * gradient: per instance, per stage (forward order), per actuator point (may be NULL);
+ * sensitivity and J: per instance.  The adjoint state at t0 stays in the region
+ * (MGD_ADJOINT_VARS). */
+int mgd_run_adjoint(mgd_region* r, double t0, double dt, int nsteps, int save_interval,
+                    const double* control, double* gradient, double* sensitivity, double* J,
+                    int use_graph);
+
+/* t_StencilOperator%apply (include/StencilOperator.f90:19-27,
+ * src/StencilOperatorImpl.f90:2374-2410) of one of the region's operators along
+ * direction dir (0-based) to ncomp host fields of size N, in place. */
+int mgd_stencil_apply(mgd_region* r, int op, int dir, double* host, int ncomp);
+
+/* Grid arrays after Grid%update as magudi stores them (for checks):
+ * 0 metrics (N x nd^2), 1 jacobian (N), 2 norm (N), 3 arcLengths (N x nd). */
+int mgd_region_get_grid(mgd_region* r, int which, double* host);
+
+/* Time (ms) of the last run's kernels on the region stream, split by kernel family:
+ * out[0] total, out[1] forward RHS, out[2] adjoint RHS; counts in launches[]. */
+int mgd_last_timings(mgd_region* r, double* out, int* launches);
+
+/* Waits for all work of the region. */
+int mgd_sync(mgd_region* r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

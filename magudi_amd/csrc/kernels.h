@@ -1,0 +1,34 @@
+// kernels.h -- launchers of the magudi RHS kernels (kernels.hip, fused2d.hip).
+#pragma once
+#include "mgd_internal.h"
+
+namespace mgd {
+
+constexpr int kBlock = 256;
+
+void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
+                        double* G, double* t, const double* control, long control_stride,
+                        const Source* src, int nsrc, const double* src_amp, hipStream_t st);
+void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
+                        double* rhs, double* w, double* G, double* t, double* T1, double* S,
+                        const double* aforce, long aforce_stride, double forcing_factor,
+                        int has_farfield, hipStream_t st);
+void launch_rk_forward(long M, int stage, double dt, double* q, double* b1, double* b2,
+                       const double* rhs, double* qcopy, hipStream_t st);
+void launch_rk_adjoint(long M, int stage, double dt, double* lam, double* b1, double* b2,
+                       const double* rhs, hipStream_t st);
+void launch_copy(long M, const double* a, double* b, hipStream_t st);
+void launch_fill(long M, double v, double* b, hipStream_t st);
+int reduce_blocks(int np);
+void launch_patch_reduce(const Geo& g, const DPatch* pt, int np, int which, const double* x,
+                         double* partial, double weight, double* acc, double* raw, hipStream_t st);
+void launch_adj_forcing(const Geo& g, const DPatch* pt, int np, const double* q, double* out,
+                        int zero, hipStream_t st);
+void launch_adj_terminal(const Geo& g, const DPatch* pt, int np, const double* af, double factor,
+                         double* lam, hipStream_t st);
+void launch_gradient(const Geo& g, const DPatch* pt, int np, const double* lam, double* out,
+                     long inst_stride, hipStream_t st);
+void launch_apply(const Geo& g, int slot, int d, int ncomp, const double* x, double* y,
+                  hipStream_t st);
+
+}  // namespace mgd

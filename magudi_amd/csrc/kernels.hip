@@ -1,0 +1,838 @@
+// kernels.hip -- generic ("plain path") kernels of the magudi RHS on gfx950.
+//
+// One thread per grid point, instance in blockIdx.y.  Stencil operands are read
+// through the cache hierarchy; every nd (1-3), inviscid and viscous, periodic and
+// bounded directions are covered.  These are the reference-shaped building blocks;
+// the fused 2-D kernels in fused2d.hip replace the hot ones for the benchmark path.
+#include "device.h"
+#include "kernels.h"
+
+namespace mgd {
+
+__device__ __forceinline__ long gidx() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
+
+// ------------------------------------------------------------------ state helpers
+template <int ND>
+__device__ __forceinline__ void load_q(const double* q, long N, long p, double* v) {
+  for (int l = 0; l < ND + 2; ++l) v[l] = q[l * N + p];
+}
+
+// Primitive variables needed by the viscous terms: w = (u_1..u_nd, T).
+template <int ND>
+__global__ void k_prim(Geo g, const double* __restrict__ q, double* __restrict__ w) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  const double* qi = q + inst * g.nv * g.N;
+  double* wi = w + inst * (ND + 1) * g.N;
+  double v[ND + 2];
+  load_q<ND>(qi, g.N, p, v);
+  const Prim<ND> s = prim<ND>(v, g.gamma);
+  for (int i = 0; i < ND; ++i) wi[i * g.N + p] = s.u[i];
+  wi[ND * g.N + p] = s.T;
+}
+
+// Gradient of w along direction d: G[(c*ND + d)] = jacobian * metric_dd * D_d w_c
+// (Grid%computeGradient, src/GridImpl.f90:1172-1421, non-curvilinear).
+template <int ND>
+__global__ void k_grad(Geo g, int d, const double* __restrict__ w, double* __restrict__ G) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  const double* wi = w + inst * (ND + 1) * g.N;
+  double* Gi = G + inst * (ND + 1) * ND * g.N;
+  int ijk[3] = {0, 0, 0};
+  unflat<ND>(g, p, ijk);
+  const Dir dd = g.dir[d];
+  const long base = p - (long)ijk[d] * dd.stride;
+  double out[ND + 1];
+  sbp_apply<ND + 1>(g.ops[SLOT_FIRST], dd, ijk[d],
+                    [&](int idx, double* v) {
+                      const long pp = base + (long)idx * dd.stride;
+                      for (int c = 0; c <= ND; ++c) v[c] = wi[c * g.N + pp];
+                    },
+                    out);
+  const double jm = jacobian<ND>(g, ijk) * metric<ND>(g, d, ijk);
+  for (int c = 0; c <= ND; ++c) Gi[(c * ND + d) * g.N + p] = jm * out[c];
+}
+
+// Stress tensor and heat flux at one point from the stored gradients.
+template <int ND>
+__device__ __forceinline__ void visc_point(const Geo& g, const double* Gi, long N, long p,
+                                           const Prim<ND>& s, double* tau, double* hf,
+                                           double* mu, double* la, double* ka) {
+  transport(g, s.T, mu, la, ka);
+  for (int i = 0; i < ND; ++i)
+    for (int j = 0; j < ND; ++j) tau[i * ND + j] = Gi[(i * ND + j) * N + p];
+  stress<ND>(tau, *mu, *la);
+  for (int j = 0; j < ND; ++j) hf[j] = -*ka * Gi[(ND * ND + j) * N + p];
+}
+
+// Contravariant total flux of direction d at point pp (inviscid minus viscous).
+template <int ND>
+__device__ __forceinline__ void flux_hat(const Geo& g, const double* qi, const double* Gi, long pp,
+                                         int d, double* f) {
+  double v[ND + 2];
+  load_q<ND>(qi, g.N, pp, v);
+  const Prim<ND> s = prim<ND>(v, g.gamma);
+  inviscid_flux<ND>(v, s, d, f);
+  if (g.viscous) {
+    double tau[ND * ND], hf[ND], mu, la, ka, fv[ND + 2];
+    visc_point<ND>(g, Gi, g.N, pp, s, tau, hf, &mu, &la, &ka);
+    viscous_flux<ND>(s.u, tau, hf, d, fv);
+    for (int l = 0; l < ND + 2; ++l) f[l] = f[l] - fv[l];
+  }
+  int ijk[3] = {0, 0, 0};
+  unflat<ND>(g, pp, ijk);
+  const double m = metric<ND>(g, d, ijk);
+  for (int l = 0; l < ND + 2; ++l) f[l] = m * f[l];
+}
+
+// rhs accumulation of -sum_d D_d Fhat_d (computeRhsForward, src/RhsHelperImpl.f90:303-344):
+// d == 0 stores D_0 F_0; later directions add; the last one forms 0 - sum.
+template <int ND>
+__global__ void k_flux_div(Geo g, int d, const double* __restrict__ q, const double* __restrict__ G,
+                           double* __restrict__ rhs) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  const double* qi = q + inst * g.nv * g.N;
+  const double* Gi = G ? G + inst * (ND + 1) * ND * g.N : nullptr;
+  double* ri = rhs + inst * g.nv * g.N;
+  int ijk[3] = {0, 0, 0};
+  unflat<ND>(g, p, ijk);
+  const Dir dd = g.dir[d];
+  const long base = p - (long)ijk[d] * dd.stride;
+  double out[ND + 2];
+  sbp_apply<ND + 2>(g.ops[SLOT_FIRST], dd, ijk[d],
+                    [&](int idx, double* f) { flux_hat<ND>(g, qi, Gi, base + (long)idx * dd.stride, d, f); },
+                    out);
+  for (int l = 0; l < ND + 2; ++l) {
+    double s = d == 0 ? out[l] : ri[l * g.N + p] + out[l];
+    if (d == ND - 1) s = 0.0 - s;
+    ri[l * g.N + p] = s;
+  }
+}
+
+// Dissipation, first half: t = D4_d x, scaled by -arcLength_d unless composite
+// (addDissipation, src/RhsHelperImpl.f90:58-77).  With composite dissipation the
+// term is added to rhs directly.
+template <int ND>
+__global__ void k_diss_a(Geo g, int d, const double* __restrict__ x, double* __restrict__ t,
+                         double* __restrict__ rhs, double amount) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  const double* xi = x + inst * g.nv * g.N;
+  int ijk[3] = {0, 0, 0};
+  unflat<ND>(g, p, ijk);
+  const Dir dd = g.dir[d];
+  const long base = p - (long)ijk[d] * dd.stride;
+  double out[ND + 2];
+  sbp_apply<ND + 2>(g.ops[SLOT_DISS], dd, ijk[d],
+                    [&](int idx, double* v) { load_q<ND>(xi, g.N, base + (long)idx * dd.stride, v); },
+                    out);
+  if (g.composite) {
+    double* ri = rhs + inst * g.nv * g.N;
+    for (int l = 0; l < ND + 2; ++l) ri[l * g.N + p] = ri[l * g.N + p] + amount * out[l];
+    return;
+  }
+  const double arc = fabs(metric<ND>(g, d, ijk));
+  double* ti = t + inst * g.nv * g.N;
+  for (int l = 0; l < ND + 2; ++l) ti[l * g.N + p] = -arc * out[l];
+}
+
+// Dissipation, second half: rhs += amount * H_d^-1 D4_d^T t.
+template <int ND>
+__global__ void k_diss_b(Geo g, int d, const double* __restrict__ t, double* __restrict__ rhs,
+                         double amount) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  const double* ti = t + inst * g.nv * g.N;
+  double* ri = rhs + inst * g.nv * g.N;
+  int ijk[3] = {0, 0, 0};
+  unflat<ND>(g, p, ijk);
+  const Dir dd = g.dir[d];
+  const long base = p - (long)ijk[d] * dd.stride;
+  double out[ND + 2];
+  sbp_apply<ND + 2>(g.ops[SLOT_DISS_T], dd, ijk[d],
+                    [&](int idx, double* v) { load_q<ND>(ti, g.N, base + (long)idx * dd.stride, v); },
+                    out);
+  const double h = g.hw[d][ijk[d]];
+  for (int l = 0; l < ND + 2; ++l) {
+    const double tl = out[l] / h;
+    ri[l * g.N + p] = ri[l * g.N + p] + amount * tl;
+  }
+}
+
+// Adjoint: T1_d = D+_d lambda for all nv components (src/RhsHelperImpl.f90:410-415);
+// with comp0 = 1 and nc = nv-1 it applies to the viscous diffusion block instead.
+template <int ND>
+__global__ void k_adj_deriv(Geo g, int d, const double* __restrict__ x, long xstride_inst,
+                            double* __restrict__ T1, int comp0) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  int ijk[3] = {0, 0, 0};
+  unflat<ND>(g, p, ijk);
+  const Dir dd = g.dir[d];
+  const long base = p - (long)ijk[d] * dd.stride;
+  constexpr int NV = ND + 2;
+  const double* xi = x + inst * xstride_inst;
+  double* Ti = T1 + (inst * ND + d) * NV * g.N;
+  if (comp0 == 0) {
+    double out[NV];
+    sbp_apply<NV>(g.ops[SLOT_ADJ], dd, ijk[d],
+                  [&](int idx, double* v) {
+                    const long pp = base + (long)idx * dd.stride;
+                    for (int l = 0; l < NV; ++l) v[l] = xi[l * g.N + pp];
+                  },
+                  out);
+    for (int l = 0; l < NV; ++l) Ti[l * g.N + p] = out[l];
+  } else {
+    // operand is T1_d itself (components 1..nv-1), written to the scratch slot x
+    double out[NV - 1];
+    const double* src = xi + (long)d * NV * g.N;
+    sbp_apply<NV - 1>(g.ops[SLOT_ADJ], dd, ijk[d],
+                      [&](int idx, double* v) {
+                        const long pp = base + (long)idx * dd.stride;
+                        for (int l = 0; l < NV - 1; ++l) v[l] = src[(l + 1) * g.N + pp];
+                      },
+                      out);
+    for (int l = 0; l < NV - 1; ++l) Ti[(l + 1) * g.N + p] = out[l];
+  }
+}
+
+// Adjoint: rhs = sum_d (A_d - Av_d)^T T1_d at each point (src/RhsHelperImpl.f90:431-491).
+template <int ND>
+__global__ void k_adj_jac(Geo g, const double* __restrict__ q, const double* __restrict__ G,
+                          const double* __restrict__ T1, double* __restrict__ rhs) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  constexpr int NV = ND + 2;
+  const double* qi = q + inst * NV * g.N;
+  const double* Ti = T1 + inst * ND * NV * g.N;
+  double* ri = rhs + inst * NV * g.N;
+  double v[NV];
+  load_q<ND>(qi, g.N, p, v);
+  const Prim<ND> s = prim<ND>(v, g.gamma);
+  int ijk[3] = {0, 0, 0};
+  unflat<ND>(g, p, ijk);
+  double tau[ND * ND], hf[ND], mu = 0, la = 0, ka = 0;
+  if (g.viscous) visc_point<ND>(g, G + inst * (ND + 1) * ND * g.N, g.N, p, s, tau, hf, &mu, &la, &ka);
+  double acc[NV];
+  for (int l = 0; l < NV; ++l) acc[l] = 0.0;
+  for (int d = 0; d < ND; ++d) {
+    double m[ND], A[NV * NV];
+    for (int i = 0; i < ND; ++i) m[i] = i == d ? metric<ND>(g, d, ijk) : 0.0;
+    inviscid_jacobian<ND>(m, s, g.gamma, A);
+    if (g.viscous) {
+      double B[NV * NV];
+      first_partial_viscous<ND>(m, tau, hf, g.power_law, g.gamma, s, B);
+      for (int k = 0; k < NV * NV; ++k) A[k] = A[k] - B[k];
+    }
+    for (int i = 0; i < NV; ++i) {
+      double t = 0.0;
+      for (int k = 0; k < NV; ++k) t += A[i * NV + k] * Ti[(d * NV + k) * g.N + p];
+      acc[i] = acc[i] + t;
+    }
+  }
+  for (int l = 0; l < NV; ++l) ri[l * g.N + p] = acc[l];
+}
+
+// Viscous adjoint, second-partial block: overwrites components 1..nv-1 of T1_j with
+// sum_i B_ij^T T1_i (src/RhsHelperImpl.f90:509-553).
+template <int ND>
+__global__ void k_adj_visc2(Geo g, const double* __restrict__ q, double* __restrict__ T1) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  constexpr int NV = ND + 2, NW = ND + 1;
+  const double* qi = q + inst * NV * g.N;
+  double* Ti = T1 + inst * ND * NV * g.N;
+  double v[NV];
+  load_q<ND>(qi, g.N, p, v);
+  const Prim<ND> s = prim<ND>(v, g.gamma);
+  double mu, la, ka;
+  transport(g, s.T, &mu, &la, &ka);
+  int ijk[3] = {0, 0, 0};
+  unflat<ND>(g, p, ijk);
+  const double jac = jacobian<ND>(g, ijk);
+  double diff[ND][NW];
+  for (int j = 0; j < ND; ++j)
+    for (int a = 0; a < NW; ++a) diff[j][a] = 0.0;
+  for (int j = 0; j < ND; ++j) {
+    double m2[ND];
+    for (int i = 0; i < ND; ++i) m2[i] = i == j ? metric<ND>(g, j, ijk) : 0.0;
+    for (int i = 0; i < ND; ++i) {
+      double m1[ND], B[NW * NW];
+      for (int k = 0; k < ND; ++k) m1[k] = k == i ? metric<ND>(g, i, ijk) : 0.0;
+      second_partial_viscous<ND>(s.u, mu, la, ka, jac, m1, m2, B);
+      for (int a = 0; a < NW; ++a) {
+        double t = 0.0;
+        for (int k = 0; k < NW; ++k) t += B[a * NW + k] * Ti[(i * NV + 1 + k) * g.N + p];
+        diff[j][a] = diff[j][a] + t;
+      }
+    }
+  }
+  for (int j = 0; j < ND; ++j)
+    for (int a = 0; a < NW; ++a) Ti[(j * NV + 1 + a) * g.N + p] = diff[j][a];
+}
+
+// Viscous adjoint chain rule (src/RhsHelperImpl.f90:555-570 with sign -1 and
+// :227-236 with sign +1): t2 = sum_j S_j (components 1..nv-1).
+template <int ND>
+__global__ void k_adj_chain(Geo g, const double* __restrict__ q, const double* __restrict__ S,
+                            double* __restrict__ rhs, double sign) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  constexpr int NV = ND + 2, NW = ND + 1;
+  const double* qi = q + inst * NV * g.N;
+  const double* Si = S + inst * ND * NV * g.N;
+  double* ri = rhs + inst * NV * g.N;
+  double v[NV];
+  load_q<ND>(qi, g.N, p, v);
+  const Prim<ND> s = prim<ND>(v, g.gamma);
+  double t2[NW];
+  for (int a = 0; a < NW; ++a) {
+    double x = Si[(1 + a) * g.N + p];
+    for (int j = 1; j < ND; ++j) x = x + Si[(j * NV + 1 + a) * g.N + p];
+    t2[a] = x;
+  }
+  t2[ND] = g.gamma * s.sv * t2[ND];
+  for (int i = 0; i < ND; ++i) t2[i] = s.sv * t2[i] - s.u[i] * t2[ND];
+  for (int l = 1; l < NV; ++l)
+    ri[l * g.N + p] = sign > 0 ? ri[l * g.N + p] + t2[l - 1] : ri[l * g.N + p] - t2[l - 1];
+  double su = s.u[0] * t2[0];
+  for (int i = 1; i < ND; ++i) su = su + s.u[i] * t2[i];
+  const double a = s.sv * v[ND + 1] * t2[ND];
+  ri[p] = sign > 0 ? ri[p] - a - su : ri[p] + a + su;
+}
+
+// Viscous far-field adjoint penalty, first half (src/RhsHelperImpl.f90:170-210):
+// S_l -= sigma_v * B(dir, l)^T lambda on far-field patch points; S zeroed beforehand.
+template <int ND>
+__global__ void k_ff_visc_adj(Geo g, const Patches* __restrict__ P, const double* __restrict__ q,
+                              const double* __restrict__ lam, double* __restrict__ S) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  constexpr int NV = ND + 2, NW = ND + 1;
+  int ijk[3] = {0, 0, 0};
+  unflat<ND>(g, p, ijk);
+  const double* qi = q + inst * NV * g.N;
+  const double* li = lam + inst * NV * g.N;
+  double* Si = S + inst * ND * NV * g.N;
+  for (int ip = 0; ip < P->count; ++ip) {
+    const DPatch& pt = P->p[ip];
+    if (pt.type != MGD_PATCH_FAR_FIELD) continue;
+    bool in = true;
+    for (int d = 0; d < ND; ++d) in = in && ijk[d] >= pt.lo[d] && ijk[d] <= pt.hi[d];
+    if (!in) continue;
+    double v[NV];
+    load_q<ND>(qi, g.N, p, v);
+    const Prim<ND> s = prim<ND>(v, g.gamma);
+    double mu, la, ka;
+    transport(g, s.T, &mu, &la, &ka);
+    const int dir = abs(pt.normal_dir) - 1;
+    const double jac = jacobian<ND>(g, ijk);
+    double m1[ND];
+    for (int k = 0; k < ND; ++k) m1[k] = k == dir ? metric<ND>(g, dir, ijk) : 0.0;
+    for (int l = 0; l < ND; ++l) {
+      double m2[ND], B[NW * NW];
+      for (int k = 0; k < ND; ++k) m2[k] = k == l ? metric<ND>(g, l, ijk) : 0.0;
+      second_partial_viscous<ND>(s.u, mu, la, ka, jac, m1, m2, B);
+      for (int a = 0; a < NW; ++a) {
+        double t = 0.0;
+        for (int k = 0; k < NW; ++k) t += B[a * NW + k] * li[(1 + k) * g.N + p];
+        Si[(l * NV + 1 + a) * g.N + p] = Si[(l * NV + 1 + a) * g.N + p] - pt.sigma_visc * t;
+      }
+    }
+  }
+}
+
+// Region%computeRhs tail (src/RegionImpl.f90:1969-2023): multiply by the Jacobian, add
+// patch terms in bc.dat order, then acoustic sources (forward mode).
+template <int ND>
+__global__ void k_finish(Geo g, int mode, const Patches* __restrict__ P, const double* __restrict__ q,
+                         const double* __restrict__ lam, const double* __restrict__ G,
+                         double* __restrict__ rhs, const double* __restrict__ control,
+                         long control_inst_stride, const double* __restrict__ aforce,
+                         long aforce_inst_stride, double forcing_factor, const Source* __restrict__ src,
+                         int nsrc, const double* __restrict__ src_amp) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  constexpr int NV = ND + 2;
+  const double* qi = q + inst * NV * g.N;
+  const double* li = lam ? lam + inst * NV * g.N : nullptr;
+  double* ri = rhs + inst * NV * g.N;
+  int ijk[3] = {0, 0, 0};
+  unflat<ND>(g, p, ijk);
+  const double jac = jacobian<ND>(g, ijk);
+  double r[NV];
+  for (int l = 0; l < NV; ++l) r[l] = ri[l * g.N + p] * jac;
+  for (int ip = 0; ip < P->count; ++ip) {
+    const DPatch& pt = P->p[ip];
+    bool in = true;
+    for (int d = 0; d < ND; ++d) in = in && ijk[d] >= pt.lo[d] && ijk[d] <= pt.hi[d];
+    if (!in) continue;
+    int k = ijk[0] - pt.lo[0];
+    if (ND > 1) k += pt.ls[0] * (ijk[1] - pt.lo[1]);
+    if (ND > 2) k += pt.ls[0] * pt.ls[1] * (ijk[2] - pt.lo[2]);
+    switch (pt.type) {
+      case MGD_PATCH_FAR_FIELD: {
+        // addFarFieldPenalty (src/FarFieldPatchImpl.f90:93-286); A+ from the target state
+        const double* A = pt.aplus + (long)k * NV * NV;
+        const double* qt = pt.aplus + (long)pt.np * NV * NV + (long)k * NV;
+        const double sj = pt.sigma_inv * jac;
+        if (mode == MGD_FORWARD) {
+          for (int i = 0; i < NV; ++i) {
+            double s = 0.0;
+            for (int c = 0; c < NV; ++c) s += A[c * NV + i] * (qi[c * g.N + p] - qt[c]);
+            r[i] = r[i] - sj * s;
+          }
+          if (g.viscous) {
+            double v[NV];
+            load_q<ND>(qi, g.N, p, v);
+            const Prim<ND> s = prim<ND>(v, g.gamma);
+            double tau[ND * ND], hf[ND], mu, la, ka;
+            visc_point<ND>(g, G + inst * (ND + 1) * ND * g.N, g.N, p, s, tau, hf, &mu, &la, &ka);
+            const int dir = abs(pt.normal_dir) - 1;
+            const double vj = pt.sigma_visc * jac;
+            for (int l = 0; l < NV; ++l) {
+              double acc = 0.0;
+              for (int d = 0; d < ND; ++d) {
+                double fv[NV];
+                viscous_flux<ND>(s.u, tau, hf, d, fv);
+                const double md = d == dir ? metric<ND>(g, dir, ijk) : 0.0;
+                acc += (fv[l] - pt.tvflux[((long)d * NV + l) * pt.np + k]) * md;
+              }
+              r[l] = r[l] + vj * acc;
+            }
+          }
+        } else {
+          for (int i = 0; i < NV; ++i) {
+            double s = 0.0;
+            for (int c = 0; c < NV; ++c) s += A[i * NV + c] * li[c * g.N + p];
+            r[i] = r[i] + sj * s;
+          }
+          if (g.viscous) {
+            double v[NV];
+            load_q<ND>(qi, g.N, p, v);
+            const Prim<ND> s = prim<ND>(v, g.gamma);
+            double tau[ND * ND], hf[ND], mu, la, ka, B[NV * NV], m[ND];
+            visc_point<ND>(g, G + inst * (ND + 1) * ND * g.N, g.N, p, s, tau, hf, &mu, &la, &ka);
+            const int dir = abs(pt.normal_dir) - 1;
+            for (int d = 0; d < ND; ++d) m[d] = d == dir ? metric<ND>(g, dir, ijk) : 0.0;
+            first_partial_viscous<ND>(m, tau, hf, g.power_law, g.gamma, s, B);
+            const double vj = pt.sigma_visc * jac;
+            for (int i = 0; i < NV; ++i) {
+              double t = 0.0;
+              for (int c = 0; c < NV; ++c) t += B[i * NV + c] * li[c * g.N + p];
+              r[i] = r[i] - vj * t;
+            }
+          }
+        }
+        break;
+      }
+      case MGD_PATCH_SPONGE: {
+        // addDamping (src/SpongePatchImpl.f90:65-166)
+        const double sp = pt.strength[k];
+        const double* qt = pt.aplus + (long)k * NV;   // target state on the patch
+        for (int l = 0; l < NV; ++l) {
+          if (mode == MGD_FORWARD) r[l] = r[l] - sp * (qi[l * g.N + p] - qt[l]);
+          else r[l] = r[l] + sp * li[l * g.N + p];
+        }
+        break;
+      }
+      case MGD_PATCH_ACTUATOR: {
+        // updateActuatorPatch (src/ActuatorPatchImpl.f90:108-181): energy equation
+        if (mode != MGD_FORWARD || !control) break;
+        r[ND + 1] = r[ND + 1] + pt.strength[k] * control[inst * control_inst_stride + k];
+        break;
+      }
+      case MGD_PATCH_COST_TARGET: {
+        // addAdjointForcing (src/CostTargetPatchImpl.f90:72-136)
+        if (mode == MGD_FORWARD || !aforce) break;
+        const double* af = aforce + inst * aforce_inst_stride;
+        for (int l = 0; l < NV; ++l) r[l] = r[l] + forcing_factor * af[(long)l * pt.np + k];
+        break;
+      }
+      case MGD_PATCH_KOLMOGOROV: {
+        // addKolmogorovForcing (src/KolmogorovForcingPatchImpl.f90:94-178)
+        const double f = pt.strength[k];
+        if (mode == MGD_FORWARD) r[1] = r[1] + qi[p] * f;
+        else r[0] = r[0] - li[g.N + p] * f;
+        break;
+      }
+    }
+  }
+  if (mode == MGD_FORWARD) {
+    // addAcousticSource (src/AcousticSourceImpl.f90:34-64)
+    for (int s = 0; s < nsrc; ++s) {
+      double rr = 0.0;
+      for (int d = 0; d < ND; ++d) {
+        const double dx = g.x[d][ijk[d]] - src[s].loc[d];
+        rr += dx * dx;
+      }
+      r[ND + 1] = r[ND + 1] + src_amp[s] * exp(-src[s].gfac * rr);
+    }
+  }
+  for (int l = 0; l < NV; ++l) ri[l * g.N + p] = r[l];
+}
+
+// ------------------------------------------------------------------ RK4 stages
+// substepForwardRK4 updates (src/RK4IntegratorImpl.f90:115-156); q_out2 (optional)
+// receives a copy of the new stage state (checkpoint window).
+__global__ void k_rk_forward(long M, int stage, double dt, double* __restrict__ q,
+                             double* __restrict__ b1, double* __restrict__ b2,
+                             const double* __restrict__ rhs, double* __restrict__ qcopy) {
+  const long i = gidx();
+  if (i >= M) return;
+  const double r = rhs[i];
+  double qn;
+  switch (stage) {
+    case 1: {
+      const double q0 = q[i];
+      b1[i] = q0;
+      b2[i] = q0 + dt * r / 6.0;
+      qn = q0 + dt * r / 2.0;
+      break;
+    }
+    case 2:
+      b2[i] = b2[i] + dt * r / 3.0;
+      qn = b1[i] + dt * r / 2.0;
+      break;
+    case 3:
+      b2[i] = b2[i] + dt * r / 3.0;
+      qn = b1[i] + dt * r;
+      break;
+    default:
+      qn = b2[i] + dt * r / 6.0;
+      break;
+  }
+  q[i] = qn;
+  if (qcopy) qcopy[i] = qn;
+}
+
+// substepAdjointRK4 updates (src/RK4IntegratorImpl.f90:215-261)
+__global__ void k_rk_adjoint(long M, int stage, double dt, double* __restrict__ lam,
+                             double* __restrict__ b1, double* __restrict__ b2,
+                             const double* __restrict__ rhs) {
+  const long i = gidx();
+  if (i >= M) return;
+  const double r = rhs[i];
+  switch (stage) {
+    case 4: {
+      const double l0 = lam[i];
+      b1[i] = l0;
+      b2[i] = l0 - dt * r / 6.0;
+      lam[i] = l0 - dt * r / 2.0;
+      break;
+    }
+    case 3:
+      b2[i] = b2[i] - dt * r / 3.0;
+      lam[i] = b1[i] - dt * r / 2.0;
+      break;
+    case 2:
+      b2[i] = b2[i] - dt * r / 3.0;
+      lam[i] = b1[i] - dt * r;
+      break;
+    default:
+      lam[i] = b2[i] - dt * r / 6.0;
+      break;
+  }
+}
+
+__global__ void k_copy(long M, const double* __restrict__ a, double* __restrict__ b) {
+  const long i = gidx();
+  if (i < M) b[i] = a[i];
+}
+
+__global__ void k_fill(long M, double v, double* __restrict__ b) {
+  const long i = gidx();
+  if (i < M) b[i] = v;
+}
+
+// ------------------------------------------------------------------ patch reductions
+// Per-point integrand over one box patch, reduced in two deterministic passes.
+template <int ND>
+__device__ __forceinline__ long patch_point(const Geo& g, const DPatch& pt, int k, int* ijk) {
+  ijk[0] = pt.lo[0] + k % pt.ls[0];
+  ijk[1] = ND > 1 ? pt.lo[1] + (k / pt.ls[0]) % pt.ls[1] : 0;
+  ijk[2] = ND > 2 ? pt.lo[2] + k / (pt.ls[0] * pt.ls[1]) : 0;
+  return ijk[0] + (long)g.n[0] * (ijk[1] + (long)g.n[1] * ijk[2]);
+}
+
+constexpr int kRedBlock = 256;
+
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double sh[kRedBlock];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kRedBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// which 0: AcousticNoise integrand norm * (p - pbar)^2 * m_T on the cost-target patch
+//          (computeAcousticNoise + computeQuadratureOnPatches);
+// which 1: ThermalActuator sensitivity norm * (lambda_E m_C)^2 on the actuator patch.
+template <int ND>
+__global__ void k_patch_reduce(Geo g, const DPatch* __restrict__ pt_, int which,
+                               const double* __restrict__ x, double* __restrict__ partial) {
+  const DPatch& pt = *pt_;
+  const long inst = blockIdx.y;
+  const int k = blockIdx.x * kRedBlock + threadIdx.x;
+  double v = 0.0;
+  if (k < pt.np) {
+    int ijk[3];
+    const long p = patch_point<ND>(g, pt, k, ijk);
+    const double* xi = x + inst * g.nv * g.N;
+    const double nrm = gnorm<ND>(g, ijk);
+    if (which == 0) {
+      double q[ND + 2];
+      load_q<ND>(xi, g.N, p, q);
+      const Prim<ND> s = prim<ND>(q, g.gamma);
+      const double F = s.p - pt.aplus[k];            // mean pressure on the patch
+      v = 1.0 * nrm * (F * F * pt.strength[k]);      // target mollifier
+    } else {
+      const double F = xi[(ND + 1) * g.N + p] * pt.strength[k] * 1.0;   // control mollifier
+      v = 1.0 * nrm * (F * F);
+    }
+  }
+  const double s = block_sum(v);
+  if (threadIdx.x == 0) partial[inst * gridDim.x + blockIdx.x] = s;
+}
+
+// acc[inst] += weight * sum(partials); out_stage (optional) keeps the raw value.
+__global__ void k_reduce_final(int nblocks, const double* __restrict__ partial, double weight,
+                               double* __restrict__ acc, double* __restrict__ raw) {
+  const long inst = blockIdx.x;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += kRedBlock) v += partial[inst * nblocks + b];
+  const double s = block_sum(v);
+  if (threadIdx.x == 0) {
+    if (raw) raw[inst] = s;
+    if (acc) acc[inst] = acc[inst] + weight * s;
+  }
+}
+
+// AcousticNoise adjoint forcing on the cost-target patch (src/AcousticNoiseImpl.f90:208-280)
+template <int ND>
+__global__ void k_adj_forcing(Geo g, const DPatch* __restrict__ pt_, const double* __restrict__ q,
+                              double* __restrict__ out, int zero) {
+  const DPatch& pt = *pt_;
+  const long inst = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pt.np) return;
+  double* o = out + inst * (long)(ND + 2) * pt.np;
+  if (zero) {
+    for (int l = 0; l < ND + 2; ++l) o[(long)l * pt.np + k] = 0.0;
+    return;
+  }
+  int ijk[3];
+  const long p = patch_point<ND>(g, pt, k, ijk);
+  double v[ND + 2];
+  load_q<ND>(q + inst * g.nv * g.N, g.N, p, v);
+  const Prim<ND> s = prim<ND>(v, g.gamma);
+  const double F = -2.0 * pt.strength[k] * 1.0 * (g.gamma - 1.0) * (s.p - pt.aplus[k]);
+  o[(long)(ND + 1) * pt.np + k] = F;
+  double ke = 0.0;
+  for (int i = 0; i < ND; ++i) {
+    o[(long)(i + 1) * pt.np + k] = -s.u[i] * F;
+    ke += s.u[i] * s.u[i];
+  }
+  o[k] = 0.5 * ke * F;
+}
+
+// adjoint terminal condition: lambda += factor * forcing on the cost-target patch
+// (loadInitialCondition, src/SolverImpl.f90:368-428)
+template <int ND>
+__global__ void k_adj_terminal(Geo g, const DPatch* __restrict__ pt_, const double* __restrict__ af,
+                               double factor, double* __restrict__ lam) {
+  const DPatch& pt = *pt_;
+  const long inst = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pt.np) return;
+  int ijk[3];
+  const long p = patch_point<ND>(g, pt, k, ijk);
+  const double* a = af + inst * (long)(ND + 2) * pt.np;
+  double* li = lam + inst * g.nv * g.N;
+  for (int l = 0; l < ND + 2; ++l) {
+    const double r = 0.0 + factor * a[(long)l * pt.np + k];
+    li[l * g.N + p] = li[l * g.N + p] + r;
+  }
+}
+
+// ThermalActuator gradient lambda_E * m_C on the actuator patch (:383-443)
+template <int ND>
+__global__ void k_gradient(Geo g, const DPatch* __restrict__ pt_, const double* __restrict__ lam,
+                           double* __restrict__ out, long inst_stride) {
+  const DPatch& pt = *pt_;
+  const long inst = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pt.np) return;
+  int ijk[3];
+  const long p = patch_point<ND>(g, pt, k, ijk);
+  out[inst * inst_stride + k] = lam[inst * g.nv * g.N + (ND + 1) * g.N + p] * (pt.strength[k] * 1.0);
+}
+
+// ------------------------------------------------------------------ launchers
+#define MGD_DISPATCH(ND_, ...)                                  \
+  switch (ND_) {                                                \
+    case 1: { constexpr int ND = 1; __VA_ARGS__; } break;        \
+    case 2: { constexpr int ND = 2; __VA_ARGS__; } break;        \
+    default: { constexpr int ND = 3; __VA_ARGS__; } break;       \
+  }
+
+static dim3 grid_pts(const Geo& g) {
+  return dim3((unsigned)((g.N + kBlock - 1) / kBlock), (unsigned)g.ninst);
+}
+
+void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
+                        double* G, double* t, const double* control, long control_stride,
+                        const Source* src, int nsrc, const double* src_amp, hipStream_t st) {
+  const dim3 grid = grid_pts(g);
+  MGD_DISPATCH(g.nd, {
+    if (g.viscous) {
+      k_prim<ND><<<grid, kBlock, 0, st>>>(g, q, w);
+      for (int d = 0; d < ND; ++d) k_grad<ND><<<grid, kBlock, 0, st>>>(g, d, w, G);
+    }
+    for (int d = 0; d < ND; ++d)
+      k_flux_div<ND><<<grid, kBlock, 0, st>>>(g, d, q, g.viscous ? G : nullptr, rhs);
+    if (g.dissipation) {
+      for (int d = 0; d < ND; ++d) {
+        k_diss_a<ND><<<grid, kBlock, 0, st>>>(g, d, q, t, rhs, g.diss_amount);
+        if (!g.composite) k_diss_b<ND><<<grid, kBlock, 0, st>>>(g, d, t, rhs, g.diss_amount);
+      }
+    }
+    k_finish<ND><<<grid, kBlock, 0, st>>>(g, MGD_FORWARD, P, q, nullptr, G, rhs, control,
+                                          control_stride, nullptr, 0, 0.0, src, nsrc, src_amp);
+  });
+}
+
+void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
+                        double* rhs, double* w, double* G, double* t, double* T1, double* S,
+                        const double* aforce, long aforce_stride, double forcing_factor,
+                        int has_farfield, hipStream_t st) {
+  const dim3 grid = grid_pts(g);
+  MGD_DISPATCH(g.nd, {
+    constexpr int NV = ND + 2;
+    if (g.viscous) {
+      k_prim<ND><<<grid, kBlock, 0, st>>>(g, q, w);
+      for (int d = 0; d < ND; ++d) k_grad<ND><<<grid, kBlock, 0, st>>>(g, d, w, G);
+    }
+    for (int d = 0; d < ND; ++d)
+      k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, lam, (long)NV * g.N, T1, 0);
+    k_adj_jac<ND><<<grid, kBlock, 0, st>>>(g, q, G, T1, rhs);
+    if (g.viscous) {
+      k_adj_visc2<ND><<<grid, kBlock, 0, st>>>(g, q, T1);
+      for (int d = 0; d < ND; ++d)
+        k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, T1, (long)ND * NV * g.N, S, 1);
+      k_adj_chain<ND><<<grid, kBlock, 0, st>>>(g, q, S, rhs, -1.0);
+    }
+    if (g.dissipation) {
+      for (int d = 0; d < ND; ++d) {
+        k_diss_a<ND><<<grid, kBlock, 0, st>>>(g, d, lam, t, rhs, -g.diss_amount);
+        if (!g.composite) k_diss_b<ND><<<grid, kBlock, 0, st>>>(g, d, t, rhs, -g.diss_amount);
+      }
+    }
+    if (g.viscous && has_farfield) {
+      const long M = (long)g.ninst * ND * NV * g.N;
+      k_fill<<<(unsigned)((M + kBlock - 1) / kBlock), kBlock, 0, st>>>(M, 0.0, T1);
+      k_ff_visc_adj<ND><<<grid, kBlock, 0, st>>>(g, P, q, lam, T1);
+      for (int d = 0; d < ND; ++d)
+        k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, T1, (long)ND * NV * g.N, S, 1);
+      k_adj_chain<ND><<<grid, kBlock, 0, st>>>(g, q, S, rhs, +1.0);
+    }
+    k_finish<ND><<<grid, kBlock, 0, st>>>(g, MGD_ADJOINT, P, q, lam, G, rhs, nullptr, 0, aforce,
+                                          aforce_stride, forcing_factor, nullptr, 0, nullptr);
+  });
+}
+
+void launch_rk_forward(long M, int stage, double dt, double* q, double* b1, double* b2,
+                       const double* rhs, double* qcopy, hipStream_t st) {
+  k_rk_forward<<<(unsigned)((M + kBlock - 1) / kBlock), kBlock, 0, st>>>(M, stage, dt, q, b1, b2, rhs,
+                                                                        qcopy);
+}
+
+void launch_rk_adjoint(long M, int stage, double dt, double* lam, double* b1, double* b2,
+                       const double* rhs, hipStream_t st) {
+  k_rk_adjoint<<<(unsigned)((M + kBlock - 1) / kBlock), kBlock, 0, st>>>(M, stage, dt, lam, b1, b2,
+                                                                         rhs);
+}
+
+void launch_copy(long M, const double* a, double* b, hipStream_t st) {
+  k_copy<<<(unsigned)((M + kBlock - 1) / kBlock), kBlock, 0, st>>>(M, a, b);
+}
+
+void launch_fill(long M, double v, double* b, hipStream_t st) {
+  k_fill<<<(unsigned)((M + kBlock - 1) / kBlock), kBlock, 0, st>>>(M, v, b);
+}
+
+int reduce_blocks(int np) { return (np + kRedBlock - 1) / kRedBlock; }
+
+void launch_patch_reduce(const Geo& g, const DPatch* pt, int np, int which, const double* x,
+                         double* partial, double weight, double* acc, double* raw, hipStream_t st) {
+  const int nb = reduce_blocks(np);
+  MGD_DISPATCH(g.nd, {
+    k_patch_reduce<ND><<<dim3(nb, g.ninst), kRedBlock, 0, st>>>(g, pt, which, x, partial);
+  });
+  k_reduce_final<<<g.ninst, kRedBlock, 0, st>>>(nb, partial, weight, acc, raw);
+}
+
+void launch_adj_forcing(const Geo& g, const DPatch* pt, int np, const double* q, double* out,
+                        int zero, hipStream_t st) {
+  MGD_DISPATCH(g.nd, {
+    k_adj_forcing<ND><<<dim3((np + kBlock - 1) / kBlock, g.ninst), kBlock, 0, st>>>(g, pt, q, out, zero);
+  });
+}
+
+void launch_adj_terminal(const Geo& g, const DPatch* pt, int np, const double* af, double factor,
+                         double* lam, hipStream_t st) {
+  MGD_DISPATCH(g.nd, {
+    k_adj_terminal<ND><<<dim3((np + kBlock - 1) / kBlock, g.ninst), kBlock, 0, st>>>(g, pt, af, factor, lam);
+  });
+}
+
+void launch_gradient(const Geo& g, const DPatch* pt, int np, const double* lam, double* out,
+                     long inst_stride, hipStream_t st) {
+  MGD_DISPATCH(g.nd, {
+    k_gradient<ND><<<dim3((np + kBlock - 1) / kBlock, g.ninst), kBlock, 0, st>>>(g, pt, lam, out,
+                                                                               inst_stride);
+  });
+}
+
+// Stencil operator on a device field (StencilOperator%apply), ncomp components.
+__global__ void k_apply(Geo g, int slot, int d, int ncomp, const double* __restrict__ x,
+                        double* __restrict__ y) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const int c = blockIdx.y;
+  int ijk[3] = {0, 0, 0};
+  ijk[0] = (int)(p % g.n[0]);
+  ijk[1] = (int)((p / g.n[0]) % g.n[1]);
+  ijk[2] = (int)(p / ((long)g.n[0] * g.n[1]));
+  const Dir dd = g.dir[d];
+  const long base = p - (long)ijk[d] * dd.stride;
+  const double* xc = x + (long)c * g.N;
+  double out[1];
+  sbp_apply<1>(g.ops[slot], dd, ijk[d],
+               [&](int idx, double* v) { v[0] = xc[base + (long)idx * dd.stride]; }, out);
+  y[(long)c * g.N + p] = out[0];
+}
+
+void launch_apply(const Geo& g, int slot, int d, int ncomp, const double* x, double* y,
+                  hipStream_t st) {
+  k_apply<<<dim3((unsigned)((g.N + kBlock - 1) / kBlock), ncomp), kBlock, 0, st>>>(g, slot, d, ncomp,
+                                                                                 x, y);
+}
+
+}  // namespace mgd

@@ -1,0 +1,985 @@
+// region.cpp -- host side of the MI355X magudi RHS library: the C ABI of
+// include/magudi_hip.h, grid set-up, patch set-up, HBM-resident state and the
+// forward / adjoint drivers (Solver%runForward / runAdjoint with uniform
+// checkpointing held in HBM).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "mgd_internal.h"
+
+namespace mgd {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+namespace {
+
+const double kRkNorm[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+
+struct DevBuf {
+  double* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  int alloc(size_t count) {
+    if (n >= count && p) return MGD_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) return MGD_OK;
+    hipError_t e = hipMalloc(&p, count * sizeof(double));
+    if (e != hipSuccess) {
+      set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(double)) +
+                " bytes: " + hipGetErrorString(e));
+      p = nullptr;
+      return MGD_ERR_HIP;
+    }
+    n = count;
+    return MGD_OK;
+  }
+};
+
+struct HostPatch {
+  mgd_patch_desc pd;
+  int lo[3], hi[3], ls[3], np;
+  std::vector<double> strength, data, tvflux;
+  DevBuf d_strength, d_data, d_tvflux;
+};
+
+}  // namespace
+}  // namespace mgd
+
+struct mgd_region {
+  mgd_region_desc desc;
+  int nd, nv, ninst;
+  long N;
+  std::vector<double> coords;            // nd x N (host copy)
+  std::vector<double> line[3];           // coordinate along each direction
+  std::vector<double> h[3], hw[3];
+  mgd::Op ops[mgd::kNumSlots];
+  mgd::Op first_host;
+  mgd::DevBuf d_h[3], d_hw[3], d_x[3], d_ops;
+  mgd::Geo geo;
+  std::vector<std::unique_ptr<mgd::HostPatch>> patches;
+  std::vector<mgd::Source> sources;
+  std::vector<double> target, ctrl_moll, tgt_moll;
+  bool finalized = false;
+  mgd::Patches hpatches;
+  mgd::DevBuf d_patches, d_sources;
+  int ip_cost = -1, ip_act = -1, has_farfield = 0;
+  // HBM-resident state
+  mgd::DevBuf q, lam, rhs, b1, b2, b1a, b2a, t, w, G, T1, S;
+  mgd::DevBuf qs, aforce, control, grad, acc_J, acc_S, partial, src_amp;
+  mgd::DevBuf snaps, window;
+  long ctrl_stride = 0;                  // control entries per instance
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms[3] = {0, 0, 0};
+  int last_launch[3] = {0, 0, 0};
+};
+
+using namespace mgd;
+
+namespace {
+
+long M_of(const mgd_region* r) { return (long)r->nv * r->N; }
+
+// Grid%computeCoordinateDerivatives along one rectilinear line (src/GridImpl.f90:621-744)
+void coordinate_derivative(const Op& op, int len, int periodicity, double plen,
+                           const std::vector<double>& x, std::vector<double>& out) {
+  out.assign(len, 0.0);
+  if (periodicity != MGD_PERIODIC_PLANE) {
+    op_apply_line(op, len, periodicity, x.data(), out.data());
+    return;
+  }
+  // PLANE: ghost coordinates shifted by the periodic length, interior stencil everywhere
+  const int hw = op.hw;
+  std::vector<double> gx(len + 2 * hw);
+  for (int i = -hw; i < len + hw; ++i) {
+    const int w = i < 0 ? i + len : (i >= len ? i - len : i);
+    double v = x[w];
+    if (i < 0) v = v - plen;
+    else if (i >= len) v = v + plen;
+    gx[i + hw] = v;
+  }
+  for (int i = 0; i < len; ++i) {
+    double s = 0.0;
+    for (int k = 1; k <= hw; ++k) s += op.c[kMaxHW + k] * (gx[i + k + hw] - gx[i - k + hw]);
+    out[i] = s;
+  }
+}
+
+long stride_of(const mgd_region* r, int d) {
+  return d == 0 ? 1 : (d == 1 ? r->desc.n[0] : (long)r->desc.n[0] * r->desc.n[1]);
+}
+
+// Incoming part of the inviscid flux Jacobian from the target state
+// (computeIncomingJacobianOfInviscidFlux2D/3D, src/CNSHelperImpl.f90:1669-2342).
+void incoming_jacobian(int nd, const double* q, const double* metrics, double g, int dirn, double* A) {
+  const int n = nd + 2;
+  double R[25] = {0}, L[25] = {0}, lam[5] = {0}, nm[3] = {0}, u[3] = {0};
+  auto Rm = [&](int i, int j) -> double& { return R[(j - 1) * n + (i - 1)]; };
+  auto Lm = [&](int i, int j) -> double& { return L[(j - 1) * n + (i - 1)]; };
+  const double sv = 1.0 / q[0];
+  double ke = 0.0;
+  for (int i = 0; i < nd; ++i) u[i] = sv * q[i + 1];
+  for (int i = 0; i < nd; ++i) ke += u[i] * u[i];
+  const double T = g * (sv * q[nd + 1] - 0.5 * ke);
+  double arc = 0.0;
+  for (int i = 0; i < nd; ++i) arc += metrics[i] * metrics[i];
+  arc = std::sqrt(arc);
+  for (int i = 0; i < nd; ++i) nm[i] = metrics[i] / arc;
+  double U = 0.0;
+  for (int i = 0; i < nd; ++i) U += nm[i] * u[i];
+  const double c = std::sqrt((g - 1.0) * T);
+  const double phi2 = 0.5 * (g - 1.0) * ke;
+  const double rho = q[0];
+  const double h = phi2 / (g - 1.0);
+  if (nd == 2) {
+    lam[0] = U; lam[1] = U; lam[2] = U + c; lam[3] = U - c;
+    Rm(1, 1) = 1.0; Rm(2, 1) = u[0]; Rm(3, 1) = u[1]; Rm(4, 1) = h;
+    Rm(1, 2) = 0.0; Rm(2, 2) = nm[1] * rho; Rm(3, 2) = -nm[0] * rho;
+    Rm(4, 2) = rho * (nm[1] * u[0] - nm[0] * u[1]);
+    Rm(1, 3) = 1.0; Rm(2, 3) = u[0] + nm[0] * c; Rm(3, 3) = u[1] + nm[1] * c; Rm(4, 3) = T + h + c * U;
+    Rm(1, 4) = 1.0; Rm(2, 4) = u[0] - nm[0] * c; Rm(3, 4) = u[1] - nm[1] * c; Rm(4, 4) = T + h - c * U;
+    Lm(1, 1) = 1.0 - phi2 / (c * c);
+    Lm(2, 1) = -sv * (nm[1] * u[0] - nm[0] * u[1]);
+    Lm(3, 1) = 0.5 * (phi2 / (c * c) - U / c);
+    Lm(4, 1) = 0.5 * (phi2 / (c * c) + U / c);
+    Lm(1, 2) = u[0] / T; Lm(2, 2) = sv * nm[1];
+    Lm(3, 2) = -0.5 * (u[0] / T - nm[0] / c); Lm(4, 2) = -0.5 * (u[0] / T + nm[0] / c);
+    Lm(1, 3) = u[1] / T; Lm(2, 3) = -sv * nm[0];
+    Lm(3, 3) = -0.5 * (u[1] / T - nm[1] / c); Lm(4, 3) = -0.5 * (u[1] / T + nm[1] / c);
+    Lm(1, 4) = -1.0 / T; Lm(2, 4) = 0.0; Lm(3, 4) = 0.5 / T; Lm(4, 4) = 0.5 / T;
+  } else if (nd == 3) {
+    lam[0] = U; lam[1] = U; lam[2] = U; lam[3] = U + c; lam[4] = U - c;
+    Rm(1, 1) = nm[0]; Rm(2, 1) = nm[0] * u[0]; Rm(3, 1) = nm[0] * u[1] + rho * nm[2];
+    Rm(4, 1) = nm[0] * u[2] - rho * nm[1];
+    Rm(5, 1) = rho * (nm[2] * u[1] - nm[1] * u[2]) + h * nm[0];
+    Rm(1, 2) = nm[1]; Rm(2, 2) = nm[1] * u[0] - rho * nm[2]; Rm(3, 2) = nm[1] * u[1];
+    Rm(4, 2) = nm[1] * u[2] + rho * nm[0];
+    Rm(5, 2) = rho * (nm[0] * u[2] - nm[2] * u[0]) + h * nm[1];
+    Rm(1, 3) = nm[2]; Rm(2, 3) = nm[2] * u[0] + rho * nm[1]; Rm(3, 3) = nm[2] * u[1] - rho * nm[0];
+    Rm(4, 3) = nm[2] * u[2];
+    Rm(5, 3) = rho * (nm[1] * u[0] - nm[0] * u[1]) + h * nm[2];
+    Rm(1, 4) = 1.0; Rm(2, 4) = u[0] + nm[0] * c; Rm(3, 4) = u[1] + nm[1] * c;
+    Rm(4, 4) = u[2] + nm[2] * c; Rm(5, 4) = T + h + c * U;
+    Rm(1, 5) = 1.0; Rm(2, 5) = u[0] - nm[0] * c; Rm(3, 5) = u[1] - nm[1] * c;
+    Rm(4, 5) = u[2] - nm[2] * c; Rm(5, 5) = T + h - c * U;
+    const double a = 1.0 - phi2 / (c * c);
+    Lm(1, 1) = nm[0] * a - sv * (nm[2] * u[1] - nm[1] * u[2]);
+    Lm(2, 1) = nm[1] * a - sv * (nm[0] * u[2] - nm[2] * u[0]);
+    Lm(3, 1) = nm[2] * a - sv * (nm[1] * u[0] - nm[0] * u[1]);
+    Lm(4, 1) = 0.5 * (phi2 / (c * c) - U / c);
+    Lm(5, 1) = 0.5 * (phi2 / (c * c) + U / c);
+    Lm(1, 2) = nm[0] * u[0] / T; Lm(2, 2) = nm[1] * u[0] / T - sv * nm[2];
+    Lm(3, 2) = nm[2] * u[0] / T + sv * nm[1];
+    Lm(4, 2) = -0.5 * (u[0] / T - nm[0] / c); Lm(5, 2) = -0.5 * (u[0] / T + nm[0] / c);
+    Lm(1, 3) = nm[0] * u[1] / T + sv * nm[2]; Lm(2, 3) = nm[1] * u[1] / T;
+    Lm(3, 3) = nm[2] * u[1] / T - sv * nm[0];
+    Lm(4, 3) = -0.5 * (u[1] / T - nm[1] / c); Lm(5, 3) = -0.5 * (u[1] / T + nm[1] / c);
+    Lm(1, 4) = nm[0] * u[2] / T - sv * nm[1]; Lm(2, 4) = nm[1] * u[2] / T + sv * nm[0];
+    Lm(3, 4) = nm[2] * u[2] / T;
+    Lm(4, 4) = -0.5 * (u[2] / T - nm[2] / c); Lm(5, 4) = -0.5 * (u[2] / T + nm[2] / c);
+    Lm(1, 5) = -nm[0] / T; Lm(2, 5) = -nm[1] / T; Lm(3, 5) = -nm[2] / T;
+    Lm(4, 5) = 0.5 / T; Lm(5, 5) = 0.5 / T;
+  } else {
+    for (int k = 0; k < n * n; ++k) A[k] = 0.0;
+    return;
+  }
+  for (int i = 0; i < n; ++i) lam[i] = arc * lam[i];
+  for (int i = 0; i < n; ++i) if ((double)dirn * lam[i] < 0.0) lam[i] = 0.0;
+  for (int j = 1; j <= n; ++j)
+    for (int i = 1; i <= n; ++i) {
+      double s = Rm(i, 1) * lam[0] * Lm(1, j);
+      for (int k = 2; k <= n; ++k) s = s + Rm(i, k) * lam[k - 1] * Lm(k, j);
+      A[(j - 1) * n + (i - 1)] = s;
+    }
+}
+
+int upload(DevBuf& b, const std::vector<double>& v) {
+  if (b.alloc(std::max<size_t>(v.size(), 1))) return MGD_ERR_HIP;
+  if (!v.empty()) MGD_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+  return MGD_OK;
+}
+
+double metric_host(const mgd_region* r, int d, const int* ijk) {
+  const int nd = r->nd;
+  if (nd == 1) return 1.0;
+  if (nd == 2) return d == 0 ? r->h[1][ijk[1]] : r->h[0][ijk[0]];
+  if (d == 0) return r->h[1][ijk[1]] * r->h[2][ijk[2]];
+  if (d == 1) return r->h[0][ijk[0]] * r->h[2][ijk[2]];
+  return r->h[0][ijk[0]] * r->h[1][ijk[1]];
+}
+
+void unflat_host(const mgd_region* r, long p, int* ijk) {
+  ijk[0] = (int)(p % r->desc.n[0]);
+  ijk[1] = (int)((p / r->desc.n[0]) % r->desc.n[1]);
+  ijk[2] = (int)(p / ((long)r->desc.n[0] * r->desc.n[1]));
+}
+
+long patch_point(const mgd_region* r, const HostPatch& P, int k) {
+  const int a = k % P.ls[0], b = (k / P.ls[0]) % P.ls[1], c = k / (P.ls[0] * P.ls[1]);
+  return (P.lo[0] + a) + (long)r->desc.n[0] * ((P.lo[1] + b) + (long)r->desc.n[1] * (P.lo[2] + c));
+}
+
+// viscous fluxes of the target state at the far-field patch points (host; set-up only)
+void target_viscous_fluxes(mgd_region* r, HostPatch& P) {
+  const int nd = r->nd, nv = r->nv;
+  const long N = r->N;
+  const double g = r->desc.gamma;
+  std::vector<double> w((size_t)(nd + 1) * N);
+  for (long p = 0; p < N; ++p) {
+    const double sv = 1.0 / r->target[p];
+    double ke = 0.0;
+    for (int i = 0; i < nd; ++i) {
+      const double u = sv * r->target[(i + 1) * N + p];
+      w[i * N + p] = u;
+      ke += u * u;
+    }
+    const double pr = (g - 1.0) * (r->target[(nd + 1) * N + p] - 0.5 * r->target[p] * ke);
+    w[nd * N + p] = g * pr / (g - 1.0) * sv;
+  }
+  const double re_inv = r->geo.re_inv, pr_inv = r->geo.pr_inv;
+  P.tvflux.assign((size_t)nd * nv * P.np, 0.0);
+  for (int k = 0; k < P.np; ++k) {
+    const long p = patch_point(r, P, k);
+    int ijk[3];
+    unflat_host(r, p, ijk);
+    double grad[4][3];
+    for (int d = 0; d < nd; ++d) {
+      const int len = r->desc.n[d];
+      const long st = stride_of(r, d);
+      const long base = p - (long)ijk[d] * st;
+      std::vector<double> in(len), out(len);
+      const double jm = (1.0 / (nd == 1 ? r->h[0][ijk[0]]
+                                : nd == 2 ? r->h[0][ijk[0]] * r->h[1][ijk[1]]
+                                          : r->h[0][ijk[0]] * r->h[1][ijk[1]] * r->h[2][ijk[2]])) *
+                        metric_host(r, d, ijk);
+      for (int c = 0; c <= nd; ++c) {
+        for (int i = 0; i < len; ++i) in[i] = w[c * N + base + i * st];
+        op_apply_line(r->first_host, len, r->desc.periodicity[d], in.data(), out.data());
+        grad[c][d] = jm * out[ijk[d]];
+      }
+    }
+    const double T = w[nd * N + p];
+    double mu, la, ka;
+    if (r->desc.power_law <= 0.0) {
+      mu = re_inv; la = (r->desc.bulk_ratio - 2.0 / 3.0) * re_inv; ka = re_inv * pr_inv;
+    } else {
+      mu = std::pow((g - 1.0) * T, r->desc.power_law) * re_inv;
+      la = (r->desc.bulk_ratio - 2.0 / 3.0) * mu;
+      ka = mu * pr_inv;
+    }
+    double tau[9];
+    for (int i = 0; i < nd; ++i)
+      for (int j = 0; j < nd; ++j) tau[i * nd + j] = grad[i][j];
+    if (nd == 2) {
+      const double dv = la * (tau[0] + tau[3]);
+      tau[0] = 2.0 * mu * tau[0] + dv; tau[1] = mu * (tau[1] + tau[2]); tau[2] = tau[1];
+      tau[3] = 2.0 * mu * tau[3] + dv;
+    } else if (nd == 3) {
+      const double dv = la * (tau[0] + tau[4] + tau[8]);
+      tau[0] = 2.0 * mu * tau[0] + dv; tau[1] = mu * (tau[1] + tau[3]);
+      tau[2] = mu * (tau[2] + tau[6]); tau[3] = tau[1]; tau[4] = 2.0 * mu * tau[4] + dv;
+      tau[5] = mu * (tau[5] + tau[7]); tau[6] = tau[2]; tau[7] = tau[5];
+      tau[8] = 2.0 * mu * tau[8] + dv;
+    } else {
+      tau[0] = (2.0 * mu + la) * tau[0];
+    }
+    double hf[3];
+    for (int j = 0; j < nd; ++j) hf[j] = -ka * grad[nd][j];
+    for (int d = 0; d < nd; ++d) {
+      P.tvflux[((size_t)d * nv + 0) * P.np + k] = 0.0;
+      double e = 0.0;
+      for (int i = 0; i < nd; ++i) {
+        P.tvflux[((size_t)d * nv + 1 + i) * P.np + k] = tau[i * nd + d];
+        e += w[i * N + p] * tau[i * nd + d];
+      }
+      P.tvflux[((size_t)d * nv + nd + 1) * P.np + k] = e - hf[d];
+    }
+  }
+}
+
+int ensure_state(mgd_region* r) {
+  const size_t M = (size_t)r->ninst * r->nv * r->N;
+  if (r->q.alloc(M) || r->lam.alloc(M) || r->rhs.alloc(M) || r->b1.alloc(M) || r->b2.alloc(M))
+    return MGD_ERR_HIP;
+  if (r->desc.dissipation && !r->desc.composite_dissipation && r->t.alloc(M)) return MGD_ERR_HIP;
+  if (r->desc.viscous) {
+    if (r->w.alloc((size_t)r->ninst * (r->nd + 1) * r->N) ||
+        r->G.alloc((size_t)r->ninst * (r->nd + 1) * r->nd * r->N))
+      return MGD_ERR_HIP;
+  }
+  return MGD_OK;
+}
+
+int ensure_adjoint(mgd_region* r) {
+  const size_t M = (size_t)r->ninst * r->nv * r->N;
+  if (r->b1a.alloc(M) || r->b2a.alloc(M) || r->T1.alloc(M * r->nd)) return MGD_ERR_HIP;
+  if (r->desc.viscous && r->S.alloc(M * r->nd)) return MGD_ERR_HIP;
+  return MGD_OK;
+}
+
+int n_act(const mgd_region* r) { return r->ip_act < 0 ? 0 : r->patches[r->ip_act]->np; }
+int n_cost(const mgd_region* r) { return r->ip_cost < 0 ? 0 : r->patches[r->ip_cost]->np; }
+const DPatch* dpatch(const mgd_region* r, int ip) {
+  return reinterpret_cast<const Patches*>(r->d_patches.p)->p + ip;
+}
+const Patches* dpatches(const mgd_region* r) { return reinterpret_cast<const Patches*>(r->d_patches.p); }
+
+// source amplitude a = amplitude cos(omega t + phase) for each stage time
+void source_table(const mgd_region* r, const std::vector<double>& times, std::vector<double>& out) {
+  const int ns = (int)r->sources.size();
+  out.assign(times.size() * std::max(ns, 1), 0.0);
+  for (size_t k = 0; k < times.size(); ++k)
+    for (int s = 0; s < ns; ++s)
+      out[k * ns + s] = r->sources[s].amplitude * std::cos(r->sources[s].omega * times[k] + r->sources[s].phase);
+}
+
+// stage times of substepForwardRK4: stage 1 at t, 2 and 3 at t + dt/2, 4 at t + dt
+std::vector<double> stage_times(double t0, double dt, int nsteps) {
+  std::vector<double> t(4 * (size_t)nsteps);
+  double time = t0;
+  for (int s = 0; s < nsteps; ++s) {
+    t[4 * s + 0] = time;
+    time = time + dt / 2.0;
+    t[4 * s + 1] = time;
+    t[4 * s + 2] = time;
+    time = time + dt / 2.0;
+    t[4 * s + 3] = time;
+  }
+  return t;
+}
+
+struct Timer {
+  mgd_region* r;
+  int slot;
+  Timer(mgd_region* r_, int s) : r(r_), slot(s) {}
+};
+
+// One forward RK4 substep on state `q` (all instances).
+void forward_stage(mgd_region* r, double* q, double* b1, double* b2, int sidx, int stage, double dt,
+                   double* qcopy) {
+  const int na = n_act(r);
+  const double* ctrl = nullptr;
+  long cstride = 0;
+  if (r->control.p && na > 0) {
+    ctrl = r->control.p + (long)sidx * na;
+    cstride = r->ctrl_stride;
+  }
+  const int ns = (int)r->sources.size();
+  launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p, r->G.p, r->t.p, ctrl, cstride,
+                     reinterpret_cast<const Source*>(r->d_sources.p), ns,
+                     r->src_amp.p + (long)sidx * std::max(ns, 1), r->stream);
+  launch_rk_forward((long)r->ninst * M_of(r), stage, dt, q, b1, b2, r->rhs.p, qcopy, r->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mgd_last_error(void) { return g_err.c_str(); }
+
+int mgd_device_count(int* count) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  *count = e == hipSuccess ? c : 0;
+  return MGD_OK;
+}
+
+int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region** out) {
+  if (!d || !coords || !out) { set_error("null argument"); return MGD_ERR_ARG; }
+  if (d->nd < 1 || d->nd > 3) { set_error("nd must be 1, 2 or 3"); return MGD_ERR_ARG; }
+  if (d->scheme != 0) { set_error("only the SBP 4-8 scheme is built"); return MGD_ERR_UNSUPPORTED; }
+  auto r = std::make_unique<mgd_region>();
+  r->desc = *d;
+  r->nd = d->nd;
+  r->nv = d->nd + 2;
+  r->ninst = std::max(1, d->n_instances);
+  for (int i = d->nd; i < 3; ++i) { r->desc.n[i] = 1; r->desc.periodicity[i] = MGD_PERIODIC_NONE; }
+  for (int i = 0; i < d->nd; ++i)
+    if (r->desc.periodicity[i] != MGD_PERIODIC_NONE && r->desc.n[i] < 9) {
+      set_error("periodic directions need at least 9 points for the SBP 4-8 interior stencil");
+      return MGD_ERR_ARG;
+    }
+  for (int i = 0; i < d->nd; ++i)
+    if (r->desc.n[i] < 2 * 12 + 1 && r->desc.periodicity[i] == MGD_PERIODIC_NONE) {
+      set_error("non-periodic directions need at least 25 points for the SBP 4-8 adjoint closure");
+      return MGD_ERR_ARG;
+    }
+  r->N = (long)r->desc.n[0] * r->desc.n[1] * r->desc.n[2];
+  const long N = r->N;
+  r->coords.assign(coords, coords + (size_t)N * r->nd);
+  // rectilinear check: coordinate d depends on index d only
+  for (int d0 = 0; d0 < r->nd; ++d0) {
+    const int len = r->desc.n[d0];
+    const long st = stride_of(r.get(), d0);
+    r->line[d0].resize(len);
+    for (int i = 0; i < len; ++i) r->line[d0][i] = r->coords[d0 * N + i * st];
+    for (long p = 0; p < N; ++p) {
+      int ijk[3];
+      unflat_host(r.get(), p, ijk);
+      if (r->coords[d0 * N + p] != r->line[d0][ijk[d0]]) {
+        set_error("grid is not rectilinear (curvilinear_domain grids are not built)");
+        return MGD_ERR_UNSUPPORTED;
+      }
+    }
+  }
+  // operators (Grid%setupSpatialDiscretization, src/GridImpl.f90:487-619)
+  Op first, diss, dissT, adj;
+  op_setup(&first, d->scheme, OP_FIRST);
+  op_setup(&diss, d->scheme, d->composite_dissipation ? OP_COMPOSITE : OP_DISS);
+  op_setup(&dissT, d->scheme, OP_DISS_T);
+  op_adjoint(first, &adj);
+  r->first_host = first;
+  r->ops[SLOT_FIRST] = first;
+  r->ops[SLOT_ADJ] = adj;
+  r->ops[SLOT_DISS] = diss;
+  r->ops[SLOT_DISS_T] = dissT;
+  // Grid%update for a rectilinear grid: per-direction coordinate derivative and norm weight
+  for (int d0 = 0; d0 < 3; ++d0) {
+    const int len = r->desc.n[d0];
+    if (d0 >= r->nd) {
+      r->h[d0].assign(1, 1.0);
+      r->hw[d0].assign(1, 1.0);
+      r->line[d0].assign(1, 0.0);
+      continue;
+    }
+    coordinate_derivative(first, len, r->desc.periodicity[d0], r->desc.periodic_length[d0],
+                          r->line[d0], r->h[d0]);
+    r->hw[d0].assign(len, 1.0);
+    if (r->desc.periodicity[d0] == MGD_PERIODIC_NONE)
+      for (int i = 0; i < first.bd; ++i) {
+        r->hw[d0][i] = first.norm[i];
+        r->hw[d0][len - 1 - i] = first.norm[i];
+      }
+  }
+  for (int d0 = 0; d0 < 3; ++d0) {
+    if (upload(r->d_h[d0], r->h[d0]) || upload(r->d_hw[d0], r->hw[d0]) ||
+        upload(r->d_x[d0], r->line[d0]))
+      return MGD_ERR_HIP;
+  }
+  if (r->d_ops.alloc(sizeof(r->ops) / sizeof(double) + 1)) return MGD_ERR_HIP;
+  MGD_HIP(hipMemcpy(r->d_ops.p, r->ops, sizeof(r->ops), hipMemcpyHostToDevice));
+  Geo& g = r->geo;
+  std::memset(&g, 0, sizeof g);
+  g.nd = r->nd; g.nv = r->nv; g.ninst = r->ninst; g.N = N;
+  for (int d0 = 0; d0 < 3; ++d0) {
+    g.n[d0] = r->desc.n[d0];
+    g.dir[d0].len = r->desc.n[d0];
+    g.dir[d0].periodic = r->desc.periodicity[d0] != MGD_PERIODIC_NONE;
+    g.dir[d0].off = r->desc.periodicity[d0] == MGD_PERIODIC_OVERLAP ? 1 : 0;
+    g.dir[d0].stride = stride_of(r.get(), d0);
+    g.h[d0] = r->d_h[d0].p;
+    g.hw[d0] = r->d_hw[d0].p;
+    g.x[d0] = r->d_x[d0].p;
+  }
+  g.ops = reinterpret_cast<const Op*>(r->d_ops.p);
+  g.gamma = d->gamma;
+  if (d->viscous) {
+    g.re_inv = d->reynolds > 0.0 ? 1.0 / d->reynolds : 0.0;
+    g.pr_inv = d->prandtl > 0.0 ? 1.0 / d->prandtl : 0.0;
+  }
+  g.power_law = d->power_law;
+  g.bulk_ratio = d->bulk_ratio;
+  g.diss_amount = d->dissipation_amount;
+  g.viscous = d->viscous;
+  g.dissipation = d->dissipation;
+  g.composite = d->composite_dissipation;
+  MGD_HIP(hipStreamCreate(&r->stream));
+  MGD_HIP(hipEventCreate(&r->ev0));
+  MGD_HIP(hipEventCreate(&r->ev1));
+  r->target.assign((size_t)r->nv * N, 0.0);
+  r->ctrl_moll.assign(N, 0.0);
+  r->tgt_moll.assign(N, 0.0);
+  if (int rc = ensure_state(r.get())) return rc;
+  *out = r.release();
+  return MGD_OK;
+}
+
+int mgd_region_destroy(mgd_region* r) {
+  if (!r) return MGD_OK;
+  if (r->stream) (void)hipStreamSynchronize(r->stream);
+  if (r->ev0) (void)hipEventDestroy(r->ev0);
+  if (r->ev1) (void)hipEventDestroy(r->ev1);
+  if (r->stream) (void)hipStreamDestroy(r->stream);
+  delete r;
+  return MGD_OK;
+}
+
+int mgd_region_add_patch(mgd_region* r, const mgd_patch_desc* pd) {
+  if (!r || !pd) { set_error("null argument"); return MGD_ERR_ARG; }
+  if ((int)r->patches.size() >= kMaxPatches) { set_error("too many patches"); return MGD_ERR_ARG; }
+  if (pd->type < MGD_PATCH_FAR_FIELD || pd->type > MGD_PATCH_KOLMOGOROV) {
+    set_error("unsupported patch type");
+    return MGD_ERR_UNSUPPORTED;
+  }
+  auto P = std::make_unique<HostPatch>();
+  P->pd = *pd;
+  P->np = 1;
+  for (int i = 0; i < 3; ++i) {
+    int a = pd->extent[2 * i], b = pd->extent[2 * i + 1];
+    if (i >= r->nd) { a = 1; b = 1; }
+    if (a < 0) a += r->desc.n[i] + 1;
+    if (b < 0) b += r->desc.n[i] + 1;
+    if (a < 1 || b > r->desc.n[i] || b < a) { set_error("invalid patch extent"); return MGD_ERR_ARG; }
+    P->lo[i] = a - 1;
+    P->hi[i] = b - 1;
+    P->ls[i] = b - a + 1;
+    P->np *= P->ls[i];
+  }
+  if (pd->type == MGD_PATCH_FAR_FIELD && (pd->normal_dir == 0 || std::abs(pd->normal_dir) > r->nd)) {
+    set_error("far-field patch needs a normal direction");
+    return MGD_ERR_ARG;
+  }
+  if (pd->type == MGD_PATCH_COST_TARGET) {
+    if (r->ip_cost >= 0) { set_error("one COST_TARGET patch is supported"); return MGD_ERR_UNSUPPORTED; }
+    r->ip_cost = (int)r->patches.size();
+  }
+  if (pd->type == MGD_PATCH_ACTUATOR) {
+    if (r->ip_act >= 0) { set_error("one ACTUATOR patch is supported"); return MGD_ERR_UNSUPPORTED; }
+    r->ip_act = (int)r->patches.size();
+  }
+  if (pd->type == MGD_PATCH_FAR_FIELD) r->has_farfield = 1;
+  r->patches.push_back(std::move(P));
+  r->finalized = false;
+  return MGD_OK;
+}
+
+int mgd_region_set_field(mgd_region* r, int which, const double* v) {
+  if (!r || !v) { set_error("null argument"); return MGD_ERR_ARG; }
+  const long N = r->N;
+  switch (which) {
+    case MGD_FIELD_TARGET_STATE: r->target.assign(v, v + (size_t)r->nv * N); break;
+    case MGD_FIELD_CONTROL_MOLLIFIER: r->ctrl_moll.assign(v, v + N); break;
+    case MGD_FIELD_TARGET_MOLLIFIER: r->tgt_moll.assign(v, v + N); break;
+    default: set_error("unknown field"); return MGD_ERR_ARG;
+  }
+  r->finalized = false;
+  return MGD_OK;
+}
+
+int mgd_region_add_source(mgd_region* r, const double* loc, double amplitude, double frequency,
+                          double radius, double phase) {
+  if (!r || !loc || radius <= 0.0) { set_error("bad acoustic source"); return MGD_ERR_ARG; }
+  Source s;
+  const double pi = 4.0 * std::atan(1.0);
+  for (int i = 0; i < 3; ++i) s.loc[i] = i < r->nd ? loc[i] : 0.0;
+  s.amplitude = amplitude;
+  s.omega = 2.0 * pi * frequency;
+  s.gfac = 9.0 / (2.0 * (radius * radius));
+  s.phase = phase;
+  r->sources.push_back(s);
+  r->finalized = false;
+  return MGD_OK;
+}
+
+int mgd_region_finalize(mgd_region* r) {
+  if (!r) { set_error("null region"); return MGD_ERR_ARG; }
+  const int nd = r->nd, nv = r->nv;
+  const long N = r->N;
+  const double g = r->desc.gamma;
+  std::memset(&r->hpatches, 0, sizeof r->hpatches);
+  r->hpatches.count = (int)r->patches.size();
+  for (size_t ip = 0; ip < r->patches.size(); ++ip) {
+    HostPatch& P = *r->patches[ip];
+    P.strength.clear();
+    P.data.clear();
+    double sig_inv = 0.0, sig_visc = 0.0;
+    switch (P.pd.type) {
+      case MGD_PATCH_FAR_FIELD: {
+        const int dir = std::abs(P.pd.normal_dir) - 1;
+        const double h0 = r->first_host.norm[0];
+        sig_inv = std::copysign(P.pd.p[0], (double)P.pd.normal_dir) / h0;
+        sig_visc = r->desc.viscous ? std::copysign(P.pd.p[1], (double)P.pd.normal_dir) / h0 : 0.0;
+        P.data.assign((size_t)P.np * nv * nv + (size_t)P.np * nv, 0.0);
+        for (int k = 0; k < P.np; ++k) {
+          const long p = patch_point(r, P, k);
+          int ijk[3];
+          unflat_host(r, p, ijk);
+          double qt[5], m[3] = {0, 0, 0};
+          for (int l = 0; l < nv; ++l) qt[l] = r->target[l * N + p];
+          m[dir] = metric_host(r, dir, ijk);
+          incoming_jacobian(nd, qt, m, g, P.pd.normal_dir, &P.data[(size_t)k * nv * nv]);
+          for (int l = 0; l < nv; ++l) P.data[(size_t)P.np * nv * nv + (size_t)k * nv + l] = qt[l];
+        }
+        if (r->desc.viscous) target_viscous_fluxes(r, P);
+        break;
+      }
+      case MGD_PATCH_SPONGE: {
+        // computeSpongeStrengths (src/PatchFactoryImpl.f90:161-374) on a rectilinear line
+        const int dir = std::abs(P.pd.normal_dir) - 1;
+        if (dir < 0 || dir >= nd) { set_error("sponge needs a normal direction"); return MGD_ERR_ARG; }
+        const std::vector<double>& hd = r->h[dir];
+        std::vector<double> cli(hd.size());
+        for (size_t i = 0; i < hd.size(); ++i) cli[i] = std::sqrt(hd[i] * hd[i]);
+        const int mn = P.lo[dir], mx = P.hi[dir];
+        P.strength.resize(P.np);
+        P.data.resize((size_t)P.np * nv);
+        for (int k = 0; k < P.np; ++k) {
+          const long p = patch_point(r, P, k);
+          int ijk[3];
+          unflat_host(r, p, ijk);
+          const int i = ijk[dir];
+          double a = 0.0, b = 0.0;
+          if (P.pd.normal_dir > 0) {
+            for (int c = mn; c <= i - 1; ++c) a += cli[c];
+            for (int c = mn; c <= mx - 1; ++c) b += cli[c];
+          } else {
+            for (int c = i + 1; c <= mx; ++c) a += cli[c];
+            for (int c = mn + 1; c <= mx; ++c) b += cli[c];
+          }
+          const double s = a / b;
+          P.strength[k] = P.pd.p[0] * std::pow(1.0 - s, P.pd.p[1]);
+          for (int l = 0; l < nv; ++l) P.data[(size_t)k * nv + l] = r->target[l * N + p];
+        }
+        break;
+      }
+      case MGD_PATCH_ACTUATOR: {
+        P.strength.resize(P.np);
+        for (int k = 0; k < P.np; ++k) P.strength[k] = r->ctrl_moll[patch_point(r, P, k)];
+        break;
+      }
+      case MGD_PATCH_COST_TARGET: {
+        // mean pressure from the target state (src/AcousticNoiseImpl.f90:56-71)
+        P.strength.resize(P.np);
+        P.data.resize(P.np);
+        for (int k = 0; k < P.np; ++k) {
+          const long p = patch_point(r, P, k);
+          P.strength[k] = r->tgt_moll[p];
+          double s = 0.0;
+          for (int i = 0; i < nd; ++i) s += r->target[(i + 1) * N + p] * r->target[(i + 1) * N + p];
+          P.data[k] = (g - 1.0) * (r->target[(nd + 1) * N + p] - 0.5 * s / r->target[p]);
+        }
+        break;
+      }
+      case MGD_PATCH_KOLMOGOROV: {
+        const double pi = 4.0 * std::atan(1.0);
+        const int wn = P.pd.p[0] > 0 ? (int)P.pd.p[0] : 0;
+        P.strength.resize(P.np);
+        for (int k = 0; k < P.np; ++k) {
+          const long p = patch_point(r, P, k);
+          const double y = nd > 1 ? r->coords[1 * N + p] : 0.0;
+          P.strength[k] = P.pd.p[1] * std::sin(2.0 * pi * wn * y);
+        }
+        break;
+      }
+    }
+    if (upload(P.d_strength, P.strength) || upload(P.d_data, P.data) || upload(P.d_tvflux, P.tvflux))
+      return MGD_ERR_HIP;
+    DPatch& D = r->hpatches.p[ip];
+    D.type = P.pd.type;
+    D.normal_dir = P.pd.normal_dir;
+    for (int i = 0; i < 3; ++i) { D.lo[i] = P.lo[i]; D.hi[i] = P.hi[i]; D.ls[i] = P.ls[i]; }
+    D.np = P.np;
+    D.sigma_inv = sig_inv;
+    D.sigma_visc = sig_visc;
+    D.strength = P.d_strength.p;
+    D.aplus = P.d_data.p;
+    D.tvflux = P.d_tvflux.p;
+  }
+  if (r->d_patches.alloc(sizeof(Patches) / sizeof(double) + 1)) return MGD_ERR_HIP;
+  MGD_HIP(hipMemcpy(r->d_patches.p, &r->hpatches, sizeof(Patches), hipMemcpyHostToDevice));
+  if (r->d_sources.alloc((sizeof(Source) * std::max<size_t>(1, r->sources.size())) / sizeof(double) + 1))
+    return MGD_ERR_HIP;
+  if (!r->sources.empty())
+    MGD_HIP(hipMemcpy(r->d_sources.p, r->sources.data(), sizeof(Source) * r->sources.size(),
+                      hipMemcpyHostToDevice));
+  if (r->acc_J.alloc(r->ninst) || r->acc_S.alloc(r->ninst)) return MGD_ERR_HIP;
+  const int npr = std::max(reduce_blocks(std::max(n_cost(r), 1)), reduce_blocks(std::max(n_act(r), 1)));
+  if (r->partial.alloc((size_t)npr * r->ninst)) return MGD_ERR_HIP;
+  if (n_cost(r) > 0 && r->aforce.alloc((size_t)r->ninst * nv * n_cost(r))) return MGD_ERR_HIP;
+  r->finalized = true;
+  return MGD_OK;
+}
+
+int mgd_state_put(mgd_region* r, int which, int inst, const double* host) {
+  if (!r || !host || inst < 0 || inst >= r->ninst) { set_error("bad argument"); return MGD_ERR_ARG; }
+  DevBuf* b = which == MGD_CONSERVED ? &r->q : which == MGD_ADJOINT_VARS ? &r->lam
+            : which == MGD_RHS ? &r->rhs : nullptr;
+  if (!b) { set_error("unknown state array"); return MGD_ERR_ARG; }
+  const size_t M = (size_t)M_of(r);
+  MGD_HIP(hipMemcpyAsync(b->p + inst * M, host, M * sizeof(double), hipMemcpyHostToDevice, r->stream));
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  return MGD_OK;
+}
+
+int mgd_state_get(mgd_region* r, int which, int inst, double* host) {
+  if (!r || !host || inst < 0 || inst >= r->ninst) { set_error("bad argument"); return MGD_ERR_ARG; }
+  DevBuf* b = which == MGD_CONSERVED ? &r->q : which == MGD_ADJOINT_VARS ? &r->lam
+            : which == MGD_RHS ? &r->rhs : nullptr;
+  if (!b) { set_error("unknown state array"); return MGD_ERR_ARG; }
+  const size_t M = (size_t)M_of(r);
+  MGD_HIP(hipMemcpyAsync(host, b->p + inst * M, M * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  return MGD_OK;
+}
+
+int mgd_compute_rhs(mgd_region* r, int mode, double time, const double* control, double factor) {
+  if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
+  const int ns = (int)r->sources.size();
+  std::vector<double> amp;
+  source_table(r, std::vector<double>{time}, amp);
+  if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
+  if (mode == MGD_FORWARD) {
+    const int na = n_act(r);
+    const double* ctrl = nullptr;
+    if (control && na > 0) {
+      std::vector<double> c(control, control + (size_t)r->ninst * na);
+      if (upload(r->control, c)) return MGD_ERR_HIP;
+      ctrl = r->control.p;
+    }
+    launch_rhs_forward(r->geo, dpatches(r), r->q.p, r->rhs.p, r->w.p, r->G.p, r->t.p, ctrl, na,
+                       reinterpret_cast<const Source*>(r->d_sources.p), ns, r->src_amp.p, r->stream);
+  } else if (mode == MGD_ADJOINT) {
+    if (int rc = ensure_adjoint(r)) return rc;
+    const int nc = n_cost(r);
+    launch_rhs_adjoint(r->geo, dpatches(r), r->q.p, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
+                       r->T1.p, r->S.p, nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc, factor,
+                       r->has_farfield, r->stream);
+  } else {
+    set_error("mode must be MGD_FORWARD or MGD_ADJOINT");
+    return MGD_ERR_ARG;
+  }
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  return MGD_OK;
+}
+
+int mgd_update_adjoint_forcing(mgd_region* r, int zero) {
+  if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
+  if (r->ip_cost < 0) return MGD_OK;
+  launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), n_cost(r), r->q.p, r->aforce.p, zero, r->stream);
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  return MGD_OK;
+}
+
+int mgd_functional(mgd_region* r, double* out) {
+  if (!r || !r->finalized || !out) { set_error("bad argument"); return MGD_ERR_ARG; }
+  if (r->ip_cost < 0) {
+    for (int i = 0; i < r->ninst; ++i) out[i] = 0.0;
+    return MGD_OK;
+  }
+  launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), n_cost(r), 0, r->q.p, r->partial.p, 0.0,
+                      nullptr, r->acc_J.p, r->stream);
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipMemcpyAsync(out, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost, r->stream));
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  return MGD_OK;
+}
+
+int mgd_run_forward(mgd_region* r, double t0, double dt, int nsteps, const double* control, double* J,
+                    int use_graph) {
+  if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
+  if (nsteps < 0 || !(dt > 0.0)) { set_error("bad time stepping"); return MGD_ERR_ARG; }
+  (void)use_graph;
+  const int na = n_act(r), nst = 4 * nsteps;
+  if (control && na > 0) {
+    std::vector<double> c(control, control + (size_t)r->ninst * nst * na);
+    if (upload(r->control, c)) return MGD_ERR_HIP;
+    r->ctrl_stride = (long)nst * na;
+  } else if (r->control.p) {
+    r->control.alloc(0);
+  }
+  std::vector<double> amp;
+  source_table(r, stage_times(t0, dt, std::max(nsteps, 1)), amp);
+  if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
+  launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
+  const long M = (long)r->ninst * M_of(r);
+  MGD_HIP(hipEventRecord(r->ev0, r->stream));
+  for (int s = 0; s < nsteps; ++s)
+    for (int i = 1; i <= 4; ++i) {
+      const int sidx = 4 * s + i - 1;
+      forward_stage(r, r->q.p, r->b1.p, r->b2.p, sidx, i, dt, nullptr);
+      if (r->ip_cost >= 0)
+        launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), n_cost(r), 0, r->q.p, r->partial.p,
+                            kRkNorm[i - 1] * dt, r->acc_J.p, nullptr, r->stream);
+    }
+  MGD_HIP(hipEventRecord(r->ev1, r->stream));
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  float ms = 0.f;
+  MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+  r->last_ms[0] = ms;
+  (void)M;
+  if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
+  return MGD_OK;
+}
+
+int mgd_run_adjoint(mgd_region* r, double t0, double dt, int nsteps, int save_interval,
+                    const double* control, double* gradient, double* sensitivity, double* J,
+                    int use_graph) {
+  if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
+  if (nsteps <= 0 || save_interval <= 0 || !(dt > 0.0)) { set_error("bad time stepping"); return MGD_ERR_ARG; }
+  (void)use_graph;
+  if (int rc = ensure_adjoint(r)) return rc;
+  const int K = save_interval;
+  const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
+  const long Mi = M_of(r);
+  const long M = (long)r->ninst * Mi;
+  if (control && na > 0) {
+    std::vector<double> c(control, control + (size_t)r->ninst * nst * na);
+    if (upload(r->control, c)) return MGD_ERR_HIP;
+    r->ctrl_stride = (long)nst * na;
+  } else if (r->control.p) {
+    r->control.alloc(0);
+  }
+  if (na > 0 && r->grad.alloc((size_t)r->ninst * nst * na)) return MGD_ERR_HIP;
+  std::vector<double> amp;
+  source_table(r, stage_times(t0, dt, nsteps), amp);
+  if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
+  const int nsnap = nsteps / K + 1;
+  if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M) || r->qs.alloc(M))
+    return MGD_ERR_HIP;
+  launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
+  launch_fill(r->ninst, 0.0, r->acc_S.p, r->stream);
+  MGD_HIP(hipEventRecord(r->ev0, r->stream));
+  // forward run with snapshots at t0 and every K steps (showProgress .q files)
+  launch_copy(M, r->q.p, r->snaps.p, r->stream);
+  for (int s = 0; s < nsteps; ++s) {
+    for (int i = 1; i <= 4; ++i) {
+      forward_stage(r, r->q.p, r->b1.p, r->b2.p, 4 * s + i - 1, i, dt, nullptr);
+      if (nc > 0)
+        launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), nc, 0, r->q.p, r->partial.p,
+                            kRkNorm[i - 1] * dt, r->acc_J.p, nullptr, r->stream);
+    }
+    if ((s + 1) % K == 0) launch_copy(M, r->q.p, r->snaps.p + (long)((s + 1) / K) * M, r->stream);
+  }
+  // adjoint terminal condition (loadInitialCondition ADJOINT, src/SolverImpl.f90:368-428)
+  launch_fill(M, 0.0, r->lam.p, r->stream);
+  if (nc > 0) {
+    launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->q.p, r->aforce.p, 0, r->stream);
+    launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -dt / 6.0, r->lam.p, r->stream);
+  }
+  int loaded = -1;
+  for (int t = nsteps - 1; t >= 0; --t) {
+    for (int i = 4; i >= 1; --i) {
+      const int ts = i == 1 ? t : t + 1;
+      const int st = i == 1 ? 4 : i - 1;
+      const bool need = loaded == -1 || ts < loaded || ts > loaded + K ||
+                        (ts == loaded && st < 4) || (ts == loaded + K && st == 4);
+      if (need) {
+        int lt;
+        if (ts % K == 0 && st == 4) lt = ts;
+        else if (ts % K == 0) lt = ts - K;
+        else lt = ts - ts % K;
+        loaded = lt;
+        launch_copy(M, r->snaps.p + (long)(lt / K) * M, r->qs.p, r->stream);
+        launch_copy(M, r->qs.p, r->window.p, r->stream);
+        if (loaded != nsteps) {
+          int idx = 1;
+          for (int s2 = loaded + 1; s2 <= loaded + K && s2 <= nsteps; ++s2)
+            for (int st2 = 1; st2 <= 4; ++st2) {
+              const bool last = s2 == loaded + K && st2 == 4;
+              forward_stage(r, r->qs.p, r->b1.p, r->b2.p, 4 * (s2 - 1) + st2 - 1, st2, dt,
+                            last ? nullptr : r->window.p + (long)idx * M);
+              if (last) break;
+              ++idx;
+            }
+        }
+      }
+      const int widx = (ts - 1 - loaded) * 4 + st;
+      const double* qcur = r->window.p + (long)widx * M;
+      const int sidx = 4 * t + i - 1;
+      if (na > 0) {
+        if (gradient)
+          launch_gradient(r->geo, dpatch(r, r->ip_act), na, r->lam.p, r->grad.p + (long)sidx * na,
+                          (long)nst * na, r->stream);
+        launch_patch_reduce(r->geo, dpatch(r, r->ip_act), na, 1, r->lam.p, r->partial.p,
+                            kRkNorm[i - 1] * dt, r->acc_S.p, nullptr, r->stream);
+      }
+      if (nc > 0)
+        launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, qcur, r->aforce.p, t == 0 && i == 1,
+                           r->stream);
+      const double factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
+      launch_rhs_adjoint(r->geo, dpatches(r), qcur, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p,
+                         r->S.p, nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc, factor,
+                         r->has_farfield, r->stream);
+      launch_rk_adjoint(M, i, dt, r->lam.p, r->b1a.p, r->b2a.p, r->rhs.p, r->stream);
+    }
+  }
+  MGD_HIP(hipEventRecord(r->ev1, r->stream));
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  float ms = 0.f;
+  MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+  r->last_ms[0] = ms;
+  if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
+  if (sensitivity) MGD_HIP(hipMemcpy(sensitivity, r->acc_S.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
+  if (gradient && na > 0)
+    MGD_HIP(hipMemcpy(gradient, r->grad.p, sizeof(double) * r->ninst * nst * na, hipMemcpyDeviceToHost));
+  return MGD_OK;
+}
+
+int mgd_stencil_apply(mgd_region* r, int op, int dir, double* host, int ncomp) {
+  if (!r || !host || ncomp <= 0 || dir < 0 || dir >= r->nd || op < 0 || op >= kNumSlots) {
+    set_error("bad argument");
+    return MGD_ERR_ARG;
+  }
+  DevBuf a, b;
+  const size_t n = (size_t)ncomp * r->N;
+  if (a.alloc(n) || b.alloc(n)) return MGD_ERR_HIP;
+  MGD_HIP(hipMemcpy(a.p, host, n * sizeof(double), hipMemcpyHostToDevice));
+  launch_apply(r->geo, op, dir, ncomp, a.p, b.p, r->stream);
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  MGD_HIP(hipMemcpy(host, b.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  return MGD_OK;
+}
+
+int mgd_region_get_grid(mgd_region* r, int which, double* out) {
+  if (!r || !out) { set_error("bad argument"); return MGD_ERR_ARG; }
+  const int nd = r->nd;
+  const long N = r->N;
+  for (long p = 0; p < N; ++p) {
+    int ijk[3];
+    unflat_host(r, p, ijk);
+    double det = r->h[0][ijk[0]];
+    if (nd > 1) det = det * r->h[1][ijk[1]];
+    if (nd > 2) det = det * r->h[2][ijk[2]];
+    switch (which) {
+      case 0:
+        for (int a = 0; a < nd; ++a)
+          for (int b = 0; b < nd; ++b) out[(a * nd + b) * N + p] = a == b ? metric_host(r, a, ijk) : 0.0;
+        break;
+      case 1: out[p] = 1.0 / det; break;
+      case 2: {
+        double s = 1.0;
+        for (int d = 0; d < nd; ++d) s = r->hw[d][ijk[d]] * s;
+        out[p] = s * det;
+        break;
+      }
+      case 3:
+        for (int d = 0; d < nd; ++d) out[d * N + p] = std::fabs(metric_host(r, d, ijk));
+        break;
+      default: set_error("unknown grid array"); return MGD_ERR_ARG;
+    }
+  }
+  return MGD_OK;
+}
+
+int mgd_last_timings(mgd_region* r, double* out, int* launches) {
+  if (!r) return MGD_ERR_ARG;
+  for (int i = 0; i < 3; ++i) {
+    if (out) out[i] = r->last_ms[i];
+    if (launches) launches[i] = r->last_launch[i];
+  }
+  return MGD_OK;
+}
+
+int mgd_sync(mgd_region* r) {
+  if (!r) return MGD_ERR_ARG;
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  return MGD_OK;
+}
+
+}  // extern "C"

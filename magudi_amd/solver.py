@@ -1,0 +1,205 @@
+"""Host-side mirror of magudi's Region / State / Patch / Solver API over the HIP path.
+
+The names and argument meanings follow the reference types:
+
+* ``Region``   -- t_Region (include/Region.f90): ``setup``, ``computeRhs(mode)``,
+                  ``getTimeStepSize``; owns the HBM-resident states of
+                  ``n_instances`` independent copies of one grid (batching).
+* ``State``    -- t_State (include/State.f90): conserved / adjoint / rhs arrays,
+                  accessed with ``region.states[i]`` as host numpy arrays.
+* ``Solver``   -- t_Solver (include/Solver.f90): ``runForward``, ``runAdjoint``.
+
+Every call goes through libmagudi_hip.so; errors raise ``MagudiError`` (the
+reference's ``gracefulExit``).  Nothing here computes the physics on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import capi
+from .capi import FORWARD, ADJOINT, check, dptr
+from .cases import Case
+
+__all__ = ["Region", "State", "Solver", "FORWARD", "ADJOINT"]
+
+
+class State:
+    """View of one instance's arrays (t_State)."""
+
+    def __init__(self, region: "Region", index: int):
+        self._r = region
+        self.index = index
+
+    def _get(self, which):
+        out = np.empty((self._r.nv, self._r.npoints))
+        check(capi.lib().mgd_state_get(self._r.handle, which, self.index, dptr(out)))
+        return out
+
+    def _put(self, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self._r.nv, self._r.npoints)
+        check(capi.lib().mgd_state_put(self._r.handle, which, self.index, dptr(a)))
+
+    @property
+    def conservedVariables(self):
+        return self._get(capi.CONSERVED)
+
+    @conservedVariables.setter
+    def conservedVariables(self, v):
+        self._put(capi.CONSERVED, v)
+
+    @property
+    def adjointVariables(self):
+        return self._get(capi.ADJOINT_VARS)
+
+    @adjointVariables.setter
+    def adjointVariables(self, v):
+        self._put(capi.ADJOINT_VARS, v)
+
+    @property
+    def rightHandSide(self):
+        return self._get(capi.RHS)
+
+
+class Region:
+    """t_Region over the HIP path.  ``Region(case, n_instances)`` performs
+    Region%setup, Grid%update and the patch / functional / controller set-up of
+    Solver%setup (sponge strengths, far-field penalties, mean pressure)."""
+
+    def __init__(self, case: Case, n_instances: int = 1):
+        self.case = case
+        self.nd = case.nd
+        self.nv = case.nv
+        self.npoints = case.npoints
+        self.n_instances = n_instances
+        d = capi.RegionDesc()
+        d.nd = case.nd
+        for i in range(3):
+            d.n[i] = case.n[i]
+            d.periodicity[i] = case.periodicity[i]
+            d.periodic_length[i] = case.periodic_length[i]
+        d.scheme = case.scheme
+        d.viscous = int(case.viscous)
+        d.dissipation = int(case.dissipation)
+        d.composite_dissipation = int(case.composite_dissipation)
+        d.gamma = case.gamma
+        d.reynolds = case.reynolds
+        d.prandtl = case.prandtl
+        d.power_law = case.power_law
+        d.bulk_ratio = case.bulk_ratio
+        d.dissipation_amount = case.dissipation_amount
+        d.n_instances = n_instances
+        coords = np.ascontiguousarray(case.coords, dtype=np.float64)
+        h = C.c_void_p()
+        check(capi.lib().mgd_region_create(C.byref(d), dptr(coords), C.byref(h)))
+        self.handle = h
+        for p in case.patches:
+            pd = capi.PatchDesc()
+            pd.type = p.type
+            pd.normal_dir = p.normal_dir
+            for i in range(6):
+                pd.extent[i] = p.extent[i]
+            for i, v in enumerate(list(p.params)[:4]):
+                pd.p[i] = v
+            check(capi.lib().mgd_region_add_patch(self.handle, C.byref(pd)))
+        for which, arr in ((capi.FIELD_TARGET_STATE, case.target),
+                           (capi.FIELD_CONTROL_MOLLIFIER, case.control_mollifier),
+                           (capi.FIELD_TARGET_MOLLIFIER, case.target_mollifier)):
+            if arr is not None:
+                a = np.ascontiguousarray(arr, dtype=np.float64)
+                check(capi.lib().mgd_region_set_field(self.handle, which, dptr(a)))
+        for s in case.sources:
+            loc = np.array((list(s.location) + [0.0, 0.0, 0.0])[:3], dtype=np.float64)
+            check(capi.lib().mgd_region_add_source(self.handle, dptr(loc), s.amplitude,
+                                                   s.frequency, s.radius, s.phase))
+        check(capi.lib().mgd_region_finalize(self.handle))
+        self.states = [State(self, i) for i in range(n_instances)]
+        act = case.patches_of(3)
+        self.n_actuator = len(case.patch_points(act[0])) if act else 0
+
+    def close(self):
+        if getattr(self, "handle", None):
+            capi.lib().mgd_region_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- t_Region procedures --------------------------------------------------
+    def computeRhs(self, mode: int, time: float = 0.0, control: Optional[np.ndarray] = None,
+                   forcing_factor: float = 1.0):
+        ctl = None if control is None else np.ascontiguousarray(control, dtype=np.float64)
+        check(capi.lib().mgd_compute_rhs(self.handle, mode, time, dptr(ctl), forcing_factor))
+
+    def updateAdjointForcing(self, no_adjoint_forcing: bool = False):
+        check(capi.lib().mgd_update_adjoint_forcing(self.handle, int(no_adjoint_forcing)))
+
+    def getTimeStepSize(self) -> float:
+        return self.case.dt
+
+    def functional(self) -> np.ndarray:
+        out = np.zeros(self.n_instances)
+        check(capi.lib().mgd_functional(self.handle, dptr(out)))
+        return out
+
+    def grid(self, which: int) -> np.ndarray:
+        size = {0: self.nd * self.nd, 1: 1, 2: 1, 3: self.nd}[which]
+        out = np.zeros((size, self.npoints))
+        check(capi.lib().mgd_region_get_grid(self.handle, which, dptr(out)))
+        return out
+
+    def applyOperator(self, op: int, direction: int, x: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(x, dtype=np.float64).copy()
+        ncomp = a.size // self.npoints
+        check(capi.lib().mgd_stencil_apply(self.handle, op, direction, dptr(a), ncomp))
+        return a
+
+    def timings(self):
+        out = np.zeros(3)
+        n = (C.c_int * 3)()
+        check(capi.lib().mgd_last_timings(self.handle, dptr(out), n))
+        return out, list(n)
+
+
+class Solver:
+    """t_Solver: time marching over a Region (RK4, uniform checkpointing in HBM)."""
+
+    def __init__(self, region: Region, dt: Optional[float] = None, nsteps: Optional[int] = None,
+                 save_interval: Optional[int] = None):
+        self.region = region
+        self.dt = dt if dt is not None else region.case.dt
+        self.nTimesteps = nsteps if nsteps is not None else region.case.nsteps
+        self.saveInterval = save_interval if save_interval is not None else region.case.save_interval
+
+    def runForward(self, control: Optional[np.ndarray] = None, t0: float = 0.0,
+                   use_graph: bool = False) -> np.ndarray:
+        """Solver%runForward from the current conserved state; returns J per instance."""
+        r = self.region
+        J = np.zeros(r.n_instances)
+        ctl = None if control is None else np.ascontiguousarray(control, dtype=np.float64)
+        check(capi.lib().mgd_run_forward(r.handle, t0, self.dt, self.nTimesteps, dptr(ctl),
+                                         dptr(J), int(use_graph)))
+        return J
+
+    def runAdjoint(self, control: Optional[np.ndarray] = None, t0: float = 0.0,
+                   want_gradient: bool = True, use_graph: bool = False):
+        """Solver%runAdjoint (forward pass with HBM snapshots, then the discrete adjoint).
+
+        Returns (sensitivity, gradient, J): sensitivity and J per instance; gradient per
+        instance, per RK stage in forward order, per actuator patch point."""
+        r = self.region
+        J = np.zeros(r.n_instances)
+        sens = np.zeros(r.n_instances)
+        grad = None
+        if want_gradient and r.n_actuator > 0:
+            grad = np.zeros((r.n_instances, 4 * self.nTimesteps, r.n_actuator))
+        ctl = None if control is None else np.ascontiguousarray(control, dtype=np.float64)
+        check(capi.lib().mgd_run_adjoint(r.handle, t0, self.dt, self.nTimesteps, self.saveInterval,
+                                         dptr(ctl), dptr(grad), dptr(sens), dptr(J),
+                                         int(use_graph)))
+        return sens, grad, J

@@ -1,0 +1,195 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Tolerances (stated by the task for FP64): per-step RHS fields 1e-13 max-norm
+relative; J 1e-12 relative; adjoint gradient and sensitivity 1e-10 relative.
+Inputs are seeded; sizes are small enough for the oracle to finish in seconds.
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from magudi_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+RHS_TOL = 1e-13
+J_TOL = 1e-12
+GRAD_TOL = 1e-10
+
+
+def maxrel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def solver_mod():
+    from magudi_amd import solver
+    return solver
+
+
+def perturbed(case, seed, amp=0.05):
+    rng = np.random.default_rng(seed)
+    q = case.target + amp * rng.standard_normal((case.nv, case.npoints))
+    q[0] = np.abs(q[0]) + 0.2
+    return q
+
+
+def viscous_monopole(n=41, visc_penalty=0.7):
+    c = cases.acoustic_monopole(n=n, nsteps=1)
+    c.viscous = True
+    c.reynolds = 200.0
+    c.prandtl = 0.7
+    c.patches = [cases.Patch(p.name, p.type, p.normal_dir, p.extent,
+                             (1.0, visc_penalty, 0, 0) if p.type == cases.FAR_FIELD else p.params)
+                 for p in c.patches]
+    return c
+
+
+def kolmogorov_small(n=16):
+    c = cases.kolmogorov3d(n=n, nsteps=1)
+    c.power_law = 0.666
+    c.bulk_ratio = 0.6
+    c.dissipation = True
+    c.dissipation_amount = 0.01
+    return c
+
+
+CASES = {
+    "monopole41": lambda: cases.acoustic_monopole(n=41, nsteps=1),
+    "monopole41_composite": lambda: cases.acoustic_monopole(n=41, nsteps=1, composite=True),
+    "viscous_farfield41": viscous_monopole,
+    "kolmogorov16": kolmogorov_small,
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_grid_arrays(solver_mod, name):
+    c = CASES[name]()
+    R = solver_mod.Region(c)
+    O = ob.Region(c)
+    for which in range(4):
+        assert maxrel(R.grid(which), O.grid(which)) < 1e-13, which
+
+
+@pytest.mark.parametrize("op", [0, 1, 2, 3])
+@pytest.mark.parametrize("direction", [0, 1])
+def test_stencil_apply(solver_mod, op, direction):
+    c = cases.acoustic_monopole(n=41, nsteps=1)
+    R = solver_mod.Region(c)
+    rng = np.random.default_rng(op * 7 + direction)
+    x = rng.standard_normal((3, c.npoints))
+    y = R.applyOperator(op, direction, x)
+    kinds = {0: ob.FIRST, 1: ob.FIRST, 2: ob.DISS, 3: ob.DISS_T}
+    o = ob.op(kinds[op])
+    if op == 1:
+        o = ob.adjoint(o)
+    ref = ob.apply(o, direction, c.n, 0, x.copy())
+    assert maxrel(y, ref) < 1e-14
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("mode", [1, -1])
+def test_rhs_parity(solver_mod, name, mode):
+    c = CASES[name]()
+    R = solver_mod.Region(c)
+    O = ob.Region(c)
+    q = perturbed(c, 11)
+    lam = np.random.default_rng(12).standard_normal((c.nv, c.npoints))
+    R.states[0].conservedVariables = q
+    R.states[0].adjointVariables = lam
+    t = 1.7
+    na = R.n_actuator
+    ctl = np.random.default_rng(13).standard_normal(na) if na else None
+    if mode == 1:
+        R.computeRhs(1, time=t, control=ctl)
+        ref = O.rhs(1, q, time=t, control=ctl)
+    else:
+        R.updateAdjointForcing()
+        R.computeRhs(-1, forcing_factor=0.5)
+        af = O.adjoint_forcing(q) if ob.lib().orc_cost_target_npoints(O.h) > 0 else None
+        ref = O.rhs(-1, q, lam, aforcing=af, factor=0.5)
+    got = R.states[0].rightHandSide
+    assert np.all(np.isfinite(got))
+    assert maxrel(got, ref) < RHS_TOL, maxrel(got, ref)
+
+
+def test_functional_parity(solver_mod):
+    c = cases.acoustic_monopole(n=41, nsteps=1)
+    R = solver_mod.Region(c)
+    O = ob.Region(c)
+    q = perturbed(c, 21, amp=0.01)
+    R.states[0].conservedVariables = q
+    J = R.functional()[0]
+    assert abs(J - O.functional(q)) / abs(O.functional(q)) < J_TOL
+
+
+def test_run_forward_parity(solver_mod):
+    c = cases.acoustic_monopole(n=41, nsteps=20)
+    c.dt = 0.2
+    R = solver_mod.Region(c)
+    O = ob.Region(c)
+    q0 = c.initial_state()
+    R.states[0].conservedVariables = q0
+    S = solver_mod.Solver(R, dt=c.dt, nsteps=c.nsteps)
+    J = S.runForward()[0]
+    Jo, qo = O.run_forward(q0, c.dt, c.nsteps)
+    assert abs(J - Jo) / abs(Jo) < J_TOL
+    assert maxrel(R.states[0].conservedVariables - q0, qo - q0) < 1e-12
+
+
+def test_run_adjoint_parity_and_fd(solver_mod):
+    c = cases.acoustic_monopole(n=41, nsteps=24, save_interval=5)
+    c.dt = 0.2
+    R = solver_mod.Region(c)
+    O = ob.Region(c)
+    q0 = c.initial_state()
+    R.states[0].conservedVariables = q0
+    S = solver_mod.Solver(R, dt=c.dt, nsteps=c.nsteps, save_interval=c.save_interval)
+    sens, grad, J = S.runAdjoint()
+    so, go, lo, Jo = O.run_adjoint(q0, c.dt, c.nsteps, c.save_interval)
+    assert abs(J[0] - Jo) / abs(Jo) < J_TOL
+    assert abs(sens[0] - so) / abs(so) < GRAD_TOL
+    assert maxrel(grad[0], go) < GRAD_TOL
+    assert maxrel(R.states[0].adjointVariables, lo) < GRAD_TOL
+    # finite-difference gradient check on the GPU path itself (README.md:80-157)
+    R.states[0].conservedVariables = q0
+    A = 1e-3
+    J1 = S.runForward(control=A * grad[0])[0]
+    assert abs((J1 - J[0]) / A - sens[0]) / sens[0] < 1e-3
+
+
+def test_batched_instances_match_single(solver_mod):
+    c = cases.acoustic_monopole(n=41, nsteps=12, save_interval=4)
+    c.dt = 0.2
+    q0 = c.initial_state()
+    R1 = solver_mod.Region(c)
+    na = R1.n_actuator
+    rng = np.random.default_rng(5)
+    amps = [0.0, 1e-2, 3e-2]
+    ctl = np.stack([a * rng.standard_normal((4 * c.nsteps, na)) for a in amps])
+    Rb = solver_mod.Region(c, n_instances=3)
+    for i in range(3):
+        Rb.states[i].conservedVariables = q0
+    Sb = solver_mod.Solver(Rb, dt=c.dt, nsteps=c.nsteps, save_interval=c.save_interval)
+    sens_b, grad_b, J_b = Sb.runAdjoint(control=ctl)
+    S1 = solver_mod.Solver(R1, dt=c.dt, nsteps=c.nsteps, save_interval=c.save_interval)
+    for i in range(3):
+        R1.states[0].conservedVariables = q0
+        s1, g1, J1 = S1.runAdjoint(control=ctl[i:i + 1])
+        assert J_b[i] == J1[0]
+        assert s1[0] == sens_b[i]
+        assert np.array_equal(g1[0], grad_b[i])
+
+
+def test_viscous_3d_adjoint_run(solver_mod):
+    c = kolmogorov_small(n=12)
+    c.nsteps = 4
+    c.save_interval = 2
+    R = solver_mod.Region(c)
+    q0 = c.initial_state()
+    R.states[0].conservedVariables = q0
+    S = solver_mod.Solver(R, dt=c.dt, nsteps=4, save_interval=2)
+    S.runForward()
+    O = ob.Region(c)
+    _, qo = O.run_forward(q0, c.dt, 4)
+    assert maxrel(R.states[0].conservedVariables - q0, qo - q0) < 1e-11
