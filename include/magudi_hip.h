@@ -171,6 +171,11 @@ int mgd_region_get_grid(mgd_region* r, int which, double* host);
  * out[0] total, out[1] forward RHS, out[2] adjoint RHS; counts in launches[]. */
 int mgd_last_timings(mgd_region* r, double* out, int* launches);
 
+/* Region options.  MGD_OPT_FORCE_PLAIN = 1 runs the generic per-operator kernels even
+ * where the fused 2-D stage kernels apply (comparison and testing). */
+#define MGD_OPT_FORCE_PLAIN 1
+int mgd_region_set_option(mgd_region* r, int option, int value);
+
 /* Waits for all work of the region. */
 int mgd_sync(mgd_region* r);
 

@@ -19,6 +19,7 @@ FORWARD, ADJOINT = 1, -1
 CONSERVED, ADJOINT_VARS, RHS = 0, 1, 2
 FIELD_TARGET_STATE, FIELD_CONTROL_MOLLIFIER, FIELD_TARGET_MOLLIFIER = 0, 1, 2
 OP_FIRST, OP_ADJOINT_FIRST, OP_DISS, OP_DISS_T = 0, 1, 2, 3
+OPT_FORCE_PLAIN = 1
 
 
 class RegionDesc(C.Structure):
@@ -77,6 +78,7 @@ def lib():
         "mgd_region_get_grid": [vp, C.c_int, dp],
         "mgd_last_timings": [vp, dp, C.POINTER(C.c_int)],
         "mgd_sync": [vp],
+        "mgd_region_set_option": [vp, C.c_int, C.c_int],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "fused2d.h"
 #include "kernels.h"
 #include "mgd_internal.h"
 
@@ -78,6 +79,11 @@ struct mgd_region {
   mgd::DevBuf q, lam, rhs, b1, b2, b1a, b2a, t, w, G, T1, S;
   mgd::DevBuf qs, aforce, control, grad, acc_J, acc_S, partial, src_amp;
   mgd::DevBuf snaps, window;
+  // fused 2-D path: ping-pong stage buffers, per-tile partials, quadrature weights
+  mgd::DevBuf qy, ly, l0, jpart, spart, ssum, wts, rank_cost, rank_act;
+  int ntiles_cost = 0, ntiles_act = 0;
+  int force_plain = 0;
+  int aforce_zero = 0;
   long ctrl_stride = 0;                  // control entries per instance
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -380,6 +386,171 @@ void forward_stage(mgd_region* r, double* q, double* b1, double* b2, int sidx, i
                      reinterpret_cast<const Source*>(r->d_sources.p), ns,
                      r->src_amp.p + (long)sidx * std::max(ns, 1), r->stream);
   launch_rk_forward((long)r->ninst * M_of(r), stage, dt, q, b1, b2, r->rhs.p, qcopy, r->stream);
+}
+
+bool fused_ok(const mgd_region* r) {
+  return !r->force_plain && r->nd == 2 && !r->desc.viscous;
+}
+
+int setup_fused(mgd_region* r) {
+  const size_t M = (size_t)r->ninst * r->nv * r->N;
+  if (r->qy.alloc(M)) return MGD_ERR_HIP;
+  std::vector<int> rank;
+  auto up_rank = [&](DevBuf& b, int ip, int* count) -> int {
+    if (ip < 0) { *count = 0; return MGD_OK; }
+    const HostPatch& P = *r->patches[ip];
+    *count = fused2d_tile_ranks(r->geo, P.lo, P.hi, rank);
+    if (b.alloc(rank.size() / 2 + 1)) return MGD_ERR_HIP;
+    MGD_HIP(hipMemcpy(b.p, rank.data(), rank.size() * sizeof(int), hipMemcpyHostToDevice));
+    return MGD_OK;
+  };
+  if (int rc = up_rank(r->rank_cost, r->ip_cost, &r->ntiles_cost)) return rc;
+  if (int rc = up_rank(r->rank_act, r->ip_act, &r->ntiles_act)) return rc;
+  return MGD_OK;
+}
+
+int prepare_quadrature(mgd_region* r, int nst, double dt) {
+  std::vector<double> w((size_t)nst);
+  for (int s = 0; s < nst; ++s) w[s] = kRkNorm[s % 4] * dt;
+  if (upload(r->wts, w)) return MGD_ERR_HIP;
+  if (r->ssum.alloc((size_t)r->ninst * nst)) return MGD_ERR_HIP;
+  if (r->ntiles_cost > 0 && r->jpart.alloc((size_t)r->ninst * nst * r->ntiles_cost)) return MGD_ERR_HIP;
+  if (r->ntiles_act > 0 && r->spart.alloc((size_t)r->ninst * nst * r->ntiles_act)) return MGD_ERR_HIP;
+  return MGD_OK;
+}
+
+FwdStage fwd_args(mgd_region* r, const double* qin, const double* q0, double* b2, double* qout,
+                  int stage, double dt, int sidx, int nst, bool with_j) {
+  FwdStage a;
+  std::memset(&a, 0, sizeof a);
+  a.qin = qin; a.q0 = q0; a.b2 = b2; a.qout = qout; a.stage = stage; a.dt = dt;
+  const int na = n_act(r);
+  if (r->control.p && na > 0) {
+    a.control = r->control.p + (long)sidx * na;
+    a.control_stride = r->ctrl_stride;
+  }
+  const int ns = (int)r->sources.size();
+  a.src = reinterpret_cast<const Source*>(r->d_sources.p);
+  a.nsrc = ns;
+  a.src_amp = r->src_amp.p + (long)sidx * std::max(ns, 1);
+  a.ip_cost = r->ip_cost;
+  if (with_j && r->ntiles_cost > 0) {
+    a.jpartial = r->jpart.p;
+    a.tile_rank = reinterpret_cast<const int*>(r->rank_cost.p);
+    a.ntiles = r->ntiles_cost;
+    a.nstage_slots = nst;
+    a.stage_slot = sidx;
+  }
+  return a;
+}
+
+// one RK4 time step of the fused path: levels q0 -> q0, stage states in x / y
+void fused_forward_step(mgd_region* r, double* q0, double* x, double* y, double* b2, int s, double dt,
+                        int nst, bool with_j, double* const* outs) {
+  const double* qin[4] = {q0, x, y, x};
+  double* qo[4] = {x, y, x, q0};
+  for (int i = 1; i <= 4; ++i) {
+    double* o = outs ? outs[i - 1] : qo[i - 1];
+    const double* in = outs ? (i == 1 ? q0 : outs[i - 2]) : qin[i - 1];
+    FwdStage a = fwd_args(r, in, q0, b2, o, i, dt, 4 * s + i - 1, nst, with_j);
+    launch_fwd_stage2d(r->geo, dpatches(r), a, r->stream);
+  }
+}
+
+int run_adjoint_fused(mgd_region* r, double dt, int nsteps, int K, double* gradient,
+                      double* sensitivity, double* J) {
+  const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
+  const long M = (long)r->ninst * M_of(r);
+  if (int rc = prepare_quadrature(r, nst, dt)) return rc;
+  if (r->ly.alloc(M) || r->l0.alloc(M)) return MGD_ERR_HIP;
+  MGD_HIP(hipEventRecord(r->ev0, r->stream));
+  launch_copy(M, r->q.p, r->snaps.p, r->stream);
+  for (int s = 0; s < nsteps; ++s) {
+    fused_forward_step(r, r->q.p, r->b1.p, r->qy.p, r->b2.p, s, dt, nst, true, nullptr);
+    if ((s + 1) % K == 0) launch_copy(M, r->q.p, r->snaps.p + (long)((s + 1) / K) * M, r->stream);
+  }
+  if (r->ntiles_cost > 0)
+    launch_stage_quadrature(r->jpart.p, nst, r->ntiles_cost, r->wts.p, r->ssum.p, r->acc_J.p,
+                            r->ninst, 0, r->stream);
+  // terminal condition on the final state
+  double* L0 = r->lam.p;
+  double* LX = r->ly.p;
+  double* LY = r->l0.p;
+  launch_fill(M, 0.0, L0, r->stream);
+  if (nc > 0) {
+    launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->q.p, r->aforce.p, 0, r->stream);
+    launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -dt / 6.0, L0, r->stream);
+  }
+  int loaded = -1;
+  for (int t = nsteps - 1; t >= 0; --t) {
+    for (int i = 4; i >= 1; --i) {
+      const int ts = i == 1 ? t : t + 1;
+      const int st = i == 1 ? 4 : i - 1;
+      const bool need = loaded == -1 || ts < loaded || ts > loaded + K ||
+                        (ts == loaded && st < 4) || (ts == loaded + K && st == 4);
+      if (need) {
+        int lt;
+        if (ts % K == 0 && st == 4) lt = ts;
+        else if (ts % K == 0) lt = ts - K;
+        else lt = ts - ts % K;
+        loaded = lt;
+        launch_copy(M, r->snaps.p + (long)(lt / K) * M, r->window.p, r->stream);
+        if (loaded != nsteps) {
+          for (int s2 = loaded + 1; s2 <= loaded + K && s2 <= nsteps; ++s2) {
+            const long base = 4L * (s2 - loaded - 1);
+            double* outs[4];
+            for (int k = 0; k < 4; ++k) outs[k] = r->window.p + (base + 1 + k) * M;
+            if (s2 == loaded + K) outs[3] = r->qs.p;   // next snapshot level: not kept
+            fused_forward_step(r, r->window.p + base * M, nullptr, nullptr, r->b2.p, s2 - 1, dt, nst,
+                               false, outs);
+          }
+        }
+      }
+      const int widx = (ts - 1 - loaded) * 4 + st;
+      const double* qcur = r->window.p + (long)widx * M;
+      const int sidx = 4 * t + i - 1;
+      AdjStage a;
+      std::memset(&a, 0, sizeof a);
+      a.lam = i == 4 ? L0 : (i == 3 ? LX : (i == 2 ? LY : LX));
+      a.lout = i == 4 ? LX : (i == 3 ? LY : (i == 2 ? LX : L0));
+      a.l0 = L0;
+      a.q = qcur;
+      a.b2 = r->b2.p;
+      a.stage = i;
+      a.dt = dt;
+      a.forcing_factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
+      a.zero_forcing = t == 0 && i == 1;
+      a.ip_act = r->ip_act;
+      if (na > 0) {
+        if (gradient) {
+          a.grad = r->grad.p + (long)sidx * na;
+          a.grad_stride = (long)nst * na;
+        }
+        if (r->ntiles_act > 0) {
+          a.spartial = r->spart.p;
+          a.tile_rank = reinterpret_cast<const int*>(r->rank_act.p);
+          a.ntiles = r->ntiles_act;
+          a.nstage_slots = nst;
+          a.stage_slot = sidx;
+        }
+      }
+      launch_adj_stage2d(r->geo, dpatches(r), a, r->stream);
+    }
+  }
+  if (na > 0 && r->ntiles_act > 0)
+    launch_stage_quadrature(r->spart.p, nst, r->ntiles_act, r->wts.p, r->ssum.p, r->acc_S.p, r->ninst,
+                            1, r->stream);
+  MGD_HIP(hipEventRecord(r->ev1, r->stream));
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  float ms = 0.f;
+  MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+  r->last_ms[0] = ms;
+  if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
+  if (sensitivity) MGD_HIP(hipMemcpy(sensitivity, r->acc_S.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
+  if (gradient && na > 0)
+    MGD_HIP(hipMemcpy(gradient, r->grad.p, sizeof(double) * r->ninst * nst * na, hipMemcpyDeviceToHost));
+  return MGD_OK;
 }
 
 }  // namespace
@@ -697,8 +868,19 @@ int mgd_region_finalize(mgd_region* r) {
   const int npr = std::max(reduce_blocks(std::max(n_cost(r), 1)), reduce_blocks(std::max(n_act(r), 1)));
   if (r->partial.alloc((size_t)npr * r->ninst)) return MGD_ERR_HIP;
   if (n_cost(r) > 0 && r->aforce.alloc((size_t)r->ninst * nv * n_cost(r))) return MGD_ERR_HIP;
+  if (r->nd == 2 && !r->desc.viscous)
+    if (int rc = setup_fused(r)) return rc;
   r->finalized = true;
   return MGD_OK;
+}
+
+int mgd_region_set_option(mgd_region* r, int option, int value) {
+  if (!r) { set_error("null region"); return MGD_ERR_ARG; }
+  switch (option) {
+    case MGD_OPT_FORCE_PLAIN: r->force_plain = value != 0; return MGD_OK;
+  }
+  set_error("unknown option");
+  return MGD_ERR_ARG;
 }
 
 int mgd_state_put(mgd_region* r, int which, int inst, const double* host) {
@@ -729,6 +911,31 @@ int mgd_compute_rhs(mgd_region* r, int mode, double time, const double* control,
   std::vector<double> amp;
   source_table(r, std::vector<double>{time}, amp);
   if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
+  if (fused_ok(r) && (mode == MGD_FORWARD || mode == MGD_ADJOINT)) {
+    if (mode == MGD_FORWARD) {
+      const int na = n_act(r);
+      if (control && na > 0) {
+        std::vector<double> c(control, control + (size_t)r->ninst * na);
+        if (upload(r->control, c)) return MGD_ERR_HIP;
+        r->ctrl_stride = na;
+      } else if (r->control.p) {
+        r->control.alloc(0);
+      }
+      FwdStage a = fwd_args(r, r->q.p, r->q.p, r->b2.p, r->b1.p, 1, 0.0, 0, 1, false);
+      a.rhs_out = r->rhs.p;
+      launch_fwd_stage2d(r->geo, dpatches(r), a, r->stream);
+    } else {
+      AdjStage a;
+      std::memset(&a, 0, sizeof a);
+      a.lam = r->lam.p; a.l0 = r->lam.p; a.q = r->q.p; a.b2 = r->b2.p; a.lout = r->b1.p;
+      a.rhs_out = r->rhs.p; a.stage = 4; a.dt = 0.0; a.forcing_factor = factor;
+      a.zero_forcing = r->aforce_zero; a.ip_act = r->ip_act;
+      launch_adj_stage2d(r->geo, dpatches(r), a, r->stream);
+    }
+    MGD_HIP(hipGetLastError());
+    MGD_HIP(hipStreamSynchronize(r->stream));
+    return MGD_OK;
+  }
   if (mode == MGD_FORWARD) {
     const int na = n_act(r);
     const double* ctrl = nullptr;
@@ -756,6 +963,7 @@ int mgd_compute_rhs(mgd_region* r, int mode, double time, const double* control,
 
 int mgd_update_adjoint_forcing(mgd_region* r, int zero) {
   if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
+  r->aforce_zero = zero != 0;
   if (r->ip_cost < 0) return MGD_OK;
   launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), n_cost(r), r->q.p, r->aforce.p, zero, r->stream);
   MGD_HIP(hipGetLastError());
@@ -795,6 +1003,23 @@ int mgd_run_forward(mgd_region* r, double t0, double dt, int nsteps, const doubl
   if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
   launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
   const long M = (long)r->ninst * M_of(r);
+  if (fused_ok(r)) {
+    if (int rc = prepare_quadrature(r, nst, dt)) return rc;
+    MGD_HIP(hipEventRecord(r->ev0, r->stream));
+    for (int s = 0; s < nsteps; ++s)
+      fused_forward_step(r, r->q.p, r->b1.p, r->qy.p, r->b2.p, s, dt, nst, true, nullptr);
+    if (r->ntiles_cost > 0)
+      launch_stage_quadrature(r->jpart.p, nst, r->ntiles_cost, r->wts.p, r->ssum.p, r->acc_J.p,
+                              r->ninst, 0, r->stream);
+    MGD_HIP(hipEventRecord(r->ev1, r->stream));
+    MGD_HIP(hipGetLastError());
+    MGD_HIP(hipStreamSynchronize(r->stream));
+    float ms = 0.f;
+    MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+    r->last_ms[0] = ms;
+    if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
+    return MGD_OK;
+  }
   MGD_HIP(hipEventRecord(r->ev0, r->stream));
   for (int s = 0; s < nsteps; ++s)
     for (int i = 1; i <= 4; ++i) {
@@ -842,6 +1067,7 @@ int mgd_run_adjoint(mgd_region* r, double t0, double dt, int nsteps, int save_in
     return MGD_ERR_HIP;
   launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
   launch_fill(r->ninst, 0.0, r->acc_S.p, r->stream);
+  if (fused_ok(r)) return run_adjoint_fused(r, dt, nsteps, K, gradient, sensitivity, J);
   MGD_HIP(hipEventRecord(r->ev0, r->stream));
   // forward run with snapshots at t0 and every K steps (showProgress .q files)
   launch_copy(M, r->q.p, r->snaps.p, r->stream);

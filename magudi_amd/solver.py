@@ -68,7 +68,7 @@ class Region:
     Region%setup, Grid%update and the patch / functional / controller set-up of
     Solver%setup (sponge strengths, far-field penalties, mean pressure)."""
 
-    def __init__(self, case: Case, n_instances: int = 1):
+    def __init__(self, case: Case, n_instances: int = 1, force_plain: bool = False):
         self.case = case
         self.nd = case.nd
         self.nv = case.nv
@@ -95,6 +95,8 @@ class Region:
         h = C.c_void_p()
         check(capi.lib().mgd_region_create(C.byref(d), dptr(coords), C.byref(h)))
         self.handle = h
+        if force_plain:
+            check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_FORCE_PLAIN, 1))
         for p in case.patches:
             pd = capi.PatchDesc()
             pd.type = p.type

@@ -87,11 +87,12 @@ def test_stencil_apply(solver_mod, op, direction):
     assert maxrel(y, ref) < 1e-14
 
 
+@pytest.mark.parametrize("plain", [False, True])
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("mode", [1, -1])
-def test_rhs_parity(solver_mod, name, mode):
+def test_rhs_parity(solver_mod, name, mode, plain):
     c = CASES[name]()
-    R = solver_mod.Region(c)
+    R = solver_mod.Region(c, force_plain=plain)
     O = ob.Region(c)
     q = perturbed(c, 11)
     lam = np.random.default_rng(12).standard_normal((c.nv, c.npoints))
@@ -123,10 +124,11 @@ def test_functional_parity(solver_mod):
     assert abs(J - O.functional(q)) / abs(O.functional(q)) < J_TOL
 
 
-def test_run_forward_parity(solver_mod):
+@pytest.mark.parametrize("plain", [False, True])
+def test_run_forward_parity(solver_mod, plain):
     c = cases.acoustic_monopole(n=41, nsteps=20)
     c.dt = 0.2
-    R = solver_mod.Region(c)
+    R = solver_mod.Region(c, force_plain=plain)
     O = ob.Region(c)
     q0 = c.initial_state()
     R.states[0].conservedVariables = q0
@@ -137,10 +139,11 @@ def test_run_forward_parity(solver_mod):
     assert maxrel(R.states[0].conservedVariables - q0, qo - q0) < 1e-12
 
 
-def test_run_adjoint_parity_and_fd(solver_mod):
+@pytest.mark.parametrize("plain", [False, True])
+def test_run_adjoint_parity_and_fd(solver_mod, plain):
     c = cases.acoustic_monopole(n=41, nsteps=24, save_interval=5)
     c.dt = 0.2
-    R = solver_mod.Region(c)
+    R = solver_mod.Region(c, force_plain=plain)
     O = ob.Region(c)
     q0 = c.initial_state()
     R.states[0].conservedVariables = q0
@@ -158,16 +161,17 @@ def test_run_adjoint_parity_and_fd(solver_mod):
     assert abs((J1 - J[0]) / A - sens[0]) / sens[0] < 1e-3
 
 
-def test_batched_instances_match_single(solver_mod):
+@pytest.mark.parametrize("plain", [False, True])
+def test_batched_instances_match_single(solver_mod, plain):
     c = cases.acoustic_monopole(n=41, nsteps=12, save_interval=4)
     c.dt = 0.2
     q0 = c.initial_state()
-    R1 = solver_mod.Region(c)
+    R1 = solver_mod.Region(c, force_plain=plain)
     na = R1.n_actuator
     rng = np.random.default_rng(5)
     amps = [0.0, 1e-2, 3e-2]
     ctl = np.stack([a * rng.standard_normal((4 * c.nsteps, na)) for a in amps])
-    Rb = solver_mod.Region(c, n_instances=3)
+    Rb = solver_mod.Region(c, n_instances=3, force_plain=plain)
     for i in range(3):
         Rb.states[i].conservedVariables = q0
     Sb = solver_mod.Solver(Rb, dt=c.dt, nsteps=c.nsteps, save_interval=c.save_interval)
@@ -193,3 +197,23 @@ def test_viscous_3d_adjoint_run(solver_mod):
     O = ob.Region(c)
     _, qo = O.run_forward(q0, c.dt, 4)
     assert maxrel(R.states[0].conservedVariables - q0, qo - q0) < 1e-11
+
+
+def test_fused_matches_plain_on_a_larger_grid(solver_mod):
+    """Fused 2-D stage kernels against the generic kernels on a 97 x 97 grid with a
+    non-multiple-of-tile size (partial tiles on both axes)."""
+    c = cases.acoustic_monopole(n=97, nsteps=6, save_interval=3)
+    c.dt = 0.1
+    q0 = c.initial_state() + 1e-3 * perturbed(c, 3, amp=1.0) * 0.0
+    out = []
+    for plain in (False, True):
+        R = solver_mod.Region(c, force_plain=plain)
+        R.states[0].conservedVariables = perturbed(c, 31, amp=0.01)
+        S = solver_mod.Solver(R, dt=c.dt, nsteps=c.nsteps, save_interval=c.save_interval)
+        sens, grad, J = S.runAdjoint()
+        out.append((sens[0], grad[0], J[0], R.states[0].adjointVariables))
+    (s1, g1, J1, l1), (s2, g2, J2, l2) = out
+    assert abs(J1 - J2) / abs(J2) < J_TOL
+    assert abs(s1 - s2) / abs(s2) < GRAD_TOL
+    assert maxrel(g1, g2) < GRAD_TOL
+    assert maxrel(l1, l2) < GRAD_TOL
