@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark of the magudi RHS hot path on one MI355X.
+
+Headline workload (one "step" = one full pass of the hot path): the 2-D acoustic
+monopole on a 401 x 401 grid, inviscid Euler, SBP 4-8 with dissipation, far-field
+SAT + sponges, forward run of 960 RK4 steps followed by the discrete adjoint over
+the same 960 steps with HBM snapshots every 32 steps and window recompute
+(Solver%runAdjoint with a uniform checkpointer).  Synthetic input: the quiescent
+state with the Gaussian monopole source and a control mollifier, all built in
+magudi_amd/cases.py.
+
+value = grid-point-steps per second of the forward+adjoint pass (N * 960 / time).
+Also reported: the 20-instance batched rate (finite-difference sweep shape), the
+generic per-operator ("plain") path, the HBM roofline fraction of the dominant
+kernel timed with HIP events on its own stream, and the CPU oracle on a bounded
+sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "FP64 grid-pt-steps/s fwd+adjoint, batched vs plain; % of HBM roofline"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=401)
+    ap.add_argument("--nsteps", type=int, default=960)
+    ap.add_argument("--save-interval", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=20)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline only (no batched / plain / CPU legs)")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    return ap.parse_args()
+
+
+def run_hot_path(R, S, steps):
+    """steps x (restart from the resident initial condition + forward + adjoint)."""
+    R.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        R.restart()
+        out = S.runAdjoint(want_gradient=False)
+    R.sync()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def cpu_baseline(n, nsteps, save_interval):
+    """CPU oracle (plain C, one thread) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    from magudi_amd import cases
+    c = cases.acoustic_monopole(n=n, nsteps=nsteps, save_interval=save_interval)
+    O = ob.Region(c)
+    q0 = c.initial_state()
+    t0 = time.perf_counter()
+    O.run_adjoint(q0, c.dt, nsteps, save_interval)
+    dt = time.perf_counter() - t0
+    return {"value": c.npoints * nsteps / dt, "unit": "grid-pt-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{n}x{n} monopole forward+adjoint, {nsteps} RK4 steps, snapshots every "
+                      f"{save_interval} (oracle/ C restatement, 1 thread, {dt:.1f} s)"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
+              "(instances shard across GPUs without communication; see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+    from magudi_amd import capi, cases
+    from magudi_amd.solver import Region, Solver
+
+    case = cases.acoustic_monopole(n=a.n, nsteps=a.nsteps, save_interval=a.save_interval)
+    q0 = case.initial_state()
+    R = Region(case)
+    R.states[0].initialCondition = q0
+    S = Solver(R)
+    for _ in range(a.warmup):
+        run_hot_path(R, S, 1)
+    t_step, (sens, _, J) = run_hot_path(R, S, a.steps)
+    pts = case.npoints * a.nsteps
+    value = pts / t_step
+
+    # dominant kernel: HIP-event timing of the fused stage kernels on the region stream
+    R.restart()
+    fwd_ms, fwd_b = R.time_kernel(capi.KERNEL_FWD_STAGE, reps=50)
+    adj_ms, adj_b = R.time_kernel(capi.KERNEL_ADJ_STAGE, reps=50)
+    cpy_ms, cpy_b = R.time_kernel(capi.KERNEL_COPY, reps=50)
+    launches_fwd = 8 * a.nsteps          # forward pass + window recompute
+    launches_adj = 4 * a.nsteps
+    kernels = {
+        "k_fwd_stage2d": {"ms": fwd_ms, "bytes": fwd_b, "launches": launches_fwd},
+        "k_adj_stage2d": {"ms": adj_ms, "bytes": adj_b, "launches": launches_adj},
+    }
+    dom = max(kernels, key=lambda k: kernels[k]["ms"] * kernels[k]["launches"])
+    d = kernels[dom]
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "avg_launch_us": round(d["ms"] * 1e3, 3),
+                "bytes_per_launch": d["bytes"]}
+    other = {k: {"avg_launch_us": round(v["ms"] * 1e3, 3),
+                 "achieved_GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)}
+             for k, v in kernels.items()}
+    other["state_copy"] = {"avg_launch_us": round(cpy_ms * 1e3, 3),
+                           "achieved_GBs": round(cpy_b / (cpy_ms * 1e-3) / 1e9, 1)}
+    kernel_share = sum(v["ms"] * v["launches"] for v in kernels.values()) / (t_step * 1e3)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "grid-pt-steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": t_step * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (acoustic monopole set-up generated in magudi_amd/cases.py)",
+        "config": {"workload": f"monopole {a.n}x{a.n} forward+adjoint, {a.nsteps} RK4 steps, "
+                               f"HBM snapshots every {a.save_interval} steps with recompute",
+                   "grid": [a.n, a.n], "nsteps": a.nsteps, "save_interval": a.save_interval,
+                   "instances": 1, "path": "fused 2-D stage kernels"},
+        "roofline": roofline,
+        "kernels": other,
+        "kernel_time_share": round(kernel_share, 3),
+        "J": float(J[0]), "Grad0": float(sens[0]),
+    }
+    if not a.no_extras:
+        # batched: a.batch instances of the same grid in every launch
+        Rb = Region(case, n_instances=a.batch)
+        for i in range(a.batch):
+            Rb.states[i].initialCondition = q0
+        Sb = Solver(Rb)
+        run_hot_path(Rb, Sb, 1)
+        tb, _ = run_hot_path(Rb, Sb, 1)
+        # plain per-instance path (generic per-operator kernels), fewer steps, same per-step work
+        ns_plain = max(a.save_interval, a.nsteps // 10)
+        Rp = Region(case, force_plain=True)
+        Rp.states[0].initialCondition = q0
+        Sp = Solver(Rp, nsteps=ns_plain)
+        run_hot_path(Rp, Sp, 1)
+        tp, _ = run_hot_path(Rp, Sp, 1)
+        out["batched"] = {"instances": a.batch, "value": a.batch * pts / tb,
+                          "ms_per_step": tb * 1e3,
+                          "speedup_vs_single_fused": (a.batch * pts / tb) / value}
+        out["plain"] = {"value": case.npoints * ns_plain / tp, "nsteps": ns_plain,
+                        "ms_per_step": tp * 1e3,
+                        "batched_speedup_vs_plain": (a.batch * pts / tb) / (case.npoints * ns_plain / tp)}
+        out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_steps, max(1, a.cpu_steps // 2))
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

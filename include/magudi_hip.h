@@ -46,6 +46,7 @@ extern "C" {
 #define MGD_ADJOINT_VARS 1
 #define MGD_RHS 2
 #define MGD_TARGET 3
+#define MGD_INITIAL_STATE 4     /* device-resident copy of an initial condition */
 
 /* Fields for mgd_region_set_field */
 #define MGD_FIELD_TARGET_STATE 0
@@ -124,6 +125,10 @@ int mgd_region_finalize(mgd_region* r);
 int mgd_state_put(mgd_region* r, int which, int instance, const double* host);
 int mgd_state_get(mgd_region* r, int which, int instance, double* host);
 
+/* Device-to-device copy of a state array of all instances (e.g. MGD_INITIAL_STATE to
+ * MGD_CONSERVED to restart from a resident initial condition).  Asynchronous. */
+int mgd_state_copy(mgd_region* r, int src, int dst);
+
 /* Region%computeRhs(mode) (src/RegionImpl.f90:1877-2027) for all instances into the
  * region's rhs buffer.  time: State%time for acoustic sources.  control: actuator
  * forcing per instance per actuator patch point (host, may be NULL).  adjoint
@@ -175,6 +180,16 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches);
  * where the fused 2-D stage kernels apply (comparison and testing). */
 #define MGD_OPT_FORCE_PLAIN 1
 int mgd_region_set_option(mgd_region* r, int option, int value);
+
+/* Times one kernel on the region stream with HIP events: `reps` back-to-back launches
+ * after one warm-up.  which: MGD_KERNEL_FWD_STAGE (fused 2-D forward RK stage 2),
+ * MGD_KERNEL_ADJ_STAGE (fused 2-D adjoint RK stage 3), MGD_KERNEL_COPY (state-sized
+ * copy, the bandwidth reference).  bytes: algorithmic HBM bytes of one launch.
+ * Benchmark support; it overwrites the region's RK and adjoint buffers. */
+#define MGD_KERNEL_FWD_STAGE 0
+#define MGD_KERNEL_ADJ_STAGE 1
+#define MGD_KERNEL_COPY 2
+int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, double* bytes);
 
 /* Waits for all work of the region. */
 int mgd_sync(mgd_region* r);

@@ -16,10 +16,11 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "magudi_hip.h")
 
 MGD_OK = 0
 FORWARD, ADJOINT = 1, -1
-CONSERVED, ADJOINT_VARS, RHS = 0, 1, 2
+CONSERVED, ADJOINT_VARS, RHS, INITIAL_STATE = 0, 1, 2, 4
 FIELD_TARGET_STATE, FIELD_CONTROL_MOLLIFIER, FIELD_TARGET_MOLLIFIER = 0, 1, 2
 OP_FIRST, OP_ADJOINT_FIRST, OP_DISS, OP_DISS_T = 0, 1, 2, 3
 OPT_FORCE_PLAIN = 1
+KERNEL_FWD_STAGE, KERNEL_ADJ_STAGE, KERNEL_COPY = 0, 1, 2
 
 
 class RegionDesc(C.Structure):
@@ -79,6 +80,8 @@ def lib():
         "mgd_last_timings": [vp, dp, C.POINTER(C.c_int)],
         "mgd_sync": [vp],
         "mgd_region_set_option": [vp, C.c_int, C.c_int],
+        "mgd_time_kernel": [vp, C.c_int, C.c_int, dp, dp],
+        "mgd_state_copy": [vp, C.c_int, C.c_int],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -77,7 +77,7 @@ struct mgd_region {
   int ip_cost = -1, ip_act = -1, has_farfield = 0;
   // HBM-resident state
   mgd::DevBuf q, lam, rhs, b1, b2, b1a, b2a, t, w, G, T1, S;
-  mgd::DevBuf qs, aforce, control, grad, acc_J, acc_S, partial, src_amp;
+  mgd::DevBuf qs, aforce, control, grad, acc_J, acc_S, partial, src_amp, init;
   mgd::DevBuf snaps, window;
   // fused 2-D path: ping-pong stage buffers, per-tile partials, quadrature weights
   mgd::DevBuf qy, ly, l0, jpart, spart, ssum, wts, rank_cost, rank_act;
@@ -885,8 +885,9 @@ int mgd_region_set_option(mgd_region* r, int option, int value) {
 
 int mgd_state_put(mgd_region* r, int which, int inst, const double* host) {
   if (!r || !host || inst < 0 || inst >= r->ninst) { set_error("bad argument"); return MGD_ERR_ARG; }
+  if (which == MGD_INITIAL_STATE && r->init.alloc((size_t)r->ninst * M_of(r))) return MGD_ERR_HIP;
   DevBuf* b = which == MGD_CONSERVED ? &r->q : which == MGD_ADJOINT_VARS ? &r->lam
-            : which == MGD_RHS ? &r->rhs : nullptr;
+            : which == MGD_RHS ? &r->rhs : which == MGD_INITIAL_STATE ? &r->init : nullptr;
   if (!b) { set_error("unknown state array"); return MGD_ERR_ARG; }
   const size_t M = (size_t)M_of(r);
   MGD_HIP(hipMemcpyAsync(b->p + inst * M, host, M * sizeof(double), hipMemcpyHostToDevice, r->stream));
@@ -902,6 +903,20 @@ int mgd_state_get(mgd_region* r, int which, int inst, double* host) {
   const size_t M = (size_t)M_of(r);
   MGD_HIP(hipMemcpyAsync(host, b->p + inst * M, M * sizeof(double), hipMemcpyDeviceToHost, r->stream));
   MGD_HIP(hipStreamSynchronize(r->stream));
+  return MGD_OK;
+}
+
+int mgd_state_copy(mgd_region* r, int src, int dst) {
+  if (!r) { set_error("null region"); return MGD_ERR_ARG; }
+  auto pick = [&](int w) -> DevBuf* {
+    return w == MGD_CONSERVED ? &r->q : w == MGD_ADJOINT_VARS ? &r->lam : w == MGD_RHS ? &r->rhs
+         : w == MGD_INITIAL_STATE ? &r->init : nullptr;
+  };
+  DevBuf* a = pick(src);
+  DevBuf* b = pick(dst);
+  if (!a || !b || !a->p || !b->p) { set_error("unknown or empty state array"); return MGD_ERR_ARG; }
+  launch_copy((long)r->ninst * M_of(r), a->p, b->p, r->stream);
+  MGD_HIP(hipGetLastError());
   return MGD_OK;
 }
 
@@ -1198,6 +1213,68 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches) {
   for (int i = 0; i < 3; ++i) {
     if (out) out[i] = r->last_ms[i];
     if (launches) launches[i] = r->last_launch[i];
+  }
+  return MGD_OK;
+}
+
+int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, double* bytes) {
+  if (!r || !r->finalized || reps <= 0) { set_error("bad argument"); return MGD_ERR_ARG; }
+  const long Mi = M_of(r), M = (long)r->ninst * Mi;
+  const double pts = (double)r->ninst * r->N;
+  const double w = 8.0 * r->nv;   // bytes of one state point
+  if (r->qy.alloc(M) || r->ly.alloc(M) || r->l0.alloc(M) || r->qs.alloc(M)) return MGD_ERR_HIP;
+  std::vector<double> amp;
+  source_table(r, std::vector<double>{0.0}, amp);
+  if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
+  if (r->control.p) r->control.alloc(0);
+  auto launch = [&]() {
+    switch (which) {
+      case MGD_KERNEL_FWD_STAGE: {
+        // stage 2: reads the stage state (with halo), buffer1 and buffer2; writes buffer2 and
+        // the next stage state
+        FwdStage a = fwd_args(r, r->qy.p, r->q.p, r->b2.p, r->qs.p, 2, 1e-3, 0, 1, false);
+        launch_fwd_stage2d(r->geo, dpatches(r), a, r->stream);
+        break;
+      }
+      case MGD_KERNEL_ADJ_STAGE: {
+        AdjStage a;
+        std::memset(&a, 0, sizeof a);
+        a.lam = r->ly.p; a.l0 = r->lam.p; a.q = r->q.p; a.b2 = r->b2.p; a.lout = r->l0.p;
+        a.stage = 3; a.dt = 1e-3; a.forcing_factor = 1.0; a.ip_act = r->ip_act;
+        launch_adj_stage2d(r->geo, dpatches(r), a, r->stream);
+        break;
+      }
+      case MGD_KERNEL_COPY:
+        launch_copy(M, r->q.p, r->qs.p, r->stream);
+        break;
+    }
+  };
+  if (which == MGD_KERNEL_FWD_STAGE || which == MGD_KERNEL_ADJ_STAGE) {
+    if (!fused_ok(r)) { set_error("no fused stage kernel for this region"); return MGD_ERR_UNSUPPORTED; }
+    // well-defined operands: every buffer holds the current state
+    launch_copy(M, r->q.p, r->qy.p, r->stream);
+    launch_copy(M, r->q.p, r->b2.p, r->stream);
+    launch_copy(M, r->q.p, r->ly.p, r->stream);
+    launch_copy(M, r->q.p, r->lam.p, r->stream);
+  } else if (which != MGD_KERNEL_COPY) {
+    set_error("unknown kernel");
+    return MGD_ERR_ARG;
+  }
+  launch();   // warm-up
+  MGD_HIP(hipEventRecord(r->ev0, r->stream));
+  for (int i = 0; i < reps; ++i) launch();
+  MGD_HIP(hipEventRecord(r->ev1, r->stream));
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipEventSynchronize(r->ev1));
+  float ms = 0.f;
+  MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+  if (ms_per_launch) *ms_per_launch = ms / reps;
+  if (bytes) {
+    switch (which) {
+      case MGD_KERNEL_FWD_STAGE: *bytes = 5.0 * w * pts; break;   // qin, q0, b2 in; b2, qout out
+      case MGD_KERNEL_ADJ_STAGE: *bytes = 6.0 * w * pts; break;   // lam, q, l0, b2 in; b2, lout out
+      default: *bytes = 2.0 * w * pts; break;
+    }
   }
   return MGD_OK;
 }

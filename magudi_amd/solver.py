@@ -62,6 +62,14 @@ class State:
     def rightHandSide(self):
         return self._get(capi.RHS)
 
+    @property
+    def initialCondition(self):
+        return self._get(capi.INITIAL_STATE)
+
+    @initialCondition.setter
+    def initialCondition(self, v):
+        self._put(capi.INITIAL_STATE, v)
+
 
 class Region:
     """t_Region over the HIP path.  ``Region(case, n_instances)`` performs
@@ -138,6 +146,13 @@ class Region:
         ctl = None if control is None else np.ascontiguousarray(control, dtype=np.float64)
         check(capi.lib().mgd_compute_rhs(self.handle, mode, time, dptr(ctl), forcing_factor))
 
+    def restart(self):
+        """Conserved state of every instance := its resident initial condition (device copy)."""
+        check(capi.lib().mgd_state_copy(self.handle, capi.INITIAL_STATE, capi.CONSERVED))
+
+    def sync(self):
+        check(capi.lib().mgd_sync(self.handle))
+
     def updateAdjointForcing(self, no_adjoint_forcing: bool = False):
         check(capi.lib().mgd_update_adjoint_forcing(self.handle, int(no_adjoint_forcing)))
 
@@ -160,6 +175,13 @@ class Region:
         ncomp = a.size // self.npoints
         check(capi.lib().mgd_stencil_apply(self.handle, op, direction, dptr(a), ncomp))
         return a
+
+    def time_kernel(self, which: int, reps: int = 20):
+        """(ms per launch, algorithmic bytes per launch) of one kernel, HIP-event timed."""
+        ms = C.c_double(0.0)
+        b = C.c_double(0.0)
+        check(capi.lib().mgd_time_kernel(self.handle, which, reps, C.byref(ms), C.byref(b)))
+        return ms.value, b.value
 
     def timings(self):
         out = np.zeros(3)
