@@ -6,6 +6,21 @@
 
 namespace mgd {
 
+// Tile partition of a 2-D grid (device view): tiles of at most 32 x 16 points with
+// origins org[0..nbx) along x and org[nbx..nbx+nby) along y; pmask: patches per tile.
+struct Tiling {
+  int nbx, nby;
+  const int* org;
+  const int* size;
+  const unsigned* pmask;
+};
+
+struct HostTiling {
+  int nbx = 0, nby = 0;
+  std::vector<int> org, size;
+  std::vector<unsigned> pmask;
+};
+
 // One forward RK4 stage: rhs at `qin`, buffer update of `b2`, new stage state to `qout`.
 // q0 is the state at the start of the time step (RK buffer1).
 struct FwdStage {
@@ -46,11 +61,14 @@ struct AdjStage {
   int ntiles, nstage_slots, stage_slot, ip_act;
 };
 
-dim3 fused2d_grid(const Geo& g);
-int fused2d_tiles(const Geo& g);
-int fused2d_tile_ranks(const Geo& g, const int* lo, const int* hi, std::vector<int>& rank);
-void launch_fwd_stage2d(const Geo& g, const Patches* P, const FwdStage& a, hipStream_t st);
-void launch_adj_stage2d(const Geo& g, const Patches* P, const AdjStage& a, hipStream_t st);
+// Balanced partition; MGD_ERR_UNSUPPORTED when an edge tile of a non-periodic direction
+// would hold fewer points than the deepest boundary closure.
+int fused2d_make_tiling(const Geo& g, const Patches& hp, HostTiling* out);
+int fused2d_tile_ranks(const HostTiling& T, const int* lo, const int* hi, std::vector<int>& rank);
+void launch_fwd_stage2d(const Geo& g, const Patches* P, const Tiling& T, const FwdStage& a,
+                        hipStream_t st);
+void launch_adj_stage2d(const Geo& g, const Patches* P, const Tiling& T, const AdjStage& a,
+                        hipStream_t st);
 void launch_stage_quadrature(const double* partial, int nstages, int ntiles, const double* w,
                              double* stage_sums, double* acc, int ninst, int reverse,
                              hipStream_t st);

@@ -82,6 +82,10 @@ struct mgd_region {
   // fused 2-D path: ping-pong stage buffers, per-tile partials, quadrature weights
   mgd::DevBuf qy, ly, l0, jpart, spart, ssum, wts, rank_cost, rank_act;
   int ntiles_cost = 0, ntiles_act = 0;
+  mgd::HostTiling htile;
+  mgd::Tiling tiling{};
+  mgd::DevBuf d_tile_org, d_tile_size, d_tile_mask;
+  bool tiling_ok = false;
   int force_plain = 0;
   int aforce_zero = 0;
   long ctrl_stride = 0;                  // control entries per instance
@@ -389,23 +393,42 @@ void forward_stage(mgd_region* r, double* q, double* b1, double* b2, int sidx, i
 }
 
 bool fused_ok(const mgd_region* r) {
-  return !r->force_plain && r->nd == 2 && !r->desc.viscous;
+  return !r->force_plain && r->nd == 2 && !r->desc.viscous && r->tiling_ok;
+}
+
+int upload_ints(DevBuf& b, const void* data, size_t bytes) {
+  if (b.alloc(bytes / sizeof(double) + 1)) return MGD_ERR_HIP;
+  MGD_HIP(hipMemcpy(b.p, data, bytes, hipMemcpyHostToDevice));
+  return MGD_OK;
 }
 
 int setup_fused(mgd_region* r) {
   const size_t M = (size_t)r->ninst * r->nv * r->N;
+  r->tiling_ok = false;
+  if (fused2d_make_tiling(r->geo, r->hpatches, &r->htile) != MGD_OK) return MGD_OK;  // plain path
+  const HostTiling& H = r->htile;
+  if (upload_ints(r->d_tile_org, H.org.data(), H.org.size() * sizeof(int)) ||
+      upload_ints(r->d_tile_size, H.size.data(), H.size.size() * sizeof(int)) ||
+      upload_ints(r->d_tile_mask, H.pmask.data(), H.pmask.size() * sizeof(unsigned)))
+    return MGD_ERR_HIP;
+  r->tiling.nbx = H.nbx;
+  r->tiling.nby = H.nby;
+  r->tiling.org = reinterpret_cast<const int*>(r->d_tile_org.p);
+  r->tiling.size = reinterpret_cast<const int*>(r->d_tile_size.p);
+  r->tiling.pmask = reinterpret_cast<const unsigned*>(r->d_tile_mask.p);
   if (r->qy.alloc(M)) return MGD_ERR_HIP;
   std::vector<int> rank;
   auto up_rank = [&](DevBuf& b, int ip, int* count) -> int {
     if (ip < 0) { *count = 0; return MGD_OK; }
     const HostPatch& P = *r->patches[ip];
-    *count = fused2d_tile_ranks(r->geo, P.lo, P.hi, rank);
+    *count = fused2d_tile_ranks(H, P.lo, P.hi, rank);
     if (b.alloc(rank.size() / 2 + 1)) return MGD_ERR_HIP;
     MGD_HIP(hipMemcpy(b.p, rank.data(), rank.size() * sizeof(int), hipMemcpyHostToDevice));
     return MGD_OK;
   };
   if (int rc = up_rank(r->rank_cost, r->ip_cost, &r->ntiles_cost)) return rc;
   if (int rc = up_rank(r->rank_act, r->ip_act, &r->ntiles_act)) return rc;
+  r->tiling_ok = true;
   return MGD_OK;
 }
 
@@ -453,7 +476,7 @@ void fused_forward_step(mgd_region* r, double* q0, double* x, double* y, double*
     double* o = outs ? outs[i - 1] : qo[i - 1];
     const double* in = outs ? (i == 1 ? q0 : outs[i - 2]) : qin[i - 1];
     FwdStage a = fwd_args(r, in, q0, b2, o, i, dt, 4 * s + i - 1, nst, with_j);
-    launch_fwd_stage2d(r->geo, dpatches(r), a, r->stream);
+    launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
   }
 }
 
@@ -534,7 +557,7 @@ int run_adjoint_fused(mgd_region* r, double dt, int nsteps, int K, double* gradi
           a.stage_slot = sidx;
         }
       }
-      launch_adj_stage2d(r->geo, dpatches(r), a, r->stream);
+      launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
     }
   }
   if (na > 0 && r->ntiles_act > 0)
@@ -938,14 +961,14 @@ int mgd_compute_rhs(mgd_region* r, int mode, double time, const double* control,
       }
       FwdStage a = fwd_args(r, r->q.p, r->q.p, r->b2.p, r->b1.p, 1, 0.0, 0, 1, false);
       a.rhs_out = r->rhs.p;
-      launch_fwd_stage2d(r->geo, dpatches(r), a, r->stream);
+      launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
     } else {
       AdjStage a;
       std::memset(&a, 0, sizeof a);
       a.lam = r->lam.p; a.l0 = r->lam.p; a.q = r->q.p; a.b2 = r->b2.p; a.lout = r->b1.p;
       a.rhs_out = r->rhs.p; a.stage = 4; a.dt = 0.0; a.forcing_factor = factor;
       a.zero_forcing = r->aforce_zero; a.ip_act = r->ip_act;
-      launch_adj_stage2d(r->geo, dpatches(r), a, r->stream);
+      launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
     }
     MGD_HIP(hipGetLastError());
     MGD_HIP(hipStreamSynchronize(r->stream));
@@ -1233,7 +1256,7 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
         // stage 2: reads the stage state (with halo), buffer1 and buffer2; writes buffer2 and
         // the next stage state
         FwdStage a = fwd_args(r, r->qy.p, r->q.p, r->b2.p, r->qs.p, 2, 1e-3, 0, 1, false);
-        launch_fwd_stage2d(r->geo, dpatches(r), a, r->stream);
+        launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
         break;
       }
       case MGD_KERNEL_ADJ_STAGE: {
@@ -1241,7 +1264,7 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
         std::memset(&a, 0, sizeof a);
         a.lam = r->ly.p; a.l0 = r->lam.p; a.q = r->q.p; a.b2 = r->b2.p; a.lout = r->l0.p;
         a.stage = 3; a.dt = 1e-3; a.forcing_factor = 1.0; a.ip_act = r->ip_act;
-        launch_adj_stage2d(r->geo, dpatches(r), a, r->stream);
+        launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
         break;
       }
       case MGD_KERNEL_COPY:
