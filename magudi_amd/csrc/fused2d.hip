@@ -25,8 +25,9 @@ namespace mgd {
 
 constexpr int TX = 32, TY = 16, HX = 4, HY = 4;
 constexpr int EX = TX + 2 * HX, EY = TY + 2 * HY;
-constexpr int NT = 256;
-constexpr int PPT = TX * TY / NT;   // points per thread (2)
+constexpr int NT = TX * TY;        // one point per thread
+constexpr int PPT = 1;
+constexpr int kStageIters = (EX * EY + NT - 1) / NT;   // staged points per thread (2)
 constexpr int NV2 = 4;
 constexpr int kMinEdgeTile = 12;    // closure depth of the adjoint 4-8 operator
 // LDS (doubles): operand [4][EY][EX]; scratch for the fluxes, reused for the
@@ -70,33 +71,44 @@ __device__ __forceinline__ void stage_operand(const Geo& g, const TileGeo& t, co
                                               double* sq, double* sfx, double* sfy) {
   const Dir dx = g.dir[0], dy = g.dir[1];
   const int wx = t.sx + 2 * HX, wy = t.sy + 2 * HY;
-  for (int e = threadIdx.x; e < wx * wy; e += NT) {
+  // all loads of this thread are issued before any is consumed
+  double v[kStageIters][NV2];
+  int ex_[kStageIters], ey_[kStageIters], gi_[kStageIters], gj_[kStageIters];
+  bool ok[kStageIters];
+#pragma unroll
+  for (int it = 0; it < kStageIters; ++it) {
+    const int e = threadIdx.x + it * NT;
     const int ex = e % wx, ey = e / wx;
     const int gi = ext_index(dx, t.ex0 + ex), gj = ext_index(dy, t.ey0 + ey);
-    if (gi < 0 || gj < 0) continue;
     const bool rowx = ey >= HY && ey < HY + t.sy;
     const bool coly = ex >= HX && ex < HX + t.sx;
-    if (!rowx && !coly) continue;   // halo corners are never read
-    const long pp = gi + (long)g.n[0] * gj;
-    double v[NV2];
+    ok[it] = e < wx * wy && gi >= 0 && gj >= 0 && (rowx || coly);
+    ex_[it] = ex; ey_[it] = ey; gi_[it] = gi; gj_[it] = gj;
+    if (ok[it]) {
+      const long pp = gi + (long)g.n[0] * gj;
 #pragma unroll
-    for (int l = 0; l < NV2; ++l) {
-      v[l] = xi[l * g.N + pp];
-      sq[(l * EY + ey) * EX + ex] = v[l];
+      for (int l = 0; l < NV2; ++l) v[it][l] = xi[l * g.N + pp];
     }
+  }
+#pragma unroll
+  for (int it = 0; it < kStageIters; ++it) {
+    if (!ok[it]) continue;
+    const int ex = ex_[it], ey = ey_[it];
+#pragma unroll
+    for (int l = 0; l < NV2; ++l) sq[(l * EY + ey) * EX + ex] = v[it][l];
     if (FLUX) {
-      const Prim<2> s = prim<2>(v, g.gamma);
-      if (rowx) {
+      const Prim<2> s = prim<2>(v[it], g.gamma);
+      if (ey >= HY && ey < HY + t.sy) {
         double f[NV2];
-        inviscid_flux<2>(v, s, 0, f);
-        const double m = g.h[1][gj];
+        inviscid_flux<2>(v[it], s, 0, f);
+        const double m = g.h[1][gj_[it]];
 #pragma unroll
         for (int l = 0; l < NV2; ++l) sfx[(l * TY + (ey - HY)) * EX + ex] = m * f[l];
       }
-      if (coly) {
+      if (ex >= HX && ex < HX + t.sx) {
         double f[NV2];
-        inviscid_flux<2>(v, s, 1, f);
-        const double m = g.h[0][gi];
+        inviscid_flux<2>(v[it], s, 1, f);
+        const double m = g.h[0][gi_[it]];
 #pragma unroll
         for (int l = 0; l < NV2; ++l) sfy[(l * EY + ey) * TX + (ex - HX)] = m * f[l];
       }
@@ -243,7 +255,7 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
   }
 }
 
-__device__ __forceinline__ double block_sum256(double v, double* red) {
+__device__ __forceinline__ double block_sum(double v, double* red) {
   __syncthreads();   // retire earlier readers of the scratch space
   red[threadIdx.x] = v;
   __syncthreads();
@@ -259,7 +271,7 @@ __device__ __forceinline__ bool in_patch(const DPatch& pt, int i, int j) {
 }
 
 // -------------------------------------------------------------------------- forward
-__global__ void __launch_bounds__(NT) k_fwd_stage2d(Geo g, const Patches* __restrict__ P,
+__global__ void __launch_bounds__(NT, 4) k_fwd_stage2d(Geo g, const Patches* __restrict__ P,
                                                     Tiling T, FwdStage a) {
   __shared__ double sq[SQ];
   __shared__ double sc[SSCR];
@@ -270,6 +282,19 @@ __global__ void __launch_bounds__(NT) k_fwd_stage2d(Geo g, const Patches* __rest
   const double* qi = a.qin + inst * NVN;
   double* sfx = sc;
   double* sfy = sc + SFX;
+  // RK operands of this thread's point, fetched before the stencil work
+  double pq0[NV2], pb2[NV2];
+  {
+    const int tx0 = threadIdx.x % TX, ty0 = threadIdx.x / TX;
+    if (tx0 < t.sx && ty0 < t.sy) {
+      const long o = (t.x0 + tx0) + (long)g.n[0] * (t.y0 + ty0);
+#pragma unroll
+      for (int l = 0; l < NV2; ++l) {
+        pq0[l] = (a.stage == 2 || a.stage == 3) ? a.q0[inst * NVN + l * g.N + o] : 0.0;
+        pb2[l] = a.stage >= 2 ? a.b2[inst * NVN + l * g.N + o] : 0.0;
+      }
+    }
+  }
   stage_operand<true>(g, t, qi, sq, sfx, sfy);
   __syncthreads();
   const Op& o1 = g.ops[SLOT_FIRST];
@@ -363,7 +388,6 @@ __global__ void __launch_bounds__(NT) k_fwd_stage2d(Geo g, const Patches* __rest
     // RK4 stage update (substepForwardRK4, src/RK4IntegratorImpl.f90:115-156)
     const double dt = a.dt;
     double* b2 = a.b2 + inst * NVN;
-    const double* q0 = a.q0 + inst * NVN;
     double qn[NV2];
 #pragma unroll
     for (int l = 0; l < NV2; ++l) {
@@ -373,13 +397,13 @@ __global__ void __launch_bounds__(NT) k_fwd_stage2d(Geo g, const Patches* __rest
         b2[o] = qp[l] + dt * rl / 6.0;
         qn[l] = qp[l] + dt * rl / 2.0;
       } else if (a.stage == 2) {
-        b2[o] = b2[o] + dt * rl / 3.0;
-        qn[l] = q0[o] + dt * rl / 2.0;
+        b2[o] = pb2[l] + dt * rl / 3.0;
+        qn[l] = pq0[l] + dt * rl / 2.0;
       } else if (a.stage == 3) {
-        b2[o] = b2[o] + dt * rl / 3.0;
-        qn[l] = q0[o] + dt * rl;
+        b2[o] = pb2[l] + dt * rl / 3.0;
+        qn[l] = pq0[l] + dt * rl;
       } else {
-        qn[l] = b2[o] + dt * rl / 6.0;
+        qn[l] = pb2[l] + dt * rl / 6.0;
       }
       a.qout[inst * NVN + o] = qn[l];
       if (a.rhs_out) a.rhs_out[inst * NVN + o] = rl;
@@ -397,7 +421,7 @@ __global__ void __launch_bounds__(NT) k_fwd_stage2d(Geo g, const Patches* __rest
   if (a.jpartial && a.ip_cost >= 0) {
     const int rank = a.tile_rank[tile];
     if (rank >= 0) {
-      const double s = block_sum256(jpart, sc);
+      const double s = block_sum(jpart, sc);
       if (threadIdx.x == 0) a.jpartial[(inst * a.nstage_slots + a.stage_slot) * a.ntiles + rank] = s;
     }
   }
@@ -414,6 +438,19 @@ __global__ void __launch_bounds__(NT) k_adj_stage2d(Geo g, const Patches* __rest
   const long NVN = (long)NV2 * g.N;
   const double* li = a.lam + inst * NVN;
   const double* qi = a.q + inst * NVN;
+  double pq[NV2], pl0[NV2], pb2[NV2];
+  {
+    const int tx0 = threadIdx.x % TX, ty0 = threadIdx.x / TX;
+    if (tx0 < t.sx && ty0 < t.sy) {
+      const long o = (t.x0 + tx0) + (long)g.n[0] * (t.y0 + ty0);
+#pragma unroll
+      for (int l = 0; l < NV2; ++l) {
+        pq[l] = qi[l * g.N + o];
+        pl0[l] = (a.stage == 3 || a.stage == 2) ? a.l0[inst * NVN + l * g.N + o] : 0.0;
+        pb2[l] = a.stage <= 3 ? a.b2[inst * NVN + l * g.N + o] : 0.0;
+      }
+    }
+  }
   stage_operand<false>(g, t, li, sq, nullptr, nullptr);
   double* stx = sc;
   double* sty = sc + STX;
@@ -437,10 +474,7 @@ __global__ void __launch_bounds__(NT) k_adj_stage2d(Geo g, const Patches* __rest
     deriv_lds(oa, g.dir[1], gj, t.ey0,
               [&](int l, int e) { return sq[(l * EY + e) * EX + tx + HX]; }, T1[1]);
     const long p = gi + (long)g.n[0] * gj;
-    double v[NV2];
-#pragma unroll
-    for (int l = 0; l < NV2; ++l) v[l] = qi[l * g.N + p];
-    const Prim<2> s = prim<2>(v, g.gamma);
+    const Prim<2> s = prim<2>(pq, g.gamma);
     const int jk[3] = {gi, gj, 0};
     double r[NV2] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -523,7 +557,6 @@ __global__ void __launch_bounds__(NT) k_adj_stage2d(Geo g, const Patches* __rest
     }
     // RK4 adjoint stage update (substepAdjointRK4, src/RK4IntegratorImpl.f90:215-261)
     const double dt = a.dt;
-    const double* b1 = a.l0 + inst * NVN;
     double* b2 = a.b2 + inst * NVN;
 #pragma unroll
     for (int l = 0; l < NV2; ++l) {
@@ -534,13 +567,13 @@ __global__ void __launch_bounds__(NT) k_adj_stage2d(Geo g, const Patches* __rest
         b2[o] = lp[l] - dt * rl / 6.0;
         ln = lp[l] - dt * rl / 2.0;
       } else if (a.stage == 3) {
-        b2[o] = b2[o] - dt * rl / 3.0;
-        ln = b1[o] - dt * rl / 2.0;
+        b2[o] = pb2[l] - dt * rl / 3.0;
+        ln = pl0[l] - dt * rl / 2.0;
       } else if (a.stage == 2) {
-        b2[o] = b2[o] - dt * rl / 3.0;
-        ln = b1[o] - dt * rl;
+        b2[o] = pb2[l] - dt * rl / 3.0;
+        ln = pl0[l] - dt * rl;
       } else {
-        ln = b2[o] - dt * rl / 6.0;
+        ln = pb2[l] - dt * rl / 6.0;
       }
       a.lout[inst * NVN + o] = ln;
       if (a.rhs_out) a.rhs_out[inst * NVN + o] = rl;
@@ -549,7 +582,7 @@ __global__ void __launch_bounds__(NT) k_adj_stage2d(Geo g, const Patches* __rest
   if (a.spartial && a.ip_act >= 0) {
     const int rank = a.tile_rank[tile];
     if (rank >= 0) {
-      const double s2 = block_sum256(spart, sc);
+      const double s2 = block_sum(spart, sc);
       if (threadIdx.x == 0) a.spartial[(inst * a.nstage_slots + a.stage_slot) * a.ntiles + rank] = s2;
     }
   }
@@ -563,7 +596,7 @@ __global__ void k_stage_sums(const double* __restrict__ partial, int nstages, in
   const long inst = blockIdx.y;
   double v = 0.0;
   for (int t = threadIdx.x; t < ntiles; t += NT) v += partial[(inst * nstages + s) * ntiles + t];
-  const double sum = block_sum256(v, red);
+  const double sum = block_sum(v, red);
   if (threadIdx.x == 0) sums[inst * nstages + s] = sum;
 }
 
