@@ -116,18 +116,40 @@ __device__ __forceinline__ void stage_operand(const Geo& g, const TileGeo& t, co
   }
 }
 
-// First-derivative-type operator along x or y at a point of the tile, operand in LDS.
-// row(l, e) returns component l at extended index e along the direction.
-template <class Row>
+// Boundary-closure row of an operator with compile-time block size BW x BD (SBP 4-8:
+// first derivative 12 x 8, adjoint 16 x 12, dissipation transpose 7 x 5, composite
+// 12 x 8).  Unrolled so the coefficient loads issue together.  f(c) returns the NV2
+// operand values at closure column c (0-based from the boundary, mirrored on the right).
+template <int BW, int BD, class F>
+__device__ __forceinline__ void closure_row(const Op& op, int i, int len, F&& f, double* out) {
+  const bool left = i < BD;
+  const int m = left ? i : len - 1 - i;
+  const double* b = left ? op.b1[m] : op.b2[m];
+  const int base = left ? 0 : len - BW;
+#pragma unroll
+  for (int l = 0; l < NV2; ++l) out[l] = 0.0;
+#pragma unroll
+  for (int c = 0; c < BW; ++c) {
+    double v[NV2];
+    f(base + c, v);
+    const double bc = b[c];
+#pragma unroll
+    for (int l = 0; l < NV2; ++l) out[l] += bc * v[l];
+  }
+}
+
+// First-derivative-type operator (BW x BD closure) along x or y at a point of the tile,
+// operand in LDS; row(l, e) returns component l at extended index e.
+template <int BW, int BD, class Row>
 __device__ __forceinline__ void deriv_lds(const Op& op, const Dir& dd, int i, int e0, Row&& row,
                                           double* out) {
-  if (closure(dd, i, op.bd)) {
-    sbp_apply<NV2>(op, dd, i,
-                   [&](int idx, double* v) {
+  if (!dd.periodic && (i < BD || i >= dd.len - BD)) {
+    closure_row<BW, BD>(op, i, dd.len,
+                        [&](int idx, double* v) {
 #pragma unroll
-                     for (int l = 0; l < NV2; ++l) v[l] = row(l, idx - e0);
-                   },
-                   out);
+                          for (int l = 0; l < NV2; ++l) v[l] = row(l, idx - e0);
+                        },
+                        out);
     return;
   }
   const int e = i - e0;
@@ -140,76 +162,65 @@ __device__ __forceinline__ void deriv_lds(const Op& op, const Dir& dd, int i, in
   }
 }
 
-// Dissipation at a closure point from LDS: D4^T(-arc D4 x)/H or the composite operator.
+// D4 x at index i along a direction, with magudi's boundary rows: on a non-periodic
+// direction rows 0,1 (and the last two) repeat the stencil centred two points inside
+// (src/StencilOperatorImpl.f90:1928-1940), summed in the closure-row order.
 template <class Row>
-__device__ __forceinline__ void diss_closure(const Geo& g, int d, int i, int e0, int jother,
-                                             Row&& row, double* out) {
-  const Dir dd = g.dir[d];
-  if (g.composite) {
-    sbp_apply<NV2>(g.ops[SLOT_DISS], dd, i,
-                   [&](int idx, double* v) {
-#pragma unroll
-                     for (int l = 0; l < NV2; ++l) v[l] = row(l, idx - e0);
-                   },
-                   out);
-    return;
+__device__ __forceinline__ double d4_at(const Dir& dd, int i, double c0, double c1, double c2,
+                                        Row&& x) {
+  if (!dd.periodic && (i < 2 || i >= dd.len - 2)) {
+    const int ctr = i < 2 ? 2 : dd.len - 3;
+    double s = 0.0;
+    s += c2 * x(ctr - 2);
+    s += c1 * x(ctr - 1);
+    s += c0 * x(ctr);
+    s += c1 * x(ctr + 1);
+    s += c2 * x(ctr + 2);
+    return s;
   }
-  sbp_apply<NV2>(g.ops[SLOT_DISS_T], dd, i,
-                 [&](int idx, double* tv) {
-                   sbp_apply<NV2>(g.ops[SLOT_DISS], dd, idx,
-                                  [&](int idx2, double* v) {
-#pragma unroll
-                                    for (int l = 0; l < NV2; ++l) v[l] = row(l, idx2 - e0);
-                                  },
-                                  tv);
-                   // arc length along d depends on the other index only (rectilinear)
-                   const double arc = fabs(d == 0 ? g.h[1][jother] : g.h[0][jother]);
-#pragma unroll
-                   for (int l = 0; l < NV2; ++l) tv[l] = -arc * tv[l];
-                 },
-                 out);
-  const double h = g.hw[d][i];
-#pragma unroll
-  for (int l = 0; l < NV2; ++l) out[l] = out[l] / h;
+  double s = 0.0;
+  s += c1 * (x(i + 1) + x(i - 1));
+  s += c2 * (x(i + 2) + x(i - 2));
+  return s + c0 * x(i);
 }
 
-// t = -arc D4 x for the tile rows (x) and tile columns (y), interior D4 rows; only
-// consumed by points outside the closure zones.
+// t = -arc D4 x on the tile rows (x direction, columns x0-2 .. x0+sx+1) and tile columns
+// (y direction, rows y0-2 .. y0+sy+1), for staged points inside the domain.
 __device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& t, const double* sq,
                                                    double* stx, double* sty) {
   const Op& od = g.ops[SLOT_DISS];
   const double c0 = od.c[kMaxHW], c1 = od.c[kMaxHW + 1], c2 = od.c[kMaxHW + 2];
+  const Dir dx = g.dir[0], dy = g.dir[1];
   const int wx = t.sx + 4;
   for (int e = threadIdx.x; e < t.sy * wx; e += NT) {
     const int ex = e % wx + 2, ey = e / wx + HY;
+    const int gi = t.ex0 + ex;
+    if (!dx.periodic && (gi < 0 || gi >= dx.len)) continue;
     const double arc = fabs(g.h[1][t.y0 + ey - HY]);
 #pragma unroll
     for (int l = 0; l < NV2; ++l) {
-      const double* r = sq + (l * EY + ey) * EX + ex;
-      double s = 0.0;
-      s += c1 * (r[1] + r[-1]);
-      s += c2 * (r[2] + r[-2]);
-      s = s + c0 * r[0];
+      const double* r = sq + (l * EY + ey) * EX;
+      const double s = d4_at(dx, gi, c0, c1, c2, [&](int k) { return r[k - t.ex0]; });
       stx[(l * TY + (ey - HY)) * (TX + 4) + (ex - 2)] = -arc * s;
     }
   }
   const int hy = t.sy + 4;
   for (int e = threadIdx.x; e < hy * t.sx; e += NT) {
     const int ex = e % t.sx + HX, ey = e / t.sx + 2;
+    const int gj = t.ey0 + ey;
+    if (!dy.periodic && (gj < 0 || gj >= dy.len)) continue;
     const double arc = fabs(g.h[0][t.x0 + ex - HX]);
 #pragma unroll
     for (int l = 0; l < NV2; ++l) {
-      const double* c = sq + (l * EY + ey) * EX + ex;
-      double s = 0.0;
-      s += c1 * (c[EX] + c[-EX]);
-      s += c2 * (c[2 * EX] + c[-2 * EX]);
-      s = s + c0 * c[0];
+      const double* c = sq + l * EY * EX + ex;
+      const double s = d4_at(dy, gj, c0, c1, c2, [&](int k) { return c[(k - t.ey0) * EX]; });
       sty[(l * (TY + 4) + (ey - 2)) * TX + (ex - HX)] = -arc * s;
     }
   }
 }
 
-// Adds amount * (dissipation in x then in y) at one point.
+// Adds amount * (dissipation in x then in y) at one point: H^-1 D4^T t (closure rows of
+// D4^T near non-periodic boundaries) or the composite operator on the staged operand.
 __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, const double* sq,
                                                 const double* stx, const double* sty, int tx,
                                                 int ty, int gi, int gj, double amount, double* r) {
@@ -218,37 +229,56 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
 #pragma unroll
   for (int d = 0; d < 2; ++d) {
     double dv[NV2];
+    const Dir& dr = g.dir[d];
     const int i = d == 0 ? gi : gj;
-    if (closure(g.dir[d], i, 8)) {
-      if (d == 0)
-        diss_closure(g, 0, gi, t.ex0, gj,
-                     [&](int l, int e) { return sq[(l * EY + ty + HY) * EX + e]; }, dv);
-      else
-        diss_closure(g, 1, gj, t.ey0, gi,
-                     [&](int l, int e) { return sq[(l * EY + e) * EX + tx + HX]; }, dv);
-    } else if (g.composite) {
+    if (g.composite) {
+      auto row = [&](int l, int e) {
+        return d == 0 ? sq[(l * EY + ty + HY) * EX + e] : sq[(l * EY + e) * EX + tx + HX];
+      };
+      const int e0 = d == 0 ? t.ex0 : t.ey0;
+      if (!dr.periodic && (i < 8 || i >= dr.len - 8)) {
+        closure_row<12, 8>(od, i, dr.len,
+                           [&](int idx, double* v) {
 #pragma unroll
-      for (int l = 0; l < NV2; ++l) {
-        const double* b = sq + (l * EY + ty + HY) * EX + tx + HX;
-        const int st = d == 0 ? 1 : EX;
-        double s = 0.0;
+                             for (int l = 0; l < NV2; ++l) v[l] = row(l, idx - e0);
+                           },
+                           dv);
+      } else {
+        const int e = i - e0;
 #pragma unroll
-        for (int k = 1; k <= 4; ++k) s += od.c[kMaxHW + k] * (b[k * st] + b[-k * st]);
-        dv[l] = s + od.c[kMaxHW] * b[0];
+        for (int l = 0; l < NV2; ++l) {
+          double s = 0.0;
+#pragma unroll
+          for (int k = 1; k <= 4; ++k) s += od.c[kMaxHW + k] * (row(l, e + k) + row(l, e - k));
+          dv[l] = s + od.c[kMaxHW] * row(l, e);
+        }
       }
     } else {
-      const double c0 = ot.c[kMaxHW], c1 = ot.c[kMaxHW + 1], c2 = ot.c[kMaxHW + 2];
+      // t index: x uses columns from x0-2, y uses rows from y0-2
+      auto tv = [&](int l, int idx) {
+        return d == 0 ? stx[(l * TY + ty) * (TX + 4) + (idx - (t.x0 - 2))]
+                      : sty[(l * (TY + 4) + (idx - (t.y0 - 2))) * TX + tx];
+      };
+      if (!dr.periodic && (i < 5 || i >= dr.len - 5)) {
+        closure_row<7, 5>(ot, i, dr.len,
+                          [&](int idx, double* v) {
 #pragma unroll
-      for (int l = 0; l < NV2; ++l) {
-        const double* b = d == 0 ? stx + (l * TY + ty) * (TX + 4) + tx + 2
-                                 : sty + (l * (TY + 4) + ty + 2) * TX + tx;
-        const int st = d == 0 ? 1 : TX;
-        double s = 0.0;
-        s += c1 * (b[st] + b[-st]);
-        s += c2 * (b[2 * st] + b[-2 * st]);
-        s = s + c0 * b[0];
-        dv[l] = s / 1.0;
+                            for (int l = 0; l < NV2; ++l) v[l] = tv(l, idx);
+                          },
+                          dv);
+      } else {
+        const double c0 = ot.c[kMaxHW], c1 = ot.c[kMaxHW + 1], c2 = ot.c[kMaxHW + 2];
+#pragma unroll
+        for (int l = 0; l < NV2; ++l) {
+          double s = 0.0;
+          s += c1 * (tv(l, i + 1) + tv(l, i - 1));
+          s += c2 * (tv(l, i + 2) + tv(l, i - 2));
+          dv[l] = s + c0 * tv(l, i);
+        }
       }
+      const double h = g.hw[d][i];
+#pragma unroll
+      for (int l = 0; l < NV2; ++l) dv[l] = dv[l] / h;
     }
 #pragma unroll
     for (int l = 0; l < NV2; ++l) r[l] = r[l] + amount * dv[l];
@@ -306,9 +336,9 @@ __global__ void __launch_bounds__(NT, 4) k_fwd_stage2d(Geo g, const Patches* __r
     if (tx >= t.sx || ty >= t.sy) continue;
     const int gi = t.x0 + tx, gj = t.y0 + ty;
     double dfx[NV2], dfy[NV2];
-    deriv_lds(o1, g.dir[0], gi, t.ex0,
+    deriv_lds<12, 8>(o1, g.dir[0], gi, t.ex0,
               [&](int l, int e) { return sfx[(l * TY + ty) * EX + e]; }, dfx);
-    deriv_lds(o1, g.dir[1], gj, t.ey0,
+    deriv_lds<12, 8>(o1, g.dir[1], gj, t.ey0,
               [&](int l, int e) { return sfy[(l * EY + e) * TX + tx]; }, dfy);
 #pragma unroll
     for (int l = 0; l < NV2; ++l) r[k][l] = 0.0 - (dfx[l] + dfy[l]);
@@ -469,9 +499,9 @@ __global__ void __launch_bounds__(NT) k_adj_stage2d(Geo g, const Patches* __rest
     if (tx >= t.sx || ty >= t.sy) continue;
     const int gi = t.x0 + tx, gj = t.y0 + ty;
     double T1[2][NV2];
-    deriv_lds(oa, g.dir[0], gi, t.ex0,
+    deriv_lds<16, 12>(oa, g.dir[0], gi, t.ex0,
               [&](int l, int e) { return sq[(l * EY + ty + HY) * EX + e]; }, T1[0]);
-    deriv_lds(oa, g.dir[1], gj, t.ey0,
+    deriv_lds<16, 12>(oa, g.dir[1], gj, t.ey0,
               [&](int l, int e) { return sq[(l * EY + e) * EX + tx + HX]; }, T1[1]);
     const long p = gi + (long)g.n[0] * gj;
     const Prim<2> s = prim<2>(pq, g.gamma);
