@@ -242,9 +242,6 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
             dv[l] += bc * tv;
           }
         }
-        const double h = g.hw[d][i];
-#pragma unroll
-        for (int l = 0; l < NV2; ++l) dv[l] = dv[l] / h;
       } else {
         const int e = i - e0;
 #pragma unroll
@@ -258,6 +255,10 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
           dv[l] = -arc * s;
         }
       }
+      // H^-1 of the first-derivative norm (its 8 boundary weights; 1 elsewhere)
+      const double h = g.hw[d][i];
+#pragma unroll
+      for (int l = 0; l < NV2; ++l) dv[l] = dv[l] / h;
     }
 #pragma unroll
     for (int l = 0; l < NV2; ++l) r[l] = r[l] + amount * dv[l];
