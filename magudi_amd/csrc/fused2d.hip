@@ -30,11 +30,12 @@ constexpr int PPT = 1;
 constexpr int kStageIters = (EX * EY + NT - 1) / NT;   // staged points per thread (2)
 constexpr int NV2 = 4;
 constexpr int kMinEdgeTile = 12;    // closure depth of the adjoint 4-8 operator
-// LDS (doubles): operand [4][EY][EX]; scratch for the fluxes, reused for the final
-// reduction.
+// LDS (doubles): operand [4][EY][EX]; scratch for the fluxes, reused for the
+// dissipation intermediates and the final reduction.
 constexpr int SQ = NV2 * EY * EX;
 constexpr int SFX = NV2 * TY * EX, SFY = NV2 * EY * TX;
-constexpr int SSCR = SFX + SFY;
+constexpr int STX = NV2 * TY * (TX + 4), STY = NV2 * (TY + 4) * TX;
+constexpr int SSCR = (SFX + SFY) > (STX + STY) ? (SFX + SFY) : (STX + STY);
 
 struct TileGeo {
   int x0, y0, sx, sy;   // origin and size of the tile
@@ -183,15 +184,46 @@ __device__ __forceinline__ double d4_at(const Dir& dd, int i, double c0, double 
   return s + c0 * x(i);
 }
 
-// Adds amount * (dissipation in x then in y) at one point (addDissipation,
-// src/RhsHelperImpl.f90:58-81).  On a rectilinear grid the arc length along a
-// direction is constant on each grid line, so away from the closures
-// H^-1 D4^T diag(-arc) D4 q = -arc (D4^T D4) q, the 9-point stencil
-// (1, -8, 28, -56, 70, -56, 28, -8, 1), applied to the staged operand directly.  Closure
-// points evaluate D4^T's boundary rows over -arc D4 q formed on the fly.
+// t = -arc D4 x on the tile rows (x direction, columns x0-2 .. x0+sx+1) and tile columns
+// (y direction, rows y0-2 .. y0+sy+1), for staged points inside the domain.
+__device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& t, const double* sq,
+                                                   double* stx, double* sty) {
+  const Op& od = g.ops[SLOT_DISS];
+  const double c0 = od.c[kMaxHW], c1 = od.c[kMaxHW + 1], c2 = od.c[kMaxHW + 2];
+  const Dir dx = g.dir[0], dy = g.dir[1];
+  const int wx = t.sx + 4;
+  for (int e = threadIdx.x; e < t.sy * wx; e += NT) {
+    const int ex = e % wx + 2, ey = e / wx + HY;
+    const int gi = t.ex0 + ex;
+    if (!dx.periodic && (gi < 0 || gi >= dx.len)) continue;
+    const double arc = fabs(g.h[1][t.y0 + ey - HY]);
+#pragma unroll
+    for (int l = 0; l < NV2; ++l) {
+      const double* r = sq + (l * EY + ey) * EX;
+      const double s = d4_at(dx, gi, c0, c1, c2, [&](int k) { return r[k - t.ex0]; });
+      stx[(l * TY + (ey - HY)) * (TX + 4) + (ex - 2)] = -arc * s;
+    }
+  }
+  const int hy = t.sy + 4;
+  for (int e = threadIdx.x; e < hy * t.sx; e += NT) {
+    const int ex = e % t.sx + HX, ey = e / t.sx + 2;
+    const int gj = t.ey0 + ey;
+    if (!dy.periodic && (gj < 0 || gj >= dy.len)) continue;
+    const double arc = fabs(g.h[0][t.x0 + ex - HX]);
+#pragma unroll
+    for (int l = 0; l < NV2; ++l) {
+      const double* c = sq + l * EY * EX + ex;
+      const double s = d4_at(dy, gj, c0, c1, c2, [&](int k) { return c[(k - t.ey0) * EX]; });
+      sty[(l * (TY + 4) + (ey - 2)) * TX + (ex - HX)] = -arc * s;
+    }
+  }
+}
+
+// Adds amount * (dissipation in x then in y) at one point: H^-1 D4^T t (closure rows of
+// D4^T near non-periodic boundaries) or the composite operator on the staged operand.
 __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, const double* sq,
-                                                int tx, int ty, int gi, int gj, double amount,
-                                                double* r) {
+                                                const double* stx, const double* sty, int tx,
+                                                int ty, int gi, int gj, double amount, double* r) {
   const Op& od = g.ops[SLOT_DISS];
   const Op& ot = g.ops[SLOT_DISS_T];
 #pragma unroll
@@ -199,11 +231,11 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
     double dv[NV2];
     const Dir& dr = g.dir[d];
     const int i = d == 0 ? gi : gj;
-    const int e0 = d == 0 ? t.ex0 : t.ey0;
-    auto row = [&](int l, int e) {
-      return d == 0 ? sq[(l * EY + ty + HY) * EX + e] : sq[(l * EY + e) * EX + tx + HX];
-    };
     if (g.composite) {
+      auto row = [&](int l, int e) {
+        return d == 0 ? sq[(l * EY + ty + HY) * EX + e] : sq[(l * EY + e) * EX + tx + HX];
+      };
+      const int e0 = d == 0 ? t.ex0 : t.ey0;
       if (!dr.periodic && (i < 8 || i >= dr.len - 8)) {
         closure_row<12, 8>(od, i, dr.len,
                            [&](int idx, double* v) {
@@ -222,40 +254,28 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
         }
       }
     } else {
-      const double arc = fabs(d == 0 ? g.h[1][gj] : g.h[0][gi]);
+      // t index: x uses columns from x0-2, y uses rows from y0-2
+      auto tv = [&](int l, int idx) {
+        return d == 0 ? stx[(l * TY + ty) * (TX + 4) + (idx - (t.x0 - 2))]
+                      : sty[(l * (TY + 4) + (idx - (t.y0 - 2))) * TX + tx];
+      };
       if (!dr.periodic && (i < 5 || i >= dr.len - 5)) {
-        // closure rows of D4^T (7 x 5 block) over t = -arc D4 q; kept rolled (few points)
-        const double c0 = od.c[kMaxHW], c1 = od.c[kMaxHW + 1], c2 = od.c[kMaxHW + 2];
-        const bool left = i < 5;
-        const int m = left ? i : dr.len - 1 - i;
-        const double* b = left ? ot.b1[m] : ot.b2[m];
-        const int base = left ? 0 : dr.len - 7;
+        closure_row<7, 5>(ot, i, dr.len,
+                          [&](int idx, double* v) {
 #pragma unroll
-        for (int l = 0; l < NV2; ++l) dv[l] = 0.0;
-#pragma unroll 1
-        for (int c = 0; c < 7; ++c) {
-          const double bc = b[c];
-#pragma unroll
-          for (int l = 0; l < NV2; ++l) {
-            const double tv = -arc * d4_at(dr, base + c, c0, c1, c2,
-                                           [&](int k) { return row(l, k - e0); });
-            dv[l] += bc * tv;
-          }
-        }
+                            for (int l = 0; l < NV2; ++l) v[l] = tv(l, idx);
+                          },
+                          dv);
       } else {
-        const int e = i - e0;
+        const double c0 = ot.c[kMaxHW], c1 = ot.c[kMaxHW + 1], c2 = ot.c[kMaxHW + 2];
 #pragma unroll
         for (int l = 0; l < NV2; ++l) {
           double s = 0.0;
-          s += -56.0 * (row(l, e + 1) + row(l, e - 1));
-          s += 28.0 * (row(l, e + 2) + row(l, e - 2));
-          s += -8.0 * (row(l, e + 3) + row(l, e - 3));
-          s += 1.0 * (row(l, e + 4) + row(l, e - 4));
-          s = s + 70.0 * row(l, e);
-          dv[l] = -arc * s;
+          s += c1 * (tv(l, i + 1) + tv(l, i - 1));
+          s += c2 * (tv(l, i + 2) + tv(l, i - 2));
+          dv[l] = s + c0 * tv(l, i);
         }
       }
-      // H^-1 of the first-derivative norm (its 8 boundary weights; 1 elsewhere)
       const double h = g.hw[d][i];
 #pragma unroll
       for (int l = 0; l < NV2; ++l) dv[l] = dv[l] / h;
@@ -325,6 +345,15 @@ __global__ void __launch_bounds__(NT, 4) k_fwd_stage2d(Geo g, const Patches* __r
 #pragma unroll
     for (int l = 0; l < NV2; ++l) r[k][l] = 0.0 - (dfx[l] + dfy[l]);
   }
+  double* stx = sc;
+  double* sty = sc + STX;
+  if (g.dissipation) {
+    __syncthreads();   // fluxes consumed
+    if (!g.composite) {
+      diss_intermediates(g, t, sq, stx, sty);
+      __syncthreads();
+    }
+  }
   const unsigned pmask = T.pmask[tile];
   double jpart = 0.0;
 #pragma unroll
@@ -332,7 +361,7 @@ __global__ void __launch_bounds__(NT, 4) k_fwd_stage2d(Geo g, const Patches* __r
     const int ty = threadIdx.x / TX + k * (NT / TX);
     if (tx >= t.sx || ty >= t.sy) continue;
     const int gi = t.x0 + tx, gj = t.y0 + ty;
-    if (g.dissipation) add_dissipation(g, t, sq, tx, ty, gi, gj, g.diss_amount, r[k]);
+    if (g.dissipation) add_dissipation(g, t, sq, stx, sty, tx, ty, gi, gj, g.diss_amount, r[k]);
     const int jk[3] = {gi, gj, 0};
     const long p = gi + (long)g.n[0] * gj;
     const double jac = jacobian<2>(g, jk);
@@ -455,6 +484,12 @@ __global__ void __launch_bounds__(NT, 4) k_adj_stage2d(Geo g, const Patches* __r
     }
   }
   stage_operand<false>(g, t, li, sq, nullptr, nullptr);
+  double* stx = sc;
+  double* sty = sc + STX;
+  if (g.dissipation && !g.composite) {
+    __syncthreads();
+    diss_intermediates(g, t, sq, stx, sty);
+  }
   __syncthreads();
   const Op& oa = g.ops[SLOT_ADJ];
   const unsigned pmask = T.pmask[tile];
@@ -488,7 +523,7 @@ __global__ void __launch_bounds__(NT, 4) k_adj_stage2d(Geo g, const Patches* __r
         r[i] = r[i] + acc;
       }
     }
-    if (g.dissipation) add_dissipation(g, t, sq, tx, ty, gi, gj, -g.diss_amount, r);
+    if (g.dissipation) add_dissipation(g, t, sq, stx, sty, tx, ty, gi, gj, -g.diss_amount, r);
     const double jac = jacobian<2>(g, jk);
     double lp[NV2];
 #pragma unroll
