@@ -44,7 +44,26 @@ def parse():
     ap.add_argument("--no-extras", action="store_true",
                     help="headline only (no batched / plain / CPU legs)")
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the HIP-event timing of the stage kernels and the copy kernel "
+                         "(the workload of the rocprofv3 counter runs)")
     return ap.parse_args()
+
+
+def measured_traffic():
+    """HBM bytes per launch from the committed rocprofv3 counter summary, or {}.
+
+    Lines of the form `traffic <kernel> <bytes per launch>` in
+    profiles/pmc_traffic_monopole401.txt (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration factor written in the same file)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_monopole401.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic":
+                out[f[1]] = float(f[2])
+    return out
 
 
 def run_hot_path(R, S, steps):
@@ -89,6 +108,15 @@ def main():
     q0 = case.initial_state()
     R = Region(case)
     R.states[0].initialCondition = q0
+    if a.kernels_only:
+        R.restart()
+        res = {}
+        for name, k in (("k_fwd_stage2d", capi.KERNEL_FWD_STAGE),
+                        ("k_adj_stage2d", capi.KERNEL_ADJ_STAGE), ("k_copy", capi.KERNEL_COPY)):
+            ms, b = R.time_kernel(k, reps=5)
+            res[name] = {"avg_launch_us": ms * 1e3, "bytes_per_launch": b}
+        print(json.dumps({"kernels_only": res}), flush=True)
+        return 0
     S = Solver(R)
     for _ in range(a.warmup):
         run_hot_path(R, S, 1)
@@ -97,6 +125,7 @@ def main():
     value = pts / t_step
 
     # dominant kernel: HIP-event timing of the fused stage kernels on the region stream
+    # (a full RK4 step of 4 stage launches per repetition; bytes = average per launch)
     R.restart()
     fwd_ms, fwd_b = R.time_kernel(capi.KERNEL_FWD_STAGE, reps=50)
     adj_ms, adj_b = R.time_kernel(capi.KERNEL_ADJ_STAGE, reps=50)
@@ -110,9 +139,10 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k]["ms"] * kernels[k]["launches"])
     d = kernels[dom]
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+    traffic = measured_traffic().get(dom) if a.n == 401 else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "avg_launch_us": round(d["ms"] * 1e3, 3),
+                "traffic": traffic, "avg_launch_us": round(d["ms"] * 1e3, 3),
                 "bytes_per_launch": d["bytes"]}
     other = {k: {"avg_launch_us": round(v["ms"] * 1e3, 3),
                  "achieved_GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)}
@@ -131,6 +161,12 @@ def main():
                    "grid": [a.n, a.n], "nsteps": a.nsteps, "save_interval": a.save_interval,
                    "instances": 1, "path": "fused 2-D stage kernels"},
         "roofline": roofline,
+        # whole job against the compulsory-traffic model of one fwd+adjoint pt-step with
+        # window recompute, per state component: forward RK4 128 B, adjoint RK4 128 B,
+        # reading the 4 forward stage states 32 B, recompute 128 B, storing its stages 32 B
+        "job_roofline": {"bytes_per_pt_step": 448 * 4,
+                         "achieved_GBs": round(value * 448 * 4 / 1e9, 1),
+                         "frac": round(value * 448 * 4 / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": other,
         "kernel_time_share": round(kernel_share, 3),
         "J": float(J[0]), "Grad0": float(sens[0]),

@@ -181,11 +181,13 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches);
 #define MGD_OPT_FORCE_PLAIN 1
 int mgd_region_set_option(mgd_region* r, int option, int value);
 
-/* Times one kernel on the region stream with HIP events: `reps` back-to-back launches
- * after one warm-up.  which: MGD_KERNEL_FWD_STAGE (fused 2-D forward RK stage 2),
- * MGD_KERNEL_ADJ_STAGE (fused 2-D adjoint RK stage 3), MGD_KERNEL_COPY (state-sized
- * copy, the bandwidth reference).  bytes: algorithmic HBM bytes of one launch.
- * Benchmark support; it overwrites the region's RK and adjoint buffers. */
+/* Times one kernel on the region stream with HIP events after one warm-up repetition.
+ * which: MGD_KERNEL_FWD_STAGE (fused 2-D forward stage kernel, `reps` RK4 steps of 4
+ * stage launches), MGD_KERNEL_ADJ_STAGE (fused 2-D adjoint stage kernel, `reps` adjoint
+ * RK4 steps of 4 launches), MGD_KERNEL_COPY (`reps` state-sized copies, the bandwidth
+ * reference).  ms_per_launch: average over the launches; bytes: algorithmic HBM bytes
+ * of one launch, averaged over the 4 stages.  Benchmark support; it overwrites the
+ * region's RK and adjoint buffers (the conserved state is kept). */
 #define MGD_KERNEL_FWD_STAGE 0
 #define MGD_KERNEL_ADJ_STAGE 1
 #define MGD_KERNEL_COPY 2

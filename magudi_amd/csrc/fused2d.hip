@@ -276,9 +276,13 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
           dv[l] = s + c0 * tv(l, i);
         }
       }
+      // H^-1: the norm weight is exactly 1 away from the first / last boundary rows, where
+      // the division is the identity and is skipped
       const double h = g.hw[d][i];
+      if (h != 1.0) {
 #pragma unroll
-      for (int l = 0; l < NV2; ++l) dv[l] = dv[l] / h;
+        for (int l = 0; l < NV2; ++l) dv[l] = dv[l] / h;
+      }
     }
 #pragma unroll
     for (int l = 0; l < NV2; ++l) r[l] = r[l] + amount * dv[l];
@@ -460,7 +464,7 @@ __global__ void __launch_bounds__(NT, 4) k_fwd_stage2d(Geo g, const Patches* __r
 }
 
 // -------------------------------------------------------------------------- adjoint
-__global__ void __launch_bounds__(NT, 4) k_adj_stage2d(Geo g, const Patches* __restrict__ P,
+__global__ void __launch_bounds__(NT) k_adj_stage2d(Geo g, const Patches* __restrict__ P,
                                                     Tiling T, AdjStage a) {
   __shared__ double sq[SQ];
   __shared__ double sc[SSCR];

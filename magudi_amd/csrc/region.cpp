@@ -1247,24 +1247,33 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
   const double w = 8.0 * r->nv;   // bytes of one state point
   if (r->qy.alloc(M) || r->ly.alloc(M) || r->l0.alloc(M) || r->qs.alloc(M)) return MGD_ERR_HIP;
   std::vector<double> amp;
-  source_table(r, std::vector<double>{0.0}, amp);
+  source_table(r, std::vector<double>{0.0, 0.5e-3, 0.5e-3, 1e-3}, amp);
   if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
   if (r->control.p) r->control.alloc(0);
+  const double dt = 1e-3;
+  int per_rep = 1;
   auto launch = [&]() {
     switch (which) {
-      case MGD_KERNEL_FWD_STAGE: {
-        // stage 2: reads the stage state (with halo), buffer1 and buffer2; writes buffer2 and
-        // the next stage state
-        FwdStage a = fwd_args(r, r->qy.p, r->q.p, r->b2.p, r->qs.p, 2, 1e-3, 0, 1, false);
-        launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+      case MGD_KERNEL_FWD_STAGE:
+        // one RK4 step: stages 1-4 on the level state qs with stage states in qy / ly
+        fused_forward_step(r, r->qs.p, r->qy.p, r->ly.p, r->b2.p, 0, dt, 1, false, nullptr);
         break;
-      }
       case MGD_KERNEL_ADJ_STAGE: {
-        AdjStage a;
-        std::memset(&a, 0, sizeof a);
-        a.lam = r->ly.p; a.l0 = r->lam.p; a.q = r->q.p; a.b2 = r->b2.p; a.lout = r->l0.p;
-        a.stage = 3; a.dt = 1e-3; a.forcing_factor = 1.0; a.ip_act = r->ip_act;
-        launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+        // one adjoint RK4 step: stages 4-1 on the level lam with stage adjoints in ly / l0,
+        // the forward stage state held at q
+        double* L0 = r->lam.p;
+        double* LX = r->ly.p;
+        double* LY = r->l0.p;
+        for (int i = 4; i >= 1; --i) {
+          AdjStage a;
+          std::memset(&a, 0, sizeof a);
+          a.lam = i == 4 ? L0 : (i == 3 ? LX : (i == 2 ? LY : LX));
+          a.lout = i == 4 ? LX : (i == 3 ? LY : (i == 2 ? LX : L0));
+          a.l0 = L0; a.q = r->q.p; a.b2 = r->b2.p;
+          a.stage = i; a.dt = dt; a.forcing_factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
+          a.ip_act = r->ip_act;
+          launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+        }
         break;
       }
       case MGD_KERNEL_COPY:
@@ -1274,10 +1283,13 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
   };
   if (which == MGD_KERNEL_FWD_STAGE || which == MGD_KERNEL_ADJ_STAGE) {
     if (!fused_ok(r)) { set_error("no fused stage kernel for this region"); return MGD_ERR_UNSUPPORTED; }
+    per_rep = 4;
     // well-defined operands: every buffer holds the current state
+    launch_copy(M, r->q.p, r->qs.p, r->stream);
     launch_copy(M, r->q.p, r->qy.p, r->stream);
     launch_copy(M, r->q.p, r->b2.p, r->stream);
     launch_copy(M, r->q.p, r->ly.p, r->stream);
+    launch_copy(M, r->q.p, r->l0.p, r->stream);
     launch_copy(M, r->q.p, r->lam.p, r->stream);
   } else if (which != MGD_KERNEL_COPY) {
     set_error("unknown kernel");
@@ -1291,11 +1303,14 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
   MGD_HIP(hipEventSynchronize(r->ev1));
   float ms = 0.f;
   MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
-  if (ms_per_launch) *ms_per_launch = ms / reps;
+  if (ms_per_launch) *ms_per_launch = ms / (reps * per_rep);
   if (bytes) {
     switch (which) {
-      case MGD_KERNEL_FWD_STAGE: *bytes = 5.0 * w * pts; break;   // qin, q0, b2 in; b2, qout out
-      case MGD_KERNEL_ADJ_STAGE: *bytes = 6.0 * w * pts; break;   // lam, q, l0, b2 in; b2, lout out
+      // state arrays per step: forward 3 + 5 + 5 + 3 (stage state, level, buffer2 in;
+      // buffer2, next stage state out), adjoint 4 + 6 + 6 + 4 (adjoint stage state, level,
+      // forward stage state, buffer2 in; buffer2, next adjoint out)
+      case MGD_KERNEL_FWD_STAGE: *bytes = 16.0 / 4.0 * w * pts; break;
+      case MGD_KERNEL_ADJ_STAGE: *bytes = 20.0 / 4.0 * w * pts; break;
       default: *bytes = 2.0 * w * pts; break;
     }
   }
