@@ -115,6 +115,14 @@ def main():
                         ("k_adj_stage2d", capi.KERNEL_ADJ_STAGE), ("k_copy", capi.KERNEL_COPY)):
             ms, b = R.time_kernel(k, reps=5)
             res[name] = {"avg_launch_us": ms * 1e3, "bytes_per_launch": b}
+        res["tile_shape"] = R.tile_shape
+        # calibration copy larger than the L2 of all XCDs (a.batch states)
+        Rb = Region(case, n_instances=a.batch)
+        for i in range(a.batch):
+            Rb.states[i].initialCondition = q0
+        Rb.restart()
+        ms, b = Rb.time_kernel(capi.KERNEL_COPY, reps=5)
+        res["k_copy_batched"] = {"avg_launch_us": ms * 1e3, "bytes_per_launch": b}
         print(json.dumps({"kernels_only": res}), flush=True)
         return 0
     S = Solver(R)
@@ -159,7 +167,9 @@ def main():
         "config": {"workload": f"monopole {a.n}x{a.n} forward+adjoint, {a.nsteps} RK4 steps, "
                                f"HBM snapshots every {a.save_interval} steps with recompute",
                    "grid": [a.n, a.n], "nsteps": a.nsteps, "save_interval": a.save_interval,
-                   "instances": 1, "path": "fused 2-D stage kernels"},
+                   "instances": 1, "path": "fused 2-D stage kernels",
+                   "tile_shape": ["32x16", "16x32", "16x16", "24x16"][R.tile_shape]
+                   if R.tile_shape >= 0 else "generic"},
         "roofline": roofline,
         # whole job against the compulsory-traffic model of one fwd+adjoint pt-step with
         # window recompute, per state component: forward RK4 128 B, adjoint RK4 128 B,

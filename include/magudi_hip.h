@@ -177,9 +177,16 @@ int mgd_region_get_grid(mgd_region* r, int which, double* host);
 int mgd_last_timings(mgd_region* r, double* out, int* launches);
 
 /* Region options.  MGD_OPT_FORCE_PLAIN = 1 runs the generic per-operator kernels even
- * where the fused 2-D stage kernels apply (comparison and testing). */
+ * where the fused 2-D stage kernels apply (comparison and testing).
+ * MGD_OPT_TILE_SHAPE selects the tile of the fused 2-D kernels: -1 (default) autotune at
+ * the first run, 0: 32 x 16, 1: 16 x 32, 2: 16 x 16, 3: 24 x 16 points.  No result
+ * depends on the tile shape. */
 #define MGD_OPT_FORCE_PLAIN 1
+#define MGD_OPT_TILE_SHAPE 2
 int mgd_region_set_option(mgd_region* r, int option, int value);
+
+/* Tile shape the fused 2-D kernels use now (-1: the region runs the generic kernels). */
+int mgd_region_get_tile_shape(mgd_region* r, int* shape);
 
 /* Times one kernel on the region stream with HIP events after one warm-up repetition.
  * which: MGD_KERNEL_FWD_STAGE (fused 2-D forward stage kernel, `reps` RK4 steps of 4

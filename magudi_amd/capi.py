@@ -19,7 +19,7 @@ FORWARD, ADJOINT = 1, -1
 CONSERVED, ADJOINT_VARS, RHS, INITIAL_STATE = 0, 1, 2, 4
 FIELD_TARGET_STATE, FIELD_CONTROL_MOLLIFIER, FIELD_TARGET_MOLLIFIER = 0, 1, 2
 OP_FIRST, OP_ADJOINT_FIRST, OP_DISS, OP_DISS_T = 0, 1, 2, 3
-OPT_FORCE_PLAIN = 1
+OPT_FORCE_PLAIN, OPT_TILE_SHAPE = 1, 2
 KERNEL_FWD_STAGE, KERNEL_ADJ_STAGE, KERNEL_COPY = 0, 1, 2
 
 
@@ -80,6 +80,7 @@ def lib():
         "mgd_last_timings": [vp, dp, C.POINTER(C.c_int)],
         "mgd_sync": [vp],
         "mgd_region_set_option": [vp, C.c_int, C.c_int],
+        "mgd_region_get_tile_shape": [vp, C.POINTER(C.c_int)],
         "mgd_time_kernel": [vp, C.c_int, C.c_int, dp, dp],
         "mgd_state_copy": [vp, C.c_int, C.c_int],
     }

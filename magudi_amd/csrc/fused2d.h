@@ -6,19 +6,29 @@
 
 namespace mgd {
 
-// Tile partition of a 2-D grid (device view): tiles of at most 32 x 16 points with
-// origins org[0..nbx) along x and org[nbx..nbx+nby) along y; pmask: patches per tile.
+// Tile shapes of the fused kernels (points along x, y): 0: 32 x 16, 1: 16 x 32,
+// 2: 16 x 16, 3: 24 x 16.
+constexpr int kNumTileShapes = 4;
+constexpr int kNumXcd = 8;   // XCDs of an MI355X; work-groups are dealt to them round-robin
+
+// Tile partition of a 2-D grid (device view): tiles of at most TX x TY points with
+// origins org[0..nbx) along x and org[nbx..nbx+nby) along y; pmask: patches per tile
+// (tile index bx + nbx * by); tile_of_block: launch block -> tile, or -1.
 struct Tiling {
-  int nbx, nby;
+  int shape;
+  int nbx, nby, nblk;
   const int* org;
   const int* size;
   const unsigned* pmask;
+  const int* tile_of_block;
 };
 
 struct HostTiling {
-  int nbx = 0, nby = 0;
+  int shape = 0;
+  int nbx = 0, nby = 0, nblk = 0;
   std::vector<int> org, size;
   std::vector<unsigned> pmask;
+  std::vector<int> tile_of_block;
 };
 
 // One forward RK4 stage: rhs at `qin`, buffer update of `b2`, new stage state to `qout`.
@@ -36,9 +46,8 @@ struct FwdStage {
   const Source* src;
   const double* src_amp;    // amplitude of each source at this stage's time
   int nsrc;
-  double* jpartial;         // [inst][stage_slot][tile rank] functional partial sums, or null
-  const int* tile_rank;
-  int ntiles, nstage_slots, stage_slot, ip_cost;
+  double* jpartial;         // [inst][slot][cost-patch point] functional terms, or null
+  int npts, nslots, slot, ip_cost;
 };
 
 // One adjoint RK4 stage (substepAdjointRK4): rhs at (q, lam), new adjoint state to lout.
@@ -56,21 +65,24 @@ struct AdjStage {
   int zero_forcing;
   double* grad;             // gradient of this stage (per instance stride below), or null
   long grad_stride;
-  double* spartial;         // [inst][stage_slot][tile rank] sensitivity partials, or null
-  const int* tile_rank;
-  int ntiles, nstage_slots, stage_slot, ip_act;
+  double* spartial;         // [inst][slot][actuator-patch point] sensitivity terms, or null
+  int npts, nslots, slot, ip_act;
 };
 
 // Balanced partition; MGD_ERR_UNSUPPORTED when an edge tile of a non-periodic direction
 // would hold fewer points than the deepest boundary closure.
-int fused2d_make_tiling(const Geo& g, const Patches& hp, HostTiling* out);
-int fused2d_tile_ranks(const HostTiling& T, const int* lo, const int* hi, std::vector<int>& rank);
+int fused2d_make_tiling(const Geo& g, const Patches& hp, int shape, HostTiling* out);
+int fused2d_shape_dims(int shape, int* tx, int* ty);
 void launch_fwd_stage2d(const Geo& g, const Patches* P, const Tiling& T, const FwdStage& a,
                         hipStream_t st);
 void launch_adj_stage2d(const Geo& g, const Patches* P, const Tiling& T, const AdjStage& a,
                         hipStream_t st);
-void launch_stage_quadrature(const double* partial, int nstages, int ntiles, const double* w,
-                             double* stage_sums, double* acc, int ninst, int reverse,
-                             hipStream_t st);
+// Stage sums of per-point quadrature terms: sums[inst][base + s] = sum over the points of
+// partial[inst][s][point] for s < nfill (a fixed-order tree, independent of the tiling).
+void launch_stage_reduce(const double* partial, int nslots, int nfill, int npts, double* sums,
+                         int nstages, int base, int ninst, hipStream_t st);
+// acc[inst] += sum_s w[s] * sums[inst][s], in stage order (reverse: descending).
+void launch_stage_accumulate(const double* sums, int nstages, const double* w, double* acc,
+                             int ninst, int reverse, hipStream_t st);
 
 }  // namespace mgd

@@ -15,39 +15,47 @@
 // for the adjoint operator) reads staged LDS data like the interior stencil does.
 // Each tile carries a bit mask of the patches that intersect it.
 //
-// Per-stage functional / sensitivity partial sums are written per tile into
-// [stage][tile] slots and reduced in a fixed order at the end of a run, so results
-// are deterministic and independent of scheduling.
+// The functional / sensitivity quadrature terms are written per patch point and stage
+// and reduced by a fixed-order tree, so J and the sensitivity are deterministic and
+// independent of the tile shape and of scheduling.
 #include "device.h"
 #include "fused2d.h"
 
 namespace mgd {
 
-constexpr int TX = 32, TY = 16, HX = 4, HY = 4;
-constexpr int EX = TX + 2 * HX, EY = TY + 2 * HY;
-constexpr int NT = TX * TY;        // one point per thread
-constexpr int PPT = 1;
-constexpr int kStageIters = (EX * EY + NT - 1) / NT;   // staged points per thread (2)
+constexpr int HX = 4, HY = 4;     // halo of the staged operand
 constexpr int NV2 = 4;
 constexpr int kMinEdgeTile = 12;    // closure depth of the adjoint 4-8 operator
-// LDS (doubles): operand [4][EY][EX]; scratch for the fluxes, reused for the
-// dissipation intermediates and the final reduction.
-constexpr int SQ = NV2 * EY * EX;
-constexpr int SFX = NV2 * TY * EX, SFY = NV2 * EY * TX;
-constexpr int STX = NV2 * TY * (TX + 4), STY = NV2 * (TY + 4) * TX;
-constexpr int SSCR = (SFX + SFY) > (STX + STY) ? (SFX + SFY) : (STX + STY);
+constexpr int kRedThreads = 256;    // block size of the reduction kernels
+constexpr int PPT = 1;              // points per thread
+
+// Tile shape: TX x TY points, one point per thread.  LDS (doubles): operand
+// [4][EY][EX]; scratch for the fluxes, reused for the dissipation intermediates and the
+// final reduction.
+template <int TX_, int TY_>
+struct Shape {
+  static constexpr int TX = TX_, TY = TY_;
+  static constexpr int EX = TX + 2 * HX, EY = TY + 2 * HY;
+  static constexpr int NT = TX * TY;
+  static constexpr int kStageIters = (EX * EY + NT - 1) / NT;   // staged points per thread
+  static constexpr int SQ = NV2 * EY * EX;
+  static constexpr int SFX = NV2 * TY * EX, SFY = NV2 * EY * TX;
+  static constexpr int STX = NV2 * TY * (TX + 4), STY = NV2 * (TY + 4) * TX;
+  static constexpr int SSCR = (SFX + SFY) > (STX + STY) ? (SFX + SFY) : (STX + STY);
+};
 
 struct TileGeo {
   int x0, y0, sx, sy;   // origin and size of the tile
   int ex0, ey0;         // global index of LDS column / row 0 (origin - halo)
 };
 
-__device__ __forceinline__ TileGeo tile_geo(const Tiling& T) {
+__device__ __forceinline__ TileGeo tile_geo(const Tiling& T, int tile) {
+  const int bx = tile % T.nbx, by = tile / T.nbx;
   TileGeo t;
-  t.x0 = T.org[blockIdx.x];
-  t.sx = T.size[blockIdx.x];
-  t.y0 = T.org[T.nbx + blockIdx.y];
-  t.sy = T.size[T.nbx + blockIdx.y];
+  t.x0 = T.org[bx];
+  t.sx = T.size[bx];
+  t.y0 = T.org[T.nbx + by];
+  t.sy = T.size[T.nbx + by];
   t.ex0 = t.x0 - HX;
   t.ey0 = t.y0 - HY;
   return t;
@@ -66,9 +74,11 @@ __device__ __forceinline__ int ext_index(const Dir& dd, int gi) {
 
 // Stages the operand of this tile (all four components) into sq.  For the forward
 // stage the contravariant fluxes of the tile rows (x) and tile columns (y) go to sfx/sfy.
-template <bool FLUX>
+template <class S, bool FLUX>
 __device__ __forceinline__ void stage_operand(const Geo& g, const TileGeo& t, const double* xi,
                                               double* sq, double* sfx, double* sfy) {
+  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NT = S::NT;
+  constexpr int kStageIters = S::kStageIters;
   const Dir dx = g.dir[0], dy = g.dir[1];
   const int wx = t.sx + 2 * HX, wy = t.sy + 2 * HY;
   // all loads of this thread are issued before any is consumed
@@ -186,8 +196,10 @@ __device__ __forceinline__ double d4_at(const Dir& dd, int i, double c0, double 
 
 // t = -arc D4 x on the tile rows (x direction, columns x0-2 .. x0+sx+1) and tile columns
 // (y direction, rows y0-2 .. y0+sy+1), for staged points inside the domain.
+template <class S>
 __device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& t, const double* sq,
                                                    double* stx, double* sty) {
+  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NT = S::NT;
   const Op& od = g.ops[SLOT_DISS];
   const double c0 = od.c[kMaxHW], c1 = od.c[kMaxHW + 1], c2 = od.c[kMaxHW + 2];
   const Dir dx = g.dir[0], dy = g.dir[1];
@@ -221,9 +233,11 @@ __device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& 
 
 // Adds amount * (dissipation in x then in y) at one point: H^-1 D4^T t (closure rows of
 // D4^T near non-periodic boundaries) or the composite operator on the staged operand.
+template <class S>
 __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, const double* sq,
                                                 const double* stx, const double* sty, int tx,
                                                 int ty, int gi, int gj, double amount, double* r) {
+  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY;
   const Op& od = g.ops[SLOT_DISS];
   const Op& ot = g.ops[SLOT_DISS_T];
 #pragma unroll
@@ -290,6 +304,7 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
 }
 
 // Block sum: wave-level shuffles, then one LDS exchange of the per-wave sums.
+template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   __syncthreads();   // retire earlier readers of the scratch space
@@ -307,13 +322,17 @@ __device__ __forceinline__ bool in_patch(const DPatch& pt, int i, int j) {
 }
 
 // -------------------------------------------------------------------------- forward
-__global__ void __launch_bounds__(NT, 4) k_fwd_stage2d(Geo g, const Patches* __restrict__ P,
-                                                    Tiling T, FwdStage a) {
-  __shared__ double sq[SQ];
-  __shared__ double sc[SSCR];
-  const TileGeo t = tile_geo(T);
-  const int tile = blockIdx.y * T.nbx + blockIdx.x;
-  const long inst = blockIdx.z;
+template <class S>
+__global__ void __launch_bounds__(S::NT, 4) k_fwd_stage2d(Geo g, const Patches* __restrict__ P,
+                                                       Tiling T, FwdStage a) {
+  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NT = S::NT;
+  constexpr int SFX = S::SFX, STX = S::STX;
+  __shared__ double sq[S::SQ];
+  __shared__ double sc[S::SSCR];
+  const int tile = T.tile_of_block[blockIdx.x];
+  if (tile < 0) return;   // block beyond the tiles of its XCD (uniform per block)
+  const TileGeo t = tile_geo(T, tile);
+  const long inst = blockIdx.y;
   const long NVN = (long)NV2 * g.N;
   const double* qi = a.qin + inst * NVN;
   double* sfx = sc;
@@ -331,7 +350,7 @@ __global__ void __launch_bounds__(NT, 4) k_fwd_stage2d(Geo g, const Patches* __r
       }
     }
   }
-  stage_operand<true>(g, t, qi, sq, sfx, sfy);
+  stage_operand<S, true>(g, t, qi, sq, sfx, sfy);
   __syncthreads();
   const Op& o1 = g.ops[SLOT_FIRST];
   double r[PPT][NV2];
@@ -354,18 +373,17 @@ __global__ void __launch_bounds__(NT, 4) k_fwd_stage2d(Geo g, const Patches* __r
   if (g.dissipation) {
     __syncthreads();   // fluxes consumed
     if (!g.composite) {
-      diss_intermediates(g, t, sq, stx, sty);
+      diss_intermediates<S>(g, t, sq, stx, sty);
       __syncthreads();
     }
   }
   const unsigned pmask = T.pmask[tile];
-  double jpart = 0.0;
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
     const int ty = threadIdx.x / TX + k * (NT / TX);
     if (tx >= t.sx || ty >= t.sy) continue;
     const int gi = t.x0 + tx, gj = t.y0 + ty;
-    if (g.dissipation) add_dissipation(g, t, sq, stx, sty, tx, ty, gi, gj, g.diss_amount, r[k]);
+    if (g.dissipation) add_dissipation<S>(g, t, sq, stx, sty, tx, ty, gi, gj, g.diss_amount, r[k]);
     const int jk[3] = {gi, gj, 0};
     const long p = gi + (long)g.n[0] * gj;
     const double jac = jacobian<2>(g, jk);
@@ -450,55 +468,49 @@ __global__ void __launch_bounds__(NT, 4) k_fwd_stage2d(Geo g, const Patches* __r
         const int kp = (gi - pt.lo[0]) + pt.ls[0] * (gj - pt.lo[1]);
         const Prim<2> s = prim<2>(qn, g.gamma);
         const double F = s.p - pt.aplus[kp];
-        jpart += 1.0 * gnorm<2>(g, jk) * (F * F * pt.strength[kp]);
+        a.jpartial[(inst * a.nslots + a.slot) * a.npts + kp] =
+            1.0 * gnorm<2>(g, jk) * (F * F * pt.strength[kp]);
       }
-    }
-  }
-  if (a.jpartial && a.ip_cost >= 0) {
-    const int rank = a.tile_rank[tile];
-    if (rank >= 0) {
-      const double s = block_sum(jpart, sc);
-      if (threadIdx.x == 0) a.jpartial[(inst * a.nstage_slots + a.stage_slot) * a.ntiles + rank] = s;
     }
   }
 }
 
 // -------------------------------------------------------------------------- adjoint
-__global__ void __launch_bounds__(NT) k_adj_stage2d(Geo g, const Patches* __restrict__ P,
-                                                    Tiling T, AdjStage a) {
-  __shared__ double sq[SQ];
-  __shared__ double sc[SSCR];
-  const TileGeo t = tile_geo(T);
-  const int tile = blockIdx.y * T.nbx + blockIdx.x;
-  const long inst = blockIdx.z;
+template <class S>
+__global__ void __launch_bounds__(S::NT, 4) k_adj_stage2d(Geo g, const Patches* __restrict__ P,
+                                                       Tiling T, AdjStage a) {
+  constexpr int TX = S::TX, EX = S::EX, EY = S::EY, NT = S::NT;
+  constexpr int STX = S::STX;
+  __shared__ double sq[S::SQ];
+  __shared__ double sc[S::SSCR];
+  const int tile = T.tile_of_block[blockIdx.x];
+  if (tile < 0) return;   // block beyond the tiles of its XCD (uniform per block)
+  const TileGeo t = tile_geo(T, tile);
+  const long inst = blockIdx.y;
   const long NVN = (long)NV2 * g.N;
   const double* li = a.lam + inst * NVN;
   const double* qi = a.q + inst * NVN;
-  double pq[NV2], pl0[NV2], pb2[NV2];
+  // forward stage state at this thread's point, fetched before the stencil work
+  double pq[NV2];
   {
     const int tx0 = threadIdx.x % TX, ty0 = threadIdx.x / TX;
     if (tx0 < t.sx && ty0 < t.sy) {
       const long o = (t.x0 + tx0) + (long)g.n[0] * (t.y0 + ty0);
 #pragma unroll
-      for (int l = 0; l < NV2; ++l) {
-        pq[l] = qi[l * g.N + o];
-        pl0[l] = (a.stage == 3 || a.stage == 2) ? a.l0[inst * NVN + l * g.N + o] : 0.0;
-        pb2[l] = a.stage <= 3 ? a.b2[inst * NVN + l * g.N + o] : 0.0;
-      }
+      for (int l = 0; l < NV2; ++l) pq[l] = qi[l * g.N + o];
     }
   }
-  stage_operand<false>(g, t, li, sq, nullptr, nullptr);
+  stage_operand<S, false>(g, t, li, sq, nullptr, nullptr);
   double* stx = sc;
   double* sty = sc + STX;
   if (g.dissipation && !g.composite) {
     __syncthreads();
-    diss_intermediates(g, t, sq, stx, sty);
+    diss_intermediates<S>(g, t, sq, stx, sty);
   }
   __syncthreads();
   const Op& oa = g.ops[SLOT_ADJ];
   const unsigned pmask = T.pmask[tile];
   const int tx = threadIdx.x % TX;
-  double spart = 0.0;
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
     const int ty = threadIdx.x / TX + k * (NT / TX);
@@ -527,7 +539,14 @@ __global__ void __launch_bounds__(NT) k_adj_stage2d(Geo g, const Patches* __rest
         r[i] = r[i] + acc;
       }
     }
-    if (g.dissipation) add_dissipation(g, t, sq, stx, sty, tx, ty, gi, gj, -g.diss_amount, r);
+    // RK operands, fetched here so that they are not live through the derivative work
+    double pl0[NV2], pb2[NV2];
+#pragma unroll
+    for (int l = 0; l < NV2; ++l) {
+      pl0[l] = (a.stage == 3 || a.stage == 2) ? a.l0[inst * NVN + l * g.N + p] : 0.0;
+      pb2[l] = a.stage <= 3 ? a.b2[inst * NVN + l * g.N + p] : 0.0;
+    }
+    if (g.dissipation) add_dissipation<S>(g, t, sq, stx, sty, tx, ty, gi, gj, -g.diss_amount, r);
     const double jac = jacobian<2>(g, jk);
     double lp[NV2];
 #pragma unroll
@@ -582,8 +601,10 @@ __global__ void __launch_bounds__(NT) k_adj_stage2d(Geo g, const Patches* __rest
         case MGD_PATCH_ACTUATOR: {
           // ThermalActuator gradient and sensitivity on the pre-update adjoint state
           if (a.grad) a.grad[inst * a.grad_stride + kp] = lp[3] * (pt.strength[kp] * 1.0);
-          const double F = lp[3] * pt.strength[kp] * 1.0;
-          spart += 1.0 * gnorm<2>(g, jk) * (F * F);
+          if (a.spartial) {
+            const double F = lp[3] * pt.strength[kp] * 1.0;
+            a.spartial[(inst * a.nslots + a.slot) * a.npts + kp] = 1.0 * gnorm<2>(g, jk) * (F * F);
+          }
           break;
         }
         case MGD_PATCH_KOLMOGOROV:
@@ -615,43 +636,66 @@ __global__ void __launch_bounds__(NT) k_adj_stage2d(Geo g, const Patches* __rest
       if (a.rhs_out) a.rhs_out[inst * NVN + o] = rl;
     }
   }
-  if (a.spartial && a.ip_act >= 0) {
-    const int rank = a.tile_rank[tile];
-    if (rank >= 0) {
-      const double s2 = block_sum(spart, sc);
-      if (threadIdx.x == 0) a.spartial[(inst * a.nstage_slots + a.stage_slot) * a.ntiles + rank] = s2;
-    }
-  }
 }
 
-// Per-stage sums of the tile partials: one block per (stage, instance).
-__global__ void k_stage_sums(const double* __restrict__ partial, int nstages, int ntiles,
-                             double* __restrict__ sums) {
-  __shared__ double red[NT];
+// Stage sums of the per-point quadrature terms of one chunk of stages: one block per
+// (stage, instance), a fixed-order tree over the points.
+__global__ void __launch_bounds__(kRedThreads) k_stage_sums(const double* __restrict__ partial,
+                                                            int nslots, int npts,
+                                                            double* __restrict__ sums,
+                                                            int nstages, int base) {
+  __shared__ double red[kRedThreads / 64];
   const int s = blockIdx.x;
   const long inst = blockIdx.y;
+  const double* src = partial + (inst * nslots + s) * (long)npts;
   double v = 0.0;
-  for (int t = threadIdx.x; t < ntiles; t += NT) v += partial[(inst * nstages + s) * ntiles + t];
-  const double sum = block_sum(v, red);
-  if (threadIdx.x == 0) sums[inst * nstages + s] = sum;
+  for (int k = threadIdx.x; k < npts; k += kRedThreads) v += src[k];
+  const double sum = block_sum<kRedThreads>(v, red);
+  if (threadIdx.x == 0) sums[inst * nstages + base + s] = sum;
 }
 
 // acc[inst] += w[s] * sums[s] over the stages in time-marching order (the running time
-// quadrature of Solver%runForward / runAdjoint); reverse = adjoint order.
-__global__ void k_stage_accumulate(const double* __restrict__ sums, int nstages,
-                                   const double* __restrict__ w, double* __restrict__ acc,
-                                   int ninst, int reverse) {
-  const int inst = blockIdx.x * blockDim.x + threadIdx.x;
-  if (inst >= ninst) return;
+// quadrature of Solver%runForward / runAdjoint); reverse = adjoint order.  One block
+// per instance: the products are formed in parallel into LDS, one chunk at a time, and
+// added by one thread in stage order (the order of the running sum).
+__global__ void __launch_bounds__(kRedThreads) k_stage_accumulate(
+    const double* __restrict__ sums, int nstages, const double* __restrict__ w,
+    double* __restrict__ acc, int reverse) {
+  constexpr int kChunk = 2048;
+  __shared__ double prod[kChunk];
+  const long inst = blockIdx.x;
   double total = acc[inst];
-  for (int s0 = 0; s0 < nstages; ++s0) {
-    const int s = reverse ? nstages - 1 - s0 : s0;
-    total = total + w[s] * sums[(long)inst * nstages + s];
+  for (int c0 = 0; c0 < nstages; c0 += kChunk) {
+    const int n = min(kChunk, nstages - c0);
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += kRedThreads) {
+      const int s = reverse ? nstages - 1 - (c0 + k) : c0 + k;
+      prod[k] = w[s] * sums[inst * nstages + s];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll 8
+      for (int k = 0; k < n; ++k) total = total + prod[k];
+    }
   }
-  acc[inst] = total;
+  if (threadIdx.x == 0) acc[inst] = total;
 }
 
 // ------------------------------------------------------------------------ host side
+using S32x16 = Shape<32, 16>;
+using S16x32 = Shape<16, 32>;
+using S16x16 = Shape<16, 16>;
+using S24x16 = Shape<24, 16>;
+
+static const int kShapeDims[kNumTileShapes][2] = {{32, 16}, {16, 32}, {16, 16}, {24, 16}};
+
+int fused2d_shape_dims(int shape, int* tx, int* ty) {
+  if (shape < 0 || shape >= kNumTileShapes) return MGD_ERR_ARG;
+  *tx = kShapeDims[shape][0];
+  *ty = kShapeDims[shape][1];
+  return MGD_OK;
+}
+
 static void balanced(int len, int T, std::vector<int>& org, std::vector<int>& size) {
   const int n = (len + T - 1) / T;
   const int base = len / n, extra = len % n;
@@ -664,12 +708,15 @@ static void balanced(int len, int T, std::vector<int>& org, std::vector<int>& si
   }
 }
 
-int fused2d_make_tiling(const Geo& g, const Patches& hp, HostTiling* out) {
+int fused2d_make_tiling(const Geo& g, const Patches& hp, int shape, HostTiling* out) {
+  int tx, ty;
+  if (fused2d_shape_dims(shape, &tx, &ty) != MGD_OK) return MGD_ERR_ARG;
+  out->shape = shape;
   out->org.clear();
   out->size.clear();
-  balanced(g.n[0], TX, out->org, out->size);
+  balanced(g.n[0], tx, out->org, out->size);
   out->nbx = (int)out->org.size();
-  balanced(g.n[1], TY, out->org, out->size);
+  balanced(g.n[1], ty, out->org, out->size);
   out->nby = (int)out->org.size() - out->nbx;
   for (int d = 0; d < 2; ++d) {
     if (g.dir[d].periodic) continue;
@@ -690,38 +737,65 @@ int fused2d_make_tiling(const Geo& g, const Patches& hp, HostTiling* out) {
       }
       out->pmask[(size_t)by * out->nbx + bx] = m;
     }
+  // XCD-aware block -> tile map.  Work-groups are dealt to the XCDs round-robin by
+  // linear block index, and each XCD has its own L2: XCD x receives blocks x, x + 8, ...
+  // which are mapped to one contiguous band of tiles (row-major), so neighbouring tiles
+  // -- whose halos overlap -- share an L2.  Bands hold ceil(ntiles / 8) tiles; the spare
+  // blocks of the last band map to -1 and exit at once.
+  const int ntiles = out->nbx * out->nby;
+  const int band = (ntiles + kNumXcd - 1) / kNumXcd;
+  out->nblk = band * kNumXcd;
+  out->tile_of_block.assign((size_t)out->nblk, -1);
+  for (int b = 0; b < out->nblk; ++b) {
+    const int pos = (b % kNumXcd) * band + b / kNumXcd;
+    out->tile_of_block[(size_t)b] = pos < ntiles ? pos : -1;
+  }
   return MGD_OK;
 }
 
-int fused2d_tile_ranks(const HostTiling& T, const int* lo, const int* hi, std::vector<int>& rank) {
-  rank.assign((size_t)T.nbx * T.nby, -1);
-  int c = 0;
-  for (int by = 0; by < T.nby; ++by)
-    for (int bx = 0; bx < T.nbx; ++bx) {
-      const int xa = T.org[bx], xb = xa + T.size[bx] - 1;
-      const int ya = T.org[T.nbx + by], yb = ya + T.size[T.nbx + by] - 1;
-      if (xb < lo[0] || xa > hi[0] || yb < lo[1] || ya > hi[1]) continue;
-      rank[(size_t)by * T.nbx + bx] = c++;
-    }
-  return c;
+template <class S>
+static void fwd_launch(const Geo& g, const Patches* P, const Tiling& T, const FwdStage& a,
+                       hipStream_t st) {
+  k_fwd_stage2d<S><<<dim3(T.nblk, g.ninst), S::NT, 0, st>>>(g, P, T, a);
+}
+
+template <class S>
+static void adj_launch(const Geo& g, const Patches* P, const Tiling& T, const AdjStage& a,
+                       hipStream_t st) {
+  k_adj_stage2d<S><<<dim3(T.nblk, g.ninst), S::NT, 0, st>>>(g, P, T, a);
 }
 
 void launch_fwd_stage2d(const Geo& g, const Patches* P, const Tiling& T, const FwdStage& a,
                         hipStream_t st) {
-  k_fwd_stage2d<<<dim3(T.nbx, T.nby, g.ninst), NT, 0, st>>>(g, P, T, a);
+  switch (T.shape) {
+    case 0: fwd_launch<S32x16>(g, P, T, a, st); break;
+    case 1: fwd_launch<S16x32>(g, P, T, a, st); break;
+    case 2: fwd_launch<S16x16>(g, P, T, a, st); break;
+    default: fwd_launch<S24x16>(g, P, T, a, st); break;
+  }
 }
 
 void launch_adj_stage2d(const Geo& g, const Patches* P, const Tiling& T, const AdjStage& a,
                         hipStream_t st) {
-  k_adj_stage2d<<<dim3(T.nbx, T.nby, g.ninst), NT, 0, st>>>(g, P, T, a);
+  switch (T.shape) {
+    case 0: adj_launch<S32x16>(g, P, T, a, st); break;
+    case 1: adj_launch<S16x32>(g, P, T, a, st); break;
+    case 2: adj_launch<S16x16>(g, P, T, a, st); break;
+    default: adj_launch<S24x16>(g, P, T, a, st); break;
+  }
 }
 
-void launch_stage_quadrature(const double* partial, int nstages, int ntiles, const double* w,
-                             double* stage_sums, double* acc, int ninst, int reverse,
-                             hipStream_t st) {
+void launch_stage_reduce(const double* partial, int nslots, int nfill, int npts, double* sums,
+                         int nstages, int base, int ninst, hipStream_t st) {
+  if (nfill <= 0 || npts <= 0) return;
+  k_stage_sums<<<dim3(nfill, ninst), kRedThreads, 0, st>>>(partial, nslots, npts, sums, nstages,
+                                                           base);
+}
+
+void launch_stage_accumulate(const double* sums, int nstages, const double* w, double* acc,
+                             int ninst, int reverse, hipStream_t st) {
   if (nstages <= 0) return;
-  k_stage_sums<<<dim3(nstages, ninst), NT, 0, st>>>(partial, nstages, ntiles, stage_sums);
-  k_stage_accumulate<<<(ninst + 63) / 64, 64, 0, st>>>(stage_sums, nstages, w, acc, ninst, reverse);
+  k_stage_accumulate<<<ninst, kRedThreads, 0, st>>>(sums, nstages, w, acc, reverse);
 }
 
 }  // namespace mgd

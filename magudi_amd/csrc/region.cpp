@@ -79,13 +79,16 @@ struct mgd_region {
   mgd::DevBuf q, lam, rhs, b1, b2, b1a, b2a, t, w, G, T1, S;
   mgd::DevBuf qs, aforce, control, grad, acc_J, acc_S, partial, src_amp, init;
   mgd::DevBuf snaps, window;
-  // fused 2-D path: ping-pong stage buffers, per-tile partials, quadrature weights
-  mgd::DevBuf qy, ly, l0, jpart, spart, ssum, wts, rank_cost, rank_act;
-  int ntiles_cost = 0, ntiles_act = 0;
+  // fused 2-D path: ping-pong stage buffers, per-point quadrature terms of one chunk of
+  // stages, stage sums, quadrature weights
+  mgd::DevBuf qy, ly, l0, jpart, spart, ssum, wts;
+  int chunk = 0;                         // stages per chunk of quadrature terms
   mgd::HostTiling htile;
   mgd::Tiling tiling{};
-  mgd::DevBuf d_tile_org, d_tile_size, d_tile_mask;
+  mgd::DevBuf d_tile_org, d_tile_size, d_tile_mask, d_tile_blk;
   bool tiling_ok = false;
+  int tile_shape = -1;                   // MGD_OPT_TILE_SHAPE: -1 autotune, else fixed
+  bool tuned = false;
   int force_plain = 0;
   int aforce_zero = 0;
   long ctrl_stride = 0;                  // control entries per instance
@@ -402,44 +405,68 @@ int upload_ints(DevBuf& b, const void* data, size_t bytes) {
   return MGD_OK;
 }
 
-int setup_fused(mgd_region* r) {
-  const size_t M = (size_t)r->ninst * r->nv * r->N;
-  r->tiling_ok = false;
-  if (fused2d_make_tiling(r->geo, r->hpatches, &r->htile) != MGD_OK) return MGD_OK;  // plain path
-  const HostTiling& H = r->htile;
+// Installs the tiling of one tile shape (host and device views).
+int install_tiling(mgd_region* r, int shape) {
+  HostTiling H;
+  if (fused2d_make_tiling(r->geo, r->hpatches, shape, &H) != MGD_OK) return MGD_ERR_UNSUPPORTED;
   if (upload_ints(r->d_tile_org, H.org.data(), H.org.size() * sizeof(int)) ||
       upload_ints(r->d_tile_size, H.size.data(), H.size.size() * sizeof(int)) ||
-      upload_ints(r->d_tile_mask, H.pmask.data(), H.pmask.size() * sizeof(unsigned)))
+      upload_ints(r->d_tile_mask, H.pmask.data(), H.pmask.size() * sizeof(unsigned)) ||
+      upload_ints(r->d_tile_blk, H.tile_of_block.data(), H.tile_of_block.size() * sizeof(int)))
     return MGD_ERR_HIP;
+  r->htile = H;
+  r->tiling.shape = H.shape;
   r->tiling.nbx = H.nbx;
   r->tiling.nby = H.nby;
+  r->tiling.nblk = H.nblk;
   r->tiling.org = reinterpret_cast<const int*>(r->d_tile_org.p);
   r->tiling.size = reinterpret_cast<const int*>(r->d_tile_size.p);
   r->tiling.pmask = reinterpret_cast<const unsigned*>(r->d_tile_mask.p);
+  r->tiling.tile_of_block = reinterpret_cast<const int*>(r->d_tile_blk.p);
+  return MGD_OK;
+}
+
+int setup_fused(mgd_region* r) {
+  const size_t M = (size_t)r->ninst * r->nv * r->N;
+  r->tiling_ok = false;
+  r->tuned = false;
+  // the first shape that fits (32 x 16 unless the grid is too small for it); the autotuner
+  // may replace it before the first run
+  int rc = MGD_ERR_UNSUPPORTED;
+  if (r->tile_shape >= 0) {
+    rc = install_tiling(r, r->tile_shape);
+  } else {
+    for (int sh = 0; sh < kNumTileShapes && rc == MGD_ERR_UNSUPPORTED; ++sh) rc = install_tiling(r, sh);
+  }
+  if (rc == MGD_ERR_UNSUPPORTED) return MGD_OK;   // plain path
+  if (rc != MGD_OK) return rc;
   if (r->qy.alloc(M)) return MGD_ERR_HIP;
-  std::vector<int> rank;
-  auto up_rank = [&](DevBuf& b, int ip, int* count) -> int {
-    if (ip < 0) { *count = 0; return MGD_OK; }
-    const HostPatch& P = *r->patches[ip];
-    *count = fused2d_tile_ranks(H, P.lo, P.hi, rank);
-    if (b.alloc(rank.size() / 2 + 1)) return MGD_ERR_HIP;
-    MGD_HIP(hipMemcpy(b.p, rank.data(), rank.size() * sizeof(int), hipMemcpyHostToDevice));
-    return MGD_OK;
-  };
-  if (int rc = up_rank(r->rank_cost, r->ip_cost, &r->ntiles_cost)) return rc;
-  if (int rc = up_rank(r->rank_act, r->ip_act, &r->ntiles_act)) return rc;
   r->tiling_ok = true;
   return MGD_OK;
 }
+
+constexpr int kChunkStages = 128;   // stages of quadrature terms held before their reduction
 
 int prepare_quadrature(mgd_region* r, int nst, double dt) {
   std::vector<double> w((size_t)nst);
   for (int s = 0; s < nst; ++s) w[s] = kRkNorm[s % 4] * dt;
   if (upload(r->wts, w)) return MGD_ERR_HIP;
-  if (r->ssum.alloc((size_t)r->ninst * nst)) return MGD_ERR_HIP;
-  if (r->ntiles_cost > 0 && r->jpart.alloc((size_t)r->ninst * nst * r->ntiles_cost)) return MGD_ERR_HIP;
-  if (r->ntiles_act > 0 && r->spart.alloc((size_t)r->ninst * nst * r->ntiles_act)) return MGD_ERR_HIP;
+  if (r->ssum.alloc((size_t)r->ninst * std::max(nst, 1))) return MGD_ERR_HIP;
+  r->chunk = std::max(1, std::min(nst, kChunkStages));
+  const int nc = n_cost(r), na = n_act(r);
+  if (nc > 0 && r->jpart.alloc((size_t)r->ninst * r->chunk * nc)) return MGD_ERR_HIP;
+  if (na > 0 && r->spart.alloc((size_t)r->ninst * r->chunk * na)) return MGD_ERR_HIP;
   return MGD_OK;
+}
+
+// Stage sums of the chunk holding stage sidx once that chunk is complete: forward order
+// completes a chunk at its last stage, adjoint order at its first.
+void reduce_chunk(mgd_region* r, const DevBuf& part, int npts, int nst, int sidx, bool reverse) {
+  if (npts <= 0) return;
+  const int base = (sidx / r->chunk) * r->chunk;
+  const int nfill = std::min(r->chunk, nst - base);
+  const bool done = reverse ? sidx == base : sidx == base + nfill - 1;
+  if (done) launch_stage_reduce(part.p, r->chunk, nfill, npts, r->ssum.p, nst, base, r->ninst, r->stream);
 }
 
 FwdStage fwd_args(mgd_region* r, const double* qin, const double* q0, double* b2, double* qout,
@@ -457,12 +484,12 @@ FwdStage fwd_args(mgd_region* r, const double* qin, const double* q0, double* b2
   a.nsrc = ns;
   a.src_amp = r->src_amp.p + (long)sidx * std::max(ns, 1);
   a.ip_cost = r->ip_cost;
-  if (with_j && r->ntiles_cost > 0) {
+  (void)nst;
+  if (with_j && n_cost(r) > 0) {
     a.jpartial = r->jpart.p;
-    a.tile_rank = reinterpret_cast<const int*>(r->rank_cost.p);
-    a.ntiles = r->ntiles_cost;
-    a.nstage_slots = nst;
-    a.stage_slot = sidx;
+    a.npts = n_cost(r);
+    a.nslots = r->chunk;
+    a.slot = sidx % r->chunk;
   }
   return a;
 }
@@ -477,7 +504,62 @@ void fused_forward_step(mgd_region* r, double* q0, double* x, double* y, double*
     const double* in = outs ? (i == 1 ? q0 : outs[i - 2]) : qin[i - 1];
     FwdStage a = fwd_args(r, in, q0, b2, o, i, dt, 4 * s + i - 1, nst, with_j);
     launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+    if (with_j) reduce_chunk(r, r->jpart, n_cost(r), nst, 4 * s + i - 1, false);
   }
+}
+
+// Autotuner: times one forward and one adjoint RK4 step (8 stage launches) with each
+// tile shape that fits the grid and keeps the fastest.  J, the sensitivity and the
+// gradient do not depend on the choice (per-point quadrature terms, fixed-order sums),
+// nor does any rhs value.  Uses scratch buffers only (qs, qy, ly, l0, b2); the conserved
+// and adjoint states are not touched.
+int tune_tiling(mgd_region* r) {
+  r->tuned = true;
+  const long M = (long)r->ninst * M_of(r);
+  if (r->qs.alloc(M) || r->qy.alloc(M) || r->ly.alloc(M) || r->l0.alloc(M)) return MGD_ERR_HIP;
+  std::vector<double> amp;
+  source_table(r, std::vector<double>{0.0, 0.5e-3, 0.5e-3, 1e-3}, amp);
+  if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
+  if (r->control.p) r->control.alloc(0);
+  const double dt = 1e-3;
+  auto step_pair = [&]() {
+    fused_forward_step(r, r->qs.p, r->qy.p, r->ly.p, r->b2.p, 0, dt, 1, false, nullptr);
+    for (int i = 4; i >= 1; --i) {
+      AdjStage a;
+      std::memset(&a, 0, sizeof a);
+      a.lam = i == 4 ? r->l0.p : (i == 3 ? r->ly.p : (i == 2 ? r->qy.p : r->ly.p));
+      a.lout = i == 4 ? r->ly.p : (i == 3 ? r->qy.p : (i == 2 ? r->ly.p : r->l0.p));
+      a.l0 = r->l0.p; a.q = r->qs.p; a.b2 = r->b2.p;
+      a.stage = i; a.dt = dt; a.forcing_factor = 1.0; a.ip_act = r->ip_act;
+      launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+    }
+  };
+  int best = -1;
+  float best_ms = 0.f;
+  for (int sh = 0; sh < kNumTileShapes; ++sh) {
+    const int rc = install_tiling(r, sh);
+    if (rc == MGD_ERR_UNSUPPORTED) continue;
+    if (rc != MGD_OK) return rc;
+    launch_copy(M, r->q.p, r->qs.p, r->stream);
+    launch_copy(M, r->q.p, r->l0.p, r->stream);
+    step_pair();   // warm-up
+    MGD_HIP(hipEventRecord(r->ev0, r->stream));
+    for (int k = 0; k < 2; ++k) step_pair();
+    MGD_HIP(hipEventRecord(r->ev1, r->stream));
+    MGD_HIP(hipGetLastError());
+    MGD_HIP(hipEventSynchronize(r->ev1));
+    float ms = 0.f;
+    MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+    if (best < 0 || ms < best_ms) { best = sh; best_ms = ms; }
+  }
+  if (best < 0) { set_error("no tile shape fits the grid"); return MGD_ERR_UNSUPPORTED; }
+  return install_tiling(r, best);
+}
+
+// Runs the autotuner once per region unless a tile shape was fixed by option.
+int maybe_tune(mgd_region* r) {
+  if (!fused_ok(r) || r->tuned || r->tile_shape >= 0) return MGD_OK;
+  return tune_tiling(r);
 }
 
 int run_adjoint_fused(mgd_region* r, double dt, int nsteps, int K, double* gradient,
@@ -492,9 +574,7 @@ int run_adjoint_fused(mgd_region* r, double dt, int nsteps, int K, double* gradi
     fused_forward_step(r, r->q.p, r->b1.p, r->qy.p, r->b2.p, s, dt, nst, true, nullptr);
     if ((s + 1) % K == 0) launch_copy(M, r->q.p, r->snaps.p + (long)((s + 1) / K) * M, r->stream);
   }
-  if (r->ntiles_cost > 0)
-    launch_stage_quadrature(r->jpart.p, nst, r->ntiles_cost, r->wts.p, r->ssum.p, r->acc_J.p,
-                            r->ninst, 0, r->stream);
+  if (nc > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, r->stream);
   // terminal condition on the final state
   double* L0 = r->lam.p;
   double* LX = r->ly.p;
@@ -549,20 +629,16 @@ int run_adjoint_fused(mgd_region* r, double dt, int nsteps, int K, double* gradi
           a.grad = r->grad.p + (long)sidx * na;
           a.grad_stride = (long)nst * na;
         }
-        if (r->ntiles_act > 0) {
-          a.spartial = r->spart.p;
-          a.tile_rank = reinterpret_cast<const int*>(r->rank_act.p);
-          a.ntiles = r->ntiles_act;
-          a.nstage_slots = nst;
-          a.stage_slot = sidx;
-        }
+        a.spartial = r->spart.p;
+        a.npts = na;
+        a.nslots = r->chunk;
+        a.slot = sidx % r->chunk;
       }
       launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+      reduce_chunk(r, r->spart, na, nst, sidx, true);
     }
   }
-  if (na > 0 && r->ntiles_act > 0)
-    launch_stage_quadrature(r->spart.p, nst, r->ntiles_act, r->wts.p, r->ssum.p, r->acc_S.p, r->ninst,
-                            1, r->stream);
+  if (na > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_S.p, r->ninst, 1, r->stream);
   MGD_HIP(hipEventRecord(r->ev1, r->stream));
   MGD_HIP(hipGetLastError());
   MGD_HIP(hipStreamSynchronize(r->stream));
@@ -897,10 +973,29 @@ int mgd_region_finalize(mgd_region* r) {
   return MGD_OK;
 }
 
+int mgd_region_get_tile_shape(mgd_region* r, int* shape) {
+  if (!r || !shape) { set_error("null argument"); return MGD_ERR_ARG; }
+  *shape = fused_ok(r) ? r->tiling.shape : -1;
+  return MGD_OK;
+}
+
 int mgd_region_set_option(mgd_region* r, int option, int value) {
   if (!r) { set_error("null region"); return MGD_ERR_ARG; }
   switch (option) {
     case MGD_OPT_FORCE_PLAIN: r->force_plain = value != 0; return MGD_OK;
+    case MGD_OPT_TILE_SHAPE: {
+      if (value < -1 || value >= kNumTileShapes) { set_error("tile shape must be -1 .. 3"); return MGD_ERR_ARG; }
+      r->tile_shape = value;
+      r->tuned = false;
+      if (r->finalized && r->nd == 2) {
+        if (value >= 0 && r->tiling_ok) {
+          const int rc = install_tiling(r, value);
+          if (rc == MGD_ERR_UNSUPPORTED) { set_error("tile shape does not fit the grid"); return rc; }
+          return rc;
+        }
+      }
+      return MGD_OK;
+    }
   }
   set_error("unknown option");
   return MGD_ERR_ARG;
@@ -1028,6 +1123,7 @@ int mgd_run_forward(mgd_region* r, double t0, double dt, int nsteps, const doubl
   if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
   if (nsteps < 0 || !(dt > 0.0)) { set_error("bad time stepping"); return MGD_ERR_ARG; }
   (void)use_graph;
+  if (int rc = maybe_tune(r)) return rc;
   const int na = n_act(r), nst = 4 * nsteps;
   if (control && na > 0) {
     std::vector<double> c(control, control + (size_t)r->ninst * nst * na);
@@ -1046,9 +1142,8 @@ int mgd_run_forward(mgd_region* r, double t0, double dt, int nsteps, const doubl
     MGD_HIP(hipEventRecord(r->ev0, r->stream));
     for (int s = 0; s < nsteps; ++s)
       fused_forward_step(r, r->q.p, r->b1.p, r->qy.p, r->b2.p, s, dt, nst, true, nullptr);
-    if (r->ntiles_cost > 0)
-      launch_stage_quadrature(r->jpart.p, nst, r->ntiles_cost, r->wts.p, r->ssum.p, r->acc_J.p,
-                              r->ninst, 0, r->stream);
+    if (n_cost(r) > 0)
+      launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, r->stream);
     MGD_HIP(hipEventRecord(r->ev1, r->stream));
     MGD_HIP(hipGetLastError());
     MGD_HIP(hipStreamSynchronize(r->stream));
@@ -1085,6 +1180,7 @@ int mgd_run_adjoint(mgd_region* r, double t0, double dt, int nsteps, int save_in
   if (nsteps <= 0 || save_interval <= 0 || !(dt > 0.0)) { set_error("bad time stepping"); return MGD_ERR_ARG; }
   (void)use_graph;
   if (int rc = ensure_adjoint(r)) return rc;
+  if (int rc = maybe_tune(r)) return rc;
   const int K = save_interval;
   const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
   const long Mi = M_of(r);
@@ -1242,6 +1338,7 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches) {
 
 int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, double* bytes) {
   if (!r || !r->finalized || reps <= 0) { set_error("bad argument"); return MGD_ERR_ARG; }
+  if (int rc = maybe_tune(r)) return rc;
   const long Mi = M_of(r), M = (long)r->ninst * Mi;
   const double pts = (double)r->ninst * r->N;
   const double w = 8.0 * r->nv;   // bytes of one state point

@@ -76,7 +76,8 @@ class Region:
     Region%setup, Grid%update and the patch / functional / controller set-up of
     Solver%setup (sponge strengths, far-field penalties, mean pressure)."""
 
-    def __init__(self, case: Case, n_instances: int = 1, force_plain: bool = False):
+    def __init__(self, case: Case, n_instances: int = 1, force_plain: bool = False,
+                 tile_shape: int = -1):
         self.case = case
         self.nd = case.nd
         self.nv = case.nv
@@ -105,6 +106,8 @@ class Region:
         self.handle = h
         if force_plain:
             check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_FORCE_PLAIN, 1))
+        if tile_shape != -1:
+            check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_TILE_SHAPE, tile_shape))
         for p in case.patches:
             pd = capi.PatchDesc()
             pd.type = p.type
@@ -175,6 +178,13 @@ class Region:
         ncomp = a.size // self.npoints
         check(capi.lib().mgd_stencil_apply(self.handle, op, direction, dptr(a), ncomp))
         return a
+
+    @property
+    def tile_shape(self) -> int:
+        """Tile shape the fused 2-D kernels use (-1: generic kernels)."""
+        v = C.c_int(0)
+        check(capi.lib().mgd_region_get_tile_shape(self.handle, C.byref(v)))
+        return v.value
 
     def time_kernel(self, which: int, reps: int = 20):
         """(ms per launch, algorithmic bytes per launch) of one kernel, HIP-event timed."""
