@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-extras", action="store_true",
                     help="headline only (no batched / plain / CPU legs)")
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--c4-n", type=int, default=256,
+                    help="grid of the 3-D viscous Kolmogorov rhs leg (0: skip)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the HIP-event timing of the stage kernels and the copy kernel "
                          "(the workload of the rocprofv3 counter runs)")
@@ -92,6 +94,26 @@ def cpu_baseline(n, nsteps, save_interval):
             "kind": "port",
             "sample": f"{n}x{n} monopole forward+adjoint, {nsteps} RK4 steps, snapshots every "
                       f"{save_interval} (oracle/ C restatement, 1 thread, {dt:.1f} s)"}
+
+
+def viscous3d_leg(n):
+    """One forward and one adjoint rhs evaluation of the 3-D viscous periodic Kolmogorov
+    flow at n^3 (generic per-operator kernels), HIP-event timed."""
+    from magudi_amd import capi, cases
+    from magudi_amd.solver import Region
+    c = cases.kolmogorov3d(n=n, nsteps=1)
+    R = Region(c)
+    R.states[0].conservedVariables = c.initial_state()
+    f_ms, f_b = R.time_kernel(capi.KERNEL_RHS_FORWARD, reps=5)
+    a_ms, a_b = R.time_kernel(capi.KERNEL_RHS_ADJOINT, reps=5)
+    del R
+    leg = {"workload": f"Kolmogorov {n}^3 viscous periodic rhs (Re_ac 500, Pr 0.7)",
+           "path": "generic per-operator kernels"}
+    for k, ms, b in (("forward_rhs", f_ms, f_b), ("adjoint_rhs", a_ms, a_b)):
+        gbs = b / (ms * 1e-3) / 1e9
+        leg[k] = {"ms": round(ms, 4), "algorithmic_bytes": b, "achieved_GBs": round(gbs, 1),
+                  "frac": round(gbs / HBM_PEAK_GBS, 4), "pt_per_s": n ** 3 / (ms * 1e-3)}
+    return leg
 
 
 def main():
@@ -202,6 +224,8 @@ def main():
         out["plain"] = {"value": case.npoints * ns_plain / tp, "nsteps": ns_plain,
                         "ms_per_step": tp * 1e3,
                         "batched_speedup_vs_plain": (a.batch * pts / tb) / (case.npoints * ns_plain / tp)}
+        if a.c4_n > 0:
+            out["viscous3d"] = viscous3d_leg(a.c4_n)
         out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_steps, max(1, a.cpu_steps // 2))
     else:
         out["cpu_baseline"] = None

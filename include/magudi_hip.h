@@ -192,12 +192,16 @@ int mgd_region_get_tile_shape(mgd_region* r, int* shape);
  * which: MGD_KERNEL_FWD_STAGE (fused 2-D forward stage kernel, `reps` RK4 steps of 4
  * stage launches), MGD_KERNEL_ADJ_STAGE (fused 2-D adjoint stage kernel, `reps` adjoint
  * RK4 steps of 4 launches), MGD_KERNEL_COPY (`reps` state-sized copies, the bandwidth
- * reference).  ms_per_launch: average over the launches; bytes: algorithmic HBM bytes
- * of one launch, averaged over the 4 stages.  Benchmark support; it overwrites the
- * region's RK and adjoint buffers (the conserved state is kept). */
+ * reference), MGD_KERNEL_RHS_FORWARD / MGD_KERNEL_RHS_ADJOINT (`reps` evaluations of the
+ * rhs by the generic per-operator kernels, any region).  ms_per_launch: average per
+ * launch (per rhs evaluation for the last two); bytes: algorithmic HBM bytes of one
+ * launch (evaluation), averaged over the 4 stages.  Benchmark support; it overwrites the
+ * region's RK, adjoint and rhs buffers (the conserved state is kept). */
 #define MGD_KERNEL_FWD_STAGE 0
 #define MGD_KERNEL_ADJ_STAGE 1
 #define MGD_KERNEL_COPY 2
+#define MGD_KERNEL_RHS_FORWARD 3
+#define MGD_KERNEL_RHS_ADJOINT 4
 int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, double* bytes);
 
 /* Waits for all work of the region. */

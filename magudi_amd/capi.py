@@ -20,7 +20,7 @@ CONSERVED, ADJOINT_VARS, RHS, INITIAL_STATE = 0, 1, 2, 4
 FIELD_TARGET_STATE, FIELD_CONTROL_MOLLIFIER, FIELD_TARGET_MOLLIFIER = 0, 1, 2
 OP_FIRST, OP_ADJOINT_FIRST, OP_DISS, OP_DISS_T = 0, 1, 2, 3
 OPT_FORCE_PLAIN, OPT_TILE_SHAPE = 1, 2
-KERNEL_FWD_STAGE, KERNEL_ADJ_STAGE, KERNEL_COPY = 0, 1, 2
+KERNEL_FWD_STAGE, KERNEL_ADJ_STAGE, KERNEL_COPY, KERNEL_RHS_FORWARD, KERNEL_RHS_ADJOINT = 0, 1, 2, 3, 4
 
 
 class RegionDesc(C.Structure):

@@ -1376,6 +1376,15 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
       case MGD_KERNEL_COPY:
         launch_copy(M, r->q.p, r->qs.p, r->stream);
         break;
+      case MGD_KERNEL_RHS_FORWARD:
+        launch_rhs_forward(r->geo, dpatches(r), r->q.p, r->rhs.p, r->w.p, r->G.p, r->t.p, nullptr,
+                           n_act(r), reinterpret_cast<const Source*>(r->d_sources.p),
+                           (int)r->sources.size(), r->src_amp.p, r->stream);
+        break;
+      case MGD_KERNEL_RHS_ADJOINT:
+        launch_rhs_adjoint(r->geo, dpatches(r), r->q.p, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
+                           r->T1.p, r->S.p, nullptr, 0, 1.0, r->has_farfield, r->stream);
+        break;
     }
   };
   if (which == MGD_KERNEL_FWD_STAGE || which == MGD_KERNEL_ADJ_STAGE) {
@@ -1388,6 +1397,11 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
     launch_copy(M, r->q.p, r->ly.p, r->stream);
     launch_copy(M, r->q.p, r->l0.p, r->stream);
     launch_copy(M, r->q.p, r->lam.p, r->stream);
+  } else if (which == MGD_KERNEL_RHS_FORWARD || which == MGD_KERNEL_RHS_ADJOINT) {
+    if (which == MGD_KERNEL_RHS_ADJOINT) {
+      if (int rc = ensure_adjoint(r)) return rc;
+      launch_copy(M, r->q.p, r->lam.p, r->stream);
+    }
   } else if (which != MGD_KERNEL_COPY) {
     set_error("unknown kernel");
     return MGD_ERR_ARG;
@@ -1408,6 +1422,8 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
       // forward stage state, buffer2 in; buffer2, next adjoint out)
       case MGD_KERNEL_FWD_STAGE: *bytes = 16.0 / 4.0 * w * pts; break;
       case MGD_KERNEL_ADJ_STAGE: *bytes = 20.0 / 4.0 * w * pts; break;
+      // one rhs evaluation: state in, rhs out (forward); state and adjoint in (adjoint)
+      case MGD_KERNEL_RHS_ADJOINT: *bytes = 3.0 * w * pts; break;
       default: *bytes = 2.0 * w * pts; break;
     }
   }
