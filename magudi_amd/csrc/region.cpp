@@ -94,6 +94,11 @@ struct mgd_region {
   long ctrl_stride = 0;                  // control entries per instance
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // window recompute of the fused adjoint run: its own stream (default priority), two
+  // window buffers, RK buffer and scratch, and the events that order it with the adjoint
+  hipStream_t stream_rec = nullptr;
+  hipEvent_t ev_fwd = nullptr, ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+  mgd::DevBuf window2, b2r, qsr;
   double last_ms[3] = {0, 0, 0};
   int last_launch[3] = {0, 0, 0};
 };
@@ -494,16 +499,18 @@ FwdStage fwd_args(mgd_region* r, const double* qin, const double* q0, double* b2
   return a;
 }
 
-// one RK4 time step of the fused path: levels q0 -> q0, stage states in x / y
+// one RK4 time step of the fused path: levels q0 -> q0, stage states in x / y; with
+// `outs`, the outputs of stages 1-4 go to outs[0..3] instead (window recompute)
 void fused_forward_step(mgd_region* r, double* q0, double* x, double* y, double* b2, int s, double dt,
-                        int nst, bool with_j, double* const* outs) {
+                        int nst, bool with_j, double* const* outs, hipStream_t st = nullptr) {
+  if (!st) st = r->stream;
   const double* qin[4] = {q0, x, y, x};
   double* qo[4] = {x, y, x, q0};
   for (int i = 1; i <= 4; ++i) {
     double* o = outs ? outs[i - 1] : qo[i - 1];
     const double* in = outs ? (i == 1 ? q0 : outs[i - 2]) : qin[i - 1];
     FwdStage a = fwd_args(r, in, q0, b2, o, i, dt, 4 * s + i - 1, nst, with_j);
-    launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+    launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, a, st);
     if (with_j) reduce_chunk(r, r->jpart, n_cost(r), nst, 4 * s + i - 1, false);
   }
 }
@@ -562,12 +569,21 @@ int maybe_tune(mgd_region* r) {
   return tune_tiling(r);
 }
 
+// Solver%runAdjoint on the fused path.  Forward pass with a snapshot every K steps, then
+// the adjoint marches back window by window (window w: steps [wK, min((w+1)K, nsteps))).
+// A window's stage states are recomputed from its snapshot into one of two window
+// buffers on a second stream, so the recompute of window w-1 runs concurrently with the
+// adjoint of window w (UniformCheckpointer%migrateTo semantics: the same states, each
+// recomputed once).  Window slot 4m holds the state at step wK+m and slots 4m+1..4m+3
+// the inputs of its stages 2-4.
 int run_adjoint_fused(mgd_region* r, double dt, int nsteps, int K, double* gradient,
                       double* sensitivity, double* J) {
   const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
   const long M = (long)r->ninst * M_of(r);
   if (int rc = prepare_quadrature(r, nst, dt)) return rc;
-  if (r->ly.alloc(M) || r->l0.alloc(M)) return MGD_ERR_HIP;
+  if (r->ly.alloc(M) || r->l0.alloc(M) || r->window2.alloc((size_t)4 * K * M) || r->b2r.alloc(M) ||
+      r->qsr.alloc(M))
+    return MGD_ERR_HIP;
   MGD_HIP(hipEventRecord(r->ev0, r->stream));
   launch_copy(M, r->q.p, r->snaps.p, r->stream);
   for (int s = 0; s < nsteps; ++s) {
@@ -575,6 +591,25 @@ int run_adjoint_fused(mgd_region* r, double dt, int nsteps, int K, double* gradi
     if ((s + 1) % K == 0) launch_copy(M, r->q.p, r->snaps.p + (long)((s + 1) / K) * M, r->stream);
   }
   if (nc > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, r->stream);
+  MGD_HIP(hipEventRecord(r->ev_fwd, r->stream));
+  MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_fwd, 0));
+  double* wbuf[2] = {r->window.p, r->window2.p};
+  const int nw = (nsteps + K - 1) / K;
+  auto recompute = [&](int w) {
+    double* W = wbuf[w % 2];
+    const int t0w = w * K, t1w = std::min(nsteps, t0w + K);
+    launch_copy(M, r->snaps.p + (long)w * M, W, r->stream_rec);
+    for (int t = t0w; t < t1w; ++t) {
+      const long base = 4L * (t - t0w);
+      double* outs[4];
+      for (int k = 0; k < 4; ++k) outs[k] = W + (base + 1 + k) * M;
+      if (t == t1w - 1) outs[3] = r->qsr.p;   // state at the next snapshot: not needed
+      fused_forward_step(r, W + base * M, nullptr, nullptr, r->b2r.p, t, dt, nst, false, outs,
+                         r->stream_rec);
+    }
+  };
+  recompute(nw - 1);
+  MGD_HIP(hipEventRecord(r->ev_ready[(nw - 1) % 2], r->stream_rec));
   // terminal condition on the final state
   double* L0 = r->lam.p;
   double* LX = r->ly.p;
@@ -584,59 +619,47 @@ int run_adjoint_fused(mgd_region* r, double dt, int nsteps, int K, double* gradi
     launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->q.p, r->aforce.p, 0, r->stream);
     launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -dt / 6.0, L0, r->stream);
   }
-  int loaded = -1;
-  for (int t = nsteps - 1; t >= 0; --t) {
-    for (int i = 4; i >= 1; --i) {
-      const int ts = i == 1 ? t : t + 1;
-      const int st = i == 1 ? 4 : i - 1;
-      const bool need = loaded == -1 || ts < loaded || ts > loaded + K ||
-                        (ts == loaded && st < 4) || (ts == loaded + K && st == 4);
-      if (need) {
-        int lt;
-        if (ts % K == 0 && st == 4) lt = ts;
-        else if (ts % K == 0) lt = ts - K;
-        else lt = ts - ts % K;
-        loaded = lt;
-        launch_copy(M, r->snaps.p + (long)(lt / K) * M, r->window.p, r->stream);
-        if (loaded != nsteps) {
-          for (int s2 = loaded + 1; s2 <= loaded + K && s2 <= nsteps; ++s2) {
-            const long base = 4L * (s2 - loaded - 1);
-            double* outs[4];
-            for (int k = 0; k < 4; ++k) outs[k] = r->window.p + (base + 1 + k) * M;
-            if (s2 == loaded + K) outs[3] = r->qs.p;   // next snapshot level: not kept
-            fused_forward_step(r, r->window.p + base * M, nullptr, nullptr, r->b2.p, s2 - 1, dt, nst,
-                               false, outs);
-          }
-        }
-      }
-      const int widx = (ts - 1 - loaded) * 4 + st;
-      const double* qcur = r->window.p + (long)widx * M;
-      const int sidx = 4 * t + i - 1;
-      AdjStage a;
-      std::memset(&a, 0, sizeof a);
-      a.lam = i == 4 ? L0 : (i == 3 ? LX : (i == 2 ? LY : LX));
-      a.lout = i == 4 ? LX : (i == 3 ? LY : (i == 2 ? LX : L0));
-      a.l0 = L0;
-      a.q = qcur;
-      a.b2 = r->b2.p;
-      a.stage = i;
-      a.dt = dt;
-      a.forcing_factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
-      a.zero_forcing = t == 0 && i == 1;
-      a.ip_act = r->ip_act;
-      if (na > 0) {
-        if (gradient) {
-          a.grad = r->grad.p + (long)sidx * na;
-          a.grad_stride = (long)nst * na;
-        }
-        a.spartial = r->spart.p;
-        a.npts = na;
-        a.nslots = r->chunk;
-        a.slot = sidx % r->chunk;
-      }
-      launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
-      reduce_chunk(r, r->spart, na, nst, sidx, true);
+  for (int w = nw - 1; w >= 0; --w) {
+    if (w >= 1) {
+      // window w-1 reuses the buffer of window w+1: wait until its adjoint has read it
+      if (w + 1 < nw) MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_free[(w + 1) % 2], 0));
+      recompute(w - 1);
+      MGD_HIP(hipEventRecord(r->ev_ready[(w - 1) % 2], r->stream_rec));
     }
+    MGD_HIP(hipStreamWaitEvent(r->stream, r->ev_ready[w % 2], 0));
+    const double* W = wbuf[w % 2];
+    const int t0w = w * K, t1w = std::min(nsteps, t0w + K);
+    for (int t = t1w - 1; t >= t0w; --t) {
+      for (int i = 4; i >= 1; --i) {
+        const double* qcur = W + (4L * (t - t0w) + (i - 1)) * M;
+        const int sidx = 4 * t + i - 1;
+        AdjStage a;
+        std::memset(&a, 0, sizeof a);
+        a.lam = i == 4 ? L0 : (i == 3 ? LX : (i == 2 ? LY : LX));
+        a.lout = i == 4 ? LX : (i == 3 ? LY : (i == 2 ? LX : L0));
+        a.l0 = L0;
+        a.q = qcur;
+        a.b2 = r->b2.p;
+        a.stage = i;
+        a.dt = dt;
+        a.forcing_factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
+        a.zero_forcing = t == 0 && i == 1;
+        a.ip_act = r->ip_act;
+        if (na > 0) {
+          if (gradient) {
+            a.grad = r->grad.p + (long)sidx * na;
+            a.grad_stride = (long)nst * na;
+          }
+          a.spartial = r->spart.p;
+          a.npts = na;
+          a.nslots = r->chunk;
+          a.slot = sidx % r->chunk;
+        }
+        launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+        reduce_chunk(r, r->spart, na, nst, sidx, true);
+      }
+    }
+    MGD_HIP(hipEventRecord(r->ev_free[w % 2], r->stream));
   }
   if (na > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_S.p, r->ninst, 1, r->stream);
   MGD_HIP(hipEventRecord(r->ev1, r->stream));
@@ -765,8 +788,14 @@ int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region
   g.dissipation = d->dissipation;
   g.composite = d->composite_dissipation;
   MGD_HIP(hipStreamCreate(&r->stream));
+  MGD_HIP(hipStreamCreate(&r->stream_rec));
   MGD_HIP(hipEventCreate(&r->ev0));
   MGD_HIP(hipEventCreate(&r->ev1));
+  MGD_HIP(hipEventCreateWithFlags(&r->ev_fwd, hipEventDisableTiming));
+  for (int k = 0; k < 2; ++k) {
+    MGD_HIP(hipEventCreateWithFlags(&r->ev_ready[k], hipEventDisableTiming));
+    MGD_HIP(hipEventCreateWithFlags(&r->ev_free[k], hipEventDisableTiming));
+  }
   r->target.assign((size_t)r->nv * N, 0.0);
   r->ctrl_moll.assign(N, 0.0);
   r->tgt_moll.assign(N, 0.0);
@@ -778,8 +807,15 @@ int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region
 int mgd_region_destroy(mgd_region* r) {
   if (!r) return MGD_OK;
   if (r->stream) (void)hipStreamSynchronize(r->stream);
+  if (r->stream_rec) (void)hipStreamSynchronize(r->stream_rec);
   if (r->ev0) (void)hipEventDestroy(r->ev0);
   if (r->ev1) (void)hipEventDestroy(r->ev1);
+  if (r->ev_fwd) (void)hipEventDestroy(r->ev_fwd);
+  for (int k = 0; k < 2; ++k) {
+    if (r->ev_ready[k]) (void)hipEventDestroy(r->ev_ready[k]);
+    if (r->ev_free[k]) (void)hipEventDestroy(r->ev_free[k]);
+  }
+  if (r->stream_rec) (void)hipStreamDestroy(r->stream_rec);
   if (r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
   return MGD_OK;
