@@ -47,6 +47,8 @@ extern "C" {
 #define MGD_RHS 2
 #define MGD_TARGET 3
 #define MGD_INITIAL_STATE 4     /* device-resident copy of an initial condition */
+#define MGD_FINAL_STATE 5       /* forward state at the end of the last checkpointed run */
+#define MGD_SCRATCH 6           /* work state for host drivers (matching conditions) */
 
 /* Fields for mgd_region_set_field */
 #define MGD_FIELD_TARGET_STATE 0
@@ -120,14 +122,32 @@ int mgd_region_add_source(mgd_region* r, const double* location, double amplitud
  * (src/FarFieldPatchImpl.f90:3-74), mean pressure (src/AcousticNoiseImpl.f90:3-71). */
 int mgd_region_finalize(mgd_region* r);
 
-/* Copies a state array of one instance (MGD_CONSERVED / MGD_ADJOINT_VARS / MGD_RHS)
- * between host and HBM.  Replaces Region%loadData / saveData for in-memory states. */
+/* Per-instance time offsets (n_instances values, NULL to clear): instance i runs at
+ * t + offsets[i] wherever a time enters (acoustic source phase, stage times of a run).
+ * Lets independent time segments (multiple shooting, bin/msforward.f90:242-365) share
+ * one batched region. */
+int mgd_region_set_time_offsets(mgd_region* r, const double* offsets);
+
+/* Copies a state array of one instance (any MGD_CONSERVED ... MGD_SCRATCH id) between
+ * host and HBM.  Replaces Region%loadData / saveData for in-memory states. */
 int mgd_state_put(mgd_region* r, int which, int instance, const double* host);
 int mgd_state_get(mgd_region* r, int which, int instance, double* host);
 
 /* Device-to-device copy of a state array of all instances (e.g. MGD_INITIAL_STATE to
  * MGD_CONSERVED to restart from a resident initial condition).  Asynchronous. */
 int mgd_state_copy(mgd_region* r, int src, int dst);
+
+/* dst[dst_inst] = c[c_inst] + s * (a[a_inst] - b[b_inst]) on one instance of state
+ * arrays (c < 0: no c term).  The matching-condition updates of the multiple-shooting
+ * drivers: mismatch ic_k - end_(k-1) (bin/msforward.f90:300-318), IC gradient
+ * lambda_(k+1) + s (ic_(k+1) - end_k) and adjoint terminal s (end_k - ic_(k+1))
+ * (bin/msadjoint.f90:340-400). */
+int mgd_state_lincomb(mgd_region* r, int dst, int dst_inst, int a, int a_inst, int b, int b_inst,
+                      double s, int c, int c_inst);
+
+/* Grid%computeInnerProduct (src/GridImpl.f90:1067-1170) per instance: out[inst] = sum over
+ * components and grid points of a * norm * b. */
+int mgd_inner_product(mgd_region* r, int a, int b, double* out);
 
 /* Region%computeRhs(mode) (src/RegionImpl.f90:1877-2027) for all instances into the
  * region's rhs buffer.  time: State%time for acoustic sources.  control: actuator
@@ -162,6 +182,21 @@ int mgd_run_forward(mgd_region* r, double t0, double dt, int nsteps, const doubl
 int mgd_run_adjoint(mgd_region* r, double t0, double dt, int nsteps, int save_interval,
                     const double* control, double* gradient, double* sensitivity, double* J,
                     int use_graph);
+
+/* The two halves of mgd_run_adjoint, for drivers that set the adjoint terminal condition
+ * from the forward result (multiple shooting, bin/msadjoint.f90:287-384).
+ * mgd_run_forward_checkpointed: the forward pass of Solver%runAdjoint from the current
+ * conserved state, keeping a snapshot every save_interval steps in HBM (the .q files of
+ * showProgress, src/SolverImpl.f90:672-912) and the final state (MGD_FINAL_STATE).
+ * mgd_run_adjoint_checkpointed: the adjoint march back over those snapshots.
+ * terminal = 0 starts from zero, terminal = 1 from the current MGD_ADJOINT_VARS
+ * (adjoint_nonzero_initial_condition, loadInitialCondition ADJOINT,
+ * src/SolverImpl.f90:368-428); the terminal cost forcing of the final state is added in
+ * both cases. */
+int mgd_run_forward_checkpointed(mgd_region* r, double t0, double dt, int nsteps,
+                                 int save_interval, const double* control, double* J);
+int mgd_run_adjoint_checkpointed(mgd_region* r, int terminal, double* gradient,
+                                 double* sensitivity);
 
 /* t_StencilOperator%apply (include/StencilOperator.f90:19-27,
  * src/StencilOperatorImpl.f90:2374-2410) of one of the region's operators along

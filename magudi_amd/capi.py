@@ -16,7 +16,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "magudi_hip.h")
 
 MGD_OK = 0
 FORWARD, ADJOINT = 1, -1
-CONSERVED, ADJOINT_VARS, RHS, INITIAL_STATE = 0, 1, 2, 4
+CONSERVED, ADJOINT_VARS, RHS, INITIAL_STATE, FINAL_STATE, SCRATCH = 0, 1, 2, 4, 5, 6
 FIELD_TARGET_STATE, FIELD_CONTROL_MOLLIFIER, FIELD_TARGET_MOLLIFIER = 0, 1, 2
 OP_FIRST, OP_ADJOINT_FIRST, OP_DISS, OP_DISS_T = 0, 1, 2, 3
 OPT_FORCE_PLAIN, OPT_TILE_SHAPE = 1, 2
@@ -83,6 +83,12 @@ def lib():
         "mgd_region_get_tile_shape": [vp, C.POINTER(C.c_int)],
         "mgd_time_kernel": [vp, C.c_int, C.c_int, dp, dp],
         "mgd_state_copy": [vp, C.c_int, C.c_int],
+        "mgd_state_lincomb": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                              C.c_double, C.c_int, C.c_int],
+        "mgd_inner_product": [vp, C.c_int, C.c_int, dp],
+        "mgd_region_set_time_offsets": [vp, dp],
+        "mgd_run_forward_checkpointed": [vp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp],
+        "mgd_run_adjoint_checkpointed": [vp, C.c_int, dp, dp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

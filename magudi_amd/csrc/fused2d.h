@@ -45,6 +45,7 @@ struct FwdStage {
   long control_stride;
   const Source* src;
   const double* src_amp;    // amplitude of each source at this stage's time
+  long src_inst_stride;     // entries between instances of src_amp (0: shared)
   int nsrc;
   double* jpartial;         // [inst][slot][cost-patch point] functional terms, or null
   int npts, nslots, slot, ip_cost;

@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(S::NT, 4) k_fwd_stage2d(Geo g, const Patches* 
       double rr2 = 0.0;
       rr2 += dx0 * dx0;
       rr2 += dy0 * dy0;
-      r[k][3] = r[k][3] + a.src_amp[s] * exp(-a.src[s].gfac * rr2);
+      r[k][3] = r[k][3] + a.src_amp[inst * a.src_inst_stride + s] * exp(-a.src[s].gfac * rr2);
     }
     // RK4 stage update (substepForwardRK4, src/RK4IntegratorImpl.f90:115-156)
     const double dt = a.dt;

@@ -8,7 +8,8 @@ constexpr int kBlock = 256;
 
 void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
                         double* G, double* t, const double* control, long control_stride,
-                        const Source* src, int nsrc, const double* src_amp, hipStream_t st);
+                        const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
+                        hipStream_t st);
 void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
                         double* rhs, double* w, double* G, double* t, double* T1, double* S,
                         const double* aforce, long aforce_stride, double forcing_factor,
@@ -28,6 +29,11 @@ void launch_adj_terminal(const Geo& g, const DPatch* pt, int np, const double* a
                          double* lam, hipStream_t st);
 void launch_gradient(const Geo& g, const DPatch* pt, int np, const double* lam, double* out,
                      long inst_stride, hipStream_t st);
+// out[inst] = <a, b>_H over the instance's state arrays; partial: ninst * ceil(N/256)
+void launch_inner_product(const Geo& g, const double* a, const double* b, double* partial,
+                          double* out, hipStream_t st);
+void launch_lincomb(long M, const double* a, const double* b, double s, const double* c, double* d,
+                    hipStream_t st);
 void launch_apply(const Geo& g, int slot, int d, int ncomp, const double* x, double* y,
                   hipStream_t st);
 

@@ -362,7 +362,7 @@ __global__ void k_finish(Geo g, int mode, const Patches* __restrict__ P, const d
                          double* __restrict__ rhs, const double* __restrict__ control,
                          long control_inst_stride, const double* __restrict__ aforce,
                          long aforce_inst_stride, double forcing_factor, const Source* __restrict__ src,
-                         int nsrc, const double* __restrict__ src_amp) {
+                         int nsrc, const double* __restrict__ src_amp, long src_inst_stride) {
   const long p = gidx();
   if (p >= g.N) return;
   const long inst = blockIdx.y;
@@ -479,7 +479,7 @@ __global__ void k_finish(Geo g, int mode, const Patches* __restrict__ P, const d
         const double dx = g.x[d][ijk[d]] - src[s].loc[d];
         rr += dx * dx;
       }
-      r[ND + 1] = r[ND + 1] + src_amp[s] * exp(-src[s].gfac * rr);
+      r[ND + 1] = r[ND + 1] + src_amp[inst * src_inst_stride + s] * exp(-src[s].gfac * rr);
     }
   }
   for (int l = 0; l < NV; ++l) ri[l * g.N + p] = r[l];
@@ -624,6 +624,34 @@ __global__ void k_reduce_final(int nblocks, const double* __restrict__ partial, 
   }
 }
 
+// Grid%computeInnerProduct (src/GridImpl.f90:1067-1170): per-block partial sums of
+// sum over components of a * norm * b at each point of one instance.
+template <int ND>
+__global__ void k_inner_partial(Geo g, const double* __restrict__ a, const double* __restrict__ b,
+                                double* __restrict__ partial) {
+  const long inst = blockIdx.y;
+  const long p = (long)blockIdx.x * kRedBlock + threadIdx.x;
+  double v = 0.0;
+  if (p < g.N) {
+    int ijk[3];
+    unflat<ND>(g, p, ijk);
+    const double nrm = gnorm<ND>(g, ijk);
+    const double* ai = a + inst * g.nv * g.N;
+    const double* bi = b + inst * g.nv * g.N;
+    for (int c = 0; c < g.nv; ++c) v += ai[c * g.N + p] * nrm * bi[c * g.N + p];
+  }
+  const double s = block_sum(v);
+  if (threadIdx.x == 0) partial[inst * gridDim.x + blockIdx.x] = s;
+}
+
+// d = c + s (a - b) (c may be null: d = s (a - b)) over one instance's state array.
+__global__ void k_lincomb(long M, const double* __restrict__ a, const double* __restrict__ b,
+                          double s, const double* __restrict__ c, double* __restrict__ d) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  d[i] = c ? c[i] + s * (a[i] - b[i]) : s * (a[i] - b[i]);
+}
+
 // AcousticNoise adjoint forcing on the cost-target patch (src/AcousticNoiseImpl.f90:208-280)
 template <int ND>
 __global__ void k_adj_forcing(Geo g, const DPatch* __restrict__ pt_, const double* __restrict__ q,
@@ -698,7 +726,8 @@ static dim3 grid_pts(const Geo& g) {
 
 void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
                         double* G, double* t, const double* control, long control_stride,
-                        const Source* src, int nsrc, const double* src_amp, hipStream_t st) {
+                        const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
+                        hipStream_t st) {
   const dim3 grid = grid_pts(g);
   MGD_DISPATCH(g.nd, {
     if (g.viscous) {
@@ -714,7 +743,8 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
       }
     }
     k_finish<ND><<<grid, kBlock, 0, st>>>(g, MGD_FORWARD, P, q, nullptr, G, rhs, control,
-                                          control_stride, nullptr, 0, 0.0, src, nsrc, src_amp);
+                                          control_stride, nullptr, 0, 0.0, src, nsrc, src_amp,
+                                          src_inst_stride);
   });
 }
 
@@ -753,7 +783,7 @@ void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
       k_adj_chain<ND><<<grid, kBlock, 0, st>>>(g, q, S, rhs, +1.0);
     }
     k_finish<ND><<<grid, kBlock, 0, st>>>(g, MGD_ADJOINT, P, q, lam, G, rhs, nullptr, 0, aforce,
-                                          aforce_stride, forcing_factor, nullptr, 0, nullptr);
+                                          aforce_stride, forcing_factor, nullptr, 0, nullptr, 0);
   });
 }
 
@@ -786,6 +816,20 @@ void launch_patch_reduce(const Geo& g, const DPatch* pt, int np, int which, cons
     k_patch_reduce<ND><<<dim3(nb, g.ninst), kRedBlock, 0, st>>>(g, pt, which, x, partial);
   });
   k_reduce_final<<<g.ninst, kRedBlock, 0, st>>>(nb, partial, weight, acc, raw);
+}
+
+void launch_inner_product(const Geo& g, const double* a, const double* b, double* partial,
+                          double* out, hipStream_t st) {
+  const int nb = (int)((g.N + kRedBlock - 1) / kRedBlock);
+  MGD_DISPATCH(g.nd, {
+    k_inner_partial<ND><<<dim3(nb, g.ninst), kRedBlock, 0, st>>>(g, a, b, partial);
+  });
+  k_reduce_final<<<g.ninst, kRedBlock, 0, st>>>(nb, partial, 1.0, nullptr, out);
+}
+
+void launch_lincomb(long M, const double* a, const double* b, double s, const double* c, double* d,
+                    hipStream_t st) {
+  k_lincomb<<<(unsigned)((M + kBlock - 1) / kBlock), kBlock, 0, st>>>(M, a, b, s, c, d);
 }
 
 void launch_adj_forcing(const Geo& g, const DPatch* pt, int np, const double* q, double* out,

@@ -99,6 +99,17 @@ struct mgd_region {
   hipStream_t stream_rec = nullptr;
   hipEvent_t ev_fwd = nullptr, ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
   mgd::DevBuf window2, b2r, qsr;
+  // checkpointed forward run the adjoint pass marches back from
+  struct {
+    bool valid = false;
+    double t0 = 0, dt = 0;
+    int nsteps = 0, K = 1;
+    std::vector<double> control;         // actuator forcing of every stage (host copy)
+  } ck;
+  mgd::DevBuf qT, scr;                   // final forward state; scratch state (MGD_SCRATCH)
+  mgd::DevBuf ipart, iout;               // inner-product partial sums and results
+  std::vector<double> toff;              // per-instance time offsets (empty: none)
+  long src_inst_stride = 0;              // source table entries per instance (0: shared)
   double last_ms[3] = {0, 0, 0};
   int last_launch[3] = {0, 0, 0};
 };
@@ -326,6 +337,17 @@ void target_viscous_fluxes(mgd_region* r, HostPatch& P) {
   }
 }
 
+// Device array of a state id (MGD_CONSERVED ... MGD_SCRATCH); `create` allocates the
+// arrays that exist only on demand (initial, final, scratch).
+DevBuf* state_buf(mgd_region* r, int which, bool create) {
+  DevBuf* b = which == MGD_CONSERVED ? &r->q : which == MGD_ADJOINT_VARS ? &r->lam
+            : which == MGD_RHS ? &r->rhs : which == MGD_INITIAL_STATE ? &r->init
+            : which == MGD_FINAL_STATE ? &r->qT : which == MGD_SCRATCH ? &r->scr : nullptr;
+  if (!b) return nullptr;
+  if (!b->p && create && b->alloc((size_t)r->ninst * r->nv * r->N)) return nullptr;
+  return b->p ? b : nullptr;
+}
+
 int ensure_state(mgd_region* r) {
   const size_t M = (size_t)r->ninst * r->nv * r->N;
   if (r->q.alloc(M) || r->lam.alloc(M) || r->rhs.alloc(M) || r->b1.alloc(M) || r->b2.alloc(M))
@@ -353,16 +375,45 @@ const DPatch* dpatch(const mgd_region* r, int ip) {
 }
 const Patches* dpatches(const mgd_region* r) { return reinterpret_cast<const Patches*>(r->d_patches.p); }
 
-// source amplitude a = amplitude cos(omega t + phase) for each stage time
-void source_table(const mgd_region* r, const std::vector<double>& times, std::vector<double>& out) {
+// stage times of substepForwardRK4: stage 1 at t, 2 and 3 at t + dt/2, 4 at t + dt
+std::vector<double> stage_times(double t0, double dt, int nsteps);
+
+// Source amplitudes a = amplitude cos(omega t + phase), laid out [instance][time][source],
+// instance i at the times `times` shifted by its time offset.  With no offsets the
+// instances share one table (inst_stride 0).
+void source_table(mgd_region* r, const std::vector<double>& times, std::vector<double>& out) {
   const int ns = (int)r->sources.size();
-  out.assign(times.size() * std::max(ns, 1), 0.0);
-  for (size_t k = 0; k < times.size(); ++k)
-    for (int s = 0; s < ns; ++s)
-      out[k * ns + s] = r->sources[s].amplitude * std::cos(r->sources[s].omega * times[k] + r->sources[s].phase);
+  const size_t nt = times.size();
+  const bool shifted = !r->toff.empty();
+  const int ni = shifted ? r->ninst : 1;
+  out.assign((size_t)ni * nt * std::max(ns, 1), 0.0);
+  for (int i = 0; i < ni; ++i)
+    for (size_t k = 0; k < nt; ++k)
+      for (int s = 0; s < ns; ++s) {
+        const double t = shifted ? times[k] + r->toff[(size_t)i] : times[k];
+        out[((size_t)i * nt + k) * ns + s] =
+            r->sources[s].amplitude * std::cos(r->sources[s].omega * t + r->sources[s].phase);
+      }
+  r->src_inst_stride = shifted ? (long)(nt * ns) : 0;
 }
 
-// stage times of substepForwardRK4: stage 1 at t, 2 and 3 at t + dt/2, 4 at t + dt
+// Stage times of a run: per instance when the instances carry time offsets (the offsets
+// are applied to the start time, then the stages are accumulated as in substepForwardRK4).
+void run_source_table(mgd_region* r, double t0, double dt, int nsteps, std::vector<double>& out) {
+  if (r->toff.empty()) { source_table(r, stage_times(t0, dt, nsteps), out); return; }
+  const int ns = (int)r->sources.size();
+  const size_t nt = 4 * (size_t)nsteps;
+  out.assign((size_t)r->ninst * nt * std::max(ns, 1), 0.0);
+  for (int i = 0; i < r->ninst; ++i) {
+    const std::vector<double> t = stage_times(t0 + r->toff[(size_t)i], dt, nsteps);
+    for (size_t k = 0; k < nt; ++k)
+      for (int s = 0; s < ns; ++s)
+        out[((size_t)i * nt + k) * ns + s] =
+            r->sources[s].amplitude * std::cos(r->sources[s].omega * t[k] + r->sources[s].phase);
+  }
+  r->src_inst_stride = (long)(nt * ns);
+}
+
 std::vector<double> stage_times(double t0, double dt, int nsteps) {
   std::vector<double> t(4 * (size_t)nsteps);
   double time = t0;
@@ -396,7 +447,7 @@ void forward_stage(mgd_region* r, double* q, double* b1, double* b2, int sidx, i
   const int ns = (int)r->sources.size();
   launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p, r->G.p, r->t.p, ctrl, cstride,
                      reinterpret_cast<const Source*>(r->d_sources.p), ns,
-                     r->src_amp.p + (long)sidx * std::max(ns, 1), r->stream);
+                     r->src_amp.p + (long)sidx * std::max(ns, 1), r->src_inst_stride, r->stream);
   launch_rk_forward((long)r->ninst * M_of(r), stage, dt, q, b1, b2, r->rhs.p, qcopy, r->stream);
 }
 
@@ -488,6 +539,7 @@ FwdStage fwd_args(mgd_region* r, const double* qin, const double* q0, double* b2
   a.src = reinterpret_cast<const Source*>(r->d_sources.p);
   a.nsrc = ns;
   a.src_amp = r->src_amp.p + (long)sidx * std::max(ns, 1);
+  a.src_inst_stride = r->src_inst_stride;
   a.ip_cost = r->ip_cost;
   (void)nst;
   if (with_j && n_cost(r) > 0) {
@@ -569,28 +621,33 @@ int maybe_tune(mgd_region* r) {
   return tune_tiling(r);
 }
 
-// Solver%runAdjoint on the fused path.  Forward pass with a snapshot every K steps, then
-// the adjoint marches back window by window (window w: steps [wK, min((w+1)K, nsteps))).
-// A window's stage states are recomputed from its snapshot into one of two window
-// buffers on a second stream, so the recompute of window w-1 runs concurrently with the
-// adjoint of window w (UniformCheckpointer%migrateTo semantics: the same states, each
-// recomputed once).  Window slot 4m holds the state at step wK+m and slots 4m+1..4m+3
-// the inputs of its stages 2-4.
-int run_adjoint_fused(mgd_region* r, double dt, int nsteps, int K, double* gradient,
-                      double* sensitivity, double* J) {
-  const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
+// Solver%runAdjoint on the fused path, split in its two passes.  Forward pass: snapshot
+// every K steps.  Adjoint pass: marches back window by window (window w: steps
+// [wK, min((w+1)K, nsteps))).  A window's stage states are recomputed from its snapshot
+// into one of two window buffers on a second stream, so the recompute of window w-1 runs
+// concurrently with the adjoint of window w (UniformCheckpointer%migrateTo semantics:
+// the same states, each recomputed once).  Window slot 4m holds the state at step wK+m
+// and slots 4m+1..4m+3 the inputs of its stages 2-4.
+int fused_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
+  const int nc = n_cost(r), nst = 4 * nsteps;
   const long M = (long)r->ninst * M_of(r);
   if (int rc = prepare_quadrature(r, nst, dt)) return rc;
-  if (r->ly.alloc(M) || r->l0.alloc(M) || r->window2.alloc((size_t)4 * K * M) || r->b2r.alloc(M) ||
-      r->qsr.alloc(M))
-    return MGD_ERR_HIP;
-  MGD_HIP(hipEventRecord(r->ev0, r->stream));
   launch_copy(M, r->q.p, r->snaps.p, r->stream);
   for (int s = 0; s < nsteps; ++s) {
     fused_forward_step(r, r->q.p, r->b1.p, r->qy.p, r->b2.p, s, dt, nst, true, nullptr);
     if ((s + 1) % K == 0) launch_copy(M, r->q.p, r->snaps.p + (long)((s + 1) / K) * M, r->stream);
   }
   if (nc > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, r->stream);
+  return MGD_OK;
+}
+
+int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool terminal, bool want_grad) {
+  const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
+  const long M = (long)r->ninst * M_of(r);
+  if (int rc = prepare_quadrature(r, nst, dt)) return rc;
+  if (r->ly.alloc(M) || r->l0.alloc(M) || r->window2.alloc((size_t)4 * K * M) || r->b2r.alloc(M) ||
+      r->qsr.alloc(M))
+    return MGD_ERR_HIP;
   MGD_HIP(hipEventRecord(r->ev_fwd, r->stream));
   MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_fwd, 0));
   double* wbuf[2] = {r->window.p, r->window2.p};
@@ -610,13 +667,14 @@ int run_adjoint_fused(mgd_region* r, double dt, int nsteps, int K, double* gradi
   };
   recompute(nw - 1);
   MGD_HIP(hipEventRecord(r->ev_ready[(nw - 1) % 2], r->stream_rec));
-  // terminal condition on the final state
+  // terminal condition (loadInitialCondition ADJOINT, src/SolverImpl.f90:368-428): zero or
+  // the given adjoint state, plus the cost forcing of the final state with factor -dt/6
   double* L0 = r->lam.p;
   double* LX = r->ly.p;
   double* LY = r->l0.p;
-  launch_fill(M, 0.0, L0, r->stream);
+  if (!terminal) launch_fill(M, 0.0, L0, r->stream);
   if (nc > 0) {
-    launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->q.p, r->aforce.p, 0, r->stream);
+    launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->qT.p, r->aforce.p, 0, r->stream);
     launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -dt / 6.0, L0, r->stream);
   }
   for (int w = nw - 1; w >= 0; --w) {
@@ -646,7 +704,7 @@ int run_adjoint_fused(mgd_region* r, double dt, int nsteps, int K, double* gradi
         a.zero_forcing = t == 0 && i == 1;
         a.ip_act = r->ip_act;
         if (na > 0) {
-          if (gradient) {
+          if (want_grad) {
             a.grad = r->grad.p + (long)sidx * na;
             a.grad_stride = (long)nst * na;
           }
@@ -662,13 +720,152 @@ int run_adjoint_fused(mgd_region* r, double dt, int nsteps, int K, double* gradi
     MGD_HIP(hipEventRecord(r->ev_free[w % 2], r->stream));
   }
   if (na > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_S.p, r->ninst, 1, r->stream);
+  return MGD_OK;
+}
+
+// Plain path of the two passes (generic per-operator kernels).
+int plain_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
+  const int nc = n_cost(r);
+  const long M = (long)r->ninst * M_of(r);
+  // forward run with snapshots at t0 and every K steps (showProgress .q files)
+  launch_copy(M, r->q.p, r->snaps.p, r->stream);
+  for (int s = 0; s < nsteps; ++s) {
+    for (int i = 1; i <= 4; ++i) {
+      forward_stage(r, r->q.p, r->b1.p, r->b2.p, 4 * s + i - 1, i, dt, nullptr);
+      if (nc > 0)
+        launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), nc, 0, r->q.p, r->partial.p,
+                            kRkNorm[i - 1] * dt, r->acc_J.p, nullptr, r->stream);
+    }
+    if ((s + 1) % K == 0) launch_copy(M, r->q.p, r->snaps.p + (long)((s + 1) / K) * M, r->stream);
+  }
+  return MGD_OK;
+}
+
+int plain_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool terminal, bool want_grad) {
+  const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
+  const long M = (long)r->ninst * M_of(r);
+  // adjoint terminal condition (loadInitialCondition ADJOINT, src/SolverImpl.f90:368-428)
+  if (!terminal) launch_fill(M, 0.0, r->lam.p, r->stream);
+  if (nc > 0) {
+    launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->qT.p, r->aforce.p, 0, r->stream);
+    launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -dt / 6.0, r->lam.p, r->stream);
+  }
+  int loaded = -1;
+  for (int t = nsteps - 1; t >= 0; --t) {
+    for (int i = 4; i >= 1; --i) {
+      const int ts = i == 1 ? t : t + 1;
+      const int st = i == 1 ? 4 : i - 1;
+      const bool need = loaded == -1 || ts < loaded || ts > loaded + K ||
+                        (ts == loaded && st < 4) || (ts == loaded + K && st == 4);
+      if (need) {
+        int lt;
+        if (ts % K == 0 && st == 4) lt = ts;
+        else if (ts % K == 0) lt = ts - K;
+        else lt = ts - ts % K;
+        loaded = lt;
+        launch_copy(M, r->snaps.p + (long)(lt / K) * M, r->qs.p, r->stream);
+        launch_copy(M, r->qs.p, r->window.p, r->stream);
+        if (loaded != nsteps) {
+          int idx = 1;
+          for (int s2 = loaded + 1; s2 <= loaded + K && s2 <= nsteps; ++s2)
+            for (int st2 = 1; st2 <= 4; ++st2) {
+              const bool last = s2 == loaded + K && st2 == 4;
+              forward_stage(r, r->qs.p, r->b1.p, r->b2.p, 4 * (s2 - 1) + st2 - 1, st2, dt,
+                            last ? nullptr : r->window.p + (long)idx * M);
+              if (last) break;
+              ++idx;
+            }
+        }
+      }
+      const int widx = (ts - 1 - loaded) * 4 + st;
+      const double* qcur = r->window.p + (long)widx * M;
+      const int sidx = 4 * t + i - 1;
+      if (na > 0) {
+        if (want_grad)
+          launch_gradient(r->geo, dpatch(r, r->ip_act), na, r->lam.p, r->grad.p + (long)sidx * na,
+                          (long)nst * na, r->stream);
+        launch_patch_reduce(r->geo, dpatch(r, r->ip_act), na, 1, r->lam.p, r->partial.p,
+                            kRkNorm[i - 1] * dt, r->acc_S.p, nullptr, r->stream);
+      }
+      if (nc > 0)
+        launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, qcur, r->aforce.p, t == 0 && i == 1,
+                           r->stream);
+      const double factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
+      launch_rhs_adjoint(r->geo, dpatches(r), qcur, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p,
+                         r->S.p, nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc, factor,
+                         r->has_farfield, r->stream);
+      launch_rk_adjoint(M, i, dt, r->lam.p, r->b1a.p, r->b2a.p, r->rhs.p, r->stream);
+    }
+  }
+  return MGD_OK;
+}
+
+// Uploads the per-run inputs (actuator control of every stage, source amplitudes at the
+// stage times) of the checkpointed run held in r->ck.
+int upload_run_inputs(mgd_region* r) {
+  const int na = n_act(r), nst = 4 * r->ck.nsteps;
+  if (!r->ck.control.empty() && na > 0) {
+    if (upload(r->control, r->ck.control)) return MGD_ERR_HIP;
+    r->ctrl_stride = (long)nst * na;
+  } else if (r->control.p) {
+    r->control.alloc(0);
+  }
+  std::vector<double> amp;
+  run_source_table(r, r->ck.t0, r->ck.dt, r->ck.nsteps, amp);
+  if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
+  return MGD_OK;
+}
+
+// Forward pass with HBM snapshots (first half of Solver%runAdjoint; the snapshots play
+// the part of the .q files the reference writes every save_interval steps).
+int forward_checkpointed(mgd_region* r, double t0, double dt, int nsteps, int K, const double* control) {
+  if (int rc = ensure_adjoint(r)) return rc;
+  if (int rc = maybe_tune(r)) return rc;
+  const int na = n_act(r), nst = 4 * nsteps;
+  const long M = (long)r->ninst * M_of(r);
+  r->ck.valid = false;
+  r->ck.t0 = t0;
+  r->ck.dt = dt;
+  r->ck.nsteps = nsteps;
+  r->ck.K = K;
+  if (control && na > 0) r->ck.control.assign(control, control + (size_t)r->ninst * nst * na);
+  else r->ck.control.clear();
+  if (int rc = upload_run_inputs(r)) return rc;
+  const int nsnap = nsteps / K + 1;
+  if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M) || r->qs.alloc(M) ||
+      r->qT.alloc(M))
+    return MGD_ERR_HIP;
+  launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
+  const int rc = fused_ok(r) ? fused_forward_pass(r, dt, nsteps, K) : plain_forward_pass(r, dt, nsteps, K);
+  if (rc) return rc;
+  launch_copy(M, r->q.p, r->qT.p, r->stream);   // final state: terminal forcing, MGD_FINAL_STATE
+  r->ck.valid = true;
+  return MGD_OK;
+}
+
+// Adjoint pass from the snapshots of the last forward_checkpointed.
+int adjoint_checkpointed(mgd_region* r, bool terminal, bool want_grad) {
+  if (!r->ck.valid) { set_error("no checkpointed forward run to march back from"); return MGD_ERR_STATE; }
+  const int na = n_act(r), nst = 4 * r->ck.nsteps;
+  if (int rc = upload_run_inputs(r)) return rc;
+  if (na > 0 && r->grad.alloc((size_t)r->ninst * nst * na)) return MGD_ERR_HIP;
+  launch_fill(r->ninst, 0.0, r->acc_S.p, r->stream);
+  return fused_ok(r) ? fused_adjoint_pass(r, r->ck.dt, r->ck.nsteps, r->ck.K, terminal, want_grad)
+                     : plain_adjoint_pass(r, r->ck.dt, r->ck.nsteps, r->ck.K, terminal, want_grad);
+}
+
+int finish_timed(mgd_region* r) {
   MGD_HIP(hipEventRecord(r->ev1, r->stream));
   MGD_HIP(hipGetLastError());
   MGD_HIP(hipStreamSynchronize(r->stream));
   float ms = 0.f;
   MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
   r->last_ms[0] = ms;
-  if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
+  return MGD_OK;
+}
+
+int copy_adjoint_outputs(mgd_region* r, double* gradient, double* sensitivity) {
+  const int na = n_act(r), nst = 4 * r->ck.nsteps;
   if (sensitivity) MGD_HIP(hipMemcpy(sensitivity, r->acc_S.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
   if (gradient && na > 0)
     MGD_HIP(hipMemcpy(gradient, r->grad.p, sizeof(double) * r->ninst * nst * na, hipMemcpyDeviceToHost));
@@ -1009,6 +1206,13 @@ int mgd_region_finalize(mgd_region* r) {
   return MGD_OK;
 }
 
+int mgd_region_set_time_offsets(mgd_region* r, const double* offsets) {
+  if (!r) { set_error("null region"); return MGD_ERR_ARG; }
+  if (offsets) r->toff.assign(offsets, offsets + r->ninst);
+  else r->toff.clear();
+  return MGD_OK;
+}
+
 int mgd_region_get_tile_shape(mgd_region* r, int* shape) {
   if (!r || !shape) { set_error("null argument"); return MGD_ERR_ARG; }
   *shape = fused_ok(r) ? r->tiling.shape : -1;
@@ -1039,9 +1243,7 @@ int mgd_region_set_option(mgd_region* r, int option, int value) {
 
 int mgd_state_put(mgd_region* r, int which, int inst, const double* host) {
   if (!r || !host || inst < 0 || inst >= r->ninst) { set_error("bad argument"); return MGD_ERR_ARG; }
-  if (which == MGD_INITIAL_STATE && r->init.alloc((size_t)r->ninst * M_of(r))) return MGD_ERR_HIP;
-  DevBuf* b = which == MGD_CONSERVED ? &r->q : which == MGD_ADJOINT_VARS ? &r->lam
-            : which == MGD_RHS ? &r->rhs : which == MGD_INITIAL_STATE ? &r->init : nullptr;
+  DevBuf* b = state_buf(r, which, true);
   if (!b) { set_error("unknown state array"); return MGD_ERR_ARG; }
   const size_t M = (size_t)M_of(r);
   MGD_HIP(hipMemcpyAsync(b->p + inst * M, host, M * sizeof(double), hipMemcpyHostToDevice, r->stream));
@@ -1051,9 +1253,8 @@ int mgd_state_put(mgd_region* r, int which, int inst, const double* host) {
 
 int mgd_state_get(mgd_region* r, int which, int inst, double* host) {
   if (!r || !host || inst < 0 || inst >= r->ninst) { set_error("bad argument"); return MGD_ERR_ARG; }
-  DevBuf* b = which == MGD_CONSERVED ? &r->q : which == MGD_ADJOINT_VARS ? &r->lam
-            : which == MGD_RHS ? &r->rhs : nullptr;
-  if (!b) { set_error("unknown state array"); return MGD_ERR_ARG; }
+  DevBuf* b = state_buf(r, which, false);
+  if (!b) { set_error("unknown or empty state array"); return MGD_ERR_ARG; }
   const size_t M = (size_t)M_of(r);
   MGD_HIP(hipMemcpyAsync(host, b->p + inst * M, M * sizeof(double), hipMemcpyDeviceToHost, r->stream));
   MGD_HIP(hipStreamSynchronize(r->stream));
@@ -1062,15 +1263,46 @@ int mgd_state_get(mgd_region* r, int which, int inst, double* host) {
 
 int mgd_state_copy(mgd_region* r, int src, int dst) {
   if (!r) { set_error("null region"); return MGD_ERR_ARG; }
-  auto pick = [&](int w) -> DevBuf* {
-    return w == MGD_CONSERVED ? &r->q : w == MGD_ADJOINT_VARS ? &r->lam : w == MGD_RHS ? &r->rhs
-         : w == MGD_INITIAL_STATE ? &r->init : nullptr;
-  };
-  DevBuf* a = pick(src);
-  DevBuf* b = pick(dst);
-  if (!a || !b || !a->p || !b->p) { set_error("unknown or empty state array"); return MGD_ERR_ARG; }
+  DevBuf* a = state_buf(r, src, false);
+  DevBuf* b = state_buf(r, dst, true);
+  if (!a || !b) { set_error("unknown or empty state array"); return MGD_ERR_ARG; }
   launch_copy((long)r->ninst * M_of(r), a->p, b->p, r->stream);
   MGD_HIP(hipGetLastError());
+  return MGD_OK;
+}
+
+int mgd_state_lincomb(mgd_region* r, int dst, int dst_inst, int a, int a_inst, int b, int b_inst,
+                      double s, int c, int c_inst) {
+  if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
+  auto ok = [&](int i) { return i >= 0 && i < r->ninst; };
+  if (!ok(dst_inst) || !ok(a_inst) || !ok(b_inst) || (c >= 0 && !ok(c_inst))) {
+    set_error("instance out of range");
+    return MGD_ERR_ARG;
+  }
+  DevBuf* A = state_buf(r, a, false);
+  DevBuf* B = state_buf(r, b, false);
+  DevBuf* Cb = c >= 0 ? state_buf(r, c, false) : nullptr;
+  DevBuf* D = state_buf(r, dst, true);
+  if (!A || !B || !D || (c >= 0 && !Cb)) { set_error("unknown or empty state array"); return MGD_ERR_ARG; }
+  const long M = M_of(r);
+  launch_lincomb(M, A->p + a_inst * M, B->p + b_inst * M, s, Cb ? Cb->p + c_inst * M : nullptr,
+                 D->p + dst_inst * M, r->stream);
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  return MGD_OK;
+}
+
+int mgd_inner_product(mgd_region* r, int a, int b, double* out) {
+  if (!r || !r->finalized || !out) { set_error("bad argument"); return MGD_ERR_ARG; }
+  DevBuf* A = state_buf(r, a, false);
+  DevBuf* B = state_buf(r, b, false);
+  if (!A || !B) { set_error("unknown or empty state array"); return MGD_ERR_ARG; }
+  const long nb = (r->N + 255) / 256;
+  if (r->ipart.alloc((size_t)nb * r->ninst) || r->iout.alloc(r->ninst)) return MGD_ERR_HIP;
+  launch_inner_product(r->geo, A->p, B->p, r->ipart.p, r->iout.p, r->stream);
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipMemcpyAsync(out, r->iout.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost, r->stream));
+  MGD_HIP(hipStreamSynchronize(r->stream));
   return MGD_OK;
 }
 
@@ -1114,7 +1346,8 @@ int mgd_compute_rhs(mgd_region* r, int mode, double time, const double* control,
       ctrl = r->control.p;
     }
     launch_rhs_forward(r->geo, dpatches(r), r->q.p, r->rhs.p, r->w.p, r->G.p, r->t.p, ctrl, na,
-                       reinterpret_cast<const Source*>(r->d_sources.p), ns, r->src_amp.p, r->stream);
+                       reinterpret_cast<const Source*>(r->d_sources.p), ns, r->src_amp.p,
+                       r->src_inst_stride, r->stream);
   } else if (mode == MGD_ADJOINT) {
     if (int rc = ensure_adjoint(r)) return rc;
     const int nc = n_cost(r);
@@ -1169,7 +1402,7 @@ int mgd_run_forward(mgd_region* r, double t0, double dt, int nsteps, const doubl
     r->control.alloc(0);
   }
   std::vector<double> amp;
-  source_table(r, stage_times(t0, dt, std::max(nsteps, 1)), amp);
+  run_source_table(r, t0, dt, std::max(nsteps, 1), amp);
   if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
   launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
   const long M = (long)r->ninst * M_of(r);
@@ -1217,103 +1450,34 @@ int mgd_run_adjoint(mgd_region* r, double t0, double dt, int nsteps, int save_in
   (void)use_graph;
   if (int rc = ensure_adjoint(r)) return rc;
   if (int rc = maybe_tune(r)) return rc;
-  const int K = save_interval;
-  const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
-  const long Mi = M_of(r);
-  const long M = (long)r->ninst * Mi;
-  if (control && na > 0) {
-    std::vector<double> c(control, control + (size_t)r->ninst * nst * na);
-    if (upload(r->control, c)) return MGD_ERR_HIP;
-    r->ctrl_stride = (long)nst * na;
-  } else if (r->control.p) {
-    r->control.alloc(0);
-  }
-  if (na > 0 && r->grad.alloc((size_t)r->ninst * nst * na)) return MGD_ERR_HIP;
-  std::vector<double> amp;
-  source_table(r, stage_times(t0, dt, nsteps), amp);
-  if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
-  const int nsnap = nsteps / K + 1;
-  if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M) || r->qs.alloc(M))
-    return MGD_ERR_HIP;
-  launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
-  launch_fill(r->ninst, 0.0, r->acc_S.p, r->stream);
-  if (fused_ok(r)) return run_adjoint_fused(r, dt, nsteps, K, gradient, sensitivity, J);
   MGD_HIP(hipEventRecord(r->ev0, r->stream));
-  // forward run with snapshots at t0 and every K steps (showProgress .q files)
-  launch_copy(M, r->q.p, r->snaps.p, r->stream);
-  for (int s = 0; s < nsteps; ++s) {
-    for (int i = 1; i <= 4; ++i) {
-      forward_stage(r, r->q.p, r->b1.p, r->b2.p, 4 * s + i - 1, i, dt, nullptr);
-      if (nc > 0)
-        launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), nc, 0, r->q.p, r->partial.p,
-                            kRkNorm[i - 1] * dt, r->acc_J.p, nullptr, r->stream);
-    }
-    if ((s + 1) % K == 0) launch_copy(M, r->q.p, r->snaps.p + (long)((s + 1) / K) * M, r->stream);
-  }
-  // adjoint terminal condition (loadInitialCondition ADJOINT, src/SolverImpl.f90:368-428)
-  launch_fill(M, 0.0, r->lam.p, r->stream);
-  if (nc > 0) {
-    launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->q.p, r->aforce.p, 0, r->stream);
-    launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -dt / 6.0, r->lam.p, r->stream);
-  }
-  int loaded = -1;
-  for (int t = nsteps - 1; t >= 0; --t) {
-    for (int i = 4; i >= 1; --i) {
-      const int ts = i == 1 ? t : t + 1;
-      const int st = i == 1 ? 4 : i - 1;
-      const bool need = loaded == -1 || ts < loaded || ts > loaded + K ||
-                        (ts == loaded && st < 4) || (ts == loaded + K && st == 4);
-      if (need) {
-        int lt;
-        if (ts % K == 0 && st == 4) lt = ts;
-        else if (ts % K == 0) lt = ts - K;
-        else lt = ts - ts % K;
-        loaded = lt;
-        launch_copy(M, r->snaps.p + (long)(lt / K) * M, r->qs.p, r->stream);
-        launch_copy(M, r->qs.p, r->window.p, r->stream);
-        if (loaded != nsteps) {
-          int idx = 1;
-          for (int s2 = loaded + 1; s2 <= loaded + K && s2 <= nsteps; ++s2)
-            for (int st2 = 1; st2 <= 4; ++st2) {
-              const bool last = s2 == loaded + K && st2 == 4;
-              forward_stage(r, r->qs.p, r->b1.p, r->b2.p, 4 * (s2 - 1) + st2 - 1, st2, dt,
-                            last ? nullptr : r->window.p + (long)idx * M);
-              if (last) break;
-              ++idx;
-            }
-        }
-      }
-      const int widx = (ts - 1 - loaded) * 4 + st;
-      const double* qcur = r->window.p + (long)widx * M;
-      const int sidx = 4 * t + i - 1;
-      if (na > 0) {
-        if (gradient)
-          launch_gradient(r->geo, dpatch(r, r->ip_act), na, r->lam.p, r->grad.p + (long)sidx * na,
-                          (long)nst * na, r->stream);
-        launch_patch_reduce(r->geo, dpatch(r, r->ip_act), na, 1, r->lam.p, r->partial.p,
-                            kRkNorm[i - 1] * dt, r->acc_S.p, nullptr, r->stream);
-      }
-      if (nc > 0)
-        launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, qcur, r->aforce.p, t == 0 && i == 1,
-                           r->stream);
-      const double factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
-      launch_rhs_adjoint(r->geo, dpatches(r), qcur, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p,
-                         r->S.p, nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc, factor,
-                         r->has_farfield, r->stream);
-      launch_rk_adjoint(M, i, dt, r->lam.p, r->b1a.p, r->b2a.p, r->rhs.p, r->stream);
-    }
-  }
-  MGD_HIP(hipEventRecord(r->ev1, r->stream));
-  MGD_HIP(hipGetLastError());
-  MGD_HIP(hipStreamSynchronize(r->stream));
-  float ms = 0.f;
-  MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
-  r->last_ms[0] = ms;
+  if (int rc = forward_checkpointed(r, t0, dt, nsteps, save_interval, control)) return rc;
+  if (int rc = adjoint_checkpointed(r, false, gradient != nullptr)) return rc;
+  if (int rc = finish_timed(r)) return rc;
   if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
-  if (sensitivity) MGD_HIP(hipMemcpy(sensitivity, r->acc_S.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
-  if (gradient && na > 0)
-    MGD_HIP(hipMemcpy(gradient, r->grad.p, sizeof(double) * r->ninst * nst * na, hipMemcpyDeviceToHost));
+  return copy_adjoint_outputs(r, gradient, sensitivity);
+}
+
+int mgd_run_forward_checkpointed(mgd_region* r, double t0, double dt, int nsteps, int save_interval,
+                                 const double* control, double* J) {
+  if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
+  if (nsteps <= 0 || save_interval <= 0 || !(dt > 0.0)) { set_error("bad time stepping"); return MGD_ERR_ARG; }
+  if (int rc = ensure_adjoint(r)) return rc;
+  if (int rc = maybe_tune(r)) return rc;
+  MGD_HIP(hipEventRecord(r->ev0, r->stream));
+  if (int rc = forward_checkpointed(r, t0, dt, nsteps, save_interval, control)) return rc;
+  if (int rc = finish_timed(r)) return rc;
+  if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
   return MGD_OK;
+}
+
+int mgd_run_adjoint_checkpointed(mgd_region* r, int terminal, double* gradient, double* sensitivity) {
+  if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
+  if (terminal != 0 && terminal != 1) { set_error("terminal must be 0 or 1"); return MGD_ERR_ARG; }
+  MGD_HIP(hipEventRecord(r->ev0, r->stream));
+  if (int rc = adjoint_checkpointed(r, terminal == 1, gradient != nullptr)) return rc;
+  if (int rc = finish_timed(r)) return rc;
+  return copy_adjoint_outputs(r, gradient, sensitivity);
 }
 
 int mgd_stencil_apply(mgd_region* r, int op, int dir, double* host, int ncomp) {
@@ -1415,7 +1579,7 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
       case MGD_KERNEL_RHS_FORWARD:
         launch_rhs_forward(r->geo, dpatches(r), r->q.p, r->rhs.p, r->w.p, r->G.p, r->t.p, nullptr,
                            n_act(r), reinterpret_cast<const Source*>(r->d_sources.p),
-                           (int)r->sources.size(), r->src_amp.p, r->stream);
+                           (int)r->sources.size(), r->src_amp.p, r->src_inst_stride, r->stream);
         break;
       case MGD_KERNEL_RHS_ADJOINT:
         launch_rhs_adjoint(r->geo, dpatches(r), r->q.p, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,

@@ -112,6 +112,11 @@ double orc_run_forward(orc_region* r, double* q, double t0, double dt, int nstep
 double orc_run_adjoint(orc_region* r, const double* q0, double t0, double dt, int nsteps,
                        int save_interval, const double* control, double* gradient,
                        double* lambda0, double* J_out);
+/* The same with a given adjoint terminal state lambdaT (NULL: zero) and the final forward
+ * state returned in qT_out (may be NULL). */
+double orc_run_adjoint_ic(orc_region* r, const double* q0, double t0, double dt, int nsteps,
+                          int save_interval, const double* control, const double* lambdaT,
+                          double* gradient, double* lambda0, double* J_out, double* qT_out);
 
 /* CNSHelper pointwise routines (src/CNSHelperImpl.f90) exposed for unit tests. */
 void orc_inviscid_jacobian(int nd, const double* q, const double* metrics, double gamma,

@@ -127,6 +127,16 @@ double orc_run_forward(orc_region* r, double* q, double t0, double dt, int nstep
 double orc_run_adjoint(orc_region* r, const double* q0, double t0, double dt, int nsteps,
                        int save_interval, const double* control, double* gradient,
                        double* lambda0, double* J_out) {
+  return orc_run_adjoint_ic(r, q0, t0, dt, nsteps, save_interval, control, NULL, gradient,
+                            lambda0, J_out, NULL);
+}
+
+/* Solver%runAdjoint with adjoint_nonzero_initial_condition (src/SolverImpl.f90:1096-1102,
+ * loadInitialCondition ADJOINT :368-428): lambda(T) = lambdaT (NULL: zero) plus the
+ * terminal cost forcing; qT_out (may be NULL) receives the final forward state. */
+double orc_run_adjoint_ic(orc_region* r, const double* q0, double t0, double dt, int nsteps,
+                          int save_interval, const double* control, const double* lambdaT,
+                          double* gradient, double* lambda0, double* J_out, double* qT_out) {
   const long M = r->N * r->nv;
   const int K = save_interval;
   const int nsnap = nsteps / K + 2;
@@ -153,11 +163,13 @@ double orc_run_adjoint(orc_region* r, const double* q0, double t0, double dt, in
     }
   }
   if (J_out) *J_out = J;
+  if (qT_out) memcpy(qT_out, q, sizeof(double) * M);
   /* adjoint terminal condition (loadInitialCondition ADJOINT) on the final state */
   const int nct = orc_cost_target_npoints(r);
   const int nv = r->nv;
   double* aforce = calloc((size_t)(nct > 0 ? nct : 1) * nv, sizeof(double));
   double* lam = calloc(M, sizeof(double));
+  if (lambdaT) memcpy(lam, lambdaT, sizeof(double) * M);
   if (nct > 0) {
     orc_adjoint_forcing(r, q, aforce);
     double* rhs = calloc(M, sizeof(double));

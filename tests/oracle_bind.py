@@ -97,6 +97,9 @@ def lib():
         L.orc_run_adjoint.restype = C.c_double
         L.orc_run_adjoint.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_int, C.c_int, dp,
                                       dp, dp, dp]
+        L.orc_run_adjoint_ic.restype = C.c_double
+        L.orc_run_adjoint_ic.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                         dp, dp, dp, dp, dp, dp]
         for f in ("orc_inviscid_jacobian", "orc_incoming_jacobian", "orc_inviscid_flux"):
             getattr(L, f).restype = None
         L.orc_inviscid_jacobian.argtypes = [C.c_int, dp, dp, C.c_double, dp]
@@ -240,3 +243,18 @@ class Region:
         sens = lib().orc_run_adjoint(self.h, _p(q0), t0, dt, nsteps, save_interval, _p(ctl),
                                      _p(grad), _p(lam0), C.cast(C.pointer(J), C.POINTER(C.c_double)))
         return sens, grad[:, :na], lam0, J.value
+
+    def run_adjoint_ic(self, q0, dt, nsteps, save_interval, control=None, t0=0.0, lamT=None):
+        """run_adjoint with a nonzero adjoint terminal state; also returns the final state."""
+        q0 = np.ascontiguousarray(q0, dtype=np.float64)
+        na = self.n_actuator
+        grad = np.zeros((nsteps * 4, max(na, 1)))
+        lam0 = np.zeros((self.nv, self.N))
+        qT = np.zeros((self.nv, self.N))
+        J = C.c_double(0.0)
+        ctl = None if control is None else np.ascontiguousarray(control, dtype=np.float64)
+        lt = None if lamT is None else np.ascontiguousarray(lamT, dtype=np.float64)
+        sens = lib().orc_run_adjoint_ic(self.h, _p(q0), t0, dt, nsteps, save_interval, _p(ctl),
+                                        _p(lt), _p(grad), _p(lam0),
+                                        C.cast(C.pointer(J), C.POINTER(C.c_double)), _p(qT))
+        return sens, grad[:, :na], lam0, J.value, qT
