@@ -16,7 +16,8 @@ module magudi_hip
   integer(c_int), parameter, public :: MGD_FORWARD = 1, MGD_ADJOINT = -1
   integer(c_int), parameter, public :: MGD_PATCH_FAR_FIELD = 1, MGD_PATCH_SPONGE = 2,         &
        MGD_PATCH_ACTUATOR = 3, MGD_PATCH_COST_TARGET = 4, MGD_PATCH_KOLMOGOROV = 5
-  integer(c_int), parameter, public :: MGD_CONSERVED = 0, MGD_ADJOINT_VARS = 1, MGD_RHS = 2
+  integer(c_int), parameter, public :: MGD_CONSERVED = 0, MGD_ADJOINT_VARS = 1, MGD_RHS = 2,   &
+       MGD_INITIAL_STATE = 4, MGD_FINAL_STATE = 5, MGD_SCRATCH = 6
   integer(c_int), parameter, public :: MGD_FIELD_TARGET_STATE = 0,                            &
        MGD_FIELD_CONTROL_MOLLIFIER = 1, MGD_FIELD_TARGET_MOLLIFIER = 2
 
@@ -131,8 +132,51 @@ module magudi_hip
        real(c_double), intent(out) :: sensitivity(*), J(*)
        integer(c_int) :: rc
      end function mgd_run_adjoint
+     function mgd_run_forward_checkpointed(r, t0, dt, nsteps, save_interval, control, J)      &
+          bind(C, name="mgd_run_forward_checkpointed") result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: r
+       real(c_double), value :: t0, dt
+       integer(c_int), value :: nsteps, save_interval
+       type(c_ptr), value :: control
+       real(c_double), intent(out) :: J(*)
+       integer(c_int) :: rc
+     end function mgd_run_forward_checkpointed
+     function mgd_run_adjoint_checkpointed(r, terminal, gradient, sensitivity)                &
+          bind(C, name="mgd_run_adjoint_checkpointed") result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: r
+       integer(c_int), value :: terminal
+       type(c_ptr), value :: gradient
+       real(c_double), intent(out) :: sensitivity(*)
+       integer(c_int) :: rc
+     end function mgd_run_adjoint_checkpointed
+     function mgd_state_lincomb(r, dst, dst_inst, a, a_inst, b, b_inst, s, c, c_inst)         &
+          bind(C, name="mgd_state_lincomb") result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: r
+       integer(c_int), value :: dst, dst_inst, a, a_inst, b, b_inst, c, c_inst
+       real(c_double), value :: s
+       integer(c_int) :: rc
+     end function mgd_state_lincomb
+     function mgd_inner_product(r, a, b, out) bind(C, name="mgd_inner_product") result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: r
+       integer(c_int), value :: a, b
+       real(c_double), intent(out) :: out(*)
+       integer(c_int) :: rc
+     end function mgd_inner_product
+     function mgd_region_set_time_offsets(r, offsets)                                          &
+          bind(C, name="mgd_region_set_time_offsets") result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: r
+       real(c_double), intent(in) :: offsets(*)
+       integer(c_int) :: rc
+     end function mgd_region_set_time_offsets
   end interface
 
+  public :: mgd_run_forward_checkpointed, mgd_run_adjoint_checkpointed, mgd_state_lincomb,      &
+       mgd_inner_product, mgd_region_set_time_offsets
   public :: mgd_last_error, mgd_region_create, mgd_region_destroy, mgd_region_add_patch,      &
        mgd_region_set_field, mgd_region_add_source, mgd_region_finalize, mgd_state_put,         &
        mgd_state_get, mgd_compute_rhs, mgd_run_forward, mgd_run_adjoint
