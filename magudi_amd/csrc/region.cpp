@@ -57,6 +57,11 @@ struct HostPatch {
 }  // namespace
 }  // namespace mgd
 
+struct GraphSlot {
+  hipGraphExec_t exec = nullptr;
+  std::vector<uintptr_t> key;
+};
+
 struct mgd_region {
   mgd_region_desc desc;
   int nd, nv, ninst;
@@ -108,6 +113,7 @@ struct mgd_region {
   } ck;
   mgd::DevBuf qT, scr;                   // final forward state; scratch state (MGD_SCRATCH)
   mgd::DevBuf ipart, iout;               // inner-product partial sums and results
+  GraphSlot graphs[2];                   // captured runs: forward, forward + adjoint
   std::vector<double> toff;              // per-instance time offsets (empty: none)
   long src_inst_stride = 0;              // source table entries per instance (0: shared)
   double last_ms[3] = {0, 0, 0};
@@ -631,7 +637,6 @@ int maybe_tune(mgd_region* r) {
 int fused_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
   const int nc = n_cost(r), nst = 4 * nsteps;
   const long M = (long)r->ninst * M_of(r);
-  if (int rc = prepare_quadrature(r, nst, dt)) return rc;
   launch_copy(M, r->q.p, r->snaps.p, r->stream);
   for (int s = 0; s < nsteps; ++s) {
     fused_forward_step(r, r->q.p, r->b1.p, r->qy.p, r->b2.p, s, dt, nst, true, nullptr);
@@ -644,10 +649,6 @@ int fused_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
 int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool terminal, bool want_grad) {
   const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
   const long M = (long)r->ninst * M_of(r);
-  if (int rc = prepare_quadrature(r, nst, dt)) return rc;
-  if (r->ly.alloc(M) || r->l0.alloc(M) || r->window2.alloc((size_t)4 * K * M) || r->b2r.alloc(M) ||
-      r->qsr.alloc(M))
-    return MGD_ERR_HIP;
   MGD_HIP(hipEventRecord(r->ev_fwd, r->stream));
   MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_fwd, 0));
   double* wbuf[2] = {r->window.p, r->window2.p};
@@ -817,8 +818,10 @@ int upload_run_inputs(mgd_region* r) {
 }
 
 // Forward pass with HBM snapshots (first half of Solver%runAdjoint; the snapshots play
-// the part of the .q files the reference writes every save_interval steps).
-int forward_checkpointed(mgd_region* r, double t0, double dt, int nsteps, int K, const double* control) {
+// the part of the .q files the reference writes every save_interval steps).  Split in a
+// preparation (allocations and host uploads) and the device work, which can be captured
+// into a HIP graph.
+int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, const double* control) {
   if (int rc = ensure_adjoint(r)) return rc;
   if (int rc = maybe_tune(r)) return rc;
   const int na = n_act(r), nst = 4 * nsteps;
@@ -835,23 +838,110 @@ int forward_checkpointed(mgd_region* r, double t0, double dt, int nsteps, int K,
   if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M) || r->qs.alloc(M) ||
       r->qT.alloc(M))
     return MGD_ERR_HIP;
-  launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
-  const int rc = fused_ok(r) ? fused_forward_pass(r, dt, nsteps, K) : plain_forward_pass(r, dt, nsteps, K);
-  if (rc) return rc;
-  launch_copy(M, r->q.p, r->qT.p, r->stream);   // final state: terminal forcing, MGD_FINAL_STATE
-  r->ck.valid = true;
+  if (fused_ok(r))
+    if (int rc = prepare_quadrature(r, nst, dt)) return rc;
   return MGD_OK;
 }
 
-// Adjoint pass from the snapshots of the last forward_checkpointed.
-int adjoint_checkpointed(mgd_region* r, bool terminal, bool want_grad) {
+int enqueue_forward_ck(mgd_region* r) {
+  const long M = (long)r->ninst * M_of(r);
+  launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
+  const int rc = fused_ok(r) ? fused_forward_pass(r, r->ck.dt, r->ck.nsteps, r->ck.K)
+                             : plain_forward_pass(r, r->ck.dt, r->ck.nsteps, r->ck.K);
+  if (rc) return rc;
+  launch_copy(M, r->q.p, r->qT.p, r->stream);   // final state: terminal forcing, MGD_FINAL_STATE
+  return MGD_OK;
+}
+
+// Adjoint pass from the snapshots of the last checkpointed forward pass.
+int prepare_adjoint_ck(mgd_region* r) {
   if (!r->ck.valid) { set_error("no checkpointed forward run to march back from"); return MGD_ERR_STATE; }
   const int na = n_act(r), nst = 4 * r->ck.nsteps;
+  const long M = (long)r->ninst * M_of(r);
   if (int rc = upload_run_inputs(r)) return rc;
   if (na > 0 && r->grad.alloc((size_t)r->ninst * nst * na)) return MGD_ERR_HIP;
+  if (fused_ok(r)) {
+    if (int rc = prepare_quadrature(r, nst, r->ck.dt)) return rc;
+    if (r->ly.alloc(M) || r->l0.alloc(M) || r->window2.alloc((size_t)4 * r->ck.K * M) ||
+        r->b2r.alloc(M) || r->qsr.alloc(M))
+      return MGD_ERR_HIP;
+  }
+  return MGD_OK;
+}
+
+int enqueue_adjoint_ck(mgd_region* r, bool terminal, bool want_grad) {
   launch_fill(r->ninst, 0.0, r->acc_S.p, r->stream);
   return fused_ok(r) ? fused_adjoint_pass(r, r->ck.dt, r->ck.nsteps, r->ck.K, terminal, want_grad)
                      : plain_adjoint_pass(r, r->ck.dt, r->ck.nsteps, r->ck.K, terminal, want_grad);
+}
+
+// Forward run without snapshots (Solver%runForward): device work of one run.
+int enqueue_forward_run(mgd_region* r, double dt, int nsteps) {
+  const int nst = 4 * nsteps;
+  launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
+  if (fused_ok(r)) {
+    for (int s = 0; s < nsteps; ++s)
+      fused_forward_step(r, r->q.p, r->b1.p, r->qy.p, r->b2.p, s, dt, nst, true, nullptr);
+    if (n_cost(r) > 0)
+      launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, r->stream);
+    return MGD_OK;
+  }
+  for (int s = 0; s < nsteps; ++s)
+    for (int i = 1; i <= 4; ++i) {
+      const int sidx = 4 * s + i - 1;
+      forward_stage(r, r->q.p, r->b1.p, r->b2.p, sidx, i, dt, nullptr);
+      if (r->ip_cost >= 0)
+        launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), n_cost(r), 0, r->q.p, r->partial.p,
+                            kRkNorm[i - 1] * dt, r->acc_J.p, nullptr, r->stream);
+    }
+  return MGD_OK;
+}
+
+// HIP graph of a run's device work, replayed while the run parameters and every device
+// buffer the work touches stay the same (the key); inputs that change between runs
+// (control, source amplitudes, states) live in those buffers and are re-uploaded before
+// each replay.  Captured work holds kernels, copies, fills and event waits only.
+std::vector<uintptr_t> graph_key(mgd_region* r, int kind, std::initializer_list<uintptr_t> extra) {
+  std::vector<uintptr_t> k = {(uintptr_t)kind, (uintptr_t)r->tiling.shape, (uintptr_t)fused_ok(r),
+                              (uintptr_t)r->ctrl_stride, (uintptr_t)r->src_inst_stride,
+                              (uintptr_t)r->chunk, (uintptr_t)r->ck.nsteps, (uintptr_t)r->ck.K};
+  const DevBuf* bufs[] = {&r->q, &r->lam, &r->rhs, &r->b1, &r->b2, &r->b1a, &r->b2a, &r->t, &r->w,
+                          &r->G, &r->T1, &r->S, &r->qs, &r->aforce, &r->control, &r->grad,
+                          &r->acc_J, &r->acc_S, &r->partial, &r->src_amp, &r->snaps, &r->window,
+                          &r->qy, &r->ly, &r->l0, &r->jpart, &r->spart, &r->ssum, &r->wts,
+                          &r->window2, &r->b2r, &r->qsr, &r->qT, &r->d_tile_org, &r->d_tile_size,
+                          &r->d_tile_mask, &r->d_tile_blk};
+  for (const DevBuf* b : bufs) k.push_back((uintptr_t)b->p);
+  for (uintptr_t e : extra) k.push_back(e);
+  return k;
+}
+
+template <class F>
+int run_device_work(mgd_region* r, bool use_graph, int slot, const std::vector<uintptr_t>& key,
+                    F&& enqueue) {
+  if (!use_graph) return enqueue();
+  GraphSlot& gs = r->graphs[slot];
+  if (!gs.exec || gs.key != key) {
+    if (gs.exec) {
+      (void)hipGraphExecDestroy(gs.exec);
+      gs.exec = nullptr;
+    }
+    MGD_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(r->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    MGD_HIP(e);
+    const hipError_t ei = hipGraphInstantiate(&gs.exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    MGD_HIP(ei);
+    gs.key = key;
+  }
+  MGD_HIP(hipGraphLaunch(gs.exec, r->stream));
+  return MGD_OK;
 }
 
 int finish_timed(mgd_region* r) {
@@ -1005,6 +1095,8 @@ int mgd_region_destroy(mgd_region* r) {
   if (!r) return MGD_OK;
   if (r->stream) (void)hipStreamSynchronize(r->stream);
   if (r->stream_rec) (void)hipStreamSynchronize(r->stream_rec);
+  for (GraphSlot& gs : r->graphs)
+    if (gs.exec) (void)hipGraphExecDestroy(gs.exec);
   if (r->ev0) (void)hipEventDestroy(r->ev0);
   if (r->ev1) (void)hipEventDestroy(r->ev1);
   if (r->ev_fwd) (void)hipEventDestroy(r->ev_fwd);
@@ -1391,7 +1483,6 @@ int mgd_run_forward(mgd_region* r, double t0, double dt, int nsteps, const doubl
                     int use_graph) {
   if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
   if (nsteps < 0 || !(dt > 0.0)) { set_error("bad time stepping"); return MGD_ERR_ARG; }
-  (void)use_graph;
   if (int rc = maybe_tune(r)) return rc;
   const int na = n_act(r), nst = 4 * nsteps;
   if (control && na > 0) {
@@ -1404,40 +1495,15 @@ int mgd_run_forward(mgd_region* r, double t0, double dt, int nsteps, const doubl
   std::vector<double> amp;
   run_source_table(r, t0, dt, std::max(nsteps, 1), amp);
   if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
-  launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
-  const long M = (long)r->ninst * M_of(r);
-  if (fused_ok(r)) {
-    if (int rc = prepare_quadrature(r, nst, dt)) return rc;
-    MGD_HIP(hipEventRecord(r->ev0, r->stream));
-    for (int s = 0; s < nsteps; ++s)
-      fused_forward_step(r, r->q.p, r->b1.p, r->qy.p, r->b2.p, s, dt, nst, true, nullptr);
-    if (n_cost(r) > 0)
-      launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, r->stream);
-    MGD_HIP(hipEventRecord(r->ev1, r->stream));
-    MGD_HIP(hipGetLastError());
-    MGD_HIP(hipStreamSynchronize(r->stream));
-    float ms = 0.f;
-    MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
-    r->last_ms[0] = ms;
-    if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
-    return MGD_OK;
-  }
+  if (fused_ok(r))
+    if (int rc = prepare_quadrature(r, std::max(nst, 1), dt)) return rc;
+  uint64_t dtb;
+  std::memcpy(&dtb, &dt, sizeof dtb);
+  const auto key = graph_key(r, 0, {(uintptr_t)nsteps, (uintptr_t)dtb});
   MGD_HIP(hipEventRecord(r->ev0, r->stream));
-  for (int s = 0; s < nsteps; ++s)
-    for (int i = 1; i <= 4; ++i) {
-      const int sidx = 4 * s + i - 1;
-      forward_stage(r, r->q.p, r->b1.p, r->b2.p, sidx, i, dt, nullptr);
-      if (r->ip_cost >= 0)
-        launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), n_cost(r), 0, r->q.p, r->partial.p,
-                            kRkNorm[i - 1] * dt, r->acc_J.p, nullptr, r->stream);
-    }
-  MGD_HIP(hipEventRecord(r->ev1, r->stream));
-  MGD_HIP(hipGetLastError());
-  MGD_HIP(hipStreamSynchronize(r->stream));
-  float ms = 0.f;
-  MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
-  r->last_ms[0] = ms;
-  (void)M;
+  if (int rc = run_device_work(r, use_graph != 0, 0, key, [&] { return enqueue_forward_run(r, dt, nsteps); }))
+    return rc;
+  if (int rc = finish_timed(r)) return rc;
   if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
   return MGD_OK;
 }
@@ -1447,12 +1513,19 @@ int mgd_run_adjoint(mgd_region* r, double t0, double dt, int nsteps, int save_in
                     int use_graph) {
   if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
   if (nsteps <= 0 || save_interval <= 0 || !(dt > 0.0)) { set_error("bad time stepping"); return MGD_ERR_ARG; }
-  (void)use_graph;
-  if (int rc = ensure_adjoint(r)) return rc;
-  if (int rc = maybe_tune(r)) return rc;
+  if (int rc = prepare_forward_ck(r, t0, dt, nsteps, save_interval, control)) return rc;
+  r->ck.valid = true;
+  if (int rc = prepare_adjoint_ck(r)) return rc;
+  uint64_t dtb;
+  std::memcpy(&dtb, &dt, sizeof dtb);
+  const bool want_grad = gradient != nullptr;
+  const auto key = graph_key(r, 1, {(uintptr_t)dtb, (uintptr_t)want_grad});
   MGD_HIP(hipEventRecord(r->ev0, r->stream));
-  if (int rc = forward_checkpointed(r, t0, dt, nsteps, save_interval, control)) return rc;
-  if (int rc = adjoint_checkpointed(r, false, gradient != nullptr)) return rc;
+  if (int rc = run_device_work(r, use_graph != 0, 1, key, [&] {
+        if (int e = enqueue_forward_ck(r)) return e;
+        return enqueue_adjoint_ck(r, false, want_grad);
+      }))
+    return rc;
   if (int rc = finish_timed(r)) return rc;
   if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
   return copy_adjoint_outputs(r, gradient, sensitivity);
@@ -1462,11 +1535,11 @@ int mgd_run_forward_checkpointed(mgd_region* r, double t0, double dt, int nsteps
                                  const double* control, double* J) {
   if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
   if (nsteps <= 0 || save_interval <= 0 || !(dt > 0.0)) { set_error("bad time stepping"); return MGD_ERR_ARG; }
-  if (int rc = ensure_adjoint(r)) return rc;
-  if (int rc = maybe_tune(r)) return rc;
+  if (int rc = prepare_forward_ck(r, t0, dt, nsteps, save_interval, control)) return rc;
   MGD_HIP(hipEventRecord(r->ev0, r->stream));
-  if (int rc = forward_checkpointed(r, t0, dt, nsteps, save_interval, control)) return rc;
+  if (int rc = enqueue_forward_ck(r)) return rc;
   if (int rc = finish_timed(r)) return rc;
+  r->ck.valid = true;
   if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
   return MGD_OK;
 }
@@ -1474,8 +1547,9 @@ int mgd_run_forward_checkpointed(mgd_region* r, double t0, double dt, int nsteps
 int mgd_run_adjoint_checkpointed(mgd_region* r, int terminal, double* gradient, double* sensitivity) {
   if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
   if (terminal != 0 && terminal != 1) { set_error("terminal must be 0 or 1"); return MGD_ERR_ARG; }
+  if (int rc = prepare_adjoint_ck(r)) return rc;
   MGD_HIP(hipEventRecord(r->ev0, r->stream));
-  if (int rc = adjoint_checkpointed(r, terminal == 1, gradient != nullptr)) return rc;
+  if (int rc = enqueue_adjoint_ck(r, terminal == 1, gradient != nullptr)) return rc;
   if (int rc = finish_timed(r)) return rc;
   return copy_adjoint_outputs(r, gradient, sensitivity);
 }
