@@ -35,6 +35,8 @@ extern "C" {
 #define MGD_PATCH_ACTUATOR 3
 #define MGD_PATCH_COST_TARGET 4
 #define MGD_PATCH_KOLMOGOROV 5
+#define MGD_PATCH_IMPENETRABLE_WALL 6   /* SAT_SLIP_WALL (src/ImpenetrableWallImpl.f90) */
+#define MGD_PATCH_ISOTHERMAL_WALL 7     /* SAT_ISOTHERMAL_WALL (src/IsothermalWallImpl.f90) */
 
 /* Grid periodicity (Grid_enum NONE / PLANE / OVERLAP) */
 #define MGD_PERIODIC_NONE 0
@@ -84,7 +86,8 @@ typedef struct {
 
 /* One bc.dat row plus its patch options.  p[]: far field (inviscid_penalty_amount,
  * viscous_penalty_amount); sponge (sponge_amount, sponge_exponent); Kolmogorov
- * (wavenumber, amplitude). */
+ * (wavenumber, amplitude); slip wall (inviscid_penalty_amount); isothermal wall
+ * (inviscid_penalty_amount, viscous_penalty_amount, wall temperature). */
 typedef struct {
   int type;
   int normal_dir;

@@ -27,6 +27,8 @@ SPONGE = 2         # SPONGE
 ACTUATOR = 3       # ACTUATOR
 COST_TARGET = 4    # COST_TARGET
 KOLMOGOROV = 5     # KOLMOGOROV_FORCING
+IMPENETRABLE_WALL = 6   # SAT_SLIP_WALL
+ISOTHERMAL_WALL = 7     # SAT_ISOTHERMAL_WALL
 
 PATCH_TYPE_NAMES = {
     "SAT_FAR_FIELD": FAR_FIELD,
@@ -34,6 +36,8 @@ PATCH_TYPE_NAMES = {
     "ACTUATOR": ACTUATOR,
     "COST_TARGET": COST_TARGET,
     "KOLMOGOROV_FORCING": KOLMOGOROV,
+    "SAT_SLIP_WALL": IMPENETRABLE_WALL,
+    "SAT_ISOTHERMAL_WALL": ISOTHERMAL_WALL,
 }
 
 NONE, PLANE, OVERLAP = 0, 1, 2      # grid periodicity types (Grid_enum)
@@ -239,6 +243,27 @@ def solenoidal_field(n: int, seed: int, rms: float) -> np.ndarray:
     u = np.real(np.fft.ifftn(uh, axes=(1, 2, 3)))
     u *= rms / np.sqrt((u ** 2).sum(axis=0).mean())
     return u.reshape(3, -1)
+
+
+def walled_monopole(n: int = 41, viscous: bool = False, nsteps: int = 8,
+                    save_interval: int = 4) -> Case:
+    """Monopole with walls on the y faces: a slip (impenetrable) wall at the bottom and,
+    for viscous flow, an isothermal no-slip wall at the top (else a slip wall); far
+    field and sponges on the x faces.  Wall patch options: (inviscid_penalty_amount,
+    viscous_penalty_amount, wall temperature)."""
+    c = acoustic_monopole(n=n, nsteps=nsteps, save_interval=save_interval)
+    tw = 1.0 / (c.gamma - 1.0)
+    keep = [p for p in c.patches if p.normal_dir not in (2, -2)]
+    top = ISOTHERMAL_WALL if viscous else IMPENETRABLE_WALL
+    keep[0:0] = [Patch("wall.S", IMPENETRABLE_WALL, 2, (1, -1, 1, 1, 1, -1), (1.0, 0.0, 0, 0)),
+                 Patch("wall.N", top, -2, (1, -1, -1, -1, 1, -1), (1.0, 1.0, tw, 0))]
+    c.patches = keep
+    c.name = f"walled_monopole{n}" + ("_viscous" if viscous else "")
+    if viscous:
+        c.viscous = True
+        c.reynolds = 200.0
+        c.prandtl = 0.7
+    return c
 
 
 def small_monopole(n: int = 41, nsteps: int = 8, save_interval: int = 4) -> Case:

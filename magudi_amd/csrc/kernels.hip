@@ -439,6 +439,57 @@ __global__ void k_finish(Geo g, int mode, const Patches* __restrict__ P, const d
         }
         break;
       }
+      case MGD_PATCH_IMPENETRABLE_WALL:
+      case MGD_PATCH_ISOTHERMAL_WALL: {
+        // addImpenetrableWallPenalty (src/ImpenetrableWallImpl.f90:60-207)
+        double v[NV], m[ND];
+        load_q<ND>(qi, g.N, p, v);
+        const Prim<ND> s = prim<ND>(v, g.gamma);
+        const int dir = abs(pt.normal_dir) - 1;
+        for (int d = 0; d < ND; ++d) m[d] = d == dir ? metric<ND>(g, dir, ijk) : 0.0;
+        double un = 0.0;
+        for (int d = 0; d < ND; ++d) un += v[d + 1] * m[d];
+        const double sj = pt.sigma_inv * jac;
+        if (mode == MGD_FORWARD) {
+          double pen[NV];
+          pen[0] = un;
+          for (int d = 0; d < ND; ++d) pen[d + 1] = un * s.u[d];
+          pen[ND + 1] = un * s.sv * (v[ND + 1] + s.p);
+          for (int l = 0; l < NV; ++l) r[l] = r[l] - sj * pen[l];
+        } else {
+          double A[NV * NV], dp[NV], ke = 0.0;
+          inviscid_jacobian<ND>(m, s, g.gamma, A);
+          for (int d = 0; d < ND; ++d) ke += s.u[d] * s.u[d];
+          dp[0] = 0.5 * ke;
+          for (int d = 0; d < ND; ++d) dp[d + 1] = -s.u[d];
+          dp[ND + 1] = 1.0;
+          for (int l = 0; l < NV; ++l) dp[l] = dp[l] * (g.gamma - 1.0);
+          for (int d = 0; d < ND; ++d)
+            for (int c = 0; c < NV; ++c) A[c * NV + d + 1] = A[c * NV + d + 1] - m[d] * dp[c];
+          for (int i = 0; i < NV; ++i) {
+            double t = 0.0;
+            for (int c = 0; c < NV; ++c) t += A[i * NV + c] * li[c * g.N + p];
+            r[i] = r[i] + sj * t;
+          }
+        }
+        if (pt.type == MGD_PATCH_ISOTHERMAL_WALL && g.viscous) {
+          // addIsothermalWallPenalty (src/IsothermalWallImpl.f90:101-330)
+          double pen[NV];
+          if (mode == MGD_FORWARD) {
+            pen[0] = 0.0;
+            for (int l = 1; l < NV; ++l) pen[l] = v[l];
+            pen[ND + 1] = pen[ND + 1] - v[0] * pt.twall / g.gamma;
+            for (int l = 0; l < NV; ++l) pen[l] = jac * pen[l];
+            for (int l = 0; l < NV; ++l) r[l] = r[l] - pt.sigma_visc * pen[l];
+          } else {
+            pen[0] = -li[(ND + 1) * g.N + p] * pt.twall / g.gamma;
+            for (int l = 1; l < NV; ++l) pen[l] = li[l * g.N + p];
+            for (int l = 0; l < NV; ++l) pen[l] = jac * pen[l];
+            for (int l = 0; l < NV; ++l) r[l] = r[l] + pt.sigma_visc * pen[l];
+          }
+        }
+        break;
+      }
       case MGD_PATCH_SPONGE: {
         // addDamping (src/SpongePatchImpl.f90:65-166)
         const double sp = pt.strength[k];

@@ -72,6 +72,7 @@ struct DPatch {
   int lo[3], hi[3], ls[3];
   int np;
   double sigma_inv, sigma_visc;
+  double twall;             // isothermal wall temperature
   const double* strength;   // sponge strength, Kolmogorov force, or the actuator / cost-target mollifier
   const double* aplus;      // far field: A+ blocks (np x nv x nv, column major) then the target state (np x nv);
                             // sponge: target state (np x nv); cost target: mean pressure (np)

@@ -457,8 +457,16 @@ void forward_stage(mgd_region* r, double* q, double* b1, double* b2, int sidx, i
   launch_rk_forward((long)r->ninst * M_of(r), stage, dt, q, b1, b2, r->rhs.p, qcopy, r->stream);
 }
 
+bool has_walls(const mgd_region* r) {
+  for (const auto& P : r->patches)
+    if (P->pd.type == MGD_PATCH_IMPENETRABLE_WALL || P->pd.type == MGD_PATCH_ISOTHERMAL_WALL) return true;
+  return false;
+}
+
+// The fused 2-D stage kernels cover inviscid 2-D regions with far-field, sponge,
+// actuator, cost-target and Kolmogorov patches; walls run on the generic kernels.
 bool fused_ok(const mgd_region* r) {
-  return !r->force_plain && r->nd == 2 && !r->desc.viscous && r->tiling_ok;
+  return !r->force_plain && r->nd == 2 && !r->desc.viscous && r->tiling_ok && !has_walls(r);
 }
 
 int upload_ints(DevBuf& b, const void* data, size_t bytes) {
@@ -1113,7 +1121,7 @@ int mgd_region_destroy(mgd_region* r) {
 int mgd_region_add_patch(mgd_region* r, const mgd_patch_desc* pd) {
   if (!r || !pd) { set_error("null argument"); return MGD_ERR_ARG; }
   if ((int)r->patches.size() >= kMaxPatches) { set_error("too many patches"); return MGD_ERR_ARG; }
-  if (pd->type < MGD_PATCH_FAR_FIELD || pd->type > MGD_PATCH_KOLMOGOROV) {
+  if (pd->type < MGD_PATCH_FAR_FIELD || pd->type > MGD_PATCH_ISOTHERMAL_WALL) {
     set_error("unsupported patch type");
     return MGD_ERR_UNSUPPORTED;
   }
@@ -1188,8 +1196,20 @@ int mgd_region_finalize(mgd_region* r) {
     HostPatch& P = *r->patches[ip];
     P.strength.clear();
     P.data.clear();
-    double sig_inv = 0.0, sig_visc = 0.0;
+    double sig_inv = 0.0, sig_visc = 0.0, twall = 0.0;
     switch (P.pd.type) {
+      case MGD_PATCH_IMPENETRABLE_WALL:
+      case MGD_PATCH_ISOTHERMAL_WALL: {
+        // setupImpenetrableWall (src/ImpenetrableWallImpl.f90:3-40) and setupIsothermalWall
+        // (src/IsothermalWallImpl.f90:3-80): sign(amount, normal) / H00; viscous amount
+        // / H00 / Re; the wall temperature is the patch option
+        const double h0 = r->first_host.norm[0];
+        sig_inv = std::copysign(P.pd.p[0], (double)P.pd.normal_dir) / h0;
+        if (P.pd.type == MGD_PATCH_ISOTHERMAL_WALL && r->desc.viscous)
+          sig_visc = P.pd.p[1] / h0 * r->geo.re_inv;
+        twall = P.pd.p[2];
+        break;
+      }
       case MGD_PATCH_FAR_FIELD: {
         const int dir = std::abs(P.pd.normal_dir) - 1;
         const double h0 = r->first_host.norm[0];
@@ -1277,6 +1297,7 @@ int mgd_region_finalize(mgd_region* r) {
     D.np = P.np;
     D.sigma_inv = sig_inv;
     D.sigma_visc = sig_visc;
+    D.twall = twall;
     D.strength = P.d_strength.p;
     D.aplus = P.d_data.p;
     D.tvflux = P.d_tvflux.p;

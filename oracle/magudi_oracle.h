@@ -37,7 +37,7 @@ enum { ORC_SBP48 = 0, ORC_SBP36 = 1, ORC_SBP24 = 2 };
 enum { ORC_NONE = 0, ORC_PLANE = 1, ORC_OVERLAP = 2 };
 enum { ORC_FORWARD = 1, ORC_ADJOINT = -1 };
 enum { ORC_FAR_FIELD = 1, ORC_SPONGE = 2, ORC_ACTUATOR = 3, ORC_COST_TARGET = 4,
-       ORC_KOLMOGOROV = 5 };
+       ORC_KOLMOGOROV = 5, ORC_IMPENETRABLE_WALL = 6, ORC_ISOTHERMAL_WALL = 7 };
 
 int  orc_op_setup(orc_op* op, int scheme, int kind);          /* setupOperator */
 void orc_op_adjoint(const orc_op* op, orc_op* adj);           /* getAdjointOperator */

@@ -618,6 +618,15 @@ int orc_region_finalize(orc_region* r) {
       const double h0 = r->first[dir].norm[0];
       P->sigma_inv = copysign(P->pd.p[0], (double)P->pd.normal_dir) / h0;
       P->sigma_visc = r->d.viscous ? copysign(P->pd.p[1], (double)P->pd.normal_dir) / h0 : 0.0;
+    } else if (P->pd.type == ORC_IMPENETRABLE_WALL || P->pd.type == ORC_ISOTHERMAL_WALL) {
+      /* setupImpenetrableWall (src/ImpenetrableWallImpl.f90:3-40): sign(amount, normal) / H00;
+       * setupIsothermalWall (src/IsothermalWallImpl.f90:3-80): viscous amount / H00 / Re */
+      const int dir = abs(P->pd.normal_dir) - 1;
+      const double h0 = r->first[dir].norm[0];
+      P->sigma_inv = copysign(P->pd.p[0], (double)P->pd.normal_dir) / h0;
+      P->sigma_visc = 0.0;
+      if (P->pd.type == ORC_ISOTHERMAL_WALL && r->d.viscous)
+        P->sigma_visc = P->pd.p[1] / h0 * r->re_inv;
     } else if (P->pd.type == ORC_KOLMOGOROV) {
       const double pi = 4.0 * atan(1.0);
       const int wn = P->pd.p[0] > 0 ? (int)P->pd.p[0] : 0;
@@ -720,6 +729,65 @@ int orc_compute_rhs(orc_region* r, int mode, const double* q, const double* lamb
                 for (int c = 0; c < nv; ++c) s += B[i * nv + c] * lambda[c * N + p];
                 rhs[i * N + p] = rhs[i * N + p] - vj * s;
               }
+            }
+          }
+        }
+        break;
+      }
+      case ORC_IMPENETRABLE_WALL:
+      case ORC_ISOTHERMAL_WALL: {
+        /* addImpenetrableWallPenalty (src/ImpenetrableWallImpl.f90:60-207) */
+        const int dir = abs(P->pd.normal_dir) - 1;
+        const double g = r->d.gamma;
+        for (int k = 0; k < P->np; ++k) {
+          const long p = patch_grid_index(r, P, k);
+          double qp[5], m[3];
+          for (int l = 0; l < nv; ++l) qp[l] = q[l * N + p];
+          for (int i = 0; i < nd; ++i) m[i] = r->metrics[(dir * nd + i) * N + p];
+          double un = 0.0;
+          for (int i = 0; i < nd; ++i) un += qp[i + 1] * m[i];
+          const double sj = P->sigma_inv * r->jacobian[p];
+          if (mode == ORC_FORWARD) {
+            double pen[5];
+            pen[0] = un;
+            for (int i = 0; i < nd; ++i) pen[i + 1] = un * r->vel[i * N + p];
+            pen[nd + 1] = un * r->sv[p] * (qp[nd + 1] + r->p[p]);
+            for (int l = 0; l < nv; ++l) rhs[l * N + p] = rhs[l * N + p] - sj * pen[l];
+          } else {
+            /* d(penalty)/dq: inviscid flux Jacobian along the normal metrics minus the
+             * pressure term of the momentum rows */
+            double A[25], dp[5], ke = 0.0;
+            orc_inviscid_jacobian(nd, qp, m, g, A);
+            for (int i = 0; i < nd; ++i) ke += r->vel[i * N + p] * r->vel[i * N + p];
+            dp[0] = 0.5 * ke;
+            for (int i = 0; i < nd; ++i) dp[i + 1] = -r->vel[i * N + p];
+            dp[nd + 1] = 1.0;
+            for (int l = 0; l < nv; ++l) dp[l] = dp[l] * (g - 1.0);
+            for (int i = 0; i < nd; ++i)
+              for (int c = 0; c < nv; ++c) A[c * nv + (i + 1)] = A[c * nv + (i + 1)] - m[i] * dp[c];
+            for (int i = 0; i < nv; ++i) {
+              double s = 0.0;
+              for (int c = 0; c < nv; ++c) s += A[i * nv + c] * lambda[c * N + p];
+              rhs[i * N + p] = rhs[i * N + p] + sj * s;
+            }
+          }
+          if (P->pd.type == ORC_ISOTHERMAL_WALL && r->d.viscous) {
+            /* addIsothermalWallPenalty (src/IsothermalWallImpl.f90:101-330) */
+            const double tw = P->pd.p[2];
+            const double jac = r->jacobian[p];
+            if (mode == ORC_FORWARD) {
+              double pen[5];
+              pen[0] = 0.0;
+              for (int l = 1; l < nv; ++l) pen[l] = qp[l];
+              pen[nd + 1] = pen[nd + 1] - qp[0] * tw / g;
+              for (int l = 0; l < nv; ++l) pen[l] = jac * pen[l];
+              for (int l = 0; l < nv; ++l) rhs[l * N + p] = rhs[l * N + p] - P->sigma_visc * pen[l];
+            } else {
+              double pen[5];
+              pen[0] = -lambda[(nd + 1) * N + p] * tw / g;
+              for (int l = 1; l < nv; ++l) pen[l] = lambda[l * N + p];
+              for (int l = 0; l < nv; ++l) pen[l] = jac * pen[l];
+              for (int l = 0; l < nv; ++l) rhs[l * N + p] = rhs[l * N + p] + P->sigma_visc * pen[l];
             }
           }
         }

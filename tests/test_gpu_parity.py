@@ -59,6 +59,8 @@ CASES = {
     "monopole41_composite": lambda: cases.acoustic_monopole(n=41, nsteps=1, composite=True),
     "viscous_farfield41": viscous_monopole,
     "kolmogorov16": kolmogorov_small,
+    "walled41": lambda: cases.walled_monopole(n=41),
+    "walled41_viscous": lambda: cases.walled_monopole(n=41, viscous=True),
 }
 
 
@@ -217,3 +219,18 @@ def test_fused_matches_plain_on_a_larger_grid(solver_mod):
     assert abs(s1 - s2) / abs(s2) < GRAD_TOL
     assert maxrel(g1, g2) < GRAD_TOL
     assert maxrel(l1, l2) < GRAD_TOL
+
+
+def test_walled_adjoint_run_parity(solver_mod):
+    """Slip and isothermal walls through a whole forward + adjoint run."""
+    c = cases.walled_monopole(n=41, viscous=True, nsteps=6, save_interval=4)
+    c.dt = 0.2
+    q0 = c.initial_state()
+    R = solver_mod.Region(c)
+    R.states[0].conservedVariables = q0
+    sens, grad, J = solver_mod.Solver(R).runAdjoint()
+    so, go, lo, Jo = ob.Region(c).run_adjoint(q0, c.dt, c.nsteps, c.save_interval)
+    assert abs(J[0] - Jo) <= J_TOL * abs(Jo)
+    assert abs(sens[0] - so) <= GRAD_TOL * abs(so)
+    assert maxrel(grad[0], go) < GRAD_TOL
+    assert maxrel(R.states[0].adjointVariables, lo) < GRAD_TOL

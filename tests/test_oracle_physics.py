@@ -131,6 +131,48 @@ def test_adjoint_relation_viscous_periodic_3d_powerlaw():
     assert errs[4] < errs[0] * 0.05 and errs.min() < 1e-5, errs
 
 
+@pytest.mark.parametrize("viscous", [False, True])
+def test_adjoint_relation_walls(viscous):
+    """Slip and isothermal wall penalties (src/ImpenetrableWallImpl.f90:60-207,
+    src/IsothermalWallImpl.f90:101-330) keep the discrete adjoint relation."""
+    c = cases.walled_monopole(n=41, viscous=viscous)
+    errs = _adjoint_relation(c, 6)
+    assert errs[4] < errs[0] * 0.05 and errs.min() < 1e-5, errs
+
+
+def test_slip_wall_penalty_vanishes_for_tangential_flow():
+    """The slip-wall penalty is proportional to the normal momentum: zero when the flow
+    at the wall is tangential."""
+    c = cases.walled_monopole(n=41)
+    c.patches = [p for p in c.patches if p.type == cases.IMPENETRABLE_WALL]
+    c.dissipation = False
+    c.sources = []
+    r = ob.Region(c)
+    q = c.target.copy()
+    q[1] = 0.1 * q[0]                         # uniform flow along x: tangential to y walls
+    q[3] = q[3] + 0.5 * q[1] ** 2 / q[0]
+    base = cases.walled_monopole(n=41)
+    base.patches = []
+    base.dissipation = False
+    base.sources = []
+    assert np.array_equal(r.rhs(1, q), ob.Region(base).rhs(1, q))
+
+
+def test_isothermal_wall_quiescent_at_wall_temperature():
+    """At rest at the wall temperature T_w = 1/(gamma-1) the no-slip penalty
+    J (0, rho u, rho v, E - rho T_w / gamma) vanishes: the rhs is zero to rounding,
+    while a hotter wall drives the energy at the wall."""
+    c = cases.walled_monopole(n=41, viscous=True)
+    c.sources = []
+    c.dissipation = False
+    r = ob.Region(c)
+    assert np.abs(r.rhs(1, c.target)).max() < 1e-12
+    hot = [cases.Patch(p.name, p.type, p.normal_dir, p.extent, (1.0, 1.0, 1.5 * p.params[2], 0))
+           if p.type == cases.ISOTHERMAL_WALL else p for p in c.patches]
+    c.patches = hot
+    assert np.abs(ob.Region(c).rhs(1, c.target)).max() > 1e-3
+
+
 def test_gradient_fd_relation():
     """(J(A) - J(0)) / A -> Grad0 along the adjoint gradient (README.md:80-157)."""
     c = cases.acoustic_monopole(n=41, nsteps=24, save_interval=5)
