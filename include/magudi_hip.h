@@ -242,6 +242,35 @@ int mgd_region_get_tile_shape(mgd_region* r, int* shape);
 #define MGD_KERNEL_RHS_ADJOINT 4
 int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, double* bytes);
 
+/* PLOT3D whole-format files, single or multi block, either byte order
+ * (src/PLOT3DFormat.c:88-294, src/PLOT3DHelperImpl.f90:554-1286).  Host only.
+ * mgd_plot3d_detect: file_type 0 grid, 1 solution, 2 function (include_function enables
+ * function files); sizes receives nx, ny, nz of up to max_grids blocks.
+ * Grids hold x, y, z (N each, z = 0 in 2-D) and iblank (1 when absent); solutions a
+ * 4-double auxiliary record (timestep, -, -, time) and 5 slots, an nv-component state
+ * using slots 1 .. nv-1 and 5; function files n_scalars components.  Arrays are
+ * component-major with the grid index fastest, as everywhere in this interface. */
+int mgd_plot3d_detect(const char* path, int include_function, int* n_grids, int* file_type,
+                      int* n_scalars, int* has_iblank, int* native_endian, int* sizes,
+                      int max_grids);
+int mgd_plot3d_read_grid(const char* path, int block, double* coords, int* iblank);
+int mgd_plot3d_read_solution(const char* path, int block, int nv, double* state, double* aux);
+int mgd_plot3d_read_function(const char* path, int block, int n_scalars, double* data);
+int mgd_plot3d_write_grid(const char* path, int nd, const int* n, const double* coords,
+                          const int* iblank);
+int mgd_plot3d_write_solution(const char* path, const int* n, int nv, const double* state,
+                              const double* aux);
+int mgd_plot3d_write_function(const char* path, const int* n, int n_scalars, const double* data);
+
+/* Actuator control-forcing and gradient files (src/ActuatorPatchImpl.f90:226-458): raw
+ * native doubles, one record of npoints values per RK stage, latest stage first.  The
+ * arrays here are in forward stage order (as mgd_run_adjoint returns the gradient).
+ * mgd_control_zaxpy: z = a x + y over whole files, y may be NULL (bin/ZAXPY.f90,
+ * src/ControlSpaceAdvancerImpl.f90:3-191). */
+int mgd_control_write(const char* path, int nstages, int npoints, const double* forward_order);
+int mgd_control_read(const char* path, int nstages, int npoints, double* forward_order);
+int mgd_control_zaxpy(const char* z, double a, const char* x, const char* y);
+
 /* Waits for all work of the region. */
 int mgd_sync(mgd_region* r);
 

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmagudi_hip.so")
-SOURCES = ["sbp.cpp", "kernels.hip", "fused2d.hip", "region.cpp"]
+SOURCES = ["sbp.cpp", "kernels.hip", "fused2d.hip", "region.cpp", "plot3d.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = [
