@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--c4-n", type=int, default=256,
                     help="grid of the 3-D viscous Kolmogorov rhs leg (0: skip)")
+    ap.add_argument("--viscous3d-only", action="store_true",
+                    help="only the 3-D viscous rhs leg (its profiling workload)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the HIP-event timing of the stage kernels and the copy kernel "
                          "(the workload of the rocprofv3 counter runs)")
@@ -126,6 +128,9 @@ def main():
     from magudi_amd import capi, cases
     from magudi_amd.solver import Region, Solver
 
+    if a.viscous3d_only:
+        print(json.dumps({"viscous3d": viscous3d_leg(a.c4_n)}), flush=True)
+        return 0
     case = cases.acoustic_monopole(n=a.n, nsteps=a.nsteps, save_interval=a.save_interval)
     q0 = case.initial_state()
     R = Region(case)

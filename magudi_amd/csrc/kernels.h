@@ -6,8 +6,10 @@ namespace mgd {
 
 constexpr int kBlock = 256;
 
+// F: flux scratch (nd * nv * N per instance) -- fluxes computed once per point and
+// differentiated from memory; null recomputes them at every stencil tap
 void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
-                        double* G, double* t, const double* control, long control_stride,
+                        double* G, double* t, double* F, const double* control, long control_stride,
                         const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
                         hipStream_t st);
 void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,

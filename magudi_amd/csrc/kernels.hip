@@ -114,6 +114,52 @@ __global__ void k_flux_div(Geo g, int d, const double* __restrict__ q, const dou
   }
 }
 
+// Contravariant total fluxes of every direction at each point, computed once:
+// F[(d*NV + l)] = Fhat_d,l (the same operations as flux_hat, so the same values).
+template <int ND>
+__global__ void k_flux_all(Geo g, const double* __restrict__ q, const double* __restrict__ G,
+                           double* __restrict__ F) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  constexpr int NV = ND + 2;
+  const double* qi = q + inst * NV * g.N;
+  const double* Gi = G ? G + inst * (ND + 1) * ND * g.N : nullptr;
+  double* Fi = F + inst * ND * NV * g.N;
+  for (int d = 0; d < ND; ++d) {
+    double f[NV];
+    flux_hat<ND>(g, qi, Gi, p, d, f);
+    for (int l = 0; l < NV; ++l) Fi[(d * NV + l) * g.N + p] = f[l];
+  }
+}
+
+// rhs accumulation of -sum_d D_d Fhat_d from the stored fluxes (same sums as k_flux_div).
+template <int ND>
+__global__ void k_div(Geo g, int d, const double* __restrict__ F, double* __restrict__ rhs) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  constexpr int NV = ND + 2;
+  const double* Fd = F + (inst * ND + d) * NV * g.N;
+  double* ri = rhs + inst * NV * g.N;
+  int ijk[3] = {0, 0, 0};
+  unflat<ND>(g, p, ijk);
+  const Dir dd = g.dir[d];
+  const long base = p - (long)ijk[d] * dd.stride;
+  double out[NV];
+  sbp_apply<NV>(g.ops[SLOT_FIRST], dd, ijk[d],
+                [&](int idx, double* f) {
+                  const long pp = base + (long)idx * dd.stride;
+                  for (int l = 0; l < NV; ++l) f[l] = Fd[l * g.N + pp];
+                },
+                out);
+  for (int l = 0; l < NV; ++l) {
+    double s = d == 0 ? out[l] : ri[l * g.N + p] + out[l];
+    if (d == ND - 1) s = 0.0 - s;
+    ri[l * g.N + p] = s;
+  }
+}
+
 // Dissipation, first half: t = D4_d x, scaled by -arcLength_d unless composite
 // (addDissipation, src/RhsHelperImpl.f90:58-77).  With composite dissipation the
 // term is added to rhs directly.
@@ -776,7 +822,7 @@ static dim3 grid_pts(const Geo& g) {
 }
 
 void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
-                        double* G, double* t, const double* control, long control_stride,
+                        double* G, double* t, double* F, const double* control, long control_stride,
                         const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
                         hipStream_t st) {
   const dim3 grid = grid_pts(g);
@@ -785,8 +831,13 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
       k_prim<ND><<<grid, kBlock, 0, st>>>(g, q, w);
       for (int d = 0; d < ND; ++d) k_grad<ND><<<grid, kBlock, 0, st>>>(g, d, w, G);
     }
-    for (int d = 0; d < ND; ++d)
-      k_flux_div<ND><<<grid, kBlock, 0, st>>>(g, d, q, g.viscous ? G : nullptr, rhs);
+    if (F) {
+      k_flux_all<ND><<<grid, kBlock, 0, st>>>(g, q, g.viscous ? G : nullptr, F);
+      for (int d = 0; d < ND; ++d) k_div<ND><<<grid, kBlock, 0, st>>>(g, d, F, rhs);
+    } else {
+      for (int d = 0; d < ND; ++d)
+        k_flux_div<ND><<<grid, kBlock, 0, st>>>(g, d, q, g.viscous ? G : nullptr, rhs);
+    }
     if (g.dissipation) {
       for (int d = 0; d < ND; ++d) {
         k_diss_a<ND><<<grid, kBlock, 0, st>>>(g, d, q, t, rhs, g.diss_amount);

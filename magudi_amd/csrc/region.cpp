@@ -356,7 +356,8 @@ DevBuf* state_buf(mgd_region* r, int which, bool create) {
 
 int ensure_state(mgd_region* r) {
   const size_t M = (size_t)r->ninst * r->nv * r->N;
-  if (r->q.alloc(M) || r->lam.alloc(M) || r->rhs.alloc(M) || r->b1.alloc(M) || r->b2.alloc(M))
+  if (r->q.alloc(M) || r->lam.alloc(M) || r->rhs.alloc(M) || r->b1.alloc(M) || r->b2.alloc(M) ||
+      r->T1.alloc(M * r->nd))
     return MGD_ERR_HIP;
   if (r->desc.dissipation && !r->desc.composite_dissipation && r->t.alloc(M)) return MGD_ERR_HIP;
   if (r->desc.viscous) {
@@ -451,7 +452,7 @@ void forward_stage(mgd_region* r, double* q, double* b1, double* b2, int sidx, i
     cstride = r->ctrl_stride;
   }
   const int ns = (int)r->sources.size();
-  launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p, r->G.p, r->t.p, ctrl, cstride,
+  launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p, ctrl, cstride,
                      reinterpret_cast<const Source*>(r->d_sources.p), ns,
                      r->src_amp.p + (long)sidx * std::max(ns, 1), r->src_inst_stride, r->stream);
   launch_rk_forward((long)r->ninst * M_of(r), stage, dt, q, b1, b2, r->rhs.p, qcopy, r->stream);
@@ -607,24 +608,33 @@ int tune_tiling(mgd_region* r) {
       launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
     }
   };
-  int best = -1;
-  float best_ms = 0.f;
-  for (int sh = 0; sh < kNumTileShapes; ++sh) {
-    const int rc = install_tiling(r, sh);
-    if (rc == MGD_ERR_UNSUPPORTED) continue;
-    if (rc != MGD_OK) return rc;
-    launch_copy(M, r->q.p, r->qs.p, r->stream);
-    launch_copy(M, r->q.p, r->l0.p, r->stream);
-    step_pair();   // warm-up
-    MGD_HIP(hipEventRecord(r->ev0, r->stream));
-    for (int k = 0; k < 2; ++k) step_pair();
-    MGD_HIP(hipEventRecord(r->ev1, r->stream));
-    MGD_HIP(hipGetLastError());
-    MGD_HIP(hipEventSynchronize(r->ev1));
-    float ms = 0.f;
-    MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
-    if (best < 0 || ms < best_ms) { best = sh; best_ms = ms; }
+  // three interleaved rounds over the shapes; each shape keeps its fastest round
+  float best_ms[kNumTileShapes];
+  bool fits[kNumTileShapes];
+  for (int sh = 0; sh < kNumTileShapes; ++sh) { best_ms[sh] = 0.f; fits[sh] = false; }
+  for (int round = 0; round < 3; ++round) {
+    for (int sh = 0; sh < kNumTileShapes; ++sh) {
+      if (round > 0 && !fits[sh]) continue;
+      const int rc = install_tiling(r, sh);
+      if (rc == MGD_ERR_UNSUPPORTED) continue;
+      if (rc != MGD_OK) return rc;
+      launch_copy(M, r->q.p, r->qs.p, r->stream);
+      launch_copy(M, r->q.p, r->l0.p, r->stream);
+      step_pair();   // warm-up
+      MGD_HIP(hipEventRecord(r->ev0, r->stream));
+      for (int k = 0; k < 2; ++k) step_pair();
+      MGD_HIP(hipEventRecord(r->ev1, r->stream));
+      MGD_HIP(hipGetLastError());
+      MGD_HIP(hipEventSynchronize(r->ev1));
+      float ms = 0.f;
+      MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+      if (!fits[sh] || ms < best_ms[sh]) best_ms[sh] = ms;
+      fits[sh] = true;
+    }
   }
+  int best = -1;
+  for (int sh = 0; sh < kNumTileShapes; ++sh)
+    if (fits[sh] && (best < 0 || best_ms[sh] < best_ms[best])) best = sh;
   if (best < 0) { set_error("no tile shape fits the grid"); return MGD_ERR_UNSUPPORTED; }
   return install_tiling(r, best);
 }
@@ -1458,7 +1468,7 @@ int mgd_compute_rhs(mgd_region* r, int mode, double time, const double* control,
       if (upload(r->control, c)) return MGD_ERR_HIP;
       ctrl = r->control.p;
     }
-    launch_rhs_forward(r->geo, dpatches(r), r->q.p, r->rhs.p, r->w.p, r->G.p, r->t.p, ctrl, na,
+    launch_rhs_forward(r->geo, dpatches(r), r->q.p, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p, ctrl, na,
                        reinterpret_cast<const Source*>(r->d_sources.p), ns, r->src_amp.p,
                        r->src_inst_stride, r->stream);
   } else if (mode == MGD_ADJOINT) {
@@ -1672,7 +1682,7 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
         launch_copy(M, r->q.p, r->qs.p, r->stream);
         break;
       case MGD_KERNEL_RHS_FORWARD:
-        launch_rhs_forward(r->geo, dpatches(r), r->q.p, r->rhs.p, r->w.p, r->G.p, r->t.p, nullptr,
+        launch_rhs_forward(r->geo, dpatches(r), r->q.p, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p, nullptr,
                            n_act(r), reinterpret_cast<const Source*>(r->d_sources.p),
                            (int)r->sources.size(), r->src_amp.p, r->src_inst_stride, r->stream);
         break;
