@@ -7,13 +7,18 @@
 
 namespace mgd {
 
+// Grid indices of point p; 32-bit unsigned division (regions hold fewer than 2^31
+// points per instance, checked at creation).
 template <int ND>
 __device__ __forceinline__ void unflat(const Geo& g, long p, int* ijk) {
-  ijk[0] = (int)(p % g.n[0]);
+  const unsigned pu = (unsigned)p;
+  const unsigned n0 = (unsigned)g.n[0];
+  ijk[0] = (int)(pu % n0);
   if (ND > 1) {
-    const long t = p / g.n[0];
-    ijk[1] = (int)(t % g.n[1]);
-    if (ND > 2) ijk[2] = (int)(t / g.n[1]);
+    const unsigned t = pu / n0;
+    const unsigned n1 = (unsigned)g.n[1];
+    ijk[1] = (int)(t % n1);
+    if (ND > 2) ijk[2] = (int)(t / n1);
   }
 }
 

@@ -11,7 +11,7 @@ constexpr int kBlock = 256;
 void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
                         double* G, double* t, double* F, const double* control, long control_stride,
                         const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
-                        hipStream_t st);
+                        int has_farfield, hipStream_t st);
 void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
                         double* rhs, double* w, double* G, double* t, double* T1, double* S,
                         const double* aforce, long aforce_stride, double forcing_factor,

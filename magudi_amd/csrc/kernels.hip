@@ -19,7 +19,7 @@ __device__ __forceinline__ void load_q(const double* q, long N, long p, double* 
 
 // Primitive variables needed by the viscous terms: w = (u_1..u_nd, T).
 template <int ND>
-__global__ void k_prim(Geo g, const double* __restrict__ q, double* __restrict__ w) {
+__global__ void __launch_bounds__(kBlock) k_prim(Geo g, const double* __restrict__ q, double* __restrict__ w) {
   const long p = gidx();
   if (p >= g.N) return;
   const long inst = blockIdx.y;
@@ -35,7 +35,7 @@ __global__ void k_prim(Geo g, const double* __restrict__ q, double* __restrict__
 // Gradient of w along direction d: G[(c*ND + d)] = jacobian * metric_dd * D_d w_c
 // (Grid%computeGradient, src/GridImpl.f90:1172-1421, non-curvilinear).
 template <int ND>
-__global__ void k_grad(Geo g, int d, const double* __restrict__ w, double* __restrict__ G) {
+__global__ void __launch_bounds__(kBlock) k_grad(Geo g, int d, const double* __restrict__ w, double* __restrict__ G) {
   const long p = gidx();
   if (p >= g.N) return;
   const long inst = blockIdx.y;
@@ -91,7 +91,7 @@ __device__ __forceinline__ void flux_hat(const Geo& g, const double* qi, const d
 // rhs accumulation of -sum_d D_d Fhat_d (computeRhsForward, src/RhsHelperImpl.f90:303-344):
 // d == 0 stores D_0 F_0; later directions add; the last one forms 0 - sum.
 template <int ND>
-__global__ void k_flux_div(Geo g, int d, const double* __restrict__ q, const double* __restrict__ G,
+__global__ void __launch_bounds__(kBlock) k_flux_div(Geo g, int d, const double* __restrict__ q, const double* __restrict__ G,
                            double* __restrict__ rhs) {
   const long p = gidx();
   if (p >= g.N) return;
@@ -117,7 +117,7 @@ __global__ void k_flux_div(Geo g, int d, const double* __restrict__ q, const dou
 // Contravariant total fluxes of every direction at each point, computed once:
 // F[(d*NV + l)] = Fhat_d,l (the same operations as flux_hat, so the same values).
 template <int ND>
-__global__ void k_flux_all(Geo g, const double* __restrict__ q, const double* __restrict__ G,
+__global__ void __launch_bounds__(kBlock) k_flux_all(Geo g, const double* __restrict__ q, const double* __restrict__ G,
                            double* __restrict__ F) {
   const long p = gidx();
   if (p >= g.N) return;
@@ -135,7 +135,7 @@ __global__ void k_flux_all(Geo g, const double* __restrict__ q, const double* __
 
 // rhs accumulation of -sum_d D_d Fhat_d from the stored fluxes (same sums as k_flux_div).
 template <int ND>
-__global__ void k_div(Geo g, int d, const double* __restrict__ F, double* __restrict__ rhs) {
+__global__ void __launch_bounds__(kBlock) k_div(Geo g, int d, const double* __restrict__ F, double* __restrict__ rhs) {
   const long p = gidx();
   if (p >= g.N) return;
   const long inst = blockIdx.y;
@@ -164,7 +164,7 @@ __global__ void k_div(Geo g, int d, const double* __restrict__ F, double* __rest
 // (addDissipation, src/RhsHelperImpl.f90:58-77).  With composite dissipation the
 // term is added to rhs directly.
 template <int ND>
-__global__ void k_diss_a(Geo g, int d, const double* __restrict__ x, double* __restrict__ t,
+__global__ void __launch_bounds__(kBlock) k_diss_a(Geo g, int d, const double* __restrict__ x, double* __restrict__ t,
                          double* __restrict__ rhs, double amount) {
   const long p = gidx();
   if (p >= g.N) return;
@@ -190,7 +190,7 @@ __global__ void k_diss_a(Geo g, int d, const double* __restrict__ x, double* __r
 
 // Dissipation, second half: rhs += amount * H_d^-1 D4_d^T t.
 template <int ND>
-__global__ void k_diss_b(Geo g, int d, const double* __restrict__ t, double* __restrict__ rhs,
+__global__ void __launch_bounds__(kBlock) k_diss_b(Geo g, int d, const double* __restrict__ t, double* __restrict__ rhs,
                          double amount) {
   const long p = gidx();
   if (p >= g.N) return;
@@ -215,7 +215,7 @@ __global__ void k_diss_b(Geo g, int d, const double* __restrict__ t, double* __r
 // Adjoint: T1_d = D+_d lambda for all nv components (src/RhsHelperImpl.f90:410-415);
 // with comp0 = 1 and nc = nv-1 it applies to the viscous diffusion block instead.
 template <int ND>
-__global__ void k_adj_deriv(Geo g, int d, const double* __restrict__ x, long xstride_inst,
+__global__ void __launch_bounds__(kBlock) k_adj_deriv(Geo g, int d, const double* __restrict__ x, long xstride_inst,
                             double* __restrict__ T1, int comp0) {
   const long p = gidx();
   if (p >= g.N) return;
@@ -252,7 +252,7 @@ __global__ void k_adj_deriv(Geo g, int d, const double* __restrict__ x, long xst
 
 // Adjoint: rhs = sum_d (A_d - Av_d)^T T1_d at each point (src/RhsHelperImpl.f90:431-491).
 template <int ND>
-__global__ void k_adj_jac(Geo g, const double* __restrict__ q, const double* __restrict__ G,
+__global__ void __launch_bounds__(kBlock) k_adj_jac(Geo g, const double* __restrict__ q, const double* __restrict__ G,
                           const double* __restrict__ T1, double* __restrict__ rhs) {
   const long p = gidx();
   if (p >= g.N) return;
@@ -291,7 +291,7 @@ __global__ void k_adj_jac(Geo g, const double* __restrict__ q, const double* __r
 // Viscous adjoint, second-partial block: overwrites components 1..nv-1 of T1_j with
 // sum_i B_ij^T T1_i (src/RhsHelperImpl.f90:509-553).
 template <int ND>
-__global__ void k_adj_visc2(Geo g, const double* __restrict__ q, double* __restrict__ T1) {
+__global__ void __launch_bounds__(kBlock) k_adj_visc2(Geo g, const double* __restrict__ q, double* __restrict__ T1) {
   const long p = gidx();
   if (p >= g.N) return;
   const long inst = blockIdx.y;
@@ -330,7 +330,7 @@ __global__ void k_adj_visc2(Geo g, const double* __restrict__ q, double* __restr
 // Viscous adjoint chain rule (src/RhsHelperImpl.f90:555-570 with sign -1 and
 // :227-236 with sign +1): t2 = sum_j S_j (components 1..nv-1).
 template <int ND>
-__global__ void k_adj_chain(Geo g, const double* __restrict__ q, const double* __restrict__ S,
+__global__ void __launch_bounds__(kBlock) k_adj_chain(Geo g, const double* __restrict__ q, const double* __restrict__ S,
                             double* __restrict__ rhs, double sign) {
   const long p = gidx();
   if (p >= g.N) return;
@@ -361,7 +361,7 @@ __global__ void k_adj_chain(Geo g, const double* __restrict__ q, const double* _
 // Viscous far-field adjoint penalty, first half (src/RhsHelperImpl.f90:170-210):
 // S_l -= sigma_v * B(dir, l)^T lambda on far-field patch points; S zeroed beforehand.
 template <int ND>
-__global__ void k_ff_visc_adj(Geo g, const Patches* __restrict__ P, const double* __restrict__ q,
+__global__ void __launch_bounds__(kBlock) k_ff_visc_adj(Geo g, const Patches* __restrict__ P, const double* __restrict__ q,
                               const double* __restrict__ lam, double* __restrict__ S) {
   const long p = gidx();
   if (p >= g.N) return;
@@ -402,8 +402,67 @@ __global__ void k_ff_visc_adj(Geo g, const Patches* __restrict__ P, const double
 
 // Region%computeRhs tail (src/RegionImpl.f90:1969-2023): multiply by the Jacobian, add
 // patch terms in bc.dat order, then acoustic sources (forward mode).
+// Viscous far-field penalty terms of each far-field patch point, formed before the patch
+// pass so that k_finish only adds them in bc.dat order (addFarFieldPenalty,
+// src/FarFieldPatchImpl.f90:93-286): forward sigma_v J sum_d (F^v_d - F^v_target,d) m_d,
+// adjoint sigma_v J B^T lambda with the first-partial viscous Jacobian B.
 template <int ND>
-__global__ void k_finish(Geo g, int mode, const Patches* __restrict__ P, const double* __restrict__ q,
+__global__ void __launch_bounds__(kBlock) k_ff_visc_terms(Geo g, int mode, const Patches* __restrict__ P,
+                                                          const double* __restrict__ q,
+                                                          const double* __restrict__ lam,
+                                                          const double* __restrict__ G) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  constexpr int NV = ND + 2;
+  const double* qi = q + inst * NV * g.N;
+  const double* li = lam ? lam + inst * NV * g.N : nullptr;
+  int ijk[3] = {0, 0, 0};
+  unflat<ND>(g, p, ijk);
+  for (int ip = 0; ip < P->count; ++ip) {
+    const DPatch& pt = P->p[ip];
+    if (pt.type != MGD_PATCH_FAR_FIELD || !pt.ffv) continue;
+    bool in = true;
+    for (int d = 0; d < ND; ++d) in = in && ijk[d] >= pt.lo[d] && ijk[d] <= pt.hi[d];
+    if (!in) continue;
+    int k = ijk[0] - pt.lo[0];
+    if (ND > 1) k += pt.ls[0] * (ijk[1] - pt.lo[1]);
+    if (ND > 2) k += pt.ls[0] * pt.ls[1] * (ijk[2] - pt.lo[2]);
+    const double jac = jacobian<ND>(g, ijk);
+    double v[NV];
+    load_q<ND>(qi, g.N, p, v);
+    const Prim<ND> s = prim<ND>(v, g.gamma);
+    double tau[ND * ND], hf[ND], mu, la, ka;
+    visc_point<ND>(g, G + inst * (ND + 1) * ND * g.N, g.N, p, s, tau, hf, &mu, &la, &ka);
+    const int dir = abs(pt.normal_dir) - 1;
+    const double vj = pt.sigma_visc * jac;
+    double* out = pt.ffv + inst * NV * pt.np;
+    if (mode == MGD_FORWARD) {
+      for (int l = 0; l < NV; ++l) {
+        double acc = 0.0;
+        for (int d = 0; d < ND; ++d) {
+          double fv[NV];
+          viscous_flux<ND>(s.u, tau, hf, d, fv);
+          const double md = d == dir ? metric<ND>(g, dir, ijk) : 0.0;
+          acc += (fv[l] - pt.tvflux[((long)d * NV + l) * pt.np + k]) * md;
+        }
+        out[(long)l * pt.np + k] = vj * acc;
+      }
+    } else {
+      double B[NV * NV], m[ND];
+      for (int d = 0; d < ND; ++d) m[d] = d == dir ? metric<ND>(g, dir, ijk) : 0.0;
+      first_partial_viscous<ND>(m, tau, hf, g.power_law, g.gamma, s, B);
+      for (int i = 0; i < NV; ++i) {
+        double t = 0.0;
+        for (int c = 0; c < NV; ++c) t += B[i * NV + c] * li[c * g.N + p];
+        out[(long)i * pt.np + k] = vj * t;
+      }
+    }
+  }
+}
+
+template <int ND>
+__global__ void __launch_bounds__(kBlock) k_finish(Geo g, int mode, const Patches* __restrict__ P, const double* __restrict__ q,
                          const double* __restrict__ lam, const double* __restrict__ G,
                          double* __restrict__ rhs, const double* __restrict__ control,
                          long control_inst_stride, const double* __restrict__ aforce,
@@ -442,23 +501,8 @@ __global__ void k_finish(Geo g, int mode, const Patches* __restrict__ P, const d
             r[i] = r[i] - sj * s;
           }
           if (g.viscous) {
-            double v[NV];
-            load_q<ND>(qi, g.N, p, v);
-            const Prim<ND> s = prim<ND>(v, g.gamma);
-            double tau[ND * ND], hf[ND], mu, la, ka;
-            visc_point<ND>(g, G + inst * (ND + 1) * ND * g.N, g.N, p, s, tau, hf, &mu, &la, &ka);
-            const int dir = abs(pt.normal_dir) - 1;
-            const double vj = pt.sigma_visc * jac;
-            for (int l = 0; l < NV; ++l) {
-              double acc = 0.0;
-              for (int d = 0; d < ND; ++d) {
-                double fv[NV];
-                viscous_flux<ND>(s.u, tau, hf, d, fv);
-                const double md = d == dir ? metric<ND>(g, dir, ijk) : 0.0;
-                acc += (fv[l] - pt.tvflux[((long)d * NV + l) * pt.np + k]) * md;
-              }
-              r[l] = r[l] + vj * acc;
-            }
+            const double* fv = pt.ffv + inst * NV * pt.np;
+            for (int l = 0; l < NV; ++l) r[l] = r[l] + fv[(long)l * pt.np + k];
           }
         } else {
           for (int i = 0; i < NV; ++i) {
@@ -467,20 +511,8 @@ __global__ void k_finish(Geo g, int mode, const Patches* __restrict__ P, const d
             r[i] = r[i] + sj * s;
           }
           if (g.viscous) {
-            double v[NV];
-            load_q<ND>(qi, g.N, p, v);
-            const Prim<ND> s = prim<ND>(v, g.gamma);
-            double tau[ND * ND], hf[ND], mu, la, ka, B[NV * NV], m[ND];
-            visc_point<ND>(g, G + inst * (ND + 1) * ND * g.N, g.N, p, s, tau, hf, &mu, &la, &ka);
-            const int dir = abs(pt.normal_dir) - 1;
-            for (int d = 0; d < ND; ++d) m[d] = d == dir ? metric<ND>(g, dir, ijk) : 0.0;
-            first_partial_viscous<ND>(m, tau, hf, g.power_law, g.gamma, s, B);
-            const double vj = pt.sigma_visc * jac;
-            for (int i = 0; i < NV; ++i) {
-              double t = 0.0;
-              for (int c = 0; c < NV; ++c) t += B[i * NV + c] * li[c * g.N + p];
-              r[i] = r[i] - vj * t;
-            }
+            const double* fv = pt.ffv + inst * NV * pt.np;
+            for (int i = 0; i < NV; ++i) r[i] = r[i] - fv[(long)i * pt.np + k];
           }
         }
         break;
@@ -585,7 +617,7 @@ __global__ void k_finish(Geo g, int mode, const Patches* __restrict__ P, const d
 // ------------------------------------------------------------------ RK4 stages
 // substepForwardRK4 updates (src/RK4IntegratorImpl.f90:115-156); q_out2 (optional)
 // receives a copy of the new stage state (checkpoint window).
-__global__ void k_rk_forward(long M, int stage, double dt, double* __restrict__ q,
+__global__ void __launch_bounds__(kBlock) k_rk_forward(long M, int stage, double dt, double* __restrict__ q,
                              double* __restrict__ b1, double* __restrict__ b2,
                              const double* __restrict__ rhs, double* __restrict__ qcopy) {
   const long i = gidx();
@@ -617,7 +649,7 @@ __global__ void k_rk_forward(long M, int stage, double dt, double* __restrict__ 
 }
 
 // substepAdjointRK4 updates (src/RK4IntegratorImpl.f90:215-261)
-__global__ void k_rk_adjoint(long M, int stage, double dt, double* __restrict__ lam,
+__global__ void __launch_bounds__(kBlock) k_rk_adjoint(long M, int stage, double dt, double* __restrict__ lam,
                              double* __restrict__ b1, double* __restrict__ b2,
                              const double* __restrict__ rhs) {
   const long i = gidx();
@@ -645,12 +677,12 @@ __global__ void k_rk_adjoint(long M, int stage, double dt, double* __restrict__ 
   }
 }
 
-__global__ void k_copy(long M, const double* __restrict__ a, double* __restrict__ b) {
+__global__ void __launch_bounds__(kBlock) k_copy(long M, const double* __restrict__ a, double* __restrict__ b) {
   const long i = gidx();
   if (i < M) b[i] = a[i];
 }
 
-__global__ void k_fill(long M, double v, double* __restrict__ b) {
+__global__ void __launch_bounds__(kBlock) k_fill(long M, double v, double* __restrict__ b) {
   const long i = gidx();
   if (i < M) b[i] = v;
 }
@@ -682,7 +714,7 @@ __device__ __forceinline__ double block_sum(double v) {
 //          (computeAcousticNoise + computeQuadratureOnPatches);
 // which 1: ThermalActuator sensitivity norm * (lambda_E m_C)^2 on the actuator patch.
 template <int ND>
-__global__ void k_patch_reduce(Geo g, const DPatch* __restrict__ pt_, int which,
+__global__ void __launch_bounds__(kBlock) k_patch_reduce(Geo g, const DPatch* __restrict__ pt_, int which,
                                const double* __restrict__ x, double* __restrict__ partial) {
   const DPatch& pt = *pt_;
   const long inst = blockIdx.y;
@@ -709,7 +741,7 @@ __global__ void k_patch_reduce(Geo g, const DPatch* __restrict__ pt_, int which,
 }
 
 // acc[inst] += weight * sum(partials); out_stage (optional) keeps the raw value.
-__global__ void k_reduce_final(int nblocks, const double* __restrict__ partial, double weight,
+__global__ void __launch_bounds__(kBlock) k_reduce_final(int nblocks, const double* __restrict__ partial, double weight,
                                double* __restrict__ acc, double* __restrict__ raw) {
   const long inst = blockIdx.x;
   double v = 0.0;
@@ -724,7 +756,7 @@ __global__ void k_reduce_final(int nblocks, const double* __restrict__ partial, 
 // Grid%computeInnerProduct (src/GridImpl.f90:1067-1170): per-block partial sums of
 // sum over components of a * norm * b at each point of one instance.
 template <int ND>
-__global__ void k_inner_partial(Geo g, const double* __restrict__ a, const double* __restrict__ b,
+__global__ void __launch_bounds__(kBlock) k_inner_partial(Geo g, const double* __restrict__ a, const double* __restrict__ b,
                                 double* __restrict__ partial) {
   const long inst = blockIdx.y;
   const long p = (long)blockIdx.x * kRedBlock + threadIdx.x;
@@ -742,7 +774,7 @@ __global__ void k_inner_partial(Geo g, const double* __restrict__ a, const doubl
 }
 
 // d = c + s (a - b) (c may be null: d = s (a - b)) over one instance's state array.
-__global__ void k_lincomb(long M, const double* __restrict__ a, const double* __restrict__ b,
+__global__ void __launch_bounds__(kBlock) k_lincomb(long M, const double* __restrict__ a, const double* __restrict__ b,
                           double s, const double* __restrict__ c, double* __restrict__ d) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
@@ -751,7 +783,7 @@ __global__ void k_lincomb(long M, const double* __restrict__ a, const double* __
 
 // AcousticNoise adjoint forcing on the cost-target patch (src/AcousticNoiseImpl.f90:208-280)
 template <int ND>
-__global__ void k_adj_forcing(Geo g, const DPatch* __restrict__ pt_, const double* __restrict__ q,
+__global__ void __launch_bounds__(kBlock) k_adj_forcing(Geo g, const DPatch* __restrict__ pt_, const double* __restrict__ q,
                               double* __restrict__ out, int zero) {
   const DPatch& pt = *pt_;
   const long inst = blockIdx.y;
@@ -780,7 +812,7 @@ __global__ void k_adj_forcing(Geo g, const DPatch* __restrict__ pt_, const doubl
 // adjoint terminal condition: lambda += factor * forcing on the cost-target patch
 // (loadInitialCondition, src/SolverImpl.f90:368-428)
 template <int ND>
-__global__ void k_adj_terminal(Geo g, const DPatch* __restrict__ pt_, const double* __restrict__ af,
+__global__ void __launch_bounds__(kBlock) k_adj_terminal(Geo g, const DPatch* __restrict__ pt_, const double* __restrict__ af,
                                double factor, double* __restrict__ lam) {
   const DPatch& pt = *pt_;
   const long inst = blockIdx.y;
@@ -798,7 +830,7 @@ __global__ void k_adj_terminal(Geo g, const DPatch* __restrict__ pt_, const doub
 
 // ThermalActuator gradient lambda_E * m_C on the actuator patch (:383-443)
 template <int ND>
-__global__ void k_gradient(Geo g, const DPatch* __restrict__ pt_, const double* __restrict__ lam,
+__global__ void __launch_bounds__(kBlock) k_gradient(Geo g, const DPatch* __restrict__ pt_, const double* __restrict__ lam,
                            double* __restrict__ out, long inst_stride) {
   const DPatch& pt = *pt_;
   const long inst = blockIdx.y;
@@ -824,7 +856,7 @@ static dim3 grid_pts(const Geo& g) {
 void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
                         double* G, double* t, double* F, const double* control, long control_stride,
                         const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
-                        hipStream_t st) {
+                        int has_farfield, hipStream_t st) {
   const dim3 grid = grid_pts(g);
   MGD_DISPATCH(g.nd, {
     if (g.viscous) {
@@ -844,6 +876,8 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
         if (!g.composite) k_diss_b<ND><<<grid, kBlock, 0, st>>>(g, d, t, rhs, g.diss_amount);
       }
     }
+    if (g.viscous && has_farfield)
+      k_ff_visc_terms<ND><<<grid, kBlock, 0, st>>>(g, MGD_FORWARD, P, q, nullptr, G);
     k_finish<ND><<<grid, kBlock, 0, st>>>(g, MGD_FORWARD, P, q, nullptr, G, rhs, control,
                                           control_stride, nullptr, 0, 0.0, src, nsrc, src_amp,
                                           src_inst_stride);
@@ -884,6 +918,8 @@ void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
         k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, T1, (long)ND * NV * g.N, S, 1);
       k_adj_chain<ND><<<grid, kBlock, 0, st>>>(g, q, S, rhs, +1.0);
     }
+    if (g.viscous && has_farfield)
+      k_ff_visc_terms<ND><<<grid, kBlock, 0, st>>>(g, MGD_ADJOINT, P, q, lam, G);
     k_finish<ND><<<grid, kBlock, 0, st>>>(g, MGD_ADJOINT, P, q, lam, G, rhs, nullptr, 0, aforce,
                                           aforce_stride, forcing_factor, nullptr, 0, nullptr, 0);
   });
@@ -957,7 +993,7 @@ void launch_gradient(const Geo& g, const DPatch* pt, int np, const double* lam, 
 }
 
 // Stencil operator on a device field (StencilOperator%apply), ncomp components.
-__global__ void k_apply(Geo g, int slot, int d, int ncomp, const double* __restrict__ x,
+__global__ void __launch_bounds__(kBlock) k_apply(Geo g, int slot, int d, int ncomp, const double* __restrict__ x,
                         double* __restrict__ y) {
   const long p = gidx();
   if (p >= g.N) return;

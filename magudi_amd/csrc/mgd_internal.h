@@ -77,6 +77,8 @@ struct DPatch {
   const double* aplus;      // far field: A+ blocks (np x nv x nv, column major) then the target state (np x nv);
                             // sponge: target state (np x nv); cost target: mean pressure (np)
   const double* tvflux;     // far field: target viscous fluxes (nd x nv x np)
+  double* ffv;              // far field, viscous: the viscous penalty term of each point
+                            // (instance x nv x np), formed before the patch pass
 };
 
 struct Patches {

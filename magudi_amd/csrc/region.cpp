@@ -51,7 +51,7 @@ struct HostPatch {
   mgd_patch_desc pd;
   int lo[3], hi[3], ls[3], np;
   std::vector<double> strength, data, tvflux;
-  DevBuf d_strength, d_data, d_tvflux;
+  DevBuf d_strength, d_data, d_tvflux, d_ffv;
 };
 
 }  // namespace
@@ -454,7 +454,8 @@ void forward_stage(mgd_region* r, double* q, double* b1, double* b2, int sidx, i
   const int ns = (int)r->sources.size();
   launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p, ctrl, cstride,
                      reinterpret_cast<const Source*>(r->d_sources.p), ns,
-                     r->src_amp.p + (long)sidx * std::max(ns, 1), r->src_inst_stride, r->stream);
+                     r->src_amp.p + (long)sidx * std::max(ns, 1), r->src_inst_stride,
+                     r->has_farfield, r->stream);
   launch_rk_forward((long)r->ninst * M_of(r), stage, dt, q, b1, b2, r->rhs.p, qcopy, r->stream);
 }
 
@@ -1014,6 +1015,7 @@ int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region
       return MGD_ERR_ARG;
     }
   r->N = (long)r->desc.n[0] * r->desc.n[1] * r->desc.n[2];
+  if (r->N >= (1L << 31)) { set_error("a region holds fewer than 2^31 grid points"); return MGD_ERR_ARG; }
   const long N = r->N;
   r->coords.assign(coords, coords + (size_t)N * r->nd);
   // rectilinear check: coordinate d depends on index d only
@@ -1311,6 +1313,11 @@ int mgd_region_finalize(mgd_region* r) {
     D.strength = P.d_strength.p;
     D.aplus = P.d_data.p;
     D.tvflux = P.d_tvflux.p;
+    D.ffv = nullptr;
+    if (P.pd.type == MGD_PATCH_FAR_FIELD && r->desc.viscous) {
+      if (P.d_ffv.alloc((size_t)r->ninst * nv * P.np)) return MGD_ERR_HIP;
+      D.ffv = P.d_ffv.p;
+    }
   }
   if (r->d_patches.alloc(sizeof(Patches) / sizeof(double) + 1)) return MGD_ERR_HIP;
   MGD_HIP(hipMemcpy(r->d_patches.p, &r->hpatches, sizeof(Patches), hipMemcpyHostToDevice));
@@ -1470,7 +1477,7 @@ int mgd_compute_rhs(mgd_region* r, int mode, double time, const double* control,
     }
     launch_rhs_forward(r->geo, dpatches(r), r->q.p, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p, ctrl, na,
                        reinterpret_cast<const Source*>(r->d_sources.p), ns, r->src_amp.p,
-                       r->src_inst_stride, r->stream);
+                       r->src_inst_stride, r->has_farfield, r->stream);
   } else if (mode == MGD_ADJOINT) {
     if (int rc = ensure_adjoint(r)) return rc;
     const int nc = n_cost(r);
@@ -1684,7 +1691,8 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
       case MGD_KERNEL_RHS_FORWARD:
         launch_rhs_forward(r->geo, dpatches(r), r->q.p, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p, nullptr,
                            n_act(r), reinterpret_cast<const Source*>(r->d_sources.p),
-                           (int)r->sources.size(), r->src_amp.p, r->src_inst_stride, r->stream);
+                           (int)r->sources.size(), r->src_amp.p, r->src_inst_stride,
+                           r->has_farfield, r->stream);
         break;
       case MGD_KERNEL_RHS_ADJOINT:
         launch_rhs_adjoint(r->geo, dpatches(r), r->q.p, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
