@@ -87,12 +87,14 @@ __device__ __forceinline__ void stage_operand(const Geo& g, const TileGeo& t, co
   bool ok[kStageIters];
 #pragma unroll
   for (int it = 0; it < kStageIters; ++it) {
+    // walk the EX x EY staging layout (compile-time width: no integer division by a
+    // run-time value); columns / rows beyond a partial tile are skipped
     const int e = threadIdx.x + it * NT;
-    const int ex = e % wx, ey = e / wx;
+    const int ex = e % EX, ey = e / EX;
     const int gi = ext_index(dx, t.ex0 + ex), gj = ext_index(dy, t.ey0 + ey);
     const bool rowx = ey >= HY && ey < HY + t.sy;
     const bool coly = ex >= HX && ex < HX + t.sx;
-    ok[it] = e < wx * wy && gi >= 0 && gj >= 0 && (rowx || coly);
+    ok[it] = e < EX * EY && ex < wx && ey < wy && gi >= 0 && gj >= 0 && (rowx || coly);
     ex_[it] = ex; ey_[it] = ey; gi_[it] = gi; gj_[it] = gj;
     if (ok[it]) {
       const long pp = gi + (long)g.n[0] * gj;
@@ -204,8 +206,9 @@ __device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& 
   const double c0 = od.c[kMaxHW], c1 = od.c[kMaxHW + 1], c2 = od.c[kMaxHW + 2];
   const Dir dx = g.dir[0], dy = g.dir[1];
   const int wx = t.sx + 4;
-  for (int e = threadIdx.x; e < t.sy * wx; e += NT) {
-    const int ex = e % wx + 2, ey = e / wx + HY;
+  for (int e = threadIdx.x; e < TY * (TX + 4); e += NT) {
+    const int ex = e % (TX + 4) + 2, ey = e / (TX + 4) + HY;
+    if (ex - 2 >= wx || ey - HY >= t.sy) continue;
     const int gi = t.ex0 + ex;
     if (!dx.periodic && (gi < 0 || gi >= dx.len)) continue;
     const double arc = fabs(g.h[1][t.y0 + ey - HY]);
@@ -217,8 +220,9 @@ __device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& 
     }
   }
   const int hy = t.sy + 4;
-  for (int e = threadIdx.x; e < hy * t.sx; e += NT) {
-    const int ex = e % t.sx + HX, ey = e / t.sx + 2;
+  for (int e = threadIdx.x; e < (TY + 4) * TX; e += NT) {
+    const int ex = e % TX + HX, ey = e / TX + 2;
+    if (ex - HX >= t.sx || ey - 2 >= hy) continue;
     const int gj = t.ey0 + ey;
     if (!dy.periodic && (gj < 0 || gj >= dy.len)) continue;
     const double arc = fabs(g.h[0][t.x0 + ex - HX]);
@@ -448,16 +452,16 @@ __global__ void __launch_bounds__(S::NT, 4) k_fwd_stage2d(Geo g, const Patches* 
       const long o = l * g.N + p;
       const double rl = r[k][l];
       if (a.stage == 1) {
-        b2[o] = qp[l] + dt * rl / 6.0;
+        b2[o] = qp[l] + div6(dt * rl);
         qn[l] = qp[l] + dt * rl / 2.0;
       } else if (a.stage == 2) {
-        b2[o] = pb2[l] + dt * rl / 3.0;
+        b2[o] = pb2[l] + div3(dt * rl);
         qn[l] = pq0[l] + dt * rl / 2.0;
       } else if (a.stage == 3) {
-        b2[o] = pb2[l] + dt * rl / 3.0;
+        b2[o] = pb2[l] + div3(dt * rl);
         qn[l] = pq0[l] + dt * rl;
       } else {
-        qn[l] = pb2[l] + dt * rl / 6.0;
+        qn[l] = pb2[l] + div6(dt * rl);
       }
       a.qout[inst * NVN + o] = qn[l];
       if (a.rhs_out) a.rhs_out[inst * NVN + o] = rl;
@@ -621,16 +625,16 @@ __global__ void __launch_bounds__(S::NT, 4) k_adj_stage2d(Geo g, const Patches* 
       const double rl = r[l];
       double ln;
       if (a.stage == 4) {
-        b2[o] = lp[l] - dt * rl / 6.0;
+        b2[o] = lp[l] - div6(dt * rl);
         ln = lp[l] - dt * rl / 2.0;
       } else if (a.stage == 3) {
-        b2[o] = pb2[l] - dt * rl / 3.0;
+        b2[o] = pb2[l] - div3(dt * rl);
         ln = pl0[l] - dt * rl / 2.0;
       } else if (a.stage == 2) {
-        b2[o] = pb2[l] - dt * rl / 3.0;
+        b2[o] = pb2[l] - div3(dt * rl);
         ln = pl0[l] - dt * rl;
       } else {
-        ln = pb2[l] - dt * rl / 6.0;
+        ln = pb2[l] - div6(dt * rl);
       }
       a.lout[inst * NVN + o] = ln;
       if (a.rhs_out) a.rhs_out[inst * NVN + o] = rl;

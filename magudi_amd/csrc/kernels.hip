@@ -628,20 +628,20 @@ __global__ void __launch_bounds__(kBlock) k_rk_forward(long M, int stage, double
     case 1: {
       const double q0 = q[i];
       b1[i] = q0;
-      b2[i] = q0 + dt * r / 6.0;
+      b2[i] = q0 + div6(dt * r);
       qn = q0 + dt * r / 2.0;
       break;
     }
     case 2:
-      b2[i] = b2[i] + dt * r / 3.0;
+      b2[i] = b2[i] + div3(dt * r);
       qn = b1[i] + dt * r / 2.0;
       break;
     case 3:
-      b2[i] = b2[i] + dt * r / 3.0;
+      b2[i] = b2[i] + div3(dt * r);
       qn = b1[i] + dt * r;
       break;
     default:
-      qn = b2[i] + dt * r / 6.0;
+      qn = b2[i] + div6(dt * r);
       break;
   }
   q[i] = qn;
@@ -659,20 +659,20 @@ __global__ void __launch_bounds__(kBlock) k_rk_adjoint(long M, int stage, double
     case 4: {
       const double l0 = lam[i];
       b1[i] = l0;
-      b2[i] = l0 - dt * r / 6.0;
+      b2[i] = l0 - div6(dt * r);
       lam[i] = l0 - dt * r / 2.0;
       break;
     }
     case 3:
-      b2[i] = b2[i] - dt * r / 3.0;
+      b2[i] = b2[i] - div3(dt * r);
       lam[i] = b1[i] - dt * r / 2.0;
       break;
     case 2:
-      b2[i] = b2[i] - dt * r / 3.0;
+      b2[i] = b2[i] - div3(dt * r);
       lam[i] = b1[i] - dt * r;
       break;
     default:
-      lam[i] = b2[i] - dt * r / 6.0;
+      lam[i] = b2[i] - div6(dt * r);
       break;
   }
 }
