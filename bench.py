@@ -118,6 +118,39 @@ def viscous3d_leg(n):
     return leg
 
 
+def uneven_leg(nsteps=32, save_interval=8):
+    """The mixed batch (1-D periodic waves of 2048 points, 2-D monopoles on odd grids
+    drawn from [201, 513] with seed 5, 513 x 257 mixing layers with 4 segments each as
+    instances): forward + adjoint of every member, all regions issued concurrently,
+    against the same regions run one after another."""
+    from magudi_amd import cases
+    from magudi_amd.group import RegionGroup
+    cs = cases.uneven_batch(seed=5, nsteps=nsteps, save_interval=save_interval)
+    ninst = [4 if c.name.startswith("mixing") else 1 for c in cs]
+    G = RegionGroup(cs, n_instances=ninst)
+    G.restart()
+    G.run_adjoint(want_gradient=False)          # autotune and warm up
+    G.restart()
+    G.sync()
+    t0 = time.perf_counter()
+    G.run_adjoint(want_gradient=False)
+    G.sync()
+    t_group = time.perf_counter() - t0
+    G.restart()
+    G.sync()
+    t0 = time.perf_counter()
+    for s in G.solvers:
+        s.runAdjoint(want_gradient=False)
+    G.sync()
+    t_seq = time.perf_counter() - t0
+    pts = G.grid_point_steps
+    return {"workload": f"{len(cs)} regions: 8 x 1-D wave 2048, 16 x 2-D monopole 201..513 "
+                        f"(seed 5), 2 x mixing layer 513x257 x 4 segments; fwd+adjoint "
+                        f"{nsteps} steps, snapshots every {save_interval}",
+            "value": pts / t_group, "unit": "grid-pt-steps/s", "seconds": t_group,
+            "sequential_value": pts / t_seq, "concurrency_speedup": t_seq / t_group}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -231,6 +264,7 @@ def main():
                         "batched_speedup_vs_plain": (a.batch * pts / tb) / (case.npoints * ns_plain / tp)}
         if a.c4_n > 0:
             out["viscous3d"] = viscous3d_leg(a.c4_n)
+        out["uneven_batch"] = uneven_leg()
         out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_steps, max(1, a.cpu_steps // 2))
     else:
         out["cpu_baseline"] = None

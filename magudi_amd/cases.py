@@ -266,6 +266,110 @@ def walled_monopole(n: int = 41, viscous: bool = False, nsteps: int = 8,
     return c
 
 
+def periodic_wave_1d(n: int = 2048, nsteps: int = 64, save_interval: int = 16,
+                     amplitude: float = 1e-2) -> Case:
+    """1-D periodic acoustic wave on [0, 1) (PLANE periodicity): a right-running
+    Gaussian pressure pulse (p' = rho' = amplitude g, u' = p'), SBP 4-8 with
+    dissipation; cost target on [0.6, 0.8], actuator on [0.2, 0.3]."""
+    x = np.arange(n) / n
+    coords = x[None, :].copy()
+    gamma = 1.4
+    q = quiescent_state(1, n, gamma)
+    g = np.exp(-((x - 0.5) / 0.05) ** 2)
+    dp = amplitude * g
+    q[0] = q[0] + dp
+    q[1] = q[0] * dp
+    q[2] = (1.0 / gamma + dp) / (gamma - 1.0) + 0.5 * q[1] ** 2 / q[0]
+    ti = _box_extent(x, 0.6, 0.8)
+    ci = _box_extent(x, 0.2, 0.3)
+    patches = [Patch("targetRegion", COST_TARGET, 0, (ti[0], ti[1], 1, -1, 1, -1)),
+               Patch("controlRegion", ACTUATOR, 0, (ci[0], ci[1], 1, -1, 1, -1))]
+    dx = 1.0 / n
+    return Case(
+        name=f"wave1d_{n}", nd=1, n=(n, 1, 1), coords=coords, periodicity=(PLANE, NONE, NONE),
+        periodic_length=(1.0, 0.0, 0.0), gamma=gamma, dissipation=True, dissipation_amount=0.005,
+        patches=patches, target=quiescent_state(1, n, gamma),
+        control_mollifier=_tanh_window(x, 0.2, 0.3, 200.0),
+        target_mollifier=_tanh_window(x, 0.6, 0.8, 200.0),
+        dt=0.5 * dx, nsteps=nsteps, save_interval=save_interval, initial=q)
+
+
+def mixing_layer(nx: int = 513, ny: int = 257, stretch: float = 1.02, velocity_ratio: float = 0.5,
+                 thickness: float = 1.0, perturbation: float = 1e-3, seed: int = 0,
+                 nsteps: int = 64, save_interval: int = 16) -> Case:
+    """2-D temporally developing mixing layer: periodic in x (PLANE), y stretched
+    geometrically away from the centre line (spacing ratio `stretch`), tanh velocity
+    profile u = (U1 + U2)/2 + (U1 - U2)/2 tanh(2 y / thickness) with U2 / U1 =
+    velocity_ratio and U1 = 0.5 (Mach 0.5), a seeded random perturbation of the
+    velocity; far field and sponges on the y faces; cost target on a band above the
+    layer, actuator on a patch in it.  Inviscid Euler with dissipation."""
+    ly_half_cells = (ny - 1) // 2
+    dy0 = 0.2 * thickness
+    ys = np.concatenate([[0.0], np.cumsum(dy0 * stretch ** np.arange(ly_half_cells))])
+    y1 = np.concatenate([-ys[::-1], ys[1:]]) if (ny - 1) % 2 == 0 else np.concatenate([-ys[::-1], ys[1:], [ys[-1] + dy0]])
+    y1 = y1[:ny]
+    lx = 30.0 * thickness
+    x1 = np.arange(nx) * (lx / nx)
+    X, Y = np.meshgrid(x1, y1, indexing="xy")
+    coords = np.stack([X.ravel(), Y.ravel()])
+    npts = nx * ny
+    gamma = 1.4
+    u1 = 0.5
+    u2 = velocity_ratio * u1
+    U = 0.5 * (u1 + u2) + 0.5 * (u1 - u2) * np.tanh(2.0 * coords[1] / thickness)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    env = np.exp(-(coords[1] / (2.0 * thickness)) ** 2)
+    du = perturbation * env * rng.standard_normal(npts)
+    dv = perturbation * env * rng.standard_normal(npts)
+    target = quiescent_state(2, npts, gamma)
+    target[1] = U
+    target[3] = target[3] + 0.5 * U ** 2
+    q = target.copy()
+    q[1] = U + du
+    q[2] = dv
+    q[3] = (1.0 / gamma) / (gamma - 1.0) + 0.5 * ((U + du) ** 2 + dv ** 2)
+    w = max(12, ny // 10)
+    patches = [
+        Patch("farField.S", FAR_FIELD, 2, (1, -1, 1, 1, 1, -1), (1.0, 0.0, 0, 0)),
+        Patch("farField.N", FAR_FIELD, -2, (1, -1, -1, -1, 1, -1), (1.0, 0.0, 0, 0)),
+        Patch("sponge.S", SPONGE, 2, (1, -1, 1, w, 1, -1), (0.2, 2.0, 0, 0)),
+        Patch("sponge.N", SPONGE, -2, (1, -1, -w, -1, 1, -1), (0.2, 2.0, 0, 0)),
+    ]
+    tj = _box_extent(y1, 2.0 * thickness, 6.0 * thickness)
+    patches.append(Patch("targetRegion", COST_TARGET, 0, (1, -1, tj[0], tj[1], 1, -1)))
+    ci = _box_extent(x1, 0.2 * lx, 0.3 * lx)
+    cj = _box_extent(y1, -1.0 * thickness, 1.0 * thickness)
+    patches.append(Patch("controlRegion", ACTUATOR, 0, (ci[0], ci[1], cj[0], cj[1], 1, -1)))
+    tgt_moll = _tanh_window(coords[1], 2.0 * thickness, 6.0 * thickness, 4.0)
+    ctrl_moll = _tanh_window(coords[0], 0.2 * lx, 0.3 * lx, 4.0) * _tanh_window(coords[1], -thickness, thickness, 4.0)
+    dxmin = min(lx / nx, dy0)
+    dt = 0.5 * dxmin / (1.0 + u1)
+    return Case(
+        name=f"mixing_layer{nx}x{ny}", nd=2, n=(nx, ny, 1), coords=coords,
+        periodicity=(PLANE, NONE, NONE), periodic_length=(lx, 0.0, 0.0), gamma=gamma,
+        dissipation=True, dissipation_amount=0.005, patches=patches, target=target,
+        control_mollifier=ctrl_moll, target_mollifier=tgt_moll, dt=dt, nsteps=nsteps,
+        save_interval=save_interval, initial=q)
+
+
+def uneven_batch(seed: int = 5, n_waves: int = 8, n_monopoles: int = 16, n_layers: int = 2,
+                 scale: float = 1.0, nsteps: int = 32, save_interval: int = 8) -> List[Case]:
+    """The mixed batch of problem types: 1-D periodic waves, 2-D monopoles on odd grid
+    sizes drawn from [201, 513] and mixing layers; `scale` shrinks the grids for tests."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out: List[Case] = []
+    for k in range(n_waves):
+        out.append(periodic_wave_1d(n=max(64, int(2048 * scale)), nsteps=nsteps, save_interval=save_interval))
+    for k in range(n_monopoles):
+        n = int(rng.integers(100, 257)) * 2 + 1
+        n = max(41, int(n * scale) | 1)
+        out.append(acoustic_monopole(n=n, nsteps=nsteps, save_interval=save_interval))
+    for k in range(n_layers):
+        out.append(mixing_layer(nx=max(64, int(513 * scale)), ny=max(41, int(257 * scale) | 1),
+                                seed=k, nsteps=nsteps, save_interval=save_interval))
+    return out
+
+
 def small_monopole(n: int = 41, nsteps: int = 8, save_interval: int = 4) -> Case:
     """Scaled-down monopole used by the unit and parity tests."""
     c = acoustic_monopole(n=n, nsteps=nsteps, save_interval=save_interval)
