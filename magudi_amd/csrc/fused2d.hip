@@ -642,19 +642,20 @@ __global__ void __launch_bounds__(S::NT, 4) k_adj_stage2d(Geo g, const Patches* 
   }
 }
 
-// Stage sums of the per-point quadrature terms of one chunk of stages: one block per
-// (stage, instance), a fixed-order tree over the points.
-__global__ void __launch_bounds__(kRedThreads) k_stage_sums(const double* __restrict__ partial,
+// Stage sums of the per-point quadrature terms of one chunk of stages: one block of
+// kSumThreads per (stage, instance), a fixed-order tree over the points.
+constexpr int kSumThreads = 1024;
+__global__ void __launch_bounds__(kSumThreads) k_stage_sums(const double* __restrict__ partial,
                                                             int nslots, int npts,
                                                             double* __restrict__ sums,
                                                             int nstages, int base) {
-  __shared__ double red[kRedThreads / 64];
+  __shared__ double red[kSumThreads / 64];
   const int s = blockIdx.x;
   const long inst = blockIdx.y;
   const double* src = partial + (inst * nslots + s) * (long)npts;
   double v = 0.0;
-  for (int k = threadIdx.x; k < npts; k += kRedThreads) v += src[k];
-  const double sum = block_sum<kRedThreads>(v, red);
+  for (int k = threadIdx.x; k < npts; k += kSumThreads) v += src[k];
+  const double sum = block_sum<kSumThreads>(v, red);
   if (threadIdx.x == 0) sums[inst * nstages + base + s] = sum;
 }
 
@@ -792,7 +793,7 @@ void launch_adj_stage2d(const Geo& g, const Patches* P, const Tiling& T, const A
 void launch_stage_reduce(const double* partial, int nslots, int nfill, int npts, double* sums,
                          int nstages, int base, int ninst, hipStream_t st) {
   if (nfill <= 0 || npts <= 0) return;
-  k_stage_sums<<<dim3(nfill, ninst), kRedThreads, 0, st>>>(partial, nslots, npts, sums, nstages,
+  k_stage_sums<<<dim3(nfill, ninst), kSumThreads, 0, st>>>(partial, nslots, npts, sums, nstages,
                                                            base);
 }
 
