@@ -118,6 +118,27 @@ def viscous3d_leg(n):
     return leg
 
 
+def small_grid_leg(n=201, nsteps=96, save_interval=32, batch=20):
+    """North-star ratio on a small grid: batched fused forward+adjoint (batch instances
+    per launch) against the generic per-instance path, whose many small launches
+    make it launch-bound at this size."""
+    from magudi_amd import cases
+    from magudi_amd.solver import Region, Solver
+    c = cases.acoustic_monopole(n=n, nsteps=nsteps, save_interval=save_interval)
+    q0 = c.initial_state()
+    out = {"grid": [n, n], "nsteps": nsteps, "instances": batch}
+    for key, ninst, plain in (("batched_fused", batch, False), ("plain_single", 1, True)):
+        R = Region(c, n_instances=ninst, force_plain=plain)
+        for i in range(ninst):
+            R.states[i].initialCondition = q0
+        S = Solver(R)
+        run_hot_path(R, S, 1)
+        t, _ = run_hot_path(R, S, 1)
+        out[key] = ninst * c.npoints * nsteps / t
+    out["ratio"] = out["batched_fused"] / out["plain_single"]
+    return out
+
+
 def uneven_leg(nsteps=32, save_interval=8):
     """The mixed batch (1-D periodic waves of 2048 points, 2-D monopoles on odd grids
     drawn from [201, 513] with seed 5, 513 x 257 mixing layers with 4 segments each as
@@ -265,6 +286,7 @@ def main():
         if a.c4_n > 0:
             out["viscous3d"] = viscous3d_leg(a.c4_n)
         out["uneven_batch"] = uneven_leg()
+        out["small_grid"] = small_grid_leg()
         out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_steps, max(1, a.cpu_steps // 2))
     else:
         out["cpu_baseline"] = None
