@@ -56,6 +56,85 @@ __global__ void __launch_bounds__(kBlock) k_grad(Geo g, int d, const double* __r
   for (int c = 0; c <= ND; ++c) Gi[(c * ND + d) * g.N + p] = jm * out[c];
 }
 
+// Gradients of w = (u_1..u_3, T) on a fully periodic 3-D grid in one pass: a BX x BY
+// column tile marches along z over a chunk of ZC planes.  Each thread keeps w of its
+// column for the 9 planes of the z stencil in registers; the plane being
+// differentiated is staged with a 4-point x / y halo in LDS.  Same sums as
+// k_prim + k_grad (interior SBP rows, wrap-around indices), so the same values.
+template <int BX, int BY, int ZC>
+__global__ void __launch_bounds__(BX * BY) k_grad3d_periodic(Geo g, const double* __restrict__ q,
+                                                              double* __restrict__ G) {
+  constexpr int NT = BX * BY, EXW = BX + 8, EYW = BY + 8, NW = 4;
+  __shared__ double sw[NW][EYW][EXW];
+  const int ntx = g.n[0] / BX;
+  const int x0 = (blockIdx.x % ntx) * BX, y0 = (blockIdx.x / ntx) * BY;
+  const int z0 = blockIdx.y * ZC;
+  const long inst = blockIdx.z;
+  const double* qi = q + inst * 5 * g.N;
+  double* Gi = G + inst * 4 * 3 * g.N;
+  const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
+  const int i = x0 + tx, j = y0 + ty;
+  const Dir dx = g.dir[0], dy = g.dir[1], dz = g.dir[2];
+  const long n01 = (long)g.n[0] * g.n[1];
+  auto load_w = [&](int ii, int jj, int kk, double* w) {
+    const long p = ii + (long)g.n[0] * jj + n01 * kk;
+    double v[5];
+    for (int l = 0; l < 5; ++l) v[l] = qi[l * g.N + p];
+    const Prim<3> s = prim<3>(v, g.gamma);
+    w[0] = s.u[0]; w[1] = s.u[1]; w[2] = s.u[2]; w[3] = s.T;
+  };
+  double Q[9][NW];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) load_w(i, j, wrap(z0 - 4 + k, dz), Q[k]);
+  const Op& op = g.ops[SLOT_FIRST];
+  const double c1 = op.c[kMaxHW + 1], c2 = op.c[kMaxHW + 2], c3 = op.c[kMaxHW + 3], c4 = op.c[kMaxHW + 4];
+  for (int s = 0; s < ZC; ++s) {
+    const int zc = z0 + s;
+    if (zc >= g.n[2]) break;   // uniform over the block
+    load_w(i, j, wrap(zc + 4, dz), Q[8]);
+    __syncthreads();            // readers of the previous plane are done
+#pragma unroll
+    for (int c = 0; c < NW; ++c) sw[c][ty + 4][tx + 4] = Q[4][c];
+    for (int e = threadIdx.x; e < EXW * EYW; e += NT) {
+      const int ex = e % EXW, ey = e / EXW;
+      const bool inx = ex >= 4 && ex < BX + 4, iny = ey >= 4 && ey < BY + 4;
+      if (inx == iny) continue;   // interior (from the queues) or corner (unused)
+      double w[NW];
+      load_w(wrap(x0 + ex - 4, dx), wrap(y0 + ey - 4, dy), zc, w);
+#pragma unroll
+      for (int c = 0; c < NW; ++c) sw[c][ey][ex] = w[c];
+    }
+    __syncthreads();
+    int ijk[3] = {i, j, zc};
+    const long p = i + (long)g.n[0] * j + n01 * zc;
+    const double jac = jacobian<3>(g, ijk);
+#pragma unroll
+    for (int c = 0; c < NW; ++c) {
+      const double* row = &sw[c][ty + 4][tx + 4];
+      double ox = 0.0, oy = 0.0, oz = 0.0;
+      ox += c1 * (row[1] - row[-1]);
+      ox += c2 * (row[2] - row[-2]);
+      ox += c3 * (row[3] - row[-3]);
+      ox += c4 * (row[4] - row[-4]);
+      oy += c1 * (row[EXW] - row[-EXW]);
+      oy += c2 * (row[2 * EXW] - row[-2 * EXW]);
+      oy += c3 * (row[3 * EXW] - row[-3 * EXW]);
+      oy += c4 * (row[4 * EXW] - row[-4 * EXW]);
+      oz += c1 * (Q[5][c] - Q[3][c]);
+      oz += c2 * (Q[6][c] - Q[2][c]);
+      oz += c3 * (Q[7][c] - Q[1][c]);
+      oz += c4 * (Q[8][c] - Q[0][c]);
+      Gi[(c * 3 + 0) * g.N + p] = (jac * metric<3>(g, 0, ijk)) * ox;
+      Gi[(c * 3 + 1) * g.N + p] = (jac * metric<3>(g, 1, ijk)) * oy;
+      Gi[(c * 3 + 2) * g.N + p] = (jac * metric<3>(g, 2, ijk)) * oz;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int c = 0; c < NW; ++c) Q[k][c] = Q[k + 1][c];
+  }
+}
+
 // Stress tensor and heat flux at one point from the stored gradients.
 template <int ND>
 __device__ __forceinline__ void visc_point(const Geo& g, const double* Gi, long N, long p,
@@ -853,16 +932,33 @@ static dim3 grid_pts(const Geo& g) {
   return dim3((unsigned)((g.N + kBlock - 1) / kBlock), (unsigned)g.ninst);
 }
 
+// k_prim + k_grad, or the one-pass periodic 3-D gradient kernel where it applies
+// (3-D, every direction periodic, x / y sizes multiples of its 32 x 16 tile).
+constexpr int kG3X = 32, kG3Y = 16, kG3Z = 64;
+static bool grad3d_applies(const Geo& g) {
+  return g.nd == 3 && g.dir[0].periodic && g.dir[1].periodic && g.dir[2].periodic &&
+         g.n[0] % kG3X == 0 && g.n[1] % kG3Y == 0 && g.nv == 5;
+}
+
+template <int ND>
+static void launch_gradients(const Geo& g, const double* q, double* w, double* G, const dim3& grid,
+                             hipStream_t st) {
+  if (ND == 3 && grad3d_applies(g)) {
+    const dim3 gg((g.n[0] / kG3X) * (g.n[1] / kG3Y), (g.n[2] + kG3Z - 1) / kG3Z, g.ninst);
+    k_grad3d_periodic<kG3X, kG3Y, kG3Z><<<gg, kG3X * kG3Y, 0, st>>>(g, q, G);
+    return;
+  }
+  k_prim<ND><<<grid, kBlock, 0, st>>>(g, q, w);
+  for (int d = 0; d < ND; ++d) k_grad<ND><<<grid, kBlock, 0, st>>>(g, d, w, G);
+}
+
 void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
                         double* G, double* t, double* F, const double* control, long control_stride,
                         const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
                         int has_farfield, hipStream_t st) {
   const dim3 grid = grid_pts(g);
   MGD_DISPATCH(g.nd, {
-    if (g.viscous) {
-      k_prim<ND><<<grid, kBlock, 0, st>>>(g, q, w);
-      for (int d = 0; d < ND; ++d) k_grad<ND><<<grid, kBlock, 0, st>>>(g, d, w, G);
-    }
+    if (g.viscous) launch_gradients<ND>(g, q, w, G, grid, st);
     if (F) {
       k_flux_all<ND><<<grid, kBlock, 0, st>>>(g, q, g.viscous ? G : nullptr, F);
       for (int d = 0; d < ND; ++d) k_div<ND><<<grid, kBlock, 0, st>>>(g, d, F, rhs);
@@ -891,10 +987,7 @@ void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
   const dim3 grid = grid_pts(g);
   MGD_DISPATCH(g.nd, {
     constexpr int NV = ND + 2;
-    if (g.viscous) {
-      k_prim<ND><<<grid, kBlock, 0, st>>>(g, q, w);
-      for (int d = 0; d < ND; ++d) k_grad<ND><<<grid, kBlock, 0, st>>>(g, d, w, G);
-    }
+    if (g.viscous) launch_gradients<ND>(g, q, w, G, grid, st);
     for (int d = 0; d < ND; ++d)
       k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, lam, (long)NV * g.N, T1, 0);
     k_adj_jac<ND><<<grid, kBlock, 0, st>>>(g, q, G, T1, rhs);

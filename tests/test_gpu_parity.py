@@ -54,6 +54,31 @@ def kolmogorov_small(n=16):
     return c
 
 
+def periodic_box(shape, seed=3):
+    """Viscous, fully periodic 3-D box of the given (nx, ny, nz) with a different grid
+    spacing per axis: sizes whose x / y extents are multiples of 32 / 16 take the
+    one-pass periodic gradient kernel, with partial and multiple z chunks."""
+    nx, ny, nz = shape
+    x1 = [np.arange(m) / m for m in shape]
+    Z, Y, X = np.meshgrid(x1[2], x1[1], x1[0], indexing="ij")
+    npts = nx * ny * nz
+    q = cases.quiescent_state(3, npts)
+    rng = np.random.default_rng(seed)
+    for c in range(3):
+        q[1 + c] = 0.05 * np.sin(2 * np.pi * (X + 2 * Y + Z) + rng.uniform(0, 6)).ravel()
+    q[4] = q[4] + 0.5 * (q[1:4] ** 2).sum(axis=0)
+    return cases.Case(
+        name=f"box{nx}x{ny}x{nz}", nd=3, n=(nx, ny, nz),
+        coords=np.stack([X.ravel(), Y.ravel(), Z.ravel()]),
+        periodicity=(cases.PLANE,) * 3, periodic_length=(1.0, 1.0, 1.0),
+        viscous=True, dissipation=True, dissipation_amount=0.01, reynolds=500.0,
+        prandtl=0.7, power_law=0.666, bulk_ratio=0.6,
+        patches=[cases.Patch("forcing", cases.KOLMOGOROV, 0, (1, -1, 1, -1, 1, -1),
+                             (2.0, 0.01, 0, 0))],
+        target=cases.quiescent_state(3, npts), control_mollifier=np.zeros(npts),
+        target_mollifier=np.zeros(npts), dt=0.002, nsteps=1, save_interval=1, initial=q)
+
+
 CASES = {
     "monopole41": lambda: cases.acoustic_monopole(n=41, nsteps=1),
     "monopole41_composite": lambda: cases.acoustic_monopole(n=41, nsteps=1, composite=True),
@@ -61,6 +86,8 @@ CASES = {
     "kolmogorov16": kolmogorov_small,
     "walled41": lambda: cases.walled_monopole(n=41),
     "walled41_viscous": lambda: cases.walled_monopole(n=41, viscous=True),
+    "box32x16x12": lambda: periodic_box((32, 16, 12)),
+    "box64x32x80": lambda: periodic_box((64, 32, 80)),
 }
 
 
