@@ -920,6 +920,109 @@ __global__ void __launch_bounds__(kBlock) k_gradient(Geo g, const DPatch* __rest
   out[inst * inst_stride + k] = lam[inst * g.nv * g.N + (ND + 1) * g.N + p] * (pt.strength[k] * 1.0);
 }
 
+// x and y parts of the flux divergence on a 3-D grid periodic in x and y, one z plane
+// per block row: each block stages the contravariant fluxes of its BX x BY points (x
+// rows and y columns, with the 4-point halo recomputed from q and G) in LDS and writes
+// D_x Fx + D_y Fy to rhs and Fz to its slot of F, for k_div along z to finish.  The
+// per-point flux and the derivative sums are those of k_flux_all and k_div (same
+// operations in the same order, so the same values).
+template <int BX, int BY>
+__global__ void __launch_bounds__(BX * BY) k_fluxdiv3d_xy(Geo g, const double* __restrict__ q,
+                                                           const double* __restrict__ G,
+                                                           double* __restrict__ F,
+                                                           double* __restrict__ rhs) {
+  constexpr int NV = 5, EXW = BX + 8, EYW = BY + 8;
+  static_assert(8 * (BX + BY) == BX * BY, "one halo point per thread");
+  __shared__ double sfx[NV][BY][EXW];
+  __shared__ double sfy[NV][EYW][BX];
+  const int ntx = g.n[0] / BX;
+  const int x0 = (blockIdx.x % ntx) * BX, y0 = (blockIdx.x / ntx) * BY, z = blockIdx.y;
+  const long inst = blockIdx.z;
+  const double* qi = q + inst * NV * g.N;
+  const double* Gi = g.viscous ? G + inst * 12 * g.N : nullptr;
+  const long n01 = (long)g.n[0] * g.n[1];
+  // flux of direction d at grid point (ii, jj, z) for each d with want[d]
+  auto fluxes = [&](int ii, int jj, const bool* want, double (*out)[NV]) {
+    const long pp = ii + (long)g.n[0] * jj + n01 * z;
+    double v[NV];
+    load_q<3>(qi, g.N, pp, v);
+    const Prim<3> s = prim<3>(v, g.gamma);
+    double tau[9], hf[3], mu, la, ka;
+    if (g.viscous) visc_point<3>(g, Gi, g.N, pp, s, tau, hf, &mu, &la, &ka);
+    const int ijk[3] = {ii, jj, z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (!want[d]) continue;
+      double f[NV];
+      inviscid_flux<3>(v, s, d, f);
+      if (g.viscous) {
+        double fv[NV];
+        viscous_flux<3>(s.u, tau, hf, d, fv);
+#pragma unroll
+        for (int l = 0; l < NV; ++l) f[l] = f[l] - fv[l];
+      }
+      const double m = metric<3>(g, d, ijk);
+#pragma unroll
+      for (int l = 0; l < NV; ++l) out[d][l] = m * f[l];
+    }
+  };
+  const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
+  const int i = x0 + tx, j = y0 + ty;
+  const long p = i + (long)g.n[0] * j + n01 * z;
+  {
+    const bool all[3] = {true, true, true};
+    double f[3][NV];
+    fluxes(i, j, all, f);
+    double* Fz = F + (inst * 3 + 2) * NV * g.N;
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      sfx[l][ty][tx + 4] = f[0][l];
+      sfy[l][ty + 4][tx] = f[1][l];
+      Fz[l * g.N + p] = f[2][l];
+    }
+  }
+  {
+    // halo: threads [0, 8 BY) take the x halo columns, the rest the y halo rows
+    const int e = threadIdx.x;
+    double f[3][NV];
+    if (e < 8 * BY) {
+      const int hx = e % 8, hy = e / 8;
+      const int ex = hx < 4 ? hx : BX + hx;
+      const bool wx[3] = {true, false, false};
+      fluxes(wrap(x0 + ex - 4, g.dir[0]), y0 + hy, wx, f);
+#pragma unroll
+      for (int l = 0; l < NV; ++l) sfx[l][hy][ex] = f[0][l];
+    } else {
+      const int e2 = e - 8 * BY;
+      const int hx = e2 % BX, hy = e2 / BX;
+      const int ey = hy < 4 ? hy : BY + hy;
+      const bool wy[3] = {false, true, false};
+      fluxes(x0 + hx, wrap(y0 + ey - 4, g.dir[1]), wy, f);
+#pragma unroll
+      for (int l = 0; l < NV; ++l) sfy[l][ey][hx] = f[1][l];
+    }
+  }
+  __syncthreads();
+  const Op& op = g.ops[SLOT_FIRST];
+  const double c1 = op.c[kMaxHW + 1], c2 = op.c[kMaxHW + 2], c3 = op.c[kMaxHW + 3], c4 = op.c[kMaxHW + 4];
+  double* ri = rhs + inst * NV * g.N;
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    const double* rx = &sfx[l][ty][tx + 4];
+    const double* cy = &sfy[l][ty + 4][tx];
+    double ox = 0.0, oy = 0.0;
+    ox += c1 * (rx[1] - rx[-1]);
+    ox += c2 * (rx[2] - rx[-2]);
+    ox += c3 * (rx[3] - rx[-3]);
+    ox += c4 * (rx[4] - rx[-4]);
+    oy += c1 * (cy[BX] - cy[-BX]);
+    oy += c2 * (cy[2 * BX] - cy[-2 * BX]);
+    oy += c3 * (cy[3 * BX] - cy[-3 * BX]);
+    oy += c4 * (cy[4 * BX] - cy[-4 * BX]);
+    ri[l * g.N + p] = ox + oy;
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 #define MGD_DISPATCH(ND_, ...)                                  \
   switch (ND_) {                                                \
@@ -938,6 +1041,14 @@ constexpr int kG3X = 32, kG3Y = 16, kG3Z = 64;
 static bool grad3d_applies(const Geo& g) {
   return g.nd == 3 && g.dir[0].periodic && g.dir[1].periodic && g.dir[2].periodic &&
          g.n[0] % kG3X == 0 && g.n[1] % kG3Y == 0 && g.nv == 5;
+}
+
+// x / y flux divergence kernel of 3-D grids periodic in x and y with sizes that are
+// multiples of its 16 x 16 tile.
+constexpr int kFD3 = 16;
+static bool fluxdiv3d_applies(const Geo& g) {
+  return g.nd == 3 && g.nv == 5 && g.dir[0].periodic && g.dir[1].periodic &&
+         g.n[0] % kFD3 == 0 && g.n[1] % kFD3 == 0;
 }
 
 template <int ND>
@@ -959,7 +1070,11 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
   const dim3 grid = grid_pts(g);
   MGD_DISPATCH(g.nd, {
     if (g.viscous) launch_gradients<ND>(g, q, w, G, grid, st);
-    if (F) {
+    if (F && ND == 3 && fluxdiv3d_applies(g)) {
+      const dim3 gg((g.n[0] / kFD3) * (g.n[1] / kFD3), g.n[2], g.ninst);
+      k_fluxdiv3d_xy<kFD3, kFD3><<<gg, kFD3 * kFD3, 0, st>>>(g, q, g.viscous ? G : nullptr, F, rhs);
+      k_div<ND><<<grid, kBlock, 0, st>>>(g, ND - 1, F, rhs);
+    } else if (F) {
       k_flux_all<ND><<<grid, kBlock, 0, st>>>(g, q, g.viscous ? G : nullptr, F);
       for (int d = 0; d < ND; ++d) k_div<ND><<<grid, kBlock, 0, st>>>(g, d, F, rhs);
     } else {
