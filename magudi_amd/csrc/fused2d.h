@@ -11,24 +11,25 @@ namespace mgd {
 constexpr int kNumTileShapes = 4;
 constexpr int kNumXcd = 8;   // XCDs of an MI355X; work-groups are dealt to them round-robin
 
-// Tile partition of a 2-D grid (device view): tiles of at most TX x TY points with
-// origins org[0..nbx) along x and org[nbx..nbx+nby) along y; pmask: patches per tile
-// (tile index bx + nbx * by); tile_of_block: launch block -> tile, or -1.
+// Tile partition of a 2-D grid (device view): nbx x nby tiles of at most TX x TY points
+// from a balanced partition of each direction -- tile b along x starts at
+// b qx + min(b, rx) and holds qx + (b < rx) points (qy, ry along y) -- and
+// pmask: patches per tile (tile index bx + nbx * by).  Launch block k works on tile
+// (k % kNumXcd) band + k / kNumXcd, or on none when that is beyond the last tile.
 struct Tiling {
   int shape;
-  int nbx, nby, nblk;
-  const int* org;
-  const int* size;
+  int nbx, nby, nblk, band;
+  int qx, rx, qy, ry;
   const unsigned* pmask;
-  const int* tile_of_block;
 };
 
 struct HostTiling {
   int shape = 0;
-  int nbx = 0, nby = 0, nblk = 0;
-  std::vector<int> org, size;
+  int nbx = 0, nby = 0, nblk = 0, band = 0;
+  int qx = 0, rx = 0, qy = 0, ry = 0;
+  std::vector<int> org, size;          // tile origins / sizes: x tiles, then y tiles
   std::vector<unsigned> pmask;
-  std::vector<int> tile_of_block;
+  std::vector<int> tile_of_block;      // launch block -> tile, or -1
 };
 
 // One forward RK4 stage: rhs at `qin`, buffer update of `b2`, new stage state to `qout`.

@@ -42,6 +42,9 @@ struct Shape {
   static constexpr int SFX = NV2 * TY * EX, SFY = NV2 * EY * TX;
   static constexpr int STX = NV2 * TY * (TX + 4), STY = NV2 * (TY + 4) * TX;
   static constexpr int SSCR = (SFX + SFY) > (STX + STY) ? (SFX + SFY) : (STX + STY);
+  // waves per SIMD that the LDS use allows (2 blocks of 512 threads or 3 of 256 per CU):
+  // the register budget is sized to it
+  static constexpr int kWavesPerEU = NT >= 512 ? 4 : 3;
 };
 
 struct TileGeo {
@@ -49,16 +52,85 @@ struct TileGeo {
   int ex0, ey0;         // global index of LDS column / row 0 (origin - halo)
 };
 
+// Tile of this launch block (XCD band map), or -1; its geometry from the balanced
+// partition, with no memory access.
+__device__ __forceinline__ int block_tile(const Tiling& T) {
+  const int b = blockIdx.x;
+  const int tile = (b % kNumXcd) * T.band + b / kNumXcd;
+  return tile < T.nbx * T.nby ? tile : -1;
+}
+
 __device__ __forceinline__ TileGeo tile_geo(const Tiling& T, int tile) {
   const int bx = tile % T.nbx, by = tile / T.nbx;
   TileGeo t;
-  t.x0 = T.org[bx];
-  t.sx = T.size[bx];
-  t.y0 = T.org[T.nbx + by];
-  t.sy = T.size[T.nbx + by];
+  t.x0 = bx * T.qx + min(bx, T.rx);
+  t.sx = T.qx + (bx < T.rx ? 1 : 0);
+  t.y0 = by * T.qy + min(by, T.ry);
+  t.sy = T.qy + (by < T.ry ? 1 : 0);
   t.ex0 = t.x0 - HX;
   t.ey0 = t.y0 - HY;
   return t;
+}
+
+// Interior stencil coefficients of the operators a stage kernel applies, read once at
+// kernel entry: d1[k] = c_k of the first-derivative-type operator (k = 1..4), dd / dt:
+// c_0..c_2 of the dissipation operator and of its transpose.
+struct Coefs {
+  double d1[5], dd[3], dt[3];
+};
+
+__device__ __forceinline__ Coefs load_coefs(const Geo& g, int first_slot) {
+  Coefs c;
+  const Op& o1 = g.ops[first_slot];
+  const Op& od = g.ops[SLOT_DISS];
+  const Op& ot = g.ops[SLOT_DISS_T];
+  c.d1[0] = 0.0;
+#pragma unroll
+  for (int k = 1; k <= 4; ++k) c.d1[k] = o1.c[kMaxHW + k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    c.dd[k] = od.c[kMaxHW + k];
+    c.dt[k] = ot.c[kMaxHW + k];
+  }
+  return c;
+}
+
+// The 1-D grid values of a tile's columns (x) and rows (y) -- h, the norm weight and the
+// coordinate -- held in LDS for the whole block: one thread loads each value at kernel
+// entry, so the later phases read them from LDS instead of each lane fetching its own
+// copy from memory.  Layout: [h | norm weight | coordinate], each TX column values
+// then TY row values.
+template <class S>
+struct TileGrid {
+  static constexpr int kRow = S::TX + S::TY;
+  static constexpr int kSize = 3 * kRow;
+  static_assert(kSize <= S::NT, "one grid value per thread");
+  const double* s;
+  __device__ double h0(int tx) const { return s[tx]; }
+  __device__ double h1(int ty) const { return s[S::TX + ty]; }
+  __device__ double hw0(int tx) const { return s[kRow + tx]; }
+  __device__ double hw1(int ty) const { return s[kRow + S::TX + ty]; }
+  __device__ double x0(int tx) const { return s[2 * kRow + tx]; }
+  __device__ double x1(int ty) const { return s[2 * kRow + S::TX + ty]; }
+};
+
+// This thread's grid value (threads below TileGrid::kSize), to be stored at its index.
+template <class S>
+__device__ __forceinline__ double load_tile_grid(const Geo& g, const TileGeo& t) {
+  constexpr int kRow = TileGrid<S>::kRow;
+  const int e = threadIdx.x;
+  if (e >= TileGrid<S>::kSize) return 0.0;
+  const int what = e / kRow, i = e % kRow;
+  const int d = i < S::TX ? 0 : 1;
+  const int k = d == 0 ? i : i - S::TX;
+  if (k >= (d == 0 ? t.sx : t.sy)) return 0.0;
+  const int idx = (d == 0 ? t.x0 : t.y0) + k;
+  // constant indices into the kernel-argument arrays (a run-time index would copy the
+  // argument block to scratch memory)
+  const double* src = what == 0 ? (d == 0 ? g.h[0] : g.h[1])
+                    : what == 1 ? (d == 0 ? g.hw[0] : g.hw[1])
+                                : (d == 0 ? g.x[0] : g.x[1]);
+  return src[idx];
 }
 
 __device__ __forceinline__ bool closure(const Dir& dd, int i, int w) {
@@ -72,55 +144,71 @@ __device__ __forceinline__ int ext_index(const Dir& dd, int gi) {
   return gi < 0 ? gi + dd.len - dd.off : gi - dd.len + dd.off;
 }
 
-// Stages the operand of this tile (all four components) into sq.  For the forward
-// stage the contravariant fluxes of the tile rows (x) and tile columns (y) go to sfx/sfy.
-template <class S, bool FLUX>
-__device__ __forceinline__ void stage_operand(const Geo& g, const TileGeo& t, const double* xi,
-                                              double* sq, double* sfx, double* sfy) {
-  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NT = S::NT;
-  constexpr int kStageIters = S::kStageIters;
-  const Dir dx = g.dir[0], dy = g.dir[1];
-  const int wx = t.sx + 2 * HX, wy = t.sy + 2 * HY;
-  // all loads of this thread are issued before any is consumed
-  double v[kStageIters][NV2];
-  int ex_[kStageIters], ey_[kStageIters], gi_[kStageIters], gj_[kStageIters];
-  bool ok[kStageIters];
+// Staged operand of this tile (all four components) in registers: stage_load issues
+// every load of this thread before any is consumed; stage_store writes them to sq and,
+// for the forward stage, the contravariant fluxes of the tile rows (x) and tile columns
+// (y) to sfx / sfy.
+template <class S>
+struct StageRegs {
+  double v[S::kStageIters][NV2];
+};
+
+// Staging slot `it` of this thread: extended-layout indices (ex, ey), the grid point
+// (gi, gj) and whether the slot holds a staged point (the corners and the columns /
+// rows beyond a partial tile are skipped).
+template <class S>
+__device__ __forceinline__ bool stage_slot(const Geo& g, const TileGeo& t, int it, int* ex,
+                                           int* ey, int* gi, int* gj) {
+  constexpr int EX = S::EX, EY = S::EY;
+  const int e = threadIdx.x + it * S::NT;
+  *ex = e % EX;
+  *ey = e / EX;
+  *gi = ext_index(g.dir[0], t.ex0 + *ex);
+  *gj = ext_index(g.dir[1], t.ey0 + *ey);
+  const bool rowx = *ey >= HY && *ey < HY + t.sy;
+  const bool coly = *ex >= HX && *ex < HX + t.sx;
+  return e < EX * EY && *ex < t.sx + 2 * HX && *ey < t.sy + 2 * HY && *gi >= 0 && *gj >= 0 &&
+         (rowx || coly);
+}
+
+template <class S>
+__device__ __forceinline__ void stage_load(const Geo& g, const TileGeo& t, const double* xi,
+                                           StageRegs<S>& r) {
 #pragma unroll
-  for (int it = 0; it < kStageIters; ++it) {
-    // walk the EX x EY staging layout (compile-time width: no integer division by a
-    // run-time value); columns / rows beyond a partial tile are skipped
-    const int e = threadIdx.x + it * NT;
-    const int ex = e % EX, ey = e / EX;
-    const int gi = ext_index(dx, t.ex0 + ex), gj = ext_index(dy, t.ey0 + ey);
-    const bool rowx = ey >= HY && ey < HY + t.sy;
-    const bool coly = ex >= HX && ex < HX + t.sx;
-    ok[it] = e < EX * EY && ex < wx && ey < wy && gi >= 0 && gj >= 0 && (rowx || coly);
-    ex_[it] = ex; ey_[it] = ey; gi_[it] = gi; gj_[it] = gj;
-    if (ok[it]) {
+  for (int it = 0; it < S::kStageIters; ++it) {
+    int ex, ey, gi, gj;
+    if (stage_slot<S>(g, t, it, &ex, &ey, &gi, &gj)) {
       const long pp = gi + (long)g.n[0] * gj;
 #pragma unroll
-      for (int l = 0; l < NV2; ++l) v[it][l] = xi[l * g.N + pp];
+      for (int l = 0; l < NV2; ++l) r.v[it][l] = xi[l * g.N + pp];
     }
   }
+}
+
+template <class S, bool FLUX>
+__device__ __forceinline__ void stage_store(const Geo& g, const TileGeo& t, const StageRegs<S>& r,
+                                            const TileGrid<S>& tg, double* sq, double* sfx,
+                                            double* sfy) {
+  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY;
 #pragma unroll
-  for (int it = 0; it < kStageIters; ++it) {
-    if (!ok[it]) continue;
-    const int ex = ex_[it], ey = ey_[it];
+  for (int it = 0; it < S::kStageIters; ++it) {
+    int ex, ey, gi, gj;
+    if (!stage_slot<S>(g, t, it, &ex, &ey, &gi, &gj)) continue;
 #pragma unroll
-    for (int l = 0; l < NV2; ++l) sq[(l * EY + ey) * EX + ex] = v[it][l];
+    for (int l = 0; l < NV2; ++l) sq[(l * EY + ey) * EX + ex] = r.v[it][l];
     if (FLUX) {
-      const Prim<2> s = prim<2>(v[it], g.gamma);
+      const Prim<2> s = prim<2>(r.v[it], g.gamma);
       if (ey >= HY && ey < HY + t.sy) {
         double f[NV2];
-        inviscid_flux<2>(v[it], s, 0, f);
-        const double m = g.h[1][gj_[it]];
+        inviscid_flux<2>(r.v[it], s, 0, f);
+        const double m = tg.h1(ey - HY);   // metric<2>(g, 0, .) of the row
 #pragma unroll
         for (int l = 0; l < NV2; ++l) sfx[(l * TY + (ey - HY)) * EX + ex] = m * f[l];
       }
       if (ex >= HX && ex < HX + t.sx) {
         double f[NV2];
-        inviscid_flux<2>(v[it], s, 1, f);
-        const double m = g.h[0][gi_[it]];
+        inviscid_flux<2>(r.v[it], s, 1, f);
+        const double m = tg.h0(ex - HX);   // metric<2>(g, 1, .) of the column
 #pragma unroll
         for (int l = 0; l < NV2; ++l) sfy[(l * EY + ey) * TX + (ex - HX)] = m * f[l];
       }
@@ -153,8 +241,8 @@ __device__ __forceinline__ void closure_row(const Op& op, int i, int len, F&& f,
 // First-derivative-type operator (BW x BD closure) along x or y at a point of the tile,
 // operand in LDS; row(l, e) returns component l at extended index e.
 template <int BW, int BD, class Row>
-__device__ __forceinline__ void deriv_lds(const Op& op, const Dir& dd, int i, int e0, Row&& row,
-                                          double* out) {
+__device__ __forceinline__ void deriv_lds(const Op& op, const double* ck, const Dir& dd, int i,
+                                          int e0, Row&& row, double* out) {
   if (!dd.periodic && (i < BD || i >= dd.len - BD)) {
     closure_row<BW, BD>(op, i, dd.len,
                         [&](int idx, double* v) {
@@ -169,7 +257,7 @@ __device__ __forceinline__ void deriv_lds(const Op& op, const Dir& dd, int i, in
   for (int l = 0; l < NV2; ++l) {
     double s = 0.0;
 #pragma unroll
-    for (int k = 1; k <= 4; ++k) s += op.c[kMaxHW + k] * (row(l, e + k) - row(l, e - k));
+    for (int k = 1; k <= 4; ++k) s += ck[k] * (row(l, e + k) - row(l, e - k));
     out[l] = s;
   }
 }
@@ -199,19 +287,22 @@ __device__ __forceinline__ double d4_at(const Dir& dd, int i, double c0, double 
 // t = -arc D4 x on the tile rows (x direction, columns x0-2 .. x0+sx+1) and tile columns
 // (y direction, rows y0-2 .. y0+sy+1), for staged points inside the domain.
 template <class S>
-__device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& t, const double* sq,
+__device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& t, const Coefs& cf,
+                                                   const TileGrid<S>& tg, const double* sq,
                                                    double* stx, double* sty) {
   constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NT = S::NT;
-  const Op& od = g.ops[SLOT_DISS];
-  const double c0 = od.c[kMaxHW], c1 = od.c[kMaxHW + 1], c2 = od.c[kMaxHW + 2];
+  const double c0 = cf.dd[0], c1 = cf.dd[1], c2 = cf.dd[2];
   const Dir dx = g.dir[0], dy = g.dir[1];
   const int wx = t.sx + 4;
-  for (int e = threadIdx.x; e < TY * (TX + 4); e += NT) {
+#pragma unroll
+  for (int it = 0; it < (TY * (TX + 4) + NT - 1) / NT; ++it) {
+    const int e = threadIdx.x + it * NT;
+    if (e >= TY * (TX + 4)) continue;
     const int ex = e % (TX + 4) + 2, ey = e / (TX + 4) + HY;
     if (ex - 2 >= wx || ey - HY >= t.sy) continue;
     const int gi = t.ex0 + ex;
     if (!dx.periodic && (gi < 0 || gi >= dx.len)) continue;
-    const double arc = fabs(g.h[1][t.y0 + ey - HY]);
+    const double arc = fabs(tg.h1(ey - HY));
 #pragma unroll
     for (int l = 0; l < NV2; ++l) {
       const double* r = sq + (l * EY + ey) * EX;
@@ -220,12 +311,15 @@ __device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& 
     }
   }
   const int hy = t.sy + 4;
-  for (int e = threadIdx.x; e < (TY + 4) * TX; e += NT) {
+#pragma unroll
+  for (int it = 0; it < ((TY + 4) * TX + NT - 1) / NT; ++it) {
+    const int e = threadIdx.x + it * NT;
+    if (e >= (TY + 4) * TX) continue;
     const int ex = e % TX + HX, ey = e / TX + 2;
     if (ex - HX >= t.sx || ey - 2 >= hy) continue;
     const int gj = t.ey0 + ey;
     if (!dy.periodic && (gj < 0 || gj >= dy.len)) continue;
-    const double arc = fabs(g.h[0][t.x0 + ex - HX]);
+    const double arc = fabs(tg.h0(ex - HX));
 #pragma unroll
     for (int l = 0; l < NV2; ++l) {
       const double* c = sq + l * EY * EX + ex;
@@ -238,7 +332,8 @@ __device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& 
 // Adds amount * (dissipation in x then in y) at one point: H^-1 D4^T t (closure rows of
 // D4^T near non-periodic boundaries) or the composite operator on the staged operand.
 template <class S>
-__device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, const double* sq,
+__device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, const Coefs& cf,
+                                                const TileGrid<S>& tg, const double* sq,
                                                 const double* stx, const double* sty, int tx,
                                                 int ty, int gi, int gj, double amount, double* r) {
   constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY;
@@ -285,7 +380,7 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
                           },
                           dv);
       } else {
-        const double c0 = ot.c[kMaxHW], c1 = ot.c[kMaxHW + 1], c2 = ot.c[kMaxHW + 2];
+        const double c0 = cf.dt[0], c1 = cf.dt[1], c2 = cf.dt[2];
 #pragma unroll
         for (int l = 0; l < NV2; ++l) {
           double s = 0.0;
@@ -296,7 +391,7 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
       }
       // H^-1: the norm weight is exactly 1 away from the first / last boundary rows, where
       // the division is the identity and is skipped
-      const double h = g.hw[d][i];
+      const double h = d == 0 ? tg.hw0(tx) : tg.hw1(ty);
       if (h != 1.0) {
 #pragma unroll
         for (int l = 0; l < NV2; ++l) dv[l] = dv[l] / h;
@@ -327,13 +422,14 @@ __device__ __forceinline__ bool in_patch(const DPatch& pt, int i, int j) {
 
 // -------------------------------------------------------------------------- forward
 template <class S>
-__global__ void __launch_bounds__(S::NT, 4) k_fwd_stage2d(Geo g, const Patches* __restrict__ P,
+__global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_fwd_stage2d(Geo g, const Patches* __restrict__ P,
                                                        Tiling T, FwdStage a) {
   constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NT = S::NT;
   constexpr int SFX = S::SFX, STX = S::STX;
   __shared__ double sq[S::SQ];
   __shared__ double sc[S::SSCR];
-  const int tile = T.tile_of_block[blockIdx.x];
+  __shared__ double sgrid[TileGrid<S>::kSize];
+  const int tile = block_tile(T);
   if (tile < 0) return;   // block beyond the tiles of its XCD (uniform per block)
   const TileGeo t = tile_geo(T, tile);
   const long inst = blockIdx.y;
@@ -341,6 +437,10 @@ __global__ void __launch_bounds__(S::NT, 4) k_fwd_stage2d(Geo g, const Patches* 
   const double* qi = a.qin + inst * NVN;
   double* sfx = sc;
   double* sfy = sc + SFX;
+  const Coefs cf = load_coefs(g, SLOT_FIRST);
+  const double gv = load_tile_grid<S>(g, t);
+  const TileGrid<S> tg{sgrid};
+  const unsigned pmask = T.pmask[tile];
   // RK operands of this thread's point, fetched before the stencil work
   double pq0[NV2], pb2[NV2];
   {
@@ -354,7 +454,13 @@ __global__ void __launch_bounds__(S::NT, 4) k_fwd_stage2d(Geo g, const Patches* 
       }
     }
   }
-  stage_operand<S, true>(g, t, qi, sq, sfx, sfy);
+  StageRegs<S> sr;
+  stage_load<S>(g, t, qi, sr);
+  // the grid values are needed by the flux metrics; this barrier waits for them only
+  // (the operand loads stay in flight)
+  if (threadIdx.x < TileGrid<S>::kSize) sgrid[threadIdx.x] = gv;
+  __syncthreads();
+  stage_store<S, true>(g, t, sr, tg, sq, sfx, sfy);
   __syncthreads();
   const Op& o1 = g.ops[SLOT_FIRST];
   double r[PPT][NV2];
@@ -365,9 +471,9 @@ __global__ void __launch_bounds__(S::NT, 4) k_fwd_stage2d(Geo g, const Patches* 
     if (tx >= t.sx || ty >= t.sy) continue;
     const int gi = t.x0 + tx, gj = t.y0 + ty;
     double dfx[NV2], dfy[NV2];
-    deriv_lds<12, 8>(o1, g.dir[0], gi, t.ex0,
+    deriv_lds<12, 8>(o1, cf.d1, g.dir[0], gi, t.ex0,
               [&](int l, int e) { return sfx[(l * TY + ty) * EX + e]; }, dfx);
-    deriv_lds<12, 8>(o1, g.dir[1], gj, t.ey0,
+    deriv_lds<12, 8>(o1, cf.d1, g.dir[1], gj, t.ey0,
               [&](int l, int e) { return sfy[(l * EY + e) * TX + tx]; }, dfy);
 #pragma unroll
     for (int l = 0; l < NV2; ++l) r[k][l] = 0.0 - (dfx[l] + dfy[l]);
@@ -377,20 +483,19 @@ __global__ void __launch_bounds__(S::NT, 4) k_fwd_stage2d(Geo g, const Patches* 
   if (g.dissipation) {
     __syncthreads();   // fluxes consumed
     if (!g.composite) {
-      diss_intermediates<S>(g, t, sq, stx, sty);
+      diss_intermediates<S>(g, t, cf, tg, sq, stx, sty);
       __syncthreads();
     }
   }
-  const unsigned pmask = T.pmask[tile];
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
     const int ty = threadIdx.x / TX + k * (NT / TX);
     if (tx >= t.sx || ty >= t.sy) continue;
     const int gi = t.x0 + tx, gj = t.y0 + ty;
-    if (g.dissipation) add_dissipation<S>(g, t, sq, stx, sty, tx, ty, gi, gj, g.diss_amount, r[k]);
-    const int jk[3] = {gi, gj, 0};
+    if (g.dissipation)
+      add_dissipation<S>(g, t, cf, tg, sq, stx, sty, tx, ty, gi, gj, g.diss_amount, r[k]);
     const long p = gi + (long)g.n[0] * gj;
-    const double jac = jacobian<2>(g, jk);
+    const double jac = 1.0 / (tg.h0(tx) * tg.h1(ty));   // jacobian<2>
     double qp[NV2];
 #pragma unroll
     for (int l = 0; l < NV2; ++l) {
@@ -436,8 +541,8 @@ __global__ void __launch_bounds__(S::NT, 4) k_fwd_stage2d(Geo g, const Patches* 
       }
     }
     for (int s = 0; s < a.nsrc; ++s) {
-      const double dx0 = g.x[0][gi] - a.src[s].loc[0];
-      const double dy0 = g.x[1][gj] - a.src[s].loc[1];
+      const double dx0 = tg.x0(tx) - a.src[s].loc[0];
+      const double dy0 = tg.x1(ty) - a.src[s].loc[1];
       double rr2 = 0.0;
       rr2 += dx0 * dx0;
       rr2 += dy0 * dy0;
@@ -472,8 +577,8 @@ __global__ void __launch_bounds__(S::NT, 4) k_fwd_stage2d(Geo g, const Patches* 
         const int kp = (gi - pt.lo[0]) + pt.ls[0] * (gj - pt.lo[1]);
         const Prim<2> s = prim<2>(qn, g.gamma);
         const double F = s.p - pt.aplus[kp];
-        a.jpartial[(inst * a.nslots + a.slot) * a.npts + kp] =
-            1.0 * gnorm<2>(g, jk) * (F * F * pt.strength[kp]);
+        const double gn = (tg.hw1(ty) * (tg.hw0(tx) * 1.0)) * (tg.h0(tx) * tg.h1(ty));   // gnorm<2>
+        a.jpartial[(inst * a.nslots + a.slot) * a.npts + kp] = 1.0 * gn * (F * F * pt.strength[kp]);
       }
     }
   }
@@ -481,19 +586,24 @@ __global__ void __launch_bounds__(S::NT, 4) k_fwd_stage2d(Geo g, const Patches* 
 
 // -------------------------------------------------------------------------- adjoint
 template <class S>
-__global__ void __launch_bounds__(S::NT, 4) k_adj_stage2d(Geo g, const Patches* __restrict__ P,
+__global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_adj_stage2d(Geo g, const Patches* __restrict__ P,
                                                        Tiling T, AdjStage a) {
   constexpr int TX = S::TX, EX = S::EX, EY = S::EY, NT = S::NT;
   constexpr int STX = S::STX;
   __shared__ double sq[S::SQ];
   __shared__ double sc[S::SSCR];
-  const int tile = T.tile_of_block[blockIdx.x];
+  __shared__ double sgrid[TileGrid<S>::kSize];
+  const int tile = block_tile(T);
   if (tile < 0) return;   // block beyond the tiles of its XCD (uniform per block)
   const TileGeo t = tile_geo(T, tile);
   const long inst = blockIdx.y;
   const long NVN = (long)NV2 * g.N;
   const double* li = a.lam + inst * NVN;
   const double* qi = a.q + inst * NVN;
+  const Coefs cf = load_coefs(g, SLOT_ADJ);
+  const double gv = load_tile_grid<S>(g, t);
+  const TileGrid<S> tg{sgrid};
+  const unsigned pmask = T.pmask[tile];
   // forward stage state at this thread's point, fetched before the stencil work
   double pq[NV2];
   {
@@ -504,16 +614,20 @@ __global__ void __launch_bounds__(S::NT, 4) k_adj_stage2d(Geo g, const Patches* 
       for (int l = 0; l < NV2; ++l) pq[l] = qi[l * g.N + o];
     }
   }
-  stage_operand<S, false>(g, t, li, sq, nullptr, nullptr);
+  {
+    StageRegs<S> sr;
+    stage_load<S>(g, t, li, sr);
+    if (threadIdx.x < TileGrid<S>::kSize) sgrid[threadIdx.x] = gv;
+    stage_store<S, false>(g, t, sr, tg, sq, nullptr, nullptr);
+  }
   double* stx = sc;
   double* sty = sc + STX;
   if (g.dissipation && !g.composite) {
     __syncthreads();
-    diss_intermediates<S>(g, t, sq, stx, sty);
+    diss_intermediates<S>(g, t, cf, tg, sq, stx, sty);
   }
   __syncthreads();
   const Op& oa = g.ops[SLOT_ADJ];
-  const unsigned pmask = T.pmask[tile];
   const int tx = threadIdx.x % TX;
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
@@ -521,19 +635,18 @@ __global__ void __launch_bounds__(S::NT, 4) k_adj_stage2d(Geo g, const Patches* 
     if (tx >= t.sx || ty >= t.sy) continue;
     const int gi = t.x0 + tx, gj = t.y0 + ty;
     double T1[2][NV2];
-    deriv_lds<16, 12>(oa, g.dir[0], gi, t.ex0,
+    deriv_lds<16, 12>(oa, cf.d1, g.dir[0], gi, t.ex0,
               [&](int l, int e) { return sq[(l * EY + ty + HY) * EX + e]; }, T1[0]);
-    deriv_lds<16, 12>(oa, g.dir[1], gj, t.ey0,
+    deriv_lds<16, 12>(oa, cf.d1, g.dir[1], gj, t.ey0,
               [&](int l, int e) { return sq[(l * EY + e) * EX + tx + HX]; }, T1[1]);
     const long p = gi + (long)g.n[0] * gj;
     const Prim<2> s = prim<2>(pq, g.gamma);
-    const int jk[3] = {gi, gj, 0};
     double r[NV2] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
       double m[2], A[16];
-      m[0] = d == 0 ? metric<2>(g, 0, jk) : 0.0;
-      m[1] = d == 1 ? metric<2>(g, 1, jk) : 0.0;
+      m[0] = d == 0 ? tg.h1(ty) : 0.0;   // metric<2>(g, 0, .)
+      m[1] = d == 1 ? tg.h0(tx) : 0.0;   // metric<2>(g, 1, .)
       inviscid_jacobian<2>(m, s, g.gamma, A);
 #pragma unroll
       for (int i = 0; i < NV2; ++i) {
@@ -550,8 +663,9 @@ __global__ void __launch_bounds__(S::NT, 4) k_adj_stage2d(Geo g, const Patches* 
       pl0[l] = (a.stage == 3 || a.stage == 2) ? a.l0[inst * NVN + l * g.N + p] : 0.0;
       pb2[l] = a.stage <= 3 ? a.b2[inst * NVN + l * g.N + p] : 0.0;
     }
-    if (g.dissipation) add_dissipation<S>(g, t, sq, stx, sty, tx, ty, gi, gj, -g.diss_amount, r);
-    const double jac = jacobian<2>(g, jk);
+    if (g.dissipation)
+      add_dissipation<S>(g, t, cf, tg, sq, stx, sty, tx, ty, gi, gj, -g.diss_amount, r);
+    const double jac = 1.0 / (tg.h0(tx) * tg.h1(ty));   // jacobian<2>
     double lp[NV2];
 #pragma unroll
     for (int l = 0; l < NV2; ++l) {
@@ -607,7 +721,8 @@ __global__ void __launch_bounds__(S::NT, 4) k_adj_stage2d(Geo g, const Patches* 
           if (a.grad) a.grad[inst * a.grad_stride + kp] = lp[3] * (pt.strength[kp] * 1.0);
           if (a.spartial) {
             const double F = lp[3] * pt.strength[kp] * 1.0;
-            a.spartial[(inst * a.nslots + a.slot) * a.npts + kp] = 1.0 * gnorm<2>(g, jk) * (F * F);
+            const double gn = (tg.hw1(ty) * (tg.hw0(tx) * 1.0)) * (tg.h0(tx) * tg.h1(ty));   // gnorm<2>
+            a.spartial[(inst * a.nslots + a.slot) * a.npts + kp] = 1.0 * gn * (F * F);
           }
           break;
         }
@@ -701,15 +816,16 @@ int fused2d_shape_dims(int shape, int* tx, int* ty) {
   return MGD_OK;
 }
 
-static void balanced(int len, int T, std::vector<int>& org, std::vector<int>& size) {
+// Balanced partition of len points into ceil(len / T) tiles: tile b starts at
+// b q + min(b, r) and holds q + (b < r) points.
+static void balanced(int len, int T, std::vector<int>& org, std::vector<int>& size, int* q,
+                     int* r) {
   const int n = (len + T - 1) / T;
-  const int base = len / n, extra = len % n;
-  int o = 0;
+  *q = len / n;
+  *r = len % n;
   for (int b = 0; b < n; ++b) {
-    const int s = base + (b < extra ? 1 : 0);
-    org.push_back(o);
-    size.push_back(s);
-    o += s;
+    org.push_back(b * *q + std::min(b, *r));
+    size.push_back(*q + (b < *r ? 1 : 0));
   }
 }
 
@@ -719,9 +835,9 @@ int fused2d_make_tiling(const Geo& g, const Patches& hp, int shape, HostTiling* 
   out->shape = shape;
   out->org.clear();
   out->size.clear();
-  balanced(g.n[0], tx, out->org, out->size);
+  balanced(g.n[0], tx, out->org, out->size, &out->qx, &out->rx);
   out->nbx = (int)out->org.size();
-  balanced(g.n[1], ty, out->org, out->size);
+  balanced(g.n[1], ty, out->org, out->size, &out->qy, &out->ry);
   out->nby = (int)out->org.size() - out->nbx;
   for (int d = 0; d < 2; ++d) {
     if (g.dir[d].periodic) continue;
@@ -749,6 +865,7 @@ int fused2d_make_tiling(const Geo& g, const Patches& hp, int shape, HostTiling* 
   // blocks of the last band map to -1 and exit at once.
   const int ntiles = out->nbx * out->nby;
   const int band = (ntiles + kNumXcd - 1) / kNumXcd;
+  out->band = band;
   out->nblk = band * kNumXcd;
   out->tile_of_block.assign((size_t)out->nblk, -1);
   for (int b = 0; b < out->nblk; ++b) {

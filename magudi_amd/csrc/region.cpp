@@ -90,7 +90,7 @@ struct mgd_region {
   int chunk = 0;                         // stages per chunk of quadrature terms
   mgd::HostTiling htile;
   mgd::Tiling tiling{};
-  mgd::DevBuf d_tile_org, d_tile_size, d_tile_mask, d_tile_blk;
+  mgd::DevBuf d_tile_mask;
   bool tiling_ok = false;
   int tile_shape = -1;                   // MGD_OPT_TILE_SHAPE: -1 autotune, else fixed
   bool tuned = false;
@@ -481,20 +481,19 @@ int upload_ints(DevBuf& b, const void* data, size_t bytes) {
 int install_tiling(mgd_region* r, int shape) {
   HostTiling H;
   if (fused2d_make_tiling(r->geo, r->hpatches, shape, &H) != MGD_OK) return MGD_ERR_UNSUPPORTED;
-  if (upload_ints(r->d_tile_org, H.org.data(), H.org.size() * sizeof(int)) ||
-      upload_ints(r->d_tile_size, H.size.data(), H.size.size() * sizeof(int)) ||
-      upload_ints(r->d_tile_mask, H.pmask.data(), H.pmask.size() * sizeof(unsigned)) ||
-      upload_ints(r->d_tile_blk, H.tile_of_block.data(), H.tile_of_block.size() * sizeof(int)))
+  if (upload_ints(r->d_tile_mask, H.pmask.data(), H.pmask.size() * sizeof(unsigned)))
     return MGD_ERR_HIP;
   r->htile = H;
   r->tiling.shape = H.shape;
   r->tiling.nbx = H.nbx;
   r->tiling.nby = H.nby;
   r->tiling.nblk = H.nblk;
-  r->tiling.org = reinterpret_cast<const int*>(r->d_tile_org.p);
-  r->tiling.size = reinterpret_cast<const int*>(r->d_tile_size.p);
+  r->tiling.band = H.band;
+  r->tiling.qx = H.qx;
+  r->tiling.rx = H.rx;
+  r->tiling.qy = H.qy;
+  r->tiling.ry = H.ry;
   r->tiling.pmask = reinterpret_cast<const unsigned*>(r->d_tile_mask.p);
-  r->tiling.tile_of_block = reinterpret_cast<const int*>(r->d_tile_blk.p);
   return MGD_OK;
 }
 
@@ -928,8 +927,7 @@ std::vector<uintptr_t> graph_key(mgd_region* r, int kind, std::initializer_list<
                           &r->G, &r->T1, &r->S, &r->qs, &r->aforce, &r->control, &r->grad,
                           &r->acc_J, &r->acc_S, &r->partial, &r->src_amp, &r->snaps, &r->window,
                           &r->qy, &r->ly, &r->l0, &r->jpart, &r->spart, &r->ssum, &r->wts,
-                          &r->window2, &r->b2r, &r->qsr, &r->qT, &r->d_tile_org, &r->d_tile_size,
-                          &r->d_tile_mask, &r->d_tile_blk};
+                          &r->window2, &r->b2r, &r->qsr, &r->qT, &r->d_tile_mask};
   for (const DevBuf* b : bufs) k.push_back((uintptr_t)b->p);
   for (uintptr_t e : extra) k.push_back(e);
   return k;
