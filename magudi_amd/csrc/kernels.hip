@@ -1023,6 +1023,124 @@ __global__ void __launch_bounds__(BX * BY) k_fluxdiv3d_xy(Geo g, const double* _
   }
 }
 
+// First-derivative-type operator (SBP 4-8 interior: skew, half width 4) along a
+// periodic direction d >= 1 by marching: each thread owns one grid line along d (lanes
+// along x, so every load is coalesced) and walks a chunk of kLineChunk points of it,
+// keeping the 9-point window of one component in registers -- each operand value is
+// read once per chunk instead of once per stencil tap.  Component c of instance i reads
+// src[i * src_inst + c * N] and writes dst[i * dst_inst + c * N]; mode 0 stores the
+// result, 1 adds it to dst, 2 stores 0 - (dst + result).  The sums are those of
+// sbp_apply's interior rows (same order), so the values are the same.
+constexpr int kLineChunk = 32;
+__global__ void __launch_bounds__(kBlock) k_deriv_line(Geo g, int d, int slot,
+                                                      const double* __restrict__ src, long src_inst,
+                                                      double* __restrict__ dst, long dst_inst,
+                                                      int ncomp, int mode) {
+  const Dir dd = g.dir[d];
+  const long nlines = g.N / dd.len;
+  const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= nlines) return;
+  const int inst = blockIdx.z / ncomp, c = blockIdx.z % ncomp;
+  // line -> first point: x index fastest, then the remaining direction
+  const long n0 = g.n[0];
+  const long i = line % n0, o = line / n0;
+  const long base = i + o * (d == 1 ? n0 * g.n[1] : n0);
+  const double* x = src + inst * src_inst + (long)c * g.N + base;
+  double* y = dst + inst * dst_inst + (long)c * g.N + base;
+  const long sd = dd.stride;
+  const int z0 = blockIdx.y * kLineChunk;
+  const int z1 = min(z0 + kLineChunk, dd.len);
+  const Op& op = g.ops[slot];
+  const double c1 = op.c[kMaxHW + 1], c2 = op.c[kMaxHW + 2], c3 = op.c[kMaxHW + 3],
+               c4 = op.c[kMaxHW + 4];
+  double Q[9];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) Q[k] = x[wrap(z0 - 4 + k, dd) * sd];
+  for (int z = z0; z < z1; ++z) {
+    Q[8] = x[wrap(z + 4, dd) * sd];
+    double out = 0.0;
+    out += c1 * (Q[5] - Q[3]);
+    out += c2 * (Q[6] - Q[2]);
+    out += c3 * (Q[7] - Q[1]);
+    out += c4 * (Q[8] - Q[0]);
+    double* yz = y + z * sd;
+    if (mode == 0) {
+      *yz = out;
+    } else {
+      double v = *yz + out;
+      if (mode == 2) v = 0.0 - v;
+      *yz = v;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) Q[k] = Q[k + 1];
+  }
+}
+
+// The same operator along a periodic x direction: a block stages 256 consecutive points
+// of one component -- a segment of one row, or whole rows when rows are shorter -- with
+// the 4-point wrap-around halo of each row segment in LDS, and applies the interior
+// stencil from there.  Requires rows of a length that divides 256 or is a multiple of
+// it, and 256 | N.  Modes as for k_deriv_line, plus 3: store 0 - result.
+__global__ void __launch_bounds__(kBlock) k_deriv_row(Geo g, int slot, const double* __restrict__ src,
+                                                     long src_inst, double* __restrict__ dst,
+                                                     long dst_inst, int ncomp, int mode) {
+  static_assert(kBlock == 256, "one staged point per thread");
+  const Dir dx = g.dir[0];
+  const int L = dx.len < kBlock ? dx.len : kBlock;   // segment length
+  __shared__ double s[kBlock + 8 * kBlock / 16];    // (L + 8) per segment, L >= 16
+  const int inst = blockIdx.z / ncomp, c = blockIdx.z % ncomp;
+  const double* x = src + inst * src_inst + (long)c * g.N;
+  double* y = dst + inst * dst_inst + (long)c * g.N;
+  const int t = threadIdx.x;
+  const long p = (long)blockIdx.x * kBlock + t;
+  const int seg = t / L, xi = t % L;
+  const int xr = (int)(p % dx.len);                 // index along the row
+  const long row0 = p - xr;
+  double* sg = s + seg * (L + 8);
+  sg[4 + xi] = x[p];
+  if (xi < 4) sg[xi] = x[row0 + wrap(xr - 4, dx)];
+  if (xi >= L - 4) sg[xi + 8] = x[row0 + wrap(xr + 4, dx)];
+  __syncthreads();
+  const Op& op = g.ops[slot];
+  const double c1 = op.c[kMaxHW + 1], c2 = op.c[kMaxHW + 2], c3 = op.c[kMaxHW + 3],
+               c4 = op.c[kMaxHW + 4];
+  const double* r = sg + 4 + xi;
+  double out = 0.0;
+  out += c1 * (r[1] - r[-1]);
+  out += c2 * (r[2] - r[-2]);
+  out += c3 * (r[3] - r[-3]);
+  out += c4 * (r[4] - r[-4]);
+  if (mode == 0) {
+    y[p] = out;
+  } else if (mode == 3) {
+    y[p] = 0.0 - out;
+  } else {
+    double v = y[p] + out;
+    if (mode == 2) v = 0.0 - v;
+    y[p] = v;
+  }
+}
+
+// Launches k_deriv_row (d = 0) or k_deriv_line (d >= 1) when direction d is periodic and
+// the grid fits the kernel; false otherwise (the caller then uses the generic kernel).
+static bool launch_deriv_line(const Geo& g, int d, int slot, const double* src, long src_inst,
+                              double* dst, long dst_inst, int ncomp, int mode, hipStream_t st) {
+  if (!g.dir[d].periodic || g.dir[d].len < 16) return false;
+  if (d == 0) {
+    const int n0 = g.dir[0].len;
+    if (!((n0 % kBlock == 0) || (kBlock % n0 == 0)) || g.N % kBlock != 0) return false;
+    const dim3 grid((unsigned)(g.N / kBlock), 1, (unsigned)(g.ninst * ncomp));
+    k_deriv_row<<<grid, kBlock, 0, st>>>(g, slot, src, src_inst, dst, dst_inst, ncomp, mode);
+    return true;
+  }
+  const long nlines = g.N / g.dir[d].len;
+  const dim3 grid((unsigned)((nlines + kBlock - 1) / kBlock),
+                  (unsigned)((g.dir[d].len + kLineChunk - 1) / kLineChunk),
+                  (unsigned)(g.ninst * ncomp));
+  k_deriv_line<<<grid, kBlock, 0, st>>>(g, d, slot, src, src_inst, dst, dst_inst, ncomp, mode);
+  return true;
+}
+
 // ------------------------------------------------------------------ launchers
 #define MGD_DISPATCH(ND_, ...)                                  \
   switch (ND_) {                                                \
@@ -1073,10 +1191,18 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
     if (F && ND == 3 && fluxdiv3d_applies(g)) {
       const dim3 gg((g.n[0] / kFD3) * (g.n[1] / kFD3), g.n[2], g.ninst);
       k_fluxdiv3d_xy<kFD3, kFD3><<<gg, kFD3 * kFD3, 0, st>>>(g, q, g.viscous ? G : nullptr, F, rhs);
-      k_div<ND><<<grid, kBlock, 0, st>>>(g, ND - 1, F, rhs);
+      if (!launch_deriv_line(g, ND - 1, SLOT_FIRST, F + (long)(ND - 1) * g.nv * g.N,
+                             (long)ND * g.nv * g.N, rhs, (long)g.nv * g.N, g.nv, 2, st))
+        k_div<ND><<<grid, kBlock, 0, st>>>(g, ND - 1, F, rhs);
     } else if (F) {
       k_flux_all<ND><<<grid, kBlock, 0, st>>>(g, q, g.viscous ? G : nullptr, F);
-      for (int d = 0; d < ND; ++d) k_div<ND><<<grid, kBlock, 0, st>>>(g, d, F, rhs);
+      for (int d = 0; d < ND; ++d) {
+        const int mode = d == 0 ? (ND == 1 ? 3 : 0) : d == ND - 1 ? 2 : 1;
+        if (!launch_deriv_line(g, d, SLOT_FIRST, F + (long)d * g.nv * g.N,
+                                         (long)ND * g.nv * g.N, rhs, (long)g.nv * g.N, g.nv,
+                                         mode, st))
+          k_div<ND><<<grid, kBlock, 0, st>>>(g, d, F, rhs);
+      }
     } else {
       for (int d = 0; d < ND; ++d)
         k_flux_div<ND><<<grid, kBlock, 0, st>>>(g, d, q, g.viscous ? G : nullptr, rhs);
@@ -1104,12 +1230,16 @@ void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
     constexpr int NV = ND + 2;
     if (g.viscous) launch_gradients<ND>(g, q, w, G, grid, st);
     for (int d = 0; d < ND; ++d)
-      k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, lam, (long)NV * g.N, T1, 0);
+      if (!launch_deriv_line(g, d, SLOT_ADJ, lam, (long)NV * g.N, T1 + (long)d * NV * g.N,
+                             (long)ND * NV * g.N, NV, 0, st))
+        k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, lam, (long)NV * g.N, T1, 0);
     k_adj_jac<ND><<<grid, kBlock, 0, st>>>(g, q, G, T1, rhs);
     if (g.viscous) {
       k_adj_visc2<ND><<<grid, kBlock, 0, st>>>(g, q, T1);
       for (int d = 0; d < ND; ++d)
-        k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, T1, (long)ND * NV * g.N, S, 1);
+        if (!launch_deriv_line(g, d, SLOT_ADJ, T1 + (long)(d * NV + 1) * g.N, (long)ND * NV * g.N,
+                               S + (long)(d * NV + 1) * g.N, (long)ND * NV * g.N, NV - 1, 0, st))
+          k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, T1, (long)ND * NV * g.N, S, 1);
       k_adj_chain<ND><<<grid, kBlock, 0, st>>>(g, q, S, rhs, -1.0);
     }
     if (g.dissipation) {
@@ -1123,7 +1253,9 @@ void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
       k_fill<<<(unsigned)((M + kBlock - 1) / kBlock), kBlock, 0, st>>>(M, 0.0, T1);
       k_ff_visc_adj<ND><<<grid, kBlock, 0, st>>>(g, P, q, lam, T1);
       for (int d = 0; d < ND; ++d)
-        k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, T1, (long)ND * NV * g.N, S, 1);
+        if (!launch_deriv_line(g, d, SLOT_ADJ, T1 + (long)(d * NV + 1) * g.N, (long)ND * NV * g.N,
+                               S + (long)(d * NV + 1) * g.N, (long)ND * NV * g.N, NV - 1, 0, st))
+          k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, T1, (long)ND * NV * g.N, S, 1);
       k_adj_chain<ND><<<grid, kBlock, 0, st>>>(g, q, S, rhs, +1.0);
     }
     if (g.viscous && has_farfield)
