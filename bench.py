@@ -249,7 +249,7 @@ def main():
                                f"HBM snapshots every {a.save_interval} steps with recompute",
                    "grid": [a.n, a.n], "nsteps": a.nsteps, "save_interval": a.save_interval,
                    "instances": 1, "path": "fused 2-D stage kernels",
-                   "tile_shape": ["32x16", "16x32", "16x16", "24x16"][R.tile_shape]
+                   "tile_shape": ["32x16", "16x32", "16x16", "24x16", "16x16/2", "16x12/2"][R.tile_shape]
                    if R.tile_shape >= 0 else "generic"},
         "roofline": roofline,
         # whole job against the compulsory-traffic model of one fwd+adjoint pt-step with

@@ -217,7 +217,8 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches);
 /* Region options.  MGD_OPT_FORCE_PLAIN = 1 runs the generic per-operator kernels even
  * where the fused 2-D stage kernels apply (comparison and testing).
  * MGD_OPT_TILE_SHAPE selects the tile of the fused 2-D kernels: -1 (default) autotune at
- * the first run, 0: 32 x 16, 1: 16 x 32, 2: 16 x 16, 3: 24 x 16 points.  No result
+ * the first run, 0: 32 x 16, 1: 16 x 32, 2: 16 x 16, 3: 24 x 16 points with one thread
+ * per point, 4: 16 x 16 and 5: 16 x 12 points with two threads per point.  No result
  * depends on the tile shape. */
 #define MGD_OPT_FORCE_PLAIN 1
 #define MGD_OPT_TILE_SHAPE 2

@@ -7,26 +7,28 @@
 namespace mgd {
 
 // Tile shapes of the fused kernels (points along x, y): 0: 32 x 16, 1: 16 x 32,
-// 2: 16 x 16, 3: 24 x 16.
-constexpr int kNumTileShapes = 4;
+// 2: 16 x 16, 3: 24 x 16, one thread per point; 4: 16 x 16 and 5: 16 x 12 with two
+// threads per point (each forming two of the four components).
+constexpr int kNumTileShapes = 6;
 constexpr int kNumXcd = 8;   // XCDs of an MI355X; work-groups are dealt to them round-robin
 
 // Tile partition of a 2-D grid (device view): nbx x nby tiles of at most TX x TY points
-// from a balanced partition of each direction -- tile b along x starts at
-// b qx + min(b, rx) and holds qx + (b < rx) points (qy, ry along y) -- and
-// pmask: patches per tile (tile index bx + nbx * by).  Launch block k works on tile
+// from a balanced partition of each direction whose larger tiles sit at both ends --
+// along x, tile b holds qx points, plus one when b < lx or b >= nbx - rx, and starts at
+// b qx + min(b, lx) + max(0, b - (nbx - rx)) (likewise along y) -- and pmask: patches
+// per tile (tile index bx + nbx * by).  Launch block k works on tile
 // (k % kNumXcd) band + k / kNumXcd, or on none when that is beyond the last tile.
 struct Tiling {
   int shape;
   int nbx, nby, nblk, band;
-  int qx, rx, qy, ry;
+  int qx, lx, rx, qy, ly, ry;
   const unsigned* pmask;
 };
 
 struct HostTiling {
   int shape = 0;
   int nbx = 0, nby = 0, nblk = 0, band = 0;
-  int qx = 0, rx = 0, qy = 0, ry = 0;
+  int qx = 0, lx = 0, rx = 0, qy = 0, ly = 0, ry = 0;
   std::vector<int> org, size;          // tile origins / sizes: x tiles, then y tiles
   std::vector<unsigned> pmask;
   std::vector<int> tile_of_block;      // launch block -> tile, or -1

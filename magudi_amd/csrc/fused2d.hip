@@ -27,24 +27,31 @@ constexpr int HX = 4, HY = 4;     // halo of the staged operand
 constexpr int NV2 = 4;
 constexpr int kMinEdgeTile = 12;    // closure depth of the adjoint 4-8 operator
 constexpr int kRedThreads = 256;    // block size of the reduction kernels
-constexpr int PPT = 1;              // points per thread
 
-// Tile shape: TX x TY points, one point per thread.  LDS (doubles): operand
-// [4][EY][EX]; scratch for the fluxes, reused for the dissipation intermediates and the
-// final reduction.
-template <int TX_, int TY_>
+// Tile shape: TX x TY points and CS threads per point.  Every thread stages operand
+// points; in the stencil and point-wise phases each of the CS threads of a point forms
+// NC = 4 / CS of its components (thread group h = threadIdx / NP owns components
+// h NC .. h NC + NC - 1; a group is a whole number of waves), so CS = 2 gives a tile
+// twice the waves, each with half the stencil work.  LDS (doubles): operand [4][EY][EX];
+// scratch for the fluxes, reused for the dissipation intermediates.
+template <int TX_, int TY_, int CS_ = 1>
 struct Shape {
-  static constexpr int TX = TX_, TY = TY_;
+  static constexpr int TX = TX_, TY = TY_, CS = CS_;
+  static constexpr int NP = TX * TY, NT = NP * CS, NC = NV2 / CS;
+  static_assert(NP % 64 == 0 && NV2 % CS == 0, "component groups are whole waves");
   static constexpr int EX = TX + 2 * HX, EY = TY + 2 * HY;
-  static constexpr int NT = TX * TY;
   static constexpr int kStageIters = (EX * EY + NT - 1) / NT;   // staged points per thread
   static constexpr int SQ = NV2 * EY * EX;
   static constexpr int SFX = NV2 * TY * EX, SFY = NV2 * EY * TX;
   static constexpr int STX = NV2 * TY * (TX + 4), STY = NV2 * (TY + 4) * TX;
   static constexpr int SSCR = (SFX + SFY) > (STX + STY) ? (SFX + SFY) : (STX + STY);
-  // waves per SIMD that the LDS use allows (2 blocks of 512 threads or 3 of 256 per CU):
-  // the register budget is sized to it
-  static constexpr int kWavesPerEU = NT >= 512 ? 4 : 3;
+  static_assert(CS == 1 || SSCR >= 2 * NV2 * NP, "the scratch space holds the adjoint exchange");
+  // waves per SIMD that the LDS use allows (160 KiB per CU); the register budget is
+  // sized to it
+  static constexpr int kLdsBytes = 8 * (SQ + SSCR + 3 * (TX + TY));
+  static constexpr int kBlocksPerCu = (160 * 1024) / kLdsBytes;
+  static constexpr int kWaves = kBlocksPerCu * NT / 64 / 4;
+  static constexpr int kWavesPerEU = kWaves < 1 ? 1 : kWaves > 8 ? 8 : kWaves;
 };
 
 struct TileGeo {
@@ -63,10 +70,10 @@ __device__ __forceinline__ int block_tile(const Tiling& T) {
 __device__ __forceinline__ TileGeo tile_geo(const Tiling& T, int tile) {
   const int bx = tile % T.nbx, by = tile / T.nbx;
   TileGeo t;
-  t.x0 = bx * T.qx + min(bx, T.rx);
-  t.sx = T.qx + (bx < T.rx ? 1 : 0);
-  t.y0 = by * T.qy + min(by, T.ry);
-  t.sy = T.qy + (by < T.ry ? 1 : 0);
+  t.x0 = bx * T.qx + min(bx, T.lx) + max(0, bx - (T.nbx - T.rx));
+  t.sx = T.qx + (bx < T.lx || bx >= T.nbx - T.rx ? 1 : 0);
+  t.y0 = by * T.qy + min(by, T.ly) + max(0, by - (T.nby - T.ry));
+  t.sy = T.qy + (by < T.ly || by >= T.nby - T.ry ? 1 : 0);
   t.ex0 = t.x0 - HX;
   t.ey0 = t.y0 - HY;
   return t;
@@ -218,47 +225,49 @@ __device__ __forceinline__ void stage_store(const Geo& g, const TileGeo& t, cons
 
 // Boundary-closure row of an operator with compile-time block size BW x BD (SBP 4-8:
 // first derivative 12 x 8, adjoint 16 x 12, dissipation transpose 7 x 5, composite
-// 12 x 8).  Unrolled so the coefficient loads issue together.  f(c) returns the NV2
-// operand values at closure column c (0-based from the boundary, mirrored on the right).
-template <int BW, int BD, class F>
+// 12 x 8) for NC components.  Unrolled so the coefficient loads issue together.  f(c, v)
+// writes the NC operand values at closure column c (0-based from the boundary,
+// mirrored on the right).
+template <int BW, int BD, int NC, class F>
 __device__ __forceinline__ void closure_row(const Op& op, int i, int len, F&& f, double* out) {
   const bool left = i < BD;
   const int m = left ? i : len - 1 - i;
   const double* b = left ? op.b1[m] : op.b2[m];
   const int base = left ? 0 : len - BW;
 #pragma unroll
-  for (int l = 0; l < NV2; ++l) out[l] = 0.0;
+  for (int j = 0; j < NC; ++j) out[j] = 0.0;
 #pragma unroll
   for (int c = 0; c < BW; ++c) {
-    double v[NV2];
+    double v[NC];
     f(base + c, v);
     const double bc = b[c];
 #pragma unroll
-    for (int l = 0; l < NV2; ++l) out[l] += bc * v[l];
+    for (int j = 0; j < NC; ++j) out[j] += bc * v[j];
   }
 }
 
-// First-derivative-type operator (BW x BD closure) along x or y at a point of the tile,
-// operand in LDS; row(l, e) returns component l at extended index e.
-template <int BW, int BD, class Row>
+// First-derivative-type operator (BW x BD closure) along x or y at a point of the tile
+// for components c0 .. c0 + NC - 1, operand in LDS; row(l, e) returns component l at
+// extended index e.
+template <int BW, int BD, int NC, class Row>
 __device__ __forceinline__ void deriv_lds(const Op& op, const double* ck, const Dir& dd, int i,
-                                          int e0, Row&& row, double* out) {
+                                          int e0, int c0, Row&& row, double* out) {
   if (!dd.periodic && (i < BD || i >= dd.len - BD)) {
-    closure_row<BW, BD>(op, i, dd.len,
-                        [&](int idx, double* v) {
+    closure_row<BW, BD, NC>(op, i, dd.len,
+                            [&](int idx, double* v) {
 #pragma unroll
-                          for (int l = 0; l < NV2; ++l) v[l] = row(l, idx - e0);
-                        },
-                        out);
+                              for (int j = 0; j < NC; ++j) v[j] = row(c0 + j, idx - e0);
+                            },
+                            out);
     return;
   }
   const int e = i - e0;
 #pragma unroll
-  for (int l = 0; l < NV2; ++l) {
+  for (int j = 0; j < NC; ++j) {
     double s = 0.0;
 #pragma unroll
-    for (int k = 1; k <= 4; ++k) s += ck[k] * (row(l, e + k) - row(l, e - k));
-    out[l] = s;
+    for (int k = 1; k <= 4; ++k) s += ck[k] * (row(c0 + j, e + k) - row(c0 + j, e - k));
+    out[j] = s;
   }
 }
 
@@ -285,18 +294,20 @@ __device__ __forceinline__ double d4_at(const Dir& dd, int i, double c0, double 
 }
 
 // t = -arc D4 x on the tile rows (x direction, columns x0-2 .. x0+sx+1) and tile columns
-// (y direction, rows y0-2 .. y0+sy+1), for staged points inside the domain.
+// (y direction, rows y0-2 .. y0+sy+1), for staged points inside the domain, components
+// c0 .. c0 + NC - 1; the positions are dealt to the NP threads of the component group
+// (lane = index within the group).
 template <class S>
 __device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& t, const Coefs& cf,
                                                    const TileGrid<S>& tg, const double* sq,
-                                                   double* stx, double* sty) {
-  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NT = S::NT;
-  const double c0 = cf.dd[0], c1 = cf.dd[1], c2 = cf.dd[2];
+                                                   double* stx, double* sty, int lane, int c0) {
+  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NP = S::NP, NC = S::NC;
+  const double k0 = cf.dd[0], k1 = cf.dd[1], k2 = cf.dd[2];
   const Dir dx = g.dir[0], dy = g.dir[1];
   const int wx = t.sx + 4;
 #pragma unroll
-  for (int it = 0; it < (TY * (TX + 4) + NT - 1) / NT; ++it) {
-    const int e = threadIdx.x + it * NT;
+  for (int it = 0; it < (TY * (TX + 4) + NP - 1) / NP; ++it) {
+    const int e = lane + it * NP;
     if (e >= TY * (TX + 4)) continue;
     const int ex = e % (TX + 4) + 2, ey = e / (TX + 4) + HY;
     if (ex - 2 >= wx || ey - HY >= t.sy) continue;
@@ -304,16 +315,17 @@ __device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& 
     if (!dx.periodic && (gi < 0 || gi >= dx.len)) continue;
     const double arc = fabs(tg.h1(ey - HY));
 #pragma unroll
-    for (int l = 0; l < NV2; ++l) {
+    for (int j = 0; j < NC; ++j) {
+      const int l = c0 + j;
       const double* r = sq + (l * EY + ey) * EX;
-      const double s = d4_at(dx, gi, c0, c1, c2, [&](int k) { return r[k - t.ex0]; });
-      stx[(l * TY + (ey - HY)) * (TX + 4) + (ex - 2)] = -arc * s;
+      const double v = d4_at(dx, gi, k0, k1, k2, [&](int k) { return r[k - t.ex0]; });
+      stx[(l * TY + (ey - HY)) * (TX + 4) + (ex - 2)] = -arc * v;
     }
   }
   const int hy = t.sy + 4;
 #pragma unroll
-  for (int it = 0; it < ((TY + 4) * TX + NT - 1) / NT; ++it) {
-    const int e = threadIdx.x + it * NT;
+  for (int it = 0; it < ((TY + 4) * TX + NP - 1) / NP; ++it) {
+    const int e = lane + it * NP;
     if (e >= (TY + 4) * TX) continue;
     const int ex = e % TX + HX, ey = e / TX + 2;
     if (ex - HX >= t.sx || ey - 2 >= hy) continue;
@@ -321,27 +333,31 @@ __device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& 
     if (!dy.periodic && (gj < 0 || gj >= dy.len)) continue;
     const double arc = fabs(tg.h0(ex - HX));
 #pragma unroll
-    for (int l = 0; l < NV2; ++l) {
+    for (int j = 0; j < NC; ++j) {
+      const int l = c0 + j;
       const double* c = sq + l * EY * EX + ex;
-      const double s = d4_at(dy, gj, c0, c1, c2, [&](int k) { return c[(k - t.ey0) * EX]; });
-      sty[(l * (TY + 4) + (ey - 2)) * TX + (ex - HX)] = -arc * s;
+      const double v = d4_at(dy, gj, k0, k1, k2, [&](int k) { return c[(k - t.ey0) * EX]; });
+      sty[(l * (TY + 4) + (ey - 2)) * TX + (ex - HX)] = -arc * v;
     }
   }
 }
 
-// Adds amount * (dissipation in x then in y) at one point: H^-1 D4^T t (closure rows of
-// D4^T near non-periodic boundaries) or the composite operator on the staged operand.
+// Dissipation terms at one point for components c0 .. c0 + NC - 1: w[d][j] = amount *
+// (H^-1 D4^T t) along direction d (closure rows of D4^T near non-periodic boundaries),
+// or amount * (composite operator on the staged operand).  The caller adds w[0] then
+// w[1].
 template <class S>
-__device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, const Coefs& cf,
-                                                const TileGrid<S>& tg, const double* sq,
-                                                const double* stx, const double* sty, int tx,
-                                                int ty, int gi, int gj, double amount, double* r) {
-  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY;
+__device__ __forceinline__ void diss_terms(const Geo& g, const TileGeo& t, const Coefs& cf,
+                                           const TileGrid<S>& tg, const double* sq,
+                                           const double* stx, const double* sty, int tx, int ty,
+                                           int gi, int gj, int c0, double amount,
+                                           double (*w)[S::NC]) {
+  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NC = S::NC;
   const Op& od = g.ops[SLOT_DISS];
   const Op& ot = g.ops[SLOT_DISS_T];
 #pragma unroll
   for (int d = 0; d < 2; ++d) {
-    double dv[NV2];
+    double dv[NC];
     const Dir& dr = g.dir[d];
     const int i = d == 0 ? gi : gj;
     if (g.composite) {
@@ -350,20 +366,21 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
       };
       const int e0 = d == 0 ? t.ex0 : t.ey0;
       if (!dr.periodic && (i < 8 || i >= dr.len - 8)) {
-        closure_row<12, 8>(od, i, dr.len,
-                           [&](int idx, double* v) {
+        closure_row<12, 8, NC>(od, i, dr.len,
+                               [&](int idx, double* v) {
 #pragma unroll
-                             for (int l = 0; l < NV2; ++l) v[l] = row(l, idx - e0);
-                           },
-                           dv);
+                                 for (int j = 0; j < NC; ++j) v[j] = row(c0 + j, idx - e0);
+                               },
+                               dv);
       } else {
         const int e = i - e0;
 #pragma unroll
-        for (int l = 0; l < NV2; ++l) {
-          double s = 0.0;
+        for (int j = 0; j < NC; ++j) {
+          double v = 0.0;
 #pragma unroll
-          for (int k = 1; k <= 4; ++k) s += od.c[kMaxHW + k] * (row(l, e + k) + row(l, e - k));
-          dv[l] = s + od.c[kMaxHW] * row(l, e);
+          for (int k = 1; k <= 4; ++k)
+            v += od.c[kMaxHW + k] * (row(c0 + j, e + k) + row(c0 + j, e - k));
+          dv[j] = v + od.c[kMaxHW] * row(c0 + j, e);
         }
       }
     } else {
@@ -373,20 +390,20 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
                       : sty[(l * (TY + 4) + (idx - (t.y0 - 2))) * TX + tx];
       };
       if (!dr.periodic && (i < 5 || i >= dr.len - 5)) {
-        closure_row<7, 5>(ot, i, dr.len,
-                          [&](int idx, double* v) {
+        closure_row<7, 5, NC>(ot, i, dr.len,
+                              [&](int idx, double* v) {
 #pragma unroll
-                            for (int l = 0; l < NV2; ++l) v[l] = tv(l, idx);
-                          },
-                          dv);
+                                for (int j = 0; j < NC; ++j) v[j] = tv(c0 + j, idx);
+                              },
+                              dv);
       } else {
-        const double c0 = cf.dt[0], c1 = cf.dt[1], c2 = cf.dt[2];
+        const double k0 = cf.dt[0], k1 = cf.dt[1], k2 = cf.dt[2];
 #pragma unroll
-        for (int l = 0; l < NV2; ++l) {
-          double s = 0.0;
-          s += c1 * (tv(l, i + 1) + tv(l, i - 1));
-          s += c2 * (tv(l, i + 2) + tv(l, i - 2));
-          dv[l] = s + c0 * tv(l, i);
+        for (int j = 0; j < NC; ++j) {
+          double v = 0.0;
+          v += k1 * (tv(c0 + j, i + 1) + tv(c0 + j, i - 1));
+          v += k2 * (tv(c0 + j, i + 2) + tv(c0 + j, i - 2));
+          dv[j] = v + k0 * tv(c0 + j, i);
         }
       }
       // H^-1: the norm weight is exactly 1 away from the first / last boundary rows, where
@@ -394,11 +411,11 @@ __device__ __forceinline__ void add_dissipation(const Geo& g, const TileGeo& t, 
       const double h = d == 0 ? tg.hw0(tx) : tg.hw1(ty);
       if (h != 1.0) {
 #pragma unroll
-        for (int l = 0; l < NV2; ++l) dv[l] = dv[l] / h;
+        for (int j = 0; j < NC; ++j) dv[j] = dv[j] / h;
       }
     }
 #pragma unroll
-    for (int l = 0; l < NV2; ++l) r[l] = r[l] + amount * dv[l];
+    for (int j = 0; j < NC; ++j) w[d][j] = amount * dv[j];
   }
 }
 
@@ -416,6 +433,42 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;
 }
 
+// x[c0 + j] of a register array with the wave-uniform group offset c0 (a multiple of
+// NC), formed by selects on constant indices (a run-time register index would move the
+// array to scratch memory).
+template <int NC>
+__device__ __forceinline__ double comp(const double* x, int c0, int j) {
+  double v = x[j];
+#pragma unroll
+  for (int k = 1; k * NC + j < NV2; ++k)
+    if (c0 == k * NC) v = x[k * NC + j];
+  return v;
+}
+
+// Rows C0 .. C0 + NC - 1 of sum_d A_d^T-side products of the adjoint rhs:
+// r[j] = sum_d sum_c A_d[C0 + j][c] T1[d][c] (computeRhsAdjoint's Jacobian part for 2-D
+// Euler), with the row offset a compile-time constant so only those rows are formed.
+template <int C0, int NC>
+__device__ __forceinline__ void adj_jacobian_rows(const double* mx, const Prim<2>& s, double gamma,
+                                                  const double (*T1)[NV2], double* r) {
+#pragma unroll
+  for (int j = 0; j < NC; ++j) r[j] = 0.0;
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    double m[2], A[16];
+    m[0] = d == 0 ? mx[0] : 0.0;
+    m[1] = d == 1 ? mx[1] : 0.0;
+    inviscid_jacobian<2>(m, s, gamma, A);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int c = 0; c < NV2; ++c) acc += A[(C0 + j) * 4 + c] * T1[d][c];
+      r[j] = r[j] + acc;
+    }
+  }
+}
+
 __device__ __forceinline__ bool in_patch(const DPatch& pt, int i, int j) {
   return i >= pt.lo[0] && i <= pt.hi[0] && j >= pt.lo[1] && j <= pt.hi[1];
 }
@@ -424,7 +477,7 @@ __device__ __forceinline__ bool in_patch(const DPatch& pt, int i, int j) {
 template <class S>
 __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_fwd_stage2d(Geo g, const Patches* __restrict__ P,
                                                        Tiling T, FwdStage a) {
-  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NT = S::NT;
+  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NP = S::NP, NC = S::NC;
   constexpr int SFX = S::SFX, STX = S::STX;
   __shared__ double sq[S::SQ];
   __shared__ double sc[S::SSCR];
@@ -441,17 +494,20 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_fwd_stage2d(Geo g, co
   const double gv = load_tile_grid<S>(g, t);
   const TileGrid<S> tg{sgrid};
   const unsigned pmask = T.pmask[tile];
-  // RK operands of this thread's point, fetched before the stencil work
-  double pq0[NV2], pb2[NV2];
-  {
-    const int tx0 = threadIdx.x % TX, ty0 = threadIdx.x / TX;
-    if (tx0 < t.sx && ty0 < t.sy) {
-      const long o = (t.x0 + tx0) + (long)g.n[0] * (t.y0 + ty0);
+  // this thread's point and component group
+  const int lane = threadIdx.x % NP, c0 = (threadIdx.x / NP) * NC;
+  const int tx = lane % TX, ty = lane / TX;
+  const bool mine = tx < t.sx && ty < t.sy;
+  const int gi = t.x0 + tx, gj = t.y0 + ty;
+  const long p = gi + (long)g.n[0] * gj;
+  // RK operands of this thread's components, fetched before the stencil work
+  double pq0[NC], pb2[NC];
+  if (mine) {
 #pragma unroll
-      for (int l = 0; l < NV2; ++l) {
-        pq0[l] = (a.stage == 2 || a.stage == 3) ? a.q0[inst * NVN + l * g.N + o] : 0.0;
-        pb2[l] = a.stage >= 2 ? a.b2[inst * NVN + l * g.N + o] : 0.0;
-      }
+    for (int j = 0; j < NC; ++j) {
+      const long o = (c0 + j) * g.N + p;
+      pq0[j] = (a.stage == 2 || a.stage == 3) ? a.q0[inst * NVN + o] : 0.0;
+      pb2[j] = a.stage >= 2 ? a.b2[inst * NVN + o] : 0.0;
     }
   }
   StageRegs<S> sr;
@@ -463,45 +519,41 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_fwd_stage2d(Geo g, co
   stage_store<S, true>(g, t, sr, tg, sq, sfx, sfy);
   __syncthreads();
   const Op& o1 = g.ops[SLOT_FIRST];
-  double r[PPT][NV2];
-  const int tx = threadIdx.x % TX;
+  double r[NC];
+  if (mine) {
+    double dfx[NC], dfy[NC];
+    deriv_lds<12, 8, NC>(o1, cf.d1, g.dir[0], gi, t.ex0, c0,
+                         [&](int l, int e) { return sfx[(l * TY + ty) * EX + e]; }, dfx);
+    deriv_lds<12, 8, NC>(o1, cf.d1, g.dir[1], gj, t.ey0, c0,
+                         [&](int l, int e) { return sfy[(l * EY + e) * TX + tx]; }, dfy);
 #pragma unroll
-  for (int k = 0; k < PPT; ++k) {
-    const int ty = threadIdx.x / TX + k * (NT / TX);
-    if (tx >= t.sx || ty >= t.sy) continue;
-    const int gi = t.x0 + tx, gj = t.y0 + ty;
-    double dfx[NV2], dfy[NV2];
-    deriv_lds<12, 8>(o1, cf.d1, g.dir[0], gi, t.ex0,
-              [&](int l, int e) { return sfx[(l * TY + ty) * EX + e]; }, dfx);
-    deriv_lds<12, 8>(o1, cf.d1, g.dir[1], gj, t.ey0,
-              [&](int l, int e) { return sfy[(l * EY + e) * TX + tx]; }, dfy);
-#pragma unroll
-    for (int l = 0; l < NV2; ++l) r[k][l] = 0.0 - (dfx[l] + dfy[l]);
+    for (int j = 0; j < NC; ++j) r[j] = 0.0 - (dfx[j] + dfy[j]);
   }
   double* stx = sc;
   double* sty = sc + STX;
   if (g.dissipation) {
     __syncthreads();   // fluxes consumed
     if (!g.composite) {
-      diss_intermediates<S>(g, t, cf, tg, sq, stx, sty);
+      diss_intermediates<S>(g, t, cf, tg, sq, stx, sty, lane, c0);
       __syncthreads();
     }
   }
+  double qn[NC];
+  if (mine) {
+    if (g.dissipation) {
+      double w[2][NC];
+      diss_terms<S>(g, t, cf, tg, sq, stx, sty, tx, ty, gi, gj, c0, g.diss_amount, w);
 #pragma unroll
-  for (int k = 0; k < PPT; ++k) {
-    const int ty = threadIdx.x / TX + k * (NT / TX);
-    if (tx >= t.sx || ty >= t.sy) continue;
-    const int gi = t.x0 + tx, gj = t.y0 + ty;
-    if (g.dissipation)
-      add_dissipation<S>(g, t, cf, tg, sq, stx, sty, tx, ty, gi, gj, g.diss_amount, r[k]);
-    const long p = gi + (long)g.n[0] * gj;
+      for (int d = 0; d < 2; ++d)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) r[j] = r[j] + w[d][j];
+    }
     const double jac = 1.0 / (tg.h0(tx) * tg.h1(ty));   // jacobian<2>
     double qp[NV2];
 #pragma unroll
-    for (int l = 0; l < NV2; ++l) {
-      r[k][l] = r[k][l] * jac;
-      qp[l] = sq[(l * EY + ty + HY) * EX + tx + HX];
-    }
+    for (int l = 0; l < NV2; ++l) qp[l] = sq[(l * EY + ty + HY) * EX + tx + HX];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) r[j] = r[j] * jac;
     unsigned m = pmask;
     while (m) {
       const int ip = __ffs(m) - 1;
@@ -515,11 +567,12 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_fwd_stage2d(Geo g, co
           const double* qt = pt.aplus + (long)pt.np * 16 + (long)kp * 4;
           const double sj = pt.sigma_inv * jac;
 #pragma unroll
-          for (int i = 0; i < NV2; ++i) {
-            double s = 0.0;
+          for (int j = 0; j < NC; ++j) {
+            const int i = c0 + j;
+            double v = 0.0;
 #pragma unroll
-            for (int c = 0; c < NV2; ++c) s += A[c * 4 + i] * (qp[c] - qt[c]);
-            r[k][i] = r[k][i] - sj * s;
+            for (int c = 0; c < NV2; ++c) v += A[c * 4 + i] * (qp[c] - qt[c]);
+            r[j] = r[j] - sj * v;
           }
           break;
         }
@@ -527,59 +580,85 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_fwd_stage2d(Geo g, co
           const double sp = pt.strength[kp];
           const double* qt = pt.aplus + (long)kp * 4;
 #pragma unroll
-          for (int l = 0; l < NV2; ++l) r[k][l] = r[k][l] - sp * (qp[l] - qt[l]);
+          for (int j = 0; j < NC; ++j) r[j] = r[j] - sp * (comp<NC>(qp, c0, j) - qt[c0 + j]);
           break;
         }
         case MGD_PATCH_ACTUATOR:
-          if (a.control) r[k][3] = r[k][3] + pt.strength[kp] * a.control[inst * a.control_stride + kp];
+#pragma unroll
+          for (int j = 0; j < NC; ++j)
+            if (c0 + j == 3 && a.control)
+              r[j] = r[j] + pt.strength[kp] * a.control[inst * a.control_stride + kp];
           break;
         case MGD_PATCH_KOLMOGOROV:
-          r[k][1] = r[k][1] + qp[0] * pt.strength[kp];
+#pragma unroll
+          for (int j = 0; j < NC; ++j)
+            if (c0 + j == 1) r[j] = r[j] + qp[0] * pt.strength[kp];
           break;
         default:
           break;
       }
     }
-    for (int s = 0; s < a.nsrc; ++s) {
-      const double dx0 = tg.x0(tx) - a.src[s].loc[0];
-      const double dy0 = tg.x1(ty) - a.src[s].loc[1];
-      double rr2 = 0.0;
-      rr2 += dx0 * dx0;
-      rr2 += dy0 * dy0;
-      r[k][3] = r[k][3] + a.src_amp[inst * a.src_inst_stride + s] * exp(-a.src[s].gfac * rr2);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      if (c0 + j != 3) continue;
+      for (int s = 0; s < a.nsrc; ++s) {
+        const double dx0 = tg.x0(tx) - a.src[s].loc[0];
+        const double dy0 = tg.x1(ty) - a.src[s].loc[1];
+        double rr2 = 0.0;
+        rr2 += dx0 * dx0;
+        rr2 += dy0 * dy0;
+        r[j] = r[j] + a.src_amp[inst * a.src_inst_stride + s] * exp(-a.src[s].gfac * rr2);
+      }
     }
     // RK4 stage update (substepForwardRK4, src/RK4IntegratorImpl.f90:115-156)
     const double dt = a.dt;
     double* b2 = a.b2 + inst * NVN;
-    double qn[NV2];
 #pragma unroll
-    for (int l = 0; l < NV2; ++l) {
+    for (int j = 0; j < NC; ++j) {
+      const int l = c0 + j;
       const long o = l * g.N + p;
-      const double rl = r[k][l];
+      const double rl = r[j];
       if (a.stage == 1) {
-        b2[o] = qp[l] + div6(dt * rl);
-        qn[l] = qp[l] + dt * rl / 2.0;
+        const double ql = comp<NC>(qp, c0, j);
+        b2[o] = ql + div6(dt * rl);
+        qn[j] = ql + dt * rl / 2.0;
       } else if (a.stage == 2) {
-        b2[o] = pb2[l] + div3(dt * rl);
-        qn[l] = pq0[l] + dt * rl / 2.0;
+        b2[o] = pb2[j] + div3(dt * rl);
+        qn[j] = pq0[j] + dt * rl / 2.0;
       } else if (a.stage == 3) {
-        b2[o] = pb2[l] + div3(dt * rl);
-        qn[l] = pq0[l] + dt * rl;
+        b2[o] = pb2[j] + div3(dt * rl);
+        qn[j] = pq0[j] + dt * rl;
       } else {
-        qn[l] = pb2[l] + div6(dt * rl);
+        qn[j] = pb2[j] + div6(dt * rl);
       }
-      a.qout[inst * NVN + o] = qn[l];
+      a.qout[inst * NVN + o] = qn[j];
       if (a.rhs_out) a.rhs_out[inst * NVN + o] = rl;
     }
-    if (a.jpartial && a.ip_cost >= 0 && ((pmask >> a.ip_cost) & 1u)) {
-      const DPatch& pt = P->p[a.ip_cost];
-      if (in_patch(pt, gi, gj)) {
-        const int kp = (gi - pt.lo[0]) + pt.ls[0] * (gj - pt.lo[1]);
-        const Prim<2> s = prim<2>(qn, g.gamma);
-        const double F = s.p - pt.aplus[kp];
-        const double gn = (tg.hw1(ty) * (tg.hw0(tx) * 1.0)) * (tg.h0(tx) * tg.h1(ty));   // gnorm<2>
-        a.jpartial[(inst * a.nslots + a.slot) * a.npts + kp] = 1.0 * gn * (F * F * pt.strength[kp]);
+  }
+  // functional term of the new stage state at cost-target points (needs all components:
+  // with CS > 1 they are exchanged through the operand space)
+  if (a.jpartial && a.ip_cost >= 0 && ((pmask >> a.ip_cost) & 1u)) {   // uniform per block
+    double qa[NV2];
+    if (S::CS == 1) {
+#pragma unroll
+      for (int j = 0; j < NC; ++j) qa[j] = qn[j];
+    } else {
+      __syncthreads();   // every read of the staged operand is done
+      if (mine) {
+#pragma unroll
+        for (int j = 0; j < NC; ++j) sq[(c0 + j) * NP + lane] = qn[j];
       }
+      __syncthreads();
+#pragma unroll
+      for (int l = 0; l < NV2; ++l) qa[l] = sq[l * NP + lane];
+    }
+    const DPatch& pt = P->p[a.ip_cost];
+    if (mine && c0 == 0 && in_patch(pt, gi, gj)) {
+      const int kp = (gi - pt.lo[0]) + pt.ls[0] * (gj - pt.lo[1]);
+      const Prim<2> s = prim<2>(qa, g.gamma);
+      const double F = s.p - pt.aplus[kp];
+      const double gn = (tg.hw1(ty) * (tg.hw0(tx) * 1.0)) * (tg.h0(tx) * tg.h1(ty));   // gnorm<2>
+      a.jpartial[(inst * a.nslots + a.slot) * a.npts + kp] = 1.0 * gn * (F * F * pt.strength[kp]);
     }
   }
 }
@@ -588,7 +667,7 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_fwd_stage2d(Geo g, co
 template <class S>
 __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_adj_stage2d(Geo g, const Patches* __restrict__ P,
                                                        Tiling T, AdjStage a) {
-  constexpr int TX = S::TX, EX = S::EX, EY = S::EY, NT = S::NT;
+  constexpr int TX = S::TX, EX = S::EX, EY = S::EY, NP = S::NP, NC = S::NC;
   constexpr int STX = S::STX;
   __shared__ double sq[S::SQ];
   __shared__ double sc[S::SSCR];
@@ -604,15 +683,16 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_adj_stage2d(Geo g, co
   const double gv = load_tile_grid<S>(g, t);
   const TileGrid<S> tg{sgrid};
   const unsigned pmask = T.pmask[tile];
+  const int lane = threadIdx.x % NP, c0 = (threadIdx.x / NP) * NC;
+  const int tx = lane % TX, ty = lane / TX;
+  const bool mine = tx < t.sx && ty < t.sy;
+  const int gi = t.x0 + tx, gj = t.y0 + ty;
+  const long p = gi + (long)g.n[0] * gj;
   // forward stage state at this thread's point, fetched before the stencil work
   double pq[NV2];
-  {
-    const int tx0 = threadIdx.x % TX, ty0 = threadIdx.x / TX;
-    if (tx0 < t.sx && ty0 < t.sy) {
-      const long o = (t.x0 + tx0) + (long)g.n[0] * (t.y0 + ty0);
+  if (mine) {
 #pragma unroll
-      for (int l = 0; l < NV2; ++l) pq[l] = qi[l * g.N + o];
-    }
+    for (int l = 0; l < NV2; ++l) pq[l] = qi[l * g.N + p];
   }
   {
     StageRegs<S> sr;
@@ -624,136 +704,162 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_adj_stage2d(Geo g, co
   double* sty = sc + STX;
   if (g.dissipation && !g.composite) {
     __syncthreads();
-    diss_intermediates<S>(g, t, cf, tg, sq, stx, sty);
+    diss_intermediates<S>(g, t, cf, tg, sq, stx, sty, lane, c0);
   }
   __syncthreads();
   const Op& oa = g.ops[SLOT_ADJ];
-  const int tx = threadIdx.x % TX;
+  double T1[2][NV2];   // D+ lambda along x and y, all components (own ones first)
+  double w[2][NC];     // dissipation terms of this thread's components
+  if (mine) {
+    double u[2][NC];
+    deriv_lds<16, 12, NC>(oa, cf.d1, g.dir[0], gi, t.ex0, c0,
+                          [&](int l, int e) { return sq[(l * EY + ty + HY) * EX + e]; }, u[0]);
+    deriv_lds<16, 12, NC>(oa, cf.d1, g.dir[1], gj, t.ey0, c0,
+                          [&](int l, int e) { return sq[(l * EY + e) * EX + tx + HX]; }, u[1]);
 #pragma unroll
-  for (int k = 0; k < PPT; ++k) {
-    const int ty = threadIdx.x / TX + k * (NT / TX);
-    if (tx >= t.sx || ty >= t.sy) continue;
-    const int gi = t.x0 + tx, gj = t.y0 + ty;
-    double T1[2][NV2];
-    deriv_lds<16, 12>(oa, cf.d1, g.dir[0], gi, t.ex0,
-              [&](int l, int e) { return sq[(l * EY + ty + HY) * EX + e]; }, T1[0]);
-    deriv_lds<16, 12>(oa, cf.d1, g.dir[1], gj, t.ey0,
-              [&](int l, int e) { return sq[(l * EY + e) * EX + tx + HX]; }, T1[1]);
-    const long p = gi + (long)g.n[0] * gj;
-    const Prim<2> s = prim<2>(pq, g.gamma);
-    double r[NV2] = {0.0, 0.0, 0.0, 0.0};
+    for (int d = 0; d < 2; ++d)
 #pragma unroll
-    for (int d = 0; d < 2; ++d) {
-      double m[2], A[16];
-      m[0] = d == 0 ? tg.h1(ty) : 0.0;   // metric<2>(g, 0, .)
-      m[1] = d == 1 ? tg.h0(tx) : 0.0;   // metric<2>(g, 1, .)
-      inviscid_jacobian<2>(m, s, g.gamma, A);
+      for (int j = 0; j < NC; ++j) T1[d][j] = u[d][j];
+    // with component groups the dissipation intermediates are read before the exchange
+    if (S::CS > 1 && g.dissipation)
+      diss_terms<S>(g, t, cf, tg, sq, stx, sty, tx, ty, gi, gj, c0, -g.diss_amount, w);
+  }
+  if (S::CS > 1) {
+    // every component of D+ lambda is needed by the Jacobian products: exchange them
+    // through the scratch space (the dissipation intermediates are no longer read)
+    __syncthreads();
+    if (mine) {
 #pragma unroll
-      for (int i = 0; i < NV2; ++i) {
-        double acc = 0.0;
+      for (int d = 0; d < 2; ++d)
 #pragma unroll
-        for (int c = 0; c < NV2; ++c) acc += A[i * 4 + c] * T1[d][c];
-        r[i] = r[i] + acc;
+        for (int j = 0; j < NC; ++j) sc[(d * NV2 + c0 + j) * NP + lane] = T1[d][j];
+    }
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+      for (int d = 0; d < 2; ++d)
+#pragma unroll
+        for (int l = 0; l < NV2; ++l) T1[d][l] = sc[(d * NV2 + l) * NP + lane];
+    }
+  }
+  if (!mine) return;
+  const Prim<2> s = prim<2>(pq, g.gamma);
+  double r[NC];
+  const double mx[2] = {tg.h1(ty), tg.h0(tx)};   // metric<2>(g, 0, .), metric<2>(g, 1, .)
+  if (c0 == 0)
+    adj_jacobian_rows<0, NC>(mx, s, g.gamma, T1, r);
+  else
+    adj_jacobian_rows<(S::CS > 1 ? NC : 0), NC>(mx, s, g.gamma, T1, r);
+  // RK operands, fetched here so that they are not live through the derivative work
+  double pl0[NC], pb2[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    const long o = (c0 + j) * g.N + p;
+    pl0[j] = (a.stage == 3 || a.stage == 2) ? a.l0[inst * NVN + o] : 0.0;
+    pb2[j] = a.stage <= 3 ? a.b2[inst * NVN + o] : 0.0;
+  }
+  if (g.dissipation) {
+    if (S::CS == 1)
+      diss_terms<S>(g, t, cf, tg, sq, stx, sty, tx, ty, gi, gj, c0, -g.diss_amount, w);
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) r[j] = r[j] + w[d][j];
+  }
+  const double jac = 1.0 / (tg.h0(tx) * tg.h1(ty));   // jacobian<2>
+#pragma unroll
+  for (int j = 0; j < NC; ++j) r[j] = r[j] * jac;
+  double lp[NV2];
+#pragma unroll
+  for (int l = 0; l < NV2; ++l) lp[l] = sq[(l * EY + ty + HY) * EX + tx + HX];
+  unsigned m = pmask;
+  while (m) {
+    const int ip = __ffs(m) - 1;
+    m &= m - 1;
+    const DPatch& pt = P->p[ip];
+    if (!in_patch(pt, gi, gj)) continue;
+    const int kp = (gi - pt.lo[0]) + pt.ls[0] * (gj - pt.lo[1]);
+    switch (pt.type) {
+      case MGD_PATCH_FAR_FIELD: {
+        const double* A = pt.aplus + (long)kp * 16;
+        const double sj = pt.sigma_inv * jac;
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+          const int i = c0 + j;
+          double acc = 0.0;
+#pragma unroll
+          for (int c = 0; c < NV2; ++c) acc += A[i * 4 + c] * lp[c];
+          r[j] = r[j] + sj * acc;
+        }
+        break;
       }
-    }
-    // RK operands, fetched here so that they are not live through the derivative work
-    double pl0[NV2], pb2[NV2];
+      case MGD_PATCH_SPONGE: {
+        const double sp = pt.strength[kp];
 #pragma unroll
-    for (int l = 0; l < NV2; ++l) {
-      pl0[l] = (a.stage == 3 || a.stage == 2) ? a.l0[inst * NVN + l * g.N + p] : 0.0;
-      pb2[l] = a.stage <= 3 ? a.b2[inst * NVN + l * g.N + p] : 0.0;
-    }
-    if (g.dissipation)
-      add_dissipation<S>(g, t, cf, tg, sq, stx, sty, tx, ty, gi, gj, -g.diss_amount, r);
-    const double jac = 1.0 / (tg.h0(tx) * tg.h1(ty));   // jacobian<2>
-    double lp[NV2];
-#pragma unroll
-    for (int l = 0; l < NV2; ++l) {
-      r[l] = r[l] * jac;
-      lp[l] = sq[(l * EY + ty + HY) * EX + tx + HX];
-    }
-    unsigned m = pmask;
-    while (m) {
-      const int ip = __ffs(m) - 1;
-      m &= m - 1;
-      const DPatch& pt = P->p[ip];
-      if (!in_patch(pt, gi, gj)) continue;
-      const int kp = (gi - pt.lo[0]) + pt.ls[0] * (gj - pt.lo[1]);
-      switch (pt.type) {
-        case MGD_PATCH_FAR_FIELD: {
-          const double* A = pt.aplus + (long)kp * 16;
-          const double sj = pt.sigma_inv * jac;
-#pragma unroll
-          for (int i = 0; i < NV2; ++i) {
-            double acc = 0.0;
-#pragma unroll
-            for (int c = 0; c < NV2; ++c) acc += A[i * 4 + c] * lp[c];
-            r[i] = r[i] + sj * acc;
-          }
-          break;
+        for (int j = 0; j < NC; ++j) r[j] = r[j] + sp * comp<NC>(lp, c0, j);
+        break;
+      }
+      case MGD_PATCH_COST_TARGET: {
+        // acoustic-noise adjoint forcing from the forward state at this point
+        // (src/AcousticNoiseImpl.f90:208-280), scaled by the stage factor
+        double af[NV2] = {0.0, 0.0, 0.0, 0.0};
+        if (!a.zero_forcing) {
+          const double F = -2.0 * pt.strength[kp] * 1.0 * (g.gamma - 1.0) * (s.p - pt.aplus[kp]);
+          af[3] = F;
+          af[1] = -s.u[0] * F;
+          af[2] = -s.u[1] * F;
+          double ke = 0.0;
+          ke += s.u[0] * s.u[0];
+          ke += s.u[1] * s.u[1];
+          af[0] = 0.5 * ke * F;
         }
-        case MGD_PATCH_SPONGE: {
-          const double sp = pt.strength[kp];
 #pragma unroll
-          for (int l = 0; l < NV2; ++l) r[l] = r[l] + sp * lp[l];
-          break;
-        }
-        case MGD_PATCH_COST_TARGET: {
-          // acoustic-noise adjoint forcing from the forward state at this point
-          // (src/AcousticNoiseImpl.f90:208-280), scaled by the stage factor
-          double af[NV2] = {0.0, 0.0, 0.0, 0.0};
-          if (!a.zero_forcing) {
-            const double F = -2.0 * pt.strength[kp] * 1.0 * (g.gamma - 1.0) * (s.p - pt.aplus[kp]);
-            af[3] = F;
-            af[1] = -s.u[0] * F;
-            af[2] = -s.u[1] * F;
-            double ke = 0.0;
-            ke += s.u[0] * s.u[0];
-            ke += s.u[1] * s.u[1];
-            af[0] = 0.5 * ke * F;
-          }
-#pragma unroll
-          for (int l = 0; l < NV2; ++l) r[l] = r[l] + a.forcing_factor * af[l];
-          break;
-        }
-        case MGD_PATCH_ACTUATOR: {
-          // ThermalActuator gradient and sensitivity on the pre-update adjoint state
+        for (int j = 0; j < NC; ++j) r[j] = r[j] + a.forcing_factor * comp<NC>(af, c0, j);
+        break;
+      }
+      case MGD_PATCH_ACTUATOR: {
+        // ThermalActuator gradient and sensitivity on the pre-update adjoint state,
+        // written by the group that owns the energy component
+        if (c0 + NC - 1 == 3) {
           if (a.grad) a.grad[inst * a.grad_stride + kp] = lp[3] * (pt.strength[kp] * 1.0);
           if (a.spartial) {
             const double F = lp[3] * pt.strength[kp] * 1.0;
             const double gn = (tg.hw1(ty) * (tg.hw0(tx) * 1.0)) * (tg.h0(tx) * tg.h1(ty));   // gnorm<2>
             a.spartial[(inst * a.nslots + a.slot) * a.npts + kp] = 1.0 * gn * (F * F);
           }
-          break;
         }
-        case MGD_PATCH_KOLMOGOROV:
-          r[0] = r[0] - lp[1] * pt.strength[kp];
-          break;
+        break;
       }
-    }
-    // RK4 adjoint stage update (substepAdjointRK4, src/RK4IntegratorImpl.f90:215-261)
-    const double dt = a.dt;
-    double* b2 = a.b2 + inst * NVN;
+      case MGD_PATCH_KOLMOGOROV:
 #pragma unroll
-    for (int l = 0; l < NV2; ++l) {
-      const long o = l * g.N + p;
-      const double rl = r[l];
-      double ln;
-      if (a.stage == 4) {
-        b2[o] = lp[l] - div6(dt * rl);
-        ln = lp[l] - dt * rl / 2.0;
-      } else if (a.stage == 3) {
-        b2[o] = pb2[l] - div3(dt * rl);
-        ln = pl0[l] - dt * rl / 2.0;
-      } else if (a.stage == 2) {
-        b2[o] = pb2[l] - div3(dt * rl);
-        ln = pl0[l] - dt * rl;
-      } else {
-        ln = pb2[l] - div6(dt * rl);
-      }
-      a.lout[inst * NVN + o] = ln;
-      if (a.rhs_out) a.rhs_out[inst * NVN + o] = rl;
+        for (int j = 0; j < NC; ++j)
+          if (c0 + j == 0) r[j] = r[j] - lp[1] * pt.strength[kp];
+        break;
     }
+  }
+  // RK4 adjoint stage update (substepAdjointRK4, src/RK4IntegratorImpl.f90:215-261)
+  const double dt = a.dt;
+  double* b2 = a.b2 + inst * NVN;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    const long o = (c0 + j) * g.N + p;
+    const double rl = r[j];
+    double ln;
+    if (a.stage == 4) {
+      const double ll = comp<NC>(lp, c0, j);
+      b2[o] = ll - div6(dt * rl);
+      ln = ll - dt * rl / 2.0;
+    } else if (a.stage == 3) {
+      b2[o] = pb2[j] - div3(dt * rl);
+      ln = pl0[j] - dt * rl / 2.0;
+    } else if (a.stage == 2) {
+      b2[o] = pb2[j] - div3(dt * rl);
+      ln = pl0[j] - dt * rl;
+    } else {
+      ln = pb2[j] - div6(dt * rl);
+    }
+    a.lout[inst * NVN + o] = ln;
+    if (a.rhs_out) a.rhs_out[inst * NVN + o] = rl;
   }
 }
 
@@ -806,8 +912,11 @@ using S32x16 = Shape<32, 16>;
 using S16x32 = Shape<16, 32>;
 using S16x16 = Shape<16, 16>;
 using S24x16 = Shape<24, 16>;
+using S16x16c2 = Shape<16, 16, 2>;
+using S16x12c2 = Shape<16, 12, 2>;
 
-static const int kShapeDims[kNumTileShapes][2] = {{32, 16}, {16, 32}, {16, 16}, {24, 16}};
+static const int kShapeDims[kNumTileShapes][2] = {{32, 16}, {16, 32}, {16, 16}, {24, 16},
+                                                  {16, 16}, {16, 12}};
 
 int fused2d_shape_dims(int shape, int* tx, int* ty) {
   if (shape < 0 || shape >= kNumTileShapes) return MGD_ERR_ARG;
@@ -816,16 +925,22 @@ int fused2d_shape_dims(int shape, int* tx, int* ty) {
   return MGD_OK;
 }
 
-// Balanced partition of len points into ceil(len / T) tiles: tile b starts at
-// b q + min(b, r) and holds q + (b < r) points.
+// Balanced partition of len points into n = ceil(len / T) tiles of q or q + 1 points,
+// the larger ones at both ends (l leading, r trailing: the edge tiles are the largest,
+// as the boundary closures need): tile b holds q + (b < l || b >= n - r) points.
 static void balanced(int len, int T, std::vector<int>& org, std::vector<int>& size, int* q,
-                     int* r) {
+                     int* l, int* r) {
   const int n = (len + T - 1) / T;
+  const int extra = len % n;
   *q = len / n;
-  *r = len % n;
+  *l = (extra + 1) / 2;
+  *r = extra / 2;
+  int o = 0;
   for (int b = 0; b < n; ++b) {
-    org.push_back(b * *q + std::min(b, *r));
-    size.push_back(*q + (b < *r ? 1 : 0));
+    const int sz = *q + (b < *l || b >= n - *r ? 1 : 0);
+    org.push_back(o);
+    size.push_back(sz);
+    o += sz;
   }
 }
 
@@ -835,9 +950,9 @@ int fused2d_make_tiling(const Geo& g, const Patches& hp, int shape, HostTiling* 
   out->shape = shape;
   out->org.clear();
   out->size.clear();
-  balanced(g.n[0], tx, out->org, out->size, &out->qx, &out->rx);
+  balanced(g.n[0], tx, out->org, out->size, &out->qx, &out->lx, &out->rx);
   out->nbx = (int)out->org.size();
-  balanced(g.n[1], ty, out->org, out->size, &out->qy, &out->ry);
+  balanced(g.n[1], ty, out->org, out->size, &out->qy, &out->ly, &out->ry);
   out->nby = (int)out->org.size() - out->nbx;
   for (int d = 0; d < 2; ++d) {
     if (g.dir[d].periodic) continue;
@@ -893,7 +1008,9 @@ void launch_fwd_stage2d(const Geo& g, const Patches* P, const Tiling& T, const F
     case 0: fwd_launch<S32x16>(g, P, T, a, st); break;
     case 1: fwd_launch<S16x32>(g, P, T, a, st); break;
     case 2: fwd_launch<S16x16>(g, P, T, a, st); break;
-    default: fwd_launch<S24x16>(g, P, T, a, st); break;
+    case 3: fwd_launch<S24x16>(g, P, T, a, st); break;
+    case 4: fwd_launch<S16x16c2>(g, P, T, a, st); break;
+    default: fwd_launch<S16x12c2>(g, P, T, a, st); break;
   }
 }
 
@@ -903,7 +1020,9 @@ void launch_adj_stage2d(const Geo& g, const Patches* P, const Tiling& T, const A
     case 0: adj_launch<S32x16>(g, P, T, a, st); break;
     case 1: adj_launch<S16x32>(g, P, T, a, st); break;
     case 2: adj_launch<S16x16>(g, P, T, a, st); break;
-    default: adj_launch<S24x16>(g, P, T, a, st); break;
+    case 3: adj_launch<S24x16>(g, P, T, a, st); break;
+    case 4: adj_launch<S16x16c2>(g, P, T, a, st); break;
+    default: adj_launch<S16x12c2>(g, P, T, a, st); break;
   }
 }
 

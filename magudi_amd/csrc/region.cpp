@@ -490,8 +490,10 @@ int install_tiling(mgd_region* r, int shape) {
   r->tiling.nblk = H.nblk;
   r->tiling.band = H.band;
   r->tiling.qx = H.qx;
+  r->tiling.lx = H.lx;
   r->tiling.rx = H.rx;
   r->tiling.qy = H.qy;
+  r->tiling.ly = H.ly;
   r->tiling.ry = H.ry;
   r->tiling.pmask = reinterpret_cast<const unsigned*>(r->d_tile_mask.p);
   return MGD_OK;
@@ -1352,7 +1354,7 @@ int mgd_region_set_option(mgd_region* r, int option, int value) {
   switch (option) {
     case MGD_OPT_FORCE_PLAIN: r->force_plain = value != 0; return MGD_OK;
     case MGD_OPT_TILE_SHAPE: {
-      if (value < -1 || value >= kNumTileShapes) { set_error("tile shape must be -1 .. 3"); return MGD_ERR_ARG; }
+      if (value < -1 || value >= kNumTileShapes) { set_error("tile shape must be -1 .. 5"); return MGD_ERR_ARG; }
       r->tile_shape = value;
       r->tuned = false;
       if (r->finalized && r->nd == 2) {

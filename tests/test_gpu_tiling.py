@@ -13,7 +13,7 @@ from magudi_amd import cases
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [0, 1, 2, 3]
+SHAPES = [0, 1, 2, 3, 4, 5]
 
 
 def maxrel(a, b):
@@ -31,8 +31,8 @@ def periodic_x_monopole(n=48):
 
 
 CASES = {
-    "monopole53": lambda: cases.acoustic_monopole(n=53, nsteps=1),
-    "monopole53_composite": lambda: cases.acoustic_monopole(n=53, nsteps=1, composite=True),
+    "monopole60": lambda: cases.acoustic_monopole(n=60, nsteps=1),
+    "monopole60_composite": lambda: cases.acoustic_monopole(n=60, nsteps=1, composite=True),
     "periodic_x48": periodic_x_monopole,
 }
 
@@ -71,7 +71,7 @@ def test_rhs_all_shapes(name, mode):
         assert maxrel(o, ref) < 1e-13, (sh, maxrel(o, ref))
 
 
-@pytest.mark.parametrize("name", ["monopole53", "periodic_x48"])
+@pytest.mark.parametrize("name", ["monopole60", "periodic_x48"])
 def test_adjoint_run_all_shapes(name):
     from magudi_amd.solver import Region, Solver
     c = CASES[name]()
