@@ -48,6 +48,8 @@ def parse():
                     help="grid of the 3-D viscous Kolmogorov rhs leg (0: skip)")
     ap.add_argument("--viscous3d-only", action="store_true",
                     help="only the 3-D viscous rhs leg (its profiling workload)")
+    ap.add_argument("--tile-shape", type=int, default=-1,
+                    help="tile shape of the fused 2-D kernels (-1: autotune)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the HIP-event timing of the stage kernels and the copy kernel "
                          "(the workload of the rocprofv3 counter runs)")
@@ -100,7 +102,8 @@ def cpu_baseline(n, nsteps, save_interval):
 
 def viscous3d_leg(n):
     """One forward and one adjoint rhs evaluation of the 3-D viscous periodic Kolmogorov
-    flow at n^3 (generic per-operator kernels), HIP-event timed."""
+    flow at n^3, HIP-event timed (per-operator kernels: one-pass periodic gradient,
+    LDS-staged flux divergence, line-march derivatives)."""
     from magudi_amd import capi, cases
     from magudi_amd.solver import Region
     c = cases.kolmogorov3d(n=n, nsteps=1)
@@ -110,7 +113,7 @@ def viscous3d_leg(n):
     a_ms, a_b = R.time_kernel(capi.KERNEL_RHS_ADJOINT, reps=5)
     del R
     leg = {"workload": f"Kolmogorov {n}^3 viscous periodic rhs (Re_ac 500, Pr 0.7)",
-           "path": "generic per-operator kernels"}
+           "path": "3-D per-operator kernels (periodic gradient / flux-divergence / line derivatives)"}
     for k, ms, b in (("forward_rhs", f_ms, f_b), ("adjoint_rhs", a_ms, a_b)):
         gbs = b / (ms * 1e-3) / 1e9
         leg[k] = {"ms": round(ms, 4), "algorithmic_bytes": b, "achieved_GBs": round(gbs, 1),
@@ -187,7 +190,7 @@ def main():
         return 0
     case = cases.acoustic_monopole(n=a.n, nsteps=a.nsteps, save_interval=a.save_interval)
     q0 = case.initial_state()
-    R = Region(case)
+    R = Region(case, tile_shape=a.tile_shape)
     R.states[0].initialCondition = q0
     if a.kernels_only:
         R.restart()
