@@ -52,6 +52,9 @@ struct Shape {
   static constexpr int kBlocksPerCu = (160 * 1024) / kLdsBytes;
   static constexpr int kWaves = kBlocksPerCu * NT / 64 / 4;
   static constexpr int kWavesPerEU = kWaves < 1 ? 1 : kWaves > 8 ? 8 : kWaves;
+  // forward stage: form the dissipation intermediates in registers alongside the flux
+  // derivatives (where the register budget has room for them)
+  static constexpr bool kDissRegs = kWavesPerEU <= 3 || CS > 1;
 };
 
 struct TileGeo {
@@ -293,20 +296,34 @@ __device__ __forceinline__ double d4_at(const Dir& dd, int i, double c0, double 
   return s + c0 * x(i);
 }
 
+// Slots of diss_intermediates_to per thread, and the intermediates of one thread held in
+// registers (so that they can be formed before the flux space they are stored over is
+// free).
+template <class S>
+struct DissSlots {
+  static constexpr int IX = (S::TY * (S::TX + 4) + S::NP - 1) / S::NP;
+  static constexpr int IY = ((S::TY + 4) * S::TX + S::NP - 1) / S::NP;
+  static constexpr int PX = S::TY * (S::TX + 4), PY = (S::TY + 4) * S::TX;   // planes
+  double vx[IX][S::NC], vy[IY][S::NC];
+  int bx[IX], by[IY];   // LDS base index, or -1
+};
+
 // t = -arc D4 x on the tile rows (x direction, columns x0-2 .. x0+sx+1) and tile columns
 // (y direction, rows y0-2 .. y0+sy+1), for staged points inside the domain, components
 // c0 .. c0 + NC - 1; the positions are dealt to the NP threads of the component group
-// (lane = index within the group).
-template <class S>
-__device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& t, const Coefs& cf,
-                                                   const TileGrid<S>& tg, const double* sq,
-                                                   double* stx, double* sty, int lane, int c0) {
+// (lane = index within the group).  Each value goes to sink_x / sink_y (it, j, base, v):
+// slot `it` of this thread, component c0 + j, LDS index base + (c0 + j) * plane.
+template <class S, class SinkX, class SinkY>
+__device__ __forceinline__ void diss_intermediates_to(const Geo& g, const TileGeo& t,
+                                                      const Coefs& cf, const TileGrid<S>& tg,
+                                                      const double* sq, int lane, int c0,
+                                                      SinkX&& sink_x, SinkY&& sink_y) {
   constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NP = S::NP, NC = S::NC;
   const double k0 = cf.dd[0], k1 = cf.dd[1], k2 = cf.dd[2];
   const Dir dx = g.dir[0], dy = g.dir[1];
   const int wx = t.sx + 4;
 #pragma unroll
-  for (int it = 0; it < (TY * (TX + 4) + NP - 1) / NP; ++it) {
+  for (int it = 0; it < DissSlots<S>::IX; ++it) {
     const int e = lane + it * NP;
     if (e >= TY * (TX + 4)) continue;
     const int ex = e % (TX + 4) + 2, ey = e / (TX + 4) + HY;
@@ -319,12 +336,12 @@ __device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& 
       const int l = c0 + j;
       const double* r = sq + (l * EY + ey) * EX;
       const double v = d4_at(dx, gi, k0, k1, k2, [&](int k) { return r[k - t.ex0]; });
-      stx[(l * TY + (ey - HY)) * (TX + 4) + (ex - 2)] = -arc * v;
+      sink_x(it, j, (ey - HY) * (TX + 4) + (ex - 2), -arc * v);
     }
   }
   const int hy = t.sy + 4;
 #pragma unroll
-  for (int it = 0; it < ((TY + 4) * TX + NP - 1) / NP; ++it) {
+  for (int it = 0; it < DissSlots<S>::IY; ++it) {
     const int e = lane + it * NP;
     if (e >= (TY + 4) * TX) continue;
     const int ex = e % TX + HX, ey = e / TX + 2;
@@ -337,8 +354,50 @@ __device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& 
       const int l = c0 + j;
       const double* c = sq + l * EY * EX + ex;
       const double v = d4_at(dy, gj, k0, k1, k2, [&](int k) { return c[(k - t.ey0) * EX]; });
-      sty[(l * (TY + 4) + (ey - 2)) * TX + (ex - HX)] = -arc * v;
+      sink_y(it, j, (ey - 2) * TX + (ex - HX), -arc * v);
     }
+  }
+}
+
+// The intermediates straight into LDS.
+template <class S>
+__device__ __forceinline__ void diss_intermediates(const Geo& g, const TileGeo& t, const Coefs& cf,
+                                                   const TileGrid<S>& tg, const double* sq,
+                                                   double* stx, double* sty, int lane, int c0) {
+  diss_intermediates_to<S>(
+      g, t, cf, tg, sq, lane, c0,
+      [&](int, int j, int base, double v) { stx[(c0 + j) * DissSlots<S>::PX + base] = v; },
+      [&](int, int j, int base, double v) { sty[(c0 + j) * DissSlots<S>::PY + base] = v; });
+}
+
+// The intermediates into registers, then (diss_store) into LDS.
+template <class S>
+__device__ __forceinline__ void diss_compute(const Geo& g, const TileGeo& t, const Coefs& cf,
+                                             const TileGrid<S>& tg, const double* sq, int lane,
+                                             int c0, DissSlots<S>& d) {
+#pragma unroll
+  for (int it = 0; it < DissSlots<S>::IX; ++it) d.bx[it] = -1;
+#pragma unroll
+  for (int it = 0; it < DissSlots<S>::IY; ++it) d.by[it] = -1;
+  diss_intermediates_to<S>(
+      g, t, cf, tg, sq, lane, c0,
+      [&](int it, int j, int base, double v) { d.bx[it] = base; d.vx[it][j] = v; },
+      [&](int it, int j, int base, double v) { d.by[it] = base; d.vy[it][j] = v; });
+}
+
+template <class S>
+__device__ __forceinline__ void diss_store(const DissSlots<S>& d, int c0, double* stx, double* sty) {
+#pragma unroll
+  for (int it = 0; it < DissSlots<S>::IX; ++it) {
+    if (d.bx[it] < 0) continue;
+#pragma unroll
+    for (int j = 0; j < S::NC; ++j) stx[(c0 + j) * DissSlots<S>::PX + d.bx[it]] = d.vx[it][j];
+  }
+#pragma unroll
+  for (int it = 0; it < DissSlots<S>::IY; ++it) {
+    if (d.by[it] < 0) continue;
+#pragma unroll
+    for (int j = 0; j < S::NC; ++j) sty[(c0 + j) * DissSlots<S>::PY + d.by[it]] = d.vy[it][j];
   }
 }
 
@@ -519,6 +578,11 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_fwd_stage2d(Geo g, co
   stage_store<S, true>(g, t, sr, tg, sq, sfx, sfy);
   __syncthreads();
   const Op& o1 = g.ops[SLOT_FIRST];
+  // the dissipation intermediates are formed from the staged operand together with the
+  // flux derivatives and stored once the flux space they overwrite is free
+  DissSlots<S> dsl;
+  const bool two_level = g.dissipation && !g.composite;
+  if (S::kDissRegs && two_level) diss_compute<S>(g, t, cf, tg, sq, lane, c0, dsl);
   double r[NC];
   if (mine) {
     double dfx[NC], dfy[NC];
@@ -531,12 +595,13 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_fwd_stage2d(Geo g, co
   }
   double* stx = sc;
   double* sty = sc + STX;
-  if (g.dissipation) {
+  if (two_level) {
     __syncthreads();   // fluxes consumed
-    if (!g.composite) {
+    if (S::kDissRegs)
+      diss_store<S>(dsl, c0, stx, sty);
+    else
       diss_intermediates<S>(g, t, cf, tg, sq, stx, sty, lane, c0);
-      __syncthreads();
-    }
+    __syncthreads();
   }
   double qn[NC];
   if (mine) {
@@ -702,11 +767,10 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_adj_stage2d(Geo g, co
   }
   double* stx = sc;
   double* sty = sc + STX;
-  if (g.dissipation && !g.composite) {
-    __syncthreads();
-    diss_intermediates<S>(g, t, cf, tg, sq, stx, sty, lane, c0);
-  }
   __syncthreads();
+  // the dissipation intermediates (into the scratch space) and D+ lambda in one phase
+  const bool two_level = g.dissipation && !g.composite;
+  if (two_level) diss_intermediates<S>(g, t, cf, tg, sq, stx, sty, lane, c0);
   const Op& oa = g.ops[SLOT_ADJ];
   double T1[2][NV2];   // D+ lambda along x and y, all components (own ones first)
   double w[2][NC];     // dissipation terms of this thread's components
@@ -720,10 +784,11 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_adj_stage2d(Geo g, co
     for (int d = 0; d < 2; ++d)
 #pragma unroll
       for (int j = 0; j < NC; ++j) T1[d][j] = u[d][j];
-    // with component groups the dissipation intermediates are read before the exchange
-    if (S::CS > 1 && g.dissipation)
-      diss_terms<S>(g, t, cf, tg, sq, stx, sty, tx, ty, gi, gj, c0, -g.diss_amount, w);
   }
+  if (two_level) __syncthreads();   // intermediates complete
+  // with component groups the dissipation terms are formed before the exchange
+  if (S::CS > 1 && mine && g.dissipation)
+    diss_terms<S>(g, t, cf, tg, sq, stx, sty, tx, ty, gi, gj, c0, -g.diss_amount, w);
   if (S::CS > 1) {
     // every component of D+ lambda is needed by the Jacobian products: exchange them
     // through the scratch space (the dissipation intermediates are no longer read)
