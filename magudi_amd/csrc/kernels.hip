@@ -367,23 +367,49 @@ __global__ void __launch_bounds__(kBlock) k_adj_jac(Geo g, const double* __restr
   for (int l = 0; l < NV; ++l) ri[l * g.N + p] = acc[l];
 }
 
-// Viscous adjoint, second-partial block: overwrites components 1..nv-1 of T1_j with
-// sum_i B_ij^T T1_i (src/RhsHelperImpl.f90:509-553).
+// Viscous regions: k_adj_jac's rhs and the second-partial block at one point, T1 and q
+// read once.  The rhs is written, then components 1..nv-1 of every T1_j are overwritten
+// with sum_i B_ij^T T1_i (src/RhsHelperImpl.f90:509-553) -- the Jacobian part in the same
+// operations as k_adj_jac, so the same values.
 template <int ND>
-__global__ void __launch_bounds__(kBlock) k_adj_visc2(Geo g, const double* __restrict__ q, double* __restrict__ T1) {
+__global__ void __launch_bounds__(kBlock) k_adj_jac_visc2(Geo g, const double* __restrict__ q,
+                                                         const double* __restrict__ G,
+                                                         double* __restrict__ T1,
+                                                         double* __restrict__ rhs) {
   const long p = gidx();
   if (p >= g.N) return;
   const long inst = blockIdx.y;
   constexpr int NV = ND + 2, NW = ND + 1;
   const double* qi = q + inst * NV * g.N;
   double* Ti = T1 + inst * ND * NV * g.N;
+  double* ri = rhs + inst * NV * g.N;
   double v[NV];
   load_q<ND>(qi, g.N, p, v);
   const Prim<ND> s = prim<ND>(v, g.gamma);
-  double mu, la, ka;
-  transport(g, s.T, &mu, &la, &ka);
   int ijk[3] = {0, 0, 0};
   unflat<ND>(g, p, ijk);
+  double t1[ND][NV];
+  for (int d = 0; d < ND; ++d)
+    for (int k = 0; k < NV; ++k) t1[d][k] = Ti[(d * NV + k) * g.N + p];
+  double tau[ND * ND], hf[ND], mu, la, ka;
+  visc_point<ND>(g, G + inst * (ND + 1) * ND * g.N, g.N, p, s, tau, hf, &mu, &la, &ka);
+  {
+    double acc[NV];
+    for (int l = 0; l < NV; ++l) acc[l] = 0.0;
+    for (int d = 0; d < ND; ++d) {
+      double m[ND], A[NV * NV], B[NV * NV];
+      for (int i = 0; i < ND; ++i) m[i] = i == d ? metric<ND>(g, d, ijk) : 0.0;
+      inviscid_jacobian<ND>(m, s, g.gamma, A);
+      first_partial_viscous<ND>(m, tau, hf, g.power_law, g.gamma, s, B);
+      for (int k = 0; k < NV * NV; ++k) A[k] = A[k] - B[k];
+      for (int i = 0; i < NV; ++i) {
+        double t = 0.0;
+        for (int k = 0; k < NV; ++k) t += A[i * NV + k] * t1[d][k];
+        acc[i] = acc[i] + t;
+      }
+    }
+    for (int l = 0; l < NV; ++l) ri[l * g.N + p] = acc[l];
+  }
   const double jac = jacobian<ND>(g, ijk);
   double diff[ND][NW];
   for (int j = 0; j < ND; ++j)
@@ -397,7 +423,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_visc2(Geo g, const double* __res
       second_partial_viscous<ND>(s.u, mu, la, ka, jac, m1, m2, B);
       for (int a = 0; a < NW; ++a) {
         double t = 0.0;
-        for (int k = 0; k < NW; ++k) t += B[a * NW + k] * Ti[(i * NV + 1 + k) * g.N + p];
+        for (int k = 0; k < NW; ++k) t += B[a * NW + k] * t1[i][1 + k];
         diff[j][a] = diff[j][a] + t;
       }
     }
@@ -406,18 +432,12 @@ __global__ void __launch_bounds__(kBlock) k_adj_visc2(Geo g, const double* __res
     for (int a = 0; a < NW; ++a) Ti[(j * NV + 1 + a) * g.N + p] = diff[j][a];
 }
 
-// Viscous adjoint chain rule (src/RhsHelperImpl.f90:555-570 with sign -1 and
-// :227-236 with sign +1): t2 = sum_j S_j (components 1..nv-1).
+// Viscous adjoint chain rule at one point (src/RhsHelperImpl.f90:555-570 with sign -1
+// and :227-236 with sign +1): t2 = sum_j S_j (components 1..nv-1), applied to r.
 template <int ND>
-__global__ void __launch_bounds__(kBlock) k_adj_chain(Geo g, const double* __restrict__ q, const double* __restrict__ S,
-                            double* __restrict__ rhs, double sign) {
-  const long p = gidx();
-  if (p >= g.N) return;
-  const long inst = blockIdx.y;
+__device__ __forceinline__ void adj_chain_point(const Geo& g, const double* qi, const double* Si,
+                                                long p, double sign, double* r) {
   constexpr int NV = ND + 2, NW = ND + 1;
-  const double* qi = q + inst * NV * g.N;
-  const double* Si = S + inst * ND * NV * g.N;
-  double* ri = rhs + inst * NV * g.N;
   double v[NV];
   load_q<ND>(qi, g.N, p, v);
   const Prim<ND> s = prim<ND>(v, g.gamma);
@@ -429,12 +449,25 @@ __global__ void __launch_bounds__(kBlock) k_adj_chain(Geo g, const double* __res
   }
   t2[ND] = g.gamma * s.sv * t2[ND];
   for (int i = 0; i < ND; ++i) t2[i] = s.sv * t2[i] - s.u[i] * t2[ND];
-  for (int l = 1; l < NV; ++l)
-    ri[l * g.N + p] = sign > 0 ? ri[l * g.N + p] + t2[l - 1] : ri[l * g.N + p] - t2[l - 1];
+  for (int l = 1; l < NV; ++l) r[l] = sign > 0 ? r[l] + t2[l - 1] : r[l] - t2[l - 1];
   double su = s.u[0] * t2[0];
   for (int i = 1; i < ND; ++i) su = su + s.u[i] * t2[i];
   const double a = s.sv * v[ND + 1] * t2[ND];
-  ri[p] = sign > 0 ? ri[p] - a - su : ri[p] + a + su;
+  r[0] = sign > 0 ? r[0] - a - su : r[0] + a + su;
+}
+
+template <int ND>
+__global__ void __launch_bounds__(kBlock) k_adj_chain(Geo g, const double* __restrict__ q, const double* __restrict__ S,
+                            double* __restrict__ rhs, double sign) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  constexpr int NV = ND + 2;
+  double* ri = rhs + inst * NV * g.N;
+  double r[NV];
+  for (int l = 0; l < NV; ++l) r[l] = ri[l * g.N + p];
+  adj_chain_point<ND>(g, q + inst * NV * g.N, S + inst * ND * NV * g.N, p, sign, r);
+  for (int l = 0; l < NV; ++l) ri[l * g.N + p] = r[l];
 }
 
 // Viscous far-field adjoint penalty, first half (src/RhsHelperImpl.f90:170-210):
@@ -546,7 +579,8 @@ __global__ void __launch_bounds__(kBlock) k_finish(Geo g, int mode, const Patche
                          double* __restrict__ rhs, const double* __restrict__ control,
                          long control_inst_stride, const double* __restrict__ aforce,
                          long aforce_inst_stride, double forcing_factor, const Source* __restrict__ src,
-                         int nsrc, const double* __restrict__ src_amp, long src_inst_stride) {
+                         int nsrc, const double* __restrict__ src_amp, long src_inst_stride,
+                         const double* __restrict__ chain_S) {
   const long p = gidx();
   if (p >= g.N) return;
   const long inst = blockIdx.y;
@@ -558,7 +592,10 @@ __global__ void __launch_bounds__(kBlock) k_finish(Geo g, int mode, const Patche
   unflat<ND>(g, p, ijk);
   const double jac = jacobian<ND>(g, ijk);
   double r[NV];
-  for (int l = 0; l < NV; ++l) r[l] = ri[l * g.N + p] * jac;
+  for (int l = 0; l < NV; ++l) r[l] = ri[l * g.N + p];
+  // adjoint, viscous, nothing else between: k_adj_chain's update (sign -1) of this point
+  if (chain_S) adj_chain_point<ND>(g, qi, chain_S + inst * ND * NV * g.N, p, -1.0, r);
+  for (int l = 0; l < NV; ++l) r[l] = r[l] * jac;
   for (int ip = 0; ip < P->count; ++ip) {
     const DPatch& pt = P->p[ip];
     bool in = true;
@@ -1217,7 +1254,7 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
       k_ff_visc_terms<ND><<<grid, kBlock, 0, st>>>(g, MGD_FORWARD, P, q, nullptr, G);
     k_finish<ND><<<grid, kBlock, 0, st>>>(g, MGD_FORWARD, P, q, nullptr, G, rhs, control,
                                           control_stride, nullptr, 0, 0.0, src, nsrc, src_amp,
-                                          src_inst_stride);
+                                          src_inst_stride, nullptr);
   });
 }
 
@@ -1233,14 +1270,18 @@ void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
       if (!launch_deriv_line(g, d, SLOT_ADJ, lam, (long)NV * g.N, T1 + (long)d * NV * g.N,
                              (long)ND * NV * g.N, NV, 0, st))
         k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, lam, (long)NV * g.N, T1, 0);
-    k_adj_jac<ND><<<grid, kBlock, 0, st>>>(g, q, G, T1, rhs);
+    const bool chain_in_finish = g.viscous && !g.dissipation && !has_farfield;
+    if (g.viscous)
+      k_adj_jac_visc2<ND><<<grid, kBlock, 0, st>>>(g, q, G, T1, rhs);
+    else
+      k_adj_jac<ND><<<grid, kBlock, 0, st>>>(g, q, G, T1, rhs);
     if (g.viscous) {
-      k_adj_visc2<ND><<<grid, kBlock, 0, st>>>(g, q, T1);
       for (int d = 0; d < ND; ++d)
         if (!launch_deriv_line(g, d, SLOT_ADJ, T1 + (long)(d * NV + 1) * g.N, (long)ND * NV * g.N,
                                S + (long)(d * NV + 1) * g.N, (long)ND * NV * g.N, NV - 1, 0, st))
           k_adj_deriv<ND><<<grid, kBlock, 0, st>>>(g, d, T1, (long)ND * NV * g.N, S, 1);
-      k_adj_chain<ND><<<grid, kBlock, 0, st>>>(g, q, S, rhs, -1.0);
+      // with no rhs term between them, k_finish applies the chain rule itself
+      if (!chain_in_finish) k_adj_chain<ND><<<grid, kBlock, 0, st>>>(g, q, S, rhs, -1.0);
     }
     if (g.dissipation) {
       for (int d = 0; d < ND; ++d) {
@@ -1261,7 +1302,8 @@ void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
     if (g.viscous && has_farfield)
       k_ff_visc_terms<ND><<<grid, kBlock, 0, st>>>(g, MGD_ADJOINT, P, q, lam, G);
     k_finish<ND><<<grid, kBlock, 0, st>>>(g, MGD_ADJOINT, P, q, lam, G, rhs, nullptr, 0, aforce,
-                                          aforce_stride, forcing_factor, nullptr, 0, nullptr, 0);
+                                          aforce_stride, forcing_factor, nullptr, 0, nullptr, 0,
+                                          chain_in_finish ? S : nullptr);
   });
 }
 

@@ -54,7 +54,7 @@ def kolmogorov_small(n=16):
     return c
 
 
-def periodic_box(shape, seed=3):
+def periodic_box(shape, seed=3, dissipation=True):
     """Viscous, fully periodic 3-D box of the given (nx, ny, nz) with a different grid
     spacing per axis: sizes whose x / y extents are multiples of 32 / 16 take the
     one-pass periodic gradient kernel, with partial and multiple z chunks."""
@@ -71,7 +71,7 @@ def periodic_box(shape, seed=3):
         name=f"box{nx}x{ny}x{nz}", nd=3, n=(nx, ny, nz),
         coords=np.stack([X.ravel(), Y.ravel(), Z.ravel()]),
         periodicity=(cases.PLANE,) * 3, periodic_length=(1.0, 1.0, 1.0),
-        viscous=True, dissipation=True, dissipation_amount=0.01, reynolds=500.0,
+        viscous=True, dissipation=dissipation, dissipation_amount=0.01, reynolds=500.0,
         prandtl=0.7, power_law=0.666, bulk_ratio=0.6,
         patches=[cases.Patch("forcing", cases.KOLMOGOROV, 0, (1, -1, 1, -1, 1, -1),
                              (2.0, 0.01, 0, 0))],
@@ -88,6 +88,9 @@ CASES = {
     "walled41_viscous": lambda: cases.walled_monopole(n=41, viscous=True),
     "box32x16x12": lambda: periodic_box((32, 16, 12)),
     "box64x32x80": lambda: periodic_box((64, 32, 80)),
+    # no dissipation and no far field: the adjoint's chain rule runs inside k_finish
+    "kolmogorov16_nodiss": lambda: cases.kolmogorov3d(n=16, nsteps=1),
+    "box32x16x12_nodiss": lambda: periodic_box((32, 16, 12), dissipation=False),
 }
 
 
