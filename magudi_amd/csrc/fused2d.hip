@@ -40,7 +40,10 @@ struct Shape {
   static constexpr int NP = TX * TY, NT = NP * CS, NC = NV2 / CS;
   static_assert(NP % 64 == 0 && NV2 % CS == 0, "component groups are whole waves");
   static constexpr int EX = TX + 2 * HX, EY = TY + 2 * HY;
-  static constexpr int kStageIters = (EX * EY + NT - 1) / NT;   // staged points per thread
+  // staged points: the tile rows over the extended width, plus the halo rows over the tile
+  // columns (the halo corners are never read)
+  static constexpr int kStaged = TY * EX + 2 * HY * TX;
+  static constexpr int kStageIters = (kStaged + NT - 1) / NT;   // staged points per thread
   static constexpr int SQ = NV2 * EY * EX;
   static constexpr int SFX = NV2 * TY * EX, SFY = NV2 * EY * TX;
   static constexpr int STX = NV2 * TY * (TX + 4), STY = NV2 * (TY + 4) * TX;
@@ -164,21 +167,31 @@ struct StageRegs {
 };
 
 // Staging slot `it` of this thread: extended-layout indices (ex, ey), the grid point
-// (gi, gj) and whether the slot holds a staged point (the corners and the columns /
-// rows beyond a partial tile are skipped).
+// (gi, gj) and whether the slot holds a staged point.  Slots enumerate the tile rows
+// over the extended width first, then the halo rows over the tile columns (the halo
+// rows below a partial tile follow its last row); slots beyond a partial tile, and
+// halo points outside a non-periodic domain, hold none.
 template <class S>
 __device__ __forceinline__ bool stage_slot(const Geo& g, const TileGeo& t, int it, int* ex,
                                            int* ey, int* gi, int* gj) {
-  constexpr int EX = S::EX, EY = S::EY;
+  constexpr int TX = S::TX, TY = S::TY, EX = S::EX;
   const int e = threadIdx.x + it * S::NT;
-  *ex = e % EX;
-  *ey = e / EX;
+  bool in;
+  if (e < TY * EX) {
+    const int r = e / EX;
+    *ex = e % EX;
+    *ey = HY + r;
+    in = r < t.sy && *ex < t.sx + 2 * HX;
+  } else {
+    const int e2 = e - TY * EX;
+    const int hr = e2 / TX, c = e2 % TX;
+    *ex = HX + c;
+    *ey = hr < HY ? hr : HY + t.sy + (hr - HY);
+    in = e2 < 2 * HY * TX && c < t.sx;
+  }
   *gi = ext_index(g.dir[0], t.ex0 + *ex);
   *gj = ext_index(g.dir[1], t.ey0 + *ey);
-  const bool rowx = *ey >= HY && *ey < HY + t.sy;
-  const bool coly = *ex >= HX && *ex < HX + t.sx;
-  return e < EX * EY && *ex < t.sx + 2 * HX && *ey < t.sy + 2 * HY && *gi >= 0 && *gj >= 0 &&
-         (rowx || coly);
+  return in && *gi >= 0 && *gj >= 0;
 }
 
 template <class S>

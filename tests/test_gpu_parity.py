@@ -91,6 +91,8 @@ CASES = {
     # no dissipation and no far field: the adjoint's chain rule runs inside k_finish
     "kolmogorov16_nodiss": lambda: cases.kolmogorov3d(n=16, nsteps=1),
     "box32x16x12_nodiss": lambda: periodic_box((32, 16, 12), dissipation=False),
+    # rows longer than one 256-point segment; z too short for the line march
+    "box512x16x9": lambda: periodic_box((512, 16, 9)),
 }
 
 
