@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=20)
     ap.add_argument("--no-extras", action="store_true",
                     help="headline only (no batched / plain / CPU legs)")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=64)
     ap.add_argument("--c4-n", type=int, default=256,
                     help="grid of the 3-D viscous Kolmogorov rhs leg (0: skip)")
     ap.add_argument("--viscous3d-only", action="store_true",
@@ -84,20 +84,23 @@ def run_hot_path(R, S, steps):
 
 
 def cpu_baseline(n, nsteps, save_interval):
-    """CPU oracle (plain C, one thread) on a bounded sample of the same workload."""
+    """CPU oracle (plain C; its per-point and per-pencil loops on every OpenMP thread the
+    host gives, OMP_NUM_THREADS) on a bounded sample of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
     from magudi_amd import cases
     c = cases.acoustic_monopole(n=n, nsteps=nsteps, save_interval=save_interval)
     O = ob.Region(c)
     q0 = c.initial_state()
+    threads = ob.lib().orc_threads()
     t0 = time.perf_counter()
     O.run_adjoint(q0, c.dt, nsteps, save_interval)
     dt = time.perf_counter() - t0
-    return {"value": c.npoints * nsteps / dt, "unit": "grid-pt-steps/s", "cores": 1,
+    return {"value": c.npoints * nsteps / dt, "unit": "grid-pt-steps/s", "cores": threads,
             "kind": "port",
             "sample": f"{n}x{n} monopole forward+adjoint, {nsteps} RK4 steps, snapshots every "
-                      f"{save_interval} (oracle/ C restatement, 1 thread, {dt:.1f} s)"}
+                      f"{save_interval} (oracle/ C restatement, {threads} OpenMP threads, "
+                      f"{dt:.1f} s wall)"}
 
 
 def viscous3d_leg(n):

@@ -119,6 +119,12 @@ double orc_run_adjoint_ic(orc_region* r, const double* q0, double t0, double dt,
                           double* gradient, double* lambda0, double* J_out, double* qT_out);
 
 /* CNSHelper pointwise routines (src/CNSHelperImpl.f90) exposed for unit tests. */
+/* OpenMP threads of the oracle's per-point and per-pencil loops (1 when built without
+ * OpenMP).  Results do not depend on the count: every value is computed as in a serial
+ * run and the quadrature sums stay serial. */
+int orc_threads(void);
+void orc_set_threads(int n);
+
 void orc_inviscid_jacobian(int nd, const double* q, const double* metrics, double gamma,
                            double* A);     /* nv x nv column-major */
 void orc_incoming_jacobian(int nd, const double* q, const double* metrics, double gamma,

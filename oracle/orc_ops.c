@@ -8,6 +8,9 @@
  *   applyOperatorNorm / NormInverse :838-1109
  * and the periodic ghost fill of src/MPIHelperImpl.f90:113-389 (single rank).
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -333,24 +336,31 @@ void orc_op_apply(const orc_op* op, int dir, const int n[3], int periodicity, do
   const long stride = dir == 0 ? 1 : (dir == 1 ? n[0] : (long)n[0] * n[1]);
   const int periodic = periodicity != ORC_NONE;
   const int off = periodicity == ORC_OVERLAP ? 1 : 0;
-  double* in = malloc(sizeof(double) * len);
-  double* out = malloc(sizeof(double) * len);
   const int na = dir == 0 ? n[1] : n[0];
   const int nb = dir == 2 ? n[1] : n[2];
-  for (int l = 0; l < ncomp; ++l)
-    for (int b = 0; b < nb; ++b)
-      for (int a = 0; a < na; ++a) {
-        long base;
-        if (dir == 0) base = (long)n[0] * (a + (long)n[1] * b);
-        else if (dir == 1) base = a + (long)n[0] * n[1] * b;
-        else base = a + (long)n[0] * b;
-        double* xl = x + l * N + base;
-        for (int i = 0; i < len; ++i) in[i] = xl[i * stride];
-        apply_pencil(op, len, periodic, off, in, out);
-        for (int i = 0; i < len; ++i) xl[i * stride] = out[i];
-      }
-  free(in);
-  free(out);
+  const long npencil = (long)ncomp * nb * na;
+  /* pencils are independent: split over the OpenMP threads, one buffer pair each */
+#pragma omp parallel
+  {
+    double* in = malloc(sizeof(double) * len);
+    double* out = malloc(sizeof(double) * len);
+#pragma omp for schedule(static)
+    for (long w = 0; w < npencil; ++w) {
+      const int l = (int)(w / ((long)nb * na));
+      const int b = (int)((w / na) % nb);
+      const int a = (int)(w % na);
+      long base;
+      if (dir == 0) base = (long)n[0] * (a + (long)n[1] * b);
+      else if (dir == 1) base = a + (long)n[0] * n[1] * b;
+      else base = a + (long)n[0] * b;
+      double* xl = x + l * N + base;
+      for (int i = 0; i < len; ++i) in[i] = xl[i * stride];
+      apply_pencil(op, len, periodic, off, in, out);
+      for (int i = 0; i < len; ++i) xl[i * stride] = out[i];
+    }
+    free(in);
+    free(out);
+  }
 }
 
 void orc_op_apply_norm(const orc_op* op, int dir, const int n[3], int periodic, double* x,
@@ -359,6 +369,7 @@ void orc_op_apply_norm(const orc_op* op, int dir, const int n[3], int periodic, 
   if (periodic) return;
   const long N = (long)n[0] * n[1] * n[2];
   for (int l = 0; l < ncomp; ++l)
+#pragma omp parallel for schedule(static)
     for (long p = 0; p < N; ++p) {
       int idx = dir == 0 ? (int)(p % n[0]) : (dir == 1 ? (int)((p / n[0]) % n[1])
                                                         : (int)(p / ((long)n[0] * n[1])));
@@ -370,4 +381,20 @@ void orc_op_apply_norm(const orc_op* op, int dir, const int n[3], int periodic, 
       if (inverse) x[l * N + p] = x[l * N + p] / op->norm[w];
       else x[l * N + p] = op->norm[w] * x[l * N + p];
     }
+}
+
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
 }

@@ -264,6 +264,7 @@ static void update_state(orc_region* r, const double* q) {
   const int nd = r->d.nd;
   const long N = r->N;
   const double g = r->d.gamma;
+  #pragma omp parallel for schedule(static)
   for (long p = 0; p < N; ++p) {
     const double sv = 1.0 / q[p];
     r->sv[p] = sv;
@@ -277,6 +278,7 @@ static void update_state(orc_region* r, const double* q) {
     r->T[p] = g * r->p[p] / (g - 1.0) * sv;
   }
   if (!r->d.viscous) return;
+  #pragma omp parallel for schedule(static)
   for (long p = 0; p < N; ++p) {
     if (r->d.power_law <= 0.0) {
       r->mu[p] = r->re_inv;
@@ -294,11 +296,13 @@ static void update_state(orc_region* r, const double* q) {
     memcpy(tmp, r->vel, sizeof(double) * N * nd);
     orc_op_apply(&r->first[j], j, r->d.n, r->d.periodicity[j], tmp, nd);
     for (int i = 0; i < nd; ++i)
+      #pragma omp parallel for schedule(static)
       for (long p = 0; p < N; ++p)
         r->tau[(i * nd + j) * N + p] =
             r->jacobian[p] * r->metrics[(j * nd + j) * N + p] * tmp[i * N + p];
   }
   /* computeStressTensor, in-place form (src/CNSHelperImpl.f90:406-455) */
+  #pragma omp parallel for schedule(static)
   for (long p = 0; p < N; ++p) {
     double* t = r->tau;
     const double mu = r->mu[p], la = r->lam[p];
@@ -327,6 +331,7 @@ static void update_state(orc_region* r, const double* q) {
   for (int j = 0; j < nd; ++j) {
     memcpy(tmp, r->T, sizeof(double) * N);
     orc_op_apply(&r->first[j], j, r->d.n, r->d.periodicity[j], tmp, 1);
+    #pragma omp parallel for schedule(static)
     for (long p = 0; p < N; ++p) {
       const double gT = r->jacobian[p] * r->metrics[(j * nd + j) * N + p] * tmp[p];
       r->hf[j * N + p] = -r->kap[p] * gT;
@@ -361,10 +366,12 @@ static void add_dissipation(orc_region* r, int mode, const double* x, double* rh
     orc_op_apply(&r->diss[d], d, r->d.n, r->d.periodicity[d], t, nv);
     if (!r->d.composite_dissipation) {
       for (int j = 0; j < nv; ++j)
+        #pragma omp parallel for schedule(static)
         for (long p = 0; p < N; ++p) t[j * N + p] = -r->arc[d * N + p] * t[j * N + p];
       orc_op_apply(&r->dissT[d], d, r->d.n, r->d.periodicity[d], t, nv);
       orc_op_apply_norm(&r->first[d], d, r->d.n, is_periodic(r, d), t, nv, 1);
     }
+    #pragma omp parallel for schedule(static)
     for (long k = 0; k < N * nv; ++k) rhs[k] = rhs[k] + amount * t[k];
   }
 }
@@ -374,7 +381,9 @@ static void rhs_forward(orc_region* r, const double* q, double* rhs) {
   const int nd = r->d.nd, nv = r->nv;
   const long N = r->N;
   double* F = r->w1;   /* transformed fluxes (N x nv x nd) */
+  #pragma omp parallel for schedule(static)
   for (long k = 0; k < N * nv; ++k) rhs[k] = 0.0;
+  #pragma omp parallel for schedule(static)
   for (long p = 0; p < N; ++p) {
     const double pr = r->p[p];
     double f[3][5];
@@ -401,6 +410,7 @@ static void rhs_forward(orc_region* r, const double* q, double* rhs) {
   }
   for (int d = 0; d < nd; ++d)
     orc_op_apply(&r->first[d], d, r->d.n, r->d.periodicity[d], F + (long)d * nv * N, nv);
+  #pragma omp parallel for schedule(static)
   for (long k = 0; k < N * nv; ++k) {
     double s = F[k];
     for (int d = 1; d < nd; ++d) s = s + F[(long)d * nv * N + k];
@@ -416,14 +426,18 @@ static void viscous_chain(orc_region* r, const double* q, double* t2, double sig
   const int nd = r->d.nd;
   const long N = r->N;
   const double g = r->d.gamma;
+  #pragma omp parallel for schedule(static)
   for (long p = 0; p < N; ++p) t2[nd * N + p] = g * r->sv[p] * t2[nd * N + p];
   for (int i = 0; i < nd; ++i)
+    #pragma omp parallel for schedule(static)
     for (long p = 0; p < N; ++p)
       t2[i * N + p] = r->sv[p] * t2[i * N + p] - r->vel[i * N + p] * t2[nd * N + p];
   for (int l = 1; l <= nd + 1; ++l)
+    #pragma omp parallel for schedule(static)
     for (long p = 0; p < N; ++p)
       rhs[l * N + p] = sign > 0 ? rhs[l * N + p] + t2[(l - 1) * N + p]
                                 : rhs[l * N + p] - t2[(l - 1) * N + p];
+  #pragma omp parallel for schedule(static)
   for (long p = 0; p < N; ++p) {
     double s = r->vel[p] * t2[p];
     for (int i = 1; i < nd; ++i) s = s + r->vel[i * N + p] * t2[i * N + p];
@@ -439,11 +453,13 @@ static void rhs_adjoint(orc_region* r, const double* q, const double* lam, doubl
   const long N = r->N;
   const double g = r->d.gamma;
   double* t1 = r->w1;   /* N x nv x nd */
+  #pragma omp parallel for schedule(static)
   for (long k = 0; k < N * nv; ++k) rhs[k] = 0.0;
   for (int d = 0; d < nd; ++d) {
     memcpy(t1 + (long)d * nv * N, lam, sizeof(double) * N * nv);
     orc_op_apply(&r->adjfirst[d], d, r->d.n, r->d.periodicity[d], t1 + (long)d * nv * N, nv);
   }
+  #pragma omp parallel for schedule(static)
   for (long p = 0; p < N; ++p) {
     double qp[5], u[3], tau[9], hf[3], A[25], B[25];
     for (int l = 0; l < nv; ++l) qp[l] = q[l * N + p];
@@ -470,6 +486,7 @@ static void rhs_adjoint(orc_region* r, const double* q, const double* lam, doubl
   if (r->d.viscous) {
     /* second-partial viscous terms (src/RhsHelperImpl.f90:509-570) */
     const int nw = nv - 1;
+    #pragma omp parallel for schedule(static)
     for (long p = 0; p < N; ++p) {
       double u[3], diff[3][4], B[16];
       for (int i = 0; i < nd; ++i) u[i] = r->vel[i * N + p];
@@ -497,6 +514,7 @@ static void rhs_adjoint(orc_region* r, const double* q, const double* lam, doubl
                    nw);
     double* t2 = r->w2;
     for (int a = 0; a < nw; ++a)
+      #pragma omp parallel for schedule(static)
       for (long p = 0; p < N; ++p) {
         double s = t1[(1 + a) * N + p];
         for (int j = 1; j < nd; ++j) s = s + t1[((long)j * nv + 1 + a) * N + p];
@@ -539,6 +557,7 @@ static void rhs_adjoint(orc_region* r, const double* q, const double* lam, doubl
         orc_op_apply(&r->adjfirst[l], l, r->d.n, r->d.periodicity[l], tt + (long)l * nw * N, nw);
       double* t2 = r->w2;
       for (int a = 0; a < nw; ++a)
+        #pragma omp parallel for schedule(static)
         for (long p = 0; p < N; ++p) {
           double s = tt[a * N + p];
           for (int l = 1; l < nd; ++l) s = s + tt[((long)l * nw + a) * N + p];
@@ -554,6 +573,7 @@ int orc_region_finalize(orc_region* r) {
   const long N = r->N;
   const double g = r->d.gamma;
   /* mean pressure from the target state (src/AcousticNoiseImpl.f90:56-71) */
+  #pragma omp parallel for schedule(static)
   for (long p = 0; p < N; ++p) {
     double s = 0.0;
     for (int i = 0; i < nd; ++i) s += r->target[(i + 1) * N + p] * r->target[(i + 1) * N + p];
@@ -569,6 +589,7 @@ int orc_region_finalize(orc_region* r) {
     coordinate_derivatives(r, dir, cd);
     const int len = r->d.n[dir];
     double* arcl = zalloc(N);
+    #pragma omp parallel for schedule(static)
     for (long p = 0; p < N; ++p) {
       double s = 0.0;
       for (int l = 0; l < nd; ++l) s += cd[l * N + p] * cd[l * N + p];
@@ -631,6 +652,7 @@ int orc_region_finalize(orc_region* r) {
       const double pi = 4.0 * atan(1.0);
       const int wn = P->pd.p[0] > 0 ? (int)P->pd.p[0] : 0;
       P->kforce = zalloc(P->np);
+      #pragma omp parallel for schedule(static)
       for (int k = 0; k < P->np; ++k) {
         const long p = patch_grid_index(r, P, k);
         P->kforce[k] = P->pd.p[1] * sin(2.0 * pi * wn * r->coords[1 * N + p]);
@@ -643,6 +665,7 @@ int orc_region_finalize(orc_region* r) {
       orc_patch* P = &r->patches[ip];
       if (P->pd.type != ORC_FAR_FIELD) continue;
       P->tvflux = zalloc((long)P->np * nv * nd);
+      #pragma omp parallel for schedule(static)
       for (int k = 0; k < P->np; ++k) {
         double fv[15];
         viscous_flux_point(r, patch_grid_index(r, P, k), fv);
@@ -678,6 +701,7 @@ int orc_compute_rhs(orc_region* r, int mode, const double* q, const double* lamb
   if (mode == ORC_FORWARD) rhs_forward(r, q, rhs);
   else rhs_adjoint(r, q, lambda, rhs);
   for (int j = 0; j < nv; ++j)
+    #pragma omp parallel for schedule(static)
     for (long p = 0; p < N; ++p) rhs[j * N + p] = rhs[j * N + p] * r->jacobian[p];
   for (int ip = 0; ip < r->npatches; ++ip) {
     const orc_patch* P = &r->patches[ip];
@@ -685,6 +709,7 @@ int orc_compute_rhs(orc_region* r, int mode, const double* q, const double* lamb
       case ORC_FAR_FIELD: {
         /* addFarFieldPenalty (src/FarFieldPatchImpl.f90:93-286) */
         const int dir = abs(P->pd.normal_dir) - 1;
+        #pragma omp parallel for schedule(static)
         for (int k = 0; k < P->np; ++k) {
           const long p = patch_grid_index(r, P, k);
           double qt[5], m[3], A[25];
@@ -739,6 +764,7 @@ int orc_compute_rhs(orc_region* r, int mode, const double* q, const double* lamb
         /* addImpenetrableWallPenalty (src/ImpenetrableWallImpl.f90:60-207) */
         const int dir = abs(P->pd.normal_dir) - 1;
         const double g = r->d.gamma;
+        #pragma omp parallel for schedule(static)
         for (int k = 0; k < P->np; ++k) {
           const long p = patch_grid_index(r, P, k);
           double qp[5], m[3];
@@ -796,6 +822,7 @@ int orc_compute_rhs(orc_region* r, int mode, const double* q, const double* lamb
       case ORC_SPONGE: {
         /* addDamping (src/SpongePatchImpl.f90:65-166) */
         for (int l = 0; l < nv; ++l)
+          #pragma omp parallel for schedule(static)
           for (int k = 0; k < P->np; ++k) {
             const long p = patch_grid_index(r, P, k);
             if (mode == ORC_FORWARD)
@@ -808,6 +835,7 @@ int orc_compute_rhs(orc_region* r, int mode, const double* q, const double* lamb
       case ORC_ACTUATOR: {
         /* updateActuatorPatch (src/ActuatorPatchImpl.f90:108-181): energy only */
         if (mode != ORC_FORWARD || !control_forcing) break;
+        #pragma omp parallel for schedule(static)
         for (int k = 0; k < P->np; ++k) {
           const long p = patch_grid_index(r, P, k);
           rhs[(nd + 1) * N + p] = rhs[(nd + 1) * N + p] + r->ctrl_moll[p] * control_forcing[k];
@@ -818,6 +846,7 @@ int orc_compute_rhs(orc_region* r, int mode, const double* q, const double* lamb
         /* addAdjointForcing (src/CostTargetPatchImpl.f90:72-136) */
         if (mode == ORC_FORWARD || !adjoint_forcing) break;
         for (int l = 0; l < nv; ++l)
+          #pragma omp parallel for schedule(static)
           for (int k = 0; k < P->np; ++k) {
             const long p = patch_grid_index(r, P, k);
             rhs[l * N + p] = rhs[l * N + p] + forcing_factor * adjoint_forcing[(long)l * P->np + k];
@@ -825,6 +854,7 @@ int orc_compute_rhs(orc_region* r, int mode, const double* q, const double* lamb
         break;
       }
       case ORC_KOLMOGOROV: {
+        #pragma omp parallel for schedule(static)
         for (int k = 0; k < P->np; ++k) {
           const long p = patch_grid_index(r, P, k);
           if (mode == ORC_FORWARD)
@@ -841,6 +871,7 @@ int orc_compute_rhs(orc_region* r, int mode, const double* q, const double* lamb
     for (int s = 0; s < r->nsources; ++s) {
       const orc_source* S = &r->sources[s];
       const double a = S->amplitude * cos(S->omega * time + S->phase);
+      #pragma omp parallel for schedule(static)
       for (long p = 0; p < N; ++p) {
         double rr = 0.0;
         for (int i = 0; i < nd; ++i) {
@@ -861,6 +892,7 @@ static double quadrature(const orc_region* r, int type, const double* integrand_
   for (int ip = 0; ip < r->npatches; ++ip) {
     const orc_patch* P = &r->patches[ip];
     if (P->pd.type != type) continue;
+    #pragma omp parallel for schedule(static)
     for (int k = 0; k < P->np; ++k) mask[patch_grid_index(r, P, k)] = 1.0;
   }
   double s = 0.0;
@@ -873,6 +905,7 @@ double orc_functional(orc_region* r, const double* q) {
   update_state(r, q);
   const long N = r->N;
   double* f = zalloc(N);
+  #pragma omp parallel for schedule(static)
   for (long p = 0; p < N; ++p) {
     const double F = r->p[p] - r->mean_p[p];
     f[p] = F * F * r->tgt_moll[p];
@@ -889,6 +922,7 @@ int orc_adjoint_forcing(orc_region* r, const double* q, double* forcing) {
   const int nd = r->d.nd;
   const long N = r->N;
   const orc_patch* P = &r->patches[ip];
+  #pragma omp parallel for schedule(static)
   for (int k = 0; k < P->np; ++k) {
     const long p = patch_grid_index(r, P, k);
     const double F = -2.0 * r->tgt_moll[p] * 1.0 * (r->d.gamma - 1.0) * (r->p[p] - r->mean_p[p]);
@@ -908,6 +942,7 @@ int orc_gradient(orc_region* r, const double* lambda, double* g) {
   if (ip < 0) return -1;
   const orc_patch* P = &r->patches[ip];
   const int nd = r->d.nd;
+  #pragma omp parallel for schedule(static)
   for (int k = 0; k < P->np; ++k) {
     const long p = patch_grid_index(r, P, k);
     g[k] = lambda[(nd + 1) * r->N + p] * (r->ctrl_moll[p] * 1.0);
@@ -919,6 +954,7 @@ double orc_sensitivity(orc_region* r, const double* lambda) {
   const long N = r->N;
   const int nd = r->d.nd;
   double* f = zalloc(N);
+  #pragma omp parallel for schedule(static)
   for (long p = 0; p < N; ++p) {
     const double F = lambda[(nd + 1) * N + p] * r->ctrl_moll[p] * 1.0;
     f[p] = F * F;

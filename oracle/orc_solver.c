@@ -40,6 +40,7 @@ static void substep_forward(rk4* k, double* q, double* time, double dt, int stag
     case 1:
       memcpy(k->buf1, q, sizeof(double) * M);
       orc_compute_rhs(k->r, ORC_FORWARD, q, NULL, *time, ctrl, NULL, 0.0, rhs);
+      #pragma omp parallel for schedule(static)
       for (long i = 0; i < M; ++i) {
         k->buf2[i] = q[i] + dt * rhs[i] / 6.0;
         q[i] = k->buf1[i] + dt * rhs[i] / 2.0;
@@ -48,6 +49,7 @@ static void substep_forward(rk4* k, double* q, double* time, double dt, int stag
     case 2:
       *time = *time + dt / 2.0;
       orc_compute_rhs(k->r, ORC_FORWARD, q, NULL, *time, ctrl, NULL, 0.0, rhs);
+      #pragma omp parallel for schedule(static)
       for (long i = 0; i < M; ++i) {
         k->buf2[i] = k->buf2[i] + dt * rhs[i] / 3.0;
         q[i] = k->buf1[i] + dt * rhs[i] / 2.0;
@@ -55,6 +57,7 @@ static void substep_forward(rk4* k, double* q, double* time, double dt, int stag
       break;
     case 3:
       orc_compute_rhs(k->r, ORC_FORWARD, q, NULL, *time, ctrl, NULL, 0.0, rhs);
+      #pragma omp parallel for schedule(static)
       for (long i = 0; i < M; ++i) {
         k->buf2[i] = k->buf2[i] + dt * rhs[i] / 3.0;
         q[i] = k->buf1[i] + dt * rhs[i];
@@ -63,6 +66,7 @@ static void substep_forward(rk4* k, double* q, double* time, double dt, int stag
     case 4:
       *time = *time + dt / 2.0;
       orc_compute_rhs(k->r, ORC_FORWARD, q, NULL, *time, ctrl, NULL, 0.0, rhs);
+      #pragma omp parallel for schedule(static)
       for (long i = 0; i < M; ++i) q[i] = k->buf2[i] + dt * rhs[i] / 6.0;
       break;
   }
@@ -77,6 +81,7 @@ static void substep_adjoint(rk4* k, const double* q, double* lam, double dt, int
     case 4:
       memcpy(k->buf1, lam, sizeof(double) * M);
       orc_compute_rhs(k->r, ORC_ADJOINT, q, lam, 0.0, NULL, aforcing, 2.0, rhs);
+      #pragma omp parallel for schedule(static)
       for (long i = 0; i < M; ++i) {
         k->buf2[i] = lam[i] - dt * rhs[i] / 6.0;
         lam[i] = k->buf1[i] - dt * rhs[i] / 2.0;
@@ -84,6 +89,7 @@ static void substep_adjoint(rk4* k, const double* q, double* lam, double dt, int
       break;
     case 3:
       orc_compute_rhs(k->r, ORC_ADJOINT, q, lam, 0.0, NULL, aforcing, 1.0, rhs);
+      #pragma omp parallel for schedule(static)
       for (long i = 0; i < M; ++i) {
         k->buf2[i] = k->buf2[i] - dt * rhs[i] / 3.0;
         lam[i] = k->buf1[i] - dt * rhs[i] / 2.0;
@@ -91,6 +97,7 @@ static void substep_adjoint(rk4* k, const double* q, double* lam, double dt, int
       break;
     case 2:
       orc_compute_rhs(k->r, ORC_ADJOINT, q, lam, 0.0, NULL, aforcing, 0.5, rhs);
+      #pragma omp parallel for schedule(static)
       for (long i = 0; i < M; ++i) {
         k->buf2[i] = k->buf2[i] - dt * rhs[i] / 3.0;
         lam[i] = k->buf1[i] - dt * rhs[i];
@@ -98,6 +105,7 @@ static void substep_adjoint(rk4* k, const double* q, double* lam, double dt, int
       break;
     case 1:
       orc_compute_rhs(k->r, ORC_ADJOINT, q, lam, 0.0, NULL, aforcing, 1.0, rhs);
+      #pragma omp parallel for schedule(static)
       for (long i = 0; i < M; ++i) lam[i] = k->buf2[i] - dt * rhs[i] / 6.0;
       break;
   }
@@ -183,6 +191,7 @@ double orc_run_adjoint_ic(orc_region* r, const double* q0, double t0, double dt,
         const long p = (P->lo[0] + a) + (long)r->d.n[0] * ((P->lo[1] + b) + (long)r->d.n[1] * (P->lo[2] + e));
         rhs[l * r->N + p] = rhs[l * r->N + p] + factor * aforce[(long)l * P->np + c];
       }
+    #pragma omp parallel for schedule(static)
     for (long i = 0; i < M; ++i) lam[i] = lam[i] + rhs[i];
     free(rhs);
   }

@@ -69,6 +69,9 @@ def lib():
             subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
         L = C.CDLL(LIB)
         dp = C.POINTER(C.c_double)
+        L.orc_threads.restype = C.c_int
+        L.orc_threads.argtypes = []
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_op_setup.argtypes = [C.POINTER(OrcOp), C.c_int, C.c_int]
         L.orc_op_adjoint.argtypes = [C.POINTER(OrcOp), C.POINTER(OrcOp)]
         L.orc_op_apply.argtypes = [C.POINTER(OrcOp), C.c_int, C.POINTER(C.c_int), C.c_int, dp, C.c_int]

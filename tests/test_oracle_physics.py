@@ -209,3 +209,33 @@ def test_initial_condition_gradient():
         J1, _ = r.run_forward(q0 + h * dq, c.dt, c.nsteps)
         errs.append(abs((J1 - J0) / h - pred) / abs(pred))
     assert errs[-1] < 1e-4 and errs[-1] < errs[0] * 0.05, errs
+
+
+def test_oracle_threads_do_not_change_results():
+    """The oracle's OpenMP loops (per point / per pencil) give the serial values bit for
+    bit, whatever the thread count: forward and adjoint rhs of a viscous 3-D periodic
+    case and a whole forward + adjoint run of the monopole."""
+    from magudi_amd import cases
+    L = ob.lib()
+    saved = L.orc_threads()
+    try:
+        c = cases.kolmogorov3d(n=12, nsteps=1)
+        c.dissipation = True
+        rng = np.random.default_rng(4)
+        q = c.initial_state() + 1e-3 * rng.standard_normal((c.nv, c.npoints))
+        lam = rng.standard_normal((c.nv, c.npoints))
+        m = cases.acoustic_monopole(n=41, nsteps=6, save_interval=3)
+        m.dt = 0.2
+        outs = []
+        for nt in (1, 3):
+            L.orc_set_threads(nt)
+            O = ob.Region(c)
+            Om = ob.Region(m)
+            outs.append((O.rhs(1, q, time=0.3), O.rhs(-1, q, lam, factor=0.5),
+                         Om.run_adjoint(m.initial_state(), m.dt, m.nsteps, m.save_interval)))
+        (f1, a1, r1), (f2, a2, r2) = outs
+        assert np.array_equal(f1, f2) and np.array_equal(a1, a2)
+        assert r1[0] == r2[0] and r1[3] == r2[3]
+        assert np.array_equal(r1[1], r2[1]) and np.array_equal(r1[2], r2[2])
+    finally:
+        L.orc_set_threads(saved)
