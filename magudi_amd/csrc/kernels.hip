@@ -969,7 +969,7 @@ __global__ void __launch_bounds__(BX * BY) k_fluxdiv3d_xy(Geo g, const double* _
                                                            double* __restrict__ F,
                                                            double* __restrict__ rhs) {
   constexpr int NV = 5, EXW = BX + 8, EYW = BY + 8;
-  static_assert(8 * (BX + BY) == BX * BY, "one halo point per thread");
+  static_assert(8 * (BX + BY) <= BX * BY, "at most one halo point per thread");
   __shared__ double sfx[NV][BY][EXW];
   __shared__ double sfy[NV][EYW][BX];
   const int ntx = g.n[0] / BX;
@@ -1019,10 +1019,12 @@ __global__ void __launch_bounds__(BX * BY) k_fluxdiv3d_xy(Geo g, const double* _
     }
   }
   {
-    // halo: threads [0, 8 BY) take the x halo columns, the rest the y halo rows
+    // halo: threads [0, 8 BY) take the x halo columns, [8 BY, 8 BY + 8 BX) the y halo rows
     const int e = threadIdx.x;
     double f[3][NV];
-    if (e < 8 * BY) {
+    if (e >= 8 * (BX + BY)) {
+      // no halo point
+    } else if (e < 8 * BY) {
       const int hx = e % 8, hy = e / 8;
       const int ex = hx < 4 ? hx : BX + hx;
       const bool wx[3] = {true, false, false};
@@ -1199,11 +1201,11 @@ static bool grad3d_applies(const Geo& g) {
 }
 
 // x / y flux divergence kernel of 3-D grids periodic in x and y with sizes that are
-// multiples of its 16 x 16 tile.
-constexpr int kFD3 = 16;
+// multiples of its 32 x 16 tile.
+constexpr int kFD3X = 32, kFD3Y = 16;
 static bool fluxdiv3d_applies(const Geo& g) {
   return g.nd == 3 && g.nv == 5 && g.dir[0].periodic && g.dir[1].periodic &&
-         g.n[0] % kFD3 == 0 && g.n[1] % kFD3 == 0;
+         g.n[0] % kFD3X == 0 && g.n[1] % kFD3Y == 0;
 }
 
 template <int ND>
@@ -1226,8 +1228,8 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
   MGD_DISPATCH(g.nd, {
     if (g.viscous) launch_gradients<ND>(g, q, w, G, grid, st);
     if (F && ND == 3 && fluxdiv3d_applies(g)) {
-      const dim3 gg((g.n[0] / kFD3) * (g.n[1] / kFD3), g.n[2], g.ninst);
-      k_fluxdiv3d_xy<kFD3, kFD3><<<gg, kFD3 * kFD3, 0, st>>>(g, q, g.viscous ? G : nullptr, F, rhs);
+      const dim3 gg((g.n[0] / kFD3X) * (g.n[1] / kFD3Y), g.n[2], g.ninst);
+      k_fluxdiv3d_xy<kFD3X, kFD3Y><<<gg, kFD3X * kFD3Y, 0, st>>>(g, q, g.viscous ? G : nullptr, F, rhs);
       if (!launch_deriv_line(g, ND - 1, SLOT_FIRST, F + (long)(ND - 1) * g.nv * g.N,
                              (long)ND * g.nv * g.N, rhs, (long)g.nv * g.N, g.nv, 2, st))
         k_div<ND><<<grid, kBlock, 0, st>>>(g, ND - 1, F, rhs);
