@@ -289,6 +289,18 @@ def main():
         out["plain"] = {"value": case.npoints * ns_plain / tp, "nsteps": ns_plain,
                         "ms_per_step": tp * 1e3,
                         "batched_speedup_vs_plain": (a.batch * pts / tb) / (case.npoints * ns_plain / tp)}
+        # the same run with every RK stage state resident in HBM (19.8 GB at 401^2 x 960
+        # steps) instead of snapshots + window recompute: same results, no recompute
+        del Rb, Sb
+        Rr = Region(case, tile_shape=R.tile_shape, resident_states=True)
+        Rr.states[0].initialCondition = q0
+        Sr = Solver(Rr)
+        run_hot_path(Rr, Sr, 1)
+        tr, (sens_r, _, J_r) = run_hot_path(Rr, Sr, a.steps)
+        out["resident_states"] = {"value": pts / tr, "ms_per_step": tr * 1e3,
+                                  "state_bytes": 8 * 4 * case.npoints * (4 * a.nsteps + 1),
+                                  "same_J_and_Grad0": bool(J_r[0] == J[0] and sens_r[0] == sens[0])}
+        del Rr, Sr
         if a.c4_n > 0:
             out["viscous3d"] = viscous3d_leg(a.c4_n)
         out["uneven_batch"] = uneven_leg()

@@ -219,9 +219,15 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches);
  * MGD_OPT_TILE_SHAPE selects the tile of the fused 2-D kernels: -1 (default) autotune at
  * the first run, 0: 32 x 16, 1: 16 x 32, 2: 16 x 16, 3: 24 x 16 points with one thread
  * per point, 4: 16 x 16 and 5: 16 x 12 points with two threads per point.  No result
- * depends on the tile shape. */
+ * depends on the tile shape.
+ * MGD_OPT_RESIDENT_STATES = 1 makes the fused 2-D path keep every RK stage state of a
+ * forward run in device memory ((4 nsteps + 1) states, e.g. 19.8 GB for 401^2 x 960
+ * steps) so that the adjoint pass reads them instead of recomputing each window from
+ * its snapshot; results are identical.  Default 0: snapshots every save_interval steps
+ * and window recompute (UniformCheckpointer, src/UniformCheckpointerImpl.f90:78-208). */
 #define MGD_OPT_FORCE_PLAIN 1
 #define MGD_OPT_TILE_SHAPE 2
+#define MGD_OPT_RESIDENT_STATES 3
 int mgd_region_set_option(mgd_region* r, int option, int value);
 
 /* Tile shape the fused 2-D kernels use now (-1: the region runs the generic kernels). */

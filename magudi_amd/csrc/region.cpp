@@ -104,6 +104,10 @@ struct mgd_region {
   hipStream_t stream_rec = nullptr;
   hipEvent_t ev_fwd = nullptr, ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
   mgd::DevBuf window2, b2r, qsr;
+  // MGD_OPT_RESIDENT_STATES: the forward pass keeps every RK stage state (4 nsteps + 1
+  // states) in HBM and the adjoint pass reads them instead of recomputing windows
+  bool resident = false;
+  mgd::DevBuf wall;
   // checkpointed forward run the adjoint pass marches back from
   struct {
     bool valid = false;
@@ -657,6 +661,20 @@ int maybe_tune(mgd_region* r) {
 int fused_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
   const int nc = n_cost(r), nst = 4 * nsteps;
   const long M = (long)r->ninst * M_of(r);
+  if (r->resident) {
+    // every stage state into the resident buffer: state 4 s + k is the input of stage
+    // k + 1 of step s, state 4 nsteps the final state
+    double* W = r->wall.p;
+    launch_copy(M, r->q.p, W, r->stream);
+    for (int s = 0; s < nsteps; ++s) {
+      double* outs[4];
+      for (int k = 0; k < 4; ++k) outs[k] = W + (4L * s + 1 + k) * M;
+      fused_forward_step(r, W + 4L * s * M, nullptr, nullptr, r->b2.p, s, dt, nst, true, outs);
+    }
+    launch_copy(M, W + 4L * nsteps * M, r->q.p, r->stream);
+    if (nc > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, r->stream);
+    return MGD_OK;
+  }
   launch_copy(M, r->q.p, r->snaps.p, r->stream);
   for (int s = 0; s < nsteps; ++s) {
     fused_forward_step(r, r->q.p, r->b1.p, r->qy.p, r->b2.p, s, dt, nst, true, nullptr);
@@ -669,8 +687,6 @@ int fused_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
 int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool terminal, bool want_grad) {
   const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
   const long M = (long)r->ninst * M_of(r);
-  MGD_HIP(hipEventRecord(r->ev_fwd, r->stream));
-  MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_fwd, 0));
   double* wbuf[2] = {r->window.p, r->window2.p};
   const int nw = (nsteps + K - 1) / K;
   auto recompute = [&](int w) {
@@ -686,8 +702,12 @@ int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
                          r->stream_rec);
     }
   };
-  recompute(nw - 1);
-  MGD_HIP(hipEventRecord(r->ev_ready[(nw - 1) % 2], r->stream_rec));
+  if (!r->resident) {
+    MGD_HIP(hipEventRecord(r->ev_fwd, r->stream));
+    MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_fwd, 0));
+    recompute(nw - 1);
+    MGD_HIP(hipEventRecord(r->ev_ready[(nw - 1) % 2], r->stream_rec));
+  }
   // terminal condition (loadInitialCondition ADJOINT, src/SolverImpl.f90:368-428): zero or
   // the given adjoint state, plus the cost forcing of the final state with factor -dt/6
   double* L0 = r->lam.p;
@@ -698,16 +718,8 @@ int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
     launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->qT.p, r->aforce.p, 0, r->stream);
     launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -dt / 6.0, L0, r->stream);
   }
-  for (int w = nw - 1; w >= 0; --w) {
-    if (w >= 1) {
-      // window w-1 reuses the buffer of window w+1: wait until its adjoint has read it
-      if (w + 1 < nw) MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_free[(w + 1) % 2], 0));
-      recompute(w - 1);
-      MGD_HIP(hipEventRecord(r->ev_ready[(w - 1) % 2], r->stream_rec));
-    }
-    MGD_HIP(hipStreamWaitEvent(r->stream, r->ev_ready[w % 2], 0));
-    const double* W = wbuf[w % 2];
-    const int t0w = w * K, t1w = std::min(nsteps, t0w + K);
+  // reversed steps t1w - 1 .. t0w, stage states of step t at W + 4 (t - t0w) M
+  auto adjoint_steps = [&](const double* W, int t0w, int t1w) {
     for (int t = t1w - 1; t >= t0w; --t) {
       for (int i = 4; i >= 1; --i) {
         const double* qcur = W + (4L * (t - t0w) + (i - 1)) * M;
@@ -738,7 +750,21 @@ int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
         reduce_chunk(r, r->spart, na, nst, sidx, true);
       }
     }
-    MGD_HIP(hipEventRecord(r->ev_free[w % 2], r->stream));
+  };
+  if (r->resident) {
+    adjoint_steps(r->wall.p, 0, nsteps);
+  } else {
+    for (int w = nw - 1; w >= 0; --w) {
+      if (w >= 1) {
+        // window w-1 reuses the buffer of window w+1: wait until its adjoint has read it
+        if (w + 1 < nw) MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_free[(w + 1) % 2], 0));
+        recompute(w - 1);
+        MGD_HIP(hipEventRecord(r->ev_ready[(w - 1) % 2], r->stream_rec));
+      }
+      MGD_HIP(hipStreamWaitEvent(r->stream, r->ev_ready[w % 2], 0));
+      adjoint_steps(wbuf[w % 2], w * K, std::min(nsteps, w * K + K));
+      MGD_HIP(hipEventRecord(r->ev_free[w % 2], r->stream));
+    }
   }
   if (na > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_S.p, r->ninst, 1, r->stream);
   return MGD_OK;
@@ -858,8 +884,13 @@ int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, c
   if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M) || r->qs.alloc(M) ||
       r->qT.alloc(M))
     return MGD_ERR_HIP;
-  if (fused_ok(r))
+  if (fused_ok(r)) {
     if (int rc = prepare_quadrature(r, nst, dt)) return rc;
+    if (r->resident && r->wall.alloc((size_t)(4L * nsteps + 1) * M)) {
+      set_error("not enough device memory to keep every stage state resident");
+      return MGD_ERR_HIP;
+    }
+  }
   return MGD_OK;
 }
 
@@ -924,12 +955,13 @@ int enqueue_forward_run(mgd_region* r, double dt, int nsteps) {
 std::vector<uintptr_t> graph_key(mgd_region* r, int kind, std::initializer_list<uintptr_t> extra) {
   std::vector<uintptr_t> k = {(uintptr_t)kind, (uintptr_t)r->tiling.shape, (uintptr_t)fused_ok(r),
                               (uintptr_t)r->ctrl_stride, (uintptr_t)r->src_inst_stride,
-                              (uintptr_t)r->chunk, (uintptr_t)r->ck.nsteps, (uintptr_t)r->ck.K};
+                              (uintptr_t)r->chunk, (uintptr_t)r->ck.nsteps, (uintptr_t)r->ck.K,
+                              (uintptr_t)r->resident};
   const DevBuf* bufs[] = {&r->q, &r->lam, &r->rhs, &r->b1, &r->b2, &r->b1a, &r->b2a, &r->t, &r->w,
                           &r->G, &r->T1, &r->S, &r->qs, &r->aforce, &r->control, &r->grad,
                           &r->acc_J, &r->acc_S, &r->partial, &r->src_amp, &r->snaps, &r->window,
                           &r->qy, &r->ly, &r->l0, &r->jpart, &r->spart, &r->ssum, &r->wts,
-                          &r->window2, &r->b2r, &r->qsr, &r->qT, &r->d_tile_mask};
+                          &r->window2, &r->b2r, &r->qsr, &r->qT, &r->d_tile_mask, &r->wall};
   for (const DevBuf* b : bufs) k.push_back((uintptr_t)b->p);
   for (uintptr_t e : extra) k.push_back(e);
   return k;
@@ -1353,6 +1385,7 @@ int mgd_region_set_option(mgd_region* r, int option, int value) {
   if (!r) { set_error("null region"); return MGD_ERR_ARG; }
   switch (option) {
     case MGD_OPT_FORCE_PLAIN: r->force_plain = value != 0; return MGD_OK;
+    case MGD_OPT_RESIDENT_STATES: r->resident = value != 0; return MGD_OK;
     case MGD_OPT_TILE_SHAPE: {
       if (value < -1 || value >= kNumTileShapes) { set_error("tile shape must be -1 .. 5"); return MGD_ERR_ARG; }
       r->tile_shape = value;

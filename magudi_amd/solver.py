@@ -77,7 +77,7 @@ class Region:
     Solver%setup (sponge strengths, far-field penalties, mean pressure)."""
 
     def __init__(self, case: Case, n_instances: int = 1, force_plain: bool = False,
-                 tile_shape: int = -1):
+                 tile_shape: int = -1, resident_states: bool = False):
         self.case = case
         self.nd = case.nd
         self.nv = case.nv
@@ -108,6 +108,9 @@ class Region:
             check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_FORCE_PLAIN, 1))
         if tile_shape != -1:
             check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_TILE_SHAPE, tile_shape))
+        if resident_states:
+            # every RK stage state of a forward run stays in HBM; no window recompute
+            check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_RESIDENT_STATES, 1))
         for p in case.patches:
             pd = capi.PatchDesc()
             pd.type = p.type
