@@ -20,6 +20,8 @@ module magudi_hip
        MGD_INITIAL_STATE = 4, MGD_FINAL_STATE = 5, MGD_SCRATCH = 6
   integer(c_int), parameter, public :: MGD_FIELD_TARGET_STATE = 0,                            &
        MGD_FIELD_CONTROL_MOLLIFIER = 1, MGD_FIELD_TARGET_MOLLIFIER = 2
+  integer(c_int), parameter, public :: MGD_OPT_FORCE_PLAIN = 1, MGD_OPT_TILE_SHAPE = 2,         &
+       MGD_OPT_RESIDENT_STATES = 3
 
   type, bind(C), public :: mgd_region_desc
      integer(c_int) :: nd = 2
@@ -84,6 +86,13 @@ module magudi_hip
        real(c_double), value :: amplitude, frequency, radius, phase
        integer(c_int) :: rc
      end function mgd_region_add_source
+     function mgd_region_set_option(r, option, val) bind(C, name="mgd_region_set_option")  &
+          result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: r
+       integer(c_int), value :: option, val
+       integer(c_int) :: rc
+     end function mgd_region_set_option
      function mgd_region_finalize(r) bind(C, name="mgd_region_finalize") result(rc)
        import :: c_ptr, c_int
        type(c_ptr), value :: r
@@ -177,6 +186,7 @@ module magudi_hip
 
   public :: mgd_run_forward_checkpointed, mgd_run_adjoint_checkpointed, mgd_state_lincomb,      &
        mgd_inner_product, mgd_region_set_time_offsets
+  public :: mgd_region_set_option
   public :: mgd_last_error, mgd_region_create, mgd_region_destroy, mgd_region_add_patch,      &
        mgd_region_set_field, mgd_region_add_source, mgd_region_finalize, mgd_state_put,         &
        mgd_state_get, mgd_compute_rhs, mgd_run_forward, mgd_run_adjoint
