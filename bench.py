@@ -170,12 +170,58 @@ def uneven_leg(nsteps=32, save_interval=8):
         s.runAdjoint(want_gradient=False)
     G.sync()
     t_seq = time.perf_counter() - t0
+    # one launch per RK stage for the 18 fused 2-D members, the 1-D waves beside them
+    G.restart()
+    G.run_adjoint_merged(want_gradient=False)   # tables and warm-up
+    G.restart()
+    G.sync()
+    t0 = time.perf_counter()
+    G.run_adjoint_merged(want_gradient=False)
+    G.sync()
+    t_merged = time.perf_counter() - t0
+    merged = len(G._merged_members())
+    G.close()
     pts = G.grid_point_steps
     return {"workload": f"{len(cs)} regions: 8 x 1-D wave 2048, 16 x 2-D monopole 201..513 "
                         f"(seed 5), 2 x mixing layer 513x257 x 4 segments; fwd+adjoint "
                         f"{nsteps} steps, snapshots every {save_interval}",
-            "value": pts / t_group, "unit": "grid-pt-steps/s", "seconds": t_group,
-            "sequential_value": pts / t_seq, "concurrency_speedup": t_seq / t_group}
+            "value": pts / min(t_group, t_merged), "unit": "grid-pt-steps/s",
+            "concurrent_value": pts / t_group, "merged_value": pts / t_merged,
+            "merged_members": merged, "sequential_value": pts / t_seq,
+            "concurrency_speedup": t_seq / t_group, "merged_speedup": t_seq / t_merged}
+
+
+def small_ensemble_leg(count=32, nsteps=32, save_interval=8, seed=3):
+    """An ensemble of small 2-D monopoles (odd sizes drawn from [41, 129], seed 3):
+    forward + adjoint of every member with one launch per RK stage for all of them,
+    against the members issued concurrently and one after another."""
+    import numpy as np
+    from magudi_amd import cases
+    from magudi_amd.group import RegionGroup
+    rng = np.random.default_rng(seed)
+    ns = [int(2 * rng.integers(20, 65) + 1) for _ in range(count)]
+    G = RegionGroup([cases.acoustic_monopole(n=n, nsteps=nsteps, save_interval=save_interval)
+                     for n in ns])
+
+    def timed(fn):
+        G.restart()
+        G.sync()
+        t0 = time.perf_counter()
+        fn()
+        G.sync()
+        return time.perf_counter() - t0
+    G.restart()
+    G.run_adjoint_merged(want_gradient=False)
+    t_m = timed(lambda: G.run_adjoint_merged(want_gradient=False))
+    G.run_adjoint(want_gradient=False)
+    t_c = timed(lambda: G.run_adjoint(want_gradient=False))
+    t_s = timed(lambda: [s.runAdjoint(want_gradient=False) for s in G.solvers])
+    G.close()
+    pts = G.grid_point_steps
+    return {"workload": f"{count} monopoles, odd sizes in [41, 129] (seed {seed}); fwd+adjoint "
+                        f"{nsteps} steps, snapshots every {save_interval}",
+            "merged_value": pts / t_m, "concurrent_value": pts / t_c, "sequential_value": pts / t_s,
+            "unit": "grid-pt-steps/s", "merged_speedup_vs_sequential": t_s / t_m}
 
 
 def main():
@@ -304,6 +350,7 @@ def main():
         if a.c4_n > 0:
             out["viscous3d"] = viscous3d_leg(a.c4_n)
         out["uneven_batch"] = uneven_leg()
+        out["small_ensemble"] = small_ensemble_leg()
         out["small_grid"] = small_grid_leg()
         out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_steps, max(1, a.cpu_steps // 2))
     else:

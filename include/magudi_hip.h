@@ -249,6 +249,23 @@ int mgd_region_get_tile_shape(mgd_region* r, int* shape);
 #define MGD_KERNEL_RHS_ADJOINT 4
 int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, double* bytes);
 
+/* Region groups: regions of different sizes run in lock step with one kernel launch per
+ * RK stage for all of them (the uneven batches of the multiple-shooting and ensemble
+ * workflows; no counterpart in the reference, which runs them as separate jobs).
+ * Members: finalized 2-D regions on the fused path, each fitting 16 x 16 tiles, without
+ * resident stage states; any number of instances each.  mgd_group_run_adjoint is
+ * mgd_run_adjoint of every member (same nsteps and save_interval, own dt[k] and
+ * controls[k]); gradients / sensitivities / Js hold one output array per member (or
+ * null).  Results equal the members' own runs.  Members keep their streams and tile
+ * shapes after the run. */
+typedef struct mgd_group mgd_group;
+int mgd_group_create(mgd_region* const* regions, int n, mgd_group** group);
+int mgd_group_destroy(mgd_group* group);
+int mgd_group_run_adjoint(mgd_group* group, double t0, const double* dt, int nsteps,
+                          int save_interval, const double* const* controls,
+                          double* const* gradients, double* const* sensitivities,
+                          double* const* Js);
+
 /* PLOT3D whole-format files, single or multi block, either byte order
  * (src/PLOT3DFormat.c:88-294, src/PLOT3DHelperImpl.f90:554-1286).  Host only.
  * mgd_plot3d_detect: file_type 0 grid, 1 solution, 2 function (include_function enables

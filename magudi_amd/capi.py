@@ -75,6 +75,10 @@ def lib():
         "mgd_functional": [vp, dp],
         "mgd_run_forward": [vp, C.c_double, C.c_double, C.c_int, dp, dp, C.c_int],
         "mgd_run_adjoint": [vp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, dp, dp, C.c_int],
+        "mgd_group_create": [C.POINTER(vp), C.c_int, C.POINTER(vp)],
+        "mgd_group_destroy": [vp],
+        "mgd_group_run_adjoint": [vp, C.c_double, dp, C.c_int, C.c_int, C.POINTER(dp),
+                                  C.POINTER(dp), C.POINTER(dp), C.POINTER(dp)],
         "mgd_stencil_apply": [vp, C.c_int, C.c_int, dp, C.c_int],
         "mgd_region_get_grid": [vp, C.c_int, dp],
         "mgd_last_timings": [vp, dp, C.POINTER(C.c_int)],

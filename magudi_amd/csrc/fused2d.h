@@ -73,6 +73,18 @@ struct AdjStage {
   int npts, nslots, slot, ip_act;
 };
 
+// A member region of a multi-region launch (device table): its geometry, patches and
+// tiling, and its first launch block (members take consecutive ranges of nblk x
+// instances blocks).
+struct GroupMember {
+  Geo g;
+  const Patches* P;
+  Tiling T;
+  int block0;
+};
+constexpr int kGroupShape = 2;     // tile shape of multi-region launches (16 x 16)
+constexpr int kMaxGroup = 64;      // members of one multi-region launch
+
 // Balanced partition; MGD_ERR_UNSUPPORTED when an edge tile of a non-periodic direction
 // would hold fewer points than the deepest boundary closure.
 int fused2d_make_tiling(const Geo& g, const Patches& hp, int shape, HostTiling* out);
@@ -81,6 +93,12 @@ void launch_fwd_stage2d(const Geo& g, const Patches* P, const Tiling& T, const F
                         hipStream_t st);
 void launch_adj_stage2d(const Geo& g, const Patches* P, const Tiling& T, const AdjStage& a,
                         hipStream_t st);
+// One stage of every member of a group in one launch (args: one stage per member, in
+// device memory); MGD_ERR_UNSUPPORTED unless shape is kGroupShape.
+int launch_fwd_stage2d_group(int shape, const GroupMember* mem, int nmem, int nblocks,
+                             const FwdStage* args, hipStream_t st);
+int launch_adj_stage2d_group(int shape, const GroupMember* mem, int nmem, int nblocks,
+                             const AdjStage* args, hipStream_t st);
 // Stage sums of per-point quadrature terms: sums[inst][base + s] = sum over the points of
 // partial[inst][s][point] for s < nfill (a fixed-order tree, independent of the tiling).
 void launch_stage_reduce(const double* partial, int nslots, int nfill, int npts, double* sums,

@@ -67,8 +67,7 @@ struct TileGeo {
 
 // Tile of this launch block (XCD band map), or -1; its geometry from the balanced
 // partition, with no memory access.
-__device__ __forceinline__ int block_tile(const Tiling& T) {
-  const int b = blockIdx.x;
+__device__ __forceinline__ int block_tile(const Tiling& T, int b) {
   const int tile = (b % kNumXcd) * T.band + b / kNumXcd;
   return tile < T.nbx * T.nby ? tile : -1;
 }
@@ -546,18 +545,19 @@ __device__ __forceinline__ bool in_patch(const DPatch& pt, int i, int j) {
 }
 
 // -------------------------------------------------------------------------- forward
+// One forward stage on tile block `blk` of instance `inst` (uniform per work-group).
 template <class S>
-__global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_fwd_stage2d(Geo g, const Patches* __restrict__ P,
-                                                       Tiling T, FwdStage a) {
+__device__ __forceinline__ void fwd_stage_body(const Geo& g, const Patches* __restrict__ P,
+                                               const Tiling& T, const FwdStage& a, int blk,
+                                               long inst) {
   constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NP = S::NP, NC = S::NC;
   constexpr int SFX = S::SFX, STX = S::STX;
   __shared__ double sq[S::SQ];
   __shared__ double sc[S::SSCR];
   __shared__ double sgrid[TileGrid<S>::kSize];
-  const int tile = block_tile(T);
+  const int tile = block_tile(T, blk);
   if (tile < 0) return;   // block beyond the tiles of its XCD (uniform per block)
   const TileGeo t = tile_geo(T, tile);
-  const long inst = blockIdx.y;
   const long NVN = (long)NV2 * g.N;
   const double* qi = a.qin + inst * NVN;
   double* sfx = sc;
@@ -742,18 +742,19 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_fwd_stage2d(Geo g, co
 }
 
 // -------------------------------------------------------------------------- adjoint
+// One adjoint stage on tile block `blk` of instance `inst` (uniform per work-group).
 template <class S>
-__global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_adj_stage2d(Geo g, const Patches* __restrict__ P,
-                                                       Tiling T, AdjStage a) {
+__device__ __forceinline__ void adj_stage_body(const Geo& g, const Patches* __restrict__ P,
+                                               const Tiling& T, const AdjStage& a, int blk,
+                                               long inst) {
   constexpr int TX = S::TX, EX = S::EX, EY = S::EY, NP = S::NP, NC = S::NC;
   constexpr int STX = S::STX;
   __shared__ double sq[S::SQ];
   __shared__ double sc[S::SSCR];
   __shared__ double sgrid[TileGrid<S>::kSize];
-  const int tile = block_tile(T);
+  const int tile = block_tile(T, blk);
   if (tile < 0) return;   // block beyond the tiles of its XCD (uniform per block)
   const TileGeo t = tile_geo(T, tile);
-  const long inst = blockIdx.y;
   const long NVN = (long)NV2 * g.N;
   const double* li = a.lam + inst * NVN;
   const double* qi = a.q + inst * NVN;
@@ -941,6 +942,71 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_adj_stage2d(Geo g, co
   }
 }
 
+template <class S>
+__global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_fwd_stage2d(Geo g, const Patches* __restrict__ P,
+                                                       Tiling T, FwdStage a) {
+  fwd_stage_body<S>(g, P, T, a, blockIdx.x, blockIdx.y);
+}
+
+template <class S>
+__global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_adj_stage2d(Geo g, const Patches* __restrict__ P,
+                                                       Tiling T, AdjStage a) {
+  adj_stage_body<S>(g, P, T, a, blockIdx.x, blockIdx.y);
+}
+
+// Several regions in one launch: launch block k belongs to the member m with
+// mem[m].block0 <= k < mem[m + 1].block0 (members in block order), which sees it as tile
+// block (k - block0) mod nblk of instance (k - block0) / nblk.  args[m]: the stage of m.
+__device__ __forceinline__ int group_member(const GroupMember* mem, int nmem, int* blk,
+                                            long* inst) {
+  int m = 0;
+  while (m + 1 < nmem && (int)blockIdx.x >= mem[m + 1].block0) ++m;
+  const int lb = (int)blockIdx.x - mem[m].block0;
+  *blk = lb % mem[m].T.nblk;
+  *inst = lb / mem[m].T.nblk;
+  return m;
+}
+
+// Copies one trivially copyable descriptor from device memory into LDS, word by word
+// across the work-group (the caller synchronises).
+template <class T>
+__device__ __forceinline__ void copy_to_lds(const T* src, T* dst) {
+  static_assert(sizeof(T) % 4 == 0, "descriptor of whole words");
+  const int* s = reinterpret_cast<const int*>(src);
+  int* d = reinterpret_cast<int*>(dst);
+  for (int i = threadIdx.x; i < (int)(sizeof(T) / 4); i += blockDim.x) d[i] = s[i];
+}
+
+// The member's descriptors are staged in LDS: the body reads them many times, and LDS
+// cannot alias the global stores of the stage, so no read is repeated after a store.
+template <class S>
+__global__ void __launch_bounds__(S::NT, S::kWavesPerEU)
+    k_fwd_stage2d_group(const GroupMember* __restrict__ mem, int nmem, const FwdStage* __restrict__ args) {
+  __shared__ GroupMember sm;
+  __shared__ FwdStage sa;
+  int blk;
+  long inst;
+  const int m = group_member(mem, nmem, &blk, &inst);
+  copy_to_lds(&mem[m], &sm);
+  copy_to_lds(&args[m], &sa);
+  __syncthreads();
+  fwd_stage_body<S>(sm.g, sm.P, sm.T, sa, blk, inst);
+}
+
+template <class S>
+__global__ void __launch_bounds__(S::NT, S::kWavesPerEU)
+    k_adj_stage2d_group(const GroupMember* __restrict__ mem, int nmem, const AdjStage* __restrict__ args) {
+  __shared__ GroupMember sm;
+  __shared__ AdjStage sa;
+  int blk;
+  long inst;
+  const int m = group_member(mem, nmem, &blk, &inst);
+  copy_to_lds(&mem[m], &sm);
+  copy_to_lds(&args[m], &sa);
+  __syncthreads();
+  adj_stage_body<S>(sm.g, sm.P, sm.T, sa, blk, inst);
+}
+
 // Stage sums of the per-point quadrature terms of one chunk of stages: one block of
 // kSumThreads per (stage, instance), a fixed-order tree over the points.
 constexpr int kSumThreads = 1024;
@@ -1069,6 +1135,18 @@ int fused2d_make_tiling(const Geo& g, const Patches& hp, int shape, HostTiling* 
 }
 
 template <class S>
+static void fwd_group_launch(const GroupMember* mem, int nmem, int nblocks, const FwdStage* args,
+                             hipStream_t st) {
+  k_fwd_stage2d_group<S><<<nblocks, S::NT, 0, st>>>(mem, nmem, args);
+}
+
+template <class S>
+static void adj_group_launch(const GroupMember* mem, int nmem, int nblocks, const AdjStage* args,
+                             hipStream_t st) {
+  k_adj_stage2d_group<S><<<nblocks, S::NT, 0, st>>>(mem, nmem, args);
+}
+
+template <class S>
 static void fwd_launch(const Geo& g, const Patches* P, const Tiling& T, const FwdStage& a,
                        hipStream_t st) {
   k_fwd_stage2d<S><<<dim3(T.nblk, g.ninst), S::NT, 0, st>>>(g, P, T, a);
@@ -1102,6 +1180,20 @@ void launch_adj_stage2d(const Geo& g, const Patches* P, const Tiling& T, const A
     case 4: adj_launch<S16x16c2>(g, P, T, a, st); break;
     default: adj_launch<S16x12c2>(g, P, T, a, st); break;
   }
+}
+
+int launch_fwd_stage2d_group(int shape, const GroupMember* mem, int nmem, int nblocks,
+                             const FwdStage* args, hipStream_t st) {
+  if (shape != kGroupShape) return MGD_ERR_UNSUPPORTED;
+  fwd_group_launch<S16x16>(mem, nmem, nblocks, args, st);
+  return MGD_OK;
+}
+
+int launch_adj_stage2d_group(int shape, const GroupMember* mem, int nmem, int nblocks,
+                             const AdjStage* args, hipStream_t st) {
+  if (shape != kGroupShape) return MGD_ERR_UNSUPPORTED;
+  adj_group_launch<S16x16>(mem, nmem, nblocks, args, st);
+  return MGD_OK;
 }
 
 void launch_stage_reduce(const double* partial, int nslots, int nfill, int npts, double* sums,

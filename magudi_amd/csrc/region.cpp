@@ -126,6 +126,17 @@ struct mgd_region {
 
 using namespace mgd;
 
+// Regions of different sizes run in lock step, one launch per RK stage for all of them
+// (mgd_group_*): the stage arguments of every member and stage are built on the host
+// once per run and read by the kernels from device tables.
+struct mgd_group {
+  std::vector<mgd_region*> m;
+  hipStream_t st = nullptr, st_rec = nullptr;
+  hipEvent_t ev_fwd = nullptr, ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+  mgd::DevBuf dmem, dfwd, drec, dadj;   // member table; forward, recompute, adjoint stage tables
+  int nblocks = 0;
+};
+
 namespace {
 
 long M_of(const mgd_region* r) { return (long)r->nv * r->N; }
@@ -1013,6 +1024,178 @@ int copy_adjoint_outputs(mgd_region* r, double* gradient, double* sensitivity) {
   return MGD_OK;
 }
 
+// ---------------------------------------------------------------- region groups
+// Stage tables of a group run: for stage index sidx and member m, entry sidx * n + m.  The
+// buffers, offsets and ping-pong of each member are those of fused_forward_pass /
+// fused_adjoint_pass, so every launch does what the member's own launch would.
+int group_tables(mgd_group* G, double dt0, const double* dts, int nsteps, int K, bool want_grad) {
+  (void)dt0;
+  const int n = (int)G->m.size(), nst = 4 * nsteps;
+  std::vector<FwdStage> fwd((size_t)nst * n), rec((size_t)nst * n);
+  std::vector<AdjStage> adj((size_t)nst * n);
+  std::vector<GroupMember> mem(n);
+  int block0 = 0;
+  for (int k = 0; k < n; ++k) {
+    mgd_region* r = G->m[k];
+    const double dt = dts[k];
+    const long M = (long)r->ninst * M_of(r);
+    const int na = n_act(r);
+    mem[k].g = r->geo;
+    mem[k].P = dpatches(r);
+    mem[k].T = r->tiling;
+    mem[k].block0 = block0;
+    block0 += r->tiling.nblk * r->ninst;
+    // forward pass: levels in q, stage states in b1 / qy (fused_forward_step)
+    const double* qin[4] = {r->q.p, r->b1.p, r->qy.p, r->b1.p};
+    double* qo[4] = {r->b1.p, r->qy.p, r->b1.p, r->q.p};
+    double* wbuf[2] = {r->window.p, r->window2.p};
+    for (int t = 0; t < nsteps; ++t) {
+      const int w = t / K, t0w = w * K, t1w = std::min(nsteps, t0w + K);
+      double* W = wbuf[w % 2];
+      const long base = 4L * (t - t0w);
+      double* outs[4];
+      for (int i = 0; i < 4; ++i) outs[i] = W + (base + 1 + i) * M;
+      if (t == t1w - 1) outs[3] = r->qsr.p;
+      for (int i = 1; i <= 4; ++i) {
+        const int sidx = 4 * t + i - 1;
+        fwd[(size_t)sidx * n + k] =
+            fwd_args(r, qin[i - 1], r->q.p, r->b2.p, qo[i - 1], i, dt, sidx, nst, true);
+        rec[(size_t)sidx * n + k] = fwd_args(r, i == 1 ? W + base * M : outs[i - 2], W + base * M,
+                                             r->b2r.p, outs[i - 1], i, dt, sidx, nst, false);
+        AdjStage a;
+        std::memset(&a, 0, sizeof a);
+        double* L0 = r->lam.p;
+        double* LX = r->ly.p;
+        double* LY = r->l0.p;
+        a.lam = i == 4 ? L0 : (i == 3 ? LX : (i == 2 ? LY : LX));
+        a.lout = i == 4 ? LX : (i == 3 ? LY : (i == 2 ? LX : L0));
+        a.l0 = L0;
+        a.q = W + (base + (i - 1)) * M;
+        a.b2 = r->b2.p;
+        a.stage = i;
+        a.dt = dt;
+        a.forcing_factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
+        a.zero_forcing = t == 0 && i == 1;
+        a.ip_act = r->ip_act;
+        if (na > 0) {
+          if (want_grad) {
+            a.grad = r->grad.p + (long)sidx * na;
+            a.grad_stride = (long)nst * na;
+          }
+          a.spartial = r->spart.p;
+          a.npts = na;
+          a.nslots = r->chunk;
+          a.slot = sidx % r->chunk;
+        }
+        adj[(size_t)sidx * n + k] = a;
+      }
+    }
+  }
+  G->nblocks = block0;
+  if (upload_ints(G->dmem, mem.data(), mem.size() * sizeof(GroupMember)) ||
+      upload_ints(G->dfwd, fwd.data(), fwd.size() * sizeof(FwdStage)) ||
+      upload_ints(G->drec, rec.data(), rec.size() * sizeof(FwdStage)) ||
+      upload_ints(G->dadj, adj.data(), adj.size() * sizeof(AdjStage)))
+    return MGD_ERR_HIP;
+  return MGD_OK;
+}
+
+const FwdStage* stage_table(const DevBuf& b, int sidx, int n) {
+  return reinterpret_cast<const FwdStage*>(b.p) + (size_t)sidx * n;
+}
+
+// fused_forward_pass + the enqueue_forward_ck bookkeeping of every member, one launch
+// per stage for all members, on the group stream (every member's stream is the group's
+// for the duration of the run).
+int group_forward_pass(mgd_group* G, int nsteps, int K) {
+  const int n = (int)G->m.size(), nst = 4 * nsteps;
+  const GroupMember* mem = reinterpret_cast<const GroupMember*>(G->dmem.p);
+  for (mgd_region* r : G->m) {
+    launch_fill(r->ninst, 0.0, r->acc_J.p, G->st);
+    launch_copy((long)r->ninst * M_of(r), r->q.p, r->snaps.p, G->st);
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    for (int i = 1; i <= 4; ++i) {
+      const int sidx = 4 * s + i - 1;
+      if (int rc = launch_fwd_stage2d_group(kGroupShape, mem, n, G->nblocks,
+                                            stage_table(G->dfwd, sidx, n), G->st))
+        return rc;
+      for (mgd_region* r : G->m) reduce_chunk(r, r->jpart, n_cost(r), nst, sidx, false);
+    }
+    if ((s + 1) % K == 0)
+      for (mgd_region* r : G->m) {
+        const long M = (long)r->ninst * M_of(r);
+        launch_copy(M, r->q.p, r->snaps.p + (long)((s + 1) / K) * M, G->st);
+      }
+  }
+  for (mgd_region* r : G->m) {
+    if (n_cost(r) > 0)
+      launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, G->st);
+    launch_copy((long)r->ninst * M_of(r), r->q.p, r->qT.p, G->st);
+  }
+  return MGD_OK;
+}
+
+// fused_adjoint_pass of every member (terminal condition zero), windows recomputed on the
+// group's second stream, one launch per stage for all members.
+int group_adjoint_pass(mgd_group* G, const double* dts, int nsteps, int K) {
+  const int n = (int)G->m.size(), nst = 4 * nsteps;
+  const GroupMember* mem = reinterpret_cast<const GroupMember*>(G->dmem.p);
+  const int nw = (nsteps + K - 1) / K;
+  for (mgd_region* r : G->m) launch_fill(r->ninst, 0.0, r->acc_S.p, G->st);
+  MGD_HIP(hipEventRecord(G->ev_fwd, G->st));
+  MGD_HIP(hipStreamWaitEvent(G->st_rec, G->ev_fwd, 0));
+  auto recompute = [&](int w) -> int {
+    const int t0w = w * K, t1w = std::min(nsteps, t0w + K);
+    for (mgd_region* r : G->m) {
+      const long M = (long)r->ninst * M_of(r);
+      launch_copy(M, r->snaps.p + (long)w * M, (w % 2 ? r->window2.p : r->window.p), G->st_rec);
+    }
+    for (int t = t0w; t < t1w; ++t)
+      for (int i = 1; i <= 4; ++i)
+        if (int rc = launch_fwd_stage2d_group(kGroupShape, mem, n, G->nblocks,
+                                              stage_table(G->drec, 4 * t + i - 1, n), G->st_rec))
+          return rc;
+    return MGD_OK;
+  };
+  if (int rc = recompute(nw - 1)) return rc;
+  MGD_HIP(hipEventRecord(G->ev_ready[(nw - 1) % 2], G->st_rec));
+  for (int k = 0; k < n; ++k) {
+    mgd_region* r = G->m[k];
+    const long M = (long)r->ninst * M_of(r);
+    const int nc = n_cost(r);
+    launch_fill(M, 0.0, r->lam.p, G->st);
+    if (nc > 0) {
+      launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->qT.p, r->aforce.p, 0, G->st);
+      launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -dts[k] / 6.0,
+                          r->lam.p, G->st);
+    }
+  }
+  const AdjStage* adj = reinterpret_cast<const AdjStage*>(G->dadj.p);
+  for (int w = nw - 1; w >= 0; --w) {
+    if (w >= 1) {
+      if (w + 1 < nw) MGD_HIP(hipStreamWaitEvent(G->st_rec, G->ev_free[(w + 1) % 2], 0));
+      if (int rc = recompute(w - 1)) return rc;
+      MGD_HIP(hipEventRecord(G->ev_ready[(w - 1) % 2], G->st_rec));
+    }
+    MGD_HIP(hipStreamWaitEvent(G->st, G->ev_ready[w % 2], 0));
+    const int t0w = w * K, t1w = std::min(nsteps, t0w + K);
+    for (int t = t1w - 1; t >= t0w; --t)
+      for (int i = 4; i >= 1; --i) {
+        const int sidx = 4 * t + i - 1;
+        if (int rc = launch_adj_stage2d_group(kGroupShape, mem, n, G->nblocks,
+                                              adj + (size_t)sidx * n, G->st))
+          return rc;
+        for (mgd_region* r : G->m) reduce_chunk(r, r->spart, n_act(r), nst, sidx, true);
+      }
+    MGD_HIP(hipEventRecord(G->ev_free[w % 2], G->st));
+  }
+  for (mgd_region* r : G->m)
+    if (n_act(r) > 0)
+      launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_S.p, r->ninst, 1, G->st);
+  return MGD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1780,6 +1963,108 @@ int mgd_sync(mgd_region* r) {
   if (!r) return MGD_ERR_ARG;
   MGD_HIP(hipStreamSynchronize(r->stream));
   return MGD_OK;
+}
+
+int mgd_group_create(mgd_region* const* regions, int n, mgd_group** out) {
+  if (!out || !regions || n < 1 || n > kMaxGroup) {
+    set_error("a group holds 1 .. 64 regions");
+    return MGD_ERR_ARG;
+  }
+  *out = nullptr;
+  auto G = std::make_unique<mgd_group>();
+  for (int k = 0; k < n; ++k) {
+    if (!regions[k] || !regions[k]->finalized) { set_error("group member not finalized"); return MGD_ERR_STATE; }
+    G->m.push_back(regions[k]);
+  }
+  MGD_HIP(hipStreamCreate(&G->st));
+  MGD_HIP(hipStreamCreate(&G->st_rec));
+  MGD_HIP(hipEventCreateWithFlags(&G->ev_fwd, hipEventDisableTiming));
+  for (int i = 0; i < 2; ++i) {
+    MGD_HIP(hipEventCreateWithFlags(&G->ev_ready[i], hipEventDisableTiming));
+    MGD_HIP(hipEventCreateWithFlags(&G->ev_free[i], hipEventDisableTiming));
+  }
+  *out = G.release();
+  return MGD_OK;
+}
+
+int mgd_group_destroy(mgd_group* G) {
+  if (!G) return MGD_OK;
+  if (G->st) (void)hipStreamSynchronize(G->st);
+  if (G->st_rec) (void)hipStreamSynchronize(G->st_rec);
+  if (G->ev_fwd) (void)hipEventDestroy(G->ev_fwd);
+  for (int i = 0; i < 2; ++i) {
+    if (G->ev_ready[i]) (void)hipEventDestroy(G->ev_ready[i]);
+    if (G->ev_free[i]) (void)hipEventDestroy(G->ev_free[i]);
+  }
+  if (G->st) (void)hipStreamDestroy(G->st);
+  if (G->st_rec) (void)hipStreamDestroy(G->st_rec);
+  delete G;
+  return MGD_OK;
+}
+
+int mgd_group_run_adjoint(mgd_group* G, double t0, const double* dt, int nsteps, int save_interval,
+                          const double* const* controls, double* const* gradients,
+                          double* const* sensitivities, double* const* Js) {
+  if (!G) { set_error("null group"); return MGD_ERR_ARG; }
+  if (nsteps <= 0 || save_interval <= 0 || !dt) { set_error("bad time stepping"); return MGD_ERR_ARG; }
+  const int n = (int)G->m.size();
+  for (int k = 0; k < n; ++k) {
+    mgd_region* r = G->m[k];
+    if (!(dt[k] > 0.0)) { set_error("bad time step"); return MGD_ERR_ARG; }
+    if (!fused_ok(r) || r->resident) {
+      set_error("group members must run on the fused 2-D path without resident stage states");
+      return MGD_ERR_UNSUPPORTED;
+    }
+  }
+  // every member works on the group's streams for the duration of the run
+  std::vector<std::pair<hipStream_t, hipStream_t>> streams;
+  std::vector<int> shapes;
+  for (mgd_region* r : G->m) {
+    streams.emplace_back(r->stream, r->stream_rec);
+    shapes.push_back(r->tiling.shape);
+    r->stream = G->st;
+    r->stream_rec = G->st_rec;
+  }
+  auto restore = [&]() {
+    for (int k = 0; k < n; ++k) {
+      mgd_region* r = G->m[k];
+      r->stream = streams[k].first;
+      r->stream_rec = streams[k].second;
+      if (r->tiling.shape != shapes[k]) (void)install_tiling(r, shapes[k]);
+    }
+  };
+  auto run = [&]() -> int {
+    for (int k = 0; k < n; ++k) {
+      mgd_region* r = G->m[k];
+      if (int rc = prepare_forward_ck(r, t0, dt[k], nsteps, save_interval, controls ? controls[k] : nullptr))
+        return rc;
+      r->ck.valid = true;
+      if (int rc = prepare_adjoint_ck(r)) return rc;
+      if (r->tiling.shape != kGroupShape && install_tiling(r, kGroupShape) != MGD_OK) {
+        set_error("a group member does not fit the 16 x 16 tiles of a group launch");
+        return MGD_ERR_UNSUPPORTED;
+      }
+    }
+    const bool want_grad = gradients != nullptr;
+    if (int rc = group_tables(G, dt[0], dt, nsteps, save_interval, want_grad)) return rc;
+    if (int rc = group_forward_pass(G, nsteps, save_interval)) return rc;
+    if (int rc = group_adjoint_pass(G, dt, nsteps, save_interval)) return rc;
+    MGD_HIP(hipGetLastError());
+    MGD_HIP(hipStreamSynchronize(G->st));
+    for (int k = 0; k < n; ++k) {
+      mgd_region* r = G->m[k];
+      if (Js && Js[k])
+        MGD_HIP(hipMemcpy(Js[k], r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
+      if (int rc = copy_adjoint_outputs(r, gradients ? gradients[k] : nullptr,
+                                        sensitivities ? sensitivities[k] : nullptr))
+        return rc;
+    }
+    return MGD_OK;
+  };
+  const int rc = run();
+  if (rc != MGD_OK) (void)hipStreamSynchronize(G->st);
+  restore();
+  return rc;
 }
 
 }  // extern "C"

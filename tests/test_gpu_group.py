@@ -57,3 +57,33 @@ def test_group_member_matches_oracle(which):
     assert abs(J[0] - Jo) <= 1e-12 * abs(Jo)
     assert abs(s[0] - so) <= 1e-10 * abs(so) + 1e-300
     assert np.abs(g[0] - go).max() <= 1e-10 * np.abs(go).max() + 1e-300
+
+
+def test_merged_group_equals_single_runs():
+    """One launch per RK stage for all fused 2-D members (mgd_group_run_adjoint), the 1-D
+    member beside them: J, sensitivity and gradient of every member equal its own run
+    bit for bit, with a control on one member and two instances on another; the members'
+    tile shapes are back to their own after the run."""
+    from magudi_amd.group import RegionGroup
+    from magudi_amd.solver import Region, Solver
+    cs = small_batch()
+    ninst = [1, 1, 2, 2]
+    G = RegionGroup(cs, n_instances=ninst)
+    shapes = [r.tile_shape for r in G.regions]
+    assert G._merged_members() == [1, 2, 3]
+    rng = np.random.default_rng(17)
+    na = G.regions[2].n_actuator
+    controls = [None, None, 1e-2 * rng.standard_normal((2, 4 * cs[2].nsteps, na)), None]
+    for rep in range(2):   # second run reuses the group and its tables
+        G.restart()
+        out = G.run_adjoint_merged(controls=controls)
+        for k, c in enumerate(cs):
+            R = Region(c, n_instances=ninst[k])
+            for i in range(R.n_instances):
+                R.states[i].conservedVariables = c.initial_state()
+            s1, g1, J1 = Solver(R).runAdjoint(control=controls[k])
+            s, g, J = out[k]
+            assert np.array_equal(J, J1) and np.array_equal(s, s1), (rep, c.name)
+            assert (g is None and g1 is None) or np.array_equal(g, g1), (rep, c.name)
+    assert [r.tile_shape for r in G.regions] == shapes
+    G.close()
