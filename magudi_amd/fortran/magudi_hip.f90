@@ -93,6 +93,30 @@ module magudi_hip
        integer(c_int), value :: option, val
        integer(c_int) :: rc
      end function mgd_region_set_option
+     ! regions in one launch per RK stage; controls / gradients / sensitivities / Js are
+     ! c_loc of arrays of c_ptr (one per member) or c_null_ptr
+     function mgd_group_create(regions, n, group) bind(C, name="mgd_group_create") result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr), intent(in) :: regions(*)
+       integer(c_int), value :: n
+       type(c_ptr), intent(out) :: group
+       integer(c_int) :: rc
+     end function mgd_group_create
+     function mgd_group_destroy(group) bind(C, name="mgd_group_destroy") result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: group
+       integer(c_int) :: rc
+     end function mgd_group_destroy
+     function mgd_group_run_adjoint(group, t0, dt, nsteps, save_interval, controls,         &
+          gradients, sensitivities, js) bind(C, name="mgd_group_run_adjoint") result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: group
+       real(c_double), value :: t0
+       real(c_double), intent(in) :: dt(*)
+       integer(c_int), value :: nsteps, save_interval
+       type(c_ptr), value :: controls, gradients, sensitivities, js
+       integer(c_int) :: rc
+     end function mgd_group_run_adjoint
      function mgd_region_finalize(r) bind(C, name="mgd_region_finalize") result(rc)
        import :: c_ptr, c_int
        type(c_ptr), value :: r
@@ -186,7 +210,7 @@ module magudi_hip
 
   public :: mgd_run_forward_checkpointed, mgd_run_adjoint_checkpointed, mgd_state_lincomb,      &
        mgd_inner_product, mgd_region_set_time_offsets
-  public :: mgd_region_set_option
+  public :: mgd_region_set_option, mgd_group_create, mgd_group_destroy, mgd_group_run_adjoint
   public :: mgd_last_error, mgd_region_create, mgd_region_destroy, mgd_region_add_patch,      &
        mgd_region_set_field, mgd_region_add_source, mgd_region_finalize, mgd_state_put,         &
        mgd_state_get, mgd_compute_rhs, mgd_run_forward, mgd_run_adjoint
