@@ -108,6 +108,9 @@ struct mgd_region {
   // states) in HBM and the adjoint pass reads them instead of recomputing windows
   bool resident = false;
   mgd::DevBuf wall;
+  // the last checkpointed forward pass left the stage states of its last window in
+  // `window` (the adjoint pass then starts without recomputing that window)
+  bool last_window = false;
   // checkpointed forward run the adjoint pass marches back from
   struct {
     bool valid = false;
@@ -668,7 +671,10 @@ int maybe_tune(mgd_region* r) {
 // into one of two window buffers on a second stream, so the recompute of window w-1 runs
 // concurrently with the adjoint of window w (UniformCheckpointer%migrateTo semantics:
 // the same states, each recomputed once).  Window slot 4m holds the state at step wK+m
-// and slots 4m+1..4m+3 the inputs of its stages 2-4.
+// and slots 4m+1..4m+3 the inputs of its stages 2-4.  The forward pass writes the stage
+// states of the last window straight into the first window buffer (the same launches,
+// other output addresses), so the adjoint pass starts at once; windows then alternate
+// between the two buffers from the last one down.
 int fused_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
   const int nc = n_cost(r), nst = 4 * nsteps;
   const long M = (long)r->ninst * M_of(r);
@@ -686,22 +692,36 @@ int fused_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
     if (nc > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, r->stream);
     return MGD_OK;
   }
+  const int t0last = ((nsteps + K - 1) / K - 1) * K;
+  double* W = r->window.p;
   launch_copy(M, r->q.p, r->snaps.p, r->stream);
   for (int s = 0; s < nsteps; ++s) {
-    fused_forward_step(r, r->q.p, r->b1.p, r->qy.p, r->b2.p, s, dt, nst, true, nullptr);
+    if (s < t0last) {
+      fused_forward_step(r, r->q.p, r->b1.p, r->qy.p, r->b2.p, s, dt, nst, true, nullptr);
+    } else {
+      if (s == t0last) launch_copy(M, r->q.p, W, r->stream);
+      const long base = 4L * (s - t0last);
+      double* outs[4];
+      for (int k = 0; k < 4; ++k) outs[k] = W + (base + 1 + k) * M;
+      if (s == nsteps - 1) outs[3] = r->q.p;   // final state
+      fused_forward_step(r, W + base * M, nullptr, nullptr, r->b2.p, s, dt, nst, true, outs);
+    }
     if ((s + 1) % K == 0) launch_copy(M, r->q.p, r->snaps.p + (long)((s + 1) / K) * M, r->stream);
   }
   if (nc > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, r->stream);
+  r->last_window = true;
   return MGD_OK;
 }
 
-int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool terminal, bool want_grad) {
+int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool terminal, bool want_grad,
+                       bool last_window) {
   const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
   const long M = (long)r->ninst * M_of(r);
   double* wbuf[2] = {r->window.p, r->window2.p};
   const int nw = (nsteps + K - 1) / K;
+  auto buf = [&](int w) { return (nw - 1 - w) % 2; };   // window buffer of window w
   auto recompute = [&](int w) {
-    double* W = wbuf[w % 2];
+    double* W = wbuf[buf(w)];
     const int t0w = w * K, t1w = std::min(nsteps, t0w + K);
     launch_copy(M, r->snaps.p + (long)w * M, W, r->stream_rec);
     for (int t = t0w; t < t1w; ++t) {
@@ -713,11 +733,15 @@ int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
                          r->stream_rec);
     }
   };
-  if (!r->resident) {
+  // the recompute stream joins only when it has windows to recompute (a captured stream
+  // that waits on the run and records nothing it rejoins with would end the capture)
+  if (!r->resident && (nw >= 2 || !last_window)) {
     MGD_HIP(hipEventRecord(r->ev_fwd, r->stream));
     MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_fwd, 0));
-    recompute(nw - 1);
-    MGD_HIP(hipEventRecord(r->ev_ready[(nw - 1) % 2], r->stream_rec));
+    if (!last_window) {
+      recompute(nw - 1);
+      MGD_HIP(hipEventRecord(r->ev_ready[buf(nw - 1)], r->stream_rec));
+    }
   }
   // terminal condition (loadInitialCondition ADJOINT, src/SolverImpl.f90:368-428): zero or
   // the given adjoint state, plus the cost forcing of the final state with factor -dt/6
@@ -768,13 +792,14 @@ int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
     for (int w = nw - 1; w >= 0; --w) {
       if (w >= 1) {
         // window w-1 reuses the buffer of window w+1: wait until its adjoint has read it
-        if (w + 1 < nw) MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_free[(w + 1) % 2], 0));
+        if (w + 1 < nw) MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_free[buf(w + 1)], 0));
         recompute(w - 1);
-        MGD_HIP(hipEventRecord(r->ev_ready[(w - 1) % 2], r->stream_rec));
+        MGD_HIP(hipEventRecord(r->ev_ready[buf(w - 1)], r->stream_rec));
       }
-      MGD_HIP(hipStreamWaitEvent(r->stream, r->ev_ready[w % 2], 0));
-      adjoint_steps(wbuf[w % 2], w * K, std::min(nsteps, w * K + K));
-      MGD_HIP(hipEventRecord(r->ev_free[w % 2], r->stream));
+      // the last window, when the forward pass wrote it, is ordered by the stream itself
+      if (w < nw - 1 || !last_window) MGD_HIP(hipStreamWaitEvent(r->stream, r->ev_ready[buf(w)], 0));
+      adjoint_steps(wbuf[buf(w)], w * K, std::min(nsteps, w * K + K));
+      MGD_HIP(hipEventRecord(r->ev_free[buf(w)], r->stream));
     }
   }
   if (na > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_S.p, r->ninst, 1, r->stream);
@@ -884,6 +909,7 @@ int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, c
   const int na = n_act(r), nst = 4 * nsteps;
   const long M = (long)r->ninst * M_of(r);
   r->ck.valid = false;
+  r->last_window = false;
   r->ck.t0 = t0;
   r->ck.dt = dt;
   r->ck.nsteps = nsteps;
@@ -932,8 +958,12 @@ int prepare_adjoint_ck(mgd_region* r) {
 }
 
 int enqueue_adjoint_ck(mgd_region* r, bool terminal, bool want_grad) {
+  // the window buffers are rewritten by every adjoint pass: the last window written by
+  // the forward pass serves the first adjoint pass after it only
+  const bool last_window = r->last_window;
+  r->last_window = false;
   launch_fill(r->ninst, 0.0, r->acc_S.p, r->stream);
-  return fused_ok(r) ? fused_adjoint_pass(r, r->ck.dt, r->ck.nsteps, r->ck.K, terminal, want_grad)
+  return fused_ok(r) ? fused_adjoint_pass(r, r->ck.dt, r->ck.nsteps, r->ck.K, terminal, want_grad, last_window)
                      : plain_adjoint_pass(r, r->ck.dt, r->ck.nsteps, r->ck.K, terminal, want_grad);
 }
 
@@ -1026,8 +1056,9 @@ int copy_adjoint_outputs(mgd_region* r, double* gradient, double* sensitivity) {
 
 // ---------------------------------------------------------------- region groups
 // Stage tables of a group run: for stage index sidx and member m, entry sidx * n + m.  The
-// buffers, offsets and ping-pong of each member are those of fused_forward_pass /
-// fused_adjoint_pass, so every launch does what the member's own launch would.
+// stage arguments of each member are those of fused_forward_pass / fused_adjoint_pass
+// (stage states ping-pong in the forward pass and every window is recomputed), so every
+// launch computes what the member's own launch would.
 int group_tables(mgd_group* G, double dt0, const double* dts, int nsteps, int K, bool want_grad) {
   (void)dt0;
   const int n = (int)G->m.size(), nst = 4 * nsteps;
