@@ -58,6 +58,8 @@ struct Shape {
   // forward stage: form the dissipation intermediates in registers alongside the flux
   // derivatives (where the register budget has room for them)
   static constexpr bool kDissRegs = kWavesPerEU <= 3 || CS > 1;
+  // the tile splits into 8 x 8 blocks of points, one wave each (see point_of_lane)
+  static constexpr bool kQuads = TX % 8 == 0 && TY % 8 == 0;
 };
 
 struct TileGeo {
@@ -70,6 +72,28 @@ struct TileGeo {
 __device__ __forceinline__ int block_tile(const Tiling& T, int b) {
   const int tile = (b % kNumXcd) * T.band + b / kNumXcd;
   return tile < T.nbx * T.nby ? tile : -1;
+}
+
+// Point (tx, ty) of a thread of the forward stage (lane = index within its component
+// group).  Row-major in general: a wave covers 64 / TX whole rows.  In a tile on a
+// non-periodic x boundary the boundary-closure columns (the first / last 8 points) would
+// split every such wave into closure and interior lanes, and a wave runs both paths;
+// there each wave covers an 8 x 8 block of points instead, so the closure columns of the
+// first-derivative operator fill whole waves.  Which thread forms a point changes no
+// value.
+template <class S>
+__device__ __forceinline__ void point_of_lane(const Geo& g, const Tiling& T, int tile, int lane,
+                                              int* tx, int* ty) {
+  const int bx = tile % T.nbx;
+  if (S::kQuads && !g.dir[0].periodic && (bx == 0 || bx == T.nbx - 1)) {
+    const int l = lane & 63, w = lane >> 6;
+    constexpr int QX = S::TX / 8;
+    *tx = (l & 7) + 8 * (w % QX);
+    *ty = (l >> 3) + 8 * (w / QX);
+  } else {
+    *tx = lane % S::TX;
+    *ty = lane / S::TX;
+  }
 }
 
 __device__ __forceinline__ TileGeo tile_geo(const Tiling& T, int tile) {
@@ -568,20 +592,11 @@ __device__ __forceinline__ void fwd_stage_body(const Geo& g, const Patches* __re
   const unsigned pmask = T.pmask[tile];
   // this thread's point and component group
   const int lane = threadIdx.x % NP, c0 = (threadIdx.x / NP) * NC;
-  const int tx = lane % TX, ty = lane / TX;
+  int tx, ty;
+  point_of_lane<S>(g, T, tile, lane, &tx, &ty);
   const bool mine = tx < t.sx && ty < t.sy;
   const int gi = t.x0 + tx, gj = t.y0 + ty;
   const long p = gi + (long)g.n[0] * gj;
-  // RK operands of this thread's components, fetched before the stencil work
-  double pq0[NC], pb2[NC];
-  if (mine) {
-#pragma unroll
-    for (int j = 0; j < NC; ++j) {
-      const long o = (c0 + j) * g.N + p;
-      pq0[j] = (a.stage == 2 || a.stage == 3) ? a.q0[inst * NVN + o] : 0.0;
-      pb2[j] = a.stage >= 2 ? a.b2[inst * NVN + o] : 0.0;
-    }
-  }
   StageRegs<S> sr;
   stage_load<S>(g, t, qi, sr);
   // the grid values are needed by the flux metrics; this barrier waits for them only
@@ -605,6 +620,19 @@ __device__ __forceinline__ void fwd_stage_body(const Geo& g, const Patches* __re
                          [&](int l, int e) { return sfy[(l * EY + e) * TX + tx]; }, dfy);
 #pragma unroll
     for (int j = 0; j < NC; ++j) r[j] = 0.0 - (dfx[j] + dfy[j]);
+  }
+  // RK operands of this thread's components, fetched after the flux derivatives: every
+  // block issues its operand loads at once at entry, and these (two thirds of the entry
+  // bytes in stages 2 and 3) would lengthen that burst; here they arrive during the
+  // dissipation terms
+  double pq0[NC], pb2[NC];
+  if (mine) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      const long o = (c0 + j) * g.N + p;
+      pq0[j] = (a.stage == 2 || a.stage == 3) ? a.q0[inst * NVN + o] : 0.0;
+      pb2[j] = a.stage >= 2 ? a.b2[inst * NVN + o] : 0.0;
+    }
   }
   double* stx = sc;
   double* sty = sc + STX;
@@ -688,7 +716,9 @@ __device__ __forceinline__ void fwd_stage_body(const Geo& g, const Patches* __re
         r[j] = r[j] + a.src_amp[inst * a.src_inst_stride + s] * exp(-a.src[s].gfac * rr2);
       }
     }
-    // RK4 stage update (substepForwardRK4, src/RK4IntegratorImpl.f90:115-156)
+    // RK4 stage update (substepForwardRK4, src/RK4IntegratorImpl.f90:115-156).  The
+    // stage outputs are written with non-temporal stores: they stream to HBM during the
+    // launch instead of waiting in L2 for the write-back at its end
     const double dt = a.dt;
     double* b2 = a.b2 + inst * NVN;
 #pragma unroll
@@ -698,18 +728,18 @@ __device__ __forceinline__ void fwd_stage_body(const Geo& g, const Patches* __re
       const double rl = r[j];
       if (a.stage == 1) {
         const double ql = comp<NC>(qp, c0, j);
-        b2[o] = ql + div6(dt * rl);
+        __builtin_nontemporal_store(ql + div6(dt * rl), &b2[o]);
         qn[j] = ql + dt * rl / 2.0;
       } else if (a.stage == 2) {
-        b2[o] = pb2[j] + div3(dt * rl);
+        __builtin_nontemporal_store(pb2[j] + div3(dt * rl), &b2[o]);
         qn[j] = pq0[j] + dt * rl / 2.0;
       } else if (a.stage == 3) {
-        b2[o] = pb2[j] + div3(dt * rl);
+        __builtin_nontemporal_store(pb2[j] + div3(dt * rl), &b2[o]);
         qn[j] = pq0[j] + dt * rl;
       } else {
         qn[j] = pb2[j] + div6(dt * rl);
       }
-      a.qout[inst * NVN + o] = qn[j];
+      __builtin_nontemporal_store(qn[j], &a.qout[inst * NVN + o]);
       if (a.rhs_out) a.rhs_out[inst * NVN + o] = rl;
     }
   }
@@ -763,6 +793,8 @@ __device__ __forceinline__ void adj_stage_body(const Geo& g, const Patches* __re
   const TileGrid<S> tg{sgrid};
   const unsigned pmask = T.pmask[tile];
   const int lane = threadIdx.x % NP, c0 = (threadIdx.x / NP) * NC;
+  // row-major points (the 8 x 8 wave blocks of point_of_lane do not pay here: the
+  // adjoint closure is 12 deep and splits the second block column)
   const int tx = lane % TX, ty = lane / TX;
   const bool mine = tx < t.sx && ty < t.sy;
   const int gi = t.x0 + tx, gj = t.y0 + ty;
@@ -916,7 +948,8 @@ __device__ __forceinline__ void adj_stage_body(const Geo& g, const Patches* __re
         break;
     }
   }
-  // RK4 adjoint stage update (substepAdjointRK4, src/RK4IntegratorImpl.f90:215-261)
+  // RK4 adjoint stage update (substepAdjointRK4, src/RK4IntegratorImpl.f90:215-261),
+  // non-temporal stores as in the forward stage
   const double dt = a.dt;
   double* b2 = a.b2 + inst * NVN;
 #pragma unroll
@@ -926,18 +959,18 @@ __device__ __forceinline__ void adj_stage_body(const Geo& g, const Patches* __re
     double ln;
     if (a.stage == 4) {
       const double ll = comp<NC>(lp, c0, j);
-      b2[o] = ll - div6(dt * rl);
+      __builtin_nontemporal_store(ll - div6(dt * rl), &b2[o]);
       ln = ll - dt * rl / 2.0;
     } else if (a.stage == 3) {
-      b2[o] = pb2[j] - div3(dt * rl);
+      __builtin_nontemporal_store(pb2[j] - div3(dt * rl), &b2[o]);
       ln = pl0[j] - dt * rl / 2.0;
     } else if (a.stage == 2) {
-      b2[o] = pb2[j] - div3(dt * rl);
+      __builtin_nontemporal_store(pb2[j] - div3(dt * rl), &b2[o]);
       ln = pl0[j] - dt * rl;
     } else {
       ln = pb2[j] - div6(dt * rl);
     }
-    a.lout[inst * NVN + o] = ln;
+    __builtin_nontemporal_store(ln, &a.lout[inst * NVN + o]);
     if (a.rhs_out) a.rhs_out[inst * NVN + o] = rl;
   }
 }
