@@ -8,10 +8,20 @@ constexpr int kBlock = 256;
 
 // F: flux scratch (nd * nv * N per instance) -- fluxes computed once per point and
 // differentiated from memory; null recomputes them at every stencil tap
+// What the finishing pass of a forward rhs has to do, as far as the host knows it:
+// far-field patches present; whether the patches are at most one Kolmogorov forcing
+// over the whole domain (`kolmogorov`: its strength array, null for none) -- then, with
+// no dissipation and no sources, the pass rides on the last derivative kernel of the
+// periodic 3-D path.
+struct FinishInfo {
+  int has_farfield = 0;
+  int simple = 0;
+  const double* kolmogorov = nullptr;
+};
 void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
                         double* G, double* t, double* F, const double* control, long control_stride,
                         const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
-                        int has_farfield, hipStream_t st);
+                        const FinishInfo& fin, hipStream_t st);
 void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
                         double* rhs, double* w, double* G, double* t, double* T1, double* S,
                         const double* aforce, long aforce_stride, double forcing_factor,

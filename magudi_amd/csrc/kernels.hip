@@ -1068,13 +1068,18 @@ __global__ void __launch_bounds__(BX * BY) k_fluxdiv3d_xy(Geo g, const double* _
 // keeping the 9-point window of one component in registers -- each operand value is
 // read once per chunk instead of once per stencil tap.  Component c of instance i reads
 // src[i * src_inst + c * N] and writes dst[i * dst_inst + c * N]; mode 0 stores the
-// result, 1 adds it to dst, 2 stores 0 - (dst + result).  The sums are those of
-// sbp_apply's interior rows (same order), so the values are the same.
+// result, 1 adds it to dst, 2 stores 0 - (dst + result), 4 (3-D, five components) also
+// applies k_finish's forward operations where the patches are at most one Kolmogorov
+// forcing over the whole domain (strength kstr, state q) -- the same operations in the
+// same order.  The sums are those of sbp_apply's interior rows (same order), so the
+// values are the same.
 constexpr int kLineChunk = 32;
 __global__ void __launch_bounds__(kBlock) k_deriv_line(Geo g, int d, int slot,
                                                       const double* __restrict__ src, long src_inst,
                                                       double* __restrict__ dst, long dst_inst,
-                                                      int ncomp, int mode) {
+                                                      int ncomp, int mode,
+                                                      const double* __restrict__ q,
+                                                      const double* __restrict__ kstr) {
   const Dir dd = g.dir[d];
   const long nlines = g.N / dd.len;
   const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1107,7 +1112,20 @@ __global__ void __launch_bounds__(kBlock) k_deriv_line(Geo g, int d, int slot,
       *yz = out;
     } else {
       double v = *yz + out;
-      if (mode == 2) v = 0.0 - v;
+      if (mode >= 2) v = 0.0 - v;
+      if (mode == 4) {
+        // k_finish of a forward rhs whose patches are at most one Kolmogorov forcing over
+        // the whole domain: Jacobian scaling, then the forcing of the x momentum
+        int ijk[3];
+        ijk[0] = (int)i;
+        ijk[d] = z;
+        ijk[3 - d] = (int)o;
+        v = v * jacobian<3>(g, ijk);
+        if (c == 1 && kstr) {
+          const long pz = base + z * sd;
+          v = v + q[(long)inst * 5 * g.N + pz] * kstr[pz];
+        }
+      }
       *yz = v;
     }
 #pragma unroll
@@ -1163,8 +1181,10 @@ __global__ void __launch_bounds__(kBlock) k_deriv_row(Geo g, int slot, const dou
 // Launches k_deriv_row (d = 0) or k_deriv_line (d >= 1) when direction d is periodic and
 // the grid fits the kernel; false otherwise (the caller then uses the generic kernel).
 static bool launch_deriv_line(const Geo& g, int d, int slot, const double* src, long src_inst,
-                              double* dst, long dst_inst, int ncomp, int mode, hipStream_t st) {
+                              double* dst, long dst_inst, int ncomp, int mode, hipStream_t st,
+                              const double* q = nullptr, const double* kstr = nullptr) {
   if (!g.dir[d].periodic || g.dir[d].len < 16) return false;
+  if (mode == 4 && (d == 0 || g.nd != 3 || g.nv != 5 || ncomp != 5)) return false;
   if (d == 0) {
     const int n0 = g.dir[0].len;
     if (!((n0 % kBlock == 0) || (kBlock % n0 == 0)) || g.N % kBlock != 0) return false;
@@ -1176,7 +1196,8 @@ static bool launch_deriv_line(const Geo& g, int d, int slot, const double* src, 
   const dim3 grid((unsigned)((nlines + kBlock - 1) / kBlock),
                   (unsigned)((g.dir[d].len + kLineChunk - 1) / kLineChunk),
                   (unsigned)(g.ninst * ncomp));
-  k_deriv_line<<<grid, kBlock, 0, st>>>(g, d, slot, src, src_inst, dst, dst_inst, ncomp, mode);
+  k_deriv_line<<<grid, kBlock, 0, st>>>(g, d, slot, src, src_inst, dst, dst_inst, ncomp, mode, q,
+                                        kstr);
   return true;
 }
 
@@ -1223,15 +1244,23 @@ static void launch_gradients(const Geo& g, const double* q, double* w, double* G
 void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
                         double* G, double* t, double* F, const double* control, long control_stride,
                         const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
-                        int has_farfield, hipStream_t st) {
+                        const FinishInfo& fin, hipStream_t st) {
   const dim3 grid = grid_pts(g);
+  const int has_farfield = fin.has_farfield;
+  bool finished = false;
   MGD_DISPATCH(g.nd, {
     if (g.viscous) launch_gradients<ND>(g, q, w, G, grid, st);
     if (F && ND == 3 && fluxdiv3d_applies(g)) {
       const dim3 gg((g.n[0] / kFD3X) * (g.n[1] / kFD3Y), g.n[2], g.ninst);
       k_fluxdiv3d_xy<kFD3X, kFD3Y><<<gg, kFD3X * kFD3Y, 0, st>>>(g, q, g.viscous ? G : nullptr, F, rhs);
-      if (!launch_deriv_line(g, ND - 1, SLOT_FIRST, F + (long)(ND - 1) * g.nv * g.N,
-                             (long)ND * g.nv * g.N, rhs, (long)g.nv * g.N, g.nv, 2, st))
+      // the z divergence also finishes the rhs where nothing else follows it
+      const bool fuse = fin.simple && !g.dissipation && nsrc == 0;
+      const double* fz = F + (long)(ND - 1) * g.nv * g.N;
+      if (fuse && launch_deriv_line(g, ND - 1, SLOT_FIRST, fz, (long)ND * g.nv * g.N, rhs,
+                                    (long)g.nv * g.N, g.nv, 4, st, q, fin.kolmogorov))
+        finished = true;
+      else if (!launch_deriv_line(g, ND - 1, SLOT_FIRST, fz, (long)ND * g.nv * g.N, rhs,
+                                  (long)g.nv * g.N, g.nv, 2, st))
         k_div<ND><<<grid, kBlock, 0, st>>>(g, ND - 1, F, rhs);
     } else if (F) {
       k_flux_all<ND><<<grid, kBlock, 0, st>>>(g, q, g.viscous ? G : nullptr, F);
@@ -1254,9 +1283,10 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
     }
     if (g.viscous && has_farfield)
       k_ff_visc_terms<ND><<<grid, kBlock, 0, st>>>(g, MGD_FORWARD, P, q, nullptr, G);
-    k_finish<ND><<<grid, kBlock, 0, st>>>(g, MGD_FORWARD, P, q, nullptr, G, rhs, control,
-                                          control_stride, nullptr, 0, 0.0, src, nsrc, src_amp,
-                                          src_inst_stride, nullptr);
+    if (!finished)
+      k_finish<ND><<<grid, kBlock, 0, st>>>(g, MGD_FORWARD, P, q, nullptr, G, rhs, control,
+                                            control_stride, nullptr, 0, 0.0, src, nsrc, src_amp,
+                                            src_inst_stride, nullptr);
   });
 }
 

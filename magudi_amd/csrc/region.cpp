@@ -80,6 +80,7 @@ struct mgd_region {
   mgd::Patches hpatches;
   mgd::DevBuf d_patches, d_sources;
   int ip_cost = -1, ip_act = -1, has_farfield = 0;
+  mgd::FinishInfo fin;                   // the forward finishing pass (set at finalize)
   // HBM-resident state
   mgd::DevBuf q, lam, rhs, b1, b2, b1a, b2a, t, w, G, T1, S;
   mgd::DevBuf qs, aforce, control, grad, acc_J, acc_S, partial, src_amp, init;
@@ -473,7 +474,7 @@ void forward_stage(mgd_region* r, double* q, double* b1, double* b2, int sidx, i
   launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p, ctrl, cstride,
                      reinterpret_cast<const Source*>(r->d_sources.p), ns,
                      r->src_amp.p + (long)sidx * std::max(ns, 1), r->src_inst_stride,
-                     r->has_farfield, r->stream);
+                     r->fin, r->stream);
   launch_rk_forward((long)r->ninst * M_of(r), stage, dt, q, b1, b2, r->rhs.p, qcopy, r->stream);
 }
 
@@ -1565,6 +1566,19 @@ int mgd_region_finalize(mgd_region* r) {
       D.ffv = P.d_ffv.p;
     }
   }
+  // the forward finishing pass: far field present; at most one Kolmogorov forcing over
+  // the whole domain and no other patch
+  r->fin.has_farfield = r->has_farfield;
+  r->fin.simple = 1;
+  r->fin.kolmogorov = nullptr;
+  for (int ip = 0; ip < r->hpatches.count; ++ip) {
+    const DPatch& D = r->hpatches.p[ip];
+    bool whole = D.type == MGD_PATCH_KOLMOGOROV && !r->fin.kolmogorov;
+    for (int d = 0; d < 3 && whole; ++d)
+      whole = D.lo[d] == 0 && D.hi[d] == (d < nd ? r->desc.n[d] - 1 : 0);
+    if (whole) r->fin.kolmogorov = D.strength;
+    else r->fin.simple = 0;
+  }
   if (r->d_patches.alloc(sizeof(Patches) / sizeof(double) + 1)) return MGD_ERR_HIP;
   MGD_HIP(hipMemcpy(r->d_patches.p, &r->hpatches, sizeof(Patches), hipMemcpyHostToDevice));
   if (r->d_sources.alloc((sizeof(Source) * std::max<size_t>(1, r->sources.size())) / sizeof(double) + 1))
@@ -1724,7 +1738,7 @@ int mgd_compute_rhs(mgd_region* r, int mode, double time, const double* control,
     }
     launch_rhs_forward(r->geo, dpatches(r), r->q.p, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p, ctrl, na,
                        reinterpret_cast<const Source*>(r->d_sources.p), ns, r->src_amp.p,
-                       r->src_inst_stride, r->has_farfield, r->stream);
+                       r->src_inst_stride, r->fin, r->stream);
   } else if (mode == MGD_ADJOINT) {
     if (int rc = ensure_adjoint(r)) return rc;
     const int nc = n_cost(r);
@@ -1939,7 +1953,7 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
         launch_rhs_forward(r->geo, dpatches(r), r->q.p, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p, nullptr,
                            n_act(r), reinterpret_cast<const Source*>(r->d_sources.p),
                            (int)r->sources.size(), r->src_amp.p, r->src_inst_stride,
-                           r->has_farfield, r->stream);
+                           r->fin, r->stream);
         break;
       case MGD_KERNEL_RHS_ADJOINT:
         launch_rhs_adjoint(r->geo, dpatches(r), r->q.p, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
