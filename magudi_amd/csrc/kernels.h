@@ -25,7 +25,7 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
 void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
                         double* rhs, double* w, double* G, double* t, double* T1, double* S,
                         const double* aforce, long aforce_stride, double forcing_factor,
-                        int has_farfield, hipStream_t st);
+                        const FinishInfo& fin, hipStream_t st);
 void launch_rk_forward(long M, int stage, double dt, double* q, double* b1, double* b2,
                        const double* rhs, double* qcopy, hipStream_t st);
 void launch_rk_adjoint(long M, int stage, double dt, double* lam, double* b1, double* b2,

@@ -433,20 +433,13 @@ __global__ void __launch_bounds__(kBlock) k_adj_jac_visc2(Geo g, const double* _
 }
 
 // Viscous adjoint chain rule at one point (src/RhsHelperImpl.f90:555-570 with sign -1
-// and :227-236 with sign +1): t2 = sum_j S_j (components 1..nv-1), applied to r.
+// and :227-236 with sign +1): t2 = sum_j S_j (components 1..nv-1), applied to r;
+// adj_chain_t2 takes t2 formed, v the conserved state at the point.
 template <int ND>
-__device__ __forceinline__ void adj_chain_point(const Geo& g, const double* qi, const double* Si,
-                                                long p, double sign, double* r) {
-  constexpr int NV = ND + 2, NW = ND + 1;
-  double v[NV];
-  load_q<ND>(qi, g.N, p, v);
+__device__ __forceinline__ void adj_chain_t2(const Geo& g, const double* v, double* t2, double sign,
+                                             double* r) {
+  constexpr int NV = ND + 2;
   const Prim<ND> s = prim<ND>(v, g.gamma);
-  double t2[NW];
-  for (int a = 0; a < NW; ++a) {
-    double x = Si[(1 + a) * g.N + p];
-    for (int j = 1; j < ND; ++j) x = x + Si[(j * NV + 1 + a) * g.N + p];
-    t2[a] = x;
-  }
   t2[ND] = g.gamma * s.sv * t2[ND];
   for (int i = 0; i < ND; ++i) t2[i] = s.sv * t2[i] - s.u[i] * t2[ND];
   for (int l = 1; l < NV; ++l) r[l] = sign > 0 ? r[l] + t2[l - 1] : r[l] - t2[l - 1];
@@ -454,6 +447,21 @@ __device__ __forceinline__ void adj_chain_point(const Geo& g, const double* qi, 
   for (int i = 1; i < ND; ++i) su = su + s.u[i] * t2[i];
   const double a = s.sv * v[ND + 1] * t2[ND];
   r[0] = sign > 0 ? r[0] - a - su : r[0] + a + su;
+}
+
+template <int ND>
+__device__ __forceinline__ void adj_chain_point(const Geo& g, const double* qi, const double* Si,
+                                                long p, double sign, double* r) {
+  constexpr int NV = ND + 2, NW = ND + 1;
+  double v[NV];
+  load_q<ND>(qi, g.N, p, v);
+  double t2[NW];
+  for (int a = 0; a < NW; ++a) {
+    double x = Si[(1 + a) * g.N + p];
+    for (int j = 1; j < ND; ++j) x = x + Si[(j * NV + 1 + a) * g.N + p];
+    t2[a] = x;
+  }
+  adj_chain_t2<ND>(g, v, t2, sign, r);
 }
 
 template <int ND>
@@ -1133,6 +1141,75 @@ __global__ void __launch_bounds__(kBlock) k_deriv_line(Geo g, int d, int slot,
   }
 }
 
+// The z pass of the adjoint's second derivative pass (D-dagger of the viscous block
+// T1[z] components 1..4) fused with k_finish's adjoint operations, for 3-D regions whose
+// patches are at most one Kolmogorov forcing over the whole domain, with no dissipation
+// and no far field (the chain rule then sits in the finishing pass).  One thread per
+// z-line as in k_deriv_line, the 9-point windows of the four components in registers.
+// At each point t2_a = A_a + (D-dagger_z T1)_a, where A = S_x + S_y was accumulated by
+// the x and y passes in S slot 0 (the sum k_finish forms, in its order); then the chain
+// rule, the Jacobian scaling and the Kolmogorov term, as k_finish applies them.
+__global__ void __launch_bounds__(kBlock) k_adj_finish_z(Geo g, const double* __restrict__ T1z,
+                                                        const double* __restrict__ A, long t_inst,
+                                                        const double* __restrict__ q,
+                                                        const double* __restrict__ lam,
+                                                        const double* __restrict__ kstr,
+                                                        double* __restrict__ rhs) {
+  constexpr int ND = 3, NV = 5, NW = 4;
+  const Dir dd = g.dir[2];
+  const long nlines = g.N / dd.len;
+  const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= nlines) return;
+  const long inst = blockIdx.z;
+  const long n0 = g.n[0];
+  const long i = line % n0, o = line / n0;
+  const long base = i + o * n0;
+  const long sd = dd.stride;
+  const double* x = T1z + inst * t_inst + base;
+  const double* ai = A + inst * t_inst + base;
+  const double* qi = q + inst * NV * g.N;
+  const double* li = lam + inst * NV * g.N;
+  double* ri = rhs + inst * NV * g.N;
+  const int z0 = blockIdx.y * kLineChunk;
+  const int z1 = min(z0 + kLineChunk, dd.len);
+  const Op& op = g.ops[SLOT_ADJ];
+  const double c1 = op.c[kMaxHW + 1], c2 = op.c[kMaxHW + 2], c3 = op.c[kMaxHW + 3],
+               c4 = op.c[kMaxHW + 4];
+  double Q[NW][9];
+#pragma unroll
+  for (int a = 0; a < NW; ++a)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) Q[a][k] = x[a * g.N + wrap(z0 - 4 + k, dd) * sd];
+  for (int z = z0; z < z1; ++z) {
+    const long p = base + z * sd;
+    double t2[NW];
+#pragma unroll
+    for (int a = 0; a < NW; ++a) {
+      Q[a][8] = x[a * g.N + wrap(z + 4, dd) * sd];
+      double out = 0.0;
+      out += c1 * (Q[a][5] - Q[a][3]);
+      out += c2 * (Q[a][6] - Q[a][2]);
+      out += c3 * (Q[a][7] - Q[a][1]);
+      out += c4 * (Q[a][8] - Q[a][0]);
+      t2[a] = ai[a * g.N + z * sd] + out;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) Q[a][k] = Q[a][k + 1];
+    }
+    double v[NV], r[NV];
+    load_q<ND>(qi, g.N, p, v);
+#pragma unroll
+    for (int l = 0; l < NV; ++l) r[l] = ri[l * g.N + p];
+    adj_chain_t2<ND>(g, v, t2, -1.0, r);
+    int ijk[3] = {(int)i, (int)o, z};
+    const double jac = jacobian<ND>(g, ijk);
+#pragma unroll
+    for (int l = 0; l < NV; ++l) r[l] = r[l] * jac;
+    if (kstr) r[0] = r[0] - li[g.N + p] * kstr[p];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) ri[l * g.N + p] = r[l];
+  }
+}
+
 // The same operator along a periodic x direction: a block stages 256 consecutive points
 // of one component -- a segment of one row, or whole rows when rows are shorter -- with
 // the 4-point wrap-around halo of each row segment in LDS, and applies the interior
@@ -1180,14 +1257,20 @@ __global__ void __launch_bounds__(kBlock) k_deriv_row(Geo g, int slot, const dou
 
 // Launches k_deriv_row (d = 0) or k_deriv_line (d >= 1) when direction d is periodic and
 // the grid fits the kernel; false otherwise (the caller then uses the generic kernel).
+// Whether k_deriv_row (d = 0) or k_deriv_line (d >= 1) serves direction d.
+static bool deriv_line_applies(const Geo& g, int d) {
+  if (!g.dir[d].periodic || g.dir[d].len < 16) return false;
+  if (d > 0) return true;
+  const int n0 = g.dir[0].len;
+  return ((n0 % kBlock == 0) || (kBlock % n0 == 0)) && g.N % kBlock == 0;
+}
+
 static bool launch_deriv_line(const Geo& g, int d, int slot, const double* src, long src_inst,
                               double* dst, long dst_inst, int ncomp, int mode, hipStream_t st,
                               const double* q = nullptr, const double* kstr = nullptr) {
-  if (!g.dir[d].periodic || g.dir[d].len < 16) return false;
+  if (!deriv_line_applies(g, d)) return false;
   if (mode == 4 && (d == 0 || g.nd != 3 || g.nv != 5 || ncomp != 5)) return false;
   if (d == 0) {
-    const int n0 = g.dir[0].len;
-    if (!((n0 % kBlock == 0) || (kBlock % n0 == 0)) || g.N % kBlock != 0) return false;
     const dim3 grid((unsigned)(g.N / kBlock), 1, (unsigned)(g.ninst * ncomp));
     k_deriv_row<<<grid, kBlock, 0, st>>>(g, slot, src, src_inst, dst, dst_inst, ncomp, mode);
     return true;
@@ -1293,8 +1376,10 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
 void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
                         double* rhs, double* w, double* G, double* t, double* T1, double* S,
                         const double* aforce, long aforce_stride, double forcing_factor,
-                        int has_farfield, hipStream_t st) {
+                        const FinishInfo& fin, hipStream_t st) {
   const dim3 grid = grid_pts(g);
+  const int has_farfield = fin.has_farfield;
+  bool finished = false;
   MGD_DISPATCH(g.nd, {
     constexpr int NV = ND + 2;
     if (g.viscous) launch_gradients<ND>(g, q, w, G, grid, st);
@@ -1307,7 +1392,23 @@ void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
       k_adj_jac_visc2<ND><<<grid, kBlock, 0, st>>>(g, q, G, T1, rhs);
     else
       k_adj_jac<ND><<<grid, kBlock, 0, st>>>(g, q, G, T1, rhs);
-    if (g.viscous) {
+    // the z pass of the second derivative pass also finishes the rhs where nothing but a
+    // whole-domain Kolmogorov forcing follows (the x and y passes accumulate in S slot 0)
+    const bool fuse = ND == 3 && g.viscous && chain_in_finish && fin.simple &&
+                      deriv_line_applies(g, 0) && deriv_line_applies(g, 1) &&
+                      deriv_line_applies(g, 2);
+    if (fuse) {
+      const long ts = (long)ND * NV * g.N;
+      launch_deriv_line(g, 0, SLOT_ADJ, T1 + (long)1 * g.N, ts, S + (long)1 * g.N, ts, NV - 1, 0, st);
+      launch_deriv_line(g, 1, SLOT_ADJ, T1 + (long)(NV + 1) * g.N, ts, S + (long)1 * g.N, ts, NV - 1,
+                        1, st);
+      const long nlines = g.N / g.dir[ND - 1].len;
+      const dim3 gz((unsigned)((nlines + kBlock - 1) / kBlock),
+                    (unsigned)((g.dir[ND - 1].len + kLineChunk - 1) / kLineChunk), (unsigned)g.ninst);
+      k_adj_finish_z<<<gz, kBlock, 0, st>>>(g, T1 + (long)((ND - 1) * NV + 1) * g.N, S + (long)1 * g.N,
+                                            ts, q, lam, fin.kolmogorov, rhs);
+      finished = true;
+    } else if (g.viscous) {
       for (int d = 0; d < ND; ++d)
         if (!launch_deriv_line(g, d, SLOT_ADJ, T1 + (long)(d * NV + 1) * g.N, (long)ND * NV * g.N,
                                S + (long)(d * NV + 1) * g.N, (long)ND * NV * g.N, NV - 1, 0, st))
@@ -1333,9 +1434,10 @@ void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
     }
     if (g.viscous && has_farfield)
       k_ff_visc_terms<ND><<<grid, kBlock, 0, st>>>(g, MGD_ADJOINT, P, q, lam, G);
-    k_finish<ND><<<grid, kBlock, 0, st>>>(g, MGD_ADJOINT, P, q, lam, G, rhs, nullptr, 0, aforce,
-                                          aforce_stride, forcing_factor, nullptr, 0, nullptr, 0,
-                                          chain_in_finish ? S : nullptr);
+    if (!finished)
+      k_finish<ND><<<grid, kBlock, 0, st>>>(g, MGD_ADJOINT, P, q, lam, G, rhs, nullptr, 0, aforce,
+                                            aforce_stride, forcing_factor, nullptr, 0, nullptr, 0,
+                                            chain_in_finish ? S : nullptr);
   });
 }
 

@@ -877,7 +877,7 @@ int plain_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
       const double factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
       launch_rhs_adjoint(r->geo, dpatches(r), qcur, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p,
                          r->S.p, nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc, factor,
-                         r->has_farfield, r->stream);
+                         r->fin, r->stream);
       launch_rk_adjoint(M, i, dt, r->lam.p, r->b1a.p, r->b2a.p, r->rhs.p, r->stream);
     }
   }
@@ -1744,7 +1744,7 @@ int mgd_compute_rhs(mgd_region* r, int mode, double time, const double* control,
     const int nc = n_cost(r);
     launch_rhs_adjoint(r->geo, dpatches(r), r->q.p, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
                        r->T1.p, r->S.p, nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc, factor,
-                       r->has_farfield, r->stream);
+                       r->fin, r->stream);
   } else {
     set_error("mode must be MGD_FORWARD or MGD_ADJOINT");
     return MGD_ERR_ARG;
@@ -1957,7 +1957,7 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
         break;
       case MGD_KERNEL_RHS_ADJOINT:
         launch_rhs_adjoint(r->geo, dpatches(r), r->q.p, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
-                           r->T1.p, r->S.p, nullptr, 0, 1.0, r->has_farfield, r->stream);
+                           r->T1.p, r->S.p, nullptr, 0, 1.0, r->fin, r->stream);
         break;
     }
   };

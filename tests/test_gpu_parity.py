@@ -148,6 +148,37 @@ def test_rhs_parity(solver_mod, name, mode, plain):
     assert maxrel(got, ref) < RHS_TOL, maxrel(got, ref)
 
 
+@pytest.mark.parametrize("mode", [1, -1])
+def test_fused_finish_equals_separate_pass(solver_mod, mode):
+    """3-D periodic rhs whose only patch is a whole-domain Kolmogorov forcing: the last
+    derivative kernel (forward: the z divergence; adjoint: the z pass of the viscous
+    block, with the chain rule) also finishes the rhs.  A sponge of amount 0 adds
+    nothing but forces the separate finishing pass; the two rhs fields must agree bit
+    for bit (same operations in the same order)."""
+    import dataclasses
+    c = periodic_box((32, 16, 16), dissipation=False)
+    c2 = dataclasses.replace(c, patches=list(c.patches) + [
+        cases.Patch("sponge.E", cases.SPONGE, 1, (1, 4, 1, -1, 1, -1), (0.0, 2.0, 0, 0))])
+    q = perturbed(c, 21)
+    lam = np.random.default_rng(22).standard_normal((c.nv, c.npoints))
+    out = []
+    for case in (c, c2):
+        R = solver_mod.Region(case)
+        R.states[0].conservedVariables = q
+        R.states[0].adjointVariables = lam
+        if mode == 1:
+            R.computeRhs(1, time=0.3)
+        else:
+            R.updateAdjointForcing()
+            R.computeRhs(-1, forcing_factor=0.5)
+        out.append(R.states[0].rightHandSide)
+    assert np.all(np.isfinite(out[0]))
+    assert np.array_equal(out[0], out[1])
+    if mode == -1:
+        ref = ob.Region(c).rhs(-1, q, lam, aforcing=None, factor=0.5)
+        assert maxrel(out[0], ref) < RHS_TOL
+
+
 def test_functional_parity(solver_mod):
     c = cases.acoustic_monopole(n=41, nsteps=1)
     R = solver_mod.Region(c)
