@@ -59,12 +59,16 @@ __global__ void __launch_bounds__(kBlock) k_grad(Geo g, int d, const double* __r
 // Gradients of w = (u_1..u_3, T) on a fully periodic 3-D grid in one pass: a BX x BY
 // column tile marches along z over a chunk of ZC planes.  Each thread keeps w of its
 // column for the 9 planes of the z stencil in registers; the plane being
-// differentiated is staged with a 4-point x / y halo in LDS.  Same sums as
-// k_prim + k_grad (interior SBP rows, wrap-around indices), so the same values.
+// differentiated is staged with a 4-point x / y halo in LDS, one halo point per thread
+// (threads below 8 (BX + BY)).  The halo point of the next plane is loaded one
+// iteration before it is used, so its latency overlaps a plane's stencil work.
+// Same sums as k_prim + k_grad (interior SBP rows, wrap-around indices), so the same
+// values.
 template <int BX, int BY, int ZC>
-__global__ void __launch_bounds__(BX * BY) k_grad3d_periodic(Geo g, const double* __restrict__ q,
+__global__ void __launch_bounds__(BX * BY, 4) k_grad3d_periodic(Geo g, const double* __restrict__ q,
                                                               double* __restrict__ G) {
-  constexpr int NT = BX * BY, EXW = BX + 8, EYW = BY + 8, NW = 4;
+  constexpr int NT = BX * BY, EXW = BX + 8, EYW = BY + 8, NW = 4, NH = 8 * (BX + BY);
+  static_assert(NH <= NT, "one halo point per thread");
   __shared__ double sw[NW][EYW][EXW];
   const int ntx = g.n[0] / BX;
   const int x0 = (blockIdx.x % ntx) * BX, y0 = (blockIdx.x / ntx) * BY;
@@ -76,33 +80,56 @@ __global__ void __launch_bounds__(BX * BY) k_grad3d_periodic(Geo g, const double
   const int i = x0 + tx, j = y0 + ty;
   const Dir dx = g.dir[0], dy = g.dir[1], dz = g.dir[2];
   const long n01 = (long)g.n[0] * g.n[1];
-  auto load_w = [&](int ii, int jj, int kk, double* w) {
+  auto load_v = [&](int ii, int jj, int kk, double* v) {
     const long p = ii + (long)g.n[0] * jj + n01 * kk;
-    double v[5];
+#pragma unroll
     for (int l = 0; l < 5; ++l) v[l] = qi[l * g.N + p];
+  };
+  auto to_w = [&](const double* v, double* w) {
     const Prim<3> s = prim<3>(v, g.gamma);
     w[0] = s.u[0]; w[1] = s.u[1]; w[2] = s.u[2]; w[3] = s.T;
   };
+  // this thread's halo point: y halo rows over the tile columns, then x halo columns over
+  // the tile rows (the corners are never read)
+  const int t = threadIdx.x;
+  const bool has_halo = t < NH;
+  int hx = 0, hy = 0;
+  if (t < 8 * BX) {
+    const int r = t / BX;
+    hy = r < 4 ? r : BY + r;
+    hx = 4 + t % BX;
+  } else if (has_halo) {
+    const int t2 = t - 8 * BX, c = t2 % 8;
+    hy = 4 + t2 / 8;
+    hx = c < 4 ? c : BX + c;
+  }
+  const int hi = wrap(x0 + hx - 4, dx), hj = wrap(y0 + hy - 4, dy);
   double Q[9][NW];
+  double v[5];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) load_w(i, j, wrap(z0 - 4 + k, dz), Q[k]);
+  for (int k = 0; k < 8; ++k) {
+    load_v(i, j, wrap(z0 - 4 + k, dz), v);
+    to_w(v, Q[k]);
+  }
+  double hv[5];
+  if (has_halo) load_v(hi, hj, z0 % dz.len, hv);   // halo, plane z0
   const Op& op = g.ops[SLOT_FIRST];
   const double c1 = op.c[kMaxHW + 1], c2 = op.c[kMaxHW + 2], c3 = op.c[kMaxHW + 3], c4 = op.c[kMaxHW + 4];
   for (int s = 0; s < ZC; ++s) {
     const int zc = z0 + s;
     if (zc >= g.n[2]) break;   // uniform over the block
-    load_w(i, j, wrap(zc + 4, dz), Q[8]);
+    load_v(i, j, wrap(zc + 4, dz), v);
+    to_w(v, Q[8]);
+    double hw[NW];
+    if (has_halo) to_w(hv, hw);
+    // next plane's halo point (the loads stay in flight through this plane's work)
+    if (has_halo && s + 1 < ZC && zc + 1 < g.n[2]) load_v(hi, hj, zc + 1, hv);
     __syncthreads();            // readers of the previous plane are done
 #pragma unroll
     for (int c = 0; c < NW; ++c) sw[c][ty + 4][tx + 4] = Q[4][c];
-    for (int e = threadIdx.x; e < EXW * EYW; e += NT) {
-      const int ex = e % EXW, ey = e / EXW;
-      const bool inx = ex >= 4 && ex < BX + 4, iny = ey >= 4 && ey < BY + 4;
-      if (inx == iny) continue;   // interior (from the queues) or corner (unused)
-      double w[NW];
-      load_w(wrap(x0 + ex - 4, dx), wrap(y0 + ey - 4, dy), zc, w);
+    if (has_halo) {
 #pragma unroll
-      for (int c = 0; c < NW; ++c) sw[c][ey][ex] = w[c];
+      for (int c = 0; c < NW; ++c) sw[c][hy][hx] = hw[c];
     }
     __syncthreads();
     int ijk[3] = {i, j, zc};
