@@ -1135,8 +1135,17 @@ __global__ void __launch_bounds__(kBlock) k_deriv_line(Geo g, int d, int slot,
   double Q[9];
 #pragma unroll
   for (int k = 0; k < 8; ++k) Q[k] = x[wrap(z0 - 4 + k, dd) * sd];
+  // the operand entering the window and the dst value of the next point are loaded one
+  // point ahead, so their latency overlaps this point's work
+  double nx = x[wrap(z0 + 4, dd) * sd];
+  double ny = mode != 0 ? y[z0 * sd] : 0.0;
   for (int z = z0; z < z1; ++z) {
-    Q[8] = x[wrap(z + 4, dd) * sd];
+    Q[8] = nx;
+    const double yv = ny;
+    if (z + 1 < z1) {
+      nx = x[wrap(z + 5, dd) * sd];
+      if (mode != 0) ny = y[(z + 1) * sd];
+    }
     double out = 0.0;
     out += c1 * (Q[5] - Q[3]);
     out += c2 * (Q[6] - Q[2]);
@@ -1146,7 +1155,7 @@ __global__ void __launch_bounds__(kBlock) k_deriv_line(Geo g, int d, int slot,
     if (mode == 0) {
       *yz = out;
     } else {
-      double v = *yz + out;
+      double v = yv + out;
       if (mode >= 2) v = 0.0 - v;
       if (mode == 4) {
         // k_finish of a forward rhs whose patches are at most one Kolmogorov forcing over
