@@ -228,8 +228,8 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
-              "(instances shard across GPUs without communication; see DESIGN.md)",
-              file=sys.stderr)
+              "(batched instances and ensemble members would shard across GPUs, one "
+              "process per GPU, without communication)", file=sys.stderr)
         return 2
     from magudi_amd import capi, cases
     from magudi_amd.solver import Region, Solver
