@@ -334,63 +334,75 @@ __device__ __forceinline__ double d4_at(const Dir& dd, int i, double c0, double 
 
 // Slots of diss_intermediates_to per thread, and the intermediates of one thread held in
 // registers (so that they can be formed before the flux space they are stored over is
-// free).
+// free).  With one thread per point the x positions (plane PX) and then the y positions
+// (plane PY) are dealt to the threads together, slot it of lane holding position
+// lane + it NP: every thread forms at most NS of them (3 for 16 x 16 tiles, where
+// dealing the planes one after the other gives the first wave 4 and the others 2).  With
+// component groups (CS > 1) the planes are dealt one after the other (slots 0 .. IX - 1
+// for x, IX .. for y), which keeps their register budget.
 template <class S>
 struct DissSlots {
-  static constexpr int IX = (S::TY * (S::TX + 4) + S::NP - 1) / S::NP;
-  static constexpr int IY = ((S::TY + 4) * S::TX + S::NP - 1) / S::NP;
   static constexpr int PX = S::TY * (S::TX + 4), PY = (S::TY + 4) * S::TX;   // planes
-  double vx[IX][S::NC], vy[IY][S::NC];
-  int bx[IX], by[IY];   // LDS base index, or -1
+  static constexpr bool kJoint = S::CS == 1;
+  static constexpr int IX = (PX + S::NP - 1) / S::NP, IY = (PY + S::NP - 1) / S::NP;
+  static constexpr int NS = kJoint ? (PX + PY + S::NP - 1) / S::NP : IX + IY;
+  double v[NS][S::NC];
+  int base[NS];   // LDS index in its plane, or -1
+  bool y[NS];     // position of the y plane
 };
 
 // t = -arc D4 x on the tile rows (x direction, columns x0-2 .. x0+sx+1) and tile columns
 // (y direction, rows y0-2 .. y0+sy+1), for staged points inside the domain, components
 // c0 .. c0 + NC - 1; the positions are dealt to the NP threads of the component group
-// (lane = index within the group).  Each value goes to sink_x / sink_y (it, j, base, v):
-// slot `it` of this thread, component c0 + j, LDS index base + (c0 + j) * plane.
+// (lane = index within the group) as DissSlots describes.  Each value goes to
+// sink_x / sink_y (it, j, base, v): slot `it` of this thread, component c0 + j, LDS
+// index base + (c0 + j) * plane of the x or y plane.
 template <class S, class SinkX, class SinkY>
 __device__ __forceinline__ void diss_intermediates_to(const Geo& g, const TileGeo& t,
                                                       const Coefs& cf, const TileGrid<S>& tg,
                                                       const double* sq, int lane, int c0,
                                                       SinkX&& sink_x, SinkY&& sink_y) {
-  constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NP = S::NP, NC = S::NC;
+  constexpr int TX = S::TX, EX = S::EX, EY = S::EY, NP = S::NP, NC = S::NC;
+  constexpr int PX = DissSlots<S>::PX, PY = DissSlots<S>::PY;
   const double k0 = cf.dd[0], k1 = cf.dd[1], k2 = cf.dd[2];
   const Dir dx = g.dir[0], dy = g.dir[1];
-  const int wx = t.sx + 4;
+  const int wx = t.sx + 4, hy = t.sy + 4;
+  constexpr bool kJoint = DissSlots<S>::kJoint;
+  constexpr int IX = DissSlots<S>::IX;
 #pragma unroll
-  for (int it = 0; it < DissSlots<S>::IX; ++it) {
-    const int e = lane + it * NP;
-    if (e >= TY * (TX + 4)) continue;
-    const int ex = e % (TX + 4) + 2, ey = e / (TX + 4) + HY;
-    if (ex - 2 >= wx || ey - HY >= t.sy) continue;
-    const int gi = t.ex0 + ex;
-    if (!dx.periodic && (gi < 0 || gi >= dx.len)) continue;
-    const double arc = fabs(tg.h1(ey - HY));
+  for (int it = 0; it < DissSlots<S>::NS; ++it) {
+    // joint: position lane + it NP of the concatenated planes; else x slots, then y
+    const int e = kJoint ? lane + it * NP : it < IX ? lane + it * NP : PX + lane + (it - IX) * NP;
+    if (e >= PX + PY || (!kJoint && it < IX && e >= PX)) continue;
+    // slot it holds positions it NP .. it NP + NP - 1: only a slot that straddles the
+    // end of the x plane needs both branches (it is a constant once unrolled)
+    if (kJoint ? ((it + 1) * NP <= PX || (it * NP < PX && e < PX)) : it < IX) {
+      const int ex = e % (TX + 4) + 2, ey = e / (TX + 4) + HY;
+      if (ex - 2 >= wx || ey - HY >= t.sy) continue;
+      const int gi = t.ex0 + ex;
+      if (!dx.periodic && (gi < 0 || gi >= dx.len)) continue;
+      const double arc = fabs(tg.h1(ey - HY));
 #pragma unroll
-    for (int j = 0; j < NC; ++j) {
-      const int l = c0 + j;
-      const double* r = sq + (l * EY + ey) * EX;
-      const double v = d4_at(dx, gi, k0, k1, k2, [&](int k) { return r[k - t.ex0]; });
-      sink_x(it, j, (ey - HY) * (TX + 4) + (ex - 2), -arc * v);
-    }
-  }
-  const int hy = t.sy + 4;
+      for (int j = 0; j < NC; ++j) {
+        const int l = c0 + j;
+        const double* r = sq + (l * EY + ey) * EX;
+        const double v = d4_at(dx, gi, k0, k1, k2, [&](int k) { return r[k - t.ex0]; });
+        sink_x(it, j, (ey - HY) * (TX + 4) + (ex - 2), -arc * v);
+      }
+    } else {
+      const int e2 = e - PX;
+      const int ex = e2 % TX + HX, ey = e2 / TX + 2;
+      if (ex - HX >= t.sx || ey - 2 >= hy) continue;
+      const int gj = t.ey0 + ey;
+      if (!dy.periodic && (gj < 0 || gj >= dy.len)) continue;
+      const double arc = fabs(tg.h0(ex - HX));
 #pragma unroll
-  for (int it = 0; it < DissSlots<S>::IY; ++it) {
-    const int e = lane + it * NP;
-    if (e >= (TY + 4) * TX) continue;
-    const int ex = e % TX + HX, ey = e / TX + 2;
-    if (ex - HX >= t.sx || ey - 2 >= hy) continue;
-    const int gj = t.ey0 + ey;
-    if (!dy.periodic && (gj < 0 || gj >= dy.len)) continue;
-    const double arc = fabs(tg.h0(ex - HX));
-#pragma unroll
-    for (int j = 0; j < NC; ++j) {
-      const int l = c0 + j;
-      const double* c = sq + l * EY * EX + ex;
-      const double v = d4_at(dy, gj, k0, k1, k2, [&](int k) { return c[(k - t.ey0) * EX]; });
-      sink_y(it, j, (ey - 2) * TX + (ex - HX), -arc * v);
+      for (int j = 0; j < NC; ++j) {
+        const int l = c0 + j;
+        const double* c = sq + l * EY * EX + ex;
+        const double v = d4_at(dy, gj, k0, k1, k2, [&](int k) { return c[(k - t.ey0) * EX]; });
+        sink_y(it, j, (ey - 2) * TX + (ex - HX), -arc * v);
+      }
     }
   }
 }
@@ -412,28 +424,28 @@ __device__ __forceinline__ void diss_compute(const Geo& g, const TileGeo& t, con
                                              const TileGrid<S>& tg, const double* sq, int lane,
                                              int c0, DissSlots<S>& d) {
 #pragma unroll
-  for (int it = 0; it < DissSlots<S>::IX; ++it) d.bx[it] = -1;
-#pragma unroll
-  for (int it = 0; it < DissSlots<S>::IY; ++it) d.by[it] = -1;
+  for (int it = 0; it < DissSlots<S>::NS; ++it) {
+    d.base[it] = -1;
+    d.y[it] = false;
+  }
   diss_intermediates_to<S>(
       g, t, cf, tg, sq, lane, c0,
-      [&](int it, int j, int base, double v) { d.bx[it] = base; d.vx[it][j] = v; },
-      [&](int it, int j, int base, double v) { d.by[it] = base; d.vy[it][j] = v; });
+      [&](int it, int j, int base, double v) { d.base[it] = base; d.y[it] = false; d.v[it][j] = v; },
+      [&](int it, int j, int base, double v) { d.base[it] = base; d.y[it] = true; d.v[it][j] = v; });
 }
 
 template <class S>
 __device__ __forceinline__ void diss_store(const DissSlots<S>& d, int c0, double* stx, double* sty) {
 #pragma unroll
-  for (int it = 0; it < DissSlots<S>::IX; ++it) {
-    if (d.bx[it] < 0) continue;
+  for (int it = 0; it < DissSlots<S>::NS; ++it) {
+    if (d.base[it] < 0) continue;
+    if (d.y[it]) {
 #pragma unroll
-    for (int j = 0; j < S::NC; ++j) stx[(c0 + j) * DissSlots<S>::PX + d.bx[it]] = d.vx[it][j];
-  }
+      for (int j = 0; j < S::NC; ++j) sty[(c0 + j) * DissSlots<S>::PY + d.base[it]] = d.v[it][j];
+    } else {
 #pragma unroll
-  for (int it = 0; it < DissSlots<S>::IY; ++it) {
-    if (d.by[it] < 0) continue;
-#pragma unroll
-    for (int j = 0; j < S::NC; ++j) sty[(c0 + j) * DissSlots<S>::PY + d.by[it]] = d.vy[it][j];
+      for (int j = 0; j < S::NC; ++j) stx[(c0 + j) * DissSlots<S>::PX + d.base[it]] = d.v[it][j];
+    }
   }
 }
 
