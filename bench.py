@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=64)
     ap.add_argument("--c4-n", type=int, default=256,
                     help="grid of the 3-D viscous Kolmogorov rhs leg (0: skip)")
+    ap.add_argument("--kolmogorov-only", action="store_true",
+                    help="only the 3-D Kolmogorov forward + adjoint run (c4-n^3, 200 steps)")
     ap.add_argument("--viscous3d-only", action="store_true",
                     help="only the 3-D viscous rhs leg (its profiling workload)")
     ap.add_argument("--tile-shape", type=int, default=-1,
@@ -122,6 +124,47 @@ def viscous3d_leg(n):
         leg[k] = {"ms": round(ms, 4), "algorithmic_bytes": b, "achieved_GBs": round(gbs, 1),
                   "frac": round(gbs / HBM_PEAK_GBS, 4), "pt_per_s": n ** 3 / (ms * 1e-3)}
     return leg
+
+
+def kolmogorov_run_leg(n, nsteps=200):
+    """The 3-D viscous periodic Kolmogorov flow as the configuration runs it: nsteps RK4
+    steps forward with HBM snapshots (every nsteps / 8), J = kinetic energy of the final
+    state, then the discrete adjoint back from dJ/dq(T) with window recompute.  A short
+    run of one window first allocates the buffers; then one timed run."""
+    from magudi_amd import cases
+    from magudi_amd.functionals import kinetic_energy
+    from magudi_amd.solver import Region, Solver
+    c = cases.kolmogorov3d(n=n, nsteps=nsteps)
+    q0 = c.initial_state()
+    R = Region(c)
+    norm = R.grid(2)[0]
+
+    def run(steps):
+        S = Solver(R, dt=c.dt, nsteps=steps, save_interval=c.save_interval)
+        R.states[0].conservedVariables = q0
+        R.sync()
+        t0 = time.perf_counter()
+        S.runForwardCheckpointed()
+        R.sync()
+        t1 = time.perf_counter()
+        J, lamT = kinetic_energy(R.states[0].conservedVariables, norm, 3)
+        t2 = time.perf_counter()
+        S.runAdjointCheckpointed(terminal=lamT, want_gradient=False)
+        R.sync()
+        t3 = time.perf_counter()
+        return J, t1 - t0, t3 - t2, t2 - t1
+
+    run(c.save_interval)
+    J, tf, ta, th = run(nsteps)
+    lam_max = float(abs(R.states[0].adjointVariables).max())
+    del R
+    t = tf + ta
+    return {"workload": f"Kolmogorov {n}^3 viscous periodic, {nsteps} RK4 steps forward + "
+                        f"adjoint from the terminal kinetic energy, snapshots every "
+                        f"{c.save_interval} steps with recompute",
+            "value": n ** 3 * nsteps / t, "unit": "grid-pt-steps/s", "seconds": round(t, 3),
+            "forward_s": round(tf, 3), "adjoint_s": round(ta, 3),
+            "host_terminal_s": round(th, 3), "J": J, "max_abs_lambda0": lam_max}
 
 
 def small_grid_leg(n=201, nsteps=96, save_interval=32, batch=20):
@@ -237,6 +280,9 @@ def main():
     if a.viscous3d_only:
         print(json.dumps({"viscous3d": viscous3d_leg(a.c4_n)}), flush=True)
         return 0
+    if a.kolmogorov_only:
+        print(json.dumps({"kolmogorov_run": kolmogorov_run_leg(a.c4_n)}), flush=True)
+        return 0
     case = cases.acoustic_monopole(n=a.n, nsteps=a.nsteps, save_interval=a.save_interval)
     q0 = case.initial_state()
     R = Region(case, tile_shape=a.tile_shape)
@@ -349,6 +395,7 @@ def main():
         del Rr, Sr
         if a.c4_n > 0:
             out["viscous3d"] = viscous3d_leg(a.c4_n)
+            out["kolmogorov_run"] = kolmogorov_run_leg(a.c4_n)
         out["uneven_batch"] = uneven_leg()
         out["small_ensemble"] = small_ensemble_leg()
         out["small_grid"] = small_grid_leg()

@@ -240,3 +240,38 @@ class Solver:
                                          dptr(ctl), dptr(grad), dptr(sens), dptr(J),
                                          int(use_graph)))
         return sens, grad, J
+
+    def runForwardCheckpointed(self, control: Optional[np.ndarray] = None,
+                               t0: float = 0.0) -> np.ndarray:
+        """First half of Solver%runAdjoint (src/SolverImpl.f90:914-1060): the forward run
+        from the current conserved state with HBM snapshots every saveInterval steps.
+        Returns J (the time-integrated functional) per instance; afterwards the
+        conserved state is the final state."""
+        r = self.region
+        J = np.zeros(r.n_instances)
+        ctl = None if control is None else np.ascontiguousarray(control, dtype=np.float64)
+        check(capi.lib().mgd_run_forward_checkpointed(r.handle, t0, self.dt, self.nTimesteps,
+                                                      self.saveInterval, dptr(ctl), dptr(J)))
+        return J
+
+    def runAdjointCheckpointed(self, terminal: Optional[np.ndarray] = None,
+                               want_gradient: bool = True):
+        """Second half of Solver%runAdjoint: the discrete adjoint marched back from the
+        snapshots of the last runForwardCheckpointed.  ``terminal`` is the adjoint state at
+        the final time per instance, shape (n_instances, nv, N) or (nv, N) for one
+        instance -- the reference's adjoint_nonzero_initial_condition
+        (src/SolverImpl.f90:368-428, 1096-1102), e.g. the gradient of a cost on the final
+        state -- or None for a zero terminal state.  Returns (sensitivity, gradient) as
+        runAdjoint does; afterwards each instance's adjoint state is the adjoint at t0."""
+        r = self.region
+        if terminal is not None:
+            t = np.asarray(terminal, dtype=np.float64).reshape(r.n_instances, r.nv, r.npoints)
+            for i in range(r.n_instances):
+                r.states[i].adjointVariables = t[i]
+        sens = np.zeros(r.n_instances)
+        grad = None
+        if want_gradient and r.n_actuator > 0:
+            grad = np.zeros((r.n_instances, 4 * self.nTimesteps, r.n_actuator))
+        check(capi.lib().mgd_run_adjoint_checkpointed(r.handle, 0 if terminal is None else 1,
+                                                      dptr(grad), dptr(sens)))
+        return sens, grad
