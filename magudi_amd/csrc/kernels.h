@@ -22,10 +22,22 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
                         double* G, double* t, double* F, const double* control, long control_stride,
                         const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
                         const FinishInfo& fin, hipStream_t st);
-void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
+// One RK4 adjoint stage update (substepAdjointRK4, src/RK4IntegratorImpl.f90:215-261)
+// that launch_rhs_adjoint may apply itself: y the adjoint state updated in place, b1 / b2
+// the RK buffers.
+struct RkStage {
+  int stage;
+  double dt;
+  double* y;
+  double* b1;
+  double* b2;
+};
+// Returns true when it applied `rk` (then rhs is not written); false when the caller
+// still has to (launch_rk_adjoint) -- always false for rk == nullptr.
+bool launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
                         double* rhs, double* w, double* G, double* t, double* T1, double* S,
                         const double* aforce, long aforce_stride, double forcing_factor,
-                        const FinishInfo& fin, hipStream_t st);
+                        const FinishInfo& fin, hipStream_t st, const RkStage* rk = nullptr);
 void launch_rk_forward(long M, int stage, double dt, double* q, double* b1, double* b2,
                        const double* rhs, double* qcopy, hipStream_t st);
 void launch_rk_adjoint(long M, int stage, double dt, double* lam, double* b1, double* b2,

@@ -1184,13 +1184,18 @@ __global__ void __launch_bounds__(kBlock) k_deriv_line(Geo g, int d, int slot,
 // z-line as in k_deriv_line, the 9-point windows of the four components in registers.
 // At each point t2_a = A_a + (D-dagger_z T1)_a, where A = S_x + S_y was accumulated by
 // the x and y passes in S slot 0 (the sum k_finish forms, in its order); then the chain
-// rule, the Jacobian scaling and the Kolmogorov term, as k_finish applies them.
+// rule, the Jacobian scaling and the Kolmogorov term, as k_finish applies them.  With
+// rk_stage > 0 the point's RK4 adjoint update follows at once (k_rk_adjoint's
+// arithmetic; lam is then y, read at the point before it is written) and rhs is not
+// written.
 __global__ void __launch_bounds__(kBlock) k_adj_finish_z(Geo g, const double* __restrict__ T1z,
                                                         const double* __restrict__ A, long t_inst,
                                                         const double* __restrict__ q,
-                                                        const double* __restrict__ lam,
+                                                        const double* lam,
                                                         const double* __restrict__ kstr,
-                                                        double* __restrict__ rhs) {
+                                                        double* __restrict__ rhs, int rk_stage,
+                                                        double dt, double* y, double* b1,
+                                                        double* b2) {
   constexpr int ND = 3, NV = 5, NW = 4;
   const Dir dd = g.dir[2];
   const long nlines = g.N / dd.len;
@@ -1241,8 +1246,36 @@ __global__ void __launch_bounds__(kBlock) k_adj_finish_z(Geo g, const double* __
 #pragma unroll
     for (int l = 0; l < NV; ++l) r[l] = r[l] * jac;
     if (kstr) r[0] = r[0] - li[g.N + p] * kstr[p];
+    if (rk_stage == 0) {
 #pragma unroll
-    for (int l = 0; l < NV; ++l) ri[l * g.N + p] = r[l];
+      for (int l = 0; l < NV; ++l) ri[l * g.N + p] = r[l];
+    } else {
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        const long ii = inst * NV * g.N + l * g.N + p;
+        const double rl = r[l];
+        switch (rk_stage) {
+          case 4: {
+            const double l0 = y[ii];
+            b1[ii] = l0;
+            b2[ii] = l0 - div6(dt * rl);
+            y[ii] = l0 - dt * rl / 2.0;
+            break;
+          }
+          case 3:
+            b2[ii] = b2[ii] - div3(dt * rl);
+            y[ii] = b1[ii] - dt * rl / 2.0;
+            break;
+          case 2:
+            b2[ii] = b2[ii] - div3(dt * rl);
+            y[ii] = b1[ii] - dt * rl;
+            break;
+          default:
+            y[ii] = b2[ii] - div6(dt * rl);
+            break;
+        }
+      }
+    }
   }
 }
 
@@ -1409,10 +1442,10 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
   });
 }
 
-void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
+bool launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
                         double* rhs, double* w, double* G, double* t, double* T1, double* S,
                         const double* aforce, long aforce_stride, double forcing_factor,
-                        const FinishInfo& fin, hipStream_t st) {
+                        const FinishInfo& fin, hipStream_t st, const RkStage* rk) {
   const dim3 grid = grid_pts(g);
   const int has_farfield = fin.has_farfield;
   bool finished = false;
@@ -1442,7 +1475,9 @@ void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
       const dim3 gz((unsigned)((nlines + kBlock - 1) / kBlock),
                     (unsigned)((g.dir[ND - 1].len + kLineChunk - 1) / kLineChunk), (unsigned)g.ninst);
       k_adj_finish_z<<<gz, kBlock, 0, st>>>(g, T1 + (long)((ND - 1) * NV + 1) * g.N, S + (long)1 * g.N,
-                                            ts, q, lam, fin.kolmogorov, rhs);
+                                            ts, q, lam, fin.kolmogorov, rhs, rk ? rk->stage : 0,
+                                            rk ? rk->dt : 0.0, rk ? rk->y : nullptr,
+                                            rk ? rk->b1 : nullptr, rk ? rk->b2 : nullptr);
       finished = true;
     } else if (g.viscous) {
       for (int d = 0; d < ND; ++d)
@@ -1475,6 +1510,7 @@ void launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
                                             aforce_stride, forcing_factor, nullptr, 0, nullptr, 0,
                                             chain_in_finish ? S : nullptr);
   });
+  return finished && rk != nullptr;
 }
 
 void launch_rk_forward(long M, int stage, double dt, double* q, double* b1, double* b2,

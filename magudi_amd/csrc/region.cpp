@@ -875,10 +875,12 @@ int plain_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
         launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, qcur, r->aforce.p, t == 0 && i == 1,
                            r->stream);
       const double factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
-      launch_rhs_adjoint(r->geo, dpatches(r), qcur, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p,
-                         r->S.p, nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc, factor,
-                         r->fin, r->stream);
-      launch_rk_adjoint(M, i, dt, r->lam.p, r->b1a.p, r->b2a.p, r->rhs.p, r->stream);
+      // the rhs launcher applies the RK update itself where its last kernel finishes the rhs
+      const RkStage rk{i, dt, r->lam.p, r->b1a.p, r->b2a.p};
+      if (!launch_rhs_adjoint(r->geo, dpatches(r), qcur, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
+                              r->T1.p, r->S.p, nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc,
+                              factor, r->fin, r->stream, &rk))
+        launch_rk_adjoint(M, i, dt, r->lam.p, r->b1a.p, r->b2a.p, r->rhs.p, r->stream);
     }
   }
   return MGD_OK;
