@@ -18,20 +18,24 @@ struct FinishInfo {
   int simple = 0;
   const double* kolmogorov = nullptr;
 };
-void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
-                        double* G, double* t, double* F, const double* control, long control_stride,
-                        const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
-                        const FinishInfo& fin, hipStream_t st);
-// One RK4 adjoint stage update (substepAdjointRK4, src/RK4IntegratorImpl.f90:215-261)
-// that launch_rhs_adjoint may apply itself: y the adjoint state updated in place, b1 / b2
-// the RK buffers.
+// One RK4 stage update that a rhs launcher may apply itself (substepForwardRK4 /
+// substepAdjointRK4, src/RK4IntegratorImpl.f90:115-156, 215-261): y the state updated in
+// place, b1 / b2 the RK buffers, copy (forward, optional) a second destination of the
+// new stage state.
 struct RkStage {
   int stage;
   double dt;
   double* y;
   double* b1;
   double* b2;
+  double* copy;
 };
+// Returns true when it applied `rk` (then rhs is not written); false when the caller
+// still has to (launch_rk_forward) -- always false for rk == nullptr.
+bool launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
+                        double* G, double* t, double* F, const double* control, long control_stride,
+                        const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
+                        const FinishInfo& fin, hipStream_t st, const RkStage* rk = nullptr);
 // Returns true when it applied `rk` (then rhs is not written); false when the caller
 // still has to (launch_rk_adjoint) -- always false for rk == nullptr.
 bool launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,

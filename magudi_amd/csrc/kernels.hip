@@ -1103,18 +1103,13 @@ __global__ void __launch_bounds__(BX * BY) k_fluxdiv3d_xy(Geo g, const double* _
 // keeping the 9-point window of one component in registers -- each operand value is
 // read once per chunk instead of once per stencil tap.  Component c of instance i reads
 // src[i * src_inst + c * N] and writes dst[i * dst_inst + c * N]; mode 0 stores the
-// result, 1 adds it to dst, 2 stores 0 - (dst + result), 4 (3-D, five components) also
-// applies k_finish's forward operations where the patches are at most one Kolmogorov
-// forcing over the whole domain (strength kstr, state q) -- the same operations in the
-// same order.  The sums are those of sbp_apply's interior rows (same order), so the
-// values are the same.
+// result, 1 adds it to dst, 2 stores 0 - (dst + result).  The sums are those of
+// sbp_apply's interior rows (same order), so the values are the same.
 constexpr int kLineChunk = 32;
 __global__ void __launch_bounds__(kBlock) k_deriv_line(Geo g, int d, int slot,
                                                       const double* __restrict__ src, long src_inst,
                                                       double* __restrict__ dst, long dst_inst,
-                                                      int ncomp, int mode,
-                                                      const double* __restrict__ q,
-                                                      const double* __restrict__ kstr) {
+                                                      int ncomp, int mode) {
   const Dir dd = g.dir[d];
   const long nlines = g.N / dd.len;
   const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1156,24 +1151,106 @@ __global__ void __launch_bounds__(kBlock) k_deriv_line(Geo g, int d, int slot,
       *yz = out;
     } else {
       double v = yv + out;
-      if (mode >= 2) v = 0.0 - v;
-      if (mode == 4) {
-        // k_finish of a forward rhs whose patches are at most one Kolmogorov forcing over
-        // the whole domain: Jacobian scaling, then the forcing of the x momentum
-        int ijk[3];
-        ijk[0] = (int)i;
-        ijk[d] = z;
-        ijk[3 - d] = (int)o;
-        v = v * jacobian<3>(g, ijk);
-        if (c == 1 && kstr) {
-          const long pz = base + z * sd;
-          v = v + q[(long)inst * 5 * g.N + pz] * kstr[pz];
-        }
-      }
+      if (mode == 2) v = 0.0 - v;
       *yz = v;
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) Q[k] = Q[k + 1];
+  }
+}
+
+// The z divergence of a forward rhs fused with k_finish's forward operations, for 3-D
+// regions whose patches are at most one Kolmogorov forcing over the whole domain, with
+// no sources and no dissipation.  One thread per z-line as in k_deriv_line, the 9-point
+// windows of the five flux components in registers; at each point
+// rhs = 0 - (x/y divergence + D_z Fz) (k_deriv_line's add-negate), then the Jacobian
+// scaling and the forcing of the x momentum, in k_finish's order.  With rk_stage > 0 the
+// point's RK4 stage update follows (k_rk_forward's arithmetic, y = q updated in place,
+// copy optional), after all five components are formed since the forcing reads the
+// density at the point, and rhs is not written.
+__global__ void __launch_bounds__(kBlock) k_fwd_finish_z(Geo g, const double* __restrict__ Fz,
+                                                        long f_inst, const double* q,
+                                                        const double* __restrict__ kstr,
+                                                        double* __restrict__ rhs, int rk_stage,
+                                                        double dt, double* y, double* b1,
+                                                        double* b2, double* copy) {
+  constexpr int ND = 3, NV = 5;
+  const Dir dd = g.dir[2];
+  const long nlines = g.N / dd.len;
+  const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= nlines) return;
+  const long inst = blockIdx.z;
+  const long n0 = g.n[0];
+  const long i = line % n0, o = line / n0;
+  const long base = i + o * n0;
+  const long sd = dd.stride;
+  const double* x = Fz + inst * f_inst + base;
+  const double* qi = q + inst * NV * g.N;
+  double* ri = rhs + inst * NV * g.N;
+  const int z0 = blockIdx.y * kLineChunk;
+  const int z1 = min(z0 + kLineChunk, dd.len);
+  const Op& op = g.ops[SLOT_FIRST];
+  const double c1 = op.c[kMaxHW + 1], c2 = op.c[kMaxHW + 2], c3 = op.c[kMaxHW + 3],
+               c4 = op.c[kMaxHW + 4];
+  double Q[NV][9];
+#pragma unroll
+  for (int l = 0; l < NV; ++l)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) Q[l][k] = x[l * g.N + wrap(z0 - 4 + k, dd) * sd];
+  for (int z = z0; z < z1; ++z) {
+    const long p = base + z * sd;
+    double r[NV];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      Q[l][8] = x[l * g.N + wrap(z + 4, dd) * sd];
+      double out = 0.0;
+      out += c1 * (Q[l][5] - Q[l][3]);
+      out += c2 * (Q[l][6] - Q[l][2]);
+      out += c3 * (Q[l][7] - Q[l][1]);
+      out += c4 * (Q[l][8] - Q[l][0]);
+      double v = ri[l * g.N + p] + out;
+      r[l] = 0.0 - v;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) Q[l][k] = Q[l][k + 1];
+    }
+    int ijk[3] = {(int)i, (int)o, z};
+    const double jac = jacobian<ND>(g, ijk);
+#pragma unroll
+    for (int l = 0; l < NV; ++l) r[l] = r[l] * jac;
+    if (kstr) r[1] = r[1] + qi[p] * kstr[p];
+    if (rk_stage == 0) {
+#pragma unroll
+      for (int l = 0; l < NV; ++l) ri[l * g.N + p] = r[l];
+      continue;
+    }
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      const long ii = inst * NV * g.N + l * g.N + p;
+      const double rl = r[l];
+      double qn;
+      switch (rk_stage) {
+        case 1: {
+          const double q0 = y[ii];
+          b1[ii] = q0;
+          b2[ii] = q0 + div6(dt * rl);
+          qn = q0 + dt * rl / 2.0;
+          break;
+        }
+        case 2:
+          b2[ii] = b2[ii] + div3(dt * rl);
+          qn = b1[ii] + dt * rl / 2.0;
+          break;
+        case 3:
+          b2[ii] = b2[ii] + div3(dt * rl);
+          qn = b1[ii] + dt * rl;
+          break;
+        default:
+          qn = b2[ii] + div6(dt * rl);
+          break;
+      }
+      y[ii] = qn;
+      if (copy) copy[ii] = qn;
+    }
   }
 }
 
@@ -1335,10 +1412,8 @@ static bool deriv_line_applies(const Geo& g, int d) {
 }
 
 static bool launch_deriv_line(const Geo& g, int d, int slot, const double* src, long src_inst,
-                              double* dst, long dst_inst, int ncomp, int mode, hipStream_t st,
-                              const double* q = nullptr, const double* kstr = nullptr) {
+                              double* dst, long dst_inst, int ncomp, int mode, hipStream_t st) {
   if (!deriv_line_applies(g, d)) return false;
-  if (mode == 4 && (d == 0 || g.nd != 3 || g.nv != 5 || ncomp != 5)) return false;
   if (d == 0) {
     const dim3 grid((unsigned)(g.N / kBlock), 1, (unsigned)(g.ninst * ncomp));
     k_deriv_row<<<grid, kBlock, 0, st>>>(g, slot, src, src_inst, dst, dst_inst, ncomp, mode);
@@ -1348,8 +1423,7 @@ static bool launch_deriv_line(const Geo& g, int d, int slot, const double* src, 
   const dim3 grid((unsigned)((nlines + kBlock - 1) / kBlock),
                   (unsigned)((g.dir[d].len + kLineChunk - 1) / kLineChunk),
                   (unsigned)(g.ninst * ncomp));
-  k_deriv_line<<<grid, kBlock, 0, st>>>(g, d, slot, src, src_inst, dst, dst_inst, ncomp, mode, q,
-                                        kstr);
+  k_deriv_line<<<grid, kBlock, 0, st>>>(g, d, slot, src, src_inst, dst, dst_inst, ncomp, mode);
   return true;
 }
 
@@ -1393,10 +1467,10 @@ static void launch_gradients(const Geo& g, const double* q, double* w, double* G
   for (int d = 0; d < ND; ++d) k_grad<ND><<<grid, kBlock, 0, st>>>(g, d, w, G);
 }
 
-void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
+bool launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
                         double* G, double* t, double* F, const double* control, long control_stride,
                         const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
-                        const FinishInfo& fin, hipStream_t st) {
+                        const FinishInfo& fin, hipStream_t st, const RkStage* rk) {
   const dim3 grid = grid_pts(g);
   const int has_farfield = fin.has_farfield;
   bool finished = false;
@@ -1406,14 +1480,21 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
       const dim3 gg((g.n[0] / kFD3X) * (g.n[1] / kFD3Y), g.n[2], g.ninst);
       k_fluxdiv3d_xy<kFD3X, kFD3Y><<<gg, kFD3X * kFD3Y, 0, st>>>(g, q, g.viscous ? G : nullptr, F, rhs);
       // the z divergence also finishes the rhs where nothing else follows it
-      const bool fuse = fin.simple && !g.dissipation && nsrc == 0;
+      const bool fuse = fin.simple && !g.dissipation && nsrc == 0 && deriv_line_applies(g, ND - 1);
       const double* fz = F + (long)(ND - 1) * g.nv * g.N;
-      if (fuse && launch_deriv_line(g, ND - 1, SLOT_FIRST, fz, (long)ND * g.nv * g.N, rhs,
-                                    (long)g.nv * g.N, g.nv, 4, st, q, fin.kolmogorov))
+      if (fuse) {
+        const long nlines = g.N / g.dir[ND - 1].len;
+        const dim3 gz((unsigned)((nlines + kBlock - 1) / kBlock),
+                      (unsigned)((g.dir[ND - 1].len + kLineChunk - 1) / kLineChunk), (unsigned)g.ninst);
+        k_fwd_finish_z<<<gz, kBlock, 0, st>>>(g, fz, (long)ND * g.nv * g.N, q, fin.kolmogorov, rhs,
+                                              rk ? rk->stage : 0, rk ? rk->dt : 0.0,
+                                              rk ? rk->y : nullptr, rk ? rk->b1 : nullptr,
+                                              rk ? rk->b2 : nullptr, rk ? rk->copy : nullptr);
         finished = true;
-      else if (!launch_deriv_line(g, ND - 1, SLOT_FIRST, fz, (long)ND * g.nv * g.N, rhs,
-                                  (long)g.nv * g.N, g.nv, 2, st))
+      } else if (!launch_deriv_line(g, ND - 1, SLOT_FIRST, fz, (long)ND * g.nv * g.N, rhs,
+                                    (long)g.nv * g.N, g.nv, 2, st)) {
         k_div<ND><<<grid, kBlock, 0, st>>>(g, ND - 1, F, rhs);
+      }
     } else if (F) {
       k_flux_all<ND><<<grid, kBlock, 0, st>>>(g, q, g.viscous ? G : nullptr, F);
       for (int d = 0; d < ND; ++d) {
@@ -1440,6 +1521,7 @@ void launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
                                             control_stride, nullptr, 0, 0.0, src, nsrc, src_amp,
                                             src_inst_stride, nullptr);
   });
+  return finished && rk != nullptr;
 }
 
 bool launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,

@@ -471,11 +471,13 @@ void forward_stage(mgd_region* r, double* q, double* b1, double* b2, int sidx, i
     cstride = r->ctrl_stride;
   }
   const int ns = (int)r->sources.size();
-  launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p, ctrl, cstride,
-                     reinterpret_cast<const Source*>(r->d_sources.p), ns,
-                     r->src_amp.p + (long)sidx * std::max(ns, 1), r->src_inst_stride,
-                     r->fin, r->stream);
-  launch_rk_forward((long)r->ninst * M_of(r), stage, dt, q, b1, b2, r->rhs.p, qcopy, r->stream);
+  // the rhs launcher applies the RK update itself where its last kernel finishes the rhs
+  const RkStage rk{stage, dt, q, b1, b2, qcopy};
+  if (!launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p, ctrl,
+                          cstride, reinterpret_cast<const Source*>(r->d_sources.p), ns,
+                          r->src_amp.p + (long)sidx * std::max(ns, 1), r->src_inst_stride, r->fin,
+                          r->stream, &rk))
+    launch_rk_forward((long)r->ninst * M_of(r), stage, dt, q, b1, b2, r->rhs.p, qcopy, r->stream);
 }
 
 bool has_walls(const mgd_region* r) {
