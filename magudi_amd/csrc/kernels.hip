@@ -995,9 +995,10 @@ __global__ void __launch_bounds__(kBlock) k_gradient(Geo g, const DPatch* __rest
 // x and y parts of the flux divergence on a 3-D grid periodic in x and y, one z plane
 // per block row: each block stages the contravariant fluxes of its BX x BY points (x
 // rows and y columns, with the 4-point halo recomputed from q and G) in LDS and writes
-// D_x Fx + D_y Fy to rhs and Fz to its slot of F, for k_div along z to finish.  The
-// per-point flux and the derivative sums are those of k_flux_all and k_div (same
-// operations in the same order, so the same values).
+// D_x Fx + D_y Fy to rhs and Fz to its slot of F, for the z divergence (k_deriv_line,
+// k_fwd_finish_z or k_div) to finish.  The per-point flux and the derivative sums are
+// those of k_flux_all and k_div (same operations in the same order, so the same
+// values).
 template <int BX, int BY>
 __global__ void __launch_bounds__(BX * BY) k_fluxdiv3d_xy(Geo g, const double* __restrict__ q,
                                                            const double* __restrict__ G,
