@@ -1402,9 +1402,8 @@ __global__ void __launch_bounds__(kBlock) k_deriv_row(Geo g, int slot, const dou
   }
 }
 
-// Launches k_deriv_row (d = 0) or k_deriv_line (d >= 1) when direction d is periodic and
-// the grid fits the kernel; false otherwise (the caller then uses the generic kernel).
-// Whether k_deriv_row (d = 0) or k_deriv_line (d >= 1) serves direction d.
+// Whether k_deriv_row (d = 0) or k_deriv_line (d >= 1) serves direction d: periodic,
+// and a grid that fits the kernel.
 static bool deriv_line_applies(const Geo& g, int d) {
   if (!g.dir[d].periodic || g.dir[d].len < 16) return false;
   if (d > 0) return true;
@@ -1412,6 +1411,8 @@ static bool deriv_line_applies(const Geo& g, int d) {
   return ((n0 % kBlock == 0) || (kBlock % n0 == 0)) && g.N % kBlock == 0;
 }
 
+// Launches k_deriv_row (d = 0) or k_deriv_line (d >= 1) where one serves direction d;
+// false otherwise (the caller then uses the generic kernel).
 static bool launch_deriv_line(const Geo& g, int d, int slot, const double* src, long src_inst,
                               double* dst, long dst_inst, int ncomp, int mode, hipStream_t st) {
   if (!deriv_line_applies(g, d)) return false;
