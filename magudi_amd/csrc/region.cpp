@@ -1318,7 +1318,8 @@ int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region
         upload(r->d_x[d0], r->line[d0]))
       return MGD_ERR_HIP;
   }
-  if (r->d_ops.alloc(sizeof(r->ops) / sizeof(double) + 1)) return MGD_ERR_HIP;
+  const size_t ops_bytes = sizeof(r->ops);   // DevBuf counts doubles
+  if (r->d_ops.alloc(ops_bytes / sizeof(double) + 1)) return MGD_ERR_HIP;
   MGD_HIP(hipMemcpy(r->d_ops.p, r->ops, sizeof(r->ops), hipMemcpyHostToDevice));
   Geo& g = r->geo;
   std::memset(&g, 0, sizeof g);
