@@ -52,3 +52,35 @@ def test_terminal_kinetic_energy_adjoint(n):
     fd = (Jp - Jm) / (2 * eps)
     an = float(np.sum(norm * (lam0 * d)))
     assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+def test_instances_match_single_runs():
+    """Two instances of a 32 x 16 x 16 periodic box in one region (every 3-D kernel,
+    including the finishing kernels with the RK updates, indexes the instance) against
+    each state run alone: bit for bit, forward and adjoint from a terminal state."""
+    from magudi_amd.solver import Region, Solver
+    from test_gpu_parity import periodic_box, perturbed
+    c = periodic_box((32, 16, 16), dissipation=False)
+    nsteps, K = 4, 2
+    qs = [perturbed(c, 31), perturbed(c, 32)]
+    rng = np.random.default_rng(33)
+    lamT = [rng.standard_normal((c.nv, c.npoints)) for _ in range(2)]
+    single = []
+    for k in range(2):
+        R = Region(c)
+        R.states[0].conservedVariables = qs[k]
+        S = Solver(R, dt=c.dt, nsteps=nsteps, save_interval=K)
+        S.runForwardCheckpointed()
+        qT = R.states[0].conservedVariables
+        S.runAdjointCheckpointed(terminal=lamT[k], want_gradient=False)
+        single.append((qT, R.states[0].adjointVariables))
+    R = Region(c, n_instances=2)
+    for k in range(2):
+        R.states[k].conservedVariables = qs[k]
+    S = Solver(R, dt=c.dt, nsteps=nsteps, save_interval=K)
+    S.runForwardCheckpointed()
+    qT2 = [R.states[k].conservedVariables for k in range(2)]
+    S.runAdjointCheckpointed(terminal=np.stack(lamT), want_gradient=False)
+    for k in range(2):
+        assert np.array_equal(qT2[k], single[k][0]), k
+        assert np.array_equal(R.states[k].adjointVariables, single[k][1]), k
