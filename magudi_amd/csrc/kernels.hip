@@ -1357,48 +1357,60 @@ __global__ void __launch_bounds__(kBlock) k_adj_finish_z(Geo g, const double* __
   }
 }
 
-// The same operator along a periodic x direction: a block stages 256 consecutive points
-// of one component -- a segment of one row, or whole rows when rows are shorter -- with
-// the 4-point wrap-around halo of each row segment in LDS, and applies the interior
-// stencil from there.  Requires rows of a length that divides 256 or is a multiple of
-// it, and 256 | N.  Modes as for k_deriv_line, plus 3: store 0 - result.
+// The same operator along a periodic x direction: a block stages `segs` (1 or
+// kRowSegs) runs of 256 consecutive points of one component -- segments of one row, or
+// whole rows when rows are shorter -- with the 4-point wrap-around halo of each row
+// segment in LDS, and applies the interior stencil from there; all its loads are issued
+// before the one barrier.  Requires rows of a length that divides 256 or is a multiple
+// of it, and 256 segs | N.  Modes as for k_deriv_line, plus 3: store 0 - result.
+constexpr int kRowSegs = 4;
 __global__ void __launch_bounds__(kBlock) k_deriv_row(Geo g, int slot, const double* __restrict__ src,
                                                      long src_inst, double* __restrict__ dst,
-                                                     long dst_inst, int ncomp, int mode) {
-  static_assert(kBlock == 256, "one staged point per thread");
+                                                     long dst_inst, int ncomp, int mode, int segs) {
+  static_assert(kBlock == 256, "one staged point per thread and run");
+  constexpr int SW = kBlock + 8 * kBlock / 16;        // (L + 8) per segment, L >= 16
   const Dir dx = g.dir[0];
   const int L = dx.len < kBlock ? dx.len : kBlock;   // segment length
-  __shared__ double s[kBlock + 8 * kBlock / 16];    // (L + 8) per segment, L >= 16
+  __shared__ double s[kRowSegs][SW];
   const int inst = blockIdx.z / ncomp, c = blockIdx.z % ncomp;
   const double* x = src + inst * src_inst + (long)c * g.N;
   double* y = dst + inst * dst_inst + (long)c * g.N;
   const int t = threadIdx.x;
-  const long p = (long)blockIdx.x * kBlock + t;
   const int seg = t / L, xi = t % L;
-  const int xr = (int)(p % dx.len);                 // index along the row
-  const long row0 = p - xr;
-  double* sg = s + seg * (L + 8);
-  sg[4 + xi] = x[p];
-  if (xi < 4) sg[xi] = x[row0 + wrap(xr - 4, dx)];
-  if (xi >= L - 4) sg[xi + 8] = x[row0 + wrap(xr + 4, dx)];
+#pragma unroll
+  for (int u = 0; u < kRowSegs; ++u) {
+    if (u >= segs) break;
+    const long p = ((long)blockIdx.x * segs + u) * kBlock + t;
+    const int xr = (int)(p % dx.len);                 // index along the row
+    const long row0 = p - xr;
+    double* sg = s[u] + seg * (L + 8);
+    sg[4 + xi] = x[p];
+    if (xi < 4) sg[xi] = x[row0 + wrap(xr - 4, dx)];
+    if (xi >= L - 4) sg[xi + 8] = x[row0 + wrap(xr + 4, dx)];
+  }
   __syncthreads();
   const Op& op = g.ops[slot];
   const double c1 = op.c[kMaxHW + 1], c2 = op.c[kMaxHW + 2], c3 = op.c[kMaxHW + 3],
                c4 = op.c[kMaxHW + 4];
-  const double* r = sg + 4 + xi;
-  double out = 0.0;
-  out += c1 * (r[1] - r[-1]);
-  out += c2 * (r[2] - r[-2]);
-  out += c3 * (r[3] - r[-3]);
-  out += c4 * (r[4] - r[-4]);
-  if (mode == 0) {
-    y[p] = out;
-  } else if (mode == 3) {
-    y[p] = 0.0 - out;
-  } else {
-    double v = y[p] + out;
-    if (mode == 2) v = 0.0 - v;
-    y[p] = v;
+#pragma unroll
+  for (int u = 0; u < kRowSegs; ++u) {
+    if (u >= segs) break;
+    const long p = ((long)blockIdx.x * segs + u) * kBlock + t;
+    const double* r = s[u] + seg * (L + 8) + 4 + xi;
+    double out = 0.0;
+    out += c1 * (r[1] - r[-1]);
+    out += c2 * (r[2] - r[-2]);
+    out += c3 * (r[3] - r[-3]);
+    out += c4 * (r[4] - r[-4]);
+    if (mode == 0) {
+      y[p] = out;
+    } else if (mode == 3) {
+      y[p] = 0.0 - out;
+    } else {
+      double v = y[p] + out;
+      if (mode == 2) v = 0.0 - v;
+      y[p] = v;
+    }
   }
 }
 
@@ -1417,8 +1429,9 @@ static bool launch_deriv_line(const Geo& g, int d, int slot, const double* src, 
                               double* dst, long dst_inst, int ncomp, int mode, hipStream_t st) {
   if (!deriv_line_applies(g, d)) return false;
   if (d == 0) {
-    const dim3 grid((unsigned)(g.N / kBlock), 1, (unsigned)(g.ninst * ncomp));
-    k_deriv_row<<<grid, kBlock, 0, st>>>(g, slot, src, src_inst, dst, dst_inst, ncomp, mode);
+    const int segs = g.N % ((long)kBlock * kRowSegs) == 0 ? kRowSegs : 1;
+    const dim3 grid((unsigned)(g.N / ((long)kBlock * segs)), 1, (unsigned)(g.ninst * ncomp));
+    k_deriv_row<<<grid, kBlock, 0, st>>>(g, slot, src, src_inst, dst, dst_inst, ncomp, mode, segs);
     return true;
   }
   const long nlines = g.N / g.dir[d].len;
