@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=64)
     ap.add_argument("--c4-n", type=int, default=256,
                     help="grid of the 3-D viscous Kolmogorov rhs leg (0: skip)")
+    ap.add_argument("--fd-sweep-only", action="store_true",
+                    help="only the batched finite-difference sweep over control amplitudes")
     ap.add_argument("--kolmogorov-only", action="store_true",
                     help="only the 3-D Kolmogorov forward + adjoint run (c4-n^3, 200 steps)")
     ap.add_argument("--viscous3d-only", action="store_true",
@@ -167,6 +169,55 @@ def kolmogorov_run_leg(n, nsteps=200):
             "host_terminal_s": round(th, 3), "J": J, "max_abs_lambda0": lam_max}
 
 
+def fd_sweep_leg(n=401, nsteps=960, save_interval=32, batch=20):
+    """The batched finite-difference gradient sweep: J0 and Grad0 from one forward +
+    adjoint run, then the forward run re-done for batch control amplitudes
+    A_k = 10^(-2 - 0.25 k) along the adjoint gradient, all amplitudes as the instances
+    of one region (one launch per RK stage), against the same re-runs one after
+    another.  Reports the relative error |(J(A_k) - J0) / A_k - Grad0| / Grad0 per
+    amplitude."""
+    from magudi_amd import cases
+    from magudi_amd.solver import Region, Solver
+    c = cases.acoustic_monopole(n=n, nsteps=nsteps, save_interval=save_interval)
+    q0 = c.initial_state()
+    R = Region(c)
+    R.states[0].conservedVariables = q0
+    S = Solver(R)
+    sens, grad, J = S.runAdjoint()
+    J0, g0 = float(J[0]), float(sens[0])
+    amps = [10.0 ** (-2.0 - 0.25 * k) for k in range(batch)]
+    ctl = np.stack([a * grad[0] for a in amps])
+    Rb = Region(c, n_instances=batch)
+    for k in range(batch):
+        Rb.states[k].initialCondition = q0
+    Sb = Solver(Rb)
+    Rb.restart()
+    Sb.runForward(control=ctl)                      # warm-up (tile autotuning)
+    Rb.restart()
+    Rb.sync()
+    t0 = time.perf_counter()
+    Jb = Sb.runForward(control=ctl)
+    Rb.sync()
+    tb = time.perf_counter() - t0
+    # the same re-runs one after another on the single-instance region
+    R.states[0].initialCondition = q0
+    Js = []
+    R.sync()
+    t0 = time.perf_counter()
+    for k in range(batch):
+        R.restart()
+        Js.append(float(S.runForward(control=ctl[k])[0]))
+    R.sync()
+    ts = time.perf_counter() - t0
+    err = [abs((float(Jb[k]) - J0) / amps[k] - g0) / abs(g0) for k in range(batch)]
+    return {"workload": f"{batch} forward re-runs of the {n}x{n} monopole ({nsteps} RK4 steps) "
+                        f"at control amplitudes 10^(-2 - 0.25 k) along the adjoint gradient",
+            "value": batch * c.npoints * nsteps / tb, "unit": "grid-pt-steps/s",
+            "sequential_value": batch * c.npoints * nsteps / ts, "speedup": ts / tb,
+            "J0": J0, "Grad0": g0, "same_J_as_single_runs": bool(all(float(Jb[k]) == Js[k] for k in range(batch))),
+            "rel_error": [[amps[k], err[k]] for k in range(batch)]}
+
+
 def small_grid_leg(n=201, nsteps=96, save_interval=32, batch=20):
     """North-star ratio on a small grid: batched fused forward+adjoint (batch instances
     per launch) against the generic per-instance path, whose many small launches
@@ -283,6 +334,10 @@ def main():
     if a.kolmogorov_only:
         print(json.dumps({"kolmogorov_run": kolmogorov_run_leg(a.c4_n)}), flush=True)
         return 0
+    if a.fd_sweep_only:
+        print(json.dumps({"fd_sweep": fd_sweep_leg(a.n, a.nsteps, a.save_interval, a.batch)}),
+              flush=True)
+        return 0
     case = cases.acoustic_monopole(n=a.n, nsteps=a.nsteps, save_interval=a.save_interval)
     q0 = case.initial_state()
     R = Region(case, tile_shape=a.tile_shape)
@@ -396,6 +451,7 @@ def main():
         if a.c4_n > 0:
             out["viscous3d"] = viscous3d_leg(a.c4_n)
             out["kolmogorov_run"] = kolmogorov_run_leg(a.c4_n)
+        out["fd_sweep"] = fd_sweep_leg(a.n, a.nsteps, a.save_interval, a.batch)
         out["uneven_batch"] = uneven_leg()
         out["small_ensemble"] = small_ensemble_leg()
         out["small_grid"] = small_grid_leg()
