@@ -219,7 +219,7 @@ def fd_sweep_leg(n=401, nsteps=960, save_interval=32, batch=20):
 
 
 def small_grid_leg(n=201, nsteps=96, save_interval=32, batch=20):
-    """North-star ratio on a small grid: batched fused forward+adjoint (batch instances
+    """Batched against plain on a small grid: batched fused forward+adjoint (batch instances
     per launch) against the generic per-instance path, whose many small launches
     make it launch-bound at this size."""
     from magudi_amd import cases
