@@ -568,6 +568,40 @@ static void rhs_adjoint(orc_region* r, const double* q, const double* lam, doubl
   }
 }
 
+static double quadrature(const orc_region* r, int type, const double* integrand_full);
+
+/* normalizeKolmogorovForcing (src/RegionImpl.f90:910-999, called at :1477): the forces of
+ * every Kolmogorov patch, summed per grid point (disperseAdd), and the count of patches
+ * covering each point are integrated over the union of those patches
+ * (computeQuadratureOnPatches); each patch's force then loses globalForcing /
+ * globalVolume, so that the forcing has zero quadrature mean over the patches. */
+static void normalize_kolmogorov(orc_region* r) {
+  const long N = r->N;
+  int any = 0;
+  for (int ip = 0; ip < r->npatches; ++ip) any |= r->patches[ip].pd.type == ORC_KOLMOGOROV;
+  if (!any) return;
+  double* gf = zalloc(N);
+  double* gv = zalloc(N);
+  for (int ip = 0; ip < r->npatches; ++ip) {
+    const orc_patch* P = &r->patches[ip];
+    if (P->pd.type != ORC_KOLMOGOROV) continue;
+    for (int k = 0; k < P->np; ++k) {
+      const long p = patch_grid_index(r, P, k);
+      gf[p] = gf[p] + P->kforce[k];
+      gv[p] = gv[p] + 1.0;
+    }
+  }
+  const double volume = quadrature(r, ORC_KOLMOGOROV, gv);
+  const double forcing = quadrature(r, ORC_KOLMOGOROV, gf);
+  for (int ip = 0; ip < r->npatches; ++ip) {
+    orc_patch* P = &r->patches[ip];
+    if (P->pd.type != ORC_KOLMOGOROV) continue;
+    for (int k = 0; k < P->np; ++k) P->kforce[k] = P->kforce[k] - forcing / volume;
+  }
+  free(gf);
+  free(gv);
+}
+
 int orc_region_finalize(orc_region* r) {
   const int nd = r->d.nd, nv = r->nv;
   const long N = r->N;
@@ -659,6 +693,7 @@ int orc_region_finalize(orc_region* r) {
       }
     }
   }
+  normalize_kolmogorov(r);
   if (anyff && r->d.viscous) {
     update_state(r, r->target);
     for (int ip = 0; ip < r->npatches; ++ip) {
