@@ -51,6 +51,7 @@ struct HostPatch {
   mgd_patch_desc pd;
   int lo[3], hi[3], ls[3], np;
   std::vector<double> strength, data, tvflux;
+  double sig_inv = 0.0, sig_visc = 0.0, twall = 0.0;   // penalty amounts set at finalize
   DevBuf d_strength, d_data, d_tvflux, d_ffv;
 };
 
@@ -1232,6 +1233,47 @@ int group_adjoint_pass(mgd_group* G, const double* dts, int nsteps, int K) {
   return MGD_OK;
 }
 
+// normalizeKolmogorovForcing (src/RegionImpl.f90:910-999, called at :1477): the forces of
+// all Kolmogorov patches summed per grid point, and the number of patches covering each
+// point, are integrated with the norm over the union of those patches
+// (computeQuadratureOnPatches, src/PatchFactoryImpl.f90:376-444); every patch's force then
+// loses globalForcing / globalVolume, so the forcing has zero quadrature mean.
+void normalize_kolmogorov(mgd_region* r) {
+  const long N = r->N;
+  const int nd = r->nd;
+  bool any = false;
+  for (const auto& P : r->patches) any = any || P->pd.type == MGD_PATCH_KOLMOGOROV;
+  if (!any) return;
+  std::vector<double> force(N, 0.0), count(N, 0.0), mask(N, 0.0);
+  for (const auto& P : r->patches) {
+    if (P->pd.type != MGD_PATCH_KOLMOGOROV) continue;
+    for (int k = 0; k < P->np; ++k) {
+      const long p = patch_point(r, *P, k);
+      force[p] = force[p] + P->strength[k];
+      count[p] = count[p] + 1.0;
+      mask[p] = 1.0;
+    }
+  }
+  double volume = 0.0, forcing = 0.0;
+  for (long p = 0; p < N; ++p) {
+    int ijk[3];
+    unflat_host(r, p, ijk);
+    double det = r->h[0][ijk[0]];
+    if (nd > 1) det = det * r->h[1][ijk[1]];
+    if (nd > 2) det = det * r->h[2][ijk[2]];
+    double w = 1.0;
+    for (int d = 0; d < nd; ++d) w = r->hw[d][ijk[d]] * w;
+    const double norm = w * det;
+    volume += mask[p] * norm * count[p];
+    forcing += mask[p] * norm * force[p];
+  }
+  const double mean = forcing / volume;
+  for (auto& P : r->patches) {
+    if (P->pd.type != MGD_PATCH_KOLMOGOROV) continue;
+    for (int k = 0; k < P->np; ++k) P->strength[k] = P->strength[k] - mean;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1552,6 +1594,13 @@ int mgd_region_finalize(mgd_region* r) {
         break;
       }
     }
+    P.sig_inv = sig_inv;
+    P.sig_visc = sig_visc;
+    P.twall = twall;
+  }
+  normalize_kolmogorov(r);
+  for (size_t ip = 0; ip < r->patches.size(); ++ip) {
+    HostPatch& P = *r->patches[ip];
     if (upload(P.d_strength, P.strength) || upload(P.d_data, P.data) || upload(P.d_tvflux, P.tvflux))
       return MGD_ERR_HIP;
     DPatch& D = r->hpatches.p[ip];
@@ -1559,9 +1608,9 @@ int mgd_region_finalize(mgd_region* r) {
     D.normal_dir = P.pd.normal_dir;
     for (int i = 0; i < 3; ++i) { D.lo[i] = P.lo[i]; D.hi[i] = P.hi[i]; D.ls[i] = P.ls[i]; }
     D.np = P.np;
-    D.sigma_inv = sig_inv;
-    D.sigma_visc = sig_visc;
-    D.twall = twall;
+    D.sigma_inv = P.sig_inv;
+    D.sigma_visc = P.sig_visc;
+    D.twall = P.twall;
     D.strength = P.d_strength.p;
     D.aplus = P.d_data.p;
     D.tvflux = P.d_tvflux.p;

@@ -79,6 +79,26 @@ def periodic_box(shape, seed=3, dissipation=True):
         target_mollifier=np.zeros(npts), dt=0.002, nsteps=1, save_interval=1, initial=q)
 
 
+def kolmogorov_subbox(shape=(32, 16, 12)):
+    """Periodic box whose two Kolmogorov forcings cover overlapping sub-boxes with a
+    y-extent that is no whole period: the forcing's quadrature mean over the union of
+    the patches is far from zero and must be removed from both
+    (normalizeKolmogorovForcing, src/RegionImpl.f90:910-999)."""
+    import dataclasses
+    c = periodic_box(shape, dissipation=False)
+    return dataclasses.replace(c, patches=[
+        cases.Patch("forcing.a", cases.KOLMOGOROV, 0, (1, -1, 2, 7, 1, -1), (1.0, 0.3, 0, 0)),
+        cases.Patch("forcing.b", cases.KOLMOGOROV, 0, (3, 20, 5, 9, 2, 8), (2.0, 0.2, 0, 0))])
+
+
+def monopole_kolmogorov(n=41):
+    """2-D monopole (fused stage kernels) with a Kolmogorov forcing on a sub-box."""
+    c = cases.acoustic_monopole(n=n, nsteps=1)
+    c.patches = list(c.patches) + [
+        cases.Patch("forcing", cases.KOLMOGOROV, 0, (5, 30, 8, 19, 1, 1), (1.0, 0.05, 0, 0))]
+    return c
+
+
 CASES = {
     "monopole41": lambda: cases.acoustic_monopole(n=41, nsteps=1),
     "monopole41_composite": lambda: cases.acoustic_monopole(n=41, nsteps=1, composite=True),
@@ -93,6 +113,8 @@ CASES = {
     "box32x16x12_nodiss": lambda: periodic_box((32, 16, 12), dissipation=False),
     # rows longer than one 256-point segment; z too short for the line march
     "box512x16x9": lambda: periodic_box((512, 16, 9)),
+    "box32x16x12_kolmogorov_subboxes": kolmogorov_subbox,
+    "monopole41_kolmogorov_subbox": monopole_kolmogorov,
 }
 
 
