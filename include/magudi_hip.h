@@ -152,6 +152,15 @@ int mgd_state_lincomb(mgd_region* r, int dst, int dst_inst, int a, int a_inst, i
  * components and grid points of a * norm * b. */
 int mgd_inner_product(mgd_region* r, int a, int b, double* out);
 
+/* mgd_inner_product without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops); it has no counterpart in
+ * the reference.  out_device: device array of n_instances doubles.  The per-block partial
+ * sums go to a fixed 2 MiB device array of the library (one per GPU), so the region may
+ * hold at most 2^26 grid points over all instances (MGD_ERR_ARG beyond).  Asynchronous on
+ * the region's stream (mgd_sync waits); calls that overlap on one GPU share the array
+ * and race. */
+int mgd_inner_product_noalloc(mgd_region* r, int a, int b, double* out_device);
+
 /* Region%computeRhs(mode) (src/RegionImpl.f90:1877-2027) for all instances into the
  * region's rhs buffer.  time: State%time for acoustic sources.  control: actuator
  * forcing per instance per actuator patch point (host, may be NULL).  adjoint

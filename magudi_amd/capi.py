@@ -90,6 +90,7 @@ def lib():
         "mgd_state_lincomb": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                               C.c_double, C.c_int, C.c_int],
         "mgd_inner_product": [vp, C.c_int, C.c_int, dp],
+        "mgd_inner_product_noalloc": [vp, C.c_int, C.c_int, vp],
         "mgd_region_set_time_offsets": [vp, dp],
         "mgd_run_forward_checkpointed": [vp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp],
         "mgd_run_adjoint_checkpointed": [vp, C.c_int, dp, dp],

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -127,6 +128,7 @@ struct mgd_region {
   long src_inst_stride = 0;              // source table entries per instance (0: shared)
   double last_ms[3] = {0, 0, 0};
   int last_launch[3] = {0, 0, 0};
+  int device = 0;                        // HIP device the region was created on
 };
 
 using namespace mgd;
@@ -1274,6 +1276,36 @@ void normalize_kolmogorov(mgd_region* r) {
   }
 }
 
+// Scratch of the allocation-free entry point (mgd_inner_product_noalloc): the per-block
+// partial sums of the inner product live in this fixed device array instead of a buffer
+// the region allocates.  Each device holds its own copy at its own address, looked up
+// once per device with hipGetSymbolAddress when a region is created there; host code
+// uses only that address, never the array's name.  Calls that overlap on one device
+// share the array and race.
+constexpr long kScratchDoubles = 1L << 18;   // 2 MiB: 2^18 blocks of 256 points
+__device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Looks up the scratch array's device address on the current device (once per device).
+int scratch_setup(int* device) {
+  int dev = 0;
+  MGD_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) { set_error("device index beyond the scratch table"); return MGD_ERR_ARG; }
+  *device = dev;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return MGD_OK;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return MGD_ERR_HIP;
+  }
+  g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  return MGD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1388,6 +1420,7 @@ int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region
   g.viscous = d->viscous;
   g.dissipation = d->dissipation;
   g.composite = d->composite_dissipation;
+  if (int rc = scratch_setup(&r->device)) return rc;
   MGD_HIP(hipStreamCreate(&r->stream));
   MGD_HIP(hipStreamCreate(&r->stream_rec));
   MGD_HIP(hipEventCreate(&r->ev0));
@@ -1748,6 +1781,25 @@ int mgd_inner_product(mgd_region* r, int a, int b, double* out) {
   MGD_HIP(hipGetLastError());
   MGD_HIP(hipMemcpyAsync(out, r->iout.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost, r->stream));
   MGD_HIP(hipStreamSynchronize(r->stream));
+  return MGD_OK;
+}
+
+int mgd_inner_product_noalloc(mgd_region* r, int a, int b, double* out_device) {
+  if (!r || !r->finalized || !out_device) { set_error("bad argument"); return MGD_ERR_ARG; }
+  DevBuf* A = state_buf(r, a, false);
+  DevBuf* B = state_buf(r, b, false);
+  if (!A || !B) { set_error("unknown or empty state array"); return MGD_ERR_ARG; }
+  const long nb = (r->N + 255) / 256;
+  if (nb * r->ninst > kScratchDoubles) {
+    set_error("mgd_inner_product_noalloc: " + std::to_string(nb * r->ninst) +
+              " partial sums exceed the fixed scratch of " + std::to_string(kScratchDoubles) +
+              " (at most 2^26 grid points over all instances)");
+    return MGD_ERR_ARG;
+  }
+  double* part = g_scratch_ptr[r->device];
+  if (!part) { set_error("scratch not set up on the region's device"); return MGD_ERR_STATE; }
+  launch_inner_product(r->geo, A->p, B->p, part, out_device, r->stream);
+  MGD_HIP(hipGetLastError());
   return MGD_OK;
 }
 
