@@ -195,6 +195,15 @@ int mgd_run_adjoint(mgd_region* r, double t0, double dt, int nsteps, int save_in
                     const double* control, double* gradient, double* sensitivity, double* J,
                     int use_graph);
 
+/* Cost on the final state, for runs whose adjoint starts from a nonzero terminal state:
+ * J per instance of the current conserved state, and its gradient dJ/dq written to
+ * MGD_ADJOINT_VARS as the terminal adjoint state for mgd_run_adjoint_checkpointed(r, 1, ..).
+ * It stands in for the user-supplied <prefix>.adjoint.q of adjoint_nonzero_initial_condition
+ * (src/SolverImpl.f90:368-428, 1096-1102); magudi's own functionals are time integrals.
+ * MGD_TERMINAL_KINETIC_ENERGY: J = sum_p H_p |rho u|^2 / (2 rho) with the SBP norm H. */
+#define MGD_TERMINAL_KINETIC_ENERGY 0
+int mgd_terminal_cost(mgd_region* r, int kind, double* J);
+
 /* The two halves of mgd_run_adjoint, for drivers that set the adjoint terminal condition
  * from the forward result (multiple shooting, bin/msadjoint.f90:287-384).
  * mgd_run_forward_checkpointed: the forward pass of Solver%runAdjoint from the current

@@ -20,6 +20,7 @@ CONSERVED, ADJOINT_VARS, RHS, INITIAL_STATE, FINAL_STATE, SCRATCH = 0, 1, 2, 4, 
 FIELD_TARGET_STATE, FIELD_CONTROL_MOLLIFIER, FIELD_TARGET_MOLLIFIER = 0, 1, 2
 OP_FIRST, OP_ADJOINT_FIRST, OP_DISS, OP_DISS_T = 0, 1, 2, 3
 OPT_FORCE_PLAIN, OPT_TILE_SHAPE, OPT_RESIDENT_STATES = 1, 2, 3
+TERMINAL_KINETIC_ENERGY = 0
 KERNEL_FWD_STAGE, KERNEL_ADJ_STAGE, KERNEL_COPY, KERNEL_RHS_FORWARD, KERNEL_RHS_ADJOINT = 0, 1, 2, 3, 4
 
 
@@ -94,6 +95,7 @@ def lib():
         "mgd_region_set_time_offsets": [vp, dp],
         "mgd_run_forward_checkpointed": [vp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp],
         "mgd_run_adjoint_checkpointed": [vp, C.c_int, dp, dp],
+        "mgd_terminal_cost": [vp, C.c_int, dp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

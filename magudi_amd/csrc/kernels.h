@@ -22,6 +22,8 @@ struct FinishInfo {
 // substepAdjointRK4, src/RK4IntegratorImpl.f90:115-156, 215-261): y the state updated in
 // place, b1 / b2 the RK buffers, copy (forward, optional) a second destination of the
 // new stage state.
+// yout (forward, optional): a state buffer other than y the new stage state may be
+// written to instead of y, for kernels that cannot update y in place.
 struct RkStage {
   int stage;
   double dt;
@@ -29,13 +31,17 @@ struct RkStage {
   double* b1;
   double* b2;
   double* copy;
+  double* yout = nullptr;
 };
-// Returns true when it applied `rk` (then rhs is not written); false when the caller
-// still has to (launch_rk_forward) -- always false for rk == nullptr.
-bool launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
-                        double* G, double* t, double* F, const double* control, long control_stride,
-                        const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
-                        const FinishInfo& fin, hipStream_t st, const RkStage* rk = nullptr);
+// What launch_rhs_forward did with `rk`.
+enum RkDone { RK_NOT_APPLIED = 0, RK_IN_PLACE = 1, RK_TO_YOUT = 2 };
+// RK_IN_PLACE / RK_TO_YOUT when it applied `rk` (then rhs is not written; with RK_TO_YOUT
+// the new stage state is in rk->yout and y is unchanged); RK_NOT_APPLIED when the caller
+// still has to (launch_rk_forward) -- always for rk == nullptr.
+int launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
+                       double* G, double* t, double* F, const double* control, long control_stride,
+                       const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
+                       const FinishInfo& fin, hipStream_t st, const RkStage* rk = nullptr);
 // Returns true when it applied `rk` (then rhs is not written); false when the caller
 // still has to (launch_rk_adjoint) -- always false for rk == nullptr.
 bool launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
@@ -60,6 +66,10 @@ void launch_gradient(const Geo& g, const DPatch* pt, int np, const double* lam, 
 // out[inst] = <a, b>_H over the instance's state arrays; partial: ninst * ceil(N/256)
 void launch_inner_product(const Geo& g, const double* a, const double* b, double* partial,
                           double* out, hipStream_t st);
+// Terminal kinetic energy: J[inst] = sum_p H_p |rho u|^2 / (2 rho), lam = dJ/dq pointwise;
+// partial: ninst * ceil(N/256)
+void launch_kinetic_energy(const Geo& g, const double* q, double* lam, double* partial, double* J,
+                           hipStream_t st);
 void launch_lincomb(long M, const double* a, const double* b, double s, const double* c, double* d,
                     hipStream_t st);
 void launch_apply(const Geo& g, int slot, int d, int ncomp, const double* x, double* y,

@@ -64,7 +64,12 @@ __global__ void __launch_bounds__(kBlock) k_grad(Geo g, int d, const double* __r
 // iteration before it is used, so its latency overlaps a plane's stencil work.
 // Same sums as k_prim + k_grad (interior SBP rows, wrap-around indices), so the same
 // values.
-template <int BX, int BY, int ZC>
+// STRESS: instead of the 12 gradient components, write what the fluxes need of them at
+// the point: the stress tensor (computeStressTensor, src/CNSHelperImpl.f90:406-455) as
+// its six distinct components (xx, xy, xz, yy, yz, zz) and the heat flux -kappa grad T
+// (src/StateImpl.f90:513-523) in slots 6..8, from the same gradient values in registers
+// (visc_point's operations, so the same values); 72 instead of 96 bytes per point.
+template <int BX, int BY, int ZC, bool STRESS = false>
 __global__ void __launch_bounds__(BX * BY, 4) k_grad3d_periodic(Geo g, const double* __restrict__ q,
                                                               double* __restrict__ G) {
   constexpr int NT = BX * BY, EXW = BX + 8, EYW = BY + 8, NW = 4, NH = 8 * (BX + BY);
@@ -135,6 +140,7 @@ __global__ void __launch_bounds__(BX * BY, 4) k_grad3d_periodic(Geo g, const dou
     int ijk[3] = {i, j, zc};
     const long p = i + (long)g.n[0] * j + n01 * zc;
     const double jac = jacobian<3>(g, ijk);
+    double gv[NW * 3];
 #pragma unroll
     for (int c = 0; c < NW; ++c) {
       const double* row = &sw[c][ty + 4][tx + 4];
@@ -151,9 +157,25 @@ __global__ void __launch_bounds__(BX * BY, 4) k_grad3d_periodic(Geo g, const dou
       oz += c2 * (Q[6][c] - Q[2][c]);
       oz += c3 * (Q[7][c] - Q[1][c]);
       oz += c4 * (Q[8][c] - Q[0][c]);
-      Gi[(c * 3 + 0) * g.N + p] = (jac * metric<3>(g, 0, ijk)) * ox;
-      Gi[(c * 3 + 1) * g.N + p] = (jac * metric<3>(g, 1, ijk)) * oy;
-      Gi[(c * 3 + 2) * g.N + p] = (jac * metric<3>(g, 2, ijk)) * oz;
+      gv[c * 3 + 0] = (jac * metric<3>(g, 0, ijk)) * ox;
+      gv[c * 3 + 1] = (jac * metric<3>(g, 1, ijk)) * oy;
+      gv[c * 3 + 2] = (jac * metric<3>(g, 2, ijk)) * oz;
+    }
+    if (STRESS) {
+      double mu, la, ka;
+      transport(g, Q[4][3], &mu, &la, &ka);
+      stress<3>(gv, mu, la);
+      Gi[0 * g.N + p] = gv[0];
+      Gi[1 * g.N + p] = gv[1];
+      Gi[2 * g.N + p] = gv[2];
+      Gi[3 * g.N + p] = gv[4];
+      Gi[4 * g.N + p] = gv[5];
+      Gi[5 * g.N + p] = gv[8];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) Gi[(6 + d) * g.N + p] = -ka * gv[9 + d];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NW * 3; ++k) Gi[k * g.N + p] = gv[k];
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k)
@@ -924,6 +946,37 @@ __global__ void __launch_bounds__(kBlock) k_inner_partial(Geo g, const double* _
   if (threadIdx.x == 0) partial[inst * gridDim.x + blockIdx.x] = s;
 }
 
+// Terminal kinetic-energy cost J = sum_p H_p |rho u|^2 / (2 rho) of each instance and its
+// pointwise derivative, the adjoint state at the final time (the terminal condition of
+// adjoint_nonzero_initial_condition, src/SolverImpl.f90:368-428, 1096-1102):
+// lam = (-|rho u|^2 / (2 rho^2), rho u_i / rho, 0).  Per-block partial sums of H_p ke_p.
+template <int ND>
+__global__ void __launch_bounds__(kBlock) k_kinetic_energy(Geo g, const double* __restrict__ q,
+                                                           double* __restrict__ lam,
+                                                           double* __restrict__ partial) {
+  const long inst = blockIdx.y;
+  const long p = (long)blockIdx.x * kRedBlock + threadIdx.x;
+  double v = 0.0;
+  if (p < g.N) {
+    int ijk[3];
+    unflat<ND>(g, p, ijk);
+    const double* qi = q + inst * (ND + 2) * g.N;
+    double* li = lam + inst * (ND + 2) * g.N;
+    const double rho = qi[p];
+    double m2 = 0.0;
+    for (int i = 0; i < ND; ++i) {
+      const double m = qi[(1 + i) * g.N + p];
+      m2 = m2 + m * m;
+      li[(1 + i) * g.N + p] = m / rho;
+    }
+    li[p] = -0.5 * m2 / (rho * rho);
+    li[(ND + 1) * g.N + p] = 0.0;
+    v = gnorm<ND>(g, ijk) * (0.5 * m2 / rho);
+  }
+  const double s = block_sum(v);
+  if (threadIdx.x == 0) partial[inst * gridDim.x + blockIdx.x] = s;
+}
+
 // d = c + s (a - b) (c may be null: d = s (a - b)) over one instance's state array.
 __global__ void __launch_bounds__(kBlock) k_lincomb(long M, const double* __restrict__ a, const double* __restrict__ b,
                           double s, const double* __restrict__ c, double* __restrict__ d) {
@@ -1255,6 +1308,207 @@ __global__ void __launch_bounds__(kBlock) k_fwd_finish_z(Geo g, const double* __
   }
 }
 
+// The whole forward rhs of a 3-D region periodic in every direction, in one z march
+// (computeRhsForward, src/RhsHelperImpl.f90:254-354, with k_finish's operations for at
+// most one whole-domain Kolmogorov forcing and no dissipation or sources): a BX x BY
+// column tile marches along z over a chunk of ZC planes.  At each lead plane zl every
+// thread forms the contravariant fluxes of its tile point (from q and, viscous, the
+// stress / heat-flux slots S written by k_grad3d_periodic<.., true>) and of one x- or
+// y-halo point; the x and y fluxes go through LDS for D_x Fx + D_y Fy, which waits in a
+// 4-plane LDS ring, and Fz enters a 9-plane register window.  Four planes behind, at
+// zo = zl - 4, rhs = 0 - ((D_x Fx + D_y Fy) + D_z Fz) -- zo's ring slot is read before
+// zl's partial takes it over -- then the Jacobian scaling and the
+// forcing; with rk_stage > 0 the RK4 stage update follows and goes to yout, which must
+// not be q (other blocks still read q), and rhs is not written.  The fluxes, sums and
+// finishing operations are those of k_fluxdiv3d_xy and k_fwd_finish_z, in the same
+// order, so the values are the same; no flux or partial rhs goes through HBM.
+template <int BX, int BY, int ZC>
+__global__ void __launch_bounds__(BX * BY, 2) k_fwd3d_march(Geo g, const double* __restrict__ q,
+                                                          const double* __restrict__ S,
+                                                          const double* __restrict__ kstr,
+                                                          double* __restrict__ rhs, int rk_stage,
+                                                          double dt, double* __restrict__ yout,
+                                                          double* __restrict__ b1,
+                                                          double* __restrict__ b2,
+                                                          double* __restrict__ copy) {
+  constexpr int NV = 5, NT = BX * BY, EXW = BX + 8, EYW = BY + 8;
+  static_assert(8 * (BX + BY) == NT, "one halo point per thread");
+  __shared__ double sfx[NV][BY][EXW];
+  __shared__ double sfy[NV][EYW][BX];
+  __shared__ double ring[4][NV][NT];
+  const int ntx = g.n[0] / BX;
+  const int x0 = (blockIdx.x % ntx) * BX, y0 = (blockIdx.x / ntx) * BY;
+  const int z0 = blockIdx.y * ZC;
+  const int z1 = min(z0 + ZC, g.n[2]);
+  const long inst = blockIdx.z;
+  const double* qi = q + inst * NV * g.N;
+  const double* Si = g.viscous ? S + inst * 12 * g.N : nullptr;
+  const long n01 = (long)g.n[0] * g.n[1];
+  const int t = threadIdx.x, tx = t % BX, ty = t / BX;
+  const int i = x0 + tx, j = y0 + ty;
+  // this thread's halo point: [0, 8 BY) the x-halo columns, [8 BY, NT) the y-halo rows
+  int hx, hy, hi, hj, hmask;
+  if (t < 8 * BY) {
+    const int c = t % 8;
+    hy = t / 8;
+    hx = c < 4 ? c : BX + c;
+    hi = wrap(x0 + hx - 4, g.dir[0]);
+    hj = y0 + hy;
+    hmask = 1;
+  } else {
+    const int t2 = t - 8 * BY, r = t2 / BX;
+    hx = t2 % BX;
+    hy = r < 4 ? r : BY + r;
+    hi = x0 + hx;
+    hj = wrap(y0 + hy - 4, g.dir[1]);
+    hmask = 2;
+  }
+  // contravariant fluxes of the directions in dmask at grid point (ii, jj, kk)
+  auto fluxes = [&](int ii, int jj, int kk, int dmask, double (*out)[NV]) {
+    const long pp = ii + (long)g.n[0] * jj + n01 * kk;
+    double v[NV];
+    load_q<3>(qi, g.N, pp, v);
+    const Prim<3> s = prim<3>(v, g.gamma);
+    double tau[9], hf[3];
+    if (g.viscous) {
+      // the stress tensor is symmetric: tau(i, d) = S of the pair
+      const double sxx = (dmask & 1) ? Si[0 * g.N + pp] : 0.0;
+      const double sxy = (dmask & 3) ? Si[1 * g.N + pp] : 0.0;
+      const double sxz = (dmask & 5) ? Si[2 * g.N + pp] : 0.0;
+      const double syy = (dmask & 2) ? Si[3 * g.N + pp] : 0.0;
+      const double syz = (dmask & 6) ? Si[4 * g.N + pp] : 0.0;
+      const double szz = (dmask & 4) ? Si[5 * g.N + pp] : 0.0;
+      tau[0] = sxx; tau[1] = sxy; tau[2] = sxz;
+      tau[3] = sxy; tau[4] = syy; tau[5] = syz;
+      tau[6] = sxz; tau[7] = syz; tau[8] = szz;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) hf[d] = (dmask >> d & 1) ? Si[(6 + d) * g.N + pp] : 0.0;
+    }
+    const int ijk[3] = {ii, jj, kk};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (!(dmask >> d & 1)) continue;
+      double f[NV];
+      inviscid_flux<3>(v, s, d, f);
+      if (g.viscous) {
+        double fv[NV];
+        viscous_flux<3>(s.u, tau, hf, d, fv);
+#pragma unroll
+        for (int l = 0; l < NV; ++l) f[l] = f[l] - fv[l];
+      }
+      const double m = metric<3>(g, d, ijk);
+#pragma unroll
+      for (int l = 0; l < NV; ++l) out[d][l] = m * f[l];
+    }
+  };
+  const Op& op = g.ops[SLOT_FIRST];
+  const double c1 = op.c[kMaxHW + 1], c2 = op.c[kMaxHW + 2], c3 = op.c[kMaxHW + 3], c4 = op.c[kMaxHW + 4];
+  const Dir dz = g.dir[2];
+  double W[9][NV];
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int l = 0; l < NV; ++l) W[k][l] = 0.0;
+  for (int zl = z0 - 4; zl < z1 + 4; ++zl) {      // uniform over the block
+    const int kz = wrap(zl, dz);
+    const bool xy = zl >= z0 && zl < z1;
+    double f[3][NV];
+    fluxes(i, j, kz, xy ? 7 : 4, f);
+#pragma unroll
+    for (int l = 0; l < NV; ++l) W[8][l] = f[2][l];
+    if (zl >= z0 + 4) {
+      const int zo = zl - 4;
+      const long p = i + (long)g.n[0] * j + n01 * zo;
+      double r[NV];
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        double out = 0.0;
+        out += c1 * (W[5][l] - W[3][l]);
+        out += c2 * (W[6][l] - W[2][l]);
+        out += c3 * (W[7][l] - W[1][l]);
+        out += c4 * (W[8][l] - W[0][l]);
+        const double v = ring[zo & 3][l][t] + out;
+        r[l] = 0.0 - v;
+      }
+      int ijk[3] = {i, j, zo};
+      const double jac = jacobian<3>(g, ijk);
+#pragma unroll
+      for (int l = 0; l < NV; ++l) r[l] = r[l] * jac;
+      if (kstr) r[1] = r[1] + qi[p] * kstr[p];
+      if (rk_stage == 0) {
+#pragma unroll
+        for (int l = 0; l < NV; ++l) rhs[inst * NV * g.N + l * g.N + p] = r[l];
+      } else {
+#pragma unroll
+        for (int l = 0; l < NV; ++l) {
+          const long ii = inst * NV * g.N + l * g.N + p;
+          const double rl = r[l];
+          double qn;
+          switch (rk_stage) {
+            case 1: {
+              const double q0 = q[ii];
+              b1[ii] = q0;
+              b2[ii] = q0 + div6(dt * rl);
+              qn = q0 + dt * rl / 2.0;
+              break;
+            }
+            case 2:
+              b2[ii] = b2[ii] + div3(dt * rl);
+              qn = b1[ii] + dt * rl / 2.0;
+              break;
+            case 3:
+              b2[ii] = b2[ii] + div3(dt * rl);
+              qn = b1[ii] + dt * rl;
+              break;
+            default:
+              qn = b2[ii] + div6(dt * rl);
+              break;
+          }
+          yout[ii] = qn;
+          if (copy) copy[ii] = qn;
+        }
+      }
+    }
+    if (xy) {
+      double fh[3][NV];
+      fluxes(hi, hj, kz, hmask, fh);
+      __syncthreads();          // readers of the previous plane's fluxes are done
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        sfx[l][ty][tx + 4] = f[0][l];
+        sfy[l][ty + 4][tx] = f[1][l];
+      }
+      if (hmask == 1) {
+#pragma unroll
+        for (int l = 0; l < NV; ++l) sfx[l][hy][hx] = fh[0][l];
+      } else {
+#pragma unroll
+        for (int l = 0; l < NV; ++l) sfy[l][hy][hx] = fh[1][l];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        const double* rx = &sfx[l][ty][tx + 4];
+        const double* cy = &sfy[l][ty + 4][tx];
+        double ox = 0.0, oy = 0.0;
+        ox += c1 * (rx[1] - rx[-1]);
+        ox += c2 * (rx[2] - rx[-2]);
+        ox += c3 * (rx[3] - rx[-3]);
+        ox += c4 * (rx[4] - rx[-4]);
+        oy += c1 * (cy[BX] - cy[-BX]);
+        oy += c2 * (cy[2 * BX] - cy[-2 * BX]);
+        oy += c3 * (cy[3 * BX] - cy[-3 * BX]);
+        oy += c4 * (cy[4 * BX] - cy[-4 * BX]);
+        ring[zl & 3][l][t] = ox + oy;   // each thread reads back only its own slots
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int l = 0; l < NV; ++l) W[k][l] = W[k + 1][l];
+  }
+}
+
 // The z pass of the adjoint's second derivative pass (D-dagger of the viscous block
 // T1[z] components 1..4) fused with k_finish's adjoint operations, for 3-D regions whose
 // patches are at most one Kolmogorov forcing over the whole domain, with no dissipation
@@ -1482,12 +1736,34 @@ static void launch_gradients(const Geo& g, const double* q, double* w, double* G
   for (int d = 0; d < ND; ++d) k_grad<ND><<<grid, kBlock, 0, st>>>(g, d, w, G);
 }
 
-bool launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
-                        double* G, double* t, double* F, const double* control, long control_stride,
-                        const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
-                        const FinishInfo& fin, hipStream_t st, const RkStage* rk) {
+// The one-march forward kernel: 3-D, periodic in every direction, x / y sizes multiples
+// of its 16 x 16 tile; viscous regions also need the stress kernel (grad3d_applies).
+constexpr int kFMX = 16, kFMY = 16, kFMZ = 32;
+static bool fwd3d_march_applies(const Geo& g) {
+  return g.nd == 3 && g.nv == 5 && g.dir[0].periodic && g.dir[1].periodic && g.dir[2].periodic &&
+         g.n[0] % kFMX == 0 && g.n[1] % kFMY == 0 && (!g.viscous || grad3d_applies(g));
+}
+
+int launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
+                       double* G, double* t, double* F, const double* control, long control_stride,
+                       const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
+                       const FinishInfo& fin, hipStream_t st, const RkStage* rk) {
   const dim3 grid = grid_pts(g);
   const int has_farfield = fin.has_farfield;
+  if (F && fin.simple && !g.dissipation && nsrc == 0 && fwd3d_march_applies(g)) {
+    // the whole rhs in one z march; with an RK stage only out of place (rk->yout)
+    if (g.viscous) {
+      const dim3 gg((g.n[0] / kG3X) * (g.n[1] / kG3Y), (g.n[2] + kG3Z - 1) / kG3Z, g.ninst);
+      k_grad3d_periodic<kG3X, kG3Y, kG3Z, true><<<gg, kG3X * kG3Y, 0, st>>>(g, q, G);
+    }
+    const bool rk_out = rk && rk->yout;
+    const dim3 gm((g.n[0] / kFMX) * (g.n[1] / kFMY), (g.n[2] + kFMZ - 1) / kFMZ, g.ninst);
+    k_fwd3d_march<kFMX, kFMY, kFMZ><<<gm, kFMX * kFMY, 0, st>>>(
+        g, q, G, fin.kolmogorov, rhs, rk_out ? rk->stage : 0, rk_out ? rk->dt : 0.0,
+        rk_out ? rk->yout : nullptr, rk_out ? rk->b1 : nullptr, rk_out ? rk->b2 : nullptr,
+        rk_out ? rk->copy : nullptr);
+    return rk_out ? RK_TO_YOUT : RK_NOT_APPLIED;
+  }
   bool finished = false;
   MGD_DISPATCH(g.nd, {
     if (g.viscous) launch_gradients<ND>(g, q, w, G, grid, st);
@@ -1536,7 +1812,7 @@ bool launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double*
                                             control_stride, nullptr, 0, 0.0, src, nsrc, src_amp,
                                             src_inst_stride, nullptr);
   });
-  return finished && rk != nullptr;
+  return finished && rk != nullptr ? RK_IN_PLACE : RK_NOT_APPLIED;
 }
 
 bool launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const double* lam,
@@ -1648,6 +1924,15 @@ void launch_inner_product(const Geo& g, const double* a, const double* b, double
     k_inner_partial<ND><<<dim3(nb, g.ninst), kRedBlock, 0, st>>>(g, a, b, partial);
   });
   k_reduce_final<<<g.ninst, kRedBlock, 0, st>>>(nb, partial, 1.0, nullptr, out);
+}
+
+void launch_kinetic_energy(const Geo& g, const double* q, double* lam, double* partial, double* J,
+                           hipStream_t st) {
+  const int nb = (int)((g.N + kRedBlock - 1) / kRedBlock);
+  MGD_DISPATCH(g.nd, {
+    k_kinetic_energy<ND><<<dim3(nb, g.ninst), kRedBlock, 0, st>>>(g, q, lam, partial);
+  });
+  k_reduce_final<<<g.ninst, kRedBlock, 0, st>>>(nb, partial, 1.0, nullptr, J);
 }
 
 void launch_lincomb(long M, const double* a, const double* b, double s, const double* c, double* d,

@@ -122,6 +122,7 @@ struct mgd_region {
     std::vector<double> control;         // actuator forcing of every stage (host copy)
   } ck;
   mgd::DevBuf qT, scr;                   // final forward state; scratch state (MGD_SCRATCH)
+  mgd::DevBuf q_alt, qs_alt;             // second state buffers of out-of-place RK stages
   mgd::DevBuf ipart, iout;               // inner-product partial sums and results
   GraphSlot graphs[2];                   // captured runs: forward, forward + adjoint
   std::vector<double> toff;              // per-instance time offsets (empty: none)
@@ -382,6 +383,8 @@ int ensure_state(mgd_region* r) {
       r->T1.alloc(M * r->nd))
     return MGD_ERR_HIP;
   if (r->desc.dissipation && !r->desc.composite_dissipation && r->t.alloc(M)) return MGD_ERR_HIP;
+  // 3-D regions: the z-march rhs kernel writes RK stage states out of place
+  if (r->nd == 3 && r->q_alt.alloc(M)) return MGD_ERR_HIP;
   if (r->desc.viscous) {
     if (r->w.alloc((size_t)r->ninst * (r->nd + 1) * r->N) ||
         r->G.alloc((size_t)r->ninst * (r->nd + 1) * r->nd * r->N))
@@ -463,9 +466,13 @@ struct Timer {
   Timer(mgd_region* r_, int s) : r(r_), slot(s) {}
 };
 
-// One forward RK4 substep on state `q` (all instances).
-void forward_stage(mgd_region* r, double* q, double* b1, double* b2, int sidx, int stage, double dt,
-                   double* qcopy) {
+// One forward RK4 substep on the state in `qb` (all instances).  Where the rhs kernels
+// write the new stage state out of place (the 3-D z march) it goes to `alt` and the two
+// buffers trade places, so qb holds the new state either way; every RK4 step has four
+// stages, so after whole steps each buffer is back where it started.
+void forward_stage(mgd_region* r, DevBuf& qb, DevBuf& alt, double* b1, double* b2, int sidx,
+                   int stage, double dt, double* qcopy) {
+  double* q = qb.p;
   const int na = n_act(r);
   const double* ctrl = nullptr;
   long cstride = 0;
@@ -475,12 +482,20 @@ void forward_stage(mgd_region* r, double* q, double* b1, double* b2, int sidx, i
   }
   const int ns = (int)r->sources.size();
   // the rhs launcher applies the RK update itself where its last kernel finishes the rhs
-  const RkStage rk{stage, dt, q, b1, b2, qcopy};
-  if (!launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p, r->G.p, r->t.p, r->T1.p, ctrl,
-                          cstride, reinterpret_cast<const Source*>(r->d_sources.p), ns,
-                          r->src_amp.p + (long)sidx * std::max(ns, 1), r->src_inst_stride, r->fin,
-                          r->stream, &rk))
-    launch_rk_forward((long)r->ninst * M_of(r), stage, dt, q, b1, b2, r->rhs.p, qcopy, r->stream);
+  const long M = (long)r->ninst * M_of(r);
+  RkStage rk{stage, dt, q, b1, b2, qcopy};
+  if (alt.p && (long)alt.n >= M) rk.yout = alt.p;   // allocated by the run preparation
+  const int done = launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p, r->G.p, r->t.p,
+                                      r->T1.p, ctrl, cstride,
+                                      reinterpret_cast<const Source*>(r->d_sources.p), ns,
+                                      r->src_amp.p + (long)sidx * std::max(ns, 1),
+                                      r->src_inst_stride, r->fin, r->stream, &rk);
+  if (done == RK_TO_YOUT) {
+    std::swap(qb.p, alt.p);
+    std::swap(qb.n, alt.n);
+  } else if (done == RK_NOT_APPLIED) {
+    launch_rk_forward(M, stage, dt, q, b1, b2, r->rhs.p, qcopy, r->stream);
+  }
 }
 
 bool has_walls(const mgd_region* r) {
@@ -820,7 +835,7 @@ int plain_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
   launch_copy(M, r->q.p, r->snaps.p, r->stream);
   for (int s = 0; s < nsteps; ++s) {
     for (int i = 1; i <= 4; ++i) {
-      forward_stage(r, r->q.p, r->b1.p, r->b2.p, 4 * s + i - 1, i, dt, nullptr);
+      forward_stage(r, r->q, r->q_alt, r->b1.p, r->b2.p, 4 * s + i - 1, i, dt, nullptr);
       if (nc > 0)
         launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), nc, 0, r->q.p, r->partial.p,
                             kRkNorm[i - 1] * dt, r->acc_J.p, nullptr, r->stream);
@@ -859,7 +874,7 @@ int plain_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
           for (int s2 = loaded + 1; s2 <= loaded + K && s2 <= nsteps; ++s2)
             for (int st2 = 1; st2 <= 4; ++st2) {
               const bool last = s2 == loaded + K && st2 == 4;
-              forward_stage(r, r->qs.p, r->b1.p, r->b2.p, 4 * (s2 - 1) + st2 - 1, st2, dt,
+              forward_stage(r, r->qs, r->qs_alt, r->b1.p, r->b2.p, 4 * (s2 - 1) + st2 - 1, st2, dt,
                             last ? nullptr : r->window.p + (long)idx * M);
               if (last) break;
               ++idx;
@@ -927,7 +942,7 @@ int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, c
   if (int rc = upload_run_inputs(r)) return rc;
   const int nsnap = nsteps / K + 1;
   if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M) || r->qs.alloc(M) ||
-      r->qT.alloc(M))
+      r->qT.alloc(M) || (r->nd == 3 && r->qs_alt.alloc(M)))
     return MGD_ERR_HIP;
   if (fused_ok(r)) {
     if (int rc = prepare_quadrature(r, nst, dt)) return rc;
@@ -989,7 +1004,7 @@ int enqueue_forward_run(mgd_region* r, double dt, int nsteps) {
   for (int s = 0; s < nsteps; ++s)
     for (int i = 1; i <= 4; ++i) {
       const int sidx = 4 * s + i - 1;
-      forward_stage(r, r->q.p, r->b1.p, r->b2.p, sidx, i, dt, nullptr);
+      forward_stage(r, r->q, r->q_alt, r->b1.p, r->b2.p, sidx, i, dt, nullptr);
       if (r->ip_cost >= 0)
         launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), n_cost(r), 0, r->q.p, r->partial.p,
                             kRkNorm[i - 1] * dt, r->acc_J.p, nullptr, r->stream);
@@ -1800,6 +1815,18 @@ int mgd_inner_product_noalloc(mgd_region* r, int a, int b, double* out_device) {
   if (!part) { set_error("scratch not set up on the region's device"); return MGD_ERR_STATE; }
   launch_inner_product(r->geo, A->p, B->p, part, out_device, r->stream);
   MGD_HIP(hipGetLastError());
+  return MGD_OK;
+}
+
+int mgd_terminal_cost(mgd_region* r, int kind, double* J) {
+  if (!r || !r->finalized || !J) { set_error("bad argument"); return MGD_ERR_ARG; }
+  if (kind != MGD_TERMINAL_KINETIC_ENERGY) { set_error("unknown terminal cost"); return MGD_ERR_ARG; }
+  const long nb = (r->N + 255) / 256;
+  if (r->ipart.alloc((size_t)nb * r->ninst) || r->iout.alloc(r->ninst)) return MGD_ERR_HIP;
+  launch_kinetic_energy(r->geo, r->q.p, r->lam.p, r->ipart.p, r->iout.p, r->stream);
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipMemcpyAsync(J, r->iout.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost, r->stream));
+  MGD_HIP(hipStreamSynchronize(r->stream));
   return MGD_OK;
 }
 

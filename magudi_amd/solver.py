@@ -170,6 +170,15 @@ class Region:
         check(capi.lib().mgd_functional(self.handle, dptr(out)))
         return out
 
+    def terminalKineticEnergy(self) -> np.ndarray:
+        """Kinetic energy J = sum_p H_p |rho u|^2 / (2 rho) of each instance's current
+        conserved state, computed on the device, which also writes dJ/dq as each
+        instance's adjoint state (the terminal condition for
+        ``Solver.runAdjointCheckpointed(terminal=True)``)."""
+        out = np.zeros(self.n_instances)
+        check(capi.lib().mgd_terminal_cost(self.handle, capi.TERMINAL_KINETIC_ENERGY, dptr(out)))
+        return out
+
     def grid(self, which: int) -> np.ndarray:
         size = {0: self.nd * self.nd, 1: 1, 2: 1, 3: self.nd}[which]
         out = np.zeros((size, self.npoints))
@@ -261,10 +270,11 @@ class Solver:
         the final time per instance, shape (n_instances, nv, N) or (nv, N) for one
         instance -- the reference's adjoint_nonzero_initial_condition
         (src/SolverImpl.f90:368-428, 1096-1102), e.g. the gradient of a cost on the final
-        state -- or None for a zero terminal state.  Returns (sensitivity, gradient) as
+        state --, True for the adjoint state already on the device (as
+        ``Region.terminalKineticEnergy`` leaves it), or None for a zero terminal state.  Returns (sensitivity, gradient) as
         runAdjoint does; afterwards each instance's adjoint state is the adjoint at t0."""
         r = self.region
-        if terminal is not None:
+        if terminal is not None and terminal is not True:
             t = np.asarray(terminal, dtype=np.float64).reshape(r.n_instances, r.nv, r.npoints)
             for i in range(r.n_instances):
                 r.states[i].adjointVariables = t[i]

@@ -54,6 +54,38 @@ def test_terminal_kinetic_energy_adjoint(n):
     assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
 
 
+@pytest.mark.parametrize("n", [16, 32])
+def test_device_terminal_kinetic_energy(n):
+    """The terminal cost on the device (mgd_terminal_cost): J and dJ/dq(T) of the final
+    state against the host formula on the same state, and the adjoint marched back from
+    the device terminal state bit for bit equal to the one from the host terminal state."""
+    from magudi_amd.solver import Region, Solver
+    nsteps, K = 6, 4
+    c = cases.kolmogorov3d(n=n, nsteps=nsteps)
+    q0 = c.initial_state()
+    R = Region(c, n_instances=2)
+    norm = R.grid(2)[0]
+    rng = np.random.default_rng(7)
+    q1 = q0.copy()
+    q1[1:4] += 0.01 * rng.standard_normal((3, c.npoints))
+    R.states[0].conservedVariables = q0
+    R.states[1].conservedVariables = q1
+    S = Solver(R, dt=c.dt, nsteps=nsteps, save_interval=K)
+    S.runForwardCheckpointed()
+    qT = [R.states[k].conservedVariables for k in range(2)]
+    J = R.terminalKineticEnergy()
+    lam_dev = [R.states[k].adjointVariables for k in range(2)]
+    S.runAdjointCheckpointed(terminal=True, want_gradient=False)
+    lam0_dev = [R.states[k].adjointVariables for k in range(2)]
+    host = [kinetic_energy(qT[k], norm, 3) for k in range(2)]
+    S.runAdjointCheckpointed(terminal=np.stack([h[1] for h in host]), want_gradient=False)
+    for k in range(2):
+        Jh, lamh = host[k]
+        assert abs(J[k] - Jh) <= 1e-13 * abs(Jh)
+        assert np.array_equal(lam_dev[k], lamh)
+        assert np.array_equal(R.states[k].adjointVariables, lam0_dev[k])
+
+
 def test_instances_match_single_runs():
     """Two instances of a 32 x 16 x 16 periodic box in one region (every 3-D kernel,
     including the finishing kernels with the RK updates, indexes the instance) against

@@ -79,6 +79,12 @@ def periodic_box(shape, seed=3, dissipation=True):
         target_mollifier=np.zeros(npts), dt=0.002, nsteps=1, save_interval=1, initial=q)
 
 
+def inviscid_box(shape):
+    """periodic_box without viscosity or dissipation: Euler fluxes only."""
+    import dataclasses
+    return dataclasses.replace(periodic_box(shape, dissipation=False), viscous=False)
+
+
 def kolmogorov_subbox(shape=(32, 16, 12)):
     """Periodic box whose two Kolmogorov forcings cover overlapping sub-boxes with a
     y-extent that is no whole period: the forcing's quadrature mean over the union of
@@ -113,6 +119,9 @@ CASES = {
     "box32x16x12_nodiss": lambda: periodic_box((32, 16, 12), dissipation=False),
     # rows longer than one 256-point segment; z too short for the line march
     "box512x16x9": lambda: periodic_box((512, 16, 9)),
+    # one z march for the whole forward rhs: several tiles in x and y, a partial z chunk
+    "box64x32x80_nodiss": lambda: periodic_box((64, 32, 80), dissipation=False),
+    "box48x16x40_inviscid_nodiss": lambda: inviscid_box((48, 16, 40)),
     "box32x16x12_kolmogorov_subboxes": kolmogorov_subbox,
     "monopole41_kolmogorov_subbox": monopole_kolmogorov,
 }
