@@ -1,19 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the magudi RHS hot path on one MI355X.
 
-Headline workload (one "step" = one full pass of the hot path): the 2-D acoustic
-monopole on a 401 x 401 grid, inviscid Euler, SBP 4-8 with dissipation, far-field
-SAT + sponges, forward run of 960 RK4 steps followed by the discrete adjoint over
-the same 960 steps with HBM snapshots every 32 steps and window recompute
-(Solver%runAdjoint with a uniform checkpointer).  Synthetic input: the quiescent
-state with the Gaussian monopole source and a control mollifier, all built in
-magudi_amd/cases.py.
+Headline workload (one "step" = one full pass of the hot path): the 3-D viscous
+periodic Kolmogorov flow on a 256^3 grid (SBP 4-8, Navier-Stokes with Re 100 at
+Mach 0.2, Pr 0.7, forcing f_x = A sin(2 pi 4 y), seeded solenoidal initial velocity),
+200 RK4 steps forward with HBM snapshots every 25 steps, J = kinetic energy of the
+final state and its gradient dJ/dq(T) on the device, then the discrete adjoint back to
+t0 with window recompute, whose result is the gradient dJ/dq0.  Every part of the pass
+runs inside the timed region; the initial condition is restored from a device copy.
+Synthetic input built in magudi_amd/cases.py.
 
-value = grid-point-steps per second of the forward+adjoint pass (N * 960 / time).
-Also reported: the 20-instance batched rate (finite-difference sweep shape), the
-generic per-operator ("plain") path, the HBM roofline fraction of the dominant
-kernel timed with HIP events on its own stream, and the CPU oracle on a bounded
-sample.
+value = grid-point-steps per second of the forward+adjoint pass (256^3 * 200 / time).
+Also reported: the dominant kernel (the 3-D z-march forward kernel) against the HBM
+roofline, timed with HIP events on the region stream, and its HBM traffic from two
+rocprofv3 counter passes run by this script; the 3-D forward and adjoint rhs; the 2-D
+monopole legs (401^2 forward + adjoint with the control gradient, 20-instance batches,
+the amplitude sweep, uneven batches and ensembles, the generic per-operator path); the
+CPU oracle on a bounded sample of the headline workload.
 """
 from __future__ import annotations
 
@@ -37,74 +40,124 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=401)
+    ap.add_argument("--c4-n", type=int, default=256, help="grid of the headline 3-D run")
+    ap.add_argument("--c4-nsteps", type=int, default=200, help="RK4 steps of the headline run")
+    ap.add_argument("--n", type=int, default=401, help="grid of the 2-D monopole legs")
     ap.add_argument("--nsteps", type=int, default=960)
     ap.add_argument("--save-interval", type=int, default=32)
     ap.add_argument("--batch", type=int, default=20)
     ap.add_argument("--no-extras", action="store_true",
-                    help="headline only (no batched / plain / CPU legs)")
-    ap.add_argument("--cpu-steps", type=int, default=64)
-    ap.add_argument("--c4-n", type=int, default=256,
-                    help="grid of the 3-D viscous Kolmogorov rhs leg (0: skip)")
+                    help="headline only (no 2-D, batched, plain or CPU legs)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes for roofline.traffic")
     ap.add_argument("--fd-sweep-only", action="store_true",
                     help="only the batched finite-difference sweep over control amplitudes")
     ap.add_argument("--kolmogorov-only", action="store_true",
-                    help="only the 3-D Kolmogorov forward + adjoint run (c4-n^3, 200 steps)")
+                    help="only the headline 3-D Kolmogorov forward + adjoint run (no extras, "
+                         "no counter passes)")
     ap.add_argument("--viscous3d-only", action="store_true",
                     help="only the 3-D viscous rhs leg (its profiling workload)")
+    ap.add_argument("--monopole-only", action="store_true",
+                    help="only the 401^2 monopole forward + adjoint legs")
     ap.add_argument("--tile-shape", type=int, default=-1,
                     help="tile shape of the fused 2-D kernels (-1: autotune)")
     ap.add_argument("--kernels-only", action="store_true",
-                    help="only the HIP-event timing of the stage kernels and the copy kernel "
-                         "(the workload of the rocprofv3 counter runs)")
+                    help="only the HIP-event timing of the headline's dominant kernel and the "
+                         "state copy (the workload of the rocprofv3 counter passes)")
+    ap.add_argument("--kernels-only-2d", action="store_true",
+                    help="the same for the fused 2-D stage kernels on the 401^2 monopole")
     return ap.parse_args()
 
 
-def measured_traffic():
-    """HBM bytes per launch from the committed rocprofv3 counter summary, or {}.
+def counter_pass(counter, args):
+    """One rocprofv3 counter pass (a child process) over `bench.py <args>`; returns
+    {kernel name: (dispatches, mean counter value per dispatch)} or None."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    out = tempfile.mkdtemp(prefix="mgd_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
+           sys.executable, os.path.join(ROOT, "bench.py"), *args]
+    try:
+        subprocess.run(cmd, cwd=out, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                       timeout=240, check=False)
+    except subprocess.TimeoutExpired:
+        return None
+    acc = {}
+    for path in glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                k = row["Kernel_Name"]
+                n, v = acc.get(k, (0, 0.0))
+                acc[k] = (n + 1, v + float(row["Counter_Value"]))
+    shutil.rmtree(out, ignore_errors=True)
+    return {k: (n, v / n) for k, (n, v) in acc.items()} or None
 
-    Lines of the form `traffic <kernel> <bytes per launch>` in
-    profiles/pmc_traffic_monopole401.txt (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy-kernel calibration factor written in the same file)."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_monopole401.txt")
-    out = {}
-    if os.path.exists(path):
-        for line in open(path):
-            f = line.split()
-            if len(f) == 3 and f[0] == "traffic":
-                out[f[1]] = float(f[2])
-    return out
+
+def measured_traffic(args, kernel, copy_bytes):
+    """HBM bytes per launch of `kernel` (a substring of its name) from FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch), each scaled by what the same counter reads for the state
+    copy kernel of known size (copy_bytes read and copy_bytes written)."""
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        c = counter_pass(counter, args)
+        if not c:
+            return None
+        kv = [v for k, (n, v) in c.items() if kernel in k]
+        cv = [v for k, (n, v) in c.items() if "k_copy" in k]
+        if not kv or not cv:
+            return None
+        factor = copy_bytes / (cv[0] * 1024.0)
+        res[counter] = {"kib_per_dispatch": kv[0], "copy_kib": cv[0], "factor": factor,
+                        "bytes": kv[0] * 1024.0 * factor}
+    return {"bytes": res["FETCH_SIZE"]["bytes"] + res["WRITE_SIZE"]["bytes"], "passes": res}
 
 
 def run_hot_path(R, S, steps):
-    """steps x (restart from the resident initial condition + forward + adjoint)."""
+    """steps x (restart from the resident initial condition + forward + adjoint with the
+    control gradient)."""
     R.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
         R.restart()
-        out = S.runAdjoint(want_gradient=False)
+        out = S.runAdjoint(want_gradient=True)
     R.sync()
     return (time.perf_counter() - t0) / steps, out
 
 
-def cpu_baseline(n, nsteps, save_interval):
-    """CPU oracle (plain C; its per-point and per-pencil loops on every OpenMP thread the
-    host gives, OMP_NUM_THREADS) on a bounded sample of the same workload."""
+def cpu_baseline(n=48, nsteps=8, save_interval=4):
+    """The CPU oracle (plain C; its per-point and per-pencil loops on every OpenMP thread
+    the host gives, OMP_NUM_THREADS) on a bounded sample of the headline workload: the
+    same Kolmogorov flow on an n^3 grid for nsteps RK4 steps, forward with snapshots,
+    terminal kinetic energy, adjoint back from dJ/dq(T).  The terminal state comes from an
+    untimed forward run first (the oracle's adjoint driver redoes the forward pass)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
     from magudi_amd import cases
-    c = cases.acoustic_monopole(n=n, nsteps=nsteps, save_interval=save_interval)
+    from magudi_amd.functionals import kinetic_energy
+    c = cases.kolmogorov3d(n=n, nsteps=nsteps)
     O = ob.Region(c)
     q0 = c.initial_state()
+    norm = O.grid(2)[0]
     threads = ob.lib().orc_threads()
+    _, qT = O.run_forward(q0, c.dt, nsteps)
     t0 = time.perf_counter()
-    O.run_adjoint(q0, c.dt, nsteps, save_interval)
+    _, lamT = kinetic_energy(qT, norm, 3)
+    O.run_adjoint_ic(q0, c.dt, nsteps, save_interval, lamT=lamT)
     dt = time.perf_counter() - t0
     return {"value": c.npoints * nsteps / dt, "unit": "grid-pt-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"{n}x{n} monopole forward+adjoint, {nsteps} RK4 steps, snapshots every "
-                      f"{save_interval} (oracle/ C restatement, {threads} OpenMP threads, "
-                      f"{dt:.1f} s wall)"}
+            "sample": f"Kolmogorov {n}^3 (the headline flow on a smaller grid), {nsteps} RK4 "
+                      f"steps forward with snapshots every {save_interval} + terminal kinetic "
+                      f"energy + adjoint (oracle/ C restatement, {threads} OpenMP threads, "
+                      f"{dt:.1f} s wall, {dt * threads:.0f} s of CPU work)"}
 
 
 def viscous3d_leg(n):
@@ -120,7 +173,8 @@ def viscous3d_leg(n):
     a_ms, a_b = R.time_kernel(capi.KERNEL_RHS_ADJOINT, reps=5)
     del R
     leg = {"workload": f"Kolmogorov {n}^3 viscous periodic rhs (Re_ac 500, Pr 0.7)",
-           "path": "3-D per-operator kernels (periodic gradient / flux-divergence / line derivatives)"}
+           "path": "3-D kernels (forward: stress pass + z march; adjoint: periodic gradient, "
+                   "line / row derivatives, Jacobian products, fused finishing pass)"}
     for k, ms, b in (("forward_rhs", f_ms, f_b), ("adjoint_rhs", a_ms, a_b)):
         gbs = b / (ms * 1e-3) / 1e9
         leg[k] = {"ms": round(ms, 4), "algorithmic_bytes": b, "achieved_GBs": round(gbs, 1),
@@ -128,45 +182,45 @@ def viscous3d_leg(n):
     return leg
 
 
-def kolmogorov_run_leg(n, nsteps=200):
-    """The 3-D viscous periodic Kolmogorov flow as the configuration runs it: nsteps RK4
-    steps forward with HBM snapshots (every nsteps / 8), J = kinetic energy of the final
-    state, then the discrete adjoint back from dJ/dq(T) with window recompute.  A short
-    run of one window first allocates the buffers; then one timed run."""
-    from magudi_amd import cases
-    from magudi_amd.functionals import kinetic_energy
-    from magudi_amd.solver import Region, Solver
-    c = cases.kolmogorov3d(n=n, nsteps=nsteps)
-    q0 = c.initial_state()
-    R = Region(c)
-    norm = R.grid(2)[0]
+class KolmogorovRun:
+    """The headline: the 3-D viscous periodic Kolmogorov flow as the configuration runs it.
+    One pass = restart from the resident initial condition, nsteps RK4 steps forward with
+    HBM snapshots (every nsteps / 8), J = kinetic energy of the final state and dJ/dq(T) on
+    the device, the discrete adjoint back from dJ/dq(T) with window recompute (its result,
+    the adjoint state at t0, is dJ/dq0 and stays on the device)."""
 
-    def run(steps):
-        S = Solver(R, dt=c.dt, nsteps=steps, save_interval=c.save_interval)
-        R.states[0].conservedVariables = q0
-        R.sync()
+    def __init__(self, n, nsteps):
+        from magudi_amd import cases
+        from magudi_amd.solver import Region, Solver
+        self.case = cases.kolmogorov3d(n=n, nsteps=nsteps)
+        self.R = Region(self.case)
+        self.R.states[0].initialCondition = self.case.initial_state()
+        self.S = Solver(self.R, dt=self.case.dt, nsteps=nsteps,
+                        save_interval=self.case.save_interval)
+        self.pts = self.case.npoints * nsteps
+
+    def one(self):
+        self.R.restart()
+        self.S.runForwardCheckpointed()
+        J = self.R.terminalKineticEnergy()
+        self.S.runAdjointCheckpointed(terminal=True, want_gradient=False)
+        return J
+
+    def timed(self, steps):
+        """(seconds per pass, J, forward seconds of the last pass)"""
+        self.R.sync()
         t0 = time.perf_counter()
-        S.runForwardCheckpointed()
-        R.sync()
+        for _ in range(steps):
+            J = self.one()
+        self.R.sync()
+        t = (time.perf_counter() - t0) / steps
+        # forward half alone, for the split
+        self.R.restart()
+        self.R.sync()
         t1 = time.perf_counter()
-        J, lamT = kinetic_energy(R.states[0].conservedVariables, norm, 3)
-        t2 = time.perf_counter()
-        S.runAdjointCheckpointed(terminal=lamT, want_gradient=False)
-        R.sync()
-        t3 = time.perf_counter()
-        return J, t1 - t0, t3 - t2, t2 - t1
-
-    run(c.save_interval)
-    J, tf, ta, th = run(nsteps)
-    lam_max = float(abs(R.states[0].adjointVariables).max())
-    del R
-    t = tf + ta
-    return {"workload": f"Kolmogorov {n}^3 viscous periodic, {nsteps} RK4 steps forward + "
-                        f"adjoint from the terminal kinetic energy, snapshots every "
-                        f"{c.save_interval} steps with recompute",
-            "value": n ** 3 * nsteps / t, "unit": "grid-pt-steps/s", "seconds": round(t, 3),
-            "forward_s": round(tf, 3), "adjoint_s": round(ta, 3),
-            "host_terminal_s": round(th, 3), "J": J, "max_abs_lambda0": lam_max}
+        self.S.runForwardCheckpointed()
+        self.R.sync()
+        return t, float(J[0]), time.perf_counter() - t1
 
 
 def fd_sweep_leg(n=401, nsteps=960, save_interval=32, batch=20):
@@ -250,27 +304,27 @@ def uneven_leg(nsteps=32, save_interval=8):
     ninst = [4 if c.name.startswith("mixing") else 1 for c in cs]
     G = RegionGroup(cs, n_instances=ninst)
     G.restart()
-    G.run_adjoint(want_gradient=False)          # autotune and warm up
+    G.run_adjoint(want_gradient=True)          # autotune and warm up
     G.restart()
     G.sync()
     t0 = time.perf_counter()
-    G.run_adjoint(want_gradient=False)
+    G.run_adjoint(want_gradient=True)
     G.sync()
     t_group = time.perf_counter() - t0
     G.restart()
     G.sync()
     t0 = time.perf_counter()
     for s in G.solvers:
-        s.runAdjoint(want_gradient=False)
+        s.runAdjoint(want_gradient=True)
     G.sync()
     t_seq = time.perf_counter() - t0
     # one launch per RK stage for the 18 fused 2-D members, the 1-D waves beside them
     G.restart()
-    G.run_adjoint_merged(want_gradient=False)   # tables and warm-up
+    G.run_adjoint_merged(want_gradient=True)   # tables and warm-up
     G.restart()
     G.sync()
     t0 = time.perf_counter()
-    G.run_adjoint_merged(want_gradient=False)
+    G.run_adjoint_merged(want_gradient=True)
     G.sync()
     t_merged = time.perf_counter() - t0
     merged = len(G._merged_members())
@@ -305,11 +359,11 @@ def small_ensemble_leg(count=32, nsteps=32, save_interval=8, seed=3):
         G.sync()
         return time.perf_counter() - t0
     G.restart()
-    G.run_adjoint_merged(want_gradient=False)
-    t_m = timed(lambda: G.run_adjoint_merged(want_gradient=False))
-    G.run_adjoint(want_gradient=False)
-    t_c = timed(lambda: G.run_adjoint(want_gradient=False))
-    t_s = timed(lambda: [s.runAdjoint(want_gradient=False) for s in G.solvers])
+    G.run_adjoint_merged(want_gradient=True)
+    t_m = timed(lambda: G.run_adjoint_merged(want_gradient=True))
+    G.run_adjoint(want_gradient=True)
+    t_c = timed(lambda: G.run_adjoint(want_gradient=True))
+    t_s = timed(lambda: [s.runAdjoint(want_gradient=True) for s in G.solvers])
     G.close()
     pts = G.grid_point_steps
     return {"workload": f"{count} monopoles, odd sizes in [41, 129] (seed {seed}); fwd+adjoint "
@@ -318,31 +372,124 @@ def small_ensemble_leg(count=32, nsteps=32, save_interval=8, seed=3):
             "unit": "grid-pt-steps/s", "merged_speedup_vs_sequential": t_s / t_m}
 
 
+def monopole_legs(a, extras):
+    """The 2-D acoustic monopole (401^2, 960 RK4 steps, snapshots every 32) forward + adjoint
+    with the control gradient on the fused stage kernels, its dominant kernel against the
+    HBM roofline, and (extras) the batched, generic, resident-state, sweep, uneven-batch,
+    ensemble and small-grid legs."""
+    from magudi_amd import capi, cases
+    from magudi_amd.solver import Region, Solver
+    case = cases.acoustic_monopole(n=a.n, nsteps=a.nsteps, save_interval=a.save_interval)
+    q0 = case.initial_state()
+    R = Region(case, tile_shape=a.tile_shape)
+    R.states[0].initialCondition = q0
+    S = Solver(R)
+    run_hot_path(R, S, 1)
+    t_step, (sens, _, J) = run_hot_path(R, S, 2)
+    pts = case.npoints * a.nsteps
+    value = pts / t_step
+    R.restart()
+    fwd_ms, fwd_b = R.time_kernel(capi.KERNEL_FWD_STAGE, reps=50)
+    adj_ms, adj_b = R.time_kernel(capi.KERNEL_ADJ_STAGE, reps=50)
+    out = {"workload": f"monopole {a.n}x{a.n} forward+adjoint with the control gradient, "
+                       f"{a.nsteps} RK4 steps, HBM snapshots every {a.save_interval} steps "
+                       f"with recompute",
+           "value": value, "unit": "grid-pt-steps/s", "ms_per_run": t_step * 1e3,
+           "path": "fused 2-D stage kernels",
+           "tile_shape": ["32x16", "16x32", "16x16", "24x16", "16x16/2", "16x12/2"][R.tile_shape]
+           if R.tile_shape >= 0 else "generic",
+           "kernels": {k: {"avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": b,
+                           "achieved_GBs": round(b / (ms * 1e-3) / 1e9, 1),
+                           "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                       for k, ms, b in (("k_fwd_stage2d", fwd_ms, fwd_b),
+                                        ("k_adj_stage2d", adj_ms, adj_b))},
+           # whole run against the compulsory-traffic model of one fwd+adjoint pt-step with
+           # window recompute, per state component: forward RK4 128 B, adjoint RK4 128 B,
+           # reading the 4 forward stage states 32 B, recompute 128 B, storing its stages 32 B
+           "job_roofline": {"bytes_per_pt_step": 448 * 4,
+                            "achieved_GBs": round(value * 448 * 4 / 1e9, 1),
+                            "frac": round(value * 448 * 4 / 1e9 / HBM_PEAK_GBS, 4)},
+           "J": float(J[0]), "Grad0": float(sens[0])}
+    if not extras:
+        return out
+    Rb = Region(case, n_instances=a.batch)
+    for i in range(a.batch):
+        Rb.states[i].initialCondition = q0
+    Sb = Solver(Rb)
+    run_hot_path(Rb, Sb, 1)
+    tb, _ = run_hot_path(Rb, Sb, 1)
+    del Rb, Sb
+    # generic per-operator path on one instance, fewer steps, same per-step work
+    ns_plain = max(a.save_interval, a.nsteps // 10)
+    Rp = Region(case, force_plain=True)
+    Rp.states[0].initialCondition = q0
+    Sp = Solver(Rp, nsteps=ns_plain)
+    run_hot_path(Rp, Sp, 1)
+    tp, _ = run_hot_path(Rp, Sp, 1)
+    del Rp, Sp
+    out["batched"] = {"instances": a.batch, "value": a.batch * pts / tb, "ms_per_run": tb * 1e3,
+                      "speedup_vs_single_fused": (a.batch * pts / tb) / value}
+    out["plain"] = {"value": case.npoints * ns_plain / tp, "nsteps": ns_plain,
+                    "ms_per_run": tp * 1e3,
+                    "batched_speedup_vs_plain": (a.batch * pts / tb) / (case.npoints * ns_plain / tp)}
+    # every RK stage state resident in HBM (19.8 GB at 401^2 x 960 steps) instead of
+    # snapshots + window recompute: same results, no recompute
+    Rr = Region(case, tile_shape=R.tile_shape, resident_states=True)
+    Rr.states[0].initialCondition = q0
+    Sr = Solver(Rr)
+    run_hot_path(Rr, Sr, 1)
+    tr, (sens_r, _, J_r) = run_hot_path(Rr, Sr, 1)
+    del Rr, Sr
+    out["resident_states"] = {"value": pts / tr, "ms_per_run": tr * 1e3,
+                              "state_bytes": 8 * 4 * case.npoints * (4 * a.nsteps + 1),
+                              "same_J_and_Grad0": bool(J_r[0] == J[0] and sens_r[0] == sens[0])}
+    return out
+
+
+def kernels_only(a):
+    """HIP-event timing of the headline's dominant kernel and of the state copy of the same
+    region (the calibration the counter passes scale by)."""
+    from magudi_amd import capi
+    K = KolmogorovRun(a.c4_n, 8)
+    K.R.restart()
+    res = {}
+    for name, k in (("k_fwd3d_march", capi.KERNEL_FWD_MARCH3D), ("k_copy", capi.KERNEL_COPY)):
+        ms, b = K.R.time_kernel(k, reps=3)
+        res[name] = {"avg_launch_us": ms * 1e3, "bytes_per_launch": b}
+    return res
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
               "(batched instances and ensemble members would shard across GPUs, one "
-              "process per GPU, without communication)", file=sys.stderr)
+              "process per GPU, without communication; a single large grid as z slabs with "
+              "a halo exchange per stage)", file=sys.stderr)
         return 2
-    from magudi_amd import capi, cases
-    from magudi_amd.solver import Region, Solver
+    from magudi_amd import capi
 
+    if a.kolmogorov_only:
+        a.no_extras = a.no_traffic = True
     if a.viscous3d_only:
         print(json.dumps({"viscous3d": viscous3d_leg(a.c4_n)}), flush=True)
-        return 0
-    if a.kolmogorov_only:
-        print(json.dumps({"kolmogorov_run": kolmogorov_run_leg(a.c4_n)}), flush=True)
         return 0
     if a.fd_sweep_only:
         print(json.dumps({"fd_sweep": fd_sweep_leg(a.n, a.nsteps, a.save_interval, a.batch)}),
               flush=True)
         return 0
-    case = cases.acoustic_monopole(n=a.n, nsteps=a.nsteps, save_interval=a.save_interval)
-    q0 = case.initial_state()
-    R = Region(case, tile_shape=a.tile_shape)
-    R.states[0].initialCondition = q0
+    if a.monopole_only:
+        print(json.dumps({"monopole401": monopole_legs(a, extras=False)}), flush=True)
+        return 0
     if a.kernels_only:
+        print(json.dumps({"kernels_only": kernels_only(a)}), flush=True)
+        return 0
+    if a.kernels_only_2d:
+        from magudi_amd import cases
+        from magudi_amd.solver import Region
+        case = cases.acoustic_monopole(n=a.n, nsteps=a.nsteps, save_interval=a.save_interval)
+        R = Region(case, tile_shape=a.tile_shape)
+        R.states[0].initialCondition = case.initial_state()
         R.restart()
         res = {}
         for name, k in (("k_fwd_stage2d", capi.KERNEL_FWD_STAGE),
@@ -350,112 +497,65 @@ def main():
             ms, b = R.time_kernel(k, reps=5)
             res[name] = {"avg_launch_us": ms * 1e3, "bytes_per_launch": b}
         res["tile_shape"] = R.tile_shape
-        # calibration copy larger than the L2 of all XCDs (a.batch states)
-        Rb = Region(case, n_instances=a.batch)
-        for i in range(a.batch):
-            Rb.states[i].initialCondition = q0
-        Rb.restart()
-        ms, b = Rb.time_kernel(capi.KERNEL_COPY, reps=5)
-        res["k_copy_batched"] = {"avg_launch_us": ms * 1e3, "bytes_per_launch": b}
         print(json.dumps({"kernels_only": res}), flush=True)
         return 0
-    S = Solver(R)
-    for _ in range(a.warmup):
-        run_hot_path(R, S, 1)
-    t_step, (sens, _, J) = run_hot_path(R, S, a.steps)
-    pts = case.npoints * a.nsteps
-    value = pts / t_step
 
-    # dominant kernel: HIP-event timing of the fused stage kernels on the region stream
-    # (a full RK4 step of 4 stage launches per repetition; bytes = average per launch)
-    R.restart()
-    fwd_ms, fwd_b = R.time_kernel(capi.KERNEL_FWD_STAGE, reps=50)
-    adj_ms, adj_b = R.time_kernel(capi.KERNEL_ADJ_STAGE, reps=50)
-    cpy_ms, cpy_b = R.time_kernel(capi.KERNEL_COPY, reps=50)
-    launches_fwd = 8 * a.nsteps          # forward pass + window recompute
-    launches_adj = 4 * a.nsteps
-    kernels = {
-        "k_fwd_stage2d": {"ms": fwd_ms, "bytes": fwd_b, "launches": launches_fwd},
-        "k_adj_stage2d": {"ms": adj_ms, "bytes": adj_b, "launches": launches_adj},
-    }
-    dom = max(kernels, key=lambda k: kernels[k]["ms"] * kernels[k]["launches"])
-    d = kernels[dom]
-    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
-    traffic = measured_traffic().get(dom) if a.n == 401 else None
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+    # ---- headline: the 3-D Kolmogorov forward + adjoint pass
+    K = KolmogorovRun(a.c4_n, a.c4_nsteps)
+    for _ in range(max(1, a.warmup)):
+        K.one()
+    t_step, J, t_fwd = K.timed(a.steps)
+    value = K.pts / t_step
+    print(f"headline: {t_step:.3f} s per pass ({value:.4g} grid-pt-steps/s)", file=sys.stderr,
+          flush=True)
+    # dominant kernel (by launches x time: 8 nsteps forward launches per pass, counting the
+    # window recompute, against 4 nsteps launches of the largest adjoint kernel)
+    K.R.restart()
+    ms, b = K.R.time_kernel(capi.KERNEL_FWD_MARCH3D, reps=3)
+    cms, cb = K.R.time_kernel(capi.KERNEL_COPY, reps=3)
+    f_ms, f_b = K.R.time_kernel(capi.KERNEL_RHS_FORWARD, reps=3)
+    a_ms, a_b = K.R.time_kernel(capi.KERNEL_RHS_ADJOINT, reps=3)
+    M_bytes = 8.0 * 5 * K.case.npoints
+    del K
+    achieved = b / (ms * 1e-3) / 1e9
+    traffic = None
+    if not a.no_traffic:
+        tr = measured_traffic(["--kernels-only", "--c4-n", str(a.c4_n)], "k_fwd3d_march", M_bytes)
+        traffic = tr["bytes"] if tr else None
+    roofline = {"bound": "hbm", "kernel": "k_fwd3d_march", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "avg_launch_us": round(d["ms"] * 1e3, 3),
-                "bytes_per_launch": d["bytes"]}
-    other = {k: {"avg_launch_us": round(v["ms"] * 1e3, 3),
-                 "achieved_GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)}
-             for k, v in kernels.items()}
-    other["state_copy"] = {"avg_launch_us": round(cpy_ms * 1e3, 3),
-                           "achieved_GBs": round(cpy_b / (cpy_ms * 1e-3) / 1e9, 1)}
-    kernel_share = sum(v["ms"] * v["launches"] for v in kernels.values()) / (t_step * 1e3)
-
+                "traffic": traffic, "avg_launch_us": round(ms * 1e3, 2), "bytes_per_launch": b}
+    n = a.c4_n
     out = {
         "metric": METRIC, "value": value, "unit": "grid-pt-steps/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": t_step * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (acoustic monopole set-up generated in magudi_amd/cases.py)",
-        "config": {"workload": f"monopole {a.n}x{a.n} forward+adjoint, {a.nsteps} RK4 steps, "
-                               f"HBM snapshots every {a.save_interval} steps with recompute",
-                   "grid": [a.n, a.n], "nsteps": a.nsteps, "save_interval": a.save_interval,
-                   "instances": 1, "path": "fused 2-D stage kernels",
-                   "tile_shape": ["32x16", "16x32", "16x16", "24x16", "16x16/2", "16x12/2"][R.tile_shape]
-                   if R.tile_shape >= 0 else "generic"},
+        "data": "synthetic (Kolmogorov flow set-up and seeded solenoidal field generated in "
+                "magudi_amd/cases.py)",
+        "config": {"workload": f"Kolmogorov {n}^3 viscous periodic forward + adjoint, "
+                               f"{a.c4_nsteps} RK4 steps, HBM snapshots every "
+                               f"{max(1, a.c4_nsteps // 8)} steps with recompute, terminal "
+                               f"kinetic energy and dJ/dq(T) on the device",
+                   "grid": [n, n, n], "nsteps": a.c4_nsteps,
+                   "save_interval": max(1, a.c4_nsteps // 8), "instances": 1,
+                   "path": "3-D kernels: stress pass + z-march forward; adjoint per-operator"},
         "roofline": roofline,
-        # whole job against the compulsory-traffic model of one fwd+adjoint pt-step with
-        # window recompute, per state component: forward RK4 128 B, adjoint RK4 128 B,
-        # reading the 4 forward stage states 32 B, recompute 128 B, storing its stages 32 B
-        "job_roofline": {"bytes_per_pt_step": 448 * 4,
-                         "achieved_GBs": round(value * 448 * 4 / 1e9, 1),
-                         "frac": round(value * 448 * 4 / 1e9 / HBM_PEAK_GBS, 4)},
-        "kernels": other,
-        "kernel_time_share": round(kernel_share, 3),
-        "J": float(J[0]), "Grad0": float(sens[0]),
+        "J": J, "forward_s": round(t_fwd, 3), "adjoint_s": round(t_step - t_fwd, 3),
+        "rhs3d": {"forward_ms": round(f_ms, 4), "adjoint_ms": round(a_ms, 4),
+                  "forward_frac_80B": round(f_b / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "adjoint_frac_120B": round(a_b / (a_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "state_copy": {"avg_launch_us": round(cms * 1e3, 2),
+                       "achieved_GBs": round(cb / (cms * 1e-3) / 1e9, 1)},
     }
+    if traffic is not None:
+        out["roofline"]["traffic_over_algorithmic"] = round(traffic / b, 3)
     if not a.no_extras:
-        # batched: a.batch instances of the same grid in every launch
-        Rb = Region(case, n_instances=a.batch)
-        for i in range(a.batch):
-            Rb.states[i].initialCondition = q0
-        Sb = Solver(Rb)
-        run_hot_path(Rb, Sb, 1)
-        tb, _ = run_hot_path(Rb, Sb, 1)
-        # plain per-instance path (generic per-operator kernels), fewer steps, same per-step work
-        ns_plain = max(a.save_interval, a.nsteps // 10)
-        Rp = Region(case, force_plain=True)
-        Rp.states[0].initialCondition = q0
-        Sp = Solver(Rp, nsteps=ns_plain)
-        run_hot_path(Rp, Sp, 1)
-        tp, _ = run_hot_path(Rp, Sp, 1)
-        out["batched"] = {"instances": a.batch, "value": a.batch * pts / tb,
-                          "ms_per_step": tb * 1e3,
-                          "speedup_vs_single_fused": (a.batch * pts / tb) / value}
-        out["plain"] = {"value": case.npoints * ns_plain / tp, "nsteps": ns_plain,
-                        "ms_per_step": tp * 1e3,
-                        "batched_speedup_vs_plain": (a.batch * pts / tb) / (case.npoints * ns_plain / tp)}
-        # the same run with every RK stage state resident in HBM (19.8 GB at 401^2 x 960
-        # steps) instead of snapshots + window recompute: same results, no recompute
-        del Rb, Sb
-        Rr = Region(case, tile_shape=R.tile_shape, resident_states=True)
-        Rr.states[0].initialCondition = q0
-        Sr = Solver(Rr)
-        run_hot_path(Rr, Sr, 1)
-        tr, (sens_r, _, J_r) = run_hot_path(Rr, Sr, a.steps)
-        out["resident_states"] = {"value": pts / tr, "ms_per_step": tr * 1e3,
-                                  "state_bytes": 8 * 4 * case.npoints * (4 * a.nsteps + 1),
-                                  "same_J_and_Grad0": bool(J_r[0] == J[0] and sens_r[0] == sens[0])}
-        del Rr, Sr
-        if a.c4_n > 0:
-            out["viscous3d"] = viscous3d_leg(a.c4_n)
-            out["kolmogorov_run"] = kolmogorov_run_leg(a.c4_n)
+        out["monopole401"] = monopole_legs(a, extras=True)
         out["fd_sweep"] = fd_sweep_leg(a.n, a.nsteps, a.save_interval, a.batch)
         out["uneven_batch"] = uneven_leg()
         out["small_ensemble"] = small_ensemble_leg()
         out["small_grid"] = small_grid_leg()
-        out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_steps, max(1, a.cpu_steps // 2))
+        out["cpu_baseline"] = cpu_baseline()
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

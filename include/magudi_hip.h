@@ -259,12 +259,16 @@ int mgd_region_get_tile_shape(mgd_region* r, int* shape);
  * rhs by the generic per-operator kernels, any region).  ms_per_launch: average per
  * launch (per rhs evaluation for the last two); bytes: algorithmic HBM bytes of one
  * launch (evaluation), averaged over the 4 stages.  Benchmark support; it overwrites the
- * region's RK, adjoint and rhs buffers (the conserved state is kept). */
+ * region's RK, adjoint and rhs buffers (the conserved state is kept).
+ * MGD_KERNEL_FWD_MARCH3D: the 3-D z-march forward kernel alone (periodic 3-D regions whose
+ * rhs it serves), `reps` RK4 steps of 4 out-of-place stage launches; bytes: its stage
+ * state, stress / heat-flux and RK operands, averaged over the 4 stages. */
 #define MGD_KERNEL_FWD_STAGE 0
 #define MGD_KERNEL_ADJ_STAGE 1
 #define MGD_KERNEL_COPY 2
 #define MGD_KERNEL_RHS_FORWARD 3
 #define MGD_KERNEL_RHS_ADJOINT 4
+#define MGD_KERNEL_FWD_MARCH3D 5
 int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, double* bytes);
 
 /* Region groups: regions of different sizes run in lock step with one kernel launch per

@@ -22,6 +22,7 @@ OP_FIRST, OP_ADJOINT_FIRST, OP_DISS, OP_DISS_T = 0, 1, 2, 3
 OPT_FORCE_PLAIN, OPT_TILE_SHAPE, OPT_RESIDENT_STATES = 1, 2, 3
 TERMINAL_KINETIC_ENERGY = 0
 KERNEL_FWD_STAGE, KERNEL_ADJ_STAGE, KERNEL_COPY, KERNEL_RHS_FORWARD, KERNEL_RHS_ADJOINT = 0, 1, 2, 3, 4
+KERNEL_FWD_MARCH3D = 5
 
 
 class RegionDesc(C.Structure):

@@ -151,13 +151,18 @@ __device__ __forceinline__ void inviscid_flux(const double* q, const Prim<ND>& s
   f[ND + 1] = ud * (q[ND + 1] + s.p);
 }
 
+// Transport variables without a viscosity power law (constant in the temperature).
+__device__ __forceinline__ void transport_const(const Geo& g, double* mu, double* la, double* ka) {
+  *mu = g.re_inv;
+  *la = (g.bulk_ratio - 2.0 / 3.0) * g.re_inv;
+  *ka = g.re_inv * g.pr_inv;
+}
+
 // Transport variables (computeTransportVariables, :89-177).
 __device__ __forceinline__ void transport(const Geo& g, double T, double* mu, double* la,
                                           double* ka) {
   if (g.power_law <= 0.0) {
-    *mu = g.re_inv;
-    *la = (g.bulk_ratio - 2.0 / 3.0) * g.re_inv;
-    *ka = g.re_inv * g.pr_inv;
+    transport_const(g, mu, la, ka);
   } else {
     *mu = pow((g.gamma - 1.0) * T, g.power_law) * g.re_inv;
     *la = (g.bulk_ratio - 2.0 / 3.0) * *mu;

@@ -48,6 +48,14 @@ bool launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
                         double* rhs, double* w, double* G, double* t, double* T1, double* S,
                         const double* aforce, long aforce_stride, double forcing_factor,
                         const FinishInfo& fin, hipStream_t st, const RkStage* rk = nullptr);
+// The 3-D z-march forward path (periodic in every direction, at most one whole-domain
+// Kolmogorov forcing, no dissipation or sources): whether it serves a region; the stress /
+// heat-flux kernel (9 slots of the 12-slot gradient buffer S); the march itself, which
+// writes rhs, or with rk (rk->yout set, not q) applies the RK4 stage out of place.
+bool fwd3d_march_path(const Geo& g, const FinishInfo& fin, int nsrc);
+void launch_stress3d(const Geo& g, const double* q, double* S, hipStream_t st);
+void launch_fwd3d_march(const Geo& g, const double* q, const double* S, const double* kstr,
+                        double* rhs, const RkStage* rk, hipStream_t st);
 void launch_rk_forward(long M, int stage, double dt, double* q, double* b1, double* b2,
                        const double* rhs, double* qcopy, hipStream_t st);
 void launch_rk_adjoint(long M, int stage, double dt, double* lam, double* b1, double* b2,

@@ -69,8 +69,10 @@ __global__ void __launch_bounds__(kBlock) k_grad(Geo g, int d, const double* __r
 // its six distinct components (xx, xy, xz, yy, yz, zz) and the heat flux -kappa grad T
 // (src/StateImpl.f90:513-523) in slots 6..8, from the same gradient values in registers
 // (visc_point's operations, so the same values); 72 instead of 96 bytes per point.
-template <int BX, int BY, int ZC, bool STRESS = false>
-__global__ void __launch_bounds__(BX * BY, 4) k_grad3d_periodic(Geo g, const double* __restrict__ q,
+// POWER: the viscosity follows a power law of the temperature (pow per point); without
+// it the transport coefficients are constants and the kernel needs far fewer registers.
+template <int BX, int BY, int ZC, bool STRESS = false, bool POWER = true>
+__global__ void __launch_bounds__(BX * BY, STRESS ? 2 : 4) k_grad3d_periodic(Geo g, const double* __restrict__ q,
                                                               double* __restrict__ G) {
   constexpr int NT = BX * BY, EXW = BX + 8, EYW = BY + 8, NW = 4, NH = 8 * (BX + BY);
   static_assert(NH <= NT, "one halo point per thread");
@@ -160,10 +162,15 @@ __global__ void __launch_bounds__(BX * BY, 4) k_grad3d_periodic(Geo g, const dou
       gv[c * 3 + 0] = (jac * metric<3>(g, 0, ijk)) * ox;
       gv[c * 3 + 1] = (jac * metric<3>(g, 1, ijk)) * oy;
       gv[c * 3 + 2] = (jac * metric<3>(g, 2, ijk)) * oz;
+      if constexpr (!STRESS) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) Gi[(c * 3 + d) * g.N + p] = gv[c * 3 + d];
+      }
     }
-    if (STRESS) {
+    if constexpr (STRESS) {
       double mu, la, ka;
-      transport(g, Q[4][3], &mu, &la, &ka);
+      if constexpr (POWER) transport(g, Q[4][3], &mu, &la, &ka);
+      else transport_const(g, &mu, &la, &ka);
       stress<3>(gv, mu, la);
       Gi[0 * g.N + p] = gv[0];
       Gi[1 * g.N + p] = gv[1];
@@ -173,9 +180,6 @@ __global__ void __launch_bounds__(BX * BY, 4) k_grad3d_periodic(Geo g, const dou
       Gi[5 * g.N + p] = gv[8];
 #pragma unroll
       for (int d = 0; d < 3; ++d) Gi[(6 + d) * g.N + p] = -ka * gv[9 + d];
-    } else {
-#pragma unroll
-      for (int k = 0; k < NW * 3; ++k) Gi[k * g.N + p] = gv[k];
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k)
@@ -1363,33 +1367,39 @@ __global__ void __launch_bounds__(BX * BY, 2) k_fwd3d_march(Geo g, const double*
     hj = wrap(y0 + hy - 4, g.dir[1]);
     hmask = 2;
   }
-  // contravariant fluxes of the directions in dmask at grid point (ii, jj, kk)
-  auto fluxes = [&](int ii, int jj, int kk, int dmask, double (*out)[NV]) {
-    const long pp = ii + (long)g.n[0] * jj + n01 * kk;
+  // operands of one point: q and, viscous, the stress / heat-flux slots the fluxes of the
+  // directions in dmask read (tau is symmetric: slot order xx xy xz yy yz zz, then the
+  // heat flux x y z); slots not needed are not loaded
+  struct Pt {
     double v[NV];
-    load_q<3>(qi, g.N, pp, v);
-    const Prim<3> s = prim<3>(v, g.gamma);
+    double sv[9];
+  };
+  auto load_pt = [&](int ii, int jj, int kk, int dmask, Pt& P) {
+    const long pp = ii + (long)g.n[0] * jj + n01 * kk;
+    load_q<3>(qi, g.N, pp, P.v);
+    if (g.viscous) {
+      const int need[9] = {dmask & 1, dmask & 3, dmask & 5, dmask & 2, dmask & 6, dmask & 4,
+                           dmask & 1, dmask & 2, dmask & 4};
+#pragma unroll
+      for (int k = 0; k < 9; ++k) P.sv[k] = need[k] ? Si[k * g.N + pp] : 0.0;
+    }
+  };
+  // contravariant fluxes of the directions in dmask at grid point (ii, jj, kk)
+  auto fluxes = [&](int ii, int jj, int kk, int dmask, const Pt& P, double (*out)[NV]) {
+    const Prim<3> s = prim<3>(P.v, g.gamma);
     double tau[9], hf[3];
     if (g.viscous) {
-      // the stress tensor is symmetric: tau(i, d) = S of the pair
-      const double sxx = (dmask & 1) ? Si[0 * g.N + pp] : 0.0;
-      const double sxy = (dmask & 3) ? Si[1 * g.N + pp] : 0.0;
-      const double sxz = (dmask & 5) ? Si[2 * g.N + pp] : 0.0;
-      const double syy = (dmask & 2) ? Si[3 * g.N + pp] : 0.0;
-      const double syz = (dmask & 6) ? Si[4 * g.N + pp] : 0.0;
-      const double szz = (dmask & 4) ? Si[5 * g.N + pp] : 0.0;
-      tau[0] = sxx; tau[1] = sxy; tau[2] = sxz;
-      tau[3] = sxy; tau[4] = syy; tau[5] = syz;
-      tau[6] = sxz; tau[7] = syz; tau[8] = szz;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) hf[d] = (dmask >> d & 1) ? Si[(6 + d) * g.N + pp] : 0.0;
+      tau[0] = P.sv[0]; tau[1] = P.sv[1]; tau[2] = P.sv[2];
+      tau[3] = P.sv[1]; tau[4] = P.sv[3]; tau[5] = P.sv[4];
+      tau[6] = P.sv[2]; tau[7] = P.sv[4]; tau[8] = P.sv[5];
+      hf[0] = P.sv[6]; hf[1] = P.sv[7]; hf[2] = P.sv[8];
     }
     const int ijk[3] = {ii, jj, kk};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       if (!(dmask >> d & 1)) continue;
       double f[NV];
-      inviscid_flux<3>(v, s, d, f);
+      inviscid_flux<3>(P.v, s, d, f);
       if (g.viscous) {
         double fv[NV];
         viscous_flux<3>(s.u, tau, hf, d, fv);
@@ -1409,13 +1419,25 @@ __global__ void __launch_bounds__(BX * BY, 2) k_fwd3d_march(Geo g, const double*
   for (int k = 0; k < 8; ++k)
 #pragma unroll
     for (int l = 0; l < NV; ++l) W[k][l] = 0.0;
+  // the operands of the next plane are loaded once this plane's fluxes are formed, so
+  // their latency overlaps the divergence and the stage update
+  Pt ct, ch, nt, nh;
+  load_pt(i, j, wrap(z0 - 4, dz), z0 - 4 >= z0 ? 7 : 4, ct);
   for (int zl = z0 - 4; zl < z1 + 4; ++zl) {      // uniform over the block
     const int kz = wrap(zl, dz);
     const bool xy = zl >= z0 && zl < z1;
-    double f[3][NV];
-    fluxes(i, j, kz, xy ? 7 : 4, f);
+    double f[3][NV], fh[3][NV];
+    fluxes(i, j, kz, xy ? 7 : 4, ct, f);
 #pragma unroll
     for (int l = 0; l < NV; ++l) W[8][l] = f[2][l];
+    if (xy) fluxes(hi, hj, kz, hmask, ch, fh);
+    // this plane's operands are spent: load the next plane's now
+    const int zn = zl + 1;
+    const bool nxy = zn >= z0 && zn < z1;
+    if (zn < z1 + 4) {
+      load_pt(i, j, wrap(zn, dz), nxy ? 7 : 4, nt);
+      if (nxy) load_pt(hi, hj, wrap(zn, dz), hmask, nh);
+    }
     if (zl >= z0 + 4) {
       const int zo = zl - 4;
       const long p = i + (long)g.n[0] * j + n01 * zo;
@@ -1470,8 +1492,6 @@ __global__ void __launch_bounds__(BX * BY, 2) k_fwd3d_march(Geo g, const double*
       }
     }
     if (xy) {
-      double fh[3][NV];
-      fluxes(hi, hj, kz, hmask, fh);
       __syncthreads();          // readers of the previous plane's fluxes are done
 #pragma unroll
       for (int l = 0; l < NV; ++l) {
@@ -1506,6 +1526,8 @@ __global__ void __launch_bounds__(BX * BY, 2) k_fwd3d_march(Geo g, const double*
     for (int k = 0; k < 8; ++k)
 #pragma unroll
       for (int l = 0; l < NV; ++l) W[k][l] = W[k + 1][l];
+    ct = nt;
+    if (nxy) ch = nh;
   }
 }
 
@@ -1744,6 +1766,26 @@ static bool fwd3d_march_applies(const Geo& g) {
          g.n[0] % kFMX == 0 && g.n[1] % kFMY == 0 && (!g.viscous || grad3d_applies(g));
 }
 
+bool fwd3d_march_path(const Geo& g, const FinishInfo& fin, int nsrc) {
+  return fin.simple && !g.dissipation && nsrc == 0 && fwd3d_march_applies(g);
+}
+
+void launch_stress3d(const Geo& g, const double* q, double* S, hipStream_t st) {
+  const dim3 gg((g.n[0] / kG3X) * (g.n[1] / kG3Y), (g.n[2] + kG3Z - 1) / kG3Z, g.ninst);
+  if (g.power_law > 0.0)
+    k_grad3d_periodic<kG3X, kG3Y, kG3Z, true, true><<<gg, kG3X * kG3Y, 0, st>>>(g, q, S);
+  else
+    k_grad3d_periodic<kG3X, kG3Y, kG3Z, true, false><<<gg, kG3X * kG3Y, 0, st>>>(g, q, S);
+}
+
+void launch_fwd3d_march(const Geo& g, const double* q, const double* S, const double* kstr,
+                        double* rhs, const RkStage* rk, hipStream_t st) {
+  const dim3 gm((g.n[0] / kFMX) * (g.n[1] / kFMY), (g.n[2] + kFMZ - 1) / kFMZ, g.ninst);
+  k_fwd3d_march<kFMX, kFMY, kFMZ><<<gm, kFMX * kFMY, 0, st>>>(
+      g, q, S, kstr, rhs, rk ? rk->stage : 0, rk ? rk->dt : 0.0, rk ? rk->yout : nullptr,
+      rk ? rk->b1 : nullptr, rk ? rk->b2 : nullptr, rk ? rk->copy : nullptr);
+}
+
 int launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,
                        double* G, double* t, double* F, const double* control, long control_stride,
                        const Source* src, int nsrc, const double* src_amp, long src_inst_stride,
@@ -1752,16 +1794,9 @@ int launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* 
   const int has_farfield = fin.has_farfield;
   if (F && fin.simple && !g.dissipation && nsrc == 0 && fwd3d_march_applies(g)) {
     // the whole rhs in one z march; with an RK stage only out of place (rk->yout)
-    if (g.viscous) {
-      const dim3 gg((g.n[0] / kG3X) * (g.n[1] / kG3Y), (g.n[2] + kG3Z - 1) / kG3Z, g.ninst);
-      k_grad3d_periodic<kG3X, kG3Y, kG3Z, true><<<gg, kG3X * kG3Y, 0, st>>>(g, q, G);
-    }
+    if (g.viscous) launch_stress3d(g, q, G, st);
     const bool rk_out = rk && rk->yout;
-    const dim3 gm((g.n[0] / kFMX) * (g.n[1] / kFMY), (g.n[2] + kFMZ - 1) / kFMZ, g.ninst);
-    k_fwd3d_march<kFMX, kFMY, kFMZ><<<gm, kFMX * kFMY, 0, st>>>(
-        g, q, G, fin.kolmogorov, rhs, rk_out ? rk->stage : 0, rk_out ? rk->dt : 0.0,
-        rk_out ? rk->yout : nullptr, rk_out ? rk->b1 : nullptr, rk_out ? rk->b2 : nullptr,
-        rk_out ? rk->copy : nullptr);
+    launch_fwd3d_march(g, q, G, fin.kolmogorov, rhs, rk_out ? rk : nullptr, st);
     return rk_out ? RK_TO_YOUT : RK_NOT_APPLIED;
   }
   bool finished = false;

@@ -2092,6 +2092,15 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
         launch_rhs_adjoint(r->geo, dpatches(r), r->q.p, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
                            r->T1.p, r->S.p, nullptr, 0, 1.0, r->fin, r->stream);
         break;
+      case MGD_KERNEL_FWD_MARCH3D:
+        // one RK4 step of march launches, ping-pong between qs and qy from the stress
+        // slots formed once (the timing does not depend on their values)
+        for (int i = 1; i <= 4; ++i) {
+          RkStage rk{i, dt, (i % 2) ? r->qs.p : r->qy.p, r->b1.p, r->b2.p, nullptr,
+                     (i % 2) ? r->qy.p : r->qs.p};
+          launch_fwd3d_march(r->geo, rk.y, r->G.p, r->fin.kolmogorov, r->rhs.p, &rk, r->stream);
+        }
+        break;
     }
   };
   if (which == MGD_KERNEL_FWD_STAGE || which == MGD_KERNEL_ADJ_STAGE) {
@@ -2104,6 +2113,16 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
     launch_copy(M, r->q.p, r->ly.p, r->stream);
     launch_copy(M, r->q.p, r->l0.p, r->stream);
     launch_copy(M, r->q.p, r->lam.p, r->stream);
+  } else if (which == MGD_KERNEL_FWD_MARCH3D) {
+    if (!fwd3d_march_path(r->geo, r->fin, (int)r->sources.size())) {
+      set_error("the 3-D z-march forward kernel does not serve this region");
+      return MGD_ERR_UNSUPPORTED;
+    }
+    per_rep = 4;
+    launch_copy(M, r->q.p, r->qs.p, r->stream);
+    launch_copy(M, r->q.p, r->b1.p, r->stream);
+    launch_copy(M, r->q.p, r->b2.p, r->stream);
+    if (r->geo.viscous) launch_stress3d(r->geo, r->q.p, r->G.p, r->stream);
   } else if (which == MGD_KERNEL_RHS_FORWARD || which == MGD_KERNEL_RHS_ADJOINT) {
     if (which == MGD_KERNEL_RHS_ADJOINT) {
       if (int rc = ensure_adjoint(r)) return rc;
@@ -2131,6 +2150,10 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
       case MGD_KERNEL_ADJ_STAGE: *bytes = 20.0 / 4.0 * w * pts; break;
       // one rhs evaluation: state in, rhs out (forward); state and adjoint in (adjoint)
       case MGD_KERNEL_RHS_ADJOINT: *bytes = 3.0 * w * pts; break;
+      // the march kernel's own operands per stage, averaged over the 4 stages: the stage
+      // state and the 9 stress / heat-flux slots in, RK buffers in (0, 2, 2, 1 states),
+      // RK buffers and the next stage state out (3, 2, 2, 1 states)
+      case MGD_KERNEL_FWD_MARCH3D: *bytes = (w + 72.0 + 5.0 / 4.0 * w + 8.0 / 4.0 * w) * pts; break;
       default: *bytes = 2.0 * w * pts; break;
     }
   }
