@@ -132,7 +132,7 @@ def run_hot_path(R, S, steps):
     return (time.perf_counter() - t0) / steps, out
 
 
-def cpu_baseline(n=48, nsteps=8, save_interval=4):
+def cpu_baseline(n=64, nsteps=8, save_interval=4):
     """The CPU oracle (plain C; its per-point and per-pencil loops on every OpenMP thread
     the host gives, OMP_NUM_THREADS) on a bounded sample of the headline workload: the
     same Kolmogorov flow on an n^3 grid for nsteps RK4 steps, forward with snapshots,

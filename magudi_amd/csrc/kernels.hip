@@ -77,9 +77,17 @@ __global__ void __launch_bounds__(BX * BY, STRESS ? 2 : 4) k_grad3d_periodic(Geo
   constexpr int NT = BX * BY, EXW = BX + 8, EYW = BY + 8, NW = 4, NH = 8 * (BX + BY);
   static_assert(NH <= NT, "one halo point per thread");
   __shared__ double sw[NW][EYW][EXW];
-  const int ntx = g.n[0] / BX;
-  const int x0 = (blockIdx.x % ntx) * BX, y0 = (blockIdx.x / ntx) * BY;
-  const int z0 = blockIdx.y * ZC;
+  // XCD-aware block map (as in k_fwd3d_march): block b takes (tile, z chunk) unit
+  // (b mod 8) * ceil(units / 8) + b / 8, tiles x-fastest, so each XCD works on one
+  // contiguous band and neighbouring tiles' halo lines meet in its L2
+  const int ntx = g.n[0] / BX, ntiles = ntx * (g.n[1] / BY);
+  const int nunits = ntiles * ((g.n[2] + ZC - 1) / ZC);
+  const int per_xcd = (nunits + 7) / 8;
+  const int unit = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (unit >= nunits) return;   // padding of the grid to a multiple of 8 (whole block)
+  const int tile = unit % ntiles;
+  const int x0 = (tile % ntx) * BX, y0 = (tile / ntx) * BY;
+  const int z0 = (unit / ntiles) * ZC;
   const long inst = blockIdx.z;
   const double* qi = q + inst * 5 * g.N;
   double* Gi = G + inst * 4 * 3 * g.N;
@@ -1340,9 +1348,18 @@ __global__ void __launch_bounds__(BX * BY, 2) k_fwd3d_march(Geo g, const double*
   __shared__ double sfx[NV][BY][EXW];
   __shared__ double sfy[NV][EYW][BX];
   __shared__ double ring[4][NV][NT];
-  const int ntx = g.n[0] / BX;
-  const int x0 = (blockIdx.x % ntx) * BX, y0 = (blockIdx.x / ntx) * BY;
-  const int z0 = blockIdx.y * ZC;
+  // XCD-aware block map: work-groups are dealt to the 8 XCDs round-robin by block index,
+  // so block b takes work unit (b mod 8) * ceil(units / 8) + b / 8 of the (tile, z chunk)
+  // units in x-fastest tile order: each XCD marches one contiguous band, and the halo
+  // lines of neighbouring tiles are fetched into its own L2 by blocks running beside it
+  const int ntx = g.n[0] / BX, ntiles = ntx * (g.n[1] / BY);
+  const int nunits = ntiles * ((g.n[2] + ZC - 1) / ZC);
+  const int per_xcd = (nunits + 7) / 8;
+  const int unit = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (unit >= nunits) return;   // padding of the grid to a multiple of 8 (whole block)
+  const int tile = unit % ntiles;
+  const int x0 = (tile % ntx) * BX, y0 = (tile / ntx) * BY;
+  const int z0 = (unit / ntiles) * ZC;
   const int z1 = min(z0 + ZC, g.n[2]);
   const long inst = blockIdx.z;
   const double* qi = q + inst * NV * g.N;
@@ -1746,12 +1763,17 @@ static bool fluxdiv3d_applies(const Geo& g) {
          g.n[0] % kFD3X == 0 && g.n[1] % kFD3Y == 0;
 }
 
+// grid of the periodic gradient kernel: (tile, z chunk) units padded to a multiple of 8
+static dim3 grad3d_grid(const Geo& g) {
+  const int units = (g.n[0] / kG3X) * (g.n[1] / kG3Y) * ((g.n[2] + kG3Z - 1) / kG3Z);
+  return dim3(8 * ((units + 7) / 8), 1, g.ninst);
+}
+
 template <int ND>
 static void launch_gradients(const Geo& g, const double* q, double* w, double* G, const dim3& grid,
                              hipStream_t st) {
   if (ND == 3 && grad3d_applies(g)) {
-    const dim3 gg((g.n[0] / kG3X) * (g.n[1] / kG3Y), (g.n[2] + kG3Z - 1) / kG3Z, g.ninst);
-    k_grad3d_periodic<kG3X, kG3Y, kG3Z><<<gg, kG3X * kG3Y, 0, st>>>(g, q, G);
+    k_grad3d_periodic<kG3X, kG3Y, kG3Z><<<grad3d_grid(g), kG3X * kG3Y, 0, st>>>(g, q, G);
     return;
   }
   k_prim<ND><<<grid, kBlock, 0, st>>>(g, q, w);
@@ -1760,7 +1782,7 @@ static void launch_gradients(const Geo& g, const double* q, double* w, double* G
 
 // The one-march forward kernel: 3-D, periodic in every direction, x / y sizes multiples
 // of its 16 x 16 tile; viscous regions also need the stress kernel (grad3d_applies).
-constexpr int kFMX = 16, kFMY = 16, kFMZ = 32;
+constexpr int kFMX = 16, kFMY = 16, kFMZ = 64;
 static bool fwd3d_march_applies(const Geo& g) {
   return g.nd == 3 && g.nv == 5 && g.dir[0].periodic && g.dir[1].periodic && g.dir[2].periodic &&
          g.n[0] % kFMX == 0 && g.n[1] % kFMY == 0 && (!g.viscous || grad3d_applies(g));
@@ -1771,7 +1793,7 @@ bool fwd3d_march_path(const Geo& g, const FinishInfo& fin, int nsrc) {
 }
 
 void launch_stress3d(const Geo& g, const double* q, double* S, hipStream_t st) {
-  const dim3 gg((g.n[0] / kG3X) * (g.n[1] / kG3Y), (g.n[2] + kG3Z - 1) / kG3Z, g.ninst);
+  const dim3 gg = grad3d_grid(g);
   if (g.power_law > 0.0)
     k_grad3d_periodic<kG3X, kG3Y, kG3Z, true, true><<<gg, kG3X * kG3Y, 0, st>>>(g, q, S);
   else
@@ -1780,7 +1802,8 @@ void launch_stress3d(const Geo& g, const double* q, double* S, hipStream_t st) {
 
 void launch_fwd3d_march(const Geo& g, const double* q, const double* S, const double* kstr,
                         double* rhs, const RkStage* rk, hipStream_t st) {
-  const dim3 gm((g.n[0] / kFMX) * (g.n[1] / kFMY), (g.n[2] + kFMZ - 1) / kFMZ, g.ninst);
+  const int units = (g.n[0] / kFMX) * (g.n[1] / kFMY) * ((g.n[2] + kFMZ - 1) / kFMZ);
+  const dim3 gm(8 * ((units + 7) / 8), 1, g.ninst);
   k_fwd3d_march<kFMX, kFMY, kFMZ><<<gm, kFMX * kFMY, 0, st>>>(
       g, q, S, kstr, rhs, rk ? rk->stage : 0, rk ? rk->dt : 0.0, rk ? rk->yout : nullptr,
       rk ? rk->b1 : nullptr, rk ? rk->b2 : nullptr, rk ? rk->copy : nullptr);
