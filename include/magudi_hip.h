@@ -219,6 +219,41 @@ int mgd_run_forward_checkpointed(mgd_region* r, double t0, double dt, int nsteps
 int mgd_run_adjoint_checkpointed(mgd_region* r, int terminal, double* gradient,
                                  double* sensitivity);
 
+/* Stage-level entry points, for a host that keeps its own Solver time loop (the reference's
+ * runForward / runAdjoint, src/SolverImpl.f90:672-1245).
+ *
+ * mgd_rk4_substep: t_TimeIntegrator%subStepForward / subStepAdjoint
+ * (include/TimeIntegrator.f90:19-21, 64-83; RK4: src/RK4IntegratorImpl.f90:65-270) on every
+ * instance: one RK4 stage of the conserved state (MGD_FORWARD, stages 1..4 in order) or of
+ * the adjoint state (MGD_ADJOINT, stages 4..1), the RK buffers held by the region.
+ * time: State%time of the stage (acoustic source phase; forward only: t, t + dt/2,
+ * t + dt/2, t + dt); control: the stage's actuator forcing per instance and actuator point
+ * (forward, or NULL).  An adjoint stage linearises about the forward stage state in
+ * MGD_CONSERVED (mgd_checkpoint_migrate puts it there), applies
+ * State%adjointForcingFactor itself (2, 1, 1/2, 1 for stages 4..1) to the cost-target
+ * forcing of the last mgd_update_adjoint_forcing.  Asynchronous on the region's stream.
+ *
+ * mgd_checkpoint_migrate: t_ReverseMigrator%migrateTo (include/ReverseMigrator.f90:21,
+ * 92-106; UniformCheckpointer, src/UniformCheckpointerImpl.f90:78-208): MGD_CONSERVED :=
+ * the forward state after stage `stage` of time step `timestep` (1-based; (0, 4) is the
+ * initial condition) of the last mgd_run_forward_checkpointed, recomputing the window of
+ * save_interval steps from its HBM snapshot when the state is not in the window held.
+ *
+ * mgd_control_gradient: ThermalActuator%updateGradient and computeSensitivity
+ * (src/ThermalActuatorImpl.f90:83-159, 383-443) at the current adjoint state: gradient per
+ * instance and actuator point (lambda_E times the control mollifier), sensitivity per
+ * instance (its H-norm square); either may be NULL.
+ *
+ * mgd_adjoint_initial_condition: loadInitialCondition(ADJOINT) (src/SolverImpl.f90:368-428)
+ * after mgd_run_forward_checkpointed: the adjoint state := 0 (keep = 0) or kept (keep = 1,
+ * adjoint_nonzero_initial_condition), plus the cost-target forcing of the final state
+ * with factor -dt/6. */
+int mgd_rk4_substep(mgd_region* r, int mode, int stage, double time, double dt,
+                    const double* control);
+int mgd_checkpoint_migrate(mgd_region* r, int timestep, int stage);
+int mgd_control_gradient(mgd_region* r, double* gradient, double* sensitivity);
+int mgd_adjoint_initial_condition(mgd_region* r, int keep);
+
 /* t_StencilOperator%apply (include/StencilOperator.f90:19-27,
  * src/StencilOperatorImpl.f90:2374-2410) of one of the region's operators along
  * direction dir (0-based) to ncomp host fields of size N, in place. */

@@ -97,6 +97,10 @@ def lib():
         "mgd_run_forward_checkpointed": [vp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp],
         "mgd_run_adjoint_checkpointed": [vp, C.c_int, dp, dp],
         "mgd_terminal_cost": [vp, C.c_int, dp],
+        "mgd_rk4_substep": [vp, C.c_int, C.c_int, C.c_double, C.c_double, dp],
+        "mgd_checkpoint_migrate": [vp, C.c_int, C.c_int],
+        "mgd_control_gradient": [vp, dp, dp],
+        "mgd_adjoint_initial_condition": [vp, C.c_int],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

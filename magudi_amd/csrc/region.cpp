@@ -123,6 +123,8 @@ struct mgd_region {
   } ck;
   mgd::DevBuf qT, scr;                   // final forward state; scratch state (MGD_SCRATCH)
   mgd::DevBuf q_alt, qs_alt;             // second state buffers of out-of-place RK stages
+  mgd::DevBuf lam_alt;                   // second adjoint buffer of out-of-place adjoint stages
+  int mig_loaded = -1;                   // snapshot step whose window mgd_checkpoint_migrate holds
   mgd::DevBuf ipart, iout;               // inner-product partial sums and results
   GraphSlot graphs[2];                   // captured runs: forward, forward + adjoint
   std::vector<double> toff;              // per-instance time offsets (empty: none)
@@ -933,6 +935,7 @@ int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, c
   const long M = (long)r->ninst * M_of(r);
   r->ck.valid = false;
   r->last_window = false;
+  r->mig_loaded = -1;
   r->ck.t0 = t0;
   r->ck.dt = dt;
   r->ck.nsteps = nsteps;
@@ -1984,6 +1987,167 @@ int mgd_run_adjoint_checkpointed(mgd_region* r, int terminal, double* gradient, 
   if (int rc = enqueue_adjoint_ck(r, terminal == 1, gradient != nullptr)) return rc;
   if (int rc = finish_timed(r)) return rc;
   return copy_adjoint_outputs(r, gradient, sensitivity);
+}
+
+// ---------------------------------------------------------------- stage-level entry points
+
+int mgd_rk4_substep(mgd_region* r, int mode, int stage, double time, double dt, const double* control) {
+  if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
+  if (stage < 1 || stage > 4 || !(dt > 0.0)) { set_error("stage must be 1..4 and dt > 0"); return MGD_ERR_ARG; }
+  if (mode != MGD_FORWARD && mode != MGD_ADJOINT) {
+    set_error("mode must be MGD_FORWARD or MGD_ADJOINT");
+    return MGD_ERR_ARG;
+  }
+  if (int rc = ensure_adjoint(r)) return rc;
+  if (int rc = maybe_tune(r)) return rc;
+  const long M = (long)r->ninst * M_of(r);
+  // the previous stage may still read the per-stage inputs about to be replaced
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  if (mode == MGD_FORWARD) {
+    // this stage's inputs: the source amplitudes at `time`, the actuator forcing
+    std::vector<double> amp;
+    source_table(r, std::vector<double>{time}, amp);
+    if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
+    const int na = n_act(r);
+    if (control && na > 0) {
+      std::vector<double> c(control, control + (size_t)r->ninst * na);
+      if (upload(r->control, c)) return MGD_ERR_HIP;
+      r->ctrl_stride = na;
+    } else if (r->control.p) {
+      r->control.alloc(0);
+    }
+    if (fused_ok(r)) {
+      // the stage kernel writes out of place: the level (buffer1) is kept in b1
+      if (r->q_alt.alloc(M)) return MGD_ERR_HIP;
+      if (stage == 1) launch_copy(M, r->q.p, r->b1.p, r->stream);
+      FwdStage a = fwd_args(r, r->q.p, r->b1.p, r->b2.p, r->q_alt.p, stage, dt, 0, 1, false);
+      launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+      std::swap(r->q.p, r->q_alt.p);
+      std::swap(r->q.n, r->q_alt.n);
+    } else {
+      forward_stage(r, r->q, r->q_alt, r->b1.p, r->b2.p, 0, stage, dt, nullptr);
+    }
+  } else {
+    const double factor = stage == 4 ? 2.0 : (stage == 2 ? 0.5 : 1.0);
+    if (fused_ok(r)) {
+      if (r->lam_alt.alloc(M)) return MGD_ERR_HIP;
+      if (stage == 4) launch_copy(M, r->lam.p, r->b1a.p, r->stream);
+      AdjStage a;
+      std::memset(&a, 0, sizeof a);
+      a.lam = r->lam.p; a.l0 = r->b1a.p; a.q = r->q.p; a.b2 = r->b2a.p; a.lout = r->lam_alt.p;
+      a.stage = stage; a.dt = dt; a.forcing_factor = factor; a.zero_forcing = r->aforce_zero;
+      a.ip_act = r->ip_act;
+      launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+      std::swap(r->lam.p, r->lam_alt.p);
+      std::swap(r->lam.n, r->lam_alt.n);
+    } else {
+      const int nc = n_cost(r);
+      const RkStage rk{stage, dt, r->lam.p, r->b1a.p, r->b2a.p};
+      if (!launch_rhs_adjoint(r->geo, dpatches(r), r->q.p, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
+                              r->T1.p, r->S.p, nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc,
+                              factor, r->fin, r->stream, &rk))
+        launch_rk_adjoint(M, stage, dt, r->lam.p, r->b1a.p, r->b2a.p, r->rhs.p, r->stream);
+    }
+  }
+  MGD_HIP(hipGetLastError());
+  return MGD_OK;
+}
+
+int mgd_checkpoint_migrate(mgd_region* r, int timestep, int stage) {
+  if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
+  if (!r->ck.valid) { set_error("no checkpointed forward run to migrate in"); return MGD_ERR_STATE; }
+  const int nsteps = r->ck.nsteps, K = r->ck.K, nst = 4 * nsteps;
+  const double dt = r->ck.dt;
+  if (timestep < 0 || timestep > nsteps || stage < 1 || stage > 4 || (timestep == 0 && stage != 4)) {
+    set_error("(timestep, stage) outside the checkpointed run");
+    return MGD_ERR_ARG;
+  }
+  const long M = (long)r->ninst * M_of(r);
+  const int ts = timestep, st = stage;
+  int loaded = r->mig_loaded;
+  const bool need = loaded == -1 || ts < loaded || ts > loaded + K || (ts == loaded && st < 4) ||
+                    (ts == loaded + K && st == 4);
+  if (need) {
+    int lt;
+    if (ts % K == 0 && st == 4) lt = ts;
+    else if (ts % K == 0) lt = ts - K;
+    else lt = ts - ts % K;
+    loaded = lt;
+    // the recompute needs the run's actuator forcing and source amplitudes
+    MGD_HIP(hipStreamSynchronize(r->stream));
+    if (int rc = upload_run_inputs(r)) return rc;
+    double* W = r->window.p;
+    launch_copy(M, r->snaps.p + (long)(lt / K) * M, W, r->stream);
+    const int t1 = std::min(nsteps, lt + K);
+    if (fused_ok(r)) {
+      if (r->b2r.alloc(M) || r->qsr.alloc(M)) return MGD_ERR_HIP;
+      for (int t = lt; t < t1; ++t) {
+        const long base = 4L * (t - lt);
+        double* outs[4];
+        for (int k = 0; k < 4; ++k) outs[k] = W + (base + 1 + k) * M;
+        if (t == lt + K - 1) outs[3] = r->qsr.p;   // the state at the next snapshot: not needed
+        fused_forward_step(r, W + base * M, nullptr, nullptr, r->b2r.p, t, dt, nst, false, outs);
+      }
+    } else {
+      launch_copy(M, W, r->qs.p, r->stream);
+      int idx = 1;
+      for (int s2 = lt + 1; s2 <= t1; ++s2)
+        for (int st2 = 1; st2 <= 4; ++st2) {
+          const bool last = s2 == lt + K && st2 == 4;
+          forward_stage(r, r->qs, r->qs_alt, r->b1.p, r->b2.p, 4 * (s2 - 1) + st2 - 1, st2, dt,
+                        last ? nullptr : W + (long)idx * M);
+          if (last) break;
+          ++idx;
+        }
+    }
+    r->mig_loaded = loaded;
+  }
+  const int widx = (ts - 1 - loaded) * 4 + st;
+  launch_copy(M, r->window.p + (long)widx * M, r->q.p, r->stream);
+  MGD_HIP(hipGetLastError());
+  return MGD_OK;
+}
+
+int mgd_control_gradient(mgd_region* r, double* gradient, double* sensitivity) {
+  if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
+  const int na = n_act(r);
+  if (na == 0) {
+    if (sensitivity)
+      for (int i = 0; i < r->ninst; ++i) sensitivity[i] = 0.0;
+    return MGD_OK;
+  }
+  if (r->iout.alloc(r->ninst)) return MGD_ERR_HIP;
+  if (gradient) {
+    if (r->grad.n < (size_t)r->ninst * na && r->grad.alloc((size_t)r->ninst * na)) return MGD_ERR_HIP;
+    launch_gradient(r->geo, dpatch(r, r->ip_act), na, r->lam.p, r->grad.p, (long)na, r->stream);
+    MGD_HIP(hipMemcpyAsync(gradient, r->grad.p, sizeof(double) * r->ninst * na,
+                           hipMemcpyDeviceToHost, r->stream));
+  }
+  if (sensitivity) {
+    launch_patch_reduce(r->geo, dpatch(r, r->ip_act), na, 1, r->lam.p, r->partial.p, 1.0, nullptr,
+                        r->iout.p, r->stream);
+    MGD_HIP(hipMemcpyAsync(sensitivity, r->iout.p, sizeof(double) * r->ninst,
+                           hipMemcpyDeviceToHost, r->stream));
+  }
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  return MGD_OK;
+}
+
+int mgd_adjoint_initial_condition(mgd_region* r, int keep) {
+  if (!r || !r->finalized) { set_error("region not finalized"); return MGD_ERR_STATE; }
+  if (!r->ck.valid) { set_error("no checkpointed forward run to start the adjoint from"); return MGD_ERR_STATE; }
+  if (int rc = ensure_adjoint(r)) return rc;
+  const long M = (long)r->ninst * M_of(r);
+  const int nc = n_cost(r);
+  if (!keep) launch_fill(M, 0.0, r->lam.p, r->stream);
+  if (nc > 0) {
+    launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->qT.p, r->aforce.p, 0, r->stream);
+    launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -r->ck.dt / 6.0, r->lam.p,
+                        r->stream);
+  }
+  MGD_HIP(hipGetLastError());
+  return MGD_OK;
 }
 
 int mgd_stencil_apply(mgd_region* r, int op, int dir, double* host, int ncomp) {

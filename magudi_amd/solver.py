@@ -23,7 +23,8 @@ from . import capi
 from .capi import FORWARD, ADJOINT, check, dptr
 from .cases import Case
 
-__all__ = ["Region", "State", "Solver", "FORWARD", "ADJOINT"]
+__all__ = ["Region", "State", "Solver", "RK4Integrator", "UniformCheckpointer", "FORWARD",
+           "ADJOINT"]
 
 
 class State:
@@ -179,6 +180,20 @@ class Region:
         check(capi.lib().mgd_terminal_cost(self.handle, capi.TERMINAL_KINETIC_ENERGY, dptr(out)))
         return out
 
+    def controlGradient(self):
+        """(gradient, sensitivity) at the current adjoint state: per instance and actuator
+        point lambda_E times the control mollifier (ThermalActuator%updateGradient), and per
+        instance its H-norm square (computeSensitivity)."""
+        grad = np.zeros((self.n_instances, max(self.n_actuator, 1)))
+        sens = np.zeros(self.n_instances)
+        check(capi.lib().mgd_control_gradient(self.handle, dptr(grad), dptr(sens)))
+        return grad[:, :self.n_actuator], sens
+
+    def adjointInitialCondition(self, keep: bool = False):
+        """loadInitialCondition(ADJOINT) after a checkpointed forward run: zero (or the
+        current adjoint state, keep=True) plus the cost-target forcing of the final state."""
+        check(capi.lib().mgd_adjoint_initial_condition(self.handle, int(keep)))
+
     def grid(self, which: int) -> np.ndarray:
         size = {0: self.nd * self.nd, 1: 1, 2: 1, 3: self.nd}[which]
         out = np.zeros((size, self.npoints))
@@ -210,6 +225,38 @@ class Region:
         n = (C.c_int * 3)()
         check(capi.lib().mgd_last_timings(self.handle, dptr(out), n))
         return out, list(n)
+
+
+class RK4Integrator:
+    """t_RK4Integrator (include/RK4Integrator.f90, src/RK4IntegratorImpl.f90:65-270): one RK4
+    stage at a time on the region's HBM-resident state; the RK buffers live in the region.
+    ``norm`` holds the quadrature weights b_i of the stages."""
+
+    nStages = 4
+    norm = (1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0)
+
+    def substepForward(self, region: "Region", time: float, timeStepSize: float, timestep: int,
+                       stage: int, control: Optional[np.ndarray] = None):
+        """Stage ``stage`` (1..4) of the forward step; ``time`` is State%time of the stage
+        (t, t + dt/2, t + dt/2, t + dt), ``control`` the stage's actuator forcing per
+        instance and actuator point."""
+        ctl = None if control is None else np.ascontiguousarray(control, dtype=np.float64)
+        check(capi.lib().mgd_rk4_substep(region.handle, FORWARD, stage, time, timeStepSize,
+                                         dptr(ctl)))
+
+    def substepAdjoint(self, region: "Region", time: float, timeStepSize: float, timestep: int,
+                       stage: int):
+        """Stage ``stage`` (4..1) of the reversed step, about the forward stage state in the
+        conserved variables (UniformCheckpointer.migrateTo puts it there)."""
+        check(capi.lib().mgd_rk4_substep(region.handle, ADJOINT, stage, time, timeStepSize, None))
+
+
+class UniformCheckpointer:
+    """t_UniformCheckpointer's migrateTo (src/UniformCheckpointerImpl.f90:78-208) over the HBM
+    snapshots of the region's last Solver.runForwardCheckpointed."""
+
+    def migrateTo(self, region: "Region", timestep: int, stage: int):
+        check(capi.lib().mgd_checkpoint_migrate(region.handle, timestep, stage))
 
 
 class Solver:

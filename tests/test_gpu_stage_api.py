@@ -1,0 +1,116 @@
+"""A run driven stage by stage through the C ABI, as the reference's Solver loops drive it
+(src/SolverImpl.f90:672-912 forward, 914-1245 adjoint): RK4Integrator.substepForward /
+substepAdjoint, UniformCheckpointer.migrateTo, the functional after every forward stage,
+the control gradient and sensitivity before every adjoint stage, the adjoint forcing
+switched off at the final stage.  The states must equal mgd_run_forward /
+mgd_run_adjoint_checkpointed's bit for bit (same kernels in the same order); J and the
+sensitivity, summed in another order than the runs' fixed-tree quadrature, to 1e-12."""
+import numpy as np
+import pytest
+
+from magudi_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(walled=False):
+    if walled:
+        c = cases.walled_monopole(n=41, viscous=True, nsteps=7, save_interval=3)
+    else:
+        c = cases.acoustic_monopole(n=41, nsteps=11, save_interval=4)
+    c.dt = 0.2
+    return c
+
+
+@pytest.mark.parametrize("plain,walled", [(False, False), (True, False), (True, True)])
+def test_stagewise_run_matches_whole_run(plain, walled):
+    from magudi_amd.solver import Region, RK4Integrator, Solver, UniformCheckpointer
+    c = _case(walled)
+    print(f"stage api: {c.name} plain={plain}", flush=True)
+    q0 = c.initial_state()
+    R = Region(c, n_instances=2, force_plain=plain)
+    na = R.n_actuator
+    rng = np.random.default_rng(3)
+    ctl = None
+    if na:
+        ctl = np.stack([a * rng.standard_normal((4 * c.nsteps, na)) for a in (0.0, 2e-2)])
+    for k in range(2):
+        R.states[k].conservedVariables = q0
+    S = Solver(R, dt=c.dt, nsteps=c.nsteps, save_interval=c.save_interval)
+    J_run = S.runForwardCheckpointed(control=ctl)
+    qT_run = [R.states[k].conservedVariables for k in range(2)]
+    sens_run, grad_run = S.runAdjointCheckpointed()
+    lam0_run = [R.states[k].adjointVariables for k in range(2)]
+
+    # forward, stage by stage (Solver%runForward)
+    ti = RK4Integrator()
+    for k in range(2):
+        R.states[k].conservedVariables = q0
+    J = np.zeros(2)
+    dt, time = c.dt, 0.0
+    for step in range(1, c.nsteps + 1):
+        for i in range(1, 5):
+            if i in (2, 4):          # State%time as substepForwardRK4 advances it
+                time = time + dt / 2.0
+            sidx = 4 * (step - 1) + i - 1
+            ti.substepForward(R, time, dt, step, i, None if ctl is None else ctl[:, sidx])
+            J += ti.norm[i - 1] * dt * R.functional()
+    for k in range(2):
+        assert np.array_equal(R.states[k].conservedVariables, qT_run[k]), k
+    assert np.allclose(J, J_run, rtol=1e-12, atol=0.0)
+
+    # adjoint, stage by stage over the snapshots of the checkpointed run (from q0 again:
+    # the stage-wise forward left the final state in the conserved variables)
+    for k in range(2):
+        R.states[k].conservedVariables = q0
+    S.runForwardCheckpointed(control=ctl)
+    R.adjointInitialCondition()
+    mig = UniformCheckpointer()
+    sens = np.zeros(2)
+    grad = np.zeros((2, 4 * c.nsteps, na)) if na else None
+    for step in range(c.nsteps - 1, -1, -1):
+        for i in range(4, 0, -1):
+            if i == 1:
+                mig.migrateTo(R, step, 4)
+            else:
+                mig.migrateTo(R, step + 1, i - 1)
+            g, s = R.controlGradient()
+            if na:
+                grad[:, 4 * step + i - 1] = g
+            sens += ti.norm[i - 1] * dt * s
+            R.updateAdjointForcing(no_adjoint_forcing=(step == 0 and i == 1))
+            ti.substepAdjoint(R, 0.0, dt, step, i)
+    for k in range(2):
+        assert np.array_equal(R.states[k].adjointVariables, lam0_run[k]), k
+    if na:
+        assert np.array_equal(grad, grad_run)
+        assert np.allclose(sens, sens_run, rtol=1e-12, atol=0.0)
+
+
+def test_migrate_serves_every_stage_state():
+    """migrateTo in any order, across windows, against the stage states of a run that keeps
+    every stage (MGD_OPT_RESIDENT_STATES is not needed: a whole forward run with the
+    stage outputs collected by substeps)."""
+    from magudi_amd.solver import Region, RK4Integrator, Solver, UniformCheckpointer
+    c = _case()
+    q0 = c.initial_state()
+    R = Region(c, force_plain=True)
+    R.states[0].conservedVariables = q0
+    ti = RK4Integrator()
+    states = {(0, 4): q0}
+    time = 0.0
+    for step in range(1, c.nsteps + 1):
+        for i in range(1, 5):
+            if i in (2, 4):
+                time = time + c.dt / 2.0
+            ti.substepForward(R, time, c.dt, step, i)
+            states[(step, i)] = R.states[0].conservedVariables
+    R.states[0].conservedVariables = q0
+    S = Solver(R, dt=c.dt, nsteps=c.nsteps, save_interval=c.save_interval)
+    S.runForwardCheckpointed()
+    mig = UniformCheckpointer()
+    order = list(states)
+    np.random.default_rng(9).shuffle(order)
+    for key in order + sorted(states, reverse=True):
+        mig.migrateTo(R, *key)
+        assert np.array_equal(R.states[0].conservedVariables, states[key]), key
