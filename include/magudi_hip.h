@@ -3,7 +3,8 @@
  * hot path (forward RHS, discrete-adjoint RHS, RK4, HBM checkpointing).
  *
  * Plain C: pointers, sizes and an int status.  0 = success, < 0 = error
- * (mgd_last_error() gives the message), > 0 = solution-limit style conditions.
+ * (mgd_last_error() gives the message), MGD_SOLUTION_LIMIT (> 0): a run or check found
+ * a density or temperature outside the limits set with mgd_region_set_solution_limits.
  * Host arrays use magudi's layout x(nGridPoints, nComponents): component-major,
  * grid index i fastest.  Device buffers are owned by the region and stay
  * resident in HBM across RK4 stages; host copies happen only in put/get calls.
@@ -24,6 +25,7 @@ extern "C" {
 #define MGD_ERR_HIP (-2)
 #define MGD_ERR_UNSUPPORTED (-3)
 #define MGD_ERR_STATE (-4)
+#define MGD_SOLUTION_LIMIT 1
 
 /* Region%computeRhs modes (include/Region.f90:3-12) */
 #define MGD_FORWARD 1
@@ -124,6 +126,19 @@ int mgd_region_add_source(mgd_region* r, const double* location, double amplitud
  * src/PatchFactoryImpl.f90:161-374), far-field penalties and A+ blocks
  * (src/FarFieldPatchImpl.f90:3-74), mean pressure (src/AcousticNoiseImpl.f90:3-71). */
 int mgd_region_finalize(mgd_region* r);
+
+/* Solution limits (enable_solution_limits with minimum_density, maximum_density,
+ * minimum_temperature, maximum_temperature; src/SolverOptionsImpl.f90:72-77).  With
+ * enable = 1, every forward stage of a run checks its input state first, as
+ * checkSolutionLimits does (src/SolverImpl.f90:189-304, 814-819): an instance whose
+ * density or temperature reaches a bound (f <= min or f >= max) or is not a number gets
+ * J = HUGE (DBL_MAX), in adjoint runs also sensitivity = HUGE, and the run returns
+ * MGD_SOLUTION_LIMIT.  The kernels run to the end (the flagged results are discarded
+ * instead of stopping at the stage).  range = {rho_min, rho_max, T_min, T_max}.
+ * Not applied to region groups.  mgd_check_solution_limits: the same check of the
+ * current conserved state for a host Solver loop; crashed[i] per instance. */
+int mgd_region_set_solution_limits(mgd_region* r, int enable, const double* range);
+int mgd_check_solution_limits(mgd_region* r, int* crashed);
 
 /* Per-instance time offsets (n_instances values, NULL to clear): instance i runs at
  * t + offsets[i] wherever a time enters (acoustic source phase, stage times of a run).

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libmagudi_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "magudi_hip.h")
 
 MGD_OK = 0
+SOLUTION_LIMIT = 1
 FORWARD, ADJOINT = 1, -1
 CONSERVED, ADJOINT_VARS, RHS, INITIAL_STATE, FINAL_STATE, SCRATCH = 0, 1, 2, 4, 5, 6
 FIELD_TARGET_STATE, FIELD_CONTROL_MOLLIFIER, FIELD_TARGET_MOLLIFIER = 0, 1, 2
@@ -101,6 +102,8 @@ def lib():
         "mgd_checkpoint_migrate": [vp, C.c_int, C.c_int],
         "mgd_control_gradient": [vp, dp, dp],
         "mgd_adjoint_initial_condition": [vp, C.c_int],
+        "mgd_region_set_solution_limits": [vp, C.c_int, dp],
+        "mgd_check_solution_limits": [vp, C.POINTER(C.c_int)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

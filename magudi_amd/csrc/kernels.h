@@ -74,6 +74,10 @@ void launch_gradient(const Geo& g, const DPatch* pt, int np, const double* lam, 
 // out[inst] = <a, b>_H over the instance's state arrays; partial: ninst * ceil(N/256)
 void launch_inner_product(const Geo& g, const double* a, const double* b, double* partial,
                           double* out, hipStream_t st);
+// Solution limits: flag[inst] = 1 where density or temperature of instance inst reaches
+// range (rho_min, rho_max, T_min, T_max) or is not a number (flags are not cleared)
+void launch_check_limits(const Geo& g, const double* q, const double* range, double* flag,
+                         hipStream_t st);
 // Terminal kinetic energy: J[inst] = sum_p H_p |rho u|^2 / (2 rho), lam = dJ/dq pointwise;
 // partial: ninst * ceil(N/256)
 void launch_kinetic_energy(const Geo& g, const double* q, double* lam, double* partial, double* J,

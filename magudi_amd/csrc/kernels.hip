@@ -958,6 +958,24 @@ __global__ void __launch_bounds__(kBlock) k_inner_partial(Geo g, const double* _
   if (threadIdx.x == 0) partial[inst * gridDim.x + blockIdx.x] = s;
 }
 
+// checkSolutionLimits (src/SolverImpl.f90:189-304, isVariableWithinRange
+// src/GridImpl.f90:1579-1620): an instance whose density or temperature reaches a bound
+// (f <= min or f >= max), or is not a number, gets flag[inst] = 1 (a plain store: every
+// writer stores the same value).
+template <int ND>
+__global__ void __launch_bounds__(kBlock) k_check_limits(Geo g, const double* __restrict__ q,
+                                                         double rmin, double rmax, double tmin,
+                                                         double tmax, double* __restrict__ flag) {
+  const long p = gidx();
+  if (p >= g.N) return;
+  const long inst = blockIdx.y;
+  double v[ND + 2];
+  load_q<ND>(q + inst * (ND + 2) * g.N, g.N, p, v);
+  const Prim<ND> s = prim<ND>(v, g.gamma);
+  const bool ok = v[0] > rmin && v[0] < rmax && s.T > tmin && s.T < tmax;
+  if (!ok) flag[inst] = 1.0;
+}
+
 // Terminal kinetic-energy cost J = sum_p H_p |rho u|^2 / (2 rho) of each instance and its
 // pointwise derivative, the adjoint state at the final time (the terminal condition of
 // adjoint_nonzero_initial_condition, src/SolverImpl.f90:368-428, 1096-1102):
@@ -1982,6 +2000,14 @@ void launch_inner_product(const Geo& g, const double* a, const double* b, double
     k_inner_partial<ND><<<dim3(nb, g.ninst), kRedBlock, 0, st>>>(g, a, b, partial);
   });
   k_reduce_final<<<g.ninst, kRedBlock, 0, st>>>(nb, partial, 1.0, nullptr, out);
+}
+
+void launch_check_limits(const Geo& g, const double* q, const double* range, double* flag,
+                         hipStream_t st) {
+  MGD_DISPATCH(g.nd, {
+    k_check_limits<ND><<<grid_pts(g), kBlock, 0, st>>>(g, q, range[0], range[1], range[2], range[3],
+                                                       flag);
+  });
 }
 
 void launch_kinetic_energy(const Geo& g, const double* q, double* lam, double* partial, double* J,

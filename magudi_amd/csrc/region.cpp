@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -125,6 +126,12 @@ struct mgd_region {
   mgd::DevBuf q_alt, qs_alt;             // second state buffers of out-of-place RK stages
   mgd::DevBuf lam_alt;                   // second adjoint buffer of out-of-place adjoint stages
   int mig_loaded = -1;                   // snapshot step whose window mgd_checkpoint_migrate holds
+  // solution limits (enable_solution_limits): rho_min, rho_max, T_min, T_max; checked on the
+  // input of every forward stage of a run while `checking` (not in recomputes)
+  int lim_on = 0;
+  double lim[4] = {0, 0, 0, 0};
+  bool checking = false;
+  mgd::DevBuf d_flag;                    // per-instance violation flags
   mgd::DevBuf ipart, iout;               // inner-product partial sums and results
   GraphSlot graphs[2];                   // captured runs: forward, forward + adjoint
   std::vector<double> toff;              // per-instance time offsets (empty: none)
@@ -462,6 +469,11 @@ std::vector<double> stage_times(double t0, double dt, int nsteps) {
   return t;
 }
 
+// checkSolutionLimits before a forward stage of a run (src/SolverImpl.f90:814-819)
+void check_stage_input(mgd_region* r, const double* q) {
+  if (r->lim_on && r->checking) launch_check_limits(r->geo, q, r->lim, r->d_flag.p, r->stream);
+}
+
 struct Timer {
   mgd_region* r;
   int slot;
@@ -475,6 +487,7 @@ struct Timer {
 void forward_stage(mgd_region* r, DevBuf& qb, DevBuf& alt, double* b1, double* b2, int sidx,
                    int stage, double dt, double* qcopy) {
   double* q = qb.p;
+  check_stage_input(r, q);
   const int na = n_act(r);
   const double* ctrl = nullptr;
   long cstride = 0;
@@ -619,6 +632,7 @@ void fused_forward_step(mgd_region* r, double* q0, double* x, double* y, double*
   for (int i = 1; i <= 4; ++i) {
     double* o = outs ? outs[i - 1] : qo[i - 1];
     const double* in = outs ? (i == 1 ? q0 : outs[i - 2]) : qin[i - 1];
+    if (st == r->stream) check_stage_input(r, in);
     FwdStage a = fwd_args(r, in, q0, b2, o, i, dt, 4 * s + i - 1, nst, with_j);
     launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, a, st);
     if (with_j) reduce_chunk(r, r->jpart, n_cost(r), nst, 4 * s + i - 1, false);
@@ -960,8 +974,11 @@ int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, c
 int enqueue_forward_ck(mgd_region* r) {
   const long M = (long)r->ninst * M_of(r);
   launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
+  if (r->lim_on) launch_fill(r->ninst, 0.0, r->d_flag.p, r->stream);
+  r->checking = true;
   const int rc = fused_ok(r) ? fused_forward_pass(r, r->ck.dt, r->ck.nsteps, r->ck.K)
                              : plain_forward_pass(r, r->ck.dt, r->ck.nsteps, r->ck.K);
+  r->checking = false;
   if (rc) return rc;
   launch_copy(M, r->q.p, r->qT.p, r->stream);   // final state: terminal forcing, MGD_FINAL_STATE
   return MGD_OK;
@@ -997,11 +1014,14 @@ int enqueue_adjoint_ck(mgd_region* r, bool terminal, bool want_grad) {
 int enqueue_forward_run(mgd_region* r, double dt, int nsteps) {
   const int nst = 4 * nsteps;
   launch_fill(r->ninst, 0.0, r->acc_J.p, r->stream);
+  if (r->lim_on) launch_fill(r->ninst, 0.0, r->d_flag.p, r->stream);
+  r->checking = true;
   if (fused_ok(r)) {
     for (int s = 0; s < nsteps; ++s)
       fused_forward_step(r, r->q.p, r->b1.p, r->qy.p, r->b2.p, s, dt, nst, true, nullptr);
     if (n_cost(r) > 0)
       launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, r->stream);
+    r->checking = false;
     return MGD_OK;
   }
   for (int s = 0; s < nsteps; ++s)
@@ -1012,7 +1032,26 @@ int enqueue_forward_run(mgd_region* r, double dt, int nsteps) {
         launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), n_cost(r), 0, r->q.p, r->partial.p,
                             kRkNorm[i - 1] * dt, r->acc_J.p, nullptr, r->stream);
     }
+  r->checking = false;
   return MGD_OK;
+}
+
+// After a run's forward pass: instances that reached a solution limit get the cost
+// HUGE (costFunctional = HUGE(0.0_wp), src/SolverImpl.f90:814-819) and, in adjoint runs,
+// the sensitivity HUGE; the return value is MGD_SOLUTION_LIMIT if any did.
+int limits_outcome(mgd_region* r, double* J, double* sensitivity) {
+  if (!r->lim_on) return MGD_OK;
+  std::vector<double> f(r->ninst, 0.0);
+  MGD_HIP(hipMemcpy(f.data(), r->d_flag.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
+  int rc = MGD_OK;
+  for (int i = 0; i < r->ninst; ++i)
+    if (f[i] != 0.0) {
+      rc = MGD_SOLUTION_LIMIT;
+      if (J) J[i] = std::numeric_limits<double>::max();
+      if (sensitivity) sensitivity[i] = std::numeric_limits<double>::max();
+    }
+  if (rc) set_error("a solution limit was reached (density or temperature out of range)");
+  return rc;
 }
 
 // HIP graph of a run's device work, replayed while the run parameters and every device
@@ -1701,6 +1740,43 @@ int mgd_region_finalize(mgd_region* r) {
   return MGD_OK;
 }
 
+int mgd_region_set_solution_limits(mgd_region* r, int enable, const double* range) {
+  if (!r || (enable && !range)) { set_error("bad argument"); return MGD_ERR_ARG; }
+  if (enable && !(range[0] < range[1] && range[2] < range[3])) {
+    set_error("solution limits need minimum < maximum for density and temperature");
+    return MGD_ERR_ARG;
+  }
+  r->lim_on = enable != 0;
+  if (r->lim_on) {
+    for (int k = 0; k < 4; ++k) r->lim[k] = range[k];
+    if (r->d_flag.alloc(r->ninst)) return MGD_ERR_HIP;
+    launch_fill(r->ninst, 0.0, r->d_flag.p, r->stream);
+    MGD_HIP(hipStreamSynchronize(r->stream));
+  }
+  return MGD_OK;
+}
+
+int mgd_check_solution_limits(mgd_region* r, int* crashed) {
+  if (!r || !r->finalized || !crashed) { set_error("bad argument"); return MGD_ERR_ARG; }
+  if (!r->lim_on) {
+    for (int i = 0; i < r->ninst; ++i) crashed[i] = 0;
+    return MGD_OK;
+  }
+  launch_fill(r->ninst, 0.0, r->d_flag.p, r->stream);
+  launch_check_limits(r->geo, r->q.p, r->lim, r->d_flag.p, r->stream);
+  MGD_HIP(hipGetLastError());
+  std::vector<double> f(r->ninst, 0.0);
+  MGD_HIP(hipMemcpyAsync(f.data(), r->d_flag.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost,
+                         r->stream));
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  int any = 0;
+  for (int i = 0; i < r->ninst; ++i) {
+    crashed[i] = f[i] != 0.0;
+    any |= crashed[i];
+  }
+  return any ? MGD_SOLUTION_LIMIT : MGD_OK;
+}
+
 int mgd_region_set_time_offsets(mgd_region* r, const double* offsets) {
   if (!r) { set_error("null region"); return MGD_ERR_ARG; }
   if (offsets) r->toff.assign(offsets, offsets + r->ninst);
@@ -1940,7 +2016,7 @@ int mgd_run_forward(mgd_region* r, double t0, double dt, int nsteps, const doubl
     return rc;
   if (int rc = finish_timed(r)) return rc;
   if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
-  return MGD_OK;
+  return limits_outcome(r, J, nullptr);
 }
 
 int mgd_run_adjoint(mgd_region* r, double t0, double dt, int nsteps, int save_interval,
@@ -1963,7 +2039,8 @@ int mgd_run_adjoint(mgd_region* r, double t0, double dt, int nsteps, int save_in
     return rc;
   if (int rc = finish_timed(r)) return rc;
   if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
-  return copy_adjoint_outputs(r, gradient, sensitivity);
+  if (int rc = copy_adjoint_outputs(r, gradient, sensitivity)) return rc;
+  return limits_outcome(r, J, sensitivity);
 }
 
 int mgd_run_forward_checkpointed(mgd_region* r, double t0, double dt, int nsteps, int save_interval,
@@ -1976,7 +2053,7 @@ int mgd_run_forward_checkpointed(mgd_region* r, double t0, double dt, int nsteps
   if (int rc = finish_timed(r)) return rc;
   r->ck.valid = true;
   if (J) MGD_HIP(hipMemcpy(J, r->acc_J.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost));
-  return MGD_OK;
+  return limits_outcome(r, J, nullptr);
 }
 
 int mgd_run_adjoint_checkpointed(mgd_region* r, int terminal, double* gradient, double* sensitivity) {

@@ -27,6 +27,15 @@ __all__ = ["Region", "State", "Solver", "RK4Integrator", "UniformCheckpointer", 
            "ADJOINT"]
 
 
+def check_run(region: "Region", rc: int):
+    """A run's status: errors raise MagudiError; MGD_SOLUTION_LIMIT marks the region's
+    solutionCrashed (the flagged instances' J / sensitivity are HUGE, as in
+    checkSolutionLimits, src/SolverImpl.f90:814-819)."""
+    region.solutionCrashed = rc == capi.SOLUTION_LIMIT
+    if rc != capi.SOLUTION_LIMIT:
+        check(rc)
+
+
 class State:
     """View of one instance's arrays (t_State)."""
 
@@ -133,6 +142,7 @@ class Region:
                                                    s.frequency, s.radius, s.phase))
         check(capi.lib().mgd_region_finalize(self.handle))
         self.states = [State(self, i) for i in range(n_instances)]
+        self.solutionCrashed = False
         act = case.patches_of(3)
         self.n_actuator = len(case.patch_points(act[0])) if act else 0
 
@@ -170,6 +180,20 @@ class Region:
         out = np.zeros(self.n_instances)
         check(capi.lib().mgd_functional(self.handle, dptr(out)))
         return out
+
+    def setSolutionLimits(self, density_range, temperature_range, enable: bool = True):
+        """enable_solution_limits with (minimum, maximum) density and temperature: every
+        forward stage of a run then checks its input state (checkSolutionLimits)."""
+        rng = np.array(list(density_range) + list(temperature_range), dtype=np.float64)
+        check(capi.lib().mgd_region_set_solution_limits(self.handle, int(enable), dptr(rng)))
+
+    def checkSolutionLimits(self) -> np.ndarray:
+        """Per instance: whether the current conserved state reaches a solution limit."""
+        out = (C.c_int * self.n_instances)()
+        rc = capi.lib().mgd_check_solution_limits(self.handle, out)
+        if rc != capi.SOLUTION_LIMIT:
+            check(rc)
+        return np.array(list(out), dtype=bool)
 
     def terminalKineticEnergy(self) -> np.ndarray:
         """Kinetic energy J = sum_p H_p |rho u|^2 / (2 rho) of each instance's current
@@ -275,7 +299,7 @@ class Solver:
         r = self.region
         J = np.zeros(r.n_instances)
         ctl = None if control is None else np.ascontiguousarray(control, dtype=np.float64)
-        check(capi.lib().mgd_run_forward(r.handle, t0, self.dt, self.nTimesteps, dptr(ctl),
+        check_run(r, capi.lib().mgd_run_forward(r.handle, t0, self.dt, self.nTimesteps, dptr(ctl),
                                          dptr(J), int(use_graph)))
         return J
 
@@ -292,7 +316,7 @@ class Solver:
         if want_gradient and r.n_actuator > 0:
             grad = np.zeros((r.n_instances, 4 * self.nTimesteps, r.n_actuator))
         ctl = None if control is None else np.ascontiguousarray(control, dtype=np.float64)
-        check(capi.lib().mgd_run_adjoint(r.handle, t0, self.dt, self.nTimesteps, self.saveInterval,
+        check_run(r, capi.lib().mgd_run_adjoint(r.handle, t0, self.dt, self.nTimesteps, self.saveInterval,
                                          dptr(ctl), dptr(grad), dptr(sens), dptr(J),
                                          int(use_graph)))
         return sens, grad, J
@@ -306,7 +330,7 @@ class Solver:
         r = self.region
         J = np.zeros(r.n_instances)
         ctl = None if control is None else np.ascontiguousarray(control, dtype=np.float64)
-        check(capi.lib().mgd_run_forward_checkpointed(r.handle, t0, self.dt, self.nTimesteps,
+        check_run(r, capi.lib().mgd_run_forward_checkpointed(r.handle, t0, self.dt, self.nTimesteps,
                                                       self.saveInterval, dptr(ctl), dptr(J)))
         return J
 
