@@ -140,6 +140,15 @@ int mgd_region_finalize(mgd_region* r);
 int mgd_region_set_solution_limits(mgd_region* r, int enable, const double* range);
 int mgd_check_solution_limits(mgd_region* r, int* crashed);
 
+/* Region%getTimeStepSize in use_constant_CFL_mode (src/RegionImpl.f90:1842-1875, through
+ * State%computeTimeStepSize, src/StateImpl.f90:605-670) and State%computeCfl
+ * (src/StateImpl.f90:539-603): per instance, computeTimeStepSize / computeCfl
+ * (src/CNSHelperImpl.f90:842-982) of the current conserved state -- the advective limit
+ * and, in viscous regions, the diffusive one.  dt / cfl: n_instances values (the
+ * reference's getTimeStepSize is their minimum). */
+int mgd_time_step_size(mgd_region* r, double cfl, double* dt);
+int mgd_cfl(mgd_region* r, double dt, double* cfl);
+
 /* Per-instance time offsets (n_instances values, NULL to clear): instance i runs at
  * t + offsets[i] wherever a time enters (acoustic source phase, stage times of a run).
  * Lets independent time segments (multiple shooting, bin/msforward.f90:242-365) share

@@ -104,6 +104,8 @@ def lib():
         "mgd_adjoint_initial_condition": [vp, C.c_int],
         "mgd_region_set_solution_limits": [vp, C.c_int, dp],
         "mgd_check_solution_limits": [vp, C.POINTER(C.c_int)],
+        "mgd_time_step_size": [vp, C.c_double, dp],
+        "mgd_cfl": [vp, C.c_double, dp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -74,6 +74,10 @@ void launch_gradient(const Geo& g, const DPatch* pt, int np, const double* lam, 
 // out[inst] = <a, b>_H over the instance's state arrays; partial: ninst * ceil(N/256)
 void launch_inner_product(const Geo& g, const double* a, const double* b, double* partial,
                           double* out, hipStream_t st);
+// computeTimeStepSize (mode 0: out = min cfl / speed, param = cfl) or computeCfl (mode 1:
+// out = max speed * dt, param = dt) per instance; partial: ninst * ceil(N/256)
+void launch_wave_speed(const Geo& g, const double* q, int mode, double param, double* partial,
+                       double* out, hipStream_t st);
 // Solution limits: flag[inst] = 1 where density or temperature of instance inst reaches
 // range (rho_min, rho_max, T_min, T_max) or is not a number (flags are not cleared)
 void launch_check_limits(const Geo& g, const double* q, const double* range, double* flag,

@@ -4,6 +4,8 @@
 // through the cache hierarchy; every nd (1-3), inviscid and viscous, periodic and
 // bounded directions are covered.  These are the reference-shaped building blocks;
 // the fused 2-D kernels in fused2d.hip replace the hot ones for the benchmark path.
+#include <cfloat>
+
 #include "device.h"
 #include "kernels.h"
 
@@ -956,6 +958,82 @@ __global__ void __launch_bounds__(kBlock) k_inner_partial(Geo g, const double* _
   }
   const double s = block_sum(v);
   if (threadIdx.x == 0) partial[inst * gridDim.x + blockIdx.x] = s;
+}
+
+// computeTimeStepSize / computeCfl (src/CNSHelperImpl.f90:842-982) per instance: the
+// advective wave speed J (c |m| + sum_j |u . m_j|), c^2 = (gamma - 1) T, and in viscous
+// regions the diffusive one J^2 sum(metrics^2) max(2 mu, kappa); mode 0 reduces
+// cfl / speed by min (the time step), mode 1 speed * dt by max (the CFL number).  The
+// grid is rectilinear: the reference's row sums and dot products add exact zeros for the
+// off-diagonal metrics, so the sums below, without them, are the same numbers.
+template <int ND>
+__global__ void __launch_bounds__(kBlock) k_wave_speed(Geo g, const double* __restrict__ q, int mode,
+                                                       double param, double* __restrict__ partial) {
+  __shared__ double sh[kRedBlock];
+  const long inst = blockIdx.y;
+  const long p = (long)blockIdx.x * kRedBlock + threadIdx.x;
+  double v = mode == 0 ? DBL_MAX : 0.0;
+  if (p < g.N) {
+    int ijk[3];
+    unflat<ND>(g, p, ijk);
+    double x[ND + 2];
+    load_q<ND>(q + inst * (ND + 2) * g.N, g.N, p, x);
+    const Prim<ND> s = prim<ND>(x, g.gamma);
+    const double jac = jacobian<ND>(g, ijk);
+    double m[ND];
+    for (int j = 0; j < ND; ++j) m[j] = metric<ND>(g, j, ijk);
+    double w = 0.0;
+    for (int j = 0; j < ND; ++j) w = w + m[j] * m[j];
+    const double c = sqrt((g.gamma - 1.0) * s.T);
+    w = c * sqrt(w);
+    for (int j = 0; j < ND; ++j) w = w + fabs(s.u[j] * m[j]);
+    w = jac * w;
+    v = mode == 0 ? param / w : w * param;
+    if (g.viscous) {
+      double mu, la, ka;
+      transport(g, s.T, &mu, &la, &ka);
+      double sm = 0.0;
+      for (int j = 0; j < ND; ++j) sm += m[j] * m[j];
+      const double a = 2.0 * mu;
+      const double wd = jac * jac * sm * (a > ka ? a : ka);
+      const double vd = mode == 0 ? param / wd : wd * param;
+      v = mode == 0 ? (vd < v ? vd : v) : (vd > v ? vd : v);
+    }
+  }
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = kRedBlock / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      const double o = sh[threadIdx.x + st];
+      const double a = sh[threadIdx.x];
+      sh[threadIdx.x] = mode == 0 ? (o < a ? o : a) : (o > a ? o : a);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[inst * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// out[inst] = min (mode 0) or max (mode 1) of the instance's partials
+__global__ void __launch_bounds__(kBlock) k_minmax_final(int nblocks, const double* __restrict__ partial,
+                                                         int mode, double* __restrict__ out) {
+  __shared__ double sh[kRedBlock];
+  const long inst = blockIdx.x;
+  double v = mode == 0 ? DBL_MAX : 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += kRedBlock) {
+    const double o = partial[inst * nblocks + b];
+    v = mode == 0 ? (o < v ? o : v) : (o > v ? o : v);
+  }
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = kRedBlock / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      const double o = sh[threadIdx.x + st];
+      const double a = sh[threadIdx.x];
+      sh[threadIdx.x] = mode == 0 ? (o < a ? o : a) : (o > a ? o : a);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[inst] = sh[0];
 }
 
 // checkSolutionLimits (src/SolverImpl.f90:189-304, isVariableWithinRange
@@ -2000,6 +2078,15 @@ void launch_inner_product(const Geo& g, const double* a, const double* b, double
     k_inner_partial<ND><<<dim3(nb, g.ninst), kRedBlock, 0, st>>>(g, a, b, partial);
   });
   k_reduce_final<<<g.ninst, kRedBlock, 0, st>>>(nb, partial, 1.0, nullptr, out);
+}
+
+void launch_wave_speed(const Geo& g, const double* q, int mode, double param, double* partial,
+                       double* out, hipStream_t st) {
+  const int nb = (int)((g.N + kRedBlock - 1) / kRedBlock);
+  MGD_DISPATCH(g.nd, {
+    k_wave_speed<ND><<<dim3(nb, g.ninst), kRedBlock, 0, st>>>(g, q, mode, param, partial);
+  });
+  k_minmax_final<<<g.ninst, kRedBlock, 0, st>>>(nb, partial, mode, out);
 }
 
 void launch_check_limits(const Geo& g, const double* q, const double* range, double* flag,

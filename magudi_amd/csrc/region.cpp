@@ -1756,6 +1756,23 @@ int mgd_region_set_solution_limits(mgd_region* r, int enable, const double* rang
   return MGD_OK;
 }
 
+// computeTimeStepSize (mode 0) / computeCfl (mode 1) of the current conserved state
+int wave_speed_reduce(mgd_region* r, int mode, double param, double* out) {
+  if (!r || !r->finalized || !out) { set_error("bad argument"); return MGD_ERR_ARG; }
+  if (!(param > 0.0)) { set_error(mode == 0 ? "cfl must be positive" : "dt must be positive"); return MGD_ERR_ARG; }
+  const long nb = (r->N + 255) / 256;
+  if (r->ipart.alloc((size_t)nb * r->ninst) || r->iout.alloc(r->ninst)) return MGD_ERR_HIP;
+  launch_wave_speed(r->geo, r->q.p, mode, param, r->ipart.p, r->iout.p, r->stream);
+  MGD_HIP(hipGetLastError());
+  MGD_HIP(hipMemcpyAsync(out, r->iout.p, sizeof(double) * r->ninst, hipMemcpyDeviceToHost, r->stream));
+  MGD_HIP(hipStreamSynchronize(r->stream));
+  return MGD_OK;
+}
+
+int mgd_time_step_size(mgd_region* r, double cfl, double* dt) { return wave_speed_reduce(r, 0, cfl, dt); }
+
+int mgd_cfl(mgd_region* r, double dt, double* cfl) { return wave_speed_reduce(r, 1, dt, cfl); }
+
 int mgd_check_solution_limits(mgd_region* r, int* crashed) {
   if (!r || !r->finalized || !crashed) { set_error("bad argument"); return MGD_ERR_ARG; }
   if (!r->lim_on) {

@@ -173,8 +173,24 @@ class Region:
     def updateAdjointForcing(self, no_adjoint_forcing: bool = False):
         check(capi.lib().mgd_update_adjoint_forcing(self.handle, int(no_adjoint_forcing)))
 
-    def getTimeStepSize(self) -> float:
-        return self.case.dt
+    def getTimeStepSize(self, cfl: Optional[float] = None) -> float:
+        """The case's fixed time_step_size, or with ``cfl`` (use_constant_CFL_mode) the
+        smallest computeTimeStepSize over the instances' current states."""
+        if cfl is None:
+            return self.case.dt
+        return float(self.computeTimeStepSize(cfl).min())
+
+    def computeTimeStepSize(self, cfl: float) -> np.ndarray:
+        """State%computeTimeStepSize of each instance's conserved state (on the device)."""
+        out = np.zeros(self.n_instances)
+        check(capi.lib().mgd_time_step_size(self.handle, cfl, dptr(out)))
+        return out
+
+    def computeCfl(self, dt: float) -> np.ndarray:
+        """State%computeCfl of each instance's conserved state at time step dt."""
+        out = np.zeros(self.n_instances)
+        check(capi.lib().mgd_cfl(self.handle, dt, dptr(out)))
+        return out
 
     def functional(self) -> np.ndarray:
         out = np.zeros(self.n_instances)

@@ -92,6 +92,9 @@ int orc_compute_rhs(orc_region* r, int mode, const double* q, const double* lamb
 
 /* AcousticNoise J_inst (src/AcousticNoiseImpl.f90:123-206) */
 double orc_functional(orc_region* r, const double* q);
+/* computeTimeStepSize / computeCfl (src/CNSHelperImpl.f90:842-982) of state q */
+double orc_time_step_size(orc_region* r, const double* q, double cfl);
+double orc_cfl(orc_region* r, const double* q, double dt);
 /* AcousticNoise adjoint forcing (src/AcousticNoiseImpl.f90:208-280), patch-point array nv x np */
 int orc_adjoint_forcing(orc_region* r, const double* q, double* forcing);
 int orc_cost_target_npoints(const orc_region* r);

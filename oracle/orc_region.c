@@ -12,6 +12,7 @@
  *   computeSpongeStrengths, computeQuadratureOnPatches src/PatchFactoryImpl.f90:161-444
  *   AcousticSource, AcousticNoise, ThermalActuator gradient/sensitivity.
  */
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -934,6 +935,77 @@ static double quadrature(const orc_region* r, int type, const double* integrand_
   for (long p = 0; p < N; ++p) s += mask[p] * r->norm[p] * integrand_full[p];
   free(mask);
   return s;
+}
+
+/* Local wave speeds of computeTimeStepSize / computeCfl (src/CNSHelperImpl.f90:842-982) at
+ * point p after update_state: advection J (c |m| + sum_j |u . m_j|) with m_j the j-th row
+ * of the metrics; diffusion J^2 sum(metrics^2) max(2 mu, kappa) (its first loop over the
+ * rows is overwritten in the reference and has no effect). */
+static double advective_speed(const orc_region* r, long p) {
+  const int nd = r->d.nd;
+  const long N = r->N;
+  const double c = sqrt((r->d.gamma - 1.0) * r->T[p]);
+  double w = 0.0;
+  for (int j = 0; j < nd; ++j) {
+    double s = 0.0;
+    for (int k = 0; k < nd; ++k) {
+      const double m = r->metrics[(j * nd + k) * N + p];
+      s += m * m;
+    }
+    w = w + s;
+  }
+  w = c * sqrt(w);
+  for (int j = 0; j < nd; ++j) {
+    double d = 0.0;
+    for (int k = 0; k < nd; ++k) d += r->vel[k * N + p] * r->metrics[(j * nd + k) * N + p];
+    w = w + fabs(d);
+  }
+  return r->jacobian[p] * w;
+}
+
+static double diffusive_speed(const orc_region* r, long p) {
+  const int nd = r->d.nd;
+  const long N = r->N;
+  double s = 0.0;
+  for (int k = 0; k < nd * nd; ++k) {
+    const double m = r->metrics[k * N + p];
+    s += m * m;
+  }
+  const double jac = r->jacobian[p];
+  const double a = 2.0 * r->mu[p], b = r->kap[p];
+  return jac * jac * s * (a > b ? a : b);
+}
+
+/* computeTimeStepSize (src/CNSHelperImpl.f90:913-982): min over points of cfl / speed. */
+double orc_time_step_size(orc_region* r, const double* q, double cfl) {
+  update_state(r, q);
+  double dt = DBL_MAX;
+  for (long p = 0; p < r->N; ++p) {
+    const double v = cfl / advective_speed(r, p);
+    dt = v < dt ? v : dt;
+  }
+  if (r->d.viscous)
+    for (long p = 0; p < r->N; ++p) {
+      const double v = cfl / diffusive_speed(r, p);
+      dt = v < dt ? v : dt;
+    }
+  return dt;
+}
+
+/* computeCfl (src/CNSHelperImpl.f90:842-911): max over points of speed * dt. */
+double orc_cfl(orc_region* r, const double* q, double dt) {
+  update_state(r, q);
+  double cfl = 0.0;
+  for (long p = 0; p < r->N; ++p) {
+    const double v = advective_speed(r, p) * dt;
+    cfl = v > cfl ? v : cfl;
+  }
+  if (r->d.viscous)
+    for (long p = 0; p < r->N; ++p) {
+      const double v = diffusive_speed(r, p) * dt;
+      cfl = v > cfl ? v : cfl;
+    }
+  return cfl;
 }
 
 double orc_functional(orc_region* r, const double* q) {

@@ -89,6 +89,10 @@ def lib():
         L.orc_compute_rhs.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_double, dp, dp, C.c_double, dp]
         L.orc_functional.restype = C.c_double
         L.orc_functional.argtypes = [C.c_void_p, dp]
+        L.orc_time_step_size.restype = C.c_double
+        L.orc_time_step_size.argtypes = [C.c_void_p, dp, C.c_double]
+        L.orc_cfl.restype = C.c_double
+        L.orc_cfl.argtypes = [C.c_void_p, dp, C.c_double]
         L.orc_adjoint_forcing.argtypes = [C.c_void_p, dp, dp]
         L.orc_cost_target_npoints.argtypes = [C.c_void_p]
         L.orc_actuator_npoints.argtypes = [C.c_void_p]
@@ -218,6 +222,12 @@ class Region:
     def functional(self, q):
         q = np.ascontiguousarray(q, dtype=np.float64)
         return lib().orc_functional(self.h, _p(q))
+
+    def time_step_size(self, q, cfl):
+        return lib().orc_time_step_size(self.h, _p(np.ascontiguousarray(q, dtype=np.float64)), cfl)
+
+    def cfl(self, q, dt):
+        return lib().orc_cfl(self.h, _p(np.ascontiguousarray(q, dtype=np.float64)), dt)
 
     def adjoint_forcing(self, q):
         n = lib().orc_cost_target_npoints(self.h)

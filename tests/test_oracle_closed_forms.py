@@ -236,3 +236,39 @@ def test_kolmogorov_forcing_closed_form(sub):
     radj = O.rhs(-1, q, lam=lam)
     assert np.abs(radj[0] + 0.2 * fe).max() <= TOL
     assert np.abs(radj[1:]).max() <= TOL
+
+
+@pytest.mark.parametrize("viscous", [False, True])
+def test_time_step_size_closed_form(viscous):
+    # computeTimeStepSize / computeCfl (src/CNSHelperImpl.f90:842-982) on a uniform state and
+    # a uniform grid of spacings hx != hy: the advective limit is
+    # cfl / (c sqrt(1/hx^2 + 1/hy^2) + |u|/hx + |v|/hy) with c^2 = (gamma - 1) T; the
+    # diffusive one cfl / ((1/hx^2 + 1/hy^2) max(2 mu, kappa)).  A transposed metric row,
+    # |u| |m| in place of |u . m_j|, gamma T in place of (gamma - 1) T or a missing
+    # Jacobian factor change the answer.
+    nx, ny, hx, hy = 33, 29, 0.3, 0.7
+    x1, y1 = np.arange(nx) * hx, np.arange(ny) * hy
+    X, Y = np.meshgrid(x1, y1, indexing="xy")
+    coords = np.stack([X.ravel(), Y.ravel()])
+    npts = nx * ny
+    rho, vel, p = 1.3, (0.4, -0.25), 0.9
+    q = _uniform(2, npts, rho, vel, p)
+    re, pr = 1.0, 0.72
+    c = cases.Case(name="cfl", nd=2, n=(nx, ny, 1), coords=coords, dissipation=False,
+                   viscous=viscous, reynolds=re, prandtl=pr, patches=[],
+                   target=q.copy(), control_mollifier=np.zeros(npts),
+                   target_mollifier=np.zeros(npts), dt=1e-3, nsteps=1, save_interval=1)
+    O = ob.Region(c)
+    cfl = 0.6
+    T = GAMMA * p / ((GAMMA - 1.0) * rho)
+    sound = math.sqrt((GAMMA - 1.0) * T)
+    adv = sound * math.sqrt(1 / hx ** 2 + 1 / hy ** 2) + abs(vel[0]) / hx + abs(vel[1]) / hy
+    expect = cfl / adv
+    if viscous:
+        mu, kap = 1.0 / re, 1.0 / (re * pr)
+        expect = min(expect, cfl / ((1 / hx ** 2 + 1 / hy ** 2) * max(2 * mu, kap)))
+        assert cfl / ((1 / hx ** 2 + 1 / hy ** 2) * max(2 * mu, kap)) < cfl / adv   # diffusion limits
+    dt = O.time_step_size(q, cfl)
+    # the metrics of the SBP closure rows carry the rounding of the coordinate derivative
+    assert abs(dt - expect) <= 1e-10 * expect
+    assert abs(O.cfl(q, dt) - cfl) <= 1e-10 * cfl
