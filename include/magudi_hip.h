@@ -64,6 +64,7 @@ extern "C" {
 #define MGD_OP_ADJOINT_FIRST_DERIVATIVE 1
 #define MGD_OP_DISSIPATION 2
 #define MGD_OP_DISSIPATION_TRANSPOSE 3
+#define MGD_OP_SECOND_DERIVATIVE 4      /* t_Grid%secondDerivative (src/StencilOperatorImpl.f90:1745-1840) */
 
 /* Solver options and simulation flags a region is built from
  * (src/SolverOptionsImpl.f90:3-140, src/SimulationFlagsImpl.f90:3-70, grid options

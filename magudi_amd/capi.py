@@ -19,7 +19,7 @@ SOLUTION_LIMIT = 1
 FORWARD, ADJOINT = 1, -1
 CONSERVED, ADJOINT_VARS, RHS, INITIAL_STATE, FINAL_STATE, SCRATCH = 0, 1, 2, 4, 5, 6
 FIELD_TARGET_STATE, FIELD_CONTROL_MOLLIFIER, FIELD_TARGET_MOLLIFIER = 0, 1, 2
-OP_FIRST, OP_ADJOINT_FIRST, OP_DISS, OP_DISS_T = 0, 1, 2, 3
+OP_FIRST, OP_ADJOINT_FIRST, OP_DISS, OP_DISS_T, OP_SECOND = 0, 1, 2, 3, 4
 OPT_FORCE_PLAIN, OPT_TILE_SHAPE, OPT_RESIDENT_STATES = 1, 2, 3
 TERMINAL_KINETIC_ENERGY = 0
 KERNEL_FWD_STAGE, KERNEL_ADJ_STAGE, KERNEL_COPY, KERNEL_RHS_FORWARD, KERNEL_RHS_ADJOINT = 0, 1, 2, 3, 4

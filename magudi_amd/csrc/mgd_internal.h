@@ -36,7 +36,8 @@ struct Op {
 };
 
 enum OpKind { OP_FIRST = 0, OP_SECOND = 1, OP_DISS = 2, OP_DISS_T = 3, OP_COMPOSITE = 4 };
-enum OpSlot { SLOT_FIRST = 0, SLOT_ADJ = 1, SLOT_DISS = 2, SLOT_DISS_T = 3, kNumSlots = 4 };
+enum OpSlot { SLOT_FIRST = 0, SLOT_ADJ = 1, SLOT_DISS = 2, SLOT_DISS_T = 3, SLOT_SECOND = 4,
+              kNumSlots = 5 };
 
 int op_setup(Op* op, int scheme, int kind);     // sbp.cpp
 void op_adjoint(const Op& op, Op* adj);         // sbp.cpp

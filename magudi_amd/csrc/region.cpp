@@ -520,9 +520,16 @@ bool has_walls(const mgd_region* r) {
 }
 
 // The fused 2-D stage kernels cover inviscid 2-D regions with far-field, sponge,
-// actuator, cost-target and Kolmogorov patches; walls run on the generic kernels.
+// actuator, cost-target and Kolmogorov patches; walls run on the generic kernels, and so
+// do OVERLAP-periodic directions: the tile's periodic halo is a shifted copy of the tile
+// frame, in which the neighbour of the last distinct point is the first point rather
+// than the repeated last grid line the reference's ghost exchange serves (the two differ
+// when the repeated line holds values of its own).
 bool fused_ok(const mgd_region* r) {
-  return !r->force_plain && r->nd == 2 && !r->desc.viscous && r->tiling_ok && !has_walls(r);
+  bool overlap = false;
+  for (int d = 0; d < r->nd; ++d) overlap = overlap || r->desc.periodicity[d] == MGD_PERIODIC_OVERLAP;
+  return !r->force_plain && r->nd == 2 && !r->desc.viscous && r->tiling_ok && !has_walls(r) &&
+         !overlap;
 }
 
 int upload_ints(DevBuf& b, const void* data, size_t bytes) {
@@ -1416,8 +1423,9 @@ int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region
     }
   }
   // operators (Grid%setupSpatialDiscretization, src/GridImpl.f90:487-619)
-  Op first, diss, dissT, adj;
+  Op first, diss, dissT, adj, second;
   op_setup(&first, d->scheme, OP_FIRST);
+  op_setup(&second, d->scheme, OP_SECOND);
   op_setup(&diss, d->scheme, d->composite_dissipation ? OP_COMPOSITE : OP_DISS);
   op_setup(&dissT, d->scheme, OP_DISS_T);
   op_adjoint(first, &adj);
@@ -1426,6 +1434,7 @@ int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region
   r->ops[SLOT_ADJ] = adj;
   r->ops[SLOT_DISS] = diss;
   r->ops[SLOT_DISS_T] = dissT;
+  r->ops[SLOT_SECOND] = second;   // grid%secondDerivative (src/GridImpl.f90:30-31, 557-567)
   // Grid%update for a rectilinear grid: per-direction coordinate derivative and norm weight
   for (int d0 = 0; d0 < 3; ++d0) {
     const int len = r->desc.n[d0];

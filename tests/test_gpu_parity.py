@@ -85,6 +85,17 @@ def inviscid_box(shape):
     return dataclasses.replace(periodic_box(shape, dissipation=False), viscous=False)
 
 
+def overlap_x_monopole(n=41):
+    """Monopole periodic in x with OVERLAP periodicity (the last grid line repeats the
+    first, period (n - 1) dx): far field and sponges on the y faces only."""
+    c = cases.acoustic_monopole(n=n, nsteps=1)
+    dx = 28.0 / (n - 1)
+    c.periodicity = (cases.OVERLAP, cases.NONE, cases.NONE)
+    c.periodic_length = ((n - 1) * dx, 0.0, 0.0)
+    c.patches = [p for p in c.patches if p.normal_dir not in (1, -1)]
+    return c
+
+
 def kolmogorov_subbox(shape=(32, 16, 12)):
     """Periodic box whose two Kolmogorov forcings cover overlapping sub-boxes with a
     y-extent that is no whole period: the forcing's quadrature mean over the union of
@@ -123,6 +134,7 @@ CASES = {
     "box64x32x80_nodiss": lambda: periodic_box((64, 32, 80), dissipation=False),
     "box48x16x40_inviscid_nodiss": lambda: inviscid_box((48, 16, 40)),
     "box32x16x12_kolmogorov_subboxes": kolmogorov_subbox,
+    "monopole41_overlap_x": overlap_x_monopole,
     "monopole41_kolmogorov_subbox": monopole_kolmogorov,
 }
 
@@ -136,7 +148,7 @@ def test_grid_arrays(solver_mod, name):
         assert maxrel(R.grid(which), O.grid(which)) < 1e-13, which
 
 
-@pytest.mark.parametrize("op", [0, 1, 2, 3])
+@pytest.mark.parametrize("op", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("direction", [0, 1])
 def test_stencil_apply(solver_mod, op, direction):
     c = cases.acoustic_monopole(n=41, nsteps=1)
@@ -144,12 +156,43 @@ def test_stencil_apply(solver_mod, op, direction):
     rng = np.random.default_rng(op * 7 + direction)
     x = rng.standard_normal((3, c.npoints))
     y = R.applyOperator(op, direction, x)
-    kinds = {0: ob.FIRST, 1: ob.FIRST, 2: ob.DISS, 3: ob.DISS_T}
+    kinds = {0: ob.FIRST, 1: ob.FIRST, 2: ob.DISS, 3: ob.DISS_T, 4: ob.SECOND}
     o = ob.op(kinds[op])
     if op == 1:
         o = ob.adjoint(o)
     ref = ob.apply(o, direction, c.n, 0, x.copy())
     assert maxrel(y, ref) < 1e-14
+
+
+@pytest.mark.parametrize("periodicity", [1, 2])
+def test_second_derivative_periodic(solver_mod, periodicity):
+    """The second-derivative operator along periodic directions (PLANE and OVERLAP, whose
+    last point repeats the first): against the oracle, and on a resolved sine against
+    -k^2 sin to the scheme's accuracy."""
+    n = 64
+    if periodicity == 1:
+        x1 = np.arange(n) / n
+    else:
+        x1 = np.arange(n) / (n - 1)
+    X, Y = np.meshgrid(x1, x1, indexing="xy")
+    npts = n * n
+    c = cases.Case(name="periodic2nd", nd=2, n=(n, n, 1), coords=np.stack([X.ravel(), Y.ravel()]),
+                   periodicity=(periodicity, periodicity, 0), periodic_length=(1.0, 1.0, 0.0),
+                   dissipation=False, target=cases.quiescent_state(2, npts),
+                   control_mollifier=np.zeros(npts), target_mollifier=np.zeros(npts), dt=1e-3,
+                   nsteps=1, save_interval=1)
+    R = solver_mod.Region(c)
+    f = np.sin(2 * np.pi * X.ravel())[None, :] + 0.1 * np.random.default_rng(4).standard_normal((1, npts))
+    for d in (0, 1):
+        y = R.applyOperator(4, d, f)
+        ref = ob.apply(ob.op(ob.SECOND), d, c.n, periodicity, f.copy())
+        assert maxrel(y, ref) < 1e-14
+    s = np.sin(2 * np.pi * X.ravel())[None, :]
+    d2 = R.applyOperator(4, 0, s)
+    h = x1[1] - x1[0]
+    exact = -(2 * np.pi) ** 2 * s * h * h   # the operator is undivided: h^2 f''
+    inner = np.abs(d2 - exact).max() / np.abs(exact).max()
+    assert inner < 1e-6
 
 
 @pytest.mark.parametrize("plain", [False, True])
