@@ -58,6 +58,7 @@ extern "C" {
 #define MGD_FIELD_TARGET_STATE 0
 #define MGD_FIELD_CONTROL_MOLLIFIER 1
 #define MGD_FIELD_TARGET_MOLLIFIER 2
+#define MGD_FIELD_MEAN_PRESSURE 3       /* mean_pressure_file (N); default: from the target state */
 
 /* Stencil operators for mgd_stencil_apply (t_Grid members, include/Grid.f90) */
 #define MGD_OP_FIRST_DERIVATIVE 0

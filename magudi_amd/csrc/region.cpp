@@ -79,6 +79,7 @@ struct mgd_region {
   std::vector<std::unique_ptr<mgd::HostPatch>> patches;
   std::vector<mgd::Source> sources;
   std::vector<double> target, ctrl_moll, tgt_moll;
+  std::vector<double> mean_p;            // mean pressure field (mean_pressure_file), or empty
   bool finalized = false;
   mgd::Patches hpatches;
   mgd::DevBuf d_patches, d_sources;
@@ -1569,6 +1570,7 @@ int mgd_region_set_field(mgd_region* r, int which, const double* v) {
     case MGD_FIELD_TARGET_STATE: r->target.assign(v, v + (size_t)r->nv * N); break;
     case MGD_FIELD_CONTROL_MOLLIFIER: r->ctrl_moll.assign(v, v + N); break;
     case MGD_FIELD_TARGET_MOLLIFIER: r->tgt_moll.assign(v, v + N); break;
+    case MGD_FIELD_MEAN_PRESSURE: r->mean_p.assign(v, v + N); break;
     default: set_error("unknown field"); return MGD_ERR_ARG;
   }
   r->finalized = false;
@@ -1669,12 +1671,17 @@ int mgd_region_finalize(mgd_region* r) {
         break;
       }
       case MGD_PATCH_COST_TARGET: {
-        // mean pressure from the target state (src/AcousticNoiseImpl.f90:56-71)
+        // mean pressure: the mean_pressure_file field when given, else from the target
+        // state (src/AcousticNoiseImpl.f90:51-71)
         P.strength.resize(P.np);
         P.data.resize(P.np);
         for (int k = 0; k < P.np; ++k) {
           const long p = patch_point(r, P, k);
           P.strength[k] = r->tgt_moll[p];
+          if (!r->mean_p.empty()) {
+            P.data[k] = r->mean_p[p];
+            continue;
+          }
           double s = 0.0;
           for (int i = 0; i < nd; ++i) s += r->target[(i + 1) * N + p] * r->target[(i + 1) * N + p];
           P.data[k] = (g - 1.0) * (r->target[(nd + 1) * N + p] - 0.5 * s / r->target[p]);

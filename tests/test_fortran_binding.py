@@ -3,8 +3,13 @@
 magudi's drivers are Fortran (bin/Forward.f90, bin/Adjoint.f90); the module
 magudi_hip.f90 is what such a driver uses.  CPU checks: every bind(C) name is an
 entry point of include/magudi_hip.h, and the bind(C) derived types have the C
-structs' sizes.  GPU check: the Fortran adjoint driver prints the same J and
-sensitivity as the Python host path for the same monopole set-up.
+structs' sizes; the magudi.inp reader.  GPU checks: a Fortran program that builds the
+monopole in code prints the same J and sensitivity as the Python host path; the forward and
+adjoint drivers, run on an example directory written with the library's PLOT3D writers
+(magudi.inp, bc.dat, grid, initial condition, mollifiers), write J, the sensitivity and the
+gradient file of the oracle's run of the same case, in the stage loop and in the whole-run
+mode, and the forward driver with controller_switch reproduces the finite-difference
+relation (J(A) - J(0)) / A = Grad0.
 """
 import os
 import re
@@ -55,7 +60,7 @@ def test_struct_sizes_match(tmp_path):
 
 @pytest.mark.gpu
 def test_fortran_driver_matches_python():
-    exe = os.path.join(FDIR, "adjoint_hip")
+    exe = os.path.join(FDIR, "monopole_adjoint_hip")
     if not os.path.exists(exe):
         pytest.skip("Fortran driver not built (needs amdflang at build time)")
     from magudi_amd import cases
@@ -71,3 +76,135 @@ def test_fortran_driver_matches_python():
     sens, _, J = Solver(R).runAdjoint()
     np.testing.assert_allclose(float(vals["J"]), J[0], rtol=1e-12)
     np.testing.assert_allclose(float(vals["Grad0"]), sens[0], rtol=1e-10)
+
+
+def _write_case_directory(d, c, prefix="Monopole", extra=""):
+    """An example directory of case c in magudi's formats: magudi.inp, bc.dat, the PLOT3D
+    grid, initial condition and mollifier files."""
+    from magudi_amd import cases, plot3d
+    names = {v: k for k, v in cases.PATCH_TYPE_NAMES.items()}
+    n = c.n
+    plot3d.write_grid(str(d / f"{prefix}.xyz"), n, c.coords)
+    plot3d.write_solution(str(d / f"{prefix}.ic.q"), n, c.initial_state())
+    plot3d.write_function(str(d / f"{prefix}.target_mollifier.f"), n, c.target_mollifier[None])
+    plot3d.write_function(str(d / f"{prefix}.control_mollifier.f"), n, c.control_mollifier[None])
+    rows = ["# Name Type Grid normDir iMin iMax jMin jMax kMin kMax"]
+    for p in c.patches:
+        rows.append(f"{p.name} {names[p.type]} 1 {p.normal_dir} " + " ".join(str(v) for v in p.extent))
+    (d / "bc.dat").write_text("\n".join(rows) + "\n")
+    sponge = [p.params[0] for p in c.patches if p.type == cases.SPONGE]
+    src = c.sources[0]
+    inp = f"""# magudi.inp of a generated monopole case
+output_prefix = "{prefix}"
+grid_file = "{prefix}.xyz"
+initial_condition_file = "{prefix}.ic.q"
+boundary_condition_file = "bc.dat"
+control_mollifier_file = "{prefix}.control_mollifier.f"
+target_mollifier_file = "{prefix}.target_mollifier.f"
+include_viscous_terms = false
+use_constant_CFL_mode = false
+curvilinear_domain = false
+enable_controller = true
+enable_functional = true
+cost_functional_type = "SOUND"
+enable_adjoint_solver = true
+defaults/discretization_scheme = "SBP 4-8"
+time_step_size = {c.dt!r}
+number_of_timesteps = {c.nsteps}
+save_interval = {c.save_interval}
+add_dissipation = true
+dissipation_amount = {c.dissipation_amount!r}
+composite_dissipation = false
+defaults/sponge_amount = {sponge[0]!r}
+number_of_acoustic_sources = 1
+acoustic_source01/amplitude = {src.amplitude!r}
+acoustic_source01/frequency = {src.frequency!r}
+acoustic_source01/radius = {src.radius!r}
+acoustic_source01/x = {src.location[0]!r}
+acoustic_source01/y = {src.location[1]!r}
+{extra}"""
+    (d / "magudi.inp").write_text(inp)
+
+
+def _read_scalar(path):
+    return float(open(path).read().split()[0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("whole_run", [False, True])
+def test_fortran_drivers_on_an_example_directory(tmp_path, whole_run):
+    """96 steps of the 41^2 monopole at dt 0.2 (the pulse crosses the annulus and reaches the
+    sponges and the far-field boundaries): the drivers against the oracle."""
+    import oracle_bind as ob
+    from magudi_amd import cases, plot3d
+    fwd, adj = os.path.join(FDIR, "forward_hip"), os.path.join(FDIR, "adjoint_hip")
+    if not (os.path.exists(fwd) and os.path.exists(adj)):
+        pytest.skip("Fortran drivers not built (needs amdflang at build time)")
+    c = cases.acoustic_monopole(n=41, nsteps=96, save_interval=10)
+    c.dt = 0.2
+    _write_case_directory(tmp_path, c)
+    flags = ["--whole-run"] if whole_run else []
+    print(f"fortran drivers in {tmp_path} whole_run={whole_run}", flush=True)
+    out = subprocess.run([adj, *flags], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    J = _read_scalar(tmp_path / "Monopole.forward_run.txt")
+    sens = _read_scalar(tmp_path / "Monopole.adjoint_run.txt")
+    na = len(c.patch_points(c.patches_of(cases.ACTUATOR)[0]))
+    grad = plot3d.read_control(str(tmp_path / "Monopole.gradient_controlRegion.dat"), na)
+    so, go, _, Jo = ob.Region(c).run_adjoint(c.initial_state(), c.dt, c.nsteps, c.save_interval)
+    assert abs(J - Jo) <= 1e-12 * abs(Jo)
+    assert abs(sens - so) <= 1e-10 * abs(so)
+    assert np.abs(grad - go).max() <= 1e-10 * np.abs(go).max()
+    # the forward driver with the control forcing A g along the gradient (ZAXPY, then
+    # controller_switch = true): (J(A) - J(0)) / A against the sensitivity
+    A = 1e-4
+    plot3d.zaxpy(str(tmp_path / "Monopole.control_forcing_controlRegion.dat"), A,
+                 str(tmp_path / "Monopole.gradient_controlRegion.dat"))
+    with open(tmp_path / "magudi.inp", "a") as f:
+        f.write("controller_switch = true\n")
+    out = subprocess.run([fwd, *flags, "--output", "JA.txt"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    JA = _read_scalar(tmp_path / "JA.txt")
+    assert abs((JA - J) / A - sens) <= 1e-3 * abs(sens)
+
+
+@pytest.mark.skipif(FC is None, reason="amdflang not available")
+def test_input_reader(tmp_path):
+    """magudi_hip_input: comments, quotes, defaults, the last definition of a key, bc.dat rows
+    (InputHelper, src/InputHelperImpl.f90:142-269)."""
+    (tmp_path / "t.inp").write_text(
+        '# comment\noutput_prefix = "Case"   # trailing\nnumber_of_timesteps = 12\n'
+        'time_step_size = 2.5e-3\nadd_dissipation = true\nnumber_of_timesteps = 16\n'
+        "patches/a.b/sponge_amount = 0.25\n")
+    (tmp_path / "bc.dat").write_text(
+        "# Name Type Grid normDir iMin iMax jMin jMax kMin kMax\n"
+        "  sponge.E  SPONGE  1  1  1 29  1 -1  1 -1\n\n  ctl ACTUATOR 1 0 3 7 -9 -2 1 -1 # c\n")
+    prog = tmp_path / "reader.f90"
+    prog.write_text(
+        "program reader\n  use, intrinsic :: iso_c_binding\n  use magudi_hip_input\n"
+        "  type(t_PatchRow), allocatable :: rows(:)\n"
+        "  call parseInputFile('t.inp')\n"
+        "  print '(A)', trim(getOption('output_prefix', 'x'))\n"
+        "  print '(I0)', getOption('number_of_timesteps', 0)\n"
+        "  print '(ES23.15E3)', getOption('time_step_size', 0.0_c_double)\n"
+        "  print '(L1)', getOption('add_dissipation', .false.)\n"
+        "  print '(ES23.15E3)', getOption('patches/a.b/sponge_amount', 1.0_c_double)\n"
+        "  print '(ES23.15E3)', getOption('missing', 7.0_c_double)\n"
+        "  call readBoundaryConditions('bc.dat', rows)\n"
+        "  print '(I0,1X,A,1X,A,1X,I0,6(1X,I0))', size(rows), trim(rows(2)%name), "
+        "trim(rows(2)%type), rows(2)%normalDirection, rows(2)%extent\n"
+        "end program reader\n")
+    mod = tmp_path / "m"
+    mod.mkdir()
+    subprocess.check_call([FC, "-c", os.path.join(FDIR, "magudi_hip_input.f90"), "-J", str(mod),
+                           "-o", str(tmp_path / "in.o")], cwd=tmp_path)
+    subprocess.check_call([FC, str(prog), str(tmp_path / "in.o"), "-I", str(mod),
+                           "-o", str(tmp_path / "reader")], cwd=tmp_path)
+    out = subprocess.check_output([str(tmp_path / "reader")], cwd=tmp_path).decode().split("\n")
+    assert out[0] == "Case"
+    assert out[1] == "16"
+    assert float(out[2]) == 2.5e-3
+    assert out[3] == "T"
+    assert float(out[4]) == 0.25 and float(out[5]) == 7.0
+    assert out[6].split() == ["2", "ctl", "ACTUATOR", "0", "3", "7", "-9", "-2", "1", "-1"]
