@@ -357,6 +357,9 @@ int mgd_group_run_adjoint(mgd_group* group, double t0, const double* dt, int nst
  * 4-double auxiliary record (timestep, -, -, time) and 5 slots, an nv-component state
  * using slots 1 .. nv-1 and 5; function files n_scalars components.  Arrays are
  * component-major with the grid index fastest, as everywhere in this interface. */
+/* Record-marker width of the PLOT3D writers: 4 (default) or 8 bytes (the reference's
+ * USE_EXTENDED_PLOT3D build, include/config.h.in:34-40).  Readers detect the width. */
+int mgd_plot3d_set_record_markers(int bytes);
 int mgd_plot3d_detect(const char* path, int include_function, int* n_grids, int* file_type,
                       int* n_scalars, int* has_iblank, int* native_endian, int* sizes,
                       int max_grids);

@@ -3,7 +3,8 @@
 //
 // PLOT3D (src/PLOT3DFormat.c:88-294, src/PLOT3DHelperImpl.f90:554-1286): Fortran
 // unformatted sequential records, each framed by a leading and trailing byte count
-// (4-byte markers), in the byte order of the writer:
+// (4-byte markers, or 8-byte ones in the reference's USE_EXTENDED_PLOT3D build,
+// include/config.h.in:34-40), in the byte order of the writer:
 //   record 1  number of grids (int32)
 //   record 2  per grid nx, ny, nz (int32), plus the number of scalars for function files
 //   then per grid
@@ -11,7 +12,10 @@
 //     solution  a 4-double auxiliary record (timestep, -, -, time), then 5 components
 //               (rho, rho u, rho v, rho w, rho E); a 2-D state fills slots 1-3 and 5
 //     function  nScalars components
-// The byte order is detected from the first record marker (must read 4).
+// The byte order and the marker width are detected from the first record: its marker
+// reads 4; with 4-byte markers the next 4 bytes are the (positive) grid count, with 8-byte
+// markers they are the zero high half of the marker.  Writers use 4-byte markers unless
+// mgd_plot3d_set_record_markers(8) was called.
 //
 // Control forcing and gradient files (src/ActuatorPatchImpl.f90:226-458): raw native
 // doubles, one record of nPatchPoints values per RK stage, in the order the adjoint
@@ -41,10 +45,13 @@ uint64_t swap64(uint64_t x) {
   return r;
 }
 
+int g_marker_bytes = 4;   // record markers the writers use
+
 struct Reader {
   std::vector<unsigned char> buf;
   size_t pos = 0;
   bool native = true;
+  bool wide = false;         // 8-byte record markers
   bool load(const char* path) {
     FILE* f = std::fopen(path, "rb");
     if (!f) return false;
@@ -76,16 +83,32 @@ struct Reader {
     pos += 8 * n;
     return true;
   }
+  bool i64(int64_t* v) {
+    if (pos + 8 > buf.size()) return false;
+    uint64_t u;
+    std::memcpy(&u, &buf[pos], 8);
+    pos += 8;
+    if (!native) u = swap64(u);
+    std::memcpy(v, &u, 8);
+    return true;
+  }
+  bool marker(int64_t* m) {
+    if (wide) return i64(m);
+    int32_t v;
+    if (!i32(&v)) return false;
+    *m = v;
+    return true;
+  }
   // a whole record: returns its byte count and leaves pos at its first byte
   bool open_record(int64_t* bytes) {
-    int32_t m;
-    if (!i32(&m)) return false;
+    int64_t m;
+    if (!marker(&m)) return false;
     *bytes = m;
-    return m >= 0 && pos + (size_t)m + 4 <= buf.size();
+    return m >= 0 && pos + (size_t)m + (wide ? 8 : 4) <= buf.size();
   }
   bool close_record(int64_t bytes) {
-    int32_t m;
-    return i32(&m) && m == bytes;
+    int64_t m;
+    return marker(&m) && m == bytes;
   }
 };
 
@@ -97,12 +120,21 @@ struct Layout {
 
 // Record structure of a PLOT3D file (detect_format, src/PLOT3DFormat.c:88-205).
 int parse(Reader& R, int include_function, Layout* L) {
-  if (R.buf.size() < 4) return io_error("empty or truncated PLOT3D file", MGD_ERR_STATE);
-  int32_t first;
+  if (R.buf.size() < 12) return io_error("empty or truncated PLOT3D file", MGD_ERR_STATE);
+  int32_t first, second;
   std::memcpy(&first, R.buf.data(), 4);
-  R.native = first == 4;
-  if (!R.native && (int32_t)swap32((uint32_t)first) != 4)
+  std::memcpy(&second, R.buf.data() + 4, 4);
+  int64_t first8;
+  std::memcpy(&first8, R.buf.data(), 8);
+  if ((first == 4 || (int32_t)swap32((uint32_t)first) == 4) && second != 0) {
+    R.wide = false;
+    R.native = first == 4;
+  } else if (first8 == 4 || (int64_t)swap64((uint64_t)first8) == 4) {
+    R.wide = true;
+    R.native = first8 == 4;
+  } else {
     return io_error("not a PLOT3D whole-format file (first record marker is not 4)", MGD_ERR_STATE);
+  }
   // every record: matching leading and trailing markers, last one followed by EOF
   R.pos = 0;
   while (R.pos < R.buf.size()) {
@@ -171,7 +203,14 @@ struct Writer {
   explicit Writer(const char* path) { f = std::fopen(path, "wb"); ok = f != nullptr; }
   ~Writer() { if (f) std::fclose(f); }
   void raw(const void* p, size_t n) { if (ok) ok = std::fwrite(p, 1, n, f) == n; }
-  void marker(int64_t b) { const int32_t m = (int32_t)b; raw(&m, 4); }
+  void marker(int64_t b) {
+    if (g_marker_bytes == 8) {
+      raw(&b, 8);
+    } else {
+      const int32_t m = (int32_t)b;
+      raw(&m, 4);
+    }
+  }
 };
 
 void write_header(Writer& W, const int* n, int nscalars) {
@@ -191,6 +230,12 @@ bool sizes_ok(const int* n) { return n && n[0] > 0 && n[1] > 0 && n[2] > 0; }
 }  // namespace
 
 extern "C" {
+
+int mgd_plot3d_set_record_markers(int bytes) {
+  if (bytes != 4 && bytes != 8) return io_error("PLOT3D record markers are 4 or 8 bytes");
+  g_marker_bytes = bytes;
+  return MGD_OK;
+}
 
 int mgd_plot3d_detect(const char* path, int include_function, int* n_grids, int* file_type,
                       int* n_scalars, int* has_iblank, int* native_endian, int* sizes, int max_grids) {

@@ -46,6 +46,7 @@ def _lib():
             "mgd_control_write": [cp, C.c_int, C.c_int, dp],
             "mgd_control_read": [cp, C.c_int, C.c_int, dp],
             "mgd_control_zaxpy": [cp, C.c_double, cp, cp],
+            "mgd_plot3d_set_record_markers": [C.c_int],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -53,6 +54,11 @@ def _lib():
             f.restype = C.c_int
         L._plot3d_sigs = True
     return L
+
+
+def set_record_markers(nbytes: int):
+    """Record-marker width the writers use: 4 (default) or 8 bytes (USE_EXTENDED_PLOT3D)."""
+    check(_lib().mgd_plot3d_set_record_markers(nbytes))
 
 
 def detect(path: str, include_function: bool = True, max_grids: int = 64) -> Dict:

@@ -113,3 +113,50 @@ def test_bc_table():
     t = rows[-2]
     assert t.normal_dir == 0 and t.extent == (93, 109, 29, 173, 1, -1)
     assert rows[4].params[:2] == (0.2, 2.0)
+
+
+def test_eight_byte_record_markers(tmp_path):
+    """The reference's USE_EXTENDED_PLOT3D build frames records with 8-byte markers
+    (include/config.h.in:34-40, src/PLOT3DFormat.c:22-25): the reference's solution file
+    re-framed with 8-byte markers (numpy, little and big endian) reads the same; the
+    writers produce that framing after set_record_markers(8)."""
+    q, aux = plot3d.read_solution(QFILE, nv=4)
+    raw = open(QFILE, "rb").read()
+    # re-frame every record of the 4-byte file with 8-byte markers
+    recs, pos = [], 0
+    while pos < len(raw):
+        m = int(np.frombuffer(raw[pos:pos + 4], dtype="<i4")[0])
+        recs.append(raw[pos + 4:pos + 4 + m])
+        pos += 8 + m
+    for order in ("<", ">"):
+        out = bytearray()
+        for r in recs:
+            mk = np.array([len(r)], dtype=order + "i8").tobytes()
+            body = r
+            if order == ">":      # swap every 4- or 8-byte item of the record body
+                if len(r) in (4, 12):
+                    body = np.frombuffer(r, dtype="<i4").astype(">i4").tobytes()
+                else:
+                    body = np.frombuffer(r, dtype="<f8").astype(">f8").tobytes()
+            out += mk + body + mk
+        f = tmp_path / f"wide{'le' if order == '<' else 'be'}.q"
+        f.write_bytes(bytes(out))
+        info = plot3d.detect(str(f))
+        assert info["file_type"] == plot3d.SOLUTION and info["sizes"][0] == (4, 4, 1)
+        assert info["native_endian"] == (order == "<")
+        q2, aux2 = plot3d.read_solution(str(f), nv=4)
+        assert np.array_equal(q2, q) and np.array_equal(aux2, aux)
+    try:
+        plot3d.set_record_markers(8)
+        w = tmp_path / "written.q"
+        plot3d.write_solution(str(w), (4, 4, 1), q, timestep=int(aux[0]), time=aux[3])
+        head = np.frombuffer(w.read_bytes()[:16], dtype="<i8")
+        assert head[0] == 4
+        q3, _ = plot3d.read_solution(str(w), nv=4)
+        assert np.array_equal(q3, q)
+        if aux[1] == 0.0 and aux[2] == 0.0:
+            assert w.read_bytes() == (tmp_path / "widele.q").read_bytes()
+    finally:
+        plot3d.set_record_markers(4)
+    with pytest.raises(Exception):
+        plot3d.set_record_markers(6)
