@@ -295,14 +295,18 @@ def small_grid_leg(n=201, nsteps=96, save_interval=32, batch=20):
 
 def uneven_leg(nsteps=32, save_interval=8):
     """The mixed batch (1-D periodic waves of 2048 points, 2-D monopoles on odd grids
-    drawn from [201, 513] with seed 5, 513 x 257 mixing layers with 4 segments each as
-    instances): forward + adjoint of every member, all regions issued concurrently,
-    against the same regions run one after another."""
+    drawn from [201, 513] with seed 5, 513 x 257 mixing layers): forward + adjoint of
+    every member, all regions issued concurrently, against the same regions run one after
+    another.  Each mixing layer is a multiple-shooting member: msforward + msadjoint over 4
+    time segments of `nsteps` steps (the segments are the instances of its region), the
+    segments starting from the states of a baseline run at t = k nsteps dt with a seeded
+    momentum perturbation (the matching conditions unmet), quadratic penalty of weight 1."""
     from magudi_amd import cases
     from magudi_amd.group import RegionGroup
     cs = cases.uneven_batch(seed=5, nsteps=nsteps, save_interval=save_interval)
-    ninst = [4 if c.name.startswith("mixing") else 1 for c in cs]
-    G = RegionGroup(cs, n_instances=ninst)
+    shooting = {k: dict(nsplit=4, nts=nsteps, penalty_weight=1.0, save_interval=save_interval)
+                for k, c in enumerate(cs) if c.name.startswith("mixing")}
+    G = RegionGroup(cs, shooting=shooting)
     G.restart()
     G.run_adjoint(want_gradient=True)          # autotune and warm up
     G.restart()
@@ -314,8 +318,11 @@ def uneven_leg(nsteps=32, save_interval=8):
     G.restart()
     G.sync()
     t0 = time.perf_counter()
-    for s in G.solvers:
-        s.runAdjoint(want_gradient=True)
+    for k, s in enumerate(G.solvers):
+        if k in G.shooting:
+            G._shooting_run(k, None, True)
+        else:
+            s.runAdjoint(want_gradient=True)
     G.sync()
     t_seq = time.perf_counter() - t0
     # one launch per RK stage for the 18 fused 2-D members, the 1-D waves beside them
@@ -331,8 +338,10 @@ def uneven_leg(nsteps=32, save_interval=8):
     G.close()
     pts = G.grid_point_steps
     return {"workload": f"{len(cs)} regions: 8 x 1-D wave 2048, 16 x 2-D monopole 201..513 "
-                        f"(seed 5), 2 x mixing layer 513x257 x 4 segments; fwd+adjoint "
-                        f"{nsteps} steps, snapshots every {save_interval}",
+                        f"(seed 5), 2 x mixing layer 513x257 multiple shooting over 4 segments of "
+                        f"{nsteps} steps; fwd+adjoint {nsteps} steps per member or segment, "
+                        f"snapshots every {save_interval}",
+            "shooting_penalty": [float(G.shooting[k][0].J_penalty) for k in sorted(G.shooting)],
             "value": pts / min(t_group, t_merged), "unit": "grid-pt-steps/s",
             "concurrent_value": pts / t_group, "merged_value": pts / t_merged,
             "merged_members": merged, "sequential_value": pts / t_seq,

@@ -27,43 +27,80 @@ __all__ = ["RegionGroup"]
 
 
 class RegionGroup:
+    """``shooting`` maps a member index to the time_splitting options of a multiple-shooting
+    member (nsplit, nts, penalty_weight, and optionally penalty_type, huber_threshold,
+    save_interval, ics, t0s): that member runs msforward + msadjoint over its nsplit time
+    segments, the segments being the instances of its region (bin/msforward.f90:242-365,
+    bin/msadjoint.f90:287-384); without ics / t0s the segments start from
+    multishoot.segment_initial_guesses."""
+
     def __init__(self, cases: Sequence[Case], n_instances: Optional[Sequence[int]] = None,
-                 force_plain: bool = False, workers: int = 8):
+                 force_plain: bool = False, workers: int = 8, shooting: Optional[dict] = None):
+        from .multishoot import MultipleShooting, segment_initial_guesses
         n_instances = list(n_instances) if n_instances is not None else [1] * len(cases)
         self.cases = list(cases)
-        self.regions = [Region(c, n_instances=ni, force_plain=force_plain)
-                        for c, ni in zip(self.cases, n_instances)]
-        self.solvers = [Solver(r) for r in self.regions]
-        for r, c in zip(self.regions, self.cases):
-            for i in range(r.n_instances):
-                r.states[i].initialCondition = c.initial_state()
+        self.shooting = {}
+        self.shooting_results = {}      # member index -> msadjoint's dict of its last run
+        self.regions, self.solvers = [], []
+        for k, (c, ni) in enumerate(zip(self.cases, n_instances)):
+            opt = (shooting or {}).get(k)
+            if opt is None:
+                r = Region(c, n_instances=ni, force_plain=force_plain)
+                for i in range(r.n_instances):
+                    r.states[i].initialCondition = c.initial_state()
+                self.regions.append(r)
+                self.solvers.append(Solver(r))
+                continue
+            opt = dict(opt)
+            ics, t0s = opt.pop("ics", None), opt.pop("t0s", None)
+            if ics is None:
+                ics, t0s = segment_initial_guesses(c, opt["nsplit"], opt["nts"])
+            M = MultipleShooting(c, force_plain=force_plain, **opt)
+            self.shooting[k] = (M, ics, t0s)
+            self.regions.append(M.region)
+            self.solvers.append(Solver(M.region, dt=M.dt, nsteps=M.nts, save_interval=M.K))
         self.workers = max(1, workers)
 
     @property
     def grid_point_steps(self) -> int:
         return sum(r.npoints * r.n_instances * s.nTimesteps for r, s in zip(self.regions, self.solvers))
 
+    def _shooting_run(self, k, control, want_gradient):
+        """msforward + msadjoint of a multiple-shooting member: (total sensitivity of the
+        objective -- the sum of the segments' control inner products and IC-gradient inner
+        products --, control gradient, J of the objective)."""
+        M, ics, t0s = self.shooting[k]
+        J = M.msforward(ics, t0s, control=control)
+        out = M.msadjoint(want_control_gradient=want_gradient)
+        self.shooting_results[k] = out
+        return (np.array([out["total"]]), out["control_grad"], np.array([J]))
+
     def _map(self, fn):
         with cf.ThreadPoolExecutor(max_workers=min(self.workers, len(self.regions))) as ex:
             return list(ex.map(fn, range(len(self.regions))))
 
     def restart(self):
-        for r in self.regions:
-            r.restart()
+        for k, r in enumerate(self.regions):
+            if k not in self.shooting:   # msforward restarts its segments itself
+                r.restart()
 
     def run_adjoint(self, controls: Optional[List[Optional[np.ndarray]]] = None,
                     want_gradient: bool = True):
-        """Solver%runAdjoint of every region, concurrently; list of (sens, grad, J)."""
+        """Solver%runAdjoint of every region (msforward + msadjoint of the shooting members),
+        concurrently; list of (sens, grad, J)."""
         controls = controls or [None] * len(self.regions)
 
         def one(k):
+            if k in self.shooting:
+                return self._shooting_run(k, controls[k], want_gradient)
             return self.solvers[k].runAdjoint(control=controls[k], want_gradient=want_gradient)
         return self._map(one)
 
     def _merged_members(self) -> List[int]:
         """Members the one-launch path takes: fused 2-D regions (16 x 16 tiles fit) with
         the step count and snapshot interval of the first of them."""
-        idx = [k for k, r in enumerate(self.regions) if r.nd == 2 and r.tile_shape >= 0]
+        idx = [k for k, r in enumerate(self.regions)
+               if r.nd == 2 and r.tile_shape >= 0 and k not in self.shooting]
         if not idx:
             return []
         s0 = self.solvers[idx[0]]
@@ -129,6 +166,8 @@ class RegionGroup:
         results = [None] * len(self.regions)
 
         def one(k):
+            if k in self.shooting:
+                return self._shooting_run(k, controls[k], want_gradient)
             return self.solvers[k].runAdjoint(control=controls[k], want_gradient=want_gradient)
         # the merged launches first, then the other members concurrently: issued at the
         # same time, their many small launches would share the hardware queues with the

@@ -31,7 +31,34 @@ from .capi import check, dptr
 from .cases import Case
 from .solver import Region
 
-__all__ = ["MultipleShooting", "penalty_norm", "penalty_gradient_factor"]
+__all__ = ["MultipleShooting", "penalty_norm", "penalty_gradient_factor", "segment_initial_guesses"]
+
+
+def segment_initial_guesses(case: Case, nsplit: int, nts: int, perturbation: float = 1e-3,
+                            seed: int = 0, dt: Optional[float] = None):
+    """Initial guesses of the segments' initial conditions, as msforward takes them from the
+    <prefix>-<timestep>.q files of a baseline run: the states of one forward run from the
+    case's initial condition at t = k nts dt (on the GPU), each but the first offset by a
+    seeded random perturbation of the momenta (so that the matching conditions are not
+    met exactly); returns (ics, t0s)."""
+    from .solver import Solver
+    dt = dt if dt is not None else case.dt
+    R = Region(case)
+    S = Solver(R, dt=dt, nsteps=nts)
+    q = case.initial_state()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ics, t0s = [q], [0.0]
+    for k in range(1, nsplit):
+        R.states[0].conservedVariables = q
+        S.runForward(t0=(k - 1) * nts * dt)
+        q = R.states[0].conservedVariables
+        ic = q.copy()
+        for c in range(1, case.nd + 1):
+            ic[c] = ic[c] + perturbation * rng.standard_normal(case.npoints)
+        ics.append(ic)
+        t0s.append(k * nts * dt)
+    R.close()
+    return ics, t0s
 
 
 def penalty_norm(kind: str, L2sq: float, threshold: float) -> float:

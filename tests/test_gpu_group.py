@@ -87,3 +87,83 @@ def test_merged_group_equals_single_runs():
             assert (g is None and g1 is None) or np.array_equal(g, g1), (rep, c.name)
     assert [r.tile_shape for r in G.regions] == shapes
     G.close()
+
+
+def shooting_inputs(O, c, nsplit, nts, seed=11):
+    """Segment initial conditions of a mixing layer as msforward reads them: the states of
+    one oracle run at t = k nts dt, each but the first with a seeded momentum bump (so the
+    matching conditions are not met), and time offsets k nts dt."""
+    x, y = c.coords[0], c.coords[1]
+    rng = np.random.default_rng(seed)
+    q = c.initial_state()
+    ics, t0s = [q], [0.0]
+    for k in range(1, nsplit):
+        _, q = O.run_forward(q, c.dt, nts, t0=(k - 1) * nts * c.dt)
+        bump = np.exp(-((x - rng.uniform(5, 25)) ** 2 + (y - rng.uniform(-2, 2)) ** 2) / 4.0)
+        ic = q.copy()
+        ic[1] = ic[1] + 1e-3 * bump
+        ic[2] = ic[2] - 5e-4 * bump
+        ics.append(ic)
+        t0s.append(k * nts * c.dt)
+    return ics, t0s
+
+
+@pytest.mark.parametrize("merged", [False, True])
+def test_group_shooting_member_matches_oracle(merged):
+    """The mixing layer as a multiple-shooting member of an uneven group (4 segments with
+    their own initial conditions and time offsets, quadratic penalty, a control on the
+    segments): J, the segment costs and mismatches, the control and IC gradients against
+    the oracle's msforward / msadjoint (J 1e-12, gradients 1e-10); the plain members and
+    the shooting member itself bit for bit against their runs alone."""
+    import ms_oracle as ms
+    from magudi_amd.group import RegionGroup
+    from magudi_amd.multishoot import MultipleShooting
+    from magudi_amd.solver import Region, Solver
+    nsplit, nts, K, w = 4, 4, 2, 20.0
+    cs = [cases.periodic_wave_1d(n=256, nsteps=6, save_interval=3),
+          cases.acoustic_monopole(n=45, nsteps=6, save_interval=3),
+          cases.mixing_layer(nx=64, ny=45, nsteps=nts, save_interval=K)]
+    cs[1].dt = 0.2
+    c = cs[2]
+    O = ob.Region(c)
+    ics, t0s = shooting_inputs(O, c, nsplit, nts)
+    na = O.n_actuator
+    assert na > 0
+    ctl = 1e-2 * np.random.default_rng(5).standard_normal((nsplit * 4 * nts, na))
+    opt = dict(nsplit=nsplit, nts=nts, penalty_weight=w, save_interval=K, ics=ics, t0s=t0s)
+    G = RegionGroup(cs, shooting={2: opt})
+    assert G.regions[2].n_instances == nsplit
+    if merged:
+        assert 2 not in G._merged_members()
+    G.restart()
+    controls = [None, None, ctl]
+    out = G.run_adjoint_merged(controls=controls) if merged else G.run_adjoint(controls=controls)
+    Jo, cost_o, L2_o, _ = ms.ms_forward(O, ics, t0s, c.dt, nts, w, control=ctl)
+    res_o = ms.ms_adjoint(O, ics, t0s, c.dt, nts, K, w, control=ctl)
+    total, grad, J = out[2]
+    M = G.shooting[2][0]
+    assert abs(J[0] - Jo) <= 1e-12 * abs(Jo)
+    assert rel(M.segment_cost, cost_o) < 1e-12
+    assert min(L2_o[1:]) > 0 and rel(M.L2sq[1:], L2_o[1:]) < 1e-12
+    assert abs(total[0] - res_o["total"]) <= 1e-10 * abs(res_o["total"])
+    assert rel(grad, res_o["control_grad"]) < 1e-10
+    res = G.shooting_results[2]
+    for k in range(nsplit):
+        assert rel(res["ic_grad"][k], res_o["ic_grad"][k]) < 1e-10, k
+    # the shooting member alone, and the plain members alone: bit for bit
+    A = MultipleShooting(c, nsplit=nsplit, nts=nts, penalty_weight=w, save_interval=K)
+    assert A.msforward(ics, t0s, control=ctl) == J[0]
+    a = A.msadjoint()
+    assert a["total"] == total[0] and np.array_equal(a["control_grad"], grad)
+    for k in range(2):
+        R = Region(cs[k])
+        R.states[0].conservedVariables = cs[k].initial_state()
+        s1, g1, J1 = Solver(R).runAdjoint()
+        s, g, Jk = out[k]
+        assert np.array_equal(Jk, J1) and np.array_equal(s, s1), cs[k].name
+    G.close()
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))

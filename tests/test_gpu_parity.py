@@ -265,8 +265,8 @@ def test_functional_parity(solver_mod):
 
 @pytest.mark.parametrize("plain", [False, True])
 def test_run_forward_parity(solver_mod, plain):
-    c = cases.acoustic_monopole(n=41, nsteps=20)
-    c.dt = 0.2
+    c = cases.acoustic_monopole(n=41, nsteps=100)
+    c.dt = 0.2        # t = 20: the pulse crosses the annulus and reaches sponges and far field
     R = solver_mod.Region(c, force_plain=plain)
     O = ob.Region(c)
     q0 = c.initial_state()
@@ -280,8 +280,8 @@ def test_run_forward_parity(solver_mod, plain):
 
 @pytest.mark.parametrize("plain", [False, True])
 def test_run_adjoint_parity_and_fd(solver_mod, plain):
-    c = cases.acoustic_monopole(n=41, nsteps=24, save_interval=5)
-    c.dt = 0.2
+    c = cases.acoustic_monopole(n=41, nsteps=100, save_interval=7)
+    c.dt = 0.2        # t = 20 (ragged last window)
     R = solver_mod.Region(c, force_plain=plain)
     O = ob.Region(c)
     q0 = c.initial_state()
