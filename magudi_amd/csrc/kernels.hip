@@ -430,10 +430,193 @@ __global__ void __launch_bounds__(kBlock) k_adj_jac(Geo g, const double* __restr
   for (int l = 0; l < NV; ++l) ri[l * g.N + p] = acc[l];
 }
 
+// The viscous adjoint Jacobian products at one point of a rectilinear grid, one direction
+// at a time: AdjPoint holds what they share (the dependent and transport variables, the
+// stress tensor and heat flux from the gradient gv of (u_1..u_ND, T) -- gv[c * ND + d] =
+// d w_c / d x_d, as k_grad writes it -- and the direction-free scalars of the flux
+// Jacobians); adj_visc_dir adds direction d's part for t = T1_d = D-dagger_d lambda:
+//   r += (A_d - Av_d)^T t                       (src/RhsHelperImpl.f90:431-491)
+//   diff[j] += B_dj^T t (components 1..ND+1)    (src/RhsHelperImpl.f90:509-553)
+// Called for d = 0, 1, .. in order, the sums are formed in the order of the reference's
+// loops (rhs over d; diff[j] over i for each j).  The metric of a rectilinear grid in
+// direction d is h_d e_d: the Jacobian entries (computeJacobianOfInviscidFlux,
+// computeFirstPartialViscousJacobian, computeSecondPartialViscousJacobian,
+// src/CNSHelperImpl.f90:1098-1444, 2412-2756) are formed one column at a time with the
+// products by the metric's zero entries -- exact zeros -- left out, which changes no
+// value (at most the sign of an exact zero) and keeps no matrix in registers.  POWER:
+// the viscosity power law is not zero (the terms in it are exact zeros otherwise).
+template <int ND>
+struct AdjPoint {
+  Prim<ND> s;
+  double tau[ND * ND], hf[ND], mu, la, ka, phi2, jac, h[ND];
+  double t2_0, t2_n, t2_k[ND];
+};
+
+template <int ND, bool POWER>
+__device__ __forceinline__ void adj_point_init(const Geo& g, const double* v, const double* gv,
+                                               const int* ijk, AdjPoint<ND>& P) {
+  const double gm = g.gamma;
+  P.s = prim<ND>(v, gm);
+  transport(g, P.s.T, &P.mu, &P.la, &P.ka);
+  for (int i = 0; i < ND * ND; ++i) P.tau[i] = gv[i];
+  stress<ND>(P.tau, P.mu, P.la);
+  for (int j = 0; j < ND; ++j) P.hf[j] = -P.ka * gv[ND * ND + j];
+  double ke = 0.0;
+  for (int i = 0; i < ND; ++i) ke += P.s.u[i] * P.s.u[i];
+  P.phi2 = 0.5 * (gm - 1.0) * ke;
+  P.jac = jacobian<ND>(g, ijk);
+  for (int d = 0; d < ND; ++d) P.h[d] = metric<ND>(g, d, ijk);
+  if (POWER) {
+    const double pl = g.power_law, sv = P.s.sv, T = P.s.T;
+    P.t2_0 = pl * gm * sv / T * (P.phi2 / (gm - 1.0) - T / gm);
+    P.t2_n = pl * gm * sv / T;
+    for (int k = 0; k < ND; ++k) P.t2_k[k] = -pl * gm * sv / T * P.s.u[k];
+  }
+}
+
+// adj_point_init from the stress / heat-flux slots at the point (sv: xx xy xz yy yz zz,
+// then the heat flux x y z, as k_grad3d_periodic<.., true> writes them -- stress() and
+// -kappa grad T of the same gradient values, so the same tau and hf).
+template <bool POWER>
+__device__ __forceinline__ void adj_point_init_stress(const Geo& g, const double* v, const double* sv,
+                                                      const int* ijk, AdjPoint<3>& P) {
+  const double gm = g.gamma;
+  P.s = prim<3>(v, gm);
+  transport(g, P.s.T, &P.mu, &P.la, &P.ka);
+  P.tau[0] = sv[0]; P.tau[1] = sv[1]; P.tau[2] = sv[2];
+  P.tau[3] = sv[1]; P.tau[4] = sv[3]; P.tau[5] = sv[4];
+  P.tau[6] = sv[2]; P.tau[7] = sv[4]; P.tau[8] = sv[5];
+  P.hf[0] = sv[6]; P.hf[1] = sv[7]; P.hf[2] = sv[8];
+  double ke = 0.0;
+  for (int i = 0; i < 3; ++i) ke += P.s.u[i] * P.s.u[i];
+  P.phi2 = 0.5 * (gm - 1.0) * ke;
+  P.jac = jacobian<3>(g, ijk);
+  for (int d = 0; d < 3; ++d) P.h[d] = metric<3>(g, d, ijk);
+  if (POWER) {
+    const double pl = g.power_law, svol = P.s.sv, T = P.s.T;
+    P.t2_0 = pl * gm * svol / T * (P.phi2 / (gm - 1.0) - T / gm);
+    P.t2_n = pl * gm * svol / T;
+    for (int k = 0; k < 3; ++k) P.t2_k[k] = -pl * gm * svol / T * P.s.u[k];
+  }
+}
+
+template <int ND, bool POWER>
+__device__ __forceinline__ void adj_visc_dir(const AdjPoint<ND>& P, double gm, int d,
+                                             const double* t, double* r, double (*diff)[ND + 1]) {
+  constexpr int NV = ND + 2, NW = ND + 1;
+  const Prim<ND>& s = P.s;
+  const double h = P.h[d], T = s.T, sv = s.sv, phi2 = P.phi2;
+  const double U = h * s.u[d];
+  double cs[ND];
+  for (int i = 0; i < ND; ++i) cs[i] = h * P.tau[i * ND + d];
+  const double ch = h * P.hf[d];
+  double us = 0.0;
+  for (int i = 0; i < ND; ++i) us += s.u[i] * cs[i];
+  const double t1v = us - ch;
+  // (A_d - Av_d)^T t, column 0 (density)
+  {
+    double acc = 0.0;
+    for (int i = 0; i < ND; ++i) {
+      double a = i == d ? phi2 * h - U * s.u[i] : 0.0 - U * s.u[i];
+      if (POWER) a = a - P.t2_0 * cs[i];
+      acc += a * t[i + 1];
+    }
+    const double bn = POWER ? P.t2_0 * t1v - sv * us : 0.0 - sv * us;
+    acc += (U * ((gm - 2.0) / (gm - 1.0) * phi2 - T) - bn) * t[NV - 1];
+    r[0] = r[0] + acc;
+  }
+  // columns 1..ND (momentum)
+  for (int k = 0; k < ND; ++k) {
+    double acc = 0.0;
+    if (k == d) acc += h * t[0];
+    for (int i = 0; i < ND; ++i) {
+      double a;
+      if (i == k) a = k == d ? U - (gm - 2.0) * s.u[k] * h : U;
+      else if (k == d) a = s.u[i] * h;
+      else if (i == d) a = 0.0 - (gm - 1.0) * s.u[k] * h;
+      else a = 0.0;
+      if (POWER) a = a - P.t2_k[k] * cs[i];
+      else if (a == 0.0 && i != k && k != d && i != d) continue;
+      acc += a * t[i + 1];
+    }
+    const double ae = k == d ? (T + phi2 / (gm - 1.0)) * h - (gm - 1.0) * U * s.u[k]
+                             : 0.0 - (gm - 1.0) * U * s.u[k];
+    const double be = POWER ? P.t2_k[k] * t1v + sv * cs[k] : sv * cs[k];
+    acc += (ae - be) * t[NV - 1];
+    r[k + 1] = r[k + 1] + acc;
+  }
+  // column ND+1 (energy)
+  {
+    double acc = 0.0;
+    for (int i = 0; i < ND; ++i) {
+      double a = i == d ? (gm - 1.0) * h : 0.0;
+      if (POWER) a = a - P.t2_n * cs[i];
+      else if (i != d) continue;
+      acc += a * t[i + 1];
+    }
+    const double ae = gm * U;
+    acc += (POWER ? ae - P.t2_n * t1v : ae) * t[NV - 1];
+    r[NV - 1] = r[NV - 1] + acc;
+  }
+  // second-partial block: B_dj^T t for every j (m1 = h_d e_d, m2 = h_j e_j), times jac
+  const double mu = P.mu, la = P.la, ka = P.ka, jac = P.jac;
+  for (int j = 0; j < ND; ++j) {
+    const double hj = P.h[j];
+    if (j == d) {
+      const double t1s = h * h;
+      const double t2s = mu * (h * s.u[d]);
+      const double t3s = la * (h * s.u[d]);
+      for (int a = 0; a < ND; ++a) {
+        const double dg = a == d ? mu * t1s + (mu + la) * h * h : mu * t1s;
+        const double en = a == d ? mu * t1s * s.u[a] + h * t2s + h * t3s : mu * t1s * s.u[a];
+        double x = 0.0;
+        x += (jac * dg) * t[1 + a];
+        x += (jac * en) * t[1 + ND];
+        diff[j][a] = diff[j][a] + x;
+      }
+      diff[j][ND] = diff[j][ND] + (jac * (ka * t1s)) * t[1 + ND];
+    } else {
+      const double t2s = mu * (hj * s.u[j]);
+      const double t3s = la * (h * s.u[d]);
+      // column a = d: rows j and ND; column a = j: rows d and ND
+      {
+        double x = 0.0;
+        x += (jac * (mu * h * hj)) * t[1 + j];
+        x += (jac * (h * t2s)) * t[1 + ND];
+        diff[j][d] = diff[j][d] + x;
+      }
+      {
+        double x = 0.0;
+        x += (jac * (la * h * hj)) * t[1 + d];
+        x += (jac * (hj * t3s)) * t[1 + ND];
+        diff[j][j] = diff[j][j] + x;
+      }
+    }
+  }
+}
+
+// All directions at one point: r = sum_d (A_d - Av_d)^T t1_d and diff[j] = sum_i B_ij^T t1_i.
+template <int ND>
+__device__ __forceinline__ void adj_visc_point(const Geo& g, const double* v, const double* gv,
+                                               const int* ijk, const double (*t1)[ND + 2],
+                                               double* r, double (*diff)[ND + 1]) {
+  for (int l = 0; l < ND + 2; ++l) r[l] = 0.0;
+  for (int j = 0; j < ND; ++j)
+    for (int a = 0; a <= ND; ++a) diff[j][a] = 0.0;
+  AdjPoint<ND> P;
+  if (g.power_law != 0.0) {
+    adj_point_init<ND, true>(g, v, gv, ijk, P);
+    for (int d = 0; d < ND; ++d) adj_visc_dir<ND, true>(P, g.gamma, d, t1[d], r, diff);
+  } else {
+    adj_point_init<ND, false>(g, v, gv, ijk, P);
+    for (int d = 0; d < ND; ++d) adj_visc_dir<ND, false>(P, g.gamma, d, t1[d], r, diff);
+  }
+}
+
 // Viscous regions: k_adj_jac's rhs and the second-partial block at one point, T1 and q
 // read once.  The rhs is written, then components 1..nv-1 of every T1_j are overwritten
-// with sum_i B_ij^T T1_i (src/RhsHelperImpl.f90:509-553) -- the Jacobian part in the same
-// operations as k_adj_jac, so the same values.
+// with sum_i B_ij^T T1_i -- the Jacobian part in the same operations as k_adj_jac, so the
+// same values.
 template <int ND>
 __global__ void __launch_bounds__(kBlock) k_adj_jac_visc2(Geo g, const double* __restrict__ q,
                                                          const double* __restrict__ G,
@@ -444,53 +627,20 @@ __global__ void __launch_bounds__(kBlock) k_adj_jac_visc2(Geo g, const double* _
   const long inst = blockIdx.y;
   constexpr int NV = ND + 2, NW = ND + 1;
   const double* qi = q + inst * NV * g.N;
+  const double* Gi = G + inst * NW * ND * g.N;
   double* Ti = T1 + inst * ND * NV * g.N;
   double* ri = rhs + inst * NV * g.N;
-  double v[NV];
+  double v[NV], gv[NW * ND];
   load_q<ND>(qi, g.N, p, v);
-  const Prim<ND> s = prim<ND>(v, g.gamma);
+  for (int k = 0; k < NW * ND; ++k) gv[k] = Gi[k * g.N + p];
   int ijk[3] = {0, 0, 0};
   unflat<ND>(g, p, ijk);
   double t1[ND][NV];
   for (int d = 0; d < ND; ++d)
     for (int k = 0; k < NV; ++k) t1[d][k] = Ti[(d * NV + k) * g.N + p];
-  double tau[ND * ND], hf[ND], mu, la, ka;
-  visc_point<ND>(g, G + inst * (ND + 1) * ND * g.N, g.N, p, s, tau, hf, &mu, &la, &ka);
-  {
-    double acc[NV];
-    for (int l = 0; l < NV; ++l) acc[l] = 0.0;
-    for (int d = 0; d < ND; ++d) {
-      double m[ND], A[NV * NV], B[NV * NV];
-      for (int i = 0; i < ND; ++i) m[i] = i == d ? metric<ND>(g, d, ijk) : 0.0;
-      inviscid_jacobian<ND>(m, s, g.gamma, A);
-      first_partial_viscous<ND>(m, tau, hf, g.power_law, g.gamma, s, B);
-      for (int k = 0; k < NV * NV; ++k) A[k] = A[k] - B[k];
-      for (int i = 0; i < NV; ++i) {
-        double t = 0.0;
-        for (int k = 0; k < NV; ++k) t += A[i * NV + k] * t1[d][k];
-        acc[i] = acc[i] + t;
-      }
-    }
-    for (int l = 0; l < NV; ++l) ri[l * g.N + p] = acc[l];
-  }
-  const double jac = jacobian<ND>(g, ijk);
-  double diff[ND][NW];
-  for (int j = 0; j < ND; ++j)
-    for (int a = 0; a < NW; ++a) diff[j][a] = 0.0;
-  for (int j = 0; j < ND; ++j) {
-    double m2[ND];
-    for (int i = 0; i < ND; ++i) m2[i] = i == j ? metric<ND>(g, j, ijk) : 0.0;
-    for (int i = 0; i < ND; ++i) {
-      double m1[ND], B[NW * NW];
-      for (int k = 0; k < ND; ++k) m1[k] = k == i ? metric<ND>(g, i, ijk) : 0.0;
-      second_partial_viscous<ND>(s.u, mu, la, ka, jac, m1, m2, B);
-      for (int a = 0; a < NW; ++a) {
-        double t = 0.0;
-        for (int k = 0; k < NW; ++k) t += B[a * NW + k] * t1[i][1 + k];
-        diff[j][a] = diff[j][a] + t;
-      }
-    }
-  }
+  double r[NV], diff[ND][NW];
+  adj_visc_point<ND>(g, v, gv, ijk, t1, r, diff);
+  for (int l = 0; l < NV; ++l) ri[l * g.N + p] = r[l];
   for (int j = 0; j < ND; ++j)
     for (int a = 0; a < NW; ++a) Ti[(j * NV + 1 + a) * g.N + p] = diff[j][a];
 }
@@ -1746,6 +1896,375 @@ __global__ void __launch_bounds__(kBlock) k_adj_finish_z(Geo g, const double* __
   }
 }
 
+// XCD-aware (tile, z chunk) work unit of block b of a 3-D march: units are dealt to the
+// 8 XCDs round-robin by block index, so block b takes unit (b mod 8) * ceil(units / 8) +
+// b / 8, tiles x-fastest: each XCD marches one contiguous band, and the halo lines of
+// neighbouring tiles are fetched into its own L2 by blocks running beside it.  False for
+// the padding blocks of a grid rounded up to a multiple of 8.
+template <int BX, int BY, int ZC>
+__device__ __forceinline__ bool march_unit(const Geo& g, int& x0, int& y0, int& z0) {
+  const int ntx = g.n[0] / BX, ntiles = ntx * (g.n[1] / BY);
+  const int nunits = ntiles * ((g.n[2] + ZC - 1) / ZC);
+  const int per_xcd = (nunits + 7) / 8;
+  const int unit = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (unit >= nunits) return false;
+  const int tile = unit % ntiles;
+  x0 = (tile % ntx) * BX;
+  y0 = (tile / ntx) * BY;
+  z0 = (unit / ntiles) * ZC;
+  return true;
+}
+
+// The x / y halo point of thread t of a BX x BY tile (8 (BX + BY) threads, one halo
+// point each): [0, 8 BY) the x-halo columns, [8 BY, NT) the y-halo rows.  (hx, hy) is its
+// place in the haloed tile, (hi, hj) its grid indices; hmask 1 for an x-halo point, 2 for
+// a y-halo point.
+template <int BX, int BY>
+__device__ __forceinline__ void march_halo(const Geo& g, int t, int x0, int y0, int& hx, int& hy,
+                                           int& hi, int& hj, int& hmask) {
+  if (t < 8 * BY) {
+    const int c = t % 8;
+    hy = t / 8;
+    hx = c < 4 ? c : BX + c;
+    hi = wrap(x0 + hx - 4, g.dir[0]);
+    hj = y0 + hy;
+    hmask = 1;
+  } else {
+    const int t2 = t - 8 * BY, r = t2 / BX;
+    hx = t2 % BX;
+    hy = r < 4 ? r : BY + r;
+    hi = x0 + hx;
+    hj = wrap(y0 + hy - 4, g.dir[1]);
+    hmask = 2;
+  }
+}
+
+// Element access at a 32-bit byte offset from a wave-uniform base pointer: the address
+// form of a global load / store with a scalar base and a 32-bit vector offset, so a
+// thread keeps one 32-bit offset per point instead of a 64-bit address per component.
+// Callers keep byte offsets below 2^32 (fewer than 2^29 points per component array).
+__device__ __forceinline__ double ld8(const double* base, unsigned off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + off);
+}
+__device__ __forceinline__ void st8(double* base, unsigned off, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
+// The first half of the adjoint rhs of a viscous 3-D region periodic in every direction,
+// in one z march: a BX x BY column tile marches along z over a chunk of ZC planes, each
+// thread keeping lambda of its column for the 9 planes of the z stencil in registers; the
+// plane being worked on is staged in LDS with its 4-point x / y halo (one halo point per
+// thread), so T1_d = D-dagger_d lambda of all three directions is formed at the point
+// without going through HBM, one direction at a time, each used as soon as it is formed.
+// With q and the stress / heat-flux slots S at the point (k_grad3d_periodic<.., true>),
+// adj_visc_dir gives the Jacobian part of the rhs -- written to rhs -- and the
+// second-partial block sum_i B_ij^T T1_i, written to components 1..4 of T1_j (where
+// k_adj3d_finish_march reads them).  The derivative sums are those of k_deriv_line /
+// k_deriv_row and the point operations those of k_adj_jac_visc2, so the values are the
+// same.  The halo point of the next plane and the lambda entering the window are loaded
+// one plane ahead.  The window of the last NZL components lives in an LDS ring of 9
+// planes (each thread reads back only its own column), the others in registers.
+template <int BX, int BY, int ZC, bool POWER>
+__global__ void __launch_bounds__(BX * BY, 2) k_adj3d_jac_march(Geo g, const double* __restrict__ q,
+                                                              const double* __restrict__ lam,
+                                                              const double* __restrict__ S,
+                                                              double* __restrict__ rhs,
+                                                              double* __restrict__ T1) {
+  constexpr int NV = 5, NW = 4, EXW = BX + 8, NT = BX * BY, NZL = 2, NZR = NV - NZL;
+  static_assert(8 * (BX + BY) == NT, "one halo point per thread");
+  __shared__ double slx[NV][BY][EXW];
+  __shared__ double sly[NV][BY + 8][BX];
+  __shared__ double zring[9][NZL][NT];
+  int x0, y0, z0;
+  if (!march_unit<BX, BY, ZC>(g, x0, y0, z0)) return;
+  const int z1 = min(z0 + ZC, g.n[2]);
+  const long inst = blockIdx.z;
+  const double* qi = q + inst * NV * g.N;
+  const double* li = lam + inst * NV * g.N;
+  const double* Si = S + inst * 12 * g.N;
+  double* ri = rhs + inst * NV * g.N;
+  double* Ti = T1 + inst * 3 * NV * g.N;
+  const int t = threadIdx.x, tx = t % BX, ty = t / BX;
+  const int i = x0 + tx, j = y0 + ty;
+  int hx, hy, hi, hj, hmask;
+  march_halo<BX, BY>(g, t, x0, y0, hx, hy, hi, hj, hmask);
+  // byte offsets of the thread's column and halo column, and of one plane
+  const unsigned n0 = (unsigned)g.n[0], plane8 = 8u * n0 * (unsigned)g.n[1];
+  const unsigned col8 = 8u * ((unsigned)i + n0 * (unsigned)j);
+  const unsigned hcol8 = 8u * ((unsigned)hi + n0 * (unsigned)hj);
+  auto load_l = [&](unsigned off, double* v) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) v[l] = ld8(li + l * g.N, off);
+  };
+  const Op& op = g.ops[SLOT_ADJ];
+  const double c1 = op.c[kMaxHW + 1], c2 = op.c[kMaxHW + 2], c3 = op.c[kMaxHW + 3], c4 = op.c[kMaxHW + 4];
+  const Dir dz = g.dir[2];
+  // window: components [0, NZR) in W, [NZR, NV) in ring slot (plane - (z0 - 4)) mod 9
+  double W[9][NZR];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    double v[NV];
+    load_l(col8 + plane8 * (unsigned)wrap(z0 - 4 + k, dz), v);
+#pragma unroll
+    for (int l = 0; l < NZR; ++l) W[k][l] = v[l];
+#pragma unroll
+    for (int l = NZR; l < NV; ++l) zring[k][l - NZR][t] = v[l];
+  }
+  double nl[NV], nh[NV];
+  load_l(col8 + plane8 * (unsigned)wrap(z0 + 4, dz), nl);
+  load_l(hcol8 + plane8 * (unsigned)z0, nh);
+  int rb = 0;                            // ring slot of plane zo - 4
+  for (int zo = z0; zo < z1; ++zo) {     // uniform over the block
+    double hv[NV], wc[NV];
+    const int r8 = rb + 8 < 9 ? rb + 8 : rb - 1;
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      if (l < NZR) W[8][l] = nl[l];
+      else zring[r8][l - NZR][t] = nl[l];
+      hv[l] = nh[l];
+    }
+    const int r4 = rb + 4 < 9 ? rb + 4 : rb - 5;
+#pragma unroll
+    for (int l = 0; l < NV; ++l) wc[l] = l < NZR ? W[4][l] : zring[r4][l - NZR][t];
+    if (zo + 1 < z1) {
+      load_l(col8 + plane8 * (unsigned)wrap(zo + 5, dz), nl);
+      load_l(hcol8 + plane8 * (unsigned)(zo + 1), nh);
+    }
+    const unsigned p8 = col8 + plane8 * (unsigned)zo;
+    double v[NV], sv[9];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) v[l] = ld8(qi + l * g.N, p8);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sv[k] = ld8(Si + k * g.N, p8);
+    __syncthreads();            // readers of the previous plane are done
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      slx[l][ty][tx + 4] = wc[l];
+      sly[l][ty + 4][tx] = wc[l];
+    }
+    if (hmask == 1) {
+#pragma unroll
+      for (int l = 0; l < NV; ++l) slx[l][hy][hx] = hv[l];
+    } else {
+#pragma unroll
+      for (int l = 0; l < NV; ++l) sly[l][hy][hx] = hv[l];
+    }
+    __syncthreads();
+    const int ijk[3] = {i, j, zo};
+    AdjPoint<3> P;
+    adj_point_init_stress<POWER>(g, v, sv, ijk, P);
+    double r[NV], diff[3][NW];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) r[l] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int a = 0; a < NW; ++a) diff[d][a] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double t1[NV];
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        double o = 0.0;
+        if (d == 0) {
+          const double* rx = &slx[l][ty][tx + 4];
+          o += c1 * (rx[1] - rx[-1]);
+          o += c2 * (rx[2] - rx[-2]);
+          o += c3 * (rx[3] - rx[-3]);
+          o += c4 * (rx[4] - rx[-4]);
+        } else if (d == 1) {
+          const double* cy = &sly[l][ty + 4][tx];
+          o += c1 * (cy[BX] - cy[-BX]);
+          o += c2 * (cy[2 * BX] - cy[-2 * BX]);
+          o += c3 * (cy[3 * BX] - cy[-3 * BX]);
+          o += c4 * (cy[4 * BX] - cy[-4 * BX]);
+        } else if (l < NZR) {
+          o += c1 * (W[5][l] - W[3][l]);
+          o += c2 * (W[6][l] - W[2][l]);
+          o += c3 * (W[7][l] - W[1][l]);
+          o += c4 * (W[8][l] - W[0][l]);
+        } else {
+          auto zw = [&](int k) {
+            const int sl = rb + k < 9 ? rb + k : rb + k - 9;
+            return zring[sl][l - NZR][t];
+          };
+          o += c1 * (zw(5) - zw(3));
+          o += c2 * (zw(6) - zw(2));
+          o += c3 * (zw(7) - zw(1));
+          o += c4 * (zw(8) - zw(0));
+        }
+        t1[l] = o;
+      }
+      adj_visc_dir<3, POWER>(P, g.gamma, d, t1, r, diff);
+    }
+#pragma unroll
+    for (int l = 0; l < NV; ++l) st8(ri + l * g.N, p8, r[l]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int a = 0; a < NW; ++a) st8(Ti + (d * NV + 1 + a) * g.N, p8, diff[d][a]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int l = 0; l < NZR; ++l) W[k][l] = W[k + 1][l];
+    rb = rb + 1 < 9 ? rb + 1 : 0;
+  }
+}
+
+// The second half, after k_adj3d_jac_march: t2 = (D-dagger_x T1_x + D-dagger_y T1_y) +
+// D-dagger_z T1_z over components 1..4 (the x / y operands of the plane staged in LDS with
+// their halo, one halo point per thread, the z operand in a 9-plane register window),
+// then, on the Jacobian part read back from rhs, k_adj_finish_z's operations: the chain
+// rule, the Jacobian scaling and the Kolmogorov term, and with rk_stage > 0 the RK4
+// adjoint update in place (y = lambda; only the point's own lambda is read), rhs not
+// written.  The sums and their order are those of k_deriv_row, k_deriv_line (x store, y
+// add) and k_adj_finish_z, so the values are the same.
+template <int BX, int BY, int ZC>
+__global__ void __launch_bounds__(BX * BY, 2) k_adj3d_finish_march(
+    Geo g, const double* __restrict__ T1, const double* __restrict__ q, const double* lam,
+    const double* __restrict__ kstr, double* __restrict__ rhs, int rk_stage, double dt, double* y,
+    double* b1, double* b2) {
+  constexpr int NV = 5, NW = 4, EXW = BX + 8;
+  static_assert(8 * (BX + BY) == BX * BY, "one halo point per thread");
+  __shared__ double snx[NW][BY][EXW];
+  __shared__ double sny[NW][BY + 8][BX];
+  int x0, y0, z0;
+  if (!march_unit<BX, BY, ZC>(g, x0, y0, z0)) return;
+  const int z1 = min(z0 + ZC, g.n[2]);
+  const long inst = blockIdx.z;
+  const long io = inst * NV * g.N;
+  const double* qi = q + io;
+  const double* li = lam + io;
+  const double* Tx = T1 + inst * 3 * NV * g.N + g.N;            // T1_x components 1..4
+  const double* Ty = Tx + (long)NV * g.N;
+  const double* Tz = Ty + (long)NV * g.N;
+  double* ri = rhs + io;
+  const int t = threadIdx.x, tx = t % BX, ty = t / BX;
+  const int i = x0 + tx, j = y0 + ty;
+  int hx, hy, hi, hj, hmask;
+  march_halo<BX, BY>(g, t, x0, y0, hx, hy, hi, hj, hmask);
+  const unsigned n0 = (unsigned)g.n[0], plane8 = 8u * n0 * (unsigned)g.n[1];
+  const unsigned col8 = 8u * ((unsigned)i + n0 * (unsigned)j);
+  const unsigned hcol8 = 8u * ((unsigned)hi + n0 * (unsigned)hj);
+  const double* Th = hmask == 1 ? Tx : Ty;
+  auto load4 = [&](const double* src, unsigned off, double* v) {
+#pragma unroll
+    for (int a = 0; a < NW; ++a) v[a] = ld8(src + a * g.N, off);
+  };
+  const Op& op = g.ops[SLOT_ADJ];
+  const double c1 = op.c[kMaxHW + 1], c2 = op.c[kMaxHW + 2], c3 = op.c[kMaxHW + 3], c4 = op.c[kMaxHW + 4];
+  const Dir dz = g.dir[2];
+  double Z[9][NW];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) load4(Tz, col8 + plane8 * (unsigned)wrap(z0 - 4 + k, dz), Z[k]);
+  double nz[NW], nx[NW], ny[NW], nh[NW];
+  load4(Tz, col8 + plane8 * (unsigned)wrap(z0 + 4, dz), nz);
+  load4(Tx, col8 + plane8 * (unsigned)z0, nx);
+  load4(Ty, col8 + plane8 * (unsigned)z0, ny);
+  load4(Th, hcol8 + plane8 * (unsigned)z0, nh);
+  for (int zo = z0; zo < z1; ++zo) {     // uniform over the block
+    double xv[NW], yv[NW], hv[NW];
+#pragma unroll
+    for (int a = 0; a < NW; ++a) {
+      Z[8][a] = nz[a];
+      xv[a] = nx[a];
+      yv[a] = ny[a];
+      hv[a] = nh[a];
+    }
+    if (zo + 1 < z1) {
+      const unsigned n8 = col8 + plane8 * (unsigned)(zo + 1);
+      load4(Tz, col8 + plane8 * (unsigned)wrap(zo + 5, dz), nz);
+      load4(Tx, n8, nx);
+      load4(Ty, n8, ny);
+      load4(Th, hcol8 + plane8 * (unsigned)(zo + 1), nh);
+    }
+    const unsigned p8 = col8 + plane8 * (unsigned)zo;
+    double v[NV], r[NV];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      v[l] = ld8(qi + l * g.N, p8);
+      r[l] = ld8(ri + l * g.N, p8);
+    }
+    __syncthreads();            // readers of the previous plane are done
+#pragma unroll
+    for (int a = 0; a < NW; ++a) {
+      snx[a][ty][tx + 4] = xv[a];
+      sny[a][ty + 4][tx] = yv[a];
+    }
+    if (hmask == 1) {
+#pragma unroll
+      for (int a = 0; a < NW; ++a) snx[a][hy][hx] = hv[a];
+    } else {
+#pragma unroll
+      for (int a = 0; a < NW; ++a) sny[a][hy][hx] = hv[a];
+    }
+    __syncthreads();
+    double t2[NW];
+#pragma unroll
+    for (int a = 0; a < NW; ++a) {
+      const double* rx = &snx[a][ty][tx + 4];
+      const double* cy = &sny[a][ty + 4][tx];
+      double ox = 0.0, oy = 0.0, oz = 0.0;
+      ox += c1 * (rx[1] - rx[-1]);
+      ox += c2 * (rx[2] - rx[-2]);
+      ox += c3 * (rx[3] - rx[-3]);
+      ox += c4 * (rx[4] - rx[-4]);
+      oy += c1 * (cy[BX] - cy[-BX]);
+      oy += c2 * (cy[2 * BX] - cy[-2 * BX]);
+      oy += c3 * (cy[3 * BX] - cy[-3 * BX]);
+      oy += c4 * (cy[4 * BX] - cy[-4 * BX]);
+      oz += c1 * (Z[5][a] - Z[3][a]);
+      oz += c2 * (Z[6][a] - Z[2][a]);
+      oz += c3 * (Z[7][a] - Z[1][a]);
+      oz += c4 * (Z[8][a] - Z[0][a]);
+      t2[a] = (ox + oy) + oz;
+    }
+    adj_chain_t2<3>(g, v, t2, -1.0, r);
+    const int ijk[3] = {i, j, zo};
+    const double jac = jacobian<3>(g, ijk);
+#pragma unroll
+    for (int l = 0; l < NV; ++l) r[l] = r[l] * jac;
+    if (kstr) r[0] = r[0] - ld8(li + g.N, p8) * ld8(kstr, p8);
+    if (rk_stage == 0) {
+#pragma unroll
+      for (int l = 0; l < NV; ++l) st8(ri + l * g.N, p8, r[l]);
+    } else {
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        const long c = io + l * g.N;
+        double* yc = y + c;
+        double* b1c = b1 + c;
+        double* b2c = b2 + c;
+        const double rl = r[l];
+        switch (rk_stage) {
+          case 4: {
+            const double l0 = ld8(yc, p8);
+            st8(b1c, p8, l0);
+            st8(b2c, p8, l0 - div6(dt * rl));
+            st8(yc, p8, l0 - dt * rl / 2.0);
+            break;
+          }
+          case 3:
+            st8(b2c, p8, ld8(b2c, p8) - div3(dt * rl));
+            st8(yc, p8, ld8(b1c, p8) - dt * rl / 2.0);
+            break;
+          case 2:
+            st8(b2c, p8, ld8(b2c, p8) - div3(dt * rl));
+            st8(yc, p8, ld8(b1c, p8) - dt * rl);
+            break;
+          default:
+            st8(yc, p8, ld8(b2c, p8) - div6(dt * rl));
+            break;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int a = 0; a < NW; ++a) Z[k][a] = Z[k + 1][a];
+  }
+}
+
 // The same operator along a periodic x direction: a block stages `segs` (1 or
 // kRowSegs) runs of 256 consecutive points of one component -- segments of one row, or
 // whole rows when rows are shorter -- with the 4-point wrap-around halo of each row
@@ -1888,6 +2407,14 @@ bool fwd3d_march_path(const Geo& g, const FinishInfo& fin, int nsrc) {
   return fin.simple && !g.dissipation && nsrc == 0 && fwd3d_march_applies(g);
 }
 
+// The two-march adjoint path: viscous, periodic in every direction with the march tile
+// sizes, no dissipation, no far field and at most one whole-domain Kolmogorov forcing (the
+// chain rule and the forcing then sit in the finishing march).
+static bool adj3d_march_path(const Geo& g, const FinishInfo& fin) {
+  return g.viscous && !g.dissipation && !fin.has_farfield && fin.simple &&
+         fwd3d_march_applies(g) && g.N < (1L << 29);   // 32-bit byte offsets
+}
+
 void launch_stress3d(const Geo& g, const double* q, double* S, hipStream_t st) {
   const dim3 gg = grad3d_grid(g);
   if (g.power_law > 0.0)
@@ -1975,6 +2502,23 @@ bool launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
                         const FinishInfo& fin, hipStream_t st, const RkStage* rk) {
   const dim3 grid = grid_pts(g);
   const int has_farfield = fin.has_farfield;
+  if (adj3d_march_path(g, fin)) {
+    // stress / heat-flux pass (9 slots of G), then the two marches; the second finishes
+    // the rhs (and the stage)
+    (void)w;
+    (void)grid;
+    launch_stress3d(g, q, G, st);
+    const int units = (g.n[0] / kFMX) * (g.n[1] / kFMY) * ((g.n[2] + kFMZ - 1) / kFMZ);
+    const dim3 gm(8 * ((units + 7) / 8), 1, g.ninst);
+    if (g.power_law != 0.0)
+      k_adj3d_jac_march<kFMX, kFMY, kFMZ, true><<<gm, kFMX * kFMY, 0, st>>>(g, q, lam, G, rhs, T1);
+    else
+      k_adj3d_jac_march<kFMX, kFMY, kFMZ, false><<<gm, kFMX * kFMY, 0, st>>>(g, q, lam, G, rhs, T1);
+    k_adj3d_finish_march<kFMX, kFMY, kFMZ><<<gm, kFMX * kFMY, 0, st>>>(
+        g, T1, q, lam, fin.kolmogorov, rhs, rk ? rk->stage : 0, rk ? rk->dt : 0.0,
+        rk ? rk->y : nullptr, rk ? rk->b1 : nullptr, rk ? rk->b2 : nullptr);
+    return rk != nullptr;
+  }
   bool finished = false;
   MGD_DISPATCH(g.nd, {
     constexpr int NV = ND + 2;
