@@ -171,10 +171,13 @@ def viscous3d_leg(n):
     R.states[0].conservedVariables = c.initial_state()
     f_ms, f_b = R.time_kernel(capi.KERNEL_RHS_FORWARD, reps=5)
     a_ms, a_b = R.time_kernel(capi.KERNEL_RHS_ADJOINT, reps=5)
+    c_ms, c_b = R.time_kernel(capi.KERNEL_COPY, reps=5)   # counter calibration
     del R
     leg = {"workload": f"Kolmogorov {n}^3 viscous periodic rhs (Re_ac 500, Pr 0.7)",
-           "path": "3-D kernels (forward: stress pass + z march; adjoint: periodic gradient, "
-                   "line / row derivatives, Jacobian products, fused finishing pass)"}
+           "path": "3-D kernels (forward: stress pass + z march; adjoint: stress pass, "
+                   "Jacobian march, finishing march)",
+           "state_copy": {"ms": round(c_ms, 4), "bytes": c_b,
+                          "achieved_GBs": round(c_b / (c_ms * 1e-3) / 1e9, 1)}}
     for k, ms, b in (("forward_rhs", f_ms, f_b), ("adjoint_rhs", a_ms, a_b)):
         gbs = b / (ms * 1e-3) / 1e9
         leg[k] = {"ms": round(ms, 4), "algorithmic_bytes": b, "achieved_GBs": round(gbs, 1),
@@ -207,20 +210,25 @@ class KolmogorovRun:
         return J
 
     def timed(self, steps):
-        """(seconds per pass, J, forward seconds of the last pass)"""
+        """(seconds per pass, J, forward seconds of a pass, z-march launches in the timed
+        passes, those of them that also wrote a window copy, their summed ms -- HIP events
+        around every launch on the region stream)"""
         self.R.sync()
+        self.R.kernel_timer(True)
         t0 = time.perf_counter()
         for _ in range(steps):
             J = self.one()
         self.R.sync()
         t = (time.perf_counter() - t0) / steps
+        launches, with_copy, ms = self.R.kernel_timer_read()
+        self.R.kernel_timer(False)
         # forward half alone, for the split
         self.R.restart()
         self.R.sync()
         t1 = time.perf_counter()
         self.S.runForwardCheckpointed()
         self.R.sync()
-        return t, float(J[0]), time.perf_counter() - t1
+        return t, float(J[0]), time.perf_counter() - t1, launches, with_copy, ms
 
 
 def fd_sweep_leg(n=401, nsteps=960, save_interval=32, batch=20):
@@ -513,19 +521,25 @@ def main():
     K = KolmogorovRun(a.c4_n, a.c4_nsteps)
     for _ in range(max(1, a.warmup)):
         K.one()
-    t_step, J, t_fwd = K.timed(a.steps)
+    t_step, J, t_fwd, launches, with_copy, march_ms = K.timed(a.steps)
     value = K.pts / t_step
     print(f"headline: {t_step:.3f} s per pass ({value:.4g} grid-pt-steps/s)", file=sys.stderr,
           flush=True)
-    # dominant kernel (by launches x time: 8 nsteps forward launches per pass, counting the
-    # window recompute, against 4 nsteps launches of the largest adjoint kernel)
+    # dominant kernel (by launches x time: 4 nsteps forward launches per pass plus those of
+    # the window recompute, against 4 nsteps launches of each adjoint kernel), timed with
+    # HIP events around each of its launches in the timed passes.  Its algorithmic bytes
+    # per launch: the stage state and the 9 stress / heat-flux slots in, the RK buffers in
+    # and out (0, 2, 2, 1 states in; 3, 2, 2, 1 out, over the 4 stages) and the new stage
+    # state out -- 242 B per point -- plus 40 B per point for the launches that also write
+    # the state into a window.
     K.R.restart()
-    ms, b = K.R.time_kernel(capi.KERNEL_FWD_MARCH3D, reps=3)
     cms, cb = K.R.time_kernel(capi.KERNEL_COPY, reps=3)
     f_ms, f_b = K.R.time_kernel(capi.KERNEL_RHS_FORWARD, reps=3)
     a_ms, a_b = K.R.time_kernel(capi.KERNEL_RHS_ADJOINT, reps=3)
     M_bytes = 8.0 * 5 * K.case.npoints
     del K
+    ms = march_ms / max(launches, 1)
+    b = (242.0 + 40.0 * with_copy / max(launches, 1)) * M_bytes / 40.0
     achieved = b / (ms * 1e-3) / 1e9
     traffic = None
     if not a.no_traffic:
@@ -533,7 +547,8 @@ def main():
         traffic = tr["bytes"] if tr else None
     roofline = {"bound": "hbm", "kernel": "k_fwd3d_march", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "avg_launch_us": round(ms * 1e3, 2), "bytes_per_launch": b}
+                "traffic": traffic, "avg_launch_us": round(ms * 1e3, 2), "bytes_per_launch": b,
+                "launches_timed": launches, "launches_with_window_copy": with_copy}
     n = a.c4_n
     out = {
         "metric": METRIC, "value": value, "unit": "grid-pt-steps/s", "n_gpus": 1,
@@ -547,7 +562,8 @@ def main():
                                f"kinetic energy and dJ/dq(T) on the device",
                    "grid": [n, n, n], "nsteps": a.c4_nsteps,
                    "save_interval": max(1, a.c4_nsteps // 8), "instances": 1,
-                   "path": "3-D kernels: stress pass + z-march forward; adjoint per-operator"},
+                   "path": "3-D kernels: stress pass + z-march forward; adjoint stress pass + "
+                           "Jacobian march + finishing march"},
         "roofline": roofline,
         "J": J, "forward_s": round(t_fwd, 3), "adjoint_s": round(t_step - t_fwd, 3),
         "rhs3d": {"forward_ms": round(f_ms, 4), "adjoint_ms": round(a_ms, 4),
@@ -557,7 +573,9 @@ def main():
                        "achieved_GBs": round(cb / (cms * 1e-3) / 1e9, 1)},
     }
     if traffic is not None:
-        out["roofline"]["traffic_over_algorithmic"] = round(traffic / b, 3)
+        # the counter passes run the kernel's launches without the window copy (242 B / pt)
+        out["roofline"]["traffic_over_algorithmic"] = round(traffic / (242.0 * M_bytes / 40.0), 3)
+        out["roofline"]["traffic_workload"] = "launches without the window copy (--kernels-only)"
     if not a.no_extras:
         out["monopole401"] = monopole_legs(a, extras=True)
         out["fd_sweep"] = fd_sweep_leg(a.n, a.nsteps, a.save_interval, a.batch)

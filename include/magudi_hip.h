@@ -303,10 +303,17 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches);
  * forward run in device memory ((4 nsteps + 1) states, e.g. 19.8 GB for 401^2 x 960
  * steps) so that the adjoint pass reads them instead of recomputing each window from
  * its snapshot; results are identical.  Default 0: snapshots every save_interval steps
- * and window recompute (UniformCheckpointer, src/UniformCheckpointerImpl.f90:78-208). */
+ * and window recompute (UniformCheckpointer, src/UniformCheckpointerImpl.f90:78-208).
+ * MGD_OPT_WINDOW_CACHE (generic-kernel path, e.g. 3-D regions): the checkpointed forward
+ * pass also keeps the stage states of its last n windows (4 save_interval states each) and
+ * the adjoint passes read them instead of recomputing those windows; identical results.
+ * -1 (default): as many windows as fit in the device memory left free at the run's
+ * preparation less a reserve of max(8 GiB, a tenth of the device), at most 8; 0: none;
+ * n > 0: at most n. */
 #define MGD_OPT_FORCE_PLAIN 1
 #define MGD_OPT_TILE_SHAPE 2
 #define MGD_OPT_RESIDENT_STATES 3
+#define MGD_OPT_WINDOW_CACHE 4
 int mgd_region_set_option(mgd_region* r, int option, int value);
 
 /* Tile shape the fused 2-D kernels use now (-1: the region runs the generic kernels). */
@@ -331,6 +338,15 @@ int mgd_region_get_tile_shape(mgd_region* r, int* shape);
 #define MGD_KERNEL_RHS_ADJOINT 4
 #define MGD_KERNEL_FWD_MARCH3D 5
 int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, double* bytes);
+
+/* Per-launch timing of the 3-D z-march forward kernel inside the runs: with on = 1 every
+ * later launch of it by the region is bracketed by two HIP events on the region stream
+ * (the record restarts); on = 0 stops recording.  No counterpart in the reference: the
+ * measurement of the kernel inside a timed run (which = MGD_KERNEL_FWD_MARCH3D only). */
+int mgd_kernel_timer(mgd_region* r, int which, int on);
+/* Launches recorded, how many of them also wrote a copy of the stage state (the window
+ * recompute and cached windows), and their summed duration in ms (waits for the last). */
+int mgd_kernel_timer_read(mgd_region* r, long* launches, long* with_copy, double* ms);
 
 /* Region groups: regions of different sizes run in lock step with one kernel launch per
  * RK stage for all of them (the uneven batches of the multiple-shooting and ensemble

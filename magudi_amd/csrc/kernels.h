@@ -32,6 +32,9 @@ struct RkStage {
   double* b2;
   double* copy;
   double* yout = nullptr;
+  // events recorded on the stream right before and after the z-march kernel, when set
+  hipEvent_t ev_begin = nullptr;
+  hipEvent_t ev_end = nullptr;
 };
 // What launch_rhs_forward did with `rk`.
 enum RkDone { RK_NOT_APPLIED = 0, RK_IN_PLACE = 1, RK_TO_YOUT = 2 };

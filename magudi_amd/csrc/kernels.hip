@@ -2427,9 +2427,11 @@ void launch_fwd3d_march(const Geo& g, const double* q, const double* S, const do
                         double* rhs, const RkStage* rk, hipStream_t st) {
   const int units = (g.n[0] / kFMX) * (g.n[1] / kFMY) * ((g.n[2] + kFMZ - 1) / kFMZ);
   const dim3 gm(8 * ((units + 7) / 8), 1, g.ninst);
+  if (rk && rk->ev_begin) (void)hipEventRecord(rk->ev_begin, st);
   k_fwd3d_march<kFMX, kFMY, kFMZ><<<gm, kFMX * kFMY, 0, st>>>(
       g, q, S, kstr, rhs, rk ? rk->stage : 0, rk ? rk->dt : 0.0, rk ? rk->yout : nullptr,
       rk ? rk->b1 : nullptr, rk ? rk->b2 : nullptr, rk ? rk->copy : nullptr);
+  if (rk && rk->ev_end) (void)hipEventRecord(rk->ev_end, st);
 }
 
 int launch_rhs_forward(const Geo& g, const Patches* P, const double* q, double* rhs, double* w,

@@ -127,6 +127,18 @@ struct mgd_region {
   mgd::DevBuf q_alt, qs_alt;             // second state buffers of out-of-place RK stages
   mgd::DevBuf lam_alt;                   // second adjoint buffer of out-of-place adjoint stages
   int mig_loaded = -1;                   // snapshot step whose window mgd_checkpoint_migrate holds
+  // window cache of the generic-kernel path: the checkpointed forward pass also keeps the
+  // stage states of its last `ncache` windows (the first starting at step cache_first) in
+  // HBM that nothing else holds, and the adjoint passes read them instead of recomputing
+  // those windows (the recompute gives the same values)
+  static constexpr int kMaxCache = 8;
+  mgd::DevBuf wcache[kMaxCache];
+  int ncache = 0, cache_first = 0;
+  int opt_cache = -1;                    // MGD_OPT_WINDOW_CACHE: -1 as memory allows, else at most n
+  // per-launch timer of the z-march forward kernel (mgd_kernel_timer): event pairs
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  long tcount = 0, tcopy = 0;
   // solution limits (enable_solution_limits): rho_min, rho_max, T_min, T_max; checked on the
   // input of every forward stage of a run while `checking` (not in recomputes)
   int lim_on = 0;
@@ -475,12 +487,6 @@ void check_stage_input(mgd_region* r, const double* q) {
   if (r->lim_on && r->checking) launch_check_limits(r->geo, q, r->lim, r->d_flag.p, r->stream);
 }
 
-struct Timer {
-  mgd_region* r;
-  int slot;
-  Timer(mgd_region* r_, int s) : r(r_), slot(s) {}
-};
-
 // One forward RK4 substep on the state in `qb` (all instances).  Where the rhs kernels
 // write the new stage state out of place (the 3-D z march) it goes to `alt` and the two
 // buffers trade places, so qb holds the new state either way; every RK4 step has four
@@ -501,6 +507,18 @@ void forward_stage(mgd_region* r, DevBuf& qb, DevBuf& alt, double* b1, double* b
   const long M = (long)r->ninst * M_of(r);
   RkStage rk{stage, dt, q, b1, b2, qcopy};
   if (alt.p && (long)alt.n >= M) rk.yout = alt.p;   // allocated by the run preparation
+  if (r->timing) {
+    const size_t k = 2 * (size_t)r->tcount;
+    while (r->tev.size() < k + 2) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) { r->timing = false; break; }
+      r->tev.push_back(e);
+    }
+    if (r->timing) {
+      rk.ev_begin = r->tev[k];
+      rk.ev_end = r->tev[k + 1];
+    }
+  }
   const int done = launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p, r->G.p, r->t.p,
                                       r->T1.p, ctrl, cstride,
                                       reinterpret_cast<const Source*>(r->d_sources.p), ns,
@@ -509,6 +527,10 @@ void forward_stage(mgd_region* r, DevBuf& qb, DevBuf& alt, double* b1, double* b
   if (done == RK_TO_YOUT) {
     std::swap(qb.p, alt.p);
     std::swap(qb.n, alt.n);
+    if (rk.ev_begin) {          // the z march ran between the two events
+      ++r->tcount;
+      if (qcopy) ++r->tcopy;
+    }
   } else if (done == RK_NOT_APPLIED) {
     launch_rk_forward(M, stage, dt, q, b1, b2, r->rhs.p, qcopy, r->stream);
   }
@@ -852,14 +874,57 @@ int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
 }
 
 // Plain path of the two passes (generic per-operator kernels).
+// Stage states of the window starting at step lt, when the last checkpointed forward pass
+// kept them; null otherwise.
+double* cached_window(mgd_region* r, int lt) {
+  if (r->ncache <= 0 || lt < r->cache_first) return nullptr;
+  const int k = (lt - r->cache_first) / std::max(r->ck.K, 1);
+  return k < r->ncache ? r->wcache[k].p : nullptr;
+}
+
+// How many of the last windows of a checkpointed run the forward pass keeps: as many as
+// MGD_OPT_WINDOW_CACHE allows and, by default, as fit in the device memory left free
+// (counting caches already held) less a reserve of max(8 GiB, a tenth of the device);
+// the generic-kernel path only (the fused 2-D path keeps its last window itself).
+int plan_window_cache(mgd_region* r, int nsteps, int K, long M) {
+  r->ncache = 0;
+  const int nwin = (nsteps + K - 1) / K;
+  int want = r->opt_cache < 0 ? nwin : std::min(r->opt_cache, nwin);
+  want = std::min(want, mgd_region::kMaxCache);
+  if (fused_ok(r)) want = 0;
+  const size_t bytes = (size_t)4 * K * M * sizeof(double);
+  if (want > 0 && r->opt_cache < 0) {
+    size_t fr = 0, tot = 0;
+    MGD_HIP(hipMemGetInfo(&fr, &tot));
+    size_t held = 0;
+    for (int k = 0; k < mgd_region::kMaxCache; ++k) held += r->wcache[k].n * sizeof(double);
+    const size_t reserve = std::max((size_t)8 << 30, tot / 10);
+    const size_t avail = fr + held > reserve ? fr + held - reserve : 0;
+    want = std::min(want, (int)(avail / bytes));
+  }
+  for (int k = want; k < mgd_region::kMaxCache; ++k) r->wcache[k].alloc(0);
+  int n = 0;
+  while (n < want && r->wcache[n].alloc((size_t)4 * K * M) == MGD_OK) ++n;
+  if (n < want) (void)hipGetLastError();   // fewer windows than planned: the rest recompute
+  r->ncache = n;
+  r->cache_first = (nwin - n) * K;
+  return MGD_OK;
+}
+
 int plain_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
   const int nc = n_cost(r);
   const long M = (long)r->ninst * M_of(r);
-  // forward run with snapshots at t0 and every K steps (showProgress .q files)
+  // forward run with snapshots at t0 and every K steps (showProgress .q files); in the
+  // cached windows every stage state also goes to the window's cache
   launch_copy(M, r->q.p, r->snaps.p, r->stream);
   for (int s = 0; s < nsteps; ++s) {
+    const int lt = s - s % K;
+    double* cw = cached_window(r, lt);
+    if (cw && s == lt) launch_copy(M, r->q.p, cw, r->stream);
     for (int i = 1; i <= 4; ++i) {
-      forward_stage(r, r->q, r->q_alt, r->b1.p, r->b2.p, 4 * s + i - 1, i, dt, nullptr);
+      const int idx = 4 * (s - lt) + i;
+      forward_stage(r, r->q, r->q_alt, r->b1.p, r->b2.p, 4 * s + i - 1, i, dt,
+                    cw && idx < 4 * K ? cw + (long)idx * M : nullptr);
       if (nc > 0)
         launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), nc, 0, r->q.p, r->partial.p,
                             kRkNorm[i - 1] * dt, r->acc_J.p, nullptr, r->stream);
@@ -879,6 +944,7 @@ int plain_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
     launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -dt / 6.0, r->lam.p, r->stream);
   }
   int loaded = -1;
+  const double* base = r->window.p;      // stage states of the loaded window
   for (int t = nsteps - 1; t >= 0; --t) {
     for (int i = 4; i >= 1; --i) {
       const int ts = i == 1 ? t : t + 1;
@@ -891,22 +957,26 @@ int plain_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
         else if (ts % K == 0) lt = ts - K;
         else lt = ts - ts % K;
         loaded = lt;
-        launch_copy(M, r->snaps.p + (long)(lt / K) * M, r->qs.p, r->stream);
-        launch_copy(M, r->qs.p, r->window.p, r->stream);
-        if (loaded != nsteps) {
-          int idx = 1;
-          for (int s2 = loaded + 1; s2 <= loaded + K && s2 <= nsteps; ++s2)
-            for (int st2 = 1; st2 <= 4; ++st2) {
-              const bool last = s2 == loaded + K && st2 == 4;
-              forward_stage(r, r->qs, r->qs_alt, r->b1.p, r->b2.p, 4 * (s2 - 1) + st2 - 1, st2, dt,
-                            last ? nullptr : r->window.p + (long)idx * M);
-              if (last) break;
-              ++idx;
-            }
+        const double* cw = lt < nsteps ? cached_window(r, lt) : nullptr;
+        base = cw ? cw : r->window.p;
+        if (!cw) {                       // else kept by the forward pass
+          launch_copy(M, r->snaps.p + (long)(lt / K) * M, r->qs.p, r->stream);
+          launch_copy(M, r->qs.p, r->window.p, r->stream);
+          if (loaded != nsteps) {
+            int idx = 1;
+            for (int s2 = loaded + 1; s2 <= loaded + K && s2 <= nsteps; ++s2)
+              for (int st2 = 1; st2 <= 4; ++st2) {
+                // the window's last stage gives the next snapshot, which no stage reads
+                if (s2 == loaded + K && st2 == 4) break;
+                forward_stage(r, r->qs, r->qs_alt, r->b1.p, r->b2.p, 4 * (s2 - 1) + st2 - 1, st2,
+                              dt, r->window.p + (long)idx * M);
+                ++idx;
+              }
+          }
         }
       }
       const int widx = (ts - 1 - loaded) * 4 + st;
-      const double* qcur = r->window.p + (long)widx * M;
+      const double* qcur = base + (long)widx * M;
       const int sidx = 4 * t + i - 1;
       if (na > 0) {
         if (want_grad)
@@ -969,6 +1039,7 @@ int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, c
   if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M) || r->qs.alloc(M) ||
       r->qT.alloc(M) || (r->nd == 3 && r->qs_alt.alloc(M)))
     return MGD_ERR_HIP;
+  if (int rc = plan_window_cache(r, nsteps, K, M)) return rc;
   if (fused_ok(r)) {
     if (int rc = prepare_quadrature(r, nst, dt)) return rc;
     if (r->resident && r->wall.alloc((size_t)(4L * nsteps + 1) * M)) {
@@ -1077,6 +1148,11 @@ std::vector<uintptr_t> graph_key(mgd_region* r, int kind, std::initializer_list<
                           &r->qy, &r->ly, &r->l0, &r->jpart, &r->spart, &r->ssum, &r->wts,
                           &r->window2, &r->b2r, &r->qsr, &r->qT, &r->d_tile_mask, &r->wall};
   for (const DevBuf* b : bufs) k.push_back((uintptr_t)b->p);
+  k.push_back((uintptr_t)r->q_alt.p);
+  k.push_back((uintptr_t)r->qs_alt.p);
+  k.push_back((uintptr_t)r->ncache);
+  k.push_back((uintptr_t)r->cache_first);
+  for (int c = 0; c < r->ncache; ++c) k.push_back((uintptr_t)r->wcache[c].p);
   for (uintptr_t e : extra) k.push_back(e);
   return k;
 }
@@ -1085,6 +1161,7 @@ template <class F>
 int run_device_work(mgd_region* r, bool use_graph, int slot, const std::vector<uintptr_t>& key,
                     F&& enqueue) {
   if (!use_graph) return enqueue();
+  r->timing = false;                     // the per-launch timer is for eager runs only
   GraphSlot& gs = r->graphs[slot];
   if (!gs.exec || gs.key != key) {
     if (gs.exec) {
@@ -1513,6 +1590,7 @@ int mgd_region_destroy(mgd_region* r) {
     if (gs.exec) (void)hipGraphExecDestroy(gs.exec);
   if (r->ev0) (void)hipEventDestroy(r->ev0);
   if (r->ev1) (void)hipEventDestroy(r->ev1);
+  for (hipEvent_t e : r->tev) (void)hipEventDestroy(e);
   if (r->ev_fwd) (void)hipEventDestroy(r->ev_fwd);
   for (int k = 0; k < 2; ++k) {
     if (r->ev_ready[k]) (void)hipEventDestroy(r->ev_ready[k]);
@@ -1828,6 +1906,7 @@ int mgd_region_set_option(mgd_region* r, int option, int value) {
   switch (option) {
     case MGD_OPT_FORCE_PLAIN: r->force_plain = value != 0; return MGD_OK;
     case MGD_OPT_RESIDENT_STATES: r->resident = value != 0; return MGD_OK;
+    case MGD_OPT_WINDOW_CACHE: r->opt_cache = value < 0 ? -1 : value; return MGD_OK;
     case MGD_OPT_TILE_SHAPE: {
       if (value < -1 || value >= kNumTileShapes) { set_error("tile shape must be -1 .. 5"); return MGD_ERR_ARG; }
       r->tile_shape = value;
@@ -2313,6 +2392,31 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches) {
     if (out) out[i] = r->last_ms[i];
     if (launches) launches[i] = r->last_launch[i];
   }
+  return MGD_OK;
+}
+
+int mgd_kernel_timer(mgd_region* r, int which, int on) {
+  if (!r) { set_error("null region"); return MGD_ERR_ARG; }
+  if (which != MGD_KERNEL_FWD_MARCH3D) { set_error("only the z-march kernel is timed per launch"); return MGD_ERR_ARG; }
+  r->timing = on != 0;
+  if (r->timing) r->tcount = r->tcopy = 0;
+  return MGD_OK;
+}
+
+int mgd_kernel_timer_read(mgd_region* r, long* launches, long* with_copy, double* ms) {
+  if (!r || !launches || !with_copy || !ms) { set_error("null argument"); return MGD_ERR_ARG; }
+  *launches = r->tcount;
+  *with_copy = r->tcopy;
+  *ms = 0.0;
+  if (r->tcount == 0) return MGD_OK;
+  MGD_HIP(hipEventSynchronize(r->tev[2 * (size_t)r->tcount - 1]));
+  double sum = 0.0;
+  for (long k = 0; k < r->tcount; ++k) {
+    float e = 0.0f;
+    MGD_HIP(hipEventElapsedTime(&e, r->tev[2 * k], r->tev[2 * k + 1]));
+    sum += e;
+  }
+  *ms = sum;
   return MGD_OK;
 }
 

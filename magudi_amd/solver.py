@@ -260,6 +260,24 @@ class Region:
         check(capi.lib().mgd_time_kernel(self.handle, which, reps, C.byref(ms), C.byref(b)))
         return ms.value, b.value
 
+    def set_window_cache(self, n: int):
+        """MGD_OPT_WINDOW_CACHE: -1 keeps as many windows of the checkpointed forward pass
+        as device memory allows (default), 0 none, n > 0 at most n."""
+        check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_WINDOW_CACHE, int(n)))
+
+    def kernel_timer(self, on: bool, which: int = None):
+        """Starts (on) or stops per-launch HIP-event timing of the z-march forward kernel
+        inside the runs."""
+        which = capi.KERNEL_FWD_MARCH3D if which is None else which
+        check(capi.lib().mgd_kernel_timer(self.handle, which, 1 if on else 0))
+
+    def kernel_timer_read(self):
+        """(launches, launches that also wrote a window copy, summed ms) since the timer
+        started."""
+        n, nc, ms = C.c_long(0), C.c_long(0), C.c_double(0.0)
+        check(capi.lib().mgd_kernel_timer_read(self.handle, C.byref(n), C.byref(nc), C.byref(ms)))
+        return n.value, nc.value, ms.value
+
     def timings(self):
         out = np.zeros(3)
         n = (C.c_int * 3)()
