@@ -522,6 +522,7 @@ def main():
     for _ in range(max(1, a.warmup)):
         K.one()
     t_step, J, t_fwd, launches, with_copy, march_ms = K.timed(a.steps)
+    zchunk = K.R.z_chunk
     value = K.pts / t_step
     print(f"headline: {t_step:.3f} s per pass ({value:.4g} grid-pt-steps/s)", file=sys.stderr,
           flush=True)
@@ -562,6 +563,7 @@ def main():
                                f"kinetic energy and dJ/dq(T) on the device",
                    "grid": [n, n, n], "nsteps": a.c4_nsteps,
                    "save_interval": max(1, a.c4_nsteps // 8), "instances": 1,
+                   "z_chunk": zchunk,
                    "path": "3-D kernels: stress pass + z-march forward; adjoint stress pass + "
                            "Jacobian march + finishing march"},
         "roofline": roofline,

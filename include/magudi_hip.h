@@ -309,15 +309,21 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches);
  * the adjoint passes read them instead of recomputing those windows; identical results.
  * -1 (default): as many windows as fit in the device memory left free at the run's
  * preparation less a reserve of max(8 GiB, a tenth of the device), at most 8; 0: none;
- * n > 0: at most n. */
+ * n > 0: at most n.
+ * MGD_OPT_Z_CHUNK: planes per work unit of the 3-D z marches (periodic 3-D regions): 0
+ * (default) autotune at the first run -- one forward RK4 step and one adjoint rhs timed
+ * for each of 32, 64 and 128 --, or one of those fixed.  No result depends on it. */
 #define MGD_OPT_FORCE_PLAIN 1
 #define MGD_OPT_TILE_SHAPE 2
 #define MGD_OPT_RESIDENT_STATES 3
 #define MGD_OPT_WINDOW_CACHE 4
+#define MGD_OPT_Z_CHUNK 5
 int mgd_region_set_option(mgd_region* r, int option, int value);
 
 /* Tile shape the fused 2-D kernels use now (-1: the region runs the generic kernels). */
 int mgd_region_get_tile_shape(mgd_region* r, int* shape);
+/* Z chunk the 3-D marches use now (MGD_OPT_Z_CHUNK; after the autotuner, if it ran). */
+int mgd_region_get_z_chunk(mgd_region* r, int* zchunk);
 
 /* Times one kernel on the region stream with HIP events after one warm-up repetition.
  * which: MGD_KERNEL_FWD_STAGE (fused 2-D forward stage kernel, `reps` RK4 steps of 4

@@ -5,6 +5,7 @@
 // bounded directions are covered.  These are the reference-shaped building blocks;
 // the fused 2-D kernels in fused2d.hip replace the hot ones for the benchmark path.
 #include <cfloat>
+#include <type_traits>
 
 #include "device.h"
 #include "kernels.h"
@@ -2364,7 +2365,7 @@ static dim3 grid_pts(const Geo& g) {
 
 // k_prim + k_grad, or the one-pass periodic 3-D gradient kernel where it applies
 // (3-D, every direction periodic, x / y sizes multiples of its 32 x 16 tile).
-constexpr int kG3X = 32, kG3Y = 16, kG3Z = 64;
+constexpr int kG3X = 32, kG3Y = 16;
 static bool grad3d_applies(const Geo& g) {
   return g.nd == 3 && g.dir[0].periodic && g.dir[1].periodic && g.dir[2].periodic &&
          g.n[0] % kG3X == 0 && g.n[1] % kG3Y == 0 && g.nv == 5;
@@ -2379,8 +2380,21 @@ static bool fluxdiv3d_applies(const Geo& g) {
 }
 
 // grid of the periodic gradient kernel: (tile, z chunk) units padded to a multiple of 8
+// The z chunk (planes per work unit) of the 3-D marches, g.zchunk: 32, 64 or 128 (the
+// autotuner's choice; no value depends on it), f called with it as a compile-time value.
+template <class F>
+static void with_zchunk(const Geo& g, F&& f) {
+  switch (g.zchunk) {
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    case 128: f(std::integral_constant<int, 128>{}); break;
+    default: f(std::integral_constant<int, 64>{}); break;
+  }
+}
+static int zchunk_of(const Geo& g) { return g.zchunk == 32 || g.zchunk == 128 ? g.zchunk : 64; }
+
 static dim3 grad3d_grid(const Geo& g) {
-  const int units = (g.n[0] / kG3X) * (g.n[1] / kG3Y) * ((g.n[2] + kG3Z - 1) / kG3Z);
+  const int zc = zchunk_of(g);
+  const int units = (g.n[0] / kG3X) * (g.n[1] / kG3Y) * ((g.n[2] + zc - 1) / zc);
   return dim3(8 * ((units + 7) / 8), 1, g.ninst);
 }
 
@@ -2388,7 +2402,9 @@ template <int ND>
 static void launch_gradients(const Geo& g, const double* q, double* w, double* G, const dim3& grid,
                              hipStream_t st) {
   if (ND == 3 && grad3d_applies(g)) {
-    k_grad3d_periodic<kG3X, kG3Y, kG3Z><<<grad3d_grid(g), kG3X * kG3Y, 0, st>>>(g, q, G);
+    with_zchunk(g, [&](auto Z) {
+      k_grad3d_periodic<kG3X, kG3Y, decltype(Z)::value><<<grad3d_grid(g), kG3X * kG3Y, 0, st>>>(g, q, G);
+    });
     return;
   }
   k_prim<ND><<<grid, kBlock, 0, st>>>(g, q, w);
@@ -2397,7 +2413,13 @@ static void launch_gradients(const Geo& g, const double* q, double* w, double* G
 
 // The one-march forward kernel: 3-D, periodic in every direction, x / y sizes multiples
 // of its 16 x 16 tile; viscous regions also need the stress kernel (grad3d_applies).
-constexpr int kFMX = 16, kFMY = 16, kFMZ = 64;
+constexpr int kFMX = 16, kFMY = 16;
+// grid of the 3-D march kernels: (tile, z chunk) units padded to a multiple of 8
+static dim3 march_grid(const Geo& g) {
+  const int zc = zchunk_of(g);
+  const int units = (g.n[0] / kFMX) * (g.n[1] / kFMY) * ((g.n[2] + zc - 1) / zc);
+  return dim3(8 * ((units + 7) / 8), 1, g.ninst);
+}
 static bool fwd3d_march_applies(const Geo& g) {
   return g.nd == 3 && g.nv == 5 && g.dir[0].periodic && g.dir[1].periodic && g.dir[2].periodic &&
          g.n[0] % kFMX == 0 && g.n[1] % kFMY == 0 && (!g.viscous || grad3d_applies(g));
@@ -2417,20 +2439,23 @@ static bool adj3d_march_path(const Geo& g, const FinishInfo& fin) {
 
 void launch_stress3d(const Geo& g, const double* q, double* S, hipStream_t st) {
   const dim3 gg = grad3d_grid(g);
-  if (g.power_law > 0.0)
-    k_grad3d_periodic<kG3X, kG3Y, kG3Z, true, true><<<gg, kG3X * kG3Y, 0, st>>>(g, q, S);
-  else
-    k_grad3d_periodic<kG3X, kG3Y, kG3Z, true, false><<<gg, kG3X * kG3Y, 0, st>>>(g, q, S);
+  with_zchunk(g, [&](auto Z) {
+    if (g.power_law > 0.0)
+      k_grad3d_periodic<kG3X, kG3Y, decltype(Z)::value, true, true><<<gg, kG3X * kG3Y, 0, st>>>(g, q, S);
+    else
+      k_grad3d_periodic<kG3X, kG3Y, decltype(Z)::value, true, false><<<gg, kG3X * kG3Y, 0, st>>>(g, q, S);
+  });
 }
 
 void launch_fwd3d_march(const Geo& g, const double* q, const double* S, const double* kstr,
                         double* rhs, const RkStage* rk, hipStream_t st) {
-  const int units = (g.n[0] / kFMX) * (g.n[1] / kFMY) * ((g.n[2] + kFMZ - 1) / kFMZ);
-  const dim3 gm(8 * ((units + 7) / 8), 1, g.ninst);
+  const dim3 gm = march_grid(g);
   if (rk && rk->ev_begin) (void)hipEventRecord(rk->ev_begin, st);
-  k_fwd3d_march<kFMX, kFMY, kFMZ><<<gm, kFMX * kFMY, 0, st>>>(
-      g, q, S, kstr, rhs, rk ? rk->stage : 0, rk ? rk->dt : 0.0, rk ? rk->yout : nullptr,
-      rk ? rk->b1 : nullptr, rk ? rk->b2 : nullptr, rk ? rk->copy : nullptr);
+  with_zchunk(g, [&](auto Z) {
+    k_fwd3d_march<kFMX, kFMY, decltype(Z)::value><<<gm, kFMX * kFMY, 0, st>>>(
+        g, q, S, kstr, rhs, rk ? rk->stage : 0, rk ? rk->dt : 0.0, rk ? rk->yout : nullptr,
+        rk ? rk->b1 : nullptr, rk ? rk->b2 : nullptr, rk ? rk->copy : nullptr);
+  });
   if (rk && rk->ev_end) (void)hipEventRecord(rk->ev_end, st);
 }
 
@@ -2510,15 +2535,16 @@ bool launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
     (void)w;
     (void)grid;
     launch_stress3d(g, q, G, st);
-    const int units = (g.n[0] / kFMX) * (g.n[1] / kFMY) * ((g.n[2] + kFMZ - 1) / kFMZ);
-    const dim3 gm(8 * ((units + 7) / 8), 1, g.ninst);
-    if (g.power_law != 0.0)
-      k_adj3d_jac_march<kFMX, kFMY, kFMZ, true><<<gm, kFMX * kFMY, 0, st>>>(g, q, lam, G, rhs, T1);
-    else
-      k_adj3d_jac_march<kFMX, kFMY, kFMZ, false><<<gm, kFMX * kFMY, 0, st>>>(g, q, lam, G, rhs, T1);
-    k_adj3d_finish_march<kFMX, kFMY, kFMZ><<<gm, kFMX * kFMY, 0, st>>>(
-        g, T1, q, lam, fin.kolmogorov, rhs, rk ? rk->stage : 0, rk ? rk->dt : 0.0,
-        rk ? rk->y : nullptr, rk ? rk->b1 : nullptr, rk ? rk->b2 : nullptr);
+    const dim3 gm = march_grid(g);
+    with_zchunk(g, [&](auto Z) {
+      if (g.power_law != 0.0)
+        k_adj3d_jac_march<kFMX, kFMY, decltype(Z)::value, true><<<gm, kFMX * kFMY, 0, st>>>(g, q, lam, G, rhs, T1);
+      else
+        k_adj3d_jac_march<kFMX, kFMY, decltype(Z)::value, false><<<gm, kFMX * kFMY, 0, st>>>(g, q, lam, G, rhs, T1);
+      k_adj3d_finish_march<kFMX, kFMY, decltype(Z)::value><<<gm, kFMX * kFMY, 0, st>>>(
+          g, T1, q, lam, fin.kolmogorov, rhs, rk ? rk->stage : 0, rk ? rk->dt : 0.0,
+          rk ? rk->y : nullptr, rk ? rk->b1 : nullptr, rk ? rk->b2 : nullptr);
+    });
     return rk != nullptr;
   }
   bool finished = false;

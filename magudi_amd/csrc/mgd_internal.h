@@ -64,6 +64,7 @@ struct Geo {
   const Op* ops;            // kNumSlots operators (direction independent)
   double gamma, re_inv, pr_inv, power_law, bulk_ratio, diss_amount;
   int viscous, dissipation, composite;
+  int zchunk;               // z planes per work unit of the 3-D marches (32, 64 or 128)
 };
 
 // Device patch descriptor.

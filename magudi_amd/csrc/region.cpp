@@ -99,6 +99,8 @@ struct mgd_region {
   bool tiling_ok = false;
   int tile_shape = -1;                   // MGD_OPT_TILE_SHAPE: -1 autotune, else fixed
   bool tuned = false;
+  int opt_zchunk = 0;                    // MGD_OPT_Z_CHUNK: 0 autotune, else 32, 64 or 128
+  bool zc_tuned = false;
   int force_plain = 0;
   int aforce_zero = 0;
   long ctrl_stride = 0;                  // control entries per instance
@@ -726,8 +728,61 @@ int tune_tiling(mgd_region* r) {
   return install_tiling(r, best);
 }
 
-// Runs the autotuner once per region unless a tile shape was fixed by option.
+// The z chunk of the 3-D marches (planes per work unit): one forward RK4 step (stress
+// pass and march per stage, out of place) and one adjoint rhs (stress pass and the two
+// marches, where they serve the region) timed with HIP events for each of 32, 64 and 128
+// planes, three interleaved rounds, the fastest round of each compared.  No value depends
+// on the chunk.
+int tune_zchunk(mgd_region* r) {
+  r->zc_tuned = true;
+  const long M = (long)r->ninst * M_of(r);
+  if (int rc = ensure_adjoint(r)) return rc;
+  if (r->qs.alloc(M) || r->qy.alloc(M)) return MGD_ERR_HIP;
+  const double dt = 1e-3;
+  auto round_trip = [&]() {
+    launch_copy(M, r->q.p, r->qs.p, r->stream);
+    launch_copy(M, r->q.p, r->b1.p, r->stream);
+    launch_copy(M, r->q.p, r->b2.p, r->stream);
+    launch_copy(M, r->q.p, r->lam.p, r->stream);
+    for (int i = 1; i <= 4; ++i) {
+      RkStage rk{i, dt, (i % 2) ? r->qs.p : r->qy.p, r->b1.p, r->b2.p, nullptr,
+                 (i % 2) ? r->qy.p : r->qs.p};
+      if (r->geo.viscous) launch_stress3d(r->geo, rk.y, r->G.p, r->stream);
+      launch_fwd3d_march(r->geo, rk.y, r->G.p, r->fin.kolmogorov, r->rhs.p, &rk, r->stream);
+    }
+    launch_rhs_adjoint(r->geo, dpatches(r), r->qs.p, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
+                       r->T1.p, r->S.p, nullptr, 0, 1.0, r->fin, r->stream);
+  };
+  const int zcs[3] = {32, 64, 128};
+  float best[3] = {0.f, 0.f, 0.f};
+  for (int round = 0; round < 3; ++round)
+    for (int k = 0; k < 3; ++k) {
+      r->geo.zchunk = zcs[k];
+      round_trip();   // warm-up
+      MGD_HIP(hipEventRecord(r->ev0, r->stream));
+      round_trip();
+      MGD_HIP(hipEventRecord(r->ev1, r->stream));
+      MGD_HIP(hipGetLastError());
+      MGD_HIP(hipEventSynchronize(r->ev1));
+      float ms = 0.f;
+      MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+      if (round == 0 || ms < best[k]) best[k] = ms;
+    }
+  int kb = 1;
+  for (int k = 0; k < 3; ++k)
+    if (best[k] < best[kb]) kb = k;
+  r->geo.zchunk = zcs[kb];
+  return MGD_OK;
+}
+
+// Runs the autotuners once per region unless fixed by option: the tile shape of the fused
+// 2-D kernels, the z chunk of the 3-D marches.
 int maybe_tune(mgd_region* r) {
+  if (r->nd == 3 && !r->zc_tuned && r->opt_zchunk == 0 &&
+      fwd3d_march_path(r->geo, r->fin, (int)r->sources.size())) {
+    if (int rc = ensure_state(r)) return rc;
+    if (int rc = tune_zchunk(r)) return rc;
+  }
   if (!fused_ok(r) || r->tuned || r->tile_shape >= 0) return MGD_OK;
   return tune_tiling(r);
 }
@@ -1564,6 +1619,7 @@ int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region
   g.viscous = d->viscous;
   g.dissipation = d->dissipation;
   g.composite = d->composite_dissipation;
+  g.zchunk = 64;
   if (int rc = scratch_setup(&r->device)) return rc;
   MGD_HIP(hipStreamCreate(&r->stream));
   MGD_HIP(hipStreamCreate(&r->stream_rec));
@@ -1895,6 +1951,12 @@ int mgd_region_set_time_offsets(mgd_region* r, const double* offsets) {
   return MGD_OK;
 }
 
+int mgd_region_get_z_chunk(mgd_region* r, int* zchunk) {
+  if (!r || !zchunk) { set_error("null argument"); return MGD_ERR_ARG; }
+  *zchunk = r->geo.zchunk;
+  return MGD_OK;
+}
+
 int mgd_region_get_tile_shape(mgd_region* r, int* shape) {
   if (!r || !shape) { set_error("null argument"); return MGD_ERR_ARG; }
   *shape = fused_ok(r) ? r->tiling.shape : -1;
@@ -1907,6 +1969,15 @@ int mgd_region_set_option(mgd_region* r, int option, int value) {
     case MGD_OPT_FORCE_PLAIN: r->force_plain = value != 0; return MGD_OK;
     case MGD_OPT_RESIDENT_STATES: r->resident = value != 0; return MGD_OK;
     case MGD_OPT_WINDOW_CACHE: r->opt_cache = value < 0 ? -1 : value; return MGD_OK;
+    case MGD_OPT_Z_CHUNK:
+      if (value != 0 && value != 32 && value != 64 && value != 128) {
+        set_error("z chunk: 0 (autotune), 32, 64 or 128");
+        return MGD_ERR_ARG;
+      }
+      r->opt_zchunk = value;
+      r->geo.zchunk = value == 0 ? 64 : value;
+      r->zc_tuned = value != 0;
+      return MGD_OK;
     case MGD_OPT_TILE_SHAPE: {
       if (value < -1 || value >= kNumTileShapes) { set_error("tile shape must be -1 .. 5"); return MGD_ERR_ARG; }
       r->tile_shape = value;

@@ -265,6 +265,18 @@ class Region:
         as device memory allows (default), 0 none, n > 0 at most n."""
         check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_WINDOW_CACHE, int(n)))
 
+    def set_z_chunk(self, zc: int):
+        """MGD_OPT_Z_CHUNK: 0 autotunes the 3-D marches' z chunk at the first run, or 32,
+        64, 128 fixes it."""
+        check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_Z_CHUNK, int(zc)))
+
+    @property
+    def z_chunk(self) -> int:
+        """The 3-D marches' z chunk in use (after the autotuner, if it ran)."""
+        z = C.c_int(0)
+        check(capi.lib().mgd_region_get_z_chunk(self.handle, C.byref(z)))
+        return z.value
+
     def kernel_timer(self, on: bool, which: int = None):
         """Starts (on) or stops per-launch HIP-event timing of the z-march forward kernel
         inside the runs."""

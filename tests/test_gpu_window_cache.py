@@ -34,7 +34,7 @@ def test_window_cache_3d_bit_for_bit():
         n_all, c_all, ms_all = R.kernel_timer_read()
         R.kernel_timer(False)
         out[cache] = (qT, R.states[0].adjointVariables, n_fwd, c_fwd, n_all, c_all)
-        assert ms_fwd > 0 and ms_all >= ms_fwd
+        assert ms_fwd > 0 and ms_all >= ms_fwd * (1 - 1e-5)
     for cache in (1, -1):
         assert np.array_equal(out[cache][0], out[0][0])
         assert np.array_equal(out[cache][1], out[0][1]), cache
@@ -64,3 +64,29 @@ def test_window_cache_2d_generic_bit_for_bit(cache):
         res.append((sens, grad, J, R.states[0].adjointVariables))
     for a, b in zip(res[0], res[1]):
         assert np.array_equal(a, b)
+
+
+def test_z_chunk_changes_no_value():
+    """The 3-D marches' z chunk (32, 64, 128 planes per work unit, and the autotuner's
+    choice) on a 64 x 32 x 80 box (partial chunks for 32 and 64): forward and adjoint runs
+    bit for bit the same."""
+    from magudi_amd.solver import Region, Solver
+    from test_gpu_parity import periodic_box, perturbed
+    c = periodic_box((64, 32, 80), dissipation=False)
+    nsteps, K = 3, 2
+    q0 = perturbed(c, 51)
+    lamT = np.random.default_rng(52).standard_normal((c.nv, c.npoints))
+    out = []
+    for zc in (32, 64, 128, 0):
+        R = Region(c)
+        R.set_z_chunk(zc)
+        R.states[0].conservedVariables = q0
+        S = Solver(R, dt=c.dt, nsteps=nsteps, save_interval=K)
+        S.runForwardCheckpointed()
+        qT = R.states[0].conservedVariables
+        S.runAdjointCheckpointed(terminal=lamT, want_gradient=False)
+        out.append((qT, R.states[0].adjointVariables))
+        assert R.z_chunk == (zc if zc else R.z_chunk) and R.z_chunk in (32, 64, 128)
+    for qT, lam in out[1:]:
+        assert np.array_equal(qT, out[0][0])
+        assert np.array_equal(lam, out[0][1])
