@@ -120,14 +120,14 @@ def measured_traffic(args, kernel, copy_bytes):
     return {"bytes": res["FETCH_SIZE"]["bytes"] + res["WRITE_SIZE"]["bytes"], "passes": res}
 
 
-def run_hot_path(R, S, steps):
+def run_hot_path(R, S, steps, use_graph=False):
     """steps x (restart from the resident initial condition + forward + adjoint with the
-    control gradient)."""
+    control gradient); use_graph replays the run's device work as one HIP graph."""
     R.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
         R.restart()
-        out = S.runAdjoint(want_gradient=True)
+        out = S.runAdjoint(want_gradient=True, use_graph=use_graph)
     R.sync()
     return (time.perf_counter() - t0) / steps, out
 
@@ -261,6 +261,15 @@ def fd_sweep_leg(n=401, nsteps=960, save_interval=32, batch=20):
     Jb = Sb.runForward(control=ctl)
     Rb.sync()
     tb = time.perf_counter() - t0
+    # the same batched run replayed as one HIP graph (captured on its first call)
+    Rb.restart()
+    Sb.runForward(control=ctl, use_graph=True)
+    Rb.restart()
+    Rb.sync()
+    t0 = time.perf_counter()
+    Jg = Sb.runForward(control=ctl, use_graph=True)
+    Rb.sync()
+    tg = time.perf_counter() - t0
     # the same re-runs one after another on the single-instance region
     R.states[0].initialCondition = q0
     Js = []
@@ -274,8 +283,11 @@ def fd_sweep_leg(n=401, nsteps=960, save_interval=32, batch=20):
     err = [abs((float(Jb[k]) - J0) / amps[k] - g0) / abs(g0) for k in range(batch)]
     return {"workload": f"{batch} forward re-runs of the {n}x{n} monopole ({nsteps} RK4 steps) "
                         f"at control amplitudes 10^(-2 - 0.25 k) along the adjoint gradient",
-            "value": batch * c.npoints * nsteps / tb, "unit": "grid-pt-steps/s",
-            "sequential_value": batch * c.npoints * nsteps / ts, "speedup": ts / tb,
+            "value": batch * c.npoints * nsteps / min(tb, tg), "unit": "grid-pt-steps/s",
+            "eager_value": batch * c.npoints * nsteps / tb,
+            "graph_value": batch * c.npoints * nsteps / tg,
+            "graph_same_J": bool(np.array_equal(Jg, Jb)),
+            "sequential_value": batch * c.npoints * nsteps / ts, "speedup": ts / min(tb, tg),
             "J0": J0, "Grad0": g0, "same_J_as_single_runs": bool(all(float(Jb[k]) == Js[k] for k in range(batch))),
             "rel_error": [[amps[k], err[k]] for k in range(batch)]}
 
@@ -283,21 +295,35 @@ def fd_sweep_leg(n=401, nsteps=960, save_interval=32, batch=20):
 def small_grid_leg(n=201, nsteps=96, save_interval=32, batch=20):
     """Batched against plain on a small grid: batched fused forward+adjoint (batch instances
     per launch) against the generic per-instance path, whose many small launches
-    make it launch-bound at this size."""
+    make it launch-bound at this size, and against the fused kernels on one instance.
+    Every run recomputes its windows from the snapshots (the plain runs with the window
+    cache off: on a small grid it would keep every window and skip the recompute the
+    fused runs do)."""
     from magudi_amd import cases
     from magudi_amd.solver import Region, Solver
     c = cases.acoustic_monopole(n=n, nsteps=nsteps, save_interval=save_interval)
     q0 = c.initial_state()
     out = {"grid": [n, n], "nsteps": nsteps, "instances": batch}
-    for key, ninst, plain in (("batched_fused", batch, False), ("plain_single", 1, True)):
+    for key, ninst, plain, graph in (("batched_fused", batch, False, False),
+                                     ("batched_fused_graph", batch, False, True),
+                                     ("fused_single", 1, False, False),
+                                     ("fused_single_graph", 1, False, True),
+                                     ("plain_single", 1, True, False)):
         R = Region(c, n_instances=ninst, force_plain=plain)
+        if plain:
+            R.set_window_cache(0)   # window recompute, as the fused runs do (see below)
         for i in range(ninst):
             R.states[i].initialCondition = q0
         S = Solver(R)
-        run_hot_path(R, S, 1)
-        t, _ = run_hot_path(R, S, 1)
+        run_hot_path(R, S, 1, graph)
+        t, _ = run_hot_path(R, S, 1, graph)
         out[key] = ninst * c.npoints * nsteps / t
-    out["ratio"] = out["batched_fused"] / out["plain_single"]
+        del R, S
+    best = max(out["batched_fused"], out["batched_fused_graph"])
+    # the north-star ratio (batched against the plain per-instance path) and the batching
+    # gain alone (against the fused kernels on one instance)
+    out["ratio"] = best / out["plain_single"]
+    out["ratio_vs_fused_single"] = best / max(out["fused_single"], out["fused_single_graph"])
     return out
 
 
@@ -363,6 +389,7 @@ def small_ensemble_leg(count=32, nsteps=32, save_interval=8, seed=3):
     import numpy as np
     from magudi_amd import cases
     from magudi_amd.group import RegionGroup
+    from magudi_amd.solver import Region, Solver
     rng = np.random.default_rng(seed)
     ns = [int(2 * rng.integers(20, 65) + 1) for _ in range(count)]
     G = RegionGroup([cases.acoustic_monopole(n=n, nsteps=nsteps, save_interval=save_interval)
@@ -381,12 +408,33 @@ def small_ensemble_leg(count=32, nsteps=32, save_interval=8, seed=3):
     G.run_adjoint(want_gradient=True)
     t_c = timed(lambda: G.run_adjoint(want_gradient=True))
     t_s = timed(lambda: [s.runAdjoint(want_gradient=True) for s in G.solvers])
+    [s.runAdjoint(want_gradient=True, use_graph=True) for s in G.solvers]
+    t_g = timed(lambda: [s.runAdjoint(want_gradient=True, use_graph=True) for s in G.solvers])
     G.close()
+    # the members one after another on the generic per-operator kernels (the plain
+    # per-instance path), fewer steps at the same per-step work
+    ns_plain = max(save_interval, nsteps // 4)
+    plain_pts = 0
+    t_p = 0.0
+    for n in ns:
+        c = cases.acoustic_monopole(n=n, nsteps=ns_plain, save_interval=save_interval)
+        Rp = Region(c, force_plain=True)
+        Rp.set_window_cache(0)      # window recompute, as the fused runs do
+        Rp.states[0].initialCondition = c.initial_state()
+        Sp = Solver(Rp)
+        run_hot_path(Rp, Sp, 1)
+        t, _ = run_hot_path(Rp, Sp, 1)
+        t_p += t
+        plain_pts += c.npoints * ns_plain
+        del Rp, Sp
     pts = G.grid_point_steps
     return {"workload": f"{count} monopoles, odd sizes in [41, 129] (seed {seed}); fwd+adjoint "
                         f"{nsteps} steps, snapshots every {save_interval}",
             "merged_value": pts / t_m, "concurrent_value": pts / t_c, "sequential_value": pts / t_s,
-            "unit": "grid-pt-steps/s", "merged_speedup_vs_sequential": t_s / t_m}
+            "sequential_graph_value": pts / t_g, "plain_sequential_value": plain_pts / t_p,
+            "unit": "grid-pt-steps/s", "merged_speedup_vs_sequential": t_s / t_m,
+            "merged_speedup_vs_sequential_graph": t_g / t_m,
+            "merged_speedup_vs_plain": (pts / t_m) / (plain_pts / t_p)}
 
 
 def monopole_legs(a, extras):
@@ -435,17 +483,21 @@ def monopole_legs(a, extras):
     Sb = Solver(Rb)
     run_hot_path(Rb, Sb, 1)
     tb, _ = run_hot_path(Rb, Sb, 1)
+    run_hot_path(Rb, Sb, 1, True)
+    tbg, _ = run_hot_path(Rb, Sb, 1, True)
     del Rb, Sb
     # generic per-operator path on one instance, fewer steps, same per-step work
     ns_plain = max(a.save_interval, a.nsteps // 10)
     Rp = Region(case, force_plain=True)
+    Rp.set_window_cache(0)          # window recompute, as the fused runs do
     Rp.states[0].initialCondition = q0
     Sp = Solver(Rp, nsteps=ns_plain)
     run_hot_path(Rp, Sp, 1)
     tp, _ = run_hot_path(Rp, Sp, 1)
     del Rp, Sp
     out["batched"] = {"instances": a.batch, "value": a.batch * pts / tb, "ms_per_run": tb * 1e3,
-                      "speedup_vs_single_fused": (a.batch * pts / tb) / value}
+                      "graph_value": a.batch * pts / tbg, "graph_ms_per_run": tbg * 1e3,
+                      "speedup_vs_single_fused": (a.batch * pts / min(tb, tbg)) / value}
     out["plain"] = {"value": case.npoints * ns_plain / tp, "nsteps": ns_plain,
                     "ms_per_run": tp * 1e3,
                     "batched_speedup_vs_plain": (a.batch * pts / tb) / (case.npoints * ns_plain / tp)}
