@@ -93,6 +93,10 @@ void launch_fwd_stage2d(const Geo& g, const Patches* P, const Tiling& T, const F
                         hipStream_t st);
 void launch_adj_stage2d(const Geo& g, const Patches* P, const Tiling& T, const AdjStage& a,
                         hipStream_t st);
+// One adjoint stage and one forward stage of independent states in one launch (the same
+// values as the two launches).
+void launch_dual_stage2d(const Geo& g, const Patches* P, const Tiling& T, const AdjStage& a,
+                         const FwdStage& f, hipStream_t st);
 // One stage of every member of a group in one launch (args: one stage per member, in
 // device memory); MGD_ERR_UNSUPPORTED unless shape is kGroupShape.
 int launch_fwd_stage2d_group(int shape, const GroupMember* mem, int nmem, int nblocks,

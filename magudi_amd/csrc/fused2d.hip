@@ -580,6 +580,22 @@ __device__ __forceinline__ bool in_patch(const DPatch& pt, int i, int j) {
   return i >= pt.lo[0] && i <= pt.hi[0] && j >= pt.lo[1] && j <= pt.hi[1];
 }
 
+// The LDS of a stage body: the staged operand, the flux / scratch space and the tile's
+// grid values.  One allocation per kernel, shared by the forward and adjoint bodies where
+// one kernel runs both (k_dual_stage2d).
+template <class S>
+struct StageLds {
+  double q[S::SQ];
+  double c[S::SSCR];
+  double grid[TileGrid<S>::kSize];
+};
+
+template <class S>
+__device__ __forceinline__ StageLds<S>& stage_lds() {
+  __shared__ StageLds<S> lds;
+  return lds;
+}
+
 // -------------------------------------------------------------------------- forward
 // One forward stage on tile block `blk` of instance `inst` (uniform per work-group).
 template <class S>
@@ -588,9 +604,10 @@ __device__ __forceinline__ void fwd_stage_body(const Geo& g, const Patches* __re
                                                long inst) {
   constexpr int TX = S::TX, TY = S::TY, EX = S::EX, EY = S::EY, NP = S::NP, NC = S::NC;
   constexpr int SFX = S::SFX, STX = S::STX;
-  __shared__ double sq[S::SQ];
-  __shared__ double sc[S::SSCR];
-  __shared__ double sgrid[TileGrid<S>::kSize];
+  StageLds<S>& lds = stage_lds<S>();
+  double(&sq)[S::SQ] = lds.q;
+  double(&sc)[S::SSCR] = lds.c;
+  double(&sgrid)[TileGrid<S>::kSize] = lds.grid;
   const int tile = block_tile(T, blk);
   if (tile < 0) return;   // block beyond the tiles of its XCD (uniform per block)
   const TileGeo t = tile_geo(T, tile);
@@ -791,9 +808,10 @@ __device__ __forceinline__ void adj_stage_body(const Geo& g, const Patches* __re
                                                long inst) {
   constexpr int TX = S::TX, EX = S::EX, EY = S::EY, NP = S::NP, NC = S::NC;
   constexpr int STX = S::STX;
-  __shared__ double sq[S::SQ];
-  __shared__ double sc[S::SSCR];
-  __shared__ double sgrid[TileGrid<S>::kSize];
+  StageLds<S>& lds = stage_lds<S>();
+  double(&sq)[S::SQ] = lds.q;
+  double(&sc)[S::SSCR] = lds.c;
+  double(&sgrid)[TileGrid<S>::kSize] = lds.grid;
   const int tile = block_tile(T, blk);
   if (tile < 0) return;   // block beyond the tiles of its XCD (uniform per block)
   const TileGeo t = tile_geo(T, tile);
@@ -999,6 +1017,18 @@ __global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_adj_stage2d(Geo g, co
   adj_stage_body<S>(g, P, T, a, blockIdx.x, blockIdx.y);
 }
 
+// One adjoint stage and one forward stage of the same region in one launch: blocks
+// [0, nblk) run the adjoint stage's tiles, blocks [nblk, 2 nblk) the forward stage's (the
+// window recompute beside the adjoint of the next window: independent states, so the
+// same values as two launches).  nblk is a multiple of 8, so a block's XCD -- its index
+// mod 8 -- is the one its tile is dealt to either way.
+template <class S>
+__global__ void __launch_bounds__(S::NT, S::kWavesPerEU) k_dual_stage2d(Geo g, const Patches* __restrict__ P,
+                                                        Tiling T, AdjStage a, FwdStage f) {
+  if ((int)blockIdx.x < T.nblk) adj_stage_body<S>(g, P, T, a, blockIdx.x, blockIdx.y);
+  else fwd_stage_body<S>(g, P, T, f, (int)blockIdx.x - T.nblk, blockIdx.y);
+}
+
 // Several regions in one launch: launch block k belongs to the member m with
 // mem[m].block0 <= k < mem[m + 1].block0 (members in block order), which sees it as tile
 // block (k - block0) mod nblk of instance (k - block0) / nblk.  args[m]: the stage of m.
@@ -1201,6 +1231,24 @@ template <class S>
 static void adj_launch(const Geo& g, const Patches* P, const Tiling& T, const AdjStage& a,
                        hipStream_t st) {
   k_adj_stage2d<S><<<dim3(T.nblk, g.ninst), S::NT, 0, st>>>(g, P, T, a);
+}
+
+template <class S>
+static void dual_launch(const Geo& g, const Patches* P, const Tiling& T, const AdjStage& a,
+                        const FwdStage& f, hipStream_t st) {
+  k_dual_stage2d<S><<<dim3(2 * T.nblk, g.ninst), S::NT, 0, st>>>(g, P, T, a, f);
+}
+
+void launch_dual_stage2d(const Geo& g, const Patches* P, const Tiling& T, const AdjStage& a,
+                         const FwdStage& f, hipStream_t st) {
+  switch (T.shape) {
+    case 0: dual_launch<S32x16>(g, P, T, a, f, st); break;
+    case 1: dual_launch<S16x32>(g, P, T, a, f, st); break;
+    case 2: dual_launch<S16x16>(g, P, T, a, f, st); break;
+    case 3: dual_launch<S24x16>(g, P, T, a, f, st); break;
+    case 4: dual_launch<S16x16c2>(g, P, T, a, f, st); break;
+    default: dual_launch<S16x12c2>(g, P, T, a, f, st); break;
+  }
 }
 
 void launch_fwd_stage2d(const Geo& g, const Patches* P, const Tiling& T, const FwdStage& a,

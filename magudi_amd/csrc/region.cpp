@@ -106,10 +106,8 @@ struct mgd_region {
   long ctrl_stride = 0;                  // control entries per instance
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // window recompute of the fused adjoint run: its own stream (default priority), two
-  // window buffers, RK buffer and scratch, and the events that order it with the adjoint
-  hipStream_t stream_rec = nullptr;
-  hipEvent_t ev_fwd = nullptr, ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+  // window recompute of the fused adjoint run: the second window buffer, its RK buffer
+  // and the scratch state
   mgd::DevBuf window2, b2r, qsr;
   // MGD_OPT_RESIDENT_STATES: the forward pass keeps every RK stage state (4 nsteps + 1
   // states) in HBM and the adjoint pass reads them instead of recomputing windows
@@ -790,9 +788,9 @@ int maybe_tune(mgd_region* r) {
 // Solver%runAdjoint on the fused path, split in its two passes.  Forward pass: snapshot
 // every K steps.  Adjoint pass: marches back window by window (window w: steps
 // [wK, min((w+1)K, nsteps))).  A window's stage states are recomputed from its snapshot
-// into one of two window buffers on a second stream, so the recompute of window w-1 runs
-// concurrently with the adjoint of window w (UniformCheckpointer%migrateTo semantics:
-// the same states, each recomputed once).  Window slot 4m holds the state at step wK+m
+// into one of two window buffers, each recompute stage of window w-1 in the same launch
+// as an adjoint stage of window w (k_dual_stage2d), so the two fill the GPU together
+// (UniformCheckpointer%migrateTo semantics: the same states, each recomputed once).  Window slot 4m holds the state at step wK+m
 // and slots 4m+1..4m+3 the inputs of its stages 2-4.  The forward pass writes the stage
 // states of the last window straight into the first window buffer (the same launches,
 // other output addresses), so the adjoint pass starts at once; windows then alternate
@@ -842,29 +840,20 @@ int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
   double* wbuf[2] = {r->window.p, r->window2.p};
   const int nw = (nsteps + K - 1) / K;
   auto buf = [&](int w) { return (nw - 1 - w) % 2; };   // window buffer of window w
-  auto recompute = [&](int w) {
+  // stage j of the recompute of window w into its buffer (slot 4m: the state at step
+  // wK + m, slots 4m + 1 .. 4m + 3 the inputs of its stages 2-4; the state at the next
+  // snapshot goes to qsr, which nothing reads)
+  auto recompute_args = [&](int w, int j) {
     double* W = wbuf[buf(w)];
     const int t0w = w * K, t1w = std::min(nsteps, t0w + K);
-    launch_copy(M, r->snaps.p + (long)w * M, W, r->stream_rec);
-    for (int t = t0w; t < t1w; ++t) {
-      const long base = 4L * (t - t0w);
-      double* outs[4];
-      for (int k = 0; k < 4; ++k) outs[k] = W + (base + 1 + k) * M;
-      if (t == t1w - 1) outs[3] = r->qsr.p;   // state at the next snapshot: not needed
-      fused_forward_step(r, W + base * M, nullptr, nullptr, r->b2r.p, t, dt, nst, false, outs,
-                         r->stream_rec);
-    }
+    const int t = t0w + j / 4, i = j % 4 + 1;
+    const long base = 4L * (t - t0w);
+    double* q0 = W + base * M;
+    double* out = t == t1w - 1 && i == 4 ? r->qsr.p : W + (base + i) * M;
+    const double* in = i == 1 ? q0 : W + (base + i - 1) * M;
+    return fwd_args(r, in, q0, r->b2r.p, out, i, dt, 4 * t + i - 1, nst, false);
   };
-  // the recompute stream joins only when it has windows to recompute (a captured stream
-  // that waits on the run and records nothing it rejoins with would end the capture)
-  if (!r->resident && (nw >= 2 || !last_window)) {
-    MGD_HIP(hipEventRecord(r->ev_fwd, r->stream));
-    MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_fwd, 0));
-    if (!last_window) {
-      recompute(nw - 1);
-      MGD_HIP(hipEventRecord(r->ev_ready[buf(nw - 1)], r->stream_rec));
-    }
-  }
+  auto recompute_stages = [&](int w) { return 4 * (std::min(nsteps, w * K + K) - w * K); };
   // terminal condition (loadInitialCondition ADJOINT, src/SolverImpl.f90:368-428): zero or
   // the given adjoint state, plus the cost forcing of the final state with factor -dt/6
   double* L0 = r->lam.p;
@@ -875,53 +864,63 @@ int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
     launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->qT.p, r->aforce.p, 0, r->stream);
     launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -dt / 6.0, L0, r->stream);
   }
-  // reversed steps t1w - 1 .. t0w, stage states of step t at W + 4 (t - t0w) M
-  auto adjoint_steps = [&](const double* W, int t0w, int t1w) {
-    for (int t = t1w - 1; t >= t0w; --t) {
-      for (int i = 4; i >= 1; --i) {
-        const double* qcur = W + (4L * (t - t0w) + (i - 1)) * M;
-        const int sidx = 4 * t + i - 1;
-        AdjStage a;
-        std::memset(&a, 0, sizeof a);
-        a.lam = i == 4 ? L0 : (i == 3 ? LX : (i == 2 ? LY : LX));
-        a.lout = i == 4 ? LX : (i == 3 ? LY : (i == 2 ? LX : L0));
-        a.l0 = L0;
-        a.q = qcur;
-        a.b2 = r->b2.p;
-        a.stage = i;
-        a.dt = dt;
-        a.forcing_factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
-        a.zero_forcing = t == 0 && i == 1;
-        a.ip_act = r->ip_act;
-        if (na > 0) {
-          if (want_grad) {
-            a.grad = r->grad.p + (long)sidx * na;
-            a.grad_stride = (long)nst * na;
-          }
-          a.spartial = r->spart.p;
-          a.npts = na;
-          a.nslots = r->chunk;
-          a.slot = sidx % r->chunk;
-        }
-        launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
-        reduce_chunk(r, r->spart, na, nst, sidx, true);
+  auto adj_args = [&](const double* W, int t0w, int t, int i) {
+    const double* qcur = W + (4L * (t - t0w) + (i - 1)) * M;
+    const int sidx = 4 * t + i - 1;
+    AdjStage a;
+    std::memset(&a, 0, sizeof a);
+    a.lam = i == 4 ? L0 : (i == 3 ? LX : (i == 2 ? LY : LX));
+    a.lout = i == 4 ? LX : (i == 3 ? LY : (i == 2 ? LX : L0));
+    a.l0 = L0;
+    a.q = qcur;
+    a.b2 = r->b2.p;
+    a.stage = i;
+    a.dt = dt;
+    a.forcing_factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
+    a.zero_forcing = t == 0 && i == 1;
+    a.ip_act = r->ip_act;
+    if (na > 0) {
+      if (want_grad) {
+        a.grad = r->grad.p + (long)sidx * na;
+        a.grad_stride = (long)nst * na;
       }
+      a.spartial = r->spart.p;
+      a.npts = na;
+      a.nslots = r->chunk;
+      a.slot = sidx % r->chunk;
     }
+    return a;
   };
   if (r->resident) {
-    adjoint_steps(r->wall.p, 0, nsteps);
-  } else {
-    for (int w = nw - 1; w >= 0; --w) {
-      if (w >= 1) {
-        // window w-1 reuses the buffer of window w+1: wait until its adjoint has read it
-        if (w + 1 < nw) MGD_HIP(hipStreamWaitEvent(r->stream_rec, r->ev_free[buf(w + 1)], 0));
-        recompute(w - 1);
-        MGD_HIP(hipEventRecord(r->ev_ready[buf(w - 1)], r->stream_rec));
+    for (int t = nsteps - 1; t >= 0; --t)
+      for (int i = 4; i >= 1; --i) {
+        launch_adj_stage2d(r->geo, dpatches(r), r->tiling, adj_args(r->wall.p, 0, t, i), r->stream);
+        reduce_chunk(r, r->spart, na, nst, 4 * t + i - 1, true);
       }
-      // the last window, when the forward pass wrote it, is ordered by the stream itself
-      if (w < nw - 1 || !last_window) MGD_HIP(hipStreamWaitEvent(r->stream, r->ev_ready[buf(w)], 0));
-      adjoint_steps(wbuf[buf(w)], w * K, std::min(nsteps, w * K + K));
-      MGD_HIP(hipEventRecord(r->ev_free[buf(w)], r->stream));
+  } else {
+    // window by window from the last one (its stage states written by the forward pass,
+    // or recomputed first); each adjoint stage of window w shares its launch with the
+    // matching stage of the recompute of window w-1 (the other window buffer), the
+    // recompute's stages left over running alone
+    if (!last_window) {
+      launch_copy(M, r->snaps.p + (long)(nw - 1) * M, wbuf[buf(nw - 1)], r->stream);
+      for (int j = 0; j < recompute_stages(nw - 1); ++j)
+        launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, recompute_args(nw - 1, j), r->stream);
+    }
+    for (int w = nw - 1; w >= 0; --w) {
+      const int t0w = w * K, t1w = std::min(nsteps, t0w + K);
+      const int nrec = w >= 1 ? recompute_stages(w - 1) : 0;
+      if (nrec > 0) launch_copy(M, r->snaps.p + (long)(w - 1) * M, wbuf[buf(w - 1)], r->stream);
+      int j = 0;
+      for (int t = t1w - 1; t >= t0w; --t)
+        for (int i = 4; i >= 1; --i) {
+          const AdjStage a = adj_args(wbuf[buf(w)], t0w, t, i);
+          if (j < nrec) launch_dual_stage2d(r->geo, dpatches(r), r->tiling, a, recompute_args(w - 1, j++), r->stream);
+          else launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+          reduce_chunk(r, r->spart, na, nst, 4 * t + i - 1, true);
+        }
+      for (; j < nrec; ++j)
+        launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, recompute_args(w - 1, j), r->stream);
     }
   }
   if (na > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_S.p, r->ninst, 1, r->stream);
@@ -1622,14 +1621,8 @@ int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region
   g.zchunk = 64;
   if (int rc = scratch_setup(&r->device)) return rc;
   MGD_HIP(hipStreamCreate(&r->stream));
-  MGD_HIP(hipStreamCreate(&r->stream_rec));
   MGD_HIP(hipEventCreate(&r->ev0));
   MGD_HIP(hipEventCreate(&r->ev1));
-  MGD_HIP(hipEventCreateWithFlags(&r->ev_fwd, hipEventDisableTiming));
-  for (int k = 0; k < 2; ++k) {
-    MGD_HIP(hipEventCreateWithFlags(&r->ev_ready[k], hipEventDisableTiming));
-    MGD_HIP(hipEventCreateWithFlags(&r->ev_free[k], hipEventDisableTiming));
-  }
   r->target.assign((size_t)r->nv * N, 0.0);
   r->ctrl_moll.assign(N, 0.0);
   r->tgt_moll.assign(N, 0.0);
@@ -1641,18 +1634,11 @@ int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region
 int mgd_region_destroy(mgd_region* r) {
   if (!r) return MGD_OK;
   if (r->stream) (void)hipStreamSynchronize(r->stream);
-  if (r->stream_rec) (void)hipStreamSynchronize(r->stream_rec);
   for (GraphSlot& gs : r->graphs)
     if (gs.exec) (void)hipGraphExecDestroy(gs.exec);
   if (r->ev0) (void)hipEventDestroy(r->ev0);
   if (r->ev1) (void)hipEventDestroy(r->ev1);
   for (hipEvent_t e : r->tev) (void)hipEventDestroy(e);
-  if (r->ev_fwd) (void)hipEventDestroy(r->ev_fwd);
-  for (int k = 0; k < 2; ++k) {
-    if (r->ev_ready[k]) (void)hipEventDestroy(r->ev_ready[k]);
-    if (r->ev_free[k]) (void)hipEventDestroy(r->ev_free[k]);
-  }
-  if (r->stream_rec) (void)hipStreamDestroy(r->stream_rec);
   if (r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
   return MGD_OK;
@@ -2667,19 +2653,17 @@ int mgd_group_run_adjoint(mgd_group* G, double t0, const double* dt, int nsteps,
     }
   }
   // every member works on the group's streams for the duration of the run
-  std::vector<std::pair<hipStream_t, hipStream_t>> streams;
+  std::vector<hipStream_t> streams;
   std::vector<int> shapes;
   for (mgd_region* r : G->m) {
-    streams.emplace_back(r->stream, r->stream_rec);
+    streams.push_back(r->stream);
     shapes.push_back(r->tiling.shape);
     r->stream = G->st;
-    r->stream_rec = G->st_rec;
   }
   auto restore = [&]() {
     for (int k = 0; k < n; ++k) {
       mgd_region* r = G->m[k];
-      r->stream = streams[k].first;
-      r->stream_rec = streams[k].second;
+      r->stream = streams[k];
       if (r->tiling.shape != shapes[k]) (void)install_tiling(r, shapes[k]);
     }
   };
