@@ -669,15 +669,18 @@ void fused_forward_step(mgd_region* r, double* q0, double* x, double* y, double*
   }
 }
 
-// Autotuner: times one forward and one adjoint RK4 step (8 stage launches) with each
+// Autotuner: times one forward and one adjoint RK4 step (8 stage launches, the adjoint
+// ones shared with recompute stages as in a run) with each
 // tile shape that fits the grid and keeps the fastest.  J, the sensitivity and the
 // gradient do not depend on the choice (per-point quadrature terms, fixed-order sums),
-// nor does any rhs value.  Uses scratch buffers only (qs, qy, ly, l0, b2); the conserved
-// and adjoint states are not touched.
+// nor does any rhs value.  Uses scratch buffers only (qs, qy, ly, l0, b2, qsr, b2r); the
+// conserved and adjoint states are not touched.
 int tune_tiling(mgd_region* r) {
   r->tuned = true;
   const long M = (long)r->ninst * M_of(r);
-  if (r->qs.alloc(M) || r->qy.alloc(M) || r->ly.alloc(M) || r->l0.alloc(M)) return MGD_ERR_HIP;
+  if (r->qs.alloc(M) || r->qy.alloc(M) || r->ly.alloc(M) || r->l0.alloc(M) || r->qsr.alloc(M) ||
+      r->b2r.alloc(M))
+    return MGD_ERR_HIP;
   std::vector<double> amp;
   source_table(r, std::vector<double>{0.0, 0.5e-3, 0.5e-3, 1e-3}, amp);
   if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
@@ -692,7 +695,9 @@ int tune_tiling(mgd_region* r) {
       a.lout = i == 4 ? r->ly.p : (i == 3 ? r->qy.p : (i == 2 ? r->ly.p : r->l0.p));
       a.l0 = r->l0.p; a.q = r->qs.p; a.b2 = r->b2.p;
       a.stage = i; a.dt = dt; a.forcing_factor = 1.0; a.ip_act = r->ip_act;
-      launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+      // as in an adjoint run: each adjoint stage shares its launch with a recompute stage
+      const FwdStage f = fwd_args(r, r->qs.p, r->qs.p, r->b2r.p, r->qsr.p, i, dt, 0, 1, false);
+      launch_dual_stage2d(r->geo, dpatches(r), r->tiling, a, f, r->stream);
     }
   };
   // three interleaved rounds over the shapes; each shape keeps its fastest round
