@@ -99,6 +99,10 @@ struct mgd_region {
   bool tiling_ok = false;
   int tile_shape = -1;                   // MGD_OPT_TILE_SHAPE: -1 autotune, else fixed
   bool tuned = false;
+  // the fused adjoint run shares each recompute stage's launch with an adjoint stage
+  // (k_dual_stage2d) or launches them one after the other (the autotuner's choice; with a
+  // fixed tile shape: shared for one instance, where single launches leave the GPU half idle)
+  bool dual = true;
   int opt_zchunk = 0;                    // MGD_OPT_Z_CHUNK: 0 autotune, else 32, 64 or 128
   bool zc_tuned = false;
   int force_plain = 0;
@@ -686,7 +690,10 @@ int tune_tiling(mgd_region* r) {
   if (upload(r->src_amp, amp)) return MGD_ERR_HIP;
   if (r->control.p) r->control.alloc(0);
   const double dt = 1e-3;
-  auto step_pair = [&]() {
+  // one forward step, then one adjoint step with one recompute step as an adjoint run
+  // issues them: each adjoint stage sharing its launch with a recompute stage (dual) or
+  // the two launched one after the other
+  auto step_pair = [&](bool dual) {
     fused_forward_step(r, r->qs.p, r->qy.p, r->ly.p, r->b2.p, 0, dt, 1, false, nullptr);
     for (int i = 4; i >= 1; --i) {
       AdjStage a;
@@ -695,39 +702,50 @@ int tune_tiling(mgd_region* r) {
       a.lout = i == 4 ? r->ly.p : (i == 3 ? r->qy.p : (i == 2 ? r->ly.p : r->l0.p));
       a.l0 = r->l0.p; a.q = r->qs.p; a.b2 = r->b2.p;
       a.stage = i; a.dt = dt; a.forcing_factor = 1.0; a.ip_act = r->ip_act;
-      // as in an adjoint run: each adjoint stage shares its launch with a recompute stage
       const FwdStage f = fwd_args(r, r->qs.p, r->qs.p, r->b2r.p, r->qsr.p, i, dt, 0, 1, false);
-      launch_dual_stage2d(r->geo, dpatches(r), r->tiling, a, f, r->stream);
+      if (dual) {
+        launch_dual_stage2d(r->geo, dpatches(r), r->tiling, a, f, r->stream);
+      } else {
+        launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+        launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, f, r->stream);
+      }
     }
   };
-  // three interleaved rounds over the shapes; each shape keeps its fastest round
-  float best_ms[kNumTileShapes];
+  // three interleaved rounds over (shape, dual or not); each keeps its fastest round
+  float best_ms[kNumTileShapes][2];
   bool fits[kNumTileShapes];
-  for (int sh = 0; sh < kNumTileShapes; ++sh) { best_ms[sh] = 0.f; fits[sh] = false; }
+  for (int sh = 0; sh < kNumTileShapes; ++sh) { best_ms[sh][0] = best_ms[sh][1] = 0.f; fits[sh] = false; }
   for (int round = 0; round < 3; ++round) {
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
       if (round > 0 && !fits[sh]) continue;
       const int rc = install_tiling(r, sh);
       if (rc == MGD_ERR_UNSUPPORTED) continue;
       if (rc != MGD_OK) return rc;
-      launch_copy(M, r->q.p, r->qs.p, r->stream);
-      launch_copy(M, r->q.p, r->l0.p, r->stream);
-      step_pair();   // warm-up
-      MGD_HIP(hipEventRecord(r->ev0, r->stream));
-      for (int k = 0; k < 2; ++k) step_pair();
-      MGD_HIP(hipEventRecord(r->ev1, r->stream));
-      MGD_HIP(hipGetLastError());
-      MGD_HIP(hipEventSynchronize(r->ev1));
-      float ms = 0.f;
-      MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
-      if (!fits[sh] || ms < best_ms[sh]) best_ms[sh] = ms;
+      for (int dual = 0; dual < 2; ++dual) {
+        launch_copy(M, r->q.p, r->qs.p, r->stream);
+        launch_copy(M, r->q.p, r->l0.p, r->stream);
+        step_pair(dual);   // warm-up
+        MGD_HIP(hipEventRecord(r->ev0, r->stream));
+        for (int k = 0; k < 2; ++k) step_pair(dual);
+        MGD_HIP(hipEventRecord(r->ev1, r->stream));
+        MGD_HIP(hipGetLastError());
+        MGD_HIP(hipEventSynchronize(r->ev1));
+        float ms = 0.f;
+        MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+        if (!fits[sh] || ms < best_ms[sh][dual]) best_ms[sh][dual] = ms;
+      }
       fits[sh] = true;
     }
   }
-  int best = -1;
+  int best = -1, best_dual = 1;
   for (int sh = 0; sh < kNumTileShapes; ++sh)
-    if (fits[sh] && (best < 0 || best_ms[sh] < best_ms[best])) best = sh;
+    for (int dual = 0; dual < 2; ++dual)
+      if (fits[sh] && (best < 0 || best_ms[sh][dual] < best_ms[best][best_dual])) {
+        best = sh;
+        best_dual = dual;
+      }
   if (best < 0) { set_error("no tile shape fits the grid"); return MGD_ERR_UNSUPPORTED; }
+  r->dual = best_dual != 0;
   return install_tiling(r, best);
 }
 
@@ -905,8 +923,8 @@ int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
   } else {
     // window by window from the last one (its stage states written by the forward pass,
     // or recomputed first); each adjoint stage of window w shares its launch with the
-    // matching stage of the recompute of window w-1 (the other window buffer), the
-    // recompute's stages left over running alone
+    // matching stage of the recompute of window w-1 (the other window buffer) or is
+    // followed by it, the recompute's stages left over running alone
     if (!last_window) {
       launch_copy(M, r->snaps.p + (long)(nw - 1) * M, wbuf[buf(nw - 1)], r->stream);
       for (int j = 0; j < recompute_stages(nw - 1); ++j)
@@ -920,8 +938,12 @@ int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
       for (int t = t1w - 1; t >= t0w; --t)
         for (int i = 4; i >= 1; --i) {
           const AdjStage a = adj_args(wbuf[buf(w)], t0w, t, i);
-          if (j < nrec) launch_dual_stage2d(r->geo, dpatches(r), r->tiling, a, recompute_args(w - 1, j++), r->stream);
-          else launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+          if (j < nrec && r->dual) {
+            launch_dual_stage2d(r->geo, dpatches(r), r->tiling, a, recompute_args(w - 1, j++), r->stream);
+          } else {
+            launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+            if (j < nrec) launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, recompute_args(w - 1, j++), r->stream);
+          }
           reduce_chunk(r, r->spart, na, nst, 4 * t + i - 1, true);
         }
       for (; j < nrec; ++j)
@@ -1200,7 +1222,7 @@ std::vector<uintptr_t> graph_key(mgd_region* r, int kind, std::initializer_list<
   std::vector<uintptr_t> k = {(uintptr_t)kind, (uintptr_t)r->tiling.shape, (uintptr_t)fused_ok(r),
                               (uintptr_t)r->ctrl_stride, (uintptr_t)r->src_inst_stride,
                               (uintptr_t)r->chunk, (uintptr_t)r->ck.nsteps, (uintptr_t)r->ck.K,
-                              (uintptr_t)r->resident};
+                              (uintptr_t)r->resident, (uintptr_t)r->dual};
   const DevBuf* bufs[] = {&r->q, &r->lam, &r->rhs, &r->b1, &r->b2, &r->b1a, &r->b2a, &r->t, &r->w,
                           &r->G, &r->T1, &r->S, &r->qs, &r->aforce, &r->control, &r->grad,
                           &r->acc_J, &r->acc_S, &r->partial, &r->src_amp, &r->snaps, &r->window,
@@ -1529,6 +1551,7 @@ int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region
   r->nd = d->nd;
   r->nv = d->nd + 2;
   r->ninst = std::max(1, d->n_instances);
+  r->dual = r->ninst == 1;
   for (int i = d->nd; i < 3; ++i) { r->desc.n[i] = 1; r->desc.periodicity[i] = MGD_PERIODIC_NONE; }
   for (int i = 0; i < d->nd; ++i)
     if (r->desc.periodicity[i] != MGD_PERIODIC_NONE && r->desc.n[i] < 9) {
