@@ -574,7 +574,7 @@ def main():
     for _ in range(max(1, a.warmup)):
         K.one()
     t_step, J, t_fwd, launches, with_copy, march_ms = K.timed(a.steps)
-    zchunk = K.R.z_chunk
+    march_shape = K.R.march_shape
     value = K.pts / t_step
     print(f"headline: {t_step:.3f} s per pass ({value:.4g} grid-pt-steps/s)", file=sys.stderr,
           flush=True)
@@ -615,7 +615,8 @@ def main():
                                f"kinetic energy and dJ/dq(T) on the device",
                    "grid": [n, n, n], "nsteps": a.c4_nsteps,
                    "save_interval": max(1, a.c4_nsteps // 8), "instances": 1,
-                   "z_chunk": zchunk,
+                   "march_shape": {"forward": list(march_shape[0]),
+                                   "adjoint": list(march_shape[1])},
                    "path": "3-D kernels: stress pass + z-march forward; adjoint stress pass + "
                            "Jacobian march + finishing march"},
         "roofline": roofline,

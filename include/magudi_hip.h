@@ -312,18 +312,24 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches);
  * n > 0: at most n.
  * MGD_OPT_Z_CHUNK: planes per work unit of the 3-D z marches (periodic 3-D regions): 0
  * (default) autotune at the first run -- one forward RK4 step and one adjoint rhs timed
- * for each of 32, 64 and 128 --, or one of those fixed.  No result depends on it. */
+ * for each of 32, 64 and 128 --, or one of those fixed.  MGD_OPT_MARCH_TILE: the width of
+ * their tiles, 16 (16 x 16 points, 256 threads) or 32 (32 x 16, 512 threads; x sizes that
+ * are multiples of 32), 0 (default) autotuned with the z chunk; the forward march and the
+ * adjoint marches are tuned apart.  No result depends on either. */
 #define MGD_OPT_FORCE_PLAIN 1
 #define MGD_OPT_TILE_SHAPE 2
 #define MGD_OPT_RESIDENT_STATES 3
 #define MGD_OPT_WINDOW_CACHE 4
 #define MGD_OPT_Z_CHUNK 5
+#define MGD_OPT_MARCH_TILE 6
 int mgd_region_set_option(mgd_region* r, int option, int value);
 
 /* Tile shape the fused 2-D kernels use now (-1: the region runs the generic kernels). */
 int mgd_region_get_tile_shape(mgd_region* r, int* shape);
-/* Z chunk the 3-D marches use now (MGD_OPT_Z_CHUNK; after the autotuner, if it ran). */
-int mgd_region_get_z_chunk(mgd_region* r, int* zchunk);
+/* Work-unit shape the 3-D marches use now (MGD_OPT_MARCH_TILE, MGD_OPT_Z_CHUNK; after the
+ * autotuner, if it ran): shape[0..1] tile width and z chunk of the forward march and the
+ * stress pass, shape[2..3] those of the adjoint marches. */
+int mgd_region_get_march_shape(mgd_region* r, int* shape);
 
 /* Times one kernel on the region stream with HIP events after one warm-up repetition.
  * which: MGD_KERNEL_FWD_STAGE (fused 2-D forward stage kernel, `reps` RK4 steps of 4

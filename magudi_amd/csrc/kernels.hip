@@ -1567,6 +1567,55 @@ __global__ void __launch_bounds__(kBlock) k_fwd_finish_z(Geo g, const double* __
   }
 }
 
+// XCD-aware (tile, z chunk) work unit of block b of a 3-D march: units are dealt to the
+// 8 XCDs round-robin by block index, so block b takes unit (b mod 8) * ceil(units / 8) +
+// b / 8, tiles x-fastest: each XCD marches one contiguous band, and the halo lines of
+// neighbouring tiles are fetched into its own L2 by blocks running beside it.  False for
+// the padding blocks of a grid rounded up to a multiple of 8.
+template <int BX, int BY, int ZC>
+__device__ __forceinline__ bool march_unit(const Geo& g, int& x0, int& y0, int& z0) {
+  const int ntx = g.n[0] / BX, ntiles = ntx * (g.n[1] / BY);
+  const int nunits = ntiles * ((g.n[2] + ZC - 1) / ZC);
+  const int per_xcd = (nunits + 7) / 8;
+  const int unit = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (unit >= nunits) return false;
+  const int tile = unit % ntiles;
+  x0 = (tile % ntx) * BX;
+  y0 = (tile / ntx) * BY;
+  z0 = (unit / ntiles) * ZC;
+  return true;
+}
+
+// The x / y halo point of thread t of a BX x BY tile (at most one per thread, 8 (BX + BY)
+// of them): [0, 8 BY) the x-halo columns, [8 BY, 8 (BX + BY)) the y-halo rows, none for
+// the threads after them.  (hx, hy) is its place in the haloed tile, (hi, hj) its grid
+// indices; hmask 1 for an x-halo point, 2 for a y-halo point, 0 for none (then hi, hj are
+// the thread's own point, so that every index stays in the grid).
+template <int BX, int BY>
+__device__ __forceinline__ void march_halo(const Geo& g, int t, int x0, int y0, int& hx, int& hy,
+                                           int& hi, int& hj, int& hmask) {
+  if (t < 8 * BY) {
+    const int c = t % 8;
+    hy = t / 8;
+    hx = c < 4 ? c : BX + c;
+    hi = wrap(x0 + hx - 4, g.dir[0]);
+    hj = y0 + hy;
+    hmask = 1;
+  } else if (t < 8 * (BX + BY)) {
+    const int t2 = t - 8 * BY, r = t2 / BX;
+    hx = t2 % BX;
+    hy = r < 4 ? r : BY + r;
+    hi = x0 + hx;
+    hj = wrap(y0 + hy - 4, g.dir[1]);
+    hmask = 2;
+  } else {
+    hx = hy = 0;
+    hi = x0 + t % BX;
+    hj = y0 + t / BX;
+    hmask = 0;
+  }
+}
+
 // The whole forward rhs of a 3-D region periodic in every direction, in one z march
 // (computeRhsForward, src/RhsHelperImpl.f90:254-354, with k_finish's operations for at
 // most one whole-domain Kolmogorov forcing and no dissipation or sources): a BX x BY
@@ -1582,7 +1631,7 @@ __global__ void __launch_bounds__(kBlock) k_fwd_finish_z(Geo g, const double* __
 // finishing operations are those of k_fluxdiv3d_xy and k_fwd_finish_z, in the same
 // order, so the values are the same; no flux or partial rhs goes through HBM.
 template <int BX, int BY, int ZC>
-__global__ void __launch_bounds__(BX * BY, 2) k_fwd3d_march(Geo g, const double* __restrict__ q,
+__global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_fwd3d_march(Geo g, const double* __restrict__ q,
                                                           const double* __restrict__ S,
                                                           const double* __restrict__ kstr,
                                                           double* __restrict__ rhs, int rk_stage,
@@ -1591,22 +1640,12 @@ __global__ void __launch_bounds__(BX * BY, 2) k_fwd3d_march(Geo g, const double*
                                                           double* __restrict__ b2,
                                                           double* __restrict__ copy) {
   constexpr int NV = 5, NT = BX * BY, EXW = BX + 8, EYW = BY + 8;
-  static_assert(8 * (BX + BY) == NT, "one halo point per thread");
+  static_assert(8 * (BX + BY) <= NT, "at most one halo point per thread");
   __shared__ double sfx[NV][BY][EXW];
   __shared__ double sfy[NV][EYW][BX];
   __shared__ double ring[4][NV][NT];
-  // XCD-aware block map: work-groups are dealt to the 8 XCDs round-robin by block index,
-  // so block b takes work unit (b mod 8) * ceil(units / 8) + b / 8 of the (tile, z chunk)
-  // units in x-fastest tile order: each XCD marches one contiguous band, and the halo
-  // lines of neighbouring tiles are fetched into its own L2 by blocks running beside it
-  const int ntx = g.n[0] / BX, ntiles = ntx * (g.n[1] / BY);
-  const int nunits = ntiles * ((g.n[2] + ZC - 1) / ZC);
-  const int per_xcd = (nunits + 7) / 8;
-  const int unit = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-  if (unit >= nunits) return;   // padding of the grid to a multiple of 8 (whole block)
-  const int tile = unit % ntiles;
-  const int x0 = (tile % ntx) * BX, y0 = (tile / ntx) * BY;
-  const int z0 = (unit / ntiles) * ZC;
+  int x0, y0, z0;
+  if (!march_unit<BX, BY, ZC>(g, x0, y0, z0)) return;
   const int z1 = min(z0 + ZC, g.n[2]);
   const long inst = blockIdx.z;
   const double* qi = q + inst * NV * g.N;
@@ -1614,23 +1653,8 @@ __global__ void __launch_bounds__(BX * BY, 2) k_fwd3d_march(Geo g, const double*
   const long n01 = (long)g.n[0] * g.n[1];
   const int t = threadIdx.x, tx = t % BX, ty = t / BX;
   const int i = x0 + tx, j = y0 + ty;
-  // this thread's halo point: [0, 8 BY) the x-halo columns, [8 BY, NT) the y-halo rows
   int hx, hy, hi, hj, hmask;
-  if (t < 8 * BY) {
-    const int c = t % 8;
-    hy = t / 8;
-    hx = c < 4 ? c : BX + c;
-    hi = wrap(x0 + hx - 4, g.dir[0]);
-    hj = y0 + hy;
-    hmask = 1;
-  } else {
-    const int t2 = t - 8 * BY, r = t2 / BX;
-    hx = t2 % BX;
-    hy = r < 4 ? r : BY + r;
-    hi = x0 + hx;
-    hj = wrap(y0 + hy - 4, g.dir[1]);
-    hmask = 2;
-  }
+  march_halo<BX, BY>(g, t, x0, y0, hx, hy, hi, hj, hmask);
   // operands of one point: q and, viscous, the stress / heat-flux slots the fluxes of the
   // directions in dmask read (tau is symmetric: slot order xx xy xz yy yz zz, then the
   // heat flux x y z); slots not needed are not loaded
@@ -1694,13 +1718,13 @@ __global__ void __launch_bounds__(BX * BY, 2) k_fwd3d_march(Geo g, const double*
     fluxes(i, j, kz, xy ? 7 : 4, ct, f);
 #pragma unroll
     for (int l = 0; l < NV; ++l) W[8][l] = f[2][l];
-    if (xy) fluxes(hi, hj, kz, hmask, ch, fh);
+    if (xy && hmask) fluxes(hi, hj, kz, hmask, ch, fh);
     // this plane's operands are spent: load the next plane's now
     const int zn = zl + 1;
     const bool nxy = zn >= z0 && zn < z1;
     if (zn < z1 + 4) {
       load_pt(i, j, wrap(zn, dz), nxy ? 7 : 4, nt);
-      if (nxy) load_pt(hi, hj, wrap(zn, dz), hmask, nh);
+      if (nxy && hmask) load_pt(hi, hj, wrap(zn, dz), hmask, nh);
     }
     if (zl >= z0 + 4) {
       const int zo = zl - 4;
@@ -1765,7 +1789,7 @@ __global__ void __launch_bounds__(BX * BY, 2) k_fwd3d_march(Geo g, const double*
       if (hmask == 1) {
 #pragma unroll
         for (int l = 0; l < NV; ++l) sfx[l][hy][hx] = fh[0][l];
-      } else {
+      } else if (hmask == 2) {
 #pragma unroll
         for (int l = 0; l < NV; ++l) sfy[l][hy][hx] = fh[1][l];
       }
@@ -1791,7 +1815,7 @@ __global__ void __launch_bounds__(BX * BY, 2) k_fwd3d_march(Geo g, const double*
 #pragma unroll
       for (int l = 0; l < NV; ++l) W[k][l] = W[k + 1][l];
     ct = nt;
-    if (nxy) ch = nh;
+    if (nxy && hmask) ch = nh;
   }
 }
 
@@ -1897,49 +1921,6 @@ __global__ void __launch_bounds__(kBlock) k_adj_finish_z(Geo g, const double* __
   }
 }
 
-// XCD-aware (tile, z chunk) work unit of block b of a 3-D march: units are dealt to the
-// 8 XCDs round-robin by block index, so block b takes unit (b mod 8) * ceil(units / 8) +
-// b / 8, tiles x-fastest: each XCD marches one contiguous band, and the halo lines of
-// neighbouring tiles are fetched into its own L2 by blocks running beside it.  False for
-// the padding blocks of a grid rounded up to a multiple of 8.
-template <int BX, int BY, int ZC>
-__device__ __forceinline__ bool march_unit(const Geo& g, int& x0, int& y0, int& z0) {
-  const int ntx = g.n[0] / BX, ntiles = ntx * (g.n[1] / BY);
-  const int nunits = ntiles * ((g.n[2] + ZC - 1) / ZC);
-  const int per_xcd = (nunits + 7) / 8;
-  const int unit = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-  if (unit >= nunits) return false;
-  const int tile = unit % ntiles;
-  x0 = (tile % ntx) * BX;
-  y0 = (tile / ntx) * BY;
-  z0 = (unit / ntiles) * ZC;
-  return true;
-}
-
-// The x / y halo point of thread t of a BX x BY tile (8 (BX + BY) threads, one halo
-// point each): [0, 8 BY) the x-halo columns, [8 BY, NT) the y-halo rows.  (hx, hy) is its
-// place in the haloed tile, (hi, hj) its grid indices; hmask 1 for an x-halo point, 2 for
-// a y-halo point.
-template <int BX, int BY>
-__device__ __forceinline__ void march_halo(const Geo& g, int t, int x0, int y0, int& hx, int& hy,
-                                           int& hi, int& hj, int& hmask) {
-  if (t < 8 * BY) {
-    const int c = t % 8;
-    hy = t / 8;
-    hx = c < 4 ? c : BX + c;
-    hi = wrap(x0 + hx - 4, g.dir[0]);
-    hj = y0 + hy;
-    hmask = 1;
-  } else {
-    const int t2 = t - 8 * BY, r = t2 / BX;
-    hx = t2 % BX;
-    hy = r < 4 ? r : BY + r;
-    hi = x0 + hx;
-    hj = wrap(y0 + hy - 4, g.dir[1]);
-    hmask = 2;
-  }
-}
-
 // Element access at a 32-bit byte offset from a wave-uniform base pointer: the address
 // form of a global load / store with a scalar base and a 32-bit vector offset, so a
 // thread keeps one 32-bit offset per point instead of a 64-bit address per component.
@@ -1966,13 +1947,13 @@ __device__ __forceinline__ void st8(double* base, unsigned off, double v) {
 // one plane ahead.  The window of the last NZL components lives in an LDS ring of 9
 // planes (each thread reads back only its own column), the others in registers.
 template <int BX, int BY, int ZC, bool POWER>
-__global__ void __launch_bounds__(BX * BY, 2) k_adj3d_jac_march(Geo g, const double* __restrict__ q,
+__global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_adj3d_jac_march(Geo g, const double* __restrict__ q,
                                                               const double* __restrict__ lam,
                                                               const double* __restrict__ S,
                                                               double* __restrict__ rhs,
                                                               double* __restrict__ T1) {
   constexpr int NV = 5, NW = 4, EXW = BX + 8, NT = BX * BY, NZL = 2, NZR = NV - NZL;
-  static_assert(8 * (BX + BY) == NT, "one halo point per thread");
+  static_assert(8 * (BX + BY) <= NT, "at most one halo point per thread");
   __shared__ double slx[NV][BY][EXW];
   __shared__ double sly[NV][BY + 8][BX];
   __shared__ double zring[9][NZL][NT];
@@ -2011,9 +1992,9 @@ __global__ void __launch_bounds__(BX * BY, 2) k_adj3d_jac_march(Geo g, const dou
 #pragma unroll
     for (int l = NZR; l < NV; ++l) zring[k][l - NZR][t] = v[l];
   }
-  double nl[NV], nh[NV];
+  double nl[NV], nh[NV] = {};
   load_l(col8 + plane8 * (unsigned)wrap(z0 + 4, dz), nl);
-  load_l(hcol8 + plane8 * (unsigned)z0, nh);
+  if (hmask) load_l(hcol8 + plane8 * (unsigned)z0, nh);
   int rb = 0;                            // ring slot of plane zo - 4
   for (int zo = z0; zo < z1; ++zo) {     // uniform over the block
     double hv[NV], wc[NV];
@@ -2029,7 +2010,7 @@ __global__ void __launch_bounds__(BX * BY, 2) k_adj3d_jac_march(Geo g, const dou
     for (int l = 0; l < NV; ++l) wc[l] = l < NZR ? W[4][l] : zring[r4][l - NZR][t];
     if (zo + 1 < z1) {
       load_l(col8 + plane8 * (unsigned)wrap(zo + 5, dz), nl);
-      load_l(hcol8 + plane8 * (unsigned)(zo + 1), nh);
+      if (hmask) load_l(hcol8 + plane8 * (unsigned)(zo + 1), nh);
     }
     const unsigned p8 = col8 + plane8 * (unsigned)zo;
     double v[NV], sv[9];
@@ -2046,7 +2027,7 @@ __global__ void __launch_bounds__(BX * BY, 2) k_adj3d_jac_march(Geo g, const dou
     if (hmask == 1) {
 #pragma unroll
       for (int l = 0; l < NV; ++l) slx[l][hy][hx] = hv[l];
-    } else {
+    } else if (hmask == 2) {
 #pragma unroll
       for (int l = 0; l < NV; ++l) sly[l][hy][hx] = hv[l];
     }
@@ -2121,12 +2102,12 @@ __global__ void __launch_bounds__(BX * BY, 2) k_adj3d_jac_march(Geo g, const dou
 // written.  The sums and their order are those of k_deriv_row, k_deriv_line (x store, y
 // add) and k_adj_finish_z, so the values are the same.
 template <int BX, int BY, int ZC>
-__global__ void __launch_bounds__(BX * BY, 2) k_adj3d_finish_march(
+__global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_adj3d_finish_march(
     Geo g, const double* __restrict__ T1, const double* __restrict__ q, const double* lam,
     const double* __restrict__ kstr, double* __restrict__ rhs, int rk_stage, double dt, double* y,
     double* b1, double* b2) {
   constexpr int NV = 5, NW = 4, EXW = BX + 8;
-  static_assert(8 * (BX + BY) == BX * BY, "one halo point per thread");
+  static_assert(8 * (BX + BY) <= BX * BY, "at most one halo point per thread");
   __shared__ double snx[NW][BY][EXW];
   __shared__ double sny[NW][BY + 8][BX];
   int x0, y0, z0;
@@ -2158,11 +2139,11 @@ __global__ void __launch_bounds__(BX * BY, 2) k_adj3d_finish_march(
   double Z[9][NW];
 #pragma unroll
   for (int k = 0; k < 8; ++k) load4(Tz, col8 + plane8 * (unsigned)wrap(z0 - 4 + k, dz), Z[k]);
-  double nz[NW], nx[NW], ny[NW], nh[NW];
+  double nz[NW], nx[NW], ny[NW], nh[NW] = {};
   load4(Tz, col8 + plane8 * (unsigned)wrap(z0 + 4, dz), nz);
   load4(Tx, col8 + plane8 * (unsigned)z0, nx);
   load4(Ty, col8 + plane8 * (unsigned)z0, ny);
-  load4(Th, hcol8 + plane8 * (unsigned)z0, nh);
+  if (hmask) load4(Th, hcol8 + plane8 * (unsigned)z0, nh);
   for (int zo = z0; zo < z1; ++zo) {     // uniform over the block
     double xv[NW], yv[NW], hv[NW];
 #pragma unroll
@@ -2177,7 +2158,7 @@ __global__ void __launch_bounds__(BX * BY, 2) k_adj3d_finish_march(
       load4(Tz, col8 + plane8 * (unsigned)wrap(zo + 5, dz), nz);
       load4(Tx, n8, nx);
       load4(Ty, n8, ny);
-      load4(Th, hcol8 + plane8 * (unsigned)(zo + 1), nh);
+      if (hmask) load4(Th, hcol8 + plane8 * (unsigned)(zo + 1), nh);
     }
     const unsigned p8 = col8 + plane8 * (unsigned)zo;
     double v[NV], r[NV];
@@ -2195,7 +2176,7 @@ __global__ void __launch_bounds__(BX * BY, 2) k_adj3d_finish_march(
     if (hmask == 1) {
 #pragma unroll
       for (int a = 0; a < NW; ++a) snx[a][hy][hx] = hv[a];
-    } else {
+    } else if (hmask == 2) {
 #pragma unroll
       for (int a = 0; a < NW; ++a) sny[a][hy][hx] = hv[a];
     }
@@ -2380,17 +2361,35 @@ static bool fluxdiv3d_applies(const Geo& g) {
 }
 
 // grid of the periodic gradient kernel: (tile, z chunk) units padded to a multiple of 8
-// The z chunk (planes per work unit) of the 3-D marches, g.zchunk: 32, 64 or 128 (the
-// autotuner's choice; no value depends on it), f called with it as a compile-time value.
+// The z chunk (planes per work unit) of the 3-D marches, g.zchunk: 32, 64 or 128, and the
+// x extent of their tiles, g.mtile: 16 (16 x 16 tiles of 256 threads, two work-groups
+// per CU) or 32 (32 x 16 tiles of 512 threads, one per CU: fewer halo points per point) --
+// the autotuner's choices; no value depends on them.  f is called with them as
+// compile-time values.
+// The forward march and the stress pass use (g.zchunk, g.mtile), the adjoint marches
+// (g.zchunk_adj, g.mtile_adj).
+static int zchunk_of(const Geo& g, bool adj = false) {
+  const int z = adj ? g.zchunk_adj : g.zchunk;
+  return z == 32 || z == 128 ? z : 64;
+}
+static int mtile_of(const Geo& g, bool adj = false) {
+  return (adj ? g.mtile_adj : g.mtile) == 32 && g.n[0] % 32 == 0 ? 32 : 16;
+}
 template <class F>
-static void with_zchunk(const Geo& g, F&& f) {
-  switch (g.zchunk) {
+static void with_zchunk(const Geo& g, bool adj, F&& f) {
+  switch (zchunk_of(g, adj)) {
     case 32: f(std::integral_constant<int, 32>{}); break;
     case 128: f(std::integral_constant<int, 128>{}); break;
     default: f(std::integral_constant<int, 64>{}); break;
   }
 }
-static int zchunk_of(const Geo& g) { return g.zchunk == 32 || g.zchunk == 128 ? g.zchunk : 64; }
+template <class F>
+static void with_march(const Geo& g, bool adj, F&& f) {
+  with_zchunk(g, adj, [&](auto Z) {
+    if (mtile_of(g, adj) == 32) f(std::integral_constant<int, 32>{}, Z);
+    else f(std::integral_constant<int, 16>{}, Z);
+  });
+}
 
 static dim3 grad3d_grid(const Geo& g) {
   const int zc = zchunk_of(g);
@@ -2402,7 +2401,7 @@ template <int ND>
 static void launch_gradients(const Geo& g, const double* q, double* w, double* G, const dim3& grid,
                              hipStream_t st) {
   if (ND == 3 && grad3d_applies(g)) {
-    with_zchunk(g, [&](auto Z) {
+    with_zchunk(g, false, [&](auto Z) {
       k_grad3d_periodic<kG3X, kG3Y, decltype(Z)::value><<<grad3d_grid(g), kG3X * kG3Y, 0, st>>>(g, q, G);
     });
     return;
@@ -2415,9 +2414,9 @@ static void launch_gradients(const Geo& g, const double* q, double* w, double* G
 // of its 16 x 16 tile; viscous regions also need the stress kernel (grad3d_applies).
 constexpr int kFMX = 16, kFMY = 16;
 // grid of the 3-D march kernels: (tile, z chunk) units padded to a multiple of 8
-static dim3 march_grid(const Geo& g) {
-  const int zc = zchunk_of(g);
-  const int units = (g.n[0] / kFMX) * (g.n[1] / kFMY) * ((g.n[2] + zc - 1) / zc);
+static dim3 march_grid(const Geo& g, bool adj) {
+  const int zc = zchunk_of(g, adj);
+  const int units = (g.n[0] / mtile_of(g, adj)) * (g.n[1] / kFMY) * ((g.n[2] + zc - 1) / zc);
   return dim3(8 * ((units + 7) / 8), 1, g.ninst);
 }
 static bool fwd3d_march_applies(const Geo& g) {
@@ -2439,7 +2438,7 @@ static bool adj3d_march_path(const Geo& g, const FinishInfo& fin) {
 
 void launch_stress3d(const Geo& g, const double* q, double* S, hipStream_t st) {
   const dim3 gg = grad3d_grid(g);
-  with_zchunk(g, [&](auto Z) {
+  with_zchunk(g, false, [&](auto Z) {
     if (g.power_law > 0.0)
       k_grad3d_periodic<kG3X, kG3Y, decltype(Z)::value, true, true><<<gg, kG3X * kG3Y, 0, st>>>(g, q, S);
     else
@@ -2449,10 +2448,11 @@ void launch_stress3d(const Geo& g, const double* q, double* S, hipStream_t st) {
 
 void launch_fwd3d_march(const Geo& g, const double* q, const double* S, const double* kstr,
                         double* rhs, const RkStage* rk, hipStream_t st) {
-  const dim3 gm = march_grid(g);
+  const dim3 gm = march_grid(g, false);
   if (rk && rk->ev_begin) (void)hipEventRecord(rk->ev_begin, st);
-  with_zchunk(g, [&](auto Z) {
-    k_fwd3d_march<kFMX, kFMY, decltype(Z)::value><<<gm, kFMX * kFMY, 0, st>>>(
+  with_march(g, false, [&](auto X, auto Z) {
+    constexpr int BX = decltype(X)::value;
+    k_fwd3d_march<BX, kFMY, decltype(Z)::value><<<gm, BX * kFMY, 0, st>>>(
         g, q, S, kstr, rhs, rk ? rk->stage : 0, rk ? rk->dt : 0.0, rk ? rk->yout : nullptr,
         rk ? rk->b1 : nullptr, rk ? rk->b2 : nullptr, rk ? rk->copy : nullptr);
   });
@@ -2535,13 +2535,14 @@ bool launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
     (void)w;
     (void)grid;
     launch_stress3d(g, q, G, st);
-    const dim3 gm = march_grid(g);
-    with_zchunk(g, [&](auto Z) {
+    const dim3 gm = march_grid(g, true);
+    with_march(g, true, [&](auto X, auto Z) {
+      constexpr int BX = decltype(X)::value, ZC = decltype(Z)::value;
       if (g.power_law != 0.0)
-        k_adj3d_jac_march<kFMX, kFMY, decltype(Z)::value, true><<<gm, kFMX * kFMY, 0, st>>>(g, q, lam, G, rhs, T1);
+        k_adj3d_jac_march<BX, kFMY, ZC, true><<<gm, BX * kFMY, 0, st>>>(g, q, lam, G, rhs, T1);
       else
-        k_adj3d_jac_march<kFMX, kFMY, decltype(Z)::value, false><<<gm, kFMX * kFMY, 0, st>>>(g, q, lam, G, rhs, T1);
-      k_adj3d_finish_march<kFMX, kFMY, decltype(Z)::value><<<gm, kFMX * kFMY, 0, st>>>(
+        k_adj3d_jac_march<BX, kFMY, ZC, false><<<gm, BX * kFMY, 0, st>>>(g, q, lam, G, rhs, T1);
+      k_adj3d_finish_march<BX, kFMY, ZC><<<gm, BX * kFMY, 0, st>>>(
           g, T1, q, lam, fin.kolmogorov, rhs, rk ? rk->stage : 0, rk ? rk->dt : 0.0,
           rk ? rk->y : nullptr, rk ? rk->b1 : nullptr, rk ? rk->b2 : nullptr);
     });

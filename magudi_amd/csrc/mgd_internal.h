@@ -64,7 +64,10 @@ struct Geo {
   const Op* ops;            // kNumSlots operators (direction independent)
   double gamma, re_inv, pr_inv, power_law, bulk_ratio, diss_amount;
   int viscous, dissipation, composite;
-  int zchunk;               // z planes per work unit of the 3-D marches (32, 64 or 128)
+  int zchunk;               // z planes per work unit of the forward 3-D march and the
+                            // stress pass (32, 64 or 128)
+  int mtile;                // x extent of the forward march tiles (16 or 32; 16 rows)
+  int zchunk_adj, mtile_adj;  // the same for the adjoint marches
 };
 
 // Device patch descriptor.

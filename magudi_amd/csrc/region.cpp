@@ -104,6 +104,7 @@ struct mgd_region {
   // fixed tile shape: shared for one instance, where single launches leave the GPU half idle)
   bool dual = true;
   int opt_zchunk = 0;                    // MGD_OPT_Z_CHUNK: 0 autotune, else 32, 64 or 128
+  int opt_mtile = 0;                     // MGD_OPT_MARCH_TILE: 0 autotune, else 16 or 32
   bool zc_tuned = false;
   int force_plain = 0;
   int aforce_zero = 0;
@@ -749,18 +750,21 @@ int tune_tiling(mgd_region* r) {
   return install_tiling(r, best);
 }
 
-// The z chunk of the 3-D marches (planes per work unit): one forward RK4 step (stress
-// pass and march per stage, out of place) and one adjoint rhs (stress pass and the two
-// marches, where they serve the region) timed with HIP events for each of 32, 64 and 128
-// planes, three interleaved rounds, the fastest round of each compared.  No value depends
-// on the chunk.
-int tune_zchunk(mgd_region* r) {
+// The shape of the 3-D marches' work units -- the z chunk (32, 64 or 128 planes) and the
+// tile width (16 or 32 points, 16 rows) -- unless fixed by option: one forward RK4 step
+// (stress pass and march per stage, out of place) and one adjoint rhs (stress pass and
+// the two marches, where they serve the region) timed with HIP events for each shape,
+// three interleaved rounds, the fastest round of each compared; the forward march (with
+// the stress pass) and the adjoint marches each keep their own.  No value depends on the
+// shape.
+int tune_march(mgd_region* r) {
   r->zc_tuned = true;
   const long M = (long)r->ninst * M_of(r);
   if (int rc = ensure_adjoint(r)) return rc;
   if (r->qs.alloc(M) || r->qy.alloc(M)) return MGD_ERR_HIP;
   const double dt = 1e-3;
-  auto round_trip = [&]() {
+  // the state copies, one forward RK4 step, then (after event `mid`, if given) one adjoint rhs
+  auto round_trip = [&](hipEvent_t mid = nullptr) {
     launch_copy(M, r->q.p, r->qs.p, r->stream);
     launch_copy(M, r->q.p, r->b1.p, r->stream);
     launch_copy(M, r->q.p, r->b2.p, r->stream);
@@ -771,38 +775,62 @@ int tune_zchunk(mgd_region* r) {
       if (r->geo.viscous) launch_stress3d(r->geo, rk.y, r->G.p, r->stream);
       launch_fwd3d_march(r->geo, rk.y, r->G.p, r->fin.kolmogorov, r->rhs.p, &rk, r->stream);
     }
+    if (mid) (void)hipEventRecord(mid, r->stream);
     launch_rhs_adjoint(r->geo, dpatches(r), r->qs.p, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
                        r->T1.p, r->S.p, nullptr, 0, 1.0, r->fin, r->stream);
   };
-  const int zcs[3] = {32, 64, 128};
-  float best[3] = {0.f, 0.f, 0.f};
-  for (int round = 0; round < 3; ++round)
-    for (int k = 0; k < 3; ++k) {
-      r->geo.zchunk = zcs[k];
+  std::vector<std::pair<int, int>> shapes;   // (tile width, z chunk)
+  for (int mt : {16, 32}) {
+    if (r->opt_mtile != 0 && mt != r->opt_mtile) continue;
+    if (mt == 32 && r->geo.n[0] % 32 != 0) continue;
+    for (int zc : {32, 64, 128})
+      if (r->opt_zchunk == 0 || zc == r->opt_zchunk) shapes.emplace_back(mt, zc);
+  }
+  if (shapes.empty()) shapes.emplace_back(16, 64);
+  // the forward step and the adjoint rhs are timed apart and each keeps its own shape
+  hipEvent_t evm = nullptr;
+  MGD_HIP(hipEventCreate(&evm));
+  std::vector<float> best_f(shapes.size(), 0.f), best_a(shapes.size(), 0.f);
+  int rc = MGD_OK;
+  for (int round = 0; round < 3 && rc == MGD_OK; ++round)
+    for (size_t k = 0; k < shapes.size(); ++k) {
+      r->geo.mtile = r->geo.mtile_adj = shapes[k].first;
+      r->geo.zchunk = r->geo.zchunk_adj = shapes[k].second;
       round_trip();   // warm-up
-      MGD_HIP(hipEventRecord(r->ev0, r->stream));
-      round_trip();
-      MGD_HIP(hipEventRecord(r->ev1, r->stream));
-      MGD_HIP(hipGetLastError());
-      MGD_HIP(hipEventSynchronize(r->ev1));
-      float ms = 0.f;
-      MGD_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
-      if (round == 0 || ms < best[k]) best[k] = ms;
+      float ms_f = 0.f, ms_a = 0.f;
+      if (hipEventRecord(r->ev0, r->stream) != hipSuccess) { rc = MGD_ERR_HIP; break; }
+      round_trip(evm);
+      if (hipEventRecord(r->ev1, r->stream) != hipSuccess || hipGetLastError() != hipSuccess ||
+          hipEventSynchronize(r->ev1) != hipSuccess ||
+          hipEventElapsedTime(&ms_f, r->ev0, evm) != hipSuccess ||
+          hipEventElapsedTime(&ms_a, evm, r->ev1) != hipSuccess) {
+        rc = MGD_ERR_HIP;
+        break;
+      }
+      if (round == 0 || ms_f < best_f[k]) best_f[k] = ms_f;
+      if (round == 0 || ms_a < best_a[k]) best_a[k] = ms_a;
     }
-  int kb = 1;
-  for (int k = 0; k < 3; ++k)
-    if (best[k] < best[kb]) kb = k;
-  r->geo.zchunk = zcs[kb];
+  (void)hipEventDestroy(evm);
+  if (rc != MGD_OK) { set_error("HIP error in the march autotuner"); return rc; }
+  size_t kf = 0, ka = 0;
+  for (size_t k = 1; k < shapes.size(); ++k) {
+    if (best_f[k] < best_f[kf]) kf = k;
+    if (best_a[k] < best_a[ka]) ka = k;
+  }
+  r->geo.mtile = shapes[kf].first;
+  r->geo.zchunk = shapes[kf].second;
+  r->geo.mtile_adj = shapes[ka].first;
+  r->geo.zchunk_adj = shapes[ka].second;
   return MGD_OK;
 }
 
 // Runs the autotuners once per region unless fixed by option: the tile shape of the fused
 // 2-D kernels, the z chunk of the 3-D marches.
 int maybe_tune(mgd_region* r) {
-  if (r->nd == 3 && !r->zc_tuned && r->opt_zchunk == 0 &&
+  if (r->nd == 3 && !r->zc_tuned && (r->opt_zchunk == 0 || r->opt_mtile == 0) &&
       fwd3d_march_path(r->geo, r->fin, (int)r->sources.size())) {
     if (int rc = ensure_state(r)) return rc;
-    if (int rc = tune_zchunk(r)) return rc;
+    if (int rc = tune_march(r)) return rc;
   }
   if (!fused_ok(r) || r->tuned || r->tile_shape >= 0) return MGD_OK;
   return tune_tiling(r);
@@ -1646,7 +1674,8 @@ int mgd_region_create(const mgd_region_desc* d, const double* coords, mgd_region
   g.viscous = d->viscous;
   g.dissipation = d->dissipation;
   g.composite = d->composite_dissipation;
-  g.zchunk = 64;
+  g.zchunk = g.zchunk_adj = 64;
+  g.mtile = g.mtile_adj = 16;
   if (int rc = scratch_setup(&r->device)) return rc;
   MGD_HIP(hipStreamCreate(&r->stream));
   MGD_HIP(hipEventCreate(&r->ev0));
@@ -1965,9 +1994,12 @@ int mgd_region_set_time_offsets(mgd_region* r, const double* offsets) {
   return MGD_OK;
 }
 
-int mgd_region_get_z_chunk(mgd_region* r, int* zchunk) {
-  if (!r || !zchunk) { set_error("null argument"); return MGD_ERR_ARG; }
-  *zchunk = r->geo.zchunk;
+int mgd_region_get_march_shape(mgd_region* r, int* shape) {
+  if (!r || !shape) { set_error("null argument"); return MGD_ERR_ARG; }
+  shape[0] = r->geo.mtile;
+  shape[1] = r->geo.zchunk;
+  shape[2] = r->geo.mtile_adj;
+  shape[3] = r->geo.zchunk_adj;
   return MGD_OK;
 }
 
@@ -1989,8 +2021,21 @@ int mgd_region_set_option(mgd_region* r, int option, int value) {
         return MGD_ERR_ARG;
       }
       r->opt_zchunk = value;
-      r->geo.zchunk = value == 0 ? 64 : value;
-      r->zc_tuned = value != 0;
+      r->geo.zchunk = r->geo.zchunk_adj = value == 0 ? 64 : value;
+      r->zc_tuned = value != 0 && r->opt_mtile != 0;
+      return MGD_OK;
+    case MGD_OPT_MARCH_TILE:
+      if (value != 0 && value != 16 && value != 32) {
+        set_error("march tile: 0 (autotune), 16 or 32");
+        return MGD_ERR_ARG;
+      }
+      if (value == 32 && r->nd == 3 && r->geo.n[0] % 32 != 0) {
+        set_error("march tile 32 needs an x size that is a multiple of 32");
+        return MGD_ERR_ARG;
+      }
+      r->opt_mtile = value;
+      r->geo.mtile = r->geo.mtile_adj = value == 0 ? 16 : value;
+      r->zc_tuned = value != 0 && r->opt_zchunk != 0;
       return MGD_OK;
     case MGD_OPT_TILE_SHAPE: {
       if (value < -1 || value >= kNumTileShapes) { set_error("tile shape must be -1 .. 5"); return MGD_ERR_ARG; }

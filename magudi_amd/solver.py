@@ -270,12 +270,28 @@ class Region:
         64, 128 fixes it."""
         check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_Z_CHUNK, int(zc)))
 
+    def set_march_tile(self, width: int):
+        """MGD_OPT_MARCH_TILE: 0 autotunes the 3-D marches' tile width with the z chunk, or
+        16 / 32 fixes it."""
+        check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_MARCH_TILE, int(width)))
+
+    @property
+    def march_shape(self):
+        """((tile width, z chunk) of the forward march, (tile width, z chunk) of the adjoint
+        marches) in use (after the autotuner, if it ran)."""
+        out = (C.c_int * 4)()
+        check(capi.lib().mgd_region_get_march_shape(self.handle, out))
+        return (out[0], out[1]), (out[2], out[3])
+
+    @property
+    def march_tile(self) -> int:
+        """The forward march's tile width in use."""
+        return self.march_shape[0][0]
+
     @property
     def z_chunk(self) -> int:
-        """The 3-D marches' z chunk in use (after the autotuner, if it ran)."""
-        z = C.c_int(0)
-        check(capi.lib().mgd_region_get_z_chunk(self.handle, C.byref(z)))
-        return z.value
+        """The forward march's z chunk in use."""
+        return self.march_shape[0][1]
 
     def kernel_timer(self, on: bool, which: int = None):
         """Starts (on) or stops per-launch HIP-event timing of the z-march forward kernel

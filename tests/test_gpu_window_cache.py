@@ -66,10 +66,10 @@ def test_window_cache_2d_generic_bit_for_bit(cache):
         assert np.array_equal(a, b)
 
 
-def test_z_chunk_changes_no_value():
-    """The 3-D marches' z chunk (32, 64, 128 planes per work unit, and the autotuner's
-    choice) on a 64 x 32 x 80 box (partial chunks for 32 and 64): forward and adjoint runs
-    bit for bit the same."""
+def test_march_shape_changes_no_value():
+    """The 3-D marches' work-unit shape -- z chunk 32, 64 or 128 planes, tiles 16 x 16 or
+    32 x 16 (threads without a halo point), and the autotuner's choice -- on a 64 x 32 x
+    80 box (partial chunks for 32 and 64): forward and adjoint runs bit for bit the same."""
     from magudi_amd.solver import Region, Solver
     from test_gpu_parity import periodic_box, perturbed
     c = periodic_box((64, 32, 80), dissipation=False)
@@ -77,8 +77,9 @@ def test_z_chunk_changes_no_value():
     q0 = perturbed(c, 51)
     lamT = np.random.default_rng(52).standard_normal((c.nv, c.npoints))
     out = []
-    for zc in (32, 64, 128, 0):
+    for mt, zc in ((16, 32), (16, 64), (16, 128), (32, 32), (32, 64), (32, 128), (0, 0)):
         R = Region(c)
+        R.set_march_tile(mt)
         R.set_z_chunk(zc)
         R.states[0].conservedVariables = q0
         S = Solver(R, dt=c.dt, nsteps=nsteps, save_interval=K)
@@ -86,7 +87,11 @@ def test_z_chunk_changes_no_value():
         qT = R.states[0].conservedVariables
         S.runAdjointCheckpointed(terminal=lamT, want_gradient=False)
         out.append((qT, R.states[0].adjointVariables))
-        assert R.z_chunk == (zc if zc else R.z_chunk) and R.z_chunk in (32, 64, 128)
+        fwd, adj = R.march_shape
+        assert fwd[0] in (16, 32) and fwd[1] in (32, 64, 128)
+        assert adj[0] in (16, 32) and adj[1] in (32, 64, 128)
+        if zc:
+            assert fwd == (mt, zc) and adj == (mt, zc)
     for qT, lam in out[1:]:
         assert np.array_equal(qT, out[0][0])
         assert np.array_equal(lam, out[0][1])
