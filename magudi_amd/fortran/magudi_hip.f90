@@ -194,6 +194,12 @@ module magudi_hip
        real(c_double), intent(out) :: sensitivity(*)
        integer(c_int) :: rc
      end function mgd_run_adjoint_checkpointed
+     function mgd_state_copy(r, src, dst) bind(C, name="mgd_state_copy") result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: r
+       integer(c_int), value :: src, dst
+       integer(c_int) :: rc
+     end function mgd_state_copy
      function mgd_state_lincomb(r, dst, dst_inst, a, a_inst, b, b_inst, s, c, c_inst)         &
           bind(C, name="mgd_state_lincomb") result(rc)
        import :: c_ptr, c_int, c_double
@@ -351,7 +357,7 @@ module magudi_hip
   end interface
 
   public :: mgd_run_forward_checkpointed, mgd_run_adjoint_checkpointed, mgd_state_lincomb,      &
-       mgd_inner_product, mgd_region_set_time_offsets
+       mgd_inner_product, mgd_region_set_time_offsets, mgd_state_copy
   public :: mgd_rk4_substep, mgd_checkpoint_migrate, mgd_control_gradient,                      &
        mgd_adjoint_initial_condition, mgd_update_adjoint_forcing, mgd_functional,               &
        mgd_region_set_solution_limits, mgd_check_solution_limits, mgd_time_step_size, mgd_sync

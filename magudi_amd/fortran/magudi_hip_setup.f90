@@ -17,6 +17,7 @@ module magudi_hip_setup
      character(len = OPTION_LENGTH) :: outputPrefix = "magudi"
      character(len = OPTION_LENGTH) :: actuatorName = ""
      integer :: nActuatorPoints = 0
+     integer :: gridSize(3) = 1                      ! the PLOT3D grid's size
      logical :: enableController = .false., enableFunctional = .false.
      logical :: controllerSwitch = .false.
      real(c_double), allocatable :: control(:,:)     ! (nActuatorPoints, 4 nTimesteps)
@@ -90,10 +91,17 @@ contains
          "mgd_plot3d_read_function " // trim(filename))
   end subroutine readFunction
 
-  subroutine setupFromInput(region, solver, run)
+  subroutine setupFromInput(region, solver, run, nInstances, loadInitialCondition,            &
+       nControlStages)
+    ! nInstances: independent copies of the state (the multiple-shooting segments);
+    ! loadInitialCondition = .false. leaves the conserved state to the caller;
+    ! nControlStages: stages of the control forcing file (default 4 number_of_timesteps)
     type(t_Region), intent(inout) :: region
     type(t_Solver), intent(inout) :: solver
     type(t_RunSetup), intent(inout) :: run
+    integer, intent(in), optional :: nInstances
+    logical, intent(in), optional :: loadInitialCondition
+    integer, intent(in), optional :: nControlStages
     type(mgd_region_desc) :: desc
     type(t_PatchRow), allocatable :: rows(:)
     character(len = OPTION_LENGTH) :: filename, key, val
@@ -101,7 +109,8 @@ contains
     integer(c_int), allocatable :: iblank(:)
     real(c_double), allocatable :: xyz(:,:), coordinates(:,:), target(:,:), f(:,:), state(:,:)
     real(c_double) :: aux(4), gamma, re, pr, params(4), loc(3), limits(4)
-    integer :: nd, nGridPoints, d, i, nv, nSources, code
+    integer :: nd, nGridPoints, d, i, nv, nSources, code, nStages
+    logical :: doLoad
 
     run%outputPrefix = getOption("output_prefix", "magudi")
 
@@ -155,6 +164,8 @@ contains
     if (.not. getOption("repeat_first_derivative", .true.))                                   &
          call inputError("repeat_first_derivative = false is not built")
     desc%n_instances = 1
+    if (present(nInstances)) desc%n_instances = max(1, nInstances)
+    run%gridSize = sizes
     call region%setup(desc, coordinates)
 
     ! boundary conditions (bc.dat rows of grid 1) with their patch options
@@ -261,13 +272,17 @@ contains
     end if
 
     ! initial condition (initial_condition_file, else the target state)
-    filename = getOption("initial_condition_file", "")
-    if (len_trim(filename) > 0) then
-       call check(mgd_plot3d_read_solution(cstr(filename), 0_c_int, int(nv, c_int), state, aux), &
-            "mgd_plot3d_read_solution " // trim(filename))
-       call region%setState(MGD_CONSERVED, 1, state(:, 1:nv))
-    else
-       call region%setState(MGD_CONSERVED, 1, target)
+    doLoad = .true.
+    if (present(loadInitialCondition)) doLoad = loadInitialCondition
+    if (doLoad) then
+       filename = getOption("initial_condition_file", "")
+       if (len_trim(filename) > 0) then
+          call check(mgd_plot3d_read_solution(cstr(filename), 0_c_int, int(nv, c_int), state,   &
+               aux), "mgd_plot3d_read_solution " // trim(filename))
+          call region%setState(MGD_CONSERVED, 1, state(:, 1:nv))
+       else
+          call region%setState(MGD_CONSERVED, 1, target)
+       end if
     end if
 
     ! time stepping
@@ -282,8 +297,10 @@ contains
        filename = trim(getOption("control_forcing_directory", "")) // trim(run%outputPrefix) //  &
             ".control_forcing_" // trim(run%actuatorName) // ".dat"
        filename = getOption("control_forcing_filename", trim(filename))
-       allocate(run%control(run%nActuatorPoints, 4 * solver%nTimesteps))
-       call check(mgd_control_read(cstr(filename), int(4 * solver%nTimesteps, c_int),          &
+       nStages = 4 * solver%nTimesteps
+       if (present(nControlStages)) nStages = nControlStages
+       allocate(run%control(run%nActuatorPoints, nStages))
+       call check(mgd_control_read(cstr(filename), int(nStages, c_int),                         &
             int(run%nActuatorPoints, c_int), run%control), "mgd_control_read " // trim(filename))
     end if
   end subroutine setupFromInput

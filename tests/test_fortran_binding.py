@@ -9,7 +9,10 @@ adjoint drivers, run on an example directory written with the library's PLOT3D w
 (magudi.inp, bc.dat, grid, initial condition, mollifiers), write J, the sensitivity and the
 gradient file of the oracle's run of the same case, in the stage loop and in the whole-run
 mode, and the forward driver with controller_switch reproduces the finite-difference
-relation (J(A) - J(0)) / A = Grad0.
+relation (J(A) - J(0)) / A = Grad0; the multiple-shooting drivers, run on such a directory with
+per-segment initial conditions and a control forcing file, write the oracle's msforward /
+msadjoint results (costs, mismatches, end snapshots, gradient inner products, IC gradients,
+matching adjoint terminals and the control gradient file).
 """
 import os
 import re
@@ -208,3 +211,123 @@ def test_input_reader(tmp_path):
     assert out[3] == "T"
     assert float(out[4]) == 0.25 and float(out[5]) == 7.0
     assert out[6].split() == ["2", "ctl", "ACTUATOR", "0", "3", "7", "-9", "-2", "1", "-1"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("penalty", ["quadratic", "huber"])
+def test_multiple_shooting_drivers_on_an_example_directory(tmp_path, penalty):
+    """msforward_hip / msadjoint_hip (bin/msforward.f90, bin/msadjoint.f90) on a generated
+    monopole directory with 3 segments of 8 steps, per-segment initial conditions
+    <prefix>-<k>.ic.q (the oracle's states at t = 8 k dt, each but the first with a seeded
+    momentum bump, times in the file headers) and a control forcing file: J, the segment
+    costs and mismatches, the end snapshots, the total and per-segment gradient inner
+    products, the IC gradients, the matching adjoint terminals and the control gradient
+    file against the oracle's msforward / msadjoint (J 1e-12, gradients 1e-10)."""
+    import ms_oracle as ms
+    import oracle_bind as ob
+    from magudi_amd import cases, plot3d
+    fwd, adj = os.path.join(FDIR, "msforward_hip"), os.path.join(FDIR, "msadjoint_hip")
+    if not (os.path.exists(fwd) and os.path.exists(adj)):
+        pytest.skip("Fortran drivers not built (needs amdflang at build time)")
+    nsplit, nts, K, w, h = 3, 8, 4, 20.0, 1e-3
+    c = cases.acoustic_monopole(n=41, nsteps=nsplit * nts, save_interval=K)
+    c.dt = 0.2
+    extra = (f"time_splitting/number_of_segments = {nsplit}\n"
+             f"time_splitting/segment_length = {nts}\n"
+             "time_splitting/start_timestep = 0\n"
+             f"time_splitting/penalty_weight = {w!r}\n"
+             f'time_splitting/penalty_type = "{penalty}"\n'
+             f"time_splitting/huber/threshold = {h!r}\n"
+             "adjoint_nonzero_initial_condition = true\n"
+             "controller_switch = true\n")
+    _write_case_directory(tmp_path, c, extra=extra)
+    O = ob.Region(c)
+    na = O.n_actuator
+    assert na > 0
+    rng = np.random.default_rng(17)
+    ctl = 1e-2 * rng.standard_normal((nsplit * 4 * nts, na))
+    plot3d.write_control(str(tmp_path / "Monopole.control_forcing_controlRegion.dat"), ctl)
+    x, y = c.coords[0], c.coords[1]
+    nv = c.nd + 2
+    q = c.initial_state()
+    ics, t0s = [q], [0.0]
+    for k in range(1, nsplit):
+        _, q = O.run_forward(q, c.dt, nts, control=ms.control_slice(ctl, k - 1, nts),
+                             t0=(k - 1) * nts * c.dt)
+        x0, y0 = rng.uniform(-2, 2, size=2)
+        bump = np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / 2.0)
+        ic = q.copy()
+        ic[1] = ic[1] + 1e-3 * bump
+        ic[2] = ic[2] - 5e-4 * bump
+        ics.append(ic)
+        t0s.append(k * nts * c.dt)
+    for k in range(nsplit):
+        plot3d.write_solution(str(tmp_path / f"Monopole-{k}.ic.q"), c.n, ics[k],
+                              timestep=k * nts, time=t0s[k])
+    Jo, cost_o, L2_o, ends = ms.ms_forward(O, ics, t0s, c.dt, nts, w, control=ctl, kind=penalty,
+                                           threshold=h)
+    res = ms.ms_adjoint(O, ics, t0s, c.dt, nts, K, w, control=ctl, kind=penalty, threshold=h)
+    print(f"msforward_hip / msadjoint_hip in {tmp_path} penalty={penalty}", flush=True)
+
+    out = subprocess.run([fwd], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    J = _read_scalar(tmp_path / "Monopole.forward_run.txt")
+    assert abs(J - Jo) <= 1e-12 * abs(Jo)
+    rows = np.loadtxt(tmp_path / "Monopole.sub_J.txt", comments="#", ndmin=2)
+    assert np.array_equal(rows[:, 0], np.arange(nsplit))
+    np.testing.assert_allclose(rows[:, 1], cost_o, rtol=1e-12)
+    assert rows[0, 2] == 0.0 and min(L2_o[1:]) > 0
+    np.testing.assert_allclose(rows[1:, 2], L2_o[1:], rtol=1e-12)
+    for k in range(nsplit):
+        qT, aux = plot3d.read_solution(str(tmp_path / f"Monopole-{k}-{(k + 1) * nts:08d}.q"), nv)
+        assert aux[0] == (k + 1) * nts and abs(aux[3] - (k + 1) * nts * c.dt) <= 1e-12
+        assert np.abs(qT - ends[k]).max() <= 1e-12 * np.abs(ends[k]).max(), k
+
+    out = subprocess.run([adj], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    total = _read_scalar(tmp_path / "Monopole.adjoint_run.txt")
+    assert abs(total - res["total"]) <= 1e-10 * abs(res["total"])
+    rows = np.loadtxt(tmp_path / "Monopole.sub_adjoint_run.txt", comments="#", ndmin=2)
+    np.testing.assert_allclose(rows[:, 1], res["ctrl_ip"], rtol=1e-10)
+    assert rows[0, 2] == 0.0
+    np.testing.assert_allclose(rows[1:, 2], res["ic_ip"][1:], rtol=1e-10)
+    for k in range(nsplit):
+        g, _ = plot3d.read_solution(str(tmp_path / f"Monopole-{k}.ic.adjoint.q"), nv)
+        go = res["ic_grad"][k]
+        assert np.abs(g - go).max() <= 1e-10 * np.abs(go).max(), k
+    for k in range(nsplit - 1):
+        lamT, _ = plot3d.read_solution(
+            str(tmp_path / f"Monopole-{k}-{(k + 1) * nts:08d}.adjoint.q"), nv)
+        lo = res["scale"] * (ends[k] - ics[k + 1])
+        assert np.abs(lamT - lo).max() <= 1e-10 * np.abs(lo).max(), k
+    grad = plot3d.read_control(str(tmp_path / "Monopole.gradient_controlRegion.dat"), na)
+    assert grad.shape == res["control_grad"].shape
+    assert np.abs(grad - res["control_grad"]).max() <= 1e-10 * np.abs(res["control_grad"]).max()
+
+
+@pytest.mark.parametrize("inp,driver,message", [
+    ('output_prefix = "X"\n', "msforward_hip", "'initial_condition_file' (placeholder ok)"),
+    ('initial_condition_file = "x"\n', "msforward_hip",
+     "Required parameter 'time_splitting/number_of_segments' not found!"),
+    ('initial_condition_file = "x"\nenable_adjoint_solver = true\n', "msadjoint_hip",
+     "must contain 'adjoint_nonzero_initial_condition'"),
+    ('initial_condition_file = "x"\ntime_splitting/number_of_segments = 2\n'
+     "time_splitting/segment_length = 4\ntime_splitting/start_timestep = 0\n"
+     'time_splitting/penalty_weight = 1.0\ntime_splitting/penalty_type = "l1"\n',
+     "msforward_hip", "expected 'quadratic' or 'huber'"),
+    ('initial_condition_file = "x"\ntime_splitting/number_of_segments = 2\n'
+     "time_splitting/segment_length = 4\ntime_splitting/start_timestep = 0\n"
+     "time_splitting/penalty_weight = 1.0\ntime_splitting/state_mollifier/enabled = true\n",
+     "msforward_hip", "state_mollifier/enabled = true is not built"),
+])
+def test_multiple_shooting_driver_input_errors(tmp_path, inp, driver, message):
+    """The drivers stop with the reference's messages (bin/msforward.f90:127,
+    bin/msadjoint.f90:131-144, src/MSPenaltyImpl.f90:138) on missing or invalid time_splitting input, before any
+    device work."""
+    exe = os.path.join(FDIR, driver)
+    if not os.path.exists(exe):
+        pytest.skip("Fortran drivers not built (needs amdflang at build time)")
+    (tmp_path / "magudi.inp").write_text(inp)
+    out = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert out.returncode != 0
+    assert message in out.stdout + out.stderr
