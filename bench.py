@@ -187,10 +187,12 @@ def viscous3d_leg(n):
 
 class KolmogorovRun:
     """The headline: the 3-D viscous periodic Kolmogorov flow as the configuration runs it.
-    One pass = restart from the resident initial condition, nsteps RK4 steps forward with
-    HBM snapshots (every nsteps / 8), J = kinetic energy of the final state and dJ/dq(T) on
-    the device, the discrete adjoint back from dJ/dq(T) with window recompute (its result,
-    the adjoint state at t0, is dJ/dq0 and stays on the device)."""
+    One pass = restart from the resident initial condition, nsteps RK4 steps forward
+    keeping the state of every step and the inner stage states of the last steps in HBM
+    (the step store; save_interval nsteps / 8), J = kinetic energy of the final state and
+    dJ/dq(T) on the device, the discrete adjoint back from dJ/dq(T), recomputing the inner
+    stage states of the steps not held (its result, the adjoint state at t0, is dJ/dq0 and
+    stays on the device)."""
 
     def __init__(self, n, nsteps):
         from magudi_amd import cases
@@ -211,7 +213,7 @@ class KolmogorovRun:
 
     def timed(self, steps):
         """(seconds per pass, J, forward seconds of a pass, z-march launches in the timed
-        passes, those of them that also wrote a window copy, their summed ms -- HIP events
+        passes, those of them that also wrote a state copy, their summed ms -- HIP events
         around every launch on the region stream)"""
         self.R.sync()
         self.R.kernel_timer(True)
@@ -575,21 +577,23 @@ def main():
         K.one()
     t_step, J, t_fwd, launches, with_copy, march_ms = K.timed(a.steps)
     march_shape = K.R.march_shape
+    plan = K.R.checkpoint_plan
     value = K.pts / t_step
     print(f"headline: {t_step:.3f} s per pass ({value:.4g} grid-pt-steps/s)", file=sys.stderr,
           flush=True)
     # dominant kernel (by launches x time: 4 nsteps forward launches per pass plus those of
-    # the window recompute, against 4 nsteps launches of each adjoint kernel), timed with
+    # the recompute, against 4 nsteps launches of each adjoint kernel), timed with
     # HIP events around each of its launches in the timed passes.  Its algorithmic bytes
     # per launch: the stage state and the 9 stress / heat-flux slots in, the RK buffers in
     # and out (0, 2, 2, 1 states in; 3, 2, 2, 1 out, over the 4 stages) and the new stage
     # state out -- 242 B per point -- plus 40 B per point for the launches that also write
-    # the state into a window.
+    # the state into the step store, the stage cache or the recompute buffer.
     K.R.restart()
     cms, cb = K.R.time_kernel(capi.KERNEL_COPY, reps=3)
     f_ms, f_b = K.R.time_kernel(capi.KERNEL_RHS_FORWARD, reps=3)
     a_ms, a_b = K.R.time_kernel(capi.KERNEL_RHS_ADJOINT, reps=3)
     M_bytes = 8.0 * 5 * K.case.npoints
+    K.R.close()                 # its step store holds most of the device memory
     del K
     ms = march_ms / max(launches, 1)
     b = (242.0 + 40.0 * with_copy / max(launches, 1)) * M_bytes / 40.0
@@ -601,7 +605,7 @@ def main():
     roofline = {"bound": "hbm", "kernel": "k_fwd3d_march", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "avg_launch_us": round(ms * 1e3, 2), "bytes_per_launch": b,
-                "launches_timed": launches, "launches_with_window_copy": with_copy}
+                "launches_timed": launches, "launches_with_state_copy": with_copy}
     n = a.c4_n
     out = {
         "metric": METRIC, "value": value, "unit": "grid-pt-steps/s", "n_gpus": 1,
@@ -610,9 +614,11 @@ def main():
         "data": "synthetic (Kolmogorov flow set-up and seeded solenoidal field generated in "
                 "magudi_amd/cases.py)",
         "config": {"workload": f"Kolmogorov {n}^3 viscous periodic forward + adjoint, "
-                               f"{a.c4_nsteps} RK4 steps, HBM snapshots every "
-                               f"{max(1, a.c4_nsteps // 8)} steps with recompute, terminal "
-                               f"kinetic energy and dJ/dq(T) on the device",
+                               f"{a.c4_nsteps} RK4 steps, the state of every step and the "
+                               f"stage states of the last {plan['held']} steps held in HBM, "
+                               f"the other steps' stages recomputed, terminal kinetic energy "
+                               f"and dJ/dq(T) on the device",
+                   "checkpoint_plan": plan,
                    "grid": [n, n, n], "nsteps": a.c4_nsteps,
                    "save_interval": max(1, a.c4_nsteps // 8), "instances": 1,
                    "march_shape": {"forward": list(march_shape[0]),
@@ -628,9 +634,9 @@ def main():
                        "achieved_GBs": round(cb / (cms * 1e-3) / 1e9, 1)},
     }
     if traffic is not None:
-        # the counter passes run the kernel's launches without the window copy (242 B / pt)
+        # the counter passes run the kernel's launches without the state copy (242 B / pt)
         out["roofline"]["traffic_over_algorithmic"] = round(traffic / (242.0 * M_bytes / 40.0), 3)
-        out["roofline"]["traffic_workload"] = "launches without the window copy (--kernels-only)"
+        out["roofline"]["traffic_workload"] = "launches without the state copy (--kernels-only)"
     if not a.no_extras:
         out["monopole401"] = monopole_legs(a, extras=True)
         out["fd_sweep"] = fd_sweep_leg(a.n, a.nsteps, a.save_interval, a.batch)

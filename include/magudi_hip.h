@@ -315,14 +315,32 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches);
  * for each of 32, 64 and 128 --, or one of those fixed.  MGD_OPT_MARCH_TILE: the width of
  * their tiles, 16 (16 x 16 points, 256 threads) or 32 (32 x 16, 512 threads; x sizes that
  * are multiples of 32), 0 (default) autotuned with the z chunk; the forward march and the
- * adjoint marches are tuned apart.  No result depends on either. */
+ * adjoint marches are tuned apart.  No result depends on either.
+ * MGD_OPT_STEP_STORE (generic-kernel path): the checkpointed forward pass keeps the state
+ * at the start of every step (nsteps states) and, in the memory left (MGD_OPT_WINDOW_CACHE
+ * n >= 0: at most n save_interval steps), the three inner stage states of its last steps;
+ * the adjoint pass recomputes the inner stage states of the other steps from the stored
+ * state, three RK stages per step instead of 4 save_interval - 1 per window from a
+ * snapshot, with no window buffer.  -1 (default): where nsteps + 3 states fit in the
+ * memory left free less the reserve and MGD_OPT_WINDOW_CACHE is at its default; 0: off
+ * (snapshots and windows); 1: on where the states of every step can be allocated.
+ * Identical results. */
 #define MGD_OPT_FORCE_PLAIN 1
 #define MGD_OPT_TILE_SHAPE 2
 #define MGD_OPT_RESIDENT_STATES 3
 #define MGD_OPT_WINDOW_CACHE 4
 #define MGD_OPT_Z_CHUNK 5
 #define MGD_OPT_MARCH_TILE 6
+#define MGD_OPT_STEP_STORE 7
 int mgd_region_set_option(mgd_region* r, int option, int value);
+
+/* Storage plan of the last checkpointed forward run: plan[0] = 1 step store, plan[1] steps
+ * whose inner stage states are held, from step plan[2] on; 0 snapshots and windows,
+ * plan[1] windows held, from step plan[2] on; 2 the fused 2-D path (plan[1] stage states
+ * held with MGD_OPT_RESIDENT_STATES, else 0: its own window recompute).  No counterpart
+ * in the reference (whose UniformCheckpointer writes .q files every save_interval
+ * steps). */
+int mgd_region_get_checkpoint_plan(mgd_region* r, int* plan);
 
 /* Tile shape the fused 2-D kernels use now (-1: the region runs the generic kernels). */
 int mgd_region_get_tile_shape(mgd_region* r, int* shape);

@@ -21,7 +21,7 @@ CONSERVED, ADJOINT_VARS, RHS, INITIAL_STATE, FINAL_STATE, SCRATCH = 0, 1, 2, 4, 
 FIELD_TARGET_STATE, FIELD_CONTROL_MOLLIFIER, FIELD_TARGET_MOLLIFIER = 0, 1, 2
 OP_FIRST, OP_ADJOINT_FIRST, OP_DISS, OP_DISS_T, OP_SECOND = 0, 1, 2, 3, 4
 OPT_FORCE_PLAIN, OPT_TILE_SHAPE, OPT_RESIDENT_STATES, OPT_WINDOW_CACHE, OPT_Z_CHUNK = 1, 2, 3, 4, 5
-OPT_MARCH_TILE = 6
+OPT_MARCH_TILE, OPT_STEP_STORE = 6, 7
 TERMINAL_KINETIC_ENERGY = 0
 KERNEL_FWD_STAGE, KERNEL_ADJ_STAGE, KERNEL_COPY, KERNEL_RHS_FORWARD, KERNEL_RHS_ADJOINT = 0, 1, 2, 3, 4
 KERNEL_FWD_MARCH3D = 5
@@ -90,6 +90,7 @@ def lib():
         "mgd_region_set_option": [vp, C.c_int, C.c_int],
         "mgd_region_get_tile_shape": [vp, C.POINTER(C.c_int)],
         "mgd_region_get_march_shape": [vp, C.POINTER(C.c_int)],
+        "mgd_region_get_checkpoint_plan": [vp, C.POINTER(C.c_int)],
         "mgd_time_kernel": [vp, C.c_int, C.c_int, dp, dp],
         "mgd_kernel_timer": [vp, C.c_int, C.c_int],
         "mgd_kernel_timer_read": [vp, C.POINTER(C.c_long), C.POINTER(C.c_long), dp],

@@ -47,6 +47,12 @@ struct DevBuf {
     n = count;
     return MGD_OK;
   }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(double); }
 };
 
 struct HostPatch {
@@ -140,6 +146,16 @@ struct mgd_region {
   mgd::DevBuf wcache[kMaxCache];
   int ncache = 0, cache_first = 0;
   int opt_cache = -1;                    // MGD_OPT_WINDOW_CACHE: -1 as memory allows, else at most n
+  // step store of the generic-kernel path: the checkpointed forward pass keeps the state at
+  // the start of every step (sstore, steps 0 .. nsteps-1) and the three inner stage states
+  // of its last `nscache` steps (scache, from step scache_first on); the adjoint pass
+  // recomputes the inner stage states of the other steps from the stored state, three
+  // stages per step into `window`, instead of whole windows from snapshots
+  int opt_step = -1;                     // MGD_OPT_STEP_STORE: -1 where it fits, 0 off, 1 on
+  bool step_mode = false;
+  mgd::DevBuf sstore, scache;
+  int nscache = 0, scache_first = 0;
+  int rec_step = -1;                     // step whose inner stage states `window` holds
   // per-launch timer of the z-march forward kernel (mgd_kernel_timer): event pairs
   bool timing = false;
   std::vector<hipEvent_t> tev;
@@ -991,10 +1007,20 @@ double* cached_window(mgd_region* r, int lt) {
   return k < r->ncache ? r->wcache[k].p : nullptr;
 }
 
+// Device memory a new storage plan of a checkpointed run may take: what is free plus the
+// storage held for earlier runs, less a reserve of max(8 GiB, a tenth of the device).
+size_t storage_budget(mgd_region* r) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+  size_t held = r->window.bytes() + r->snaps.bytes() + r->sstore.bytes() + r->scache.bytes();
+  for (int k = 0; k < mgd_region::kMaxCache; ++k) held += r->wcache[k].bytes();
+  const size_t reserve = std::max((size_t)8 << 30, tot / 10);
+  return fr + held > reserve ? fr + held - reserve : 0;
+}
+
 // How many of the last windows of a checkpointed run the forward pass keeps: as many as
-// MGD_OPT_WINDOW_CACHE allows and, by default, as fit in the device memory left free
-// (counting caches already held) less a reserve of max(8 GiB, a tenth of the device);
-// the generic-kernel path only (the fused 2-D path keeps its last window itself).
+// MGD_OPT_WINDOW_CACHE allows and, by default, as fit in storage_budget (the generic-kernel
+// path only; the fused 2-D path keeps its last window itself).
 int plan_window_cache(mgd_region* r, int nsteps, int K, long M) {
   r->ncache = 0;
   const int nwin = (nsteps + K - 1) / K;
@@ -1002,16 +1028,15 @@ int plan_window_cache(mgd_region* r, int nsteps, int K, long M) {
   want = std::min(want, mgd_region::kMaxCache);
   if (fused_ok(r)) want = 0;
   const size_t bytes = (size_t)4 * K * M * sizeof(double);
+  for (int k = want; k < mgd_region::kMaxCache; ++k) r->wcache[k].release();
   if (want > 0 && r->opt_cache < 0) {
-    size_t fr = 0, tot = 0;
-    MGD_HIP(hipMemGetInfo(&fr, &tot));
-    size_t held = 0;
-    for (int k = 0; k < mgd_region::kMaxCache; ++k) held += r->wcache[k].n * sizeof(double);
-    const size_t reserve = std::max((size_t)8 << 30, tot / 10);
-    const size_t avail = fr + held > reserve ? fr + held - reserve : 0;
+    // counts the window buffer and snapshots of this run as taken
+    const size_t budget = storage_budget(r);
+    const size_t taken = r->window.bytes() + r->snaps.bytes();
+    const size_t avail = budget > taken ? budget - taken : 0;
     want = std::min(want, (int)(avail / bytes));
+    for (int k = want; k < mgd_region::kMaxCache; ++k) r->wcache[k].release();
   }
-  for (int k = want; k < mgd_region::kMaxCache; ++k) r->wcache[k].alloc(0);
   int n = 0;
   while (n < want && r->wcache[n].alloc((size_t)4 * K * M) == MGD_OK) ++n;
   if (n < want) (void)hipGetLastError();   // fewer windows than planned: the rest recompute
@@ -1020,9 +1045,106 @@ int plan_window_cache(mgd_region* r, int nsteps, int K, long M) {
   return MGD_OK;
 }
 
+// Storage of a checkpointed run on the generic-kernel path.  Step mode (MGD_OPT_STEP_STORE
+// 1, or -1 with MGD_OPT_WINDOW_CACHE at its default, where the states of every step and
+// three scratch states fit in storage_budget): the state at the start of every step, and
+// the inner stage states of as many of the last steps as the rest of the budget holds
+// (MGD_OPT_WINDOW_CACHE n >= 0: at most n save_interval steps); the adjoint pass recomputes
+// three stages per step not held, against 4 save_interval - 1 per window in window mode.
+// Window mode: snapshots every K steps, a window buffer and whole cached windows.
+int plan_plain_storage(mgd_region* r, int nsteps, int K, long M) {
+  const size_t S = (size_t)M * sizeof(double);
+  r->rec_step = -1;
+  r->step_mode = false;
+  bool step = r->opt_step == 1;
+  size_t budget = 0;
+  if (r->opt_step != 0) {
+    budget = storage_budget(r);
+    if (r->opt_step < 0) step = r->opt_cache < 0 && (size_t)(nsteps + 3) * S <= budget;
+  }
+  if (step) {
+    for (int k = 0; k < mgd_region::kMaxCache; ++k) r->wcache[k].release();
+    r->ncache = 0;
+    r->snaps.release();
+    if (r->window.n != (size_t)3 * M) r->window.release();
+    if (r->sstore.n != (size_t)nsteps * M) r->sstore.release();
+    if (r->window.alloc((size_t)3 * M) || r->sstore.alloc((size_t)nsteps * M)) {
+      // taken meanwhile (regions of a group plan one after another): window mode
+      (void)hipGetLastError();
+      step = false;
+    }
+  }
+  if (step) {
+    const size_t rest = budget > (size_t)(nsteps + 3) * S ? budget - (size_t)(nsteps + 3) * S : 0;
+    int nsc = (int)std::min<size_t>((size_t)nsteps, rest / (3 * S));
+    if (r->opt_cache >= 0) nsc = std::min(nsc, r->opt_cache * K);
+    if (r->scache.n != (size_t)3 * nsc * M) r->scache.release();
+    while (nsc > 0 && r->scache.alloc((size_t)3 * nsc * M) != MGD_OK) {
+      (void)hipGetLastError();             // fewer steps than planned: the rest recompute
+      nsc /= 2;
+    }
+    r->nscache = nsc;
+    r->scache_first = nsteps - nsc;
+    r->step_mode = true;
+    return MGD_OK;
+  }
+  r->sstore.release();
+  r->scache.release();
+  r->nscache = 0;
+  const int nsnap = nsteps / K + 1;
+  if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M)) return MGD_ERR_HIP;
+  return plan_window_cache(r, nsteps, K, M);
+}
+
+// Step mode: the inner stage states Y2, Y3, Y4 of step t (the outputs of its stages 1-3)
+// from the stored state of step t into `window`; the scratch state buffers keep their
+// places.
+void recompute_step(mgd_region* r, int t, double dt) {
+  const long M = (long)r->ninst * M_of(r);
+  launch_copy(M, r->sstore.p + (long)t * M, r->qs.p, r->stream);
+  double* const p0 = r->qs.p;
+  for (int st = 1; st <= 3; ++st)
+    forward_stage(r, r->qs, r->qs_alt, r->b1.p, r->b2.p, 4 * t + st - 1, st, dt,
+                  r->window.p + (long)(st - 1) * M);
+  if (r->qs.p != p0) {
+    std::swap(r->qs.p, r->qs_alt.p);
+    std::swap(r->qs.n, r->qs_alt.n);
+  }
+  r->rec_step = t;
+}
+
+// Step mode: forward stage state `stage` (1: the state at the start of step t, 2-4: the
+// outputs of its stages 1-3), recomputing the step's inner states where they are not held.
+const double* step_stage_state(mgd_region* r, int t, int stage, double dt, bool recompute) {
+  const long M = (long)r->ninst * M_of(r);
+  if (stage == 1) return r->sstore.p + (long)t * M;
+  if (t >= r->scache_first) return r->scache.p + ((long)(t - r->scache_first) * 3 + stage - 2) * M;
+  if (recompute) recompute_step(r, t, dt);
+  return r->window.p + (long)(stage - 2) * M;
+}
+
 int plain_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
   const int nc = n_cost(r);
   const long M = (long)r->ninst * M_of(r);
+  if (r->step_mode) {
+    // the state at the start of every step (stage 4 of the step before writes it) and the
+    // inner stage states of the last nscache steps
+    r->rec_step = -1;
+    launch_copy(M, r->q.p, r->sstore.p, r->stream);
+    for (int s = 0; s < nsteps; ++s)
+      for (int i = 1; i <= 4; ++i) {
+        double* copy = nullptr;
+        if (i < 4 && s >= r->scache_first)
+          copy = r->scache.p + ((long)(s - r->scache_first) * 3 + i - 1) * M;
+        else if (i == 4 && s + 1 < nsteps)
+          copy = r->sstore.p + (long)(s + 1) * M;
+        forward_stage(r, r->q, r->q_alt, r->b1.p, r->b2.p, 4 * s + i - 1, i, dt, copy);
+        if (nc > 0)
+          launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), nc, 0, r->q.p, r->partial.p,
+                              kRkNorm[i - 1] * dt, r->acc_J.p, nullptr, r->stream);
+      }
+    return MGD_OK;
+  }
   // forward run with snapshots at t0 and every K steps (showProgress .q files); in the
   // cached windows every stage state also goes to the window's cache
   launch_copy(M, r->q.p, r->snaps.p, r->stream);
@@ -1043,14 +1165,50 @@ int plain_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
   return MGD_OK;
 }
 
-int plain_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool terminal, bool want_grad) {
+// One adjoint RK4 stage (stage i of step t) of the generic-kernel path, linearised about
+// the forward stage state qcur.
+void plain_adjoint_stage(mgd_region* r, double dt, int nsteps, int t, int i, const double* qcur,
+                         bool want_grad) {
   const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
+  const long M = (long)r->ninst * M_of(r);
+  const int sidx = 4 * t + i - 1;
+  if (na > 0) {
+    if (want_grad)
+      launch_gradient(r->geo, dpatch(r, r->ip_act), na, r->lam.p, r->grad.p + (long)sidx * na,
+                      (long)nst * na, r->stream);
+    launch_patch_reduce(r->geo, dpatch(r, r->ip_act), na, 1, r->lam.p, r->partial.p,
+                        kRkNorm[i - 1] * dt, r->acc_S.p, nullptr, r->stream);
+  }
+  if (nc > 0)
+    launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, qcur, r->aforce.p, t == 0 && i == 1,
+                       r->stream);
+  const double factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
+  // the rhs launcher applies the RK update itself where its last kernel finishes the rhs
+  const RkStage rk{i, dt, r->lam.p, r->b1a.p, r->b2a.p};
+  if (!launch_rhs_adjoint(r->geo, dpatches(r), qcur, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
+                          r->T1.p, r->S.p, nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc,
+                          factor, r->fin, r->stream, &rk))
+    launch_rk_adjoint(M, i, dt, r->lam.p, r->b1a.p, r->b2a.p, r->rhs.p, r->stream);
+}
+
+int plain_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool terminal, bool want_grad) {
+  const int nc = n_cost(r);
   const long M = (long)r->ninst * M_of(r);
   // adjoint terminal condition (loadInitialCondition ADJOINT, src/SolverImpl.f90:368-428)
   if (!terminal) launch_fill(M, 0.0, r->lam.p, r->stream);
   if (nc > 0) {
     launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->qT.p, r->aforce.p, 0, r->stream);
     launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -dt / 6.0, r->lam.p, r->stream);
+  }
+  if (r->step_mode) {
+    // every step's inner stage states are recomputed in this pass (a captured pass replays
+    // the recompute too), the held ones read from the step cache
+    for (int t = nsteps - 1; t >= 0; --t) {
+      for (int i = 4; i >= 1; --i)
+        plain_adjoint_stage(r, dt, nsteps, t, i, step_stage_state(r, t, i, dt, i == 4), want_grad);
+    }
+    r->rec_step = -1;
+    return MGD_OK;
   }
   int loaded = -1;
   const double* base = r->window.p;      // stage states of the loaded window
@@ -1085,25 +1243,7 @@ int plain_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
         }
       }
       const int widx = (ts - 1 - loaded) * 4 + st;
-      const double* qcur = base + (long)widx * M;
-      const int sidx = 4 * t + i - 1;
-      if (na > 0) {
-        if (want_grad)
-          launch_gradient(r->geo, dpatch(r, r->ip_act), na, r->lam.p, r->grad.p + (long)sidx * na,
-                          (long)nst * na, r->stream);
-        launch_patch_reduce(r->geo, dpatch(r, r->ip_act), na, 1, r->lam.p, r->partial.p,
-                            kRkNorm[i - 1] * dt, r->acc_S.p, nullptr, r->stream);
-      }
-      if (nc > 0)
-        launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, qcur, r->aforce.p, t == 0 && i == 1,
-                           r->stream);
-      const double factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
-      // the rhs launcher applies the RK update itself where its last kernel finishes the rhs
-      const RkStage rk{i, dt, r->lam.p, r->b1a.p, r->b2a.p};
-      if (!launch_rhs_adjoint(r->geo, dpatches(r), qcur, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
-                              r->T1.p, r->S.p, nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc,
-                              factor, r->fin, r->stream, &rk))
-        launch_rk_adjoint(M, i, dt, r->lam.p, r->b1a.p, r->b2a.p, r->rhs.p, r->stream);
+      plain_adjoint_stage(r, dt, nsteps, t, i, base + (long)widx * M, want_grad);
     }
   }
   return MGD_OK;
@@ -1144,11 +1284,17 @@ int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, c
   if (control && na > 0) r->ck.control.assign(control, control + (size_t)r->ninst * nst * na);
   else r->ck.control.clear();
   if (int rc = upload_run_inputs(r)) return rc;
-  const int nsnap = nsteps / K + 1;
-  if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M) || r->qs.alloc(M) ||
-      r->qT.alloc(M) || (r->nd == 3 && r->qs_alt.alloc(M)))
-    return MGD_ERR_HIP;
-  if (int rc = plan_window_cache(r, nsteps, K, M)) return rc;
+  if (r->qs.alloc(M) || r->qT.alloc(M) || (r->nd == 3 && r->qs_alt.alloc(M))) return MGD_ERR_HIP;
+  if (!fused_ok(r)) {
+    if (int rc = plan_plain_storage(r, nsteps, K, M)) return rc;
+  } else {
+    const int nsnap = nsteps / K + 1;
+    r->step_mode = false;
+    r->sstore.release();
+    r->scache.release();
+    if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M)) return MGD_ERR_HIP;
+    if (int rc = plan_window_cache(r, nsteps, K, M)) return rc;
+  }
   if (fused_ok(r)) {
     if (int rc = prepare_quadrature(r, nst, dt)) return rc;
     if (r->resident && r->wall.alloc((size_t)(4L * nsteps + 1) * M)) {
@@ -1262,6 +1408,10 @@ std::vector<uintptr_t> graph_key(mgd_region* r, int kind, std::initializer_list<
   k.push_back((uintptr_t)r->ncache);
   k.push_back((uintptr_t)r->cache_first);
   for (int c = 0; c < r->ncache; ++c) k.push_back((uintptr_t)r->wcache[c].p);
+  k.push_back((uintptr_t)r->step_mode);
+  k.push_back((uintptr_t)r->nscache);
+  k.push_back((uintptr_t)r->sstore.p);
+  k.push_back((uintptr_t)r->scache.p);
   for (uintptr_t e : extra) k.push_back(e);
   return k;
 }
@@ -2003,6 +2153,25 @@ int mgd_region_get_march_shape(mgd_region* r, int* shape) {
   return MGD_OK;
 }
 
+int mgd_region_get_checkpoint_plan(mgd_region* r, int* plan) {
+  if (!r || !plan) { set_error("null argument"); return MGD_ERR_ARG; }
+  if (!r->ck.valid && r->ck.nsteps == 0) { set_error("no checkpointed forward run prepared"); return MGD_ERR_STATE; }
+  if (fused_ok(r)) {
+    plan[0] = 2;
+    plan[1] = r->resident ? 4 * r->ck.nsteps + 1 : 0;
+    plan[2] = 0;
+  } else if (r->step_mode) {
+    plan[0] = 1;
+    plan[1] = r->nscache;
+    plan[2] = r->scache_first;
+  } else {
+    plan[0] = 0;
+    plan[1] = r->ncache;
+    plan[2] = r->cache_first;
+  }
+  return MGD_OK;
+}
+
 int mgd_region_get_tile_shape(mgd_region* r, int* shape) {
   if (!r || !shape) { set_error("null argument"); return MGD_ERR_ARG; }
   *shape = fused_ok(r) ? r->tiling.shape : -1;
@@ -2015,6 +2184,13 @@ int mgd_region_set_option(mgd_region* r, int option, int value) {
     case MGD_OPT_FORCE_PLAIN: r->force_plain = value != 0; return MGD_OK;
     case MGD_OPT_RESIDENT_STATES: r->resident = value != 0; return MGD_OK;
     case MGD_OPT_WINDOW_CACHE: r->opt_cache = value < 0 ? -1 : value; return MGD_OK;
+    case MGD_OPT_STEP_STORE:
+      if (value < -1 || value > 1) {
+        set_error("step store: -1 (where it fits), 0 (off) or 1 (on)");
+        return MGD_ERR_ARG;
+      }
+      r->opt_step = value;
+      return MGD_OK;
     case MGD_OPT_Z_CHUNK:
       if (value != 0 && value != 32 && value != 64 && value != 128) {
         set_error("z chunk: 0 (autotune), 32, 64 or 128");
@@ -2383,6 +2559,25 @@ int mgd_checkpoint_migrate(mgd_region* r, int timestep, int stage) {
   }
   const long M = (long)r->ninst * M_of(r);
   const int ts = timestep, st = stage;
+  if (r->step_mode) {
+    // the state at the start of step ts, or the output of stage st of step ts - 1
+    const double* src;
+    if (st == 4) {
+      src = ts < nsteps ? r->sstore.p + (long)ts * M : r->qT.p;
+    } else {
+      const int t = ts - 1;
+      const bool rec = t < r->scache_first && r->rec_step != t;
+      if (rec) {
+        // the recompute needs the run's actuator forcing and source amplitudes
+        MGD_HIP(hipStreamSynchronize(r->stream));
+        if (int rc = upload_run_inputs(r)) return rc;
+      }
+      src = step_stage_state(r, t, st + 1, dt, rec);
+    }
+    launch_copy(M, src, r->q.p, r->stream);
+    MGD_HIP(hipGetLastError());
+    return MGD_OK;
+  }
   int loaded = r->mig_loaded;
   const bool need = loaded == -1 || ts < loaded || ts > loaded + K || (ts == loaded && st < 4) ||
                     (ts == loaded + K && st == 4);

@@ -25,7 +25,8 @@ module magudi_hip
   integer(c_int), parameter, public :: MGD_FIELD_TARGET_STATE = 0,                            &
        MGD_FIELD_CONTROL_MOLLIFIER = 1, MGD_FIELD_TARGET_MOLLIFIER = 2
   integer(c_int), parameter, public :: MGD_OPT_FORCE_PLAIN = 1, MGD_OPT_TILE_SHAPE = 2,         &
-       MGD_OPT_RESIDENT_STATES = 3
+       MGD_OPT_RESIDENT_STATES = 3, MGD_OPT_WINDOW_CACHE = 4, MGD_OPT_Z_CHUNK = 5,               &
+       MGD_OPT_MARCH_TILE = 6, MGD_OPT_STEP_STORE = 7
   integer(c_int), parameter, public :: MGD_PATCH_IMPENETRABLE_WALL = 6,                        &
        MGD_PATCH_ISOTHERMAL_WALL = 7
   integer(c_int), parameter, public :: MGD_PERIODIC_NONE = 0, MGD_PERIODIC_PLANE = 1,            &

@@ -15,6 +15,7 @@ reference's ``gracefulExit``).  Nothing here computes the physics on the CPU.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -40,7 +41,9 @@ class State:
     """View of one instance's arrays (t_State)."""
 
     def __init__(self, region: "Region", index: int):
-        self._r = region
+        # a weak reference: the region's device memory goes as soon as the last reference
+        # to the region does, not at the next garbage collection
+        self._r = weakref.proxy(region)
         self.index = index
 
     def _get(self, which):
@@ -265,6 +268,14 @@ class Region:
         as device memory allows (default), 0 none, n > 0 at most n."""
         check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_WINDOW_CACHE, int(n)))
 
+    def set_step_store(self, mode: int):
+        """MGD_OPT_STEP_STORE: 1 keeps the state at the start of every step of the
+        checkpointed forward pass (and the inner stage states of its last steps, as memory
+        and set_window_cache allow), so that the adjoint pass recomputes three stages per
+        step not held instead of whole windows; 0 snapshots and windows; -1 (default) step
+        store where it fits and the window cache is at its default."""
+        check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_STEP_STORE, int(mode)))
+
     def set_z_chunk(self, zc: int):
         """MGD_OPT_Z_CHUNK: 0 autotunes the 3-D marches' z chunk at the first run, or 32,
         64, 128 fixes it."""
@@ -282,6 +293,17 @@ class Region:
         out = (C.c_int * 4)()
         check(capi.lib().mgd_region_get_march_shape(self.handle, out))
         return (out[0], out[1]), (out[2], out[3])
+
+    @property
+    def checkpoint_plan(self) -> dict:
+        """Storage plan of the last checkpointed forward run: {"mode": "step store",
+        "held": steps whose inner stage states are kept, "first": the first of them},
+        {"mode": "windows", "held": windows kept, "first": their first step} or
+        {"mode": "fused", "held": resident stage states (0: window recompute)}."""
+        out = (C.c_int * 3)()
+        check(capi.lib().mgd_region_get_checkpoint_plan(self.handle, out))
+        mode = {0: "windows", 1: "step store", 2: "fused"}[out[0]]
+        return {"mode": mode, "held": out[1], "first": out[2]}
 
     @property
     def march_tile(self) -> int:

@@ -87,14 +87,19 @@ def test_stagewise_run_matches_whole_run(plain, walled):
         assert np.allclose(sens, sens_run, rtol=1e-12, atol=0.0)
 
 
-def test_migrate_serves_every_stage_state():
+@pytest.mark.parametrize("step,cache", [(-1, -1), (0, -1), (0, 0), (1, 0), (1, 1)])
+def test_migrate_serves_every_stage_state(step, cache):
     """migrateTo in any order, across windows, against the stage states of a run that keeps
     every stage (MGD_OPT_RESIDENT_STATES is not needed: a whole forward run with the
-    stage outputs collected by substeps)."""
+    stage outputs collected by substeps); with the step store (MGD_OPT_STEP_STORE, every
+    step's inner states held, none, or those of the last save_interval steps) and with
+    snapshots and windows (every window cached, none)."""
     from magudi_amd.solver import Region, RK4Integrator, Solver, UniformCheckpointer
     c = _case()
     q0 = c.initial_state()
     R = Region(c, force_plain=True)
+    R.set_step_store(step)
+    R.set_window_cache(cache)
     R.states[0].conservedVariables = q0
     ti = RK4Integrator()
     states = {(0, 4): q0}

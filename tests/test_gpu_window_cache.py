@@ -1,10 +1,13 @@
-"""The window cache of checkpointed runs on the generic-kernel path (MGD_OPT_WINDOW_CACHE):
-the forward pass keeps the stage states of its last windows and the adjoint pass reads
-them instead of recomputing those windows.  Runs with no cache, a partial cache and the
-default (every window, on these small grids) must give the same J, sensitivity, gradient
-and adjoint state bit for bit -- on a 3-D box (z-march kernels, a ragged last window) and
-on a 2-D monopole forced onto the generic kernels (actuator and cost target).  The
-per-launch timer of the z-march kernel counts the forward and recompute launches."""
+"""Checkpoint storage of the generic-kernel path.  Window mode (MGD_OPT_STEP_STORE 0):
+snapshots every save_interval steps, and the window cache (MGD_OPT_WINDOW_CACHE) -- the
+forward pass keeps the stage states of its last windows and the adjoint pass reads them
+instead of recomputing those windows.  Step mode (MGD_OPT_STEP_STORE 1, and the default
+where it fits): the forward pass keeps the state at the start of every step and the inner
+stage states of its last steps; the adjoint pass recomputes three stages per step not held.
+Every storage plan must give the same J, sensitivity, gradient and adjoint state bit for
+bit -- on a 3-D box (z-march kernels, a ragged last window) and on a 2-D monopole forced
+onto the generic kernels (actuator and cost target).  The per-launch timer of the z-march
+kernel counts the forward and recompute launches and the stage copies."""
 import numpy as np
 import pytest
 
@@ -13,16 +16,19 @@ from magudi_amd import cases
 pytestmark = pytest.mark.gpu
 
 
-def test_window_cache_3d_bit_for_bit():
+def test_checkpoint_storage_3d_bit_for_bit():
     from magudi_amd.solver import Region, Solver
     from test_gpu_parity import periodic_box, perturbed
     c = periodic_box((32, 16, 16), dissipation=False)
     nsteps, K = 7, 3                        # windows start at 0, 3, 6 (the last one step)
     q0 = perturbed(c, 41)
     lamT = np.random.default_rng(42).standard_normal((c.nv, c.npoints))
-    out = {}
-    for cache in (0, 1, -1):
+    plans = [(0, 0), (0, 1), (0, -1), (1, 0), (1, 1), (1, -1), (-1, -1)]
+    out, plans_seen = {}, {}
+    for step, cache in plans:
+        print(f"storage step={step} cache={cache}", flush=True)
         R = Region(c)
+        R.set_step_store(step)
         R.set_window_cache(cache)
         R.states[0].conservedVariables = q0
         S = Solver(R, dt=c.dt, nsteps=nsteps, save_interval=K)
@@ -33,29 +39,44 @@ def test_window_cache_3d_bit_for_bit():
         S.runAdjointCheckpointed(terminal=lamT, want_gradient=False)
         n_all, c_all, ms_all = R.kernel_timer_read()
         R.kernel_timer(False)
-        out[cache] = (qT, R.states[0].adjointVariables, n_fwd, c_fwd, n_all, c_all)
+        out[(step, cache)] = (qT, R.states[0].adjointVariables, n_fwd, c_fwd, n_all, c_all)
+        plan = R.checkpoint_plan
+        assert plan["mode"] == ("windows" if step == 0 else "step store"), plan
+        plans_seen[(step, cache)] = plan
         assert ms_fwd > 0 and ms_all >= ms_fwd * (1 - 1e-5)
-    for cache in (1, -1):
-        assert np.array_equal(out[cache][0], out[0][0])
-        assert np.array_equal(out[cache][1], out[0][1]), cache
-    # forward: 4 launches per step; copies into the cached windows (all but the last
-    # stage of a window).  Recompute: the uncached windows of the adjoint pass.
-    assert out[0][2] == 4 * nsteps and out[0][3] == 0
-    assert out[-1][2] == 4 * nsteps and out[-1][3] == 4 * 3 - 1 + 4 * 3 - 1 + 4 * 1
-    assert out[-1][4] == out[-1][2]                      # nothing recomputed
-    assert out[0][4] == 4 * nsteps + 4 * 3 + 4 * 3 + 4 * 1 - 2
-    assert out[1][4] == 4 * nsteps + 4 * 3 + 4 * 3 - 2   # the last window was kept
+    ref = out[(0, 0)]
+    for key in plans[1:]:
+        assert np.array_equal(out[key][0], ref[0]), key
+        assert np.array_equal(out[key][1], ref[1]), key
+    for key in plans:
+        assert out[key][2] == 4 * nsteps, key
+    # window mode -- forward: copies into the cached windows (all but the last stage of a
+    # window); adjoint: the uncached windows recomputed
+    assert out[(0, 0)][3] == 0
+    assert plans_seen[(0, -1)] == {"mode": "windows", "held": 3, "first": 0}
+    assert plans_seen[(0, 1)] == {"mode": "windows", "held": 1, "first": 6}
+    assert out[(0, -1)][3] == 4 * 3 - 1 + 4 * 3 - 1 + 4 * 1
+    assert out[(0, -1)][4] == out[(0, -1)][2]           # nothing recomputed
+    assert out[(0, 0)][4] == 4 * nsteps + 4 * 3 + 4 * 3 + 4 * 1 - 2
+    assert out[(0, 1)][4] == 4 * nsteps + 4 * 3 + 4 * 3 - 2   # the last window was kept
+    # step mode -- forward: the state at the start of steps 1 .. 6 and the inner stage
+    # states of the held steps; adjoint: three stages per step not held
+    for key, held in (((1, 0), 0), ((1, 1), K), ((1, -1), nsteps), ((-1, -1), nsteps)):
+        assert plans_seen[key] == {"mode": "step store", "held": held, "first": nsteps - held}
+        assert out[key][3] == nsteps - 1 + 3 * held, key
+        assert out[key][4] == 4 * nsteps + 3 * (nsteps - held), key
 
 
-@pytest.mark.parametrize("cache", [1, -1])
-def test_window_cache_2d_generic_bit_for_bit(cache):
+@pytest.mark.parametrize("step,cache", [(0, 1), (0, -1), (1, 0), (1, 2), (-1, -1)])
+def test_checkpoint_storage_2d_generic_bit_for_bit(step, cache):
     from magudi_amd.solver import Region, Solver
     c = cases.acoustic_monopole(n=41, nsteps=10, save_interval=4)
     c.dt = 0.2
     q0 = c.initial_state()
     res = []
-    for opt in (0, cache):
+    for st, opt in ((0, 0), (step, cache)):
         R = Region(c, force_plain=True)
+        R.set_step_store(st)
         R.set_window_cache(opt)
         R.states[0].conservedVariables = q0
         rng = np.random.default_rng(9)
