@@ -35,6 +35,9 @@ struct RkStage {
   // events recorded on the stream right before and after the z-march kernel, when set
   hipEvent_t ev_begin = nullptr;
   hipEvent_t ev_end = nullptr;
+  // adjoint stages on the 3-D march path (stress_reuse_path): G already holds the stress /
+  // heat-flux slots of the stage state, so the stress pass is skipped
+  bool stress_ready = false;
 };
 // What launch_rhs_forward did with `rk`.
 enum RkDone { RK_NOT_APPLIED = 0, RK_IN_PLACE = 1, RK_TO_YOUT = 2 };
@@ -56,6 +59,10 @@ bool launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
 // heat-flux kernel (9 slots of the 12-slot gradient buffer S); the march itself, which
 // writes rhs, or with rk (rk->yout set, not q) applies the RK4 stage out of place.
 bool fwd3d_march_path(const Geo& g, const FinishInfo& fin, int nsrc);
+// Whether the forward and the adjoint rhs both run on the 3-D marches, each from the stress
+// / heat-flux slots of its state that launch_stress3d writes into G: then slots the forward
+// stage computed serve the adjoint stage about the same state.
+bool stress_reuse_path(const Geo& g, const FinishInfo& fin, int nsrc);
 void launch_stress3d(const Geo& g, const double* q, double* S, hipStream_t st);
 void launch_fwd3d_march(const Geo& g, const double* q, const double* S, const double* kstr,
                         double* rhs, const RkStage* rk, hipStream_t st);

@@ -2436,6 +2436,10 @@ static bool adj3d_march_path(const Geo& g, const FinishInfo& fin) {
          fwd3d_march_applies(g) && g.N < (1L << 29);   // 32-bit byte offsets
 }
 
+bool stress_reuse_path(const Geo& g, const FinishInfo& fin, int nsrc) {
+  return g.viscous && fwd3d_march_path(g, fin, nsrc) && adj3d_march_path(g, fin);
+}
+
 void launch_stress3d(const Geo& g, const double* q, double* S, hipStream_t st) {
   const dim3 gg = grad3d_grid(g);
   with_zchunk(g, false, [&](auto Z) {
@@ -2530,11 +2534,11 @@ bool launch_rhs_adjoint(const Geo& g, const Patches* P, const double* q, const d
   const dim3 grid = grid_pts(g);
   const int has_farfield = fin.has_farfield;
   if (adj3d_march_path(g, fin)) {
-    // stress / heat-flux pass (9 slots of G), then the two marches; the second finishes
-    // the rhs (and the stage)
+    // stress / heat-flux pass (9 slots of G; skipped where the caller's G already holds
+    // them), then the two marches; the second finishes the rhs (and the stage)
     (void)w;
     (void)grid;
-    launch_stress3d(g, q, G, st);
+    if (!(rk && rk->stress_ready)) launch_stress3d(g, q, G, st);
     const dim3 gm = march_grid(g, true);
     with_march(g, true, [&](auto X, auto Z) {
       constexpr int BX = decltype(X)::value, ZC = decltype(Z)::value;

@@ -156,6 +156,11 @@ struct mgd_region {
   mgd::DevBuf sstore, scache;
   int nscache = 0, scache_first = 0;
   int rec_step = -1;                     // step whose inner stage states `window` holds
+  // step mode on the 3-D march paths: the stress / heat-flux slots of the recomputed
+  // stages 1-3 (three gradient buffers), which the adjoint stages about the same states
+  // read instead of running the stress pass again
+  mgd::DevBuf sbuf;
+  bool sreuse = false;
   // per-launch timer of the z-march forward kernel (mgd_kernel_timer): event pairs
   bool timing = false;
   std::vector<hipEvent_t> tev;
@@ -513,7 +518,7 @@ void check_stage_input(mgd_region* r, const double* q) {
 // buffers trade places, so qb holds the new state either way; every RK4 step has four
 // stages, so after whole steps each buffer is back where it started.
 void forward_stage(mgd_region* r, DevBuf& qb, DevBuf& alt, double* b1, double* b2, int sidx,
-                   int stage, double dt, double* qcopy) {
+                   int stage, double dt, double* qcopy, double* Gbuf = nullptr) {
   double* q = qb.p;
   check_stage_input(r, q);
   const int na = n_act(r);
@@ -540,7 +545,8 @@ void forward_stage(mgd_region* r, DevBuf& qb, DevBuf& alt, double* b1, double* b
       rk.ev_end = r->tev[k + 1];
     }
   }
-  const int done = launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p, r->G.p, r->t.p,
+  const int done = launch_rhs_forward(r->geo, dpatches(r), q, r->rhs.p, r->w.p,
+                                      Gbuf ? Gbuf : r->G.p, r->t.p,
                                       r->T1.p, ctrl, cstride,
                                       reinterpret_cast<const Source*>(r->d_sources.p), ns,
                                       r->src_amp.p + (long)sidx * std::max(ns, 1),
@@ -1012,7 +1018,8 @@ double* cached_window(mgd_region* r, int lt) {
 size_t storage_budget(mgd_region* r) {
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
-  size_t held = r->window.bytes() + r->snaps.bytes() + r->sstore.bytes() + r->scache.bytes();
+  size_t held = r->window.bytes() + r->snaps.bytes() + r->sstore.bytes() + r->scache.bytes() +
+                r->sbuf.bytes();
   for (int k = 0; k < mgd_region::kMaxCache; ++k) held += r->wcache[k].bytes();
   const size_t reserve = std::max((size_t)8 << 30, tot / 10);
   return fr + held > reserve ? fr + held - reserve : 0;
@@ -1062,6 +1069,7 @@ int plan_plain_storage(mgd_region* r, int nsteps, int K, long M) {
     budget = storage_budget(r);
     if (r->opt_step < 0) step = r->opt_cache < 0 && (size_t)(nsteps + 3) * S <= budget;
   }
+  size_t fixed = (size_t)(nsteps + 3) * S;
   if (step) {
     for (int k = 0; k < mgd_region::kMaxCache; ++k) r->wcache[k].release();
     r->ncache = 0;
@@ -1074,10 +1082,29 @@ int plan_plain_storage(mgd_region* r, int nsteps, int K, long M) {
       step = false;
     }
   }
+  r->sreuse = false;
+  if (step && stress_reuse_path(r->geo, r->fin, (int)r->sources.size()) && r->G.n > 0 &&
+      fixed + 3 * r->G.bytes() <= budget) {
+    if (r->sbuf.alloc(3 * r->G.n) == MGD_OK) {
+      r->sreuse = true;
+      fixed += 3 * r->G.bytes();
+    } else {
+      (void)hipGetLastError();
+    }
+  }
+  if (!r->sreuse) r->sbuf.release();
   if (step) {
-    const size_t rest = budget > (size_t)(nsteps + 3) * S ? budget - (size_t)(nsteps + 3) * S : 0;
+    const size_t rest = budget > fixed ? budget - fixed : 0;
     int nsc = (int)std::min<size_t>((size_t)nsteps, rest / (3 * S));
     if (r->opt_cache >= 0) nsc = std::min(nsc, r->opt_cache * K);
+    // a cache held from an earlier run stays unless the plan is well above it (the free
+    // memory moves a little between runs; reallocating ~100 GB each run would cost more
+    // than the few steps it adds)
+    const size_t step3 = (size_t)3 * M;
+    const int have = r->scache.n % step3 == 0 ? (int)(r->scache.n / step3) : -1;
+    if (have > 0 && have <= nsteps && (r->opt_cache < 0 || have <= r->opt_cache * K) &&
+        nsc <= have + std::max(2, have / 16))
+      nsc = have;
     if (r->scache.n != (size_t)3 * nsc * M) r->scache.release();
     while (nsc > 0 && r->scache.alloc((size_t)3 * nsc * M) != MGD_OK) {
       (void)hipGetLastError();             // fewer steps than planned: the rest recompute
@@ -1090,6 +1117,7 @@ int plan_plain_storage(mgd_region* r, int nsteps, int K, long M) {
   }
   r->sstore.release();
   r->scache.release();
+  r->sbuf.release();
   r->nscache = 0;
   const int nsnap = nsteps / K + 1;
   if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M)) return MGD_ERR_HIP;
@@ -1105,7 +1133,8 @@ void recompute_step(mgd_region* r, int t, double dt) {
   double* const p0 = r->qs.p;
   for (int st = 1; st <= 3; ++st)
     forward_stage(r, r->qs, r->qs_alt, r->b1.p, r->b2.p, 4 * t + st - 1, st, dt,
-                  r->window.p + (long)(st - 1) * M);
+                  r->window.p + (long)(st - 1) * M,
+                  r->sreuse ? r->sbuf.p + (long)(st - 1) * (long)r->G.n : nullptr);
   if (r->qs.p != p0) {
     std::swap(r->qs.p, r->qs_alt.p);
     std::swap(r->qs.n, r->qs_alt.n);
@@ -1168,7 +1197,7 @@ int plain_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
 // One adjoint RK4 stage (stage i of step t) of the generic-kernel path, linearised about
 // the forward stage state qcur.
 void plain_adjoint_stage(mgd_region* r, double dt, int nsteps, int t, int i, const double* qcur,
-                         bool want_grad) {
+                         bool want_grad, double* stress = nullptr) {
   const int na = n_act(r), nc = n_cost(r), nst = 4 * nsteps;
   const long M = (long)r->ninst * M_of(r);
   const int sidx = 4 * t + i - 1;
@@ -1184,10 +1213,12 @@ void plain_adjoint_stage(mgd_region* r, double dt, int nsteps, int t, int i, con
                        r->stream);
   const double factor = i == 4 ? 2.0 : (i == 2 ? 0.5 : 1.0);
   // the rhs launcher applies the RK update itself where its last kernel finishes the rhs
-  const RkStage rk{i, dt, r->lam.p, r->b1a.p, r->b2a.p};
-  if (!launch_rhs_adjoint(r->geo, dpatches(r), qcur, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
-                          r->T1.p, r->S.p, nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc,
-                          factor, r->fin, r->stream, &rk))
+  RkStage rk{i, dt, r->lam.p, r->b1a.p, r->b2a.p};
+  rk.stress_ready = stress != nullptr;
+  if (!launch_rhs_adjoint(r->geo, dpatches(r), qcur, r->lam.p, r->rhs.p, r->w.p,
+                          stress ? stress : r->G.p, r->t.p, r->T1.p, r->S.p,
+                          nc > 0 ? r->aforce.p : nullptr, (long)r->nv * nc, factor, r->fin,
+                          r->stream, &rk))
     launch_rk_adjoint(M, i, dt, r->lam.p, r->b1a.p, r->b2a.p, r->rhs.p, r->stream);
 }
 
@@ -1204,8 +1235,11 @@ int plain_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
     // every step's inner stage states are recomputed in this pass (a captured pass replays
     // the recompute too), the held ones read from the step cache
     for (int t = nsteps - 1; t >= 0; --t) {
+      // a recomputed step left the stress slots of its stage states 1-3 in sbuf
+      const bool reuse = r->sreuse && t < r->scache_first;
       for (int i = 4; i >= 1; --i)
-        plain_adjoint_stage(r, dt, nsteps, t, i, step_stage_state(r, t, i, dt, i == 4), want_grad);
+        plain_adjoint_stage(r, dt, nsteps, t, i, step_stage_state(r, t, i, dt, i == 4), want_grad,
+                            reuse && i < 4 ? r->sbuf.p + (long)(i - 1) * (long)r->G.n : nullptr);
     }
     r->rec_step = -1;
     return MGD_OK;
@@ -1412,6 +1446,8 @@ std::vector<uintptr_t> graph_key(mgd_region* r, int kind, std::initializer_list<
   k.push_back((uintptr_t)r->nscache);
   k.push_back((uintptr_t)r->sstore.p);
   k.push_back((uintptr_t)r->scache.p);
+  k.push_back((uintptr_t)r->sreuse);
+  k.push_back((uintptr_t)r->sbuf.p);
   for (uintptr_t e : extra) k.push_back(e);
   return k;
 }

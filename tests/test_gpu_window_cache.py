@@ -116,3 +116,30 @@ def test_march_shape_changes_no_value():
     for qT, lam in out[1:]:
         assert np.array_equal(qT, out[0][0])
         assert np.array_equal(lam, out[0][1])
+
+
+def test_step_store_kolmogorov_bit_for_bit():
+    """The Kolmogorov flow (constant transport coefficients) on 32^3, 6 steps from the
+    terminal kinetic energy: the step store recomputing every step -- whose adjoint stages
+    1-3 read the stress slots of the recompute instead of running the stress pass --, and
+    holding the inner states of the last 2 steps, against snapshots and windows, bit for
+    bit."""
+    from magudi_amd.solver import Region, Solver
+    c = cases.kolmogorov3d(n=32, nsteps=6)
+    out = {}
+    for step, cache in ((0, 0), (1, 0), (1, 1)):
+        print(f"kolmogorov storage step={step} cache={cache}", flush=True)
+        R = Region(c)
+        R.set_step_store(step)
+        R.set_window_cache(cache)
+        R.states[0].initialCondition = c.initial_state()
+        S = Solver(R, dt=c.dt, nsteps=c.nsteps, save_interval=2)
+        R.restart()
+        S.runForwardCheckpointed()
+        J = R.terminalKineticEnergy()
+        S.runAdjointCheckpointed(terminal=True, want_gradient=False)
+        out[(step, cache)] = (J, R.states[0].conservedVariables, R.states[0].adjointVariables)
+        assert R.checkpoint_plan["held"] == (2 if (step, cache) == (1, 1) else 0)
+    for key in ((1, 0), (1, 1)):
+        for a, b in zip(out[key], out[(0, 0)]):
+            assert np.array_equal(a, b), key
