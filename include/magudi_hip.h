@@ -299,11 +299,14 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches);
  * the first run, 0: 32 x 16, 1: 16 x 32, 2: 16 x 16, 3: 24 x 16 points with one thread
  * per point, 4: 16 x 16 and 5: 16 x 12 points with two threads per point.  No result
  * depends on the tile shape.
- * MGD_OPT_RESIDENT_STATES = 1 makes the fused 2-D path keep every RK stage state of a
- * forward run in device memory ((4 nsteps + 1) states, e.g. 19.8 GB for 401^2 x 960
- * steps) so that the adjoint pass reads them instead of recomputing each window from
- * its snapshot; results are identical.  Default 0: snapshots every save_interval steps
- * and window recompute (UniformCheckpointer, src/UniformCheckpointerImpl.f90:78-208).
+ * MGD_OPT_RESIDENT_STATES (fused 2-D path): 1 keeps every RK stage state of a
+ * checkpointed forward run in device memory ((4 nsteps + 1) states, e.g. 19.8 GB for
+ * 401^2 x 960 steps) so that the adjoint pass reads them instead of recomputing each
+ * window from its snapshot; 0: snapshots every save_interval steps and window recompute
+ * (UniformCheckpointer, src/UniformCheckpointerImpl.f90:78-208); -1 (default): resident
+ * where the states fit in the device memory left free at the run's preparation less a
+ * reserve of max(8 GiB, a tenth of the device), except for the members of a group
+ * launch.  Identical results.
  * MGD_OPT_WINDOW_CACHE (generic-kernel path, e.g. 3-D regions): the checkpointed forward
  * pass also keeps the stage states of its last n windows (4 save_interval states each) and
  * the adjoint passes read them instead of recomputing those windows; identical results.

@@ -122,7 +122,9 @@ struct mgd_region {
   mgd::DevBuf window2, b2r, qsr;
   // MGD_OPT_RESIDENT_STATES: the forward pass keeps every RK stage state (4 nsteps + 1
   // states) in HBM and the adjoint pass reads them instead of recomputing windows
-  bool resident = false;
+  bool resident = false;                 // this run's choice
+  int opt_resident = -1;                 // -1 where every stage state fits, 0 off, 1 on
+  bool group_run = false;                // preparing as a member of a group launch
   mgd::DevBuf wall;
   // the last checkpointed forward pass left the stage states of its last window in
   // `window` (the adjoint pass then starts without recomputing that window)
@@ -1019,7 +1021,7 @@ size_t storage_budget(mgd_region* r) {
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
   size_t held = r->window.bytes() + r->snaps.bytes() + r->sstore.bytes() + r->scache.bytes() +
-                r->sbuf.bytes();
+                r->sbuf.bytes() + r->wall.bytes();
   for (int k = 0; k < mgd_region::kMaxCache; ++k) held += r->wcache[k].bytes();
   const size_t reserve = std::max((size_t)8 << 30, tot / 10);
   return fr + held > reserve ? fr + held - reserve : 0;
@@ -1326,15 +1328,27 @@ int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, c
     r->step_mode = false;
     r->sstore.release();
     r->scache.release();
+    r->sbuf.release();
+    // every stage state resident (MGD_OPT_RESIDENT_STATES): asked for, or by default where
+    // they fit in storage_budget (not for members of a group launch, which recompute)
+    const size_t need = (size_t)(4L * nsteps + 1) * M;
+    bool res = r->opt_resident == 1;
+    if (r->opt_resident < 0 && !r->group_run) res = need * sizeof(double) <= storage_budget(r);
+    if (!res) r->wall.release();
+    if (res && r->wall.alloc(need)) {
+      if (r->opt_resident == 1) {
+        set_error("not enough device memory to keep every stage state resident");
+        return MGD_ERR_HIP;
+      }
+      (void)hipGetLastError();
+      res = false;
+    }
+    r->resident = res;
     if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M)) return MGD_ERR_HIP;
     if (int rc = plan_window_cache(r, nsteps, K, M)) return rc;
   }
   if (fused_ok(r)) {
     if (int rc = prepare_quadrature(r, nst, dt)) return rc;
-    if (r->resident && r->wall.alloc((size_t)(4L * nsteps + 1) * M)) {
-      set_error("not enough device memory to keep every stage state resident");
-      return MGD_ERR_HIP;
-    }
   }
   return MGD_OK;
 }
@@ -2218,7 +2232,13 @@ int mgd_region_set_option(mgd_region* r, int option, int value) {
   if (!r) { set_error("null region"); return MGD_ERR_ARG; }
   switch (option) {
     case MGD_OPT_FORCE_PLAIN: r->force_plain = value != 0; return MGD_OK;
-    case MGD_OPT_RESIDENT_STATES: r->resident = value != 0; return MGD_OK;
+    case MGD_OPT_RESIDENT_STATES:
+      if (value < -1 || value > 1) {
+        set_error("resident states: -1 (where they fit), 0 (off) or 1 (on)");
+        return MGD_ERR_ARG;
+      }
+      r->opt_resident = value;
+      return MGD_OK;
     case MGD_OPT_WINDOW_CACHE: r->opt_cache = value < 0 ? -1 : value; return MGD_OK;
     case MGD_OPT_STEP_STORE:
       if (value < -1 || value > 1) {
@@ -2595,6 +2615,12 @@ int mgd_checkpoint_migrate(mgd_region* r, int timestep, int stage) {
   }
   const long M = (long)r->ninst * M_of(r);
   const int ts = timestep, st = stage;
+  if (fused_ok(r) && r->resident) {
+    // state 4 (ts - 1) + st of the resident stage states (4 ts: the state at step ts)
+    launch_copy(M, r->wall.p + (4L * (ts - 1) + st) * M, r->q.p, r->stream);
+    MGD_HIP(hipGetLastError());
+    return MGD_OK;
+  }
   if (r->step_mode) {
     // the state at the start of step ts, or the output of stage st of step ts - 1
     const double* src;
@@ -2951,7 +2977,7 @@ int mgd_group_run_adjoint(mgd_group* G, double t0, const double* dt, int nsteps,
   for (int k = 0; k < n; ++k) {
     mgd_region* r = G->m[k];
     if (!(dt[k] > 0.0)) { set_error("bad time step"); return MGD_ERR_ARG; }
-    if (!fused_ok(r) || r->resident) {
+    if (!fused_ok(r) || r->opt_resident == 1) {
       set_error("group members must run on the fused 2-D path without resident stage states");
       return MGD_ERR_UNSUPPORTED;
     }
@@ -2964,9 +2990,11 @@ int mgd_group_run_adjoint(mgd_group* G, double t0, const double* dt, int nsteps,
     shapes.push_back(r->tiling.shape);
     r->stream = G->st;
   }
+  for (mgd_region* r : G->m) r->group_run = true;
   auto restore = [&]() {
     for (int k = 0; k < n; ++k) {
       mgd_region* r = G->m[k];
+      r->group_run = false;
       r->stream = streams[k];
       if (r->tiling.shape != shapes[k]) (void)install_tiling(r, shapes[k]);
     }

@@ -90,7 +90,7 @@ class Region:
     Solver%setup (sponge strengths, far-field penalties, mean pressure)."""
 
     def __init__(self, case: Case, n_instances: int = 1, force_plain: bool = False,
-                 tile_shape: int = -1, resident_states: bool = False):
+                 tile_shape: int = -1, resident_states: Optional[bool] = None):
         self.case = case
         self.nd = case.nd
         self.nv = case.nv
@@ -121,9 +121,12 @@ class Region:
             check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_FORCE_PLAIN, 1))
         if tile_shape != -1:
             check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_TILE_SHAPE, tile_shape))
-        if resident_states:
-            # every RK stage state of a forward run stays in HBM; no window recompute
-            check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_RESIDENT_STATES, 1))
+        if resident_states is not None:
+            # True: every RK stage state of a checkpointed forward run stays in HBM (no window
+            # recompute); False: snapshots and window recompute; None (default): resident
+            # where the states fit in the device memory left free
+            check(capi.lib().mgd_region_set_option(self.handle, capi.OPT_RESIDENT_STATES,
+                                                   1 if resident_states else 0))
         for p in case.patches:
             pd = capi.PatchDesc()
             pd.type = p.type
