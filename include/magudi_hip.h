@@ -209,11 +209,12 @@ int mgd_functional(mgd_region* r, double* out);
 int mgd_run_forward(mgd_region* r, double t0, double dt, int nsteps, const double* control,
                     double* J, int use_graph);
 
-/* Solver%runAdjoint with uniform checkpointing held in HBM
+/* Solver%runAdjoint with its checkpoints held in HBM
  * (src/SolverImpl.f90:914-1245, src/UniformCheckpointerImpl.f90:78-208,
  * src/RK4IntegratorImpl.f90:164-270) starting from the current conserved state as
- * the initial condition: runs the forward pass (storing a snapshot every
- * save_interval steps), then marches the adjoint back with window recompute.
+ * the initial condition: runs the forward pass (keeping its checkpoints as
+ * mgd_run_forward_checkpointed does), then marches the adjoint back, recomputing the
+ * stage states not held.
  * gradient: per instance, per stage (forward order), per actuator point (may be NULL);
  * sensitivity and J: per instance.  The adjoint state at t0 stays in the region
  * (MGD_ADJOINT_VARS). */
@@ -233,9 +234,12 @@ int mgd_terminal_cost(mgd_region* r, int kind, double* J);
 /* The two halves of mgd_run_adjoint, for drivers that set the adjoint terminal condition
  * from the forward result (multiple shooting, bin/msadjoint.f90:287-384).
  * mgd_run_forward_checkpointed: the forward pass of Solver%runAdjoint from the current
- * conserved state, keeping a snapshot every save_interval steps in HBM (the .q files of
- * showProgress, src/SolverImpl.f90:672-912) and the final state (MGD_FINAL_STATE).
- * mgd_run_adjoint_checkpointed: the adjoint march back over those snapshots.
+ * conserved state, keeping its checkpoints in HBM (the .q files of showProgress,
+ * src/SolverImpl.f90:672-912: the state of every step and stage states of the last
+ * steps, snapshots every save_interval steps and windows, or every stage state, per
+ * MGD_OPT_STEP_STORE / MGD_OPT_WINDOW_CACHE / MGD_OPT_RESIDENT_STATES) and the final
+ * state (MGD_FINAL_STATE).
+ * mgd_run_adjoint_checkpointed: the adjoint march back over those checkpoints.
  * terminal = 0 starts from zero, terminal = 1 from the current MGD_ADJOINT_VARS
  * (adjoint_nonzero_initial_condition, loadInitialCondition ADJOINT,
  * src/SolverImpl.f90:368-428); the terminal cost forcing of the final state is added in
@@ -262,8 +266,10 @@ int mgd_run_adjoint_checkpointed(mgd_region* r, int terminal, double* gradient,
  * mgd_checkpoint_migrate: t_ReverseMigrator%migrateTo (include/ReverseMigrator.f90:21,
  * 92-106; UniformCheckpointer, src/UniformCheckpointerImpl.f90:78-208): MGD_CONSERVED :=
  * the forward state after stage `stage` of time step `timestep` (1-based; (0, 4) is the
- * initial condition) of the last mgd_run_forward_checkpointed, recomputing the window of
- * save_interval steps from its HBM snapshot when the state is not in the window held.
+ * initial condition) of the last mgd_run_forward_checkpointed, from the run's HBM storage
+ * (MGD_OPT_STEP_STORE, MGD_OPT_RESIDENT_STATES): recomputing the step's inner stage
+ * states from the stored step state, or the window of save_interval steps from its
+ * snapshot, when the state is not held.
  *
  * mgd_control_gradient: ThermalActuator%updateGradient and computeSensitivity
  * (src/ThermalActuatorImpl.f90:83-159, 383-443) at the current adjoint state: gradient per

@@ -364,7 +364,7 @@ class RK4Integrator:
 
 class UniformCheckpointer:
     """t_UniformCheckpointer's migrateTo (src/UniformCheckpointerImpl.f90:78-208) over the HBM
-    snapshots of the region's last Solver.runForwardCheckpointed."""
+    checkpoints of the region's last Solver.runForwardCheckpointed."""
 
     def migrateTo(self, region: "Region", timestep: int, stage: int):
         check(capi.lib().mgd_checkpoint_migrate(region.handle, timestep, stage))
@@ -392,7 +392,7 @@ class Solver:
 
     def runAdjoint(self, control: Optional[np.ndarray] = None, t0: float = 0.0,
                    want_gradient: bool = True, use_graph: bool = False):
-        """Solver%runAdjoint (forward pass with HBM snapshots, then the discrete adjoint).
+        """Solver%runAdjoint (forward pass with HBM checkpoints, then the discrete adjoint).
 
         Returns (sensitivity, gradient, J): sensitivity and J per instance; gradient per
         instance, per RK stage in forward order, per actuator patch point."""
@@ -411,7 +411,7 @@ class Solver:
     def runForwardCheckpointed(self, control: Optional[np.ndarray] = None,
                                t0: float = 0.0) -> np.ndarray:
         """First half of Solver%runAdjoint (src/SolverImpl.f90:914-1060): the forward run
-        from the current conserved state with HBM snapshots every saveInterval steps.
+        from the current conserved state with HBM checkpoints (Region.checkpoint_plan).
         Returns J (the time-integrated functional) per instance; afterwards the
         conserved state is the final state."""
         r = self.region
@@ -424,7 +424,7 @@ class Solver:
     def runAdjointCheckpointed(self, terminal: Optional[np.ndarray] = None,
                                want_gradient: bool = True):
         """Second half of Solver%runAdjoint: the discrete adjoint marched back from the
-        snapshots of the last runForwardCheckpointed.  ``terminal`` is the adjoint state at
+        checkpoints of the last runForwardCheckpointed.  ``terminal`` is the adjoint state at
         the final time per instance, shape (n_instances, nv, N) or (nv, N) for one
         instance -- the reference's adjoint_nonzero_initial_condition
         (src/SolverImpl.f90:368-428, 1096-1102), e.g. the gradient of a cost on the final
