@@ -590,10 +590,11 @@ def main():
     # dominant kernel (by launches x time: 4 nsteps forward launches per pass plus those of
     # the recompute, against 4 nsteps launches of each adjoint kernel), timed with
     # HIP events around each of its launches in the timed passes.  Its algorithmic bytes
-    # per launch: the stage state and the 9 stress / heat-flux slots in, the RK buffers in
-    # and out (0, 2, 2, 1 states in; 3, 2, 2, 1 out, over the 4 stages) and the new stage
-    # state out -- 242 B per point -- plus 40 B per point for the launches that also write
-    # the state into the step store, the stage cache or the recompute buffer.
+    # per launch: the stage state and the 9 stress / heat-flux slots in, the RK buffers
+    # and the new stage state (0, 2, 2, 1 states in; 2, 2, 2, 1 out, over the 4 stages:
+    # stages 2 and 3 read the state at the start of the step where stage 1 read it) --
+    # 232 B per point -- plus 40 B per point for the launches that also write the state
+    # into the step store, the stage cache or the recompute buffer.
     K.R.restart()
     cms, cb = K.R.time_kernel(capi.KERNEL_COPY, reps=3)
     f_ms, f_b = K.R.time_kernel(capi.KERNEL_RHS_FORWARD, reps=3)
@@ -602,7 +603,7 @@ def main():
     K.R.close()                 # its step store holds most of the device memory
     del K
     ms = march_ms / max(launches, 1)
-    b = (242.0 + 40.0 * with_copy / max(launches, 1)) * M_bytes / 40.0
+    b = (232.0 + 40.0 * with_copy / max(launches, 1)) * M_bytes / 40.0
     achieved = b / (ms * 1e-3) / 1e9
     traffic = None
     if not a.no_traffic:
@@ -640,8 +641,8 @@ def main():
                        "achieved_GBs": round(cb / (cms * 1e-3) / 1e9, 1)},
     }
     if traffic is not None:
-        # the counter passes run the kernel's launches without the state copy (242 B / pt)
-        out["roofline"]["traffic_over_algorithmic"] = round(traffic / (242.0 * M_bytes / 40.0), 3)
+        # the counter passes run the kernel's launches without the state copy (232 B / pt)
+        out["roofline"]["traffic_over_algorithmic"] = round(traffic / (232.0 * M_bytes / 40.0), 3)
         out["roofline"]["traffic_workload"] = "launches without the state copy (--kernels-only)"
     if not a.no_extras:
         out["monopole401"] = monopole_legs(a, extras=True)

@@ -311,13 +311,13 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches);
  * window from its snapshot; 0: snapshots every save_interval steps and window recompute
  * (UniformCheckpointer, src/UniformCheckpointerImpl.f90:78-208); -1 (default): resident
  * where the states fit in the device memory left free at the run's preparation less a
- * reserve of max(8 GiB, a tenth of the device), except for the members of a group
+ * reserve of max(8 GiB, 1/32 of the device), except for the members of a group
  * launch.  Identical results.
  * MGD_OPT_WINDOW_CACHE (generic-kernel path, e.g. 3-D regions): the checkpointed forward
  * pass also keeps the stage states of its last n windows (4 save_interval states each) and
  * the adjoint passes read them instead of recomputing those windows; identical results.
  * -1 (default): as many windows as fit in the device memory left free at the run's
- * preparation less a reserve of max(8 GiB, a tenth of the device), at most 8; 0: none;
+ * preparation less a reserve of max(8 GiB, 1/32 of the device), at most 8; 0: none;
  * n > 0: at most n.
  * MGD_OPT_Z_CHUNK: planes per work unit of the 3-D z marches (periodic 3-D regions): 0
  * (default) autotune at the first run -- one forward RK4 step and one adjoint rhs timed

@@ -1627,7 +1627,8 @@ __device__ __forceinline__ void march_halo(const Geo& g, int t, int x0, int y0, 
 // zo = zl - 4, rhs = 0 - ((D_x Fx + D_y Fy) + D_z Fz) -- zo's ring slot is read before
 // zl's partial takes it over -- then the Jacobian scaling and the
 // forcing; with rk_stage > 0 the RK4 stage update follows and goes to yout, which must
-// not be q (other blocks still read q), and rhs is not written.  The fluxes, sums and
+// not be q (other blocks still read q), and rhs is not written; at stage 1 b1 may be
+// null (the caller keeps q itself as the state at the start of the step).  The fluxes, sums and
 // finishing operations are those of k_fluxdiv3d_xy and k_fwd_finish_z, in the same
 // order, so the values are the same; no flux or partial rhs goes through HBM.
 template <int BX, int BY, int ZC>
@@ -1757,7 +1758,7 @@ __global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_fwd3d_march(Geo g,
           switch (rk_stage) {
             case 1: {
               const double q0 = q[ii];
-              b1[ii] = q0;
+              if (b1) b1[ii] = q0;
               b2[ii] = q0 + div6(dt * rl);
               qn = q0 + dt * rl / 2.0;
               break;

@@ -138,6 +138,9 @@ struct mgd_region {
   } ck;
   mgd::DevBuf qT, scr;                   // final forward state; scratch state (MGD_SCRATCH)
   mgd::DevBuf q_alt, qs_alt;             // second state buffers of out-of-place RK stages
+  // third state buffers of the 3-D z-march stages: stages 2 and 3 write there, so the
+  // state at the start of the step stays where stage 1 read it and no copy of it is made
+  mgd::DevBuf q_alt2, qs_alt2;
   mgd::DevBuf lam_alt;                   // second adjoint buffer of out-of-place adjoint stages
   int mig_loaded = -1;                   // snapshot step whose window mgd_checkpoint_migrate holds
   // window cache of the generic-kernel path: the checkpointed forward pass also keeps the
@@ -434,7 +437,7 @@ int ensure_state(mgd_region* r) {
     return MGD_ERR_HIP;
   if (r->desc.dissipation && !r->desc.composite_dissipation && r->t.alloc(M)) return MGD_ERR_HIP;
   // 3-D regions: the z-march rhs kernel writes RK stage states out of place
-  if (r->nd == 3 && r->q_alt.alloc(M)) return MGD_ERR_HIP;
+  if (r->nd == 3 && (r->q_alt.alloc(M) || r->q_alt2.alloc(M))) return MGD_ERR_HIP;
   if (r->desc.viscous) {
     if (r->w.alloc((size_t)r->ninst * (r->nd + 1) * r->N) ||
         r->G.alloc((size_t)r->ninst * (r->nd + 1) * r->nd * r->N))
@@ -517,8 +520,12 @@ void check_stage_input(mgd_region* r, const double* q) {
 
 // One forward RK4 substep on the state in `qb` (all instances).  Where the rhs kernels
 // write the new stage state out of place (the 3-D z march) it goes to `alt` and the two
-// buffers trade places, so qb holds the new state either way; every RK4 step has four
-// stages, so after whole steps each buffer is back where it started.
+// buffers trade places, so qb holds the new state either way.  With the third buffer of
+// the pair (q_alt2, qs_alt2) stages 2 and 3 write there instead and read the state at the
+// start of the step from `alt`, where stage 1 left it, so stage 1 writes no copy of it
+// (buffers A B C: stage 1 A -> B, 2 B -> C, 3 C -> B, 4 B -> A, reading A in 2 and 3);
+// every RK4 step has four stages, so after whole steps each buffer is back where it
+// started.
 void forward_stage(mgd_region* r, DevBuf& qb, DevBuf& alt, double* b1, double* b2, int sidx,
                    int stage, double dt, double* qcopy, double* Gbuf = nullptr) {
   double* q = qb.p;
@@ -535,6 +542,15 @@ void forward_stage(mgd_region* r, DevBuf& qb, DevBuf& alt, double* b1, double* b
   const long M = (long)r->ninst * M_of(r);
   RkStage rk{stage, dt, q, b1, b2, qcopy};
   if (alt.p && (long)alt.n >= M) rk.yout = alt.p;   // allocated by the run preparation
+  DevBuf* alt2 = &qb == &r->q ? &r->q_alt2 : (&qb == &r->qs ? &r->qs_alt2 : nullptr);
+  const bool rot = rk.yout && alt2 && alt2->p && (long)alt2->n >= M && r->T1.p &&
+                   fwd3d_march_path(r->geo, r->fin, (int)r->sources.size());
+  if (rot && stage == 1) {
+    rk.b1 = nullptr;
+  } else if (rot && (stage == 2 || stage == 3)) {
+    rk.b1 = alt.p;
+    rk.yout = alt2->p;
+  }
   if (r->timing) {
     const size_t k = 2 * (size_t)r->tcount;
     while (r->tev.size() < k + 2) {
@@ -554,8 +570,9 @@ void forward_stage(mgd_region* r, DevBuf& qb, DevBuf& alt, double* b1, double* b
                                       r->src_amp.p + (long)sidx * std::max(ns, 1),
                                       r->src_inst_stride, r->fin, r->stream, &rk);
   if (done == RK_TO_YOUT) {
-    std::swap(qb.p, alt.p);
-    std::swap(qb.n, alt.n);
+    DevBuf& out = rot && (stage == 2 || stage == 3) ? *alt2 : alt;
+    std::swap(qb.p, out.p);
+    std::swap(qb.n, out.n);
     if (rk.ev_begin) {          // the z march ran between the two events
       ++r->tcount;
       if (qcopy) ++r->tcopy;
@@ -1016,14 +1033,15 @@ double* cached_window(mgd_region* r, int lt) {
 }
 
 // Device memory a new storage plan of a checkpointed run may take: what is free plus the
-// storage held for earlier runs, less a reserve of max(8 GiB, a tenth of the device).
+// storage held for earlier runs, less a reserve of max(8 GiB, 1/32 of the device) for
+// what the runtime and other allocations of the process take later.
 size_t storage_budget(mgd_region* r) {
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
   size_t held = r->window.bytes() + r->snaps.bytes() + r->sstore.bytes() + r->scache.bytes() +
                 r->sbuf.bytes() + r->wall.bytes();
   for (int k = 0; k < mgd_region::kMaxCache; ++k) held += r->wcache[k].bytes();
-  const size_t reserve = std::max((size_t)8 << 30, tot / 10);
+  const size_t reserve = std::max((size_t)8 << 30, tot / 32);
   return fr + held > reserve ? fr + held - reserve : 0;
 }
 
@@ -1320,7 +1338,8 @@ int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, c
   if (control && na > 0) r->ck.control.assign(control, control + (size_t)r->ninst * nst * na);
   else r->ck.control.clear();
   if (int rc = upload_run_inputs(r)) return rc;
-  if (r->qs.alloc(M) || r->qT.alloc(M) || (r->nd == 3 && r->qs_alt.alloc(M))) return MGD_ERR_HIP;
+  if (r->qs.alloc(M) || r->qT.alloc(M) || (r->nd == 3 && (r->qs_alt.alloc(M) || r->qs_alt2.alloc(M))))
+    return MGD_ERR_HIP;
   if (!fused_ok(r)) {
     if (int rc = plan_plain_storage(r, nsteps, K, M)) return rc;
   } else {
@@ -1453,6 +1472,8 @@ std::vector<uintptr_t> graph_key(mgd_region* r, int kind, std::initializer_list<
   for (const DevBuf* b : bufs) k.push_back((uintptr_t)b->p);
   k.push_back((uintptr_t)r->q_alt.p);
   k.push_back((uintptr_t)r->qs_alt.p);
+  k.push_back((uintptr_t)r->q_alt2.p);
+  k.push_back((uintptr_t)r->qs_alt2.p);
   k.push_back((uintptr_t)r->ncache);
   k.push_back((uintptr_t)r->cache_first);
   for (int c = 0; c < r->ncache; ++c) k.push_back((uintptr_t)r->wcache[c].p);
@@ -2857,15 +2878,18 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
         launch_rhs_adjoint(r->geo, dpatches(r), r->q.p, r->lam.p, r->rhs.p, r->w.p, r->G.p, r->t.p,
                            r->T1.p, r->S.p, nullptr, 0, 1.0, r->fin, r->stream);
         break;
-      case MGD_KERNEL_FWD_MARCH3D:
-        // one RK4 step of march launches, ping-pong between qs and qy from the stress
-        // slots formed once (the timing does not depend on their values)
+      case MGD_KERNEL_FWD_MARCH3D: {
+        // one RK4 step of march launches as the runs issue them (qs -> qy -> qs_alt2 -> qy
+        // -> qs, stages 2 and 3 reading the state at the start of the step from qs), from
+        // the stress slots formed once (the timing does not depend on their values)
+        double* const in[4] = {r->qs.p, r->qy.p, r->qs_alt2.p, r->qy.p};
+        double* const out[4] = {r->qy.p, r->qs_alt2.p, r->qy.p, r->qs.p};
         for (int i = 1; i <= 4; ++i) {
-          RkStage rk{i, dt, (i % 2) ? r->qs.p : r->qy.p, r->b1.p, r->b2.p, nullptr,
-                     (i % 2) ? r->qy.p : r->qs.p};
+          RkStage rk{i, dt, in[i - 1], i == 1 ? nullptr : r->qs.p, r->b2.p, nullptr, out[i - 1]};
           launch_fwd3d_march(r->geo, rk.y, r->G.p, r->fin.kolmogorov, r->rhs.p, &rk, r->stream);
         }
         break;
+      }
     }
   };
   if (which == MGD_KERNEL_FWD_STAGE || which == MGD_KERNEL_ADJ_STAGE) {
@@ -2884,8 +2908,8 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
       return MGD_ERR_UNSUPPORTED;
     }
     per_rep = 4;
+    if (r->qs_alt2.alloc(M)) return MGD_ERR_HIP;
     launch_copy(M, r->q.p, r->qs.p, r->stream);
-    launch_copy(M, r->q.p, r->b1.p, r->stream);
     launch_copy(M, r->q.p, r->b2.p, r->stream);
     if (r->geo.viscous) launch_stress3d(r->geo, r->q.p, r->G.p, r->stream);
   } else if (which == MGD_KERNEL_RHS_FORWARD || which == MGD_KERNEL_RHS_ADJOINT) {
@@ -2918,7 +2942,7 @@ int mgd_time_kernel(mgd_region* r, int which, int reps, double* ms_per_launch, d
       // the march kernel's own operands per stage, averaged over the 4 stages: the stage
       // state and the 9 stress / heat-flux slots in, RK buffers in (0, 2, 2, 1 states),
       // RK buffers and the next stage state out (3, 2, 2, 1 states)
-      case MGD_KERNEL_FWD_MARCH3D: *bytes = (w + 72.0 + 5.0 / 4.0 * w + 8.0 / 4.0 * w) * pts; break;
+      case MGD_KERNEL_FWD_MARCH3D: *bytes = (w + 72.0 + 5.0 / 4.0 * w + 7.0 / 4.0 * w) * pts; break;
       default: *bytes = 2.0 * w * pts; break;
     }
   }
