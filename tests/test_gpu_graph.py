@@ -53,3 +53,40 @@ def test_adjoint_graph_replays(plain):
         assert np.array_equal(J_g, J_e) and np.array_equal(s_g, s_e), rep
         assert np.array_equal(g_g, g_e), rep
         assert np.array_equal(R.states[1].adjointVariables, lam_e), rep
+
+
+@pytest.mark.parametrize("plan", ["step store, every step recomputed", "step store, 3 steps held",
+                                  "windows", "fused 2-D windows"])
+def test_adjoint_graph_replays_with_recompute(plan):
+    """Replays of runs that recompute stage states, with the control gradient: the step
+    store and snapshots with windows on the generic kernels (a 1-D periodic wave with
+    cost target and actuator), and the fused 2-D window recompute (each recompute stage in
+    one launch with an adjoint stage) -- bit for bit against eager."""
+    from magudi_amd.solver import Region, Solver
+    print(f"graph replay: {plan}", flush=True)
+    if plan.startswith("fused"):
+        c = cases.acoustic_monopole(n=41, nsteps=8, save_interval=3)
+        c.dt = 0.2
+        R = Region(c, resident_states=False)
+    else:
+        c = cases.periodic_wave_1d(n=256, nsteps=8, save_interval=3)
+        R = Region(c)
+        R.set_step_store(0 if plan == "windows" else 1)
+        R.set_window_cache(1 if plan.endswith("held") else 0)
+    R.states[0].initialCondition = c.initial_state()
+    S = Solver(R, dt=c.dt, nsteps=c.nsteps, save_interval=c.save_interval)
+    ctl = 1e-2 * np.random.default_rng(6).standard_normal((4 * c.nsteps, R.n_actuator))
+    R.restart()
+    s_e, g_e, J_e = S.runAdjoint(control=ctl)
+    lam_e = R.states[0].adjointVariables
+    assert np.abs(lam_e).max() > 0 and abs(s_e[0]) > 0
+    plan_e = R.checkpoint_plan
+    for rep in range(3):
+        R.restart()
+        R.computeRhs(1, time=0.1)
+        R.restart()
+        s_g, g_g, J_g = S.runAdjoint(control=ctl, use_graph=True)
+        assert R.checkpoint_plan == plan_e
+        assert np.array_equal(J_g, J_e) and np.array_equal(s_g, s_e), rep
+        assert np.array_equal(g_g, g_e), rep
+        assert np.array_equal(R.states[0].adjointVariables, lam_e), rep
