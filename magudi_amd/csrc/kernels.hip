@@ -63,8 +63,9 @@ __global__ void __launch_bounds__(kBlock) k_grad(Geo g, int d, const double* __r
 // column tile marches along z over a chunk of ZC planes.  Each thread keeps w of its
 // column for the 9 planes of the z stencil in registers; the plane being
 // differentiated is staged with a 4-point x / y halo in LDS, one halo point per thread
-// (threads below 8 (BX + BY)).  The halo point of the next plane is loaded one
-// iteration before it is used, so its latency overlaps a plane's stencil work.
+// (threads below 8 (BX + BY)).  The halo point of the next plane (and, where the
+// registers allow, the column's next lead point) is loaded one iteration before it is
+// used, so its latency overlaps a plane's stencil work.
 // Same sums as k_prim + k_grad (interior SBP rows, wrap-around indices), so the same
 // values.
 // STRESS: instead of the 12 gradient components, write what the fluxes need of them at
@@ -131,17 +132,25 @@ __global__ void __launch_bounds__(BX * BY, STRESS ? 2 : 4) k_grad3d_periodic(Geo
   }
   double hv[5];
   if (has_halo) load_v(hi, hj, z0 % dz.len, hv);   // halo, plane z0
+  // with constant transport coefficients the registers allow the column's lead point to
+  // be loaded a plane ahead too
+  constexpr bool kAhead = STRESS && !POWER;
+  if (kAhead) load_v(i, j, wrap(z0 + 4, dz), v);   // the column's lead plane z0 + 4
   const Op& op = g.ops[SLOT_FIRST];
   const double c1 = op.c[kMaxHW + 1], c2 = op.c[kMaxHW + 2], c3 = op.c[kMaxHW + 3], c4 = op.c[kMaxHW + 4];
   for (int s = 0; s < ZC; ++s) {
     const int zc = z0 + s;
     if (zc >= g.n[2]) break;   // uniform over the block
-    load_v(i, j, wrap(zc + 4, dz), v);
+    if (!kAhead) load_v(i, j, wrap(zc + 4, dz), v);
     to_w(v, Q[8]);
     double hw[NW];
     if (has_halo) to_w(hv, hw);
-    // next plane's halo point (the loads stay in flight through this plane's work)
-    if (has_halo && s + 1 < ZC && zc + 1 < g.n[2]) load_v(hi, hj, zc + 1, hv);
+    // the next plane's halo point (and lead point): the loads stay in flight through this
+    // plane's work
+    if (s + 1 < ZC && zc + 1 < g.n[2]) {
+      if (kAhead) load_v(i, j, wrap(zc + 5, dz), v);
+      if (has_halo) load_v(hi, hj, zc + 1, hv);
+    }
     __syncthreads();            // readers of the previous plane are done
 #pragma unroll
     for (int c = 0; c < NW; ++c) sw[c][ty + 4][tx + 4] = Q[4][c];
@@ -1720,6 +1729,24 @@ __global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_fwd3d_march(Geo g,
 #pragma unroll
     for (int l = 0; l < NV; ++l) W[8][l] = f[2][l];
     if (xy && hmask) fluxes(hi, hj, kz, hmask, ch, fh);
+    // the operands of the output plane zl - 4 that the finish reads (RK buffers, the
+    // Kolmogorov term's density and strength) are loaded with the next plane's, once this
+    // plane's fluxes are formed, so their latency overlaps the divergence
+    const bool outp = zl >= z0 + 4;
+    double ra[NV], rb[NV], rho_k = 0.0, kk_k = 0.0;
+    if (outp) {
+      const long p = i + (long)g.n[0] * j + n01 * (zl - 4);
+      if (kstr) {
+        rho_k = qi[p];
+        kk_k = kstr[p];
+      }
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        const long ii = inst * NV * g.N + l * g.N + p;
+        ra[l] = rk_stage == 1 ? q[ii] : (rk_stage == 2 || rk_stage == 3) ? b1[ii] : 0.0;
+        rb[l] = rk_stage >= 2 ? b2[ii] : 0.0;
+      }
+    }
     // this plane's operands are spent: load the next plane's now
     const int zn = zl + 1;
     const bool nxy = zn >= z0 && zn < z1;
@@ -1745,7 +1772,7 @@ __global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_fwd3d_march(Geo g,
       const double jac = jacobian<3>(g, ijk);
 #pragma unroll
       for (int l = 0; l < NV; ++l) r[l] = r[l] * jac;
-      if (kstr) r[1] = r[1] + qi[p] * kstr[p];
+      if (kstr) r[1] = r[1] + rho_k * kk_k;
       if (rk_stage == 0) {
 #pragma unroll
         for (int l = 0; l < NV; ++l) rhs[inst * NV * g.N + l * g.N + p] = r[l];
@@ -1757,22 +1784,22 @@ __global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_fwd3d_march(Geo g,
           double qn;
           switch (rk_stage) {
             case 1: {
-              const double q0 = q[ii];
+              const double q0 = ra[l];
               if (b1) b1[ii] = q0;
               b2[ii] = q0 + div6(dt * rl);
               qn = q0 + dt * rl / 2.0;
               break;
             }
             case 2:
-              b2[ii] = b2[ii] + div3(dt * rl);
-              qn = b1[ii] + dt * rl / 2.0;
+              b2[ii] = rb[l] + div3(dt * rl);
+              qn = ra[l] + dt * rl / 2.0;
               break;
             case 3:
-              b2[ii] = b2[ii] + div3(dt * rl);
-              qn = b1[ii] + dt * rl;
+              b2[ii] = rb[l] + div3(dt * rl);
+              qn = ra[l] + dt * rl;
               break;
             default:
-              qn = b2[ii] + div6(dt * rl);
+              qn = rb[l] + div6(dt * rl);
               break;
           }
           yout[ii] = qn;
@@ -2168,6 +2195,19 @@ __global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_adj3d_finish_march
       v[l] = ld8(qi + l * g.N, p8);
       r[l] = ld8(ri + l * g.N, p8);
     }
+    // the point's RK operands and Kolmogorov-term values, loaded here so that their
+    // latency overlaps the derivative work (stage 4 reads lambda before it writes it)
+    double ra[NV], rb[NV], lk = 0.0, kk = 0.0;
+    if (kstr) {
+      lk = ld8(li + g.N, p8);
+      kk = ld8(kstr, p8);
+    }
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      const long c = io + l * g.N;
+      ra[l] = rk_stage == 4 ? ld8(y + c, p8) : (rk_stage == 2 || rk_stage == 3) ? ld8(b1 + c, p8) : 0.0;
+      rb[l] = rk_stage >= 1 && rk_stage <= 3 ? ld8(b2 + c, p8) : 0.0;
+    }
     __syncthreads();            // readers of the previous plane are done
 #pragma unroll
     for (int a = 0; a < NW; ++a) {
@@ -2207,7 +2247,7 @@ __global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_adj3d_finish_march
     const double jac = jacobian<3>(g, ijk);
 #pragma unroll
     for (int l = 0; l < NV; ++l) r[l] = r[l] * jac;
-    if (kstr) r[0] = r[0] - ld8(li + g.N, p8) * ld8(kstr, p8);
+    if (kstr) r[0] = r[0] - lk * kk;
     if (rk_stage == 0) {
 #pragma unroll
       for (int l = 0; l < NV; ++l) st8(ri + l * g.N, p8, r[l]);
@@ -2221,22 +2261,22 @@ __global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_adj3d_finish_march
         const double rl = r[l];
         switch (rk_stage) {
           case 4: {
-            const double l0 = ld8(yc, p8);
+            const double l0 = ra[l];
             st8(b1c, p8, l0);
             st8(b2c, p8, l0 - div6(dt * rl));
             st8(yc, p8, l0 - dt * rl / 2.0);
             break;
           }
           case 3:
-            st8(b2c, p8, ld8(b2c, p8) - div3(dt * rl));
-            st8(yc, p8, ld8(b1c, p8) - dt * rl / 2.0);
+            st8(b2c, p8, rb[l] - div3(dt * rl));
+            st8(yc, p8, ra[l] - dt * rl / 2.0);
             break;
           case 2:
-            st8(b2c, p8, ld8(b2c, p8) - div3(dt * rl));
-            st8(yc, p8, ld8(b1c, p8) - dt * rl);
+            st8(b2c, p8, rb[l] - div3(dt * rl));
+            st8(yc, p8, ra[l] - dt * rl);
             break;
           default:
-            st8(yc, p8, ld8(b2c, p8) - div6(dt * rl));
+            st8(yc, p8, rb[l] - div6(dt * rl));
             break;
         }
       }
