@@ -1640,7 +1640,9 @@ __device__ __forceinline__ void march_halo(const Geo& g, int t, int x0, int y0, 
 // null (the caller keeps q itself as the state at the start of the step).  The fluxes, sums and
 // finishing operations are those of k_fluxdiv3d_xy and k_fwd_finish_z, in the same
 // order, so the values are the same; no flux or partial rhs goes through HBM.
-template <int BX, int BY, int ZC>
+// RK: the variant for RK stages (rk_stage > 0), which loads the RK operands of its output
+// plane early; the rhs-only variant leaves them out and keeps the registers for the rest.
+template <int BX, int BY, int ZC, bool RK>
 __global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_fwd3d_march(Geo g, const double* __restrict__ q,
                                                           const double* __restrict__ S,
                                                           const double* __restrict__ kstr,
@@ -1740,11 +1742,13 @@ __global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_fwd3d_march(Geo g,
         rho_k = qi[p];
         kk_k = kstr[p];
       }
+      if constexpr (RK) {
 #pragma unroll
-      for (int l = 0; l < NV; ++l) {
-        const long ii = inst * NV * g.N + l * g.N + p;
-        ra[l] = rk_stage == 1 ? q[ii] : (rk_stage == 2 || rk_stage == 3) ? b1[ii] : 0.0;
-        rb[l] = rk_stage >= 2 ? b2[ii] : 0.0;
+        for (int l = 0; l < NV; ++l) {
+          const long ii = inst * NV * g.N + l * g.N + p;
+          ra[l] = rk_stage == 1 ? q[ii] : (rk_stage == 2 || rk_stage == 3) ? b1[ii] : 0.0;
+          rb[l] = rk_stage >= 2 ? b2[ii] : 0.0;
+        }
       }
     }
     // this plane's operands are spent: load the next plane's now
@@ -1773,7 +1777,7 @@ __global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_fwd3d_march(Geo g,
 #pragma unroll
       for (int l = 0; l < NV; ++l) r[l] = r[l] * jac;
       if (kstr) r[1] = r[1] + rho_k * kk_k;
-      if (rk_stage == 0) {
+      if (!RK) {
 #pragma unroll
         for (int l = 0; l < NV; ++l) rhs[inst * NV * g.N + l * g.N + p] = r[l];
       } else {
@@ -2497,9 +2501,12 @@ void launch_fwd3d_march(const Geo& g, const double* q, const double* S, const do
   if (rk && rk->ev_begin) (void)hipEventRecord(rk->ev_begin, st);
   with_march(g, false, [&](auto X, auto Z) {
     constexpr int BX = decltype(X)::value;
-    k_fwd3d_march<BX, kFMY, decltype(Z)::value><<<gm, BX * kFMY, 0, st>>>(
-        g, q, S, kstr, rhs, rk ? rk->stage : 0, rk ? rk->dt : 0.0, rk ? rk->yout : nullptr,
-        rk ? rk->b1 : nullptr, rk ? rk->b2 : nullptr, rk ? rk->copy : nullptr);
+    if (rk)
+      k_fwd3d_march<BX, kFMY, decltype(Z)::value, true><<<gm, BX * kFMY, 0, st>>>(
+          g, q, S, kstr, rhs, rk->stage, rk->dt, rk->yout, rk->b1, rk->b2, rk->copy);
+    else
+      k_fwd3d_march<BX, kFMY, decltype(Z)::value, false><<<gm, BX * kFMY, 0, st>>>(
+          g, q, S, kstr, rhs, 0, 0.0, nullptr, nullptr, nullptr, nullptr);
   });
   if (rk && rk->ev_end) (void)hipEventRecord(rk->ev_end, st);
 }
