@@ -582,6 +582,50 @@ void forward_stage(mgd_region* r, DevBuf& qb, DevBuf& alt, double* b1, double* b
   }
 }
 
+// Whether the generic-kernel path's forward stages run on the 3-D z march (out of place:
+// the stage writes its output wherever the caller says, and reads the state at the start
+// of the step from where stage 1 read it).
+bool march_stages(const mgd_region* r) {
+  return r->nd == 3 && r->T1.p && fwd3d_march_path(r->geo, r->fin, (int)r->sources.size());
+}
+
+// One forward RK4 stage on the 3-D z march from the state at `in` to `out` (march_stages):
+// `start` is the state at the start of the step (the input of stage 1, read by stages 2
+// and 3), b2 the RK accumulator.  out must differ from in and start (other blocks still
+// read in).  The step store passes its own slots, so no stage output is written twice.
+void march_stage(mgd_region* r, const double* in, double* out, const double* start, int sidx,
+                 int stage, double dt, double* Gbuf = nullptr) {
+  check_stage_input(r, in);
+  const int na = n_act(r);
+  const double* ctrl = nullptr;
+  long cstride = 0;
+  if (r->control.p && na > 0) {
+    ctrl = r->control.p + (long)sidx * na;
+    cstride = r->ctrl_stride;
+  }
+  const int ns = (int)r->sources.size();
+  RkStage rk{stage, dt, const_cast<double*>(in),
+             stage == 2 || stage == 3 ? const_cast<double*>(start) : nullptr, r->b2.p, nullptr, out};
+  if (r->timing) {
+    const size_t k = 2 * (size_t)r->tcount;
+    while (r->tev.size() < k + 2) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) { r->timing = false; break; }
+      r->tev.push_back(e);
+    }
+    if (r->timing) {
+      rk.ev_begin = r->tev[k];
+      rk.ev_end = r->tev[k + 1];
+    }
+  }
+  const int done = launch_rhs_forward(r->geo, dpatches(r), in, r->rhs.p, r->w.p,
+                                      Gbuf ? Gbuf : r->G.p, r->t.p, r->T1.p, ctrl, cstride,
+                                      reinterpret_cast<const Source*>(r->d_sources.p), ns,
+                                      r->src_amp.p + (long)sidx * std::max(ns, 1),
+                                      r->src_inst_stride, r->fin, r->stream, &rk);
+  if (done == RK_TO_YOUT && rk.ev_begin) ++r->tcount;
+}
+
 bool has_walls(const mgd_region* r) {
   for (const auto& P : r->patches)
     if (P->pd.type == MGD_PATCH_IMPENETRABLE_WALL || P->pd.type == MGD_PATCH_ISOTHERMAL_WALL) return true;
@@ -1149,6 +1193,16 @@ int plan_plain_storage(mgd_region* r, int nsteps, int K, long M) {
 // places.
 void recompute_step(mgd_region* r, int t, double dt) {
   const long M = (long)r->ninst * M_of(r);
+  if (march_stages(r)) {
+    // from the stored state straight into the three slots of `window`
+    const double* start = r->sstore.p + (long)t * M;
+    for (int st = 1; st <= 3; ++st)
+      march_stage(r, st == 1 ? start : r->window.p + (long)(st - 2) * M,
+                  r->window.p + (long)(st - 1) * M, start, 4 * t + st - 1, st, dt,
+                  r->sreuse ? r->sbuf.p + (long)(st - 1) * (long)r->G.n : nullptr);
+    r->rec_step = t;
+    return;
+  }
   launch_copy(M, r->sstore.p + (long)t * M, r->qs.p, r->stream);
   double* const p0 = r->qs.p;
   for (int st = 1; st <= 3; ++st)
@@ -1175,6 +1229,31 @@ const double* step_stage_state(mgd_region* r, int t, int stage, double dt, bool 
 int plain_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
   const int nc = n_cost(r);
   const long M = (long)r->ninst * M_of(r);
+  if (r->step_mode && march_stages(r)) {
+    // the 3-D marches write each stage state straight where it is kept: the state at the
+    // start of step s + 1 into the store (the final state into q), the inner stage states
+    // of a held step into the step cache, the others into the two scratch state buffers
+    r->rec_step = -1;
+    launch_copy(M, r->q.p, r->sstore.p, r->stream);
+    for (int s = 0; s < nsteps; ++s) {
+      const double* start = r->sstore.p + (long)s * M;
+      const bool held = s >= r->scache_first;
+      double* y[5];
+      y[1] = const_cast<double*>(start);
+      for (int k = 2; k <= 4; ++k)
+        y[k] = held ? r->scache.p + ((long)(s - r->scache_first) * 3 + k - 2) * M
+                    : (k == 3 ? r->q_alt2.p : r->q_alt.p);
+      double* const y5 = s + 1 < nsteps ? r->sstore.p + (long)(s + 1) * M : r->q.p;
+      for (int i = 1; i <= 4; ++i) {
+        double* out = i < 4 ? y[i + 1] : y5;
+        march_stage(r, y[i], out, start, 4 * s + i - 1, i, dt);
+        if (nc > 0)
+          launch_patch_reduce(r->geo, dpatch(r, r->ip_cost), nc, 0, out, r->partial.p,
+                              kRkNorm[i - 1] * dt, r->acc_J.p, nullptr, r->stream);
+      }
+    }
+    return MGD_OK;
+  }
   if (r->step_mode) {
     // the state at the start of every step (stage 4 of the step before writes it) and the
     // inner stage states of the last nscache steps

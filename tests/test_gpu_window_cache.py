@@ -4,8 +4,9 @@ forward pass keeps the stage states of its last windows and the adjoint pass rea
 instead of recomputing those windows.  Step mode (MGD_OPT_STEP_STORE 1, and the default
 where it fits): the forward pass keeps the state at the start of every step and the inner
 stage states of its last steps; the adjoint pass recomputes three stages per step not held.
-Every storage plan must give the same J, sensitivity, gradient and adjoint state bit for
-bit -- on a 3-D box (z-march kernels, a ragged last window) and on a 2-D monopole forced
+On the 3-D marches the stages write their outputs straight into the store, the step
+cache and the recompute buffer.  Every storage plan must give the same J, sensitivity,
+gradient and adjoint state bit for bit -- on a 3-D box (z-march kernels, a ragged last window) and on a 2-D monopole forced
 onto the generic kernels (actuator and cost target).  The per-launch timer of the z-march
 kernel counts the forward and recompute launches and the stage copies."""
 import numpy as np
@@ -59,11 +60,12 @@ def test_checkpoint_storage_3d_bit_for_bit():
     assert out[(0, -1)][4] == out[(0, -1)][2]           # nothing recomputed
     assert out[(0, 0)][4] == 4 * nsteps + 4 * 3 + 4 * 3 + 4 * 1 - 2
     assert out[(0, 1)][4] == 4 * nsteps + 4 * 3 + 4 * 3 - 2   # the last window was kept
-    # step mode -- forward: the state at the start of steps 1 .. 6 and the inner stage
-    # states of the held steps; adjoint: three stages per step not held
+    # step mode -- the marches write the state at the start of each step and the inner
+    # stage states of the held steps straight into the store and the step cache (no
+    # copies); adjoint: three stages per step not held, into the recompute buffer
     for key, held in (((1, 0), 0), ((1, 1), K), ((1, -1), nsteps), ((-1, -1), nsteps)):
         assert plans_seen[key] == {"mode": "step store", "held": held, "first": nsteps - held}
-        assert out[key][3] == nsteps - 1 + 3 * held, key
+        assert out[key][3] == 0 and out[key][5] == 0, key
         assert out[key][4] == 4 * nsteps + 3 * (nsteps - held), key
 
 
