@@ -1637,7 +1637,8 @@ __device__ __forceinline__ void march_halo(const Geo& g, int t, int x0, int y0, 
 // zl's partial takes it over -- then the Jacobian scaling and the
 // forcing; with rk_stage > 0 the RK4 stage update follows and goes to yout, which must
 // not be q (other blocks still read q), and rhs is not written; at stage 1 b1 may be
-// null (the caller keeps q itself as the state at the start of the step).  The fluxes, sums and
+// null (the caller keeps q itself as the state at the start of the step), and at stages
+// 1-3 b2 may be null (a recompute that stops before stage 4 needs no accumulator).  The fluxes, sums and
 // finishing operations are those of k_fluxdiv3d_xy and k_fwd_finish_z, in the same
 // order, so the values are the same; no flux or partial rhs goes through HBM.
 // RK: the variant for RK stages (rk_stage > 0), which loads the RK operands of its output
@@ -1747,7 +1748,7 @@ __global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_fwd3d_march(Geo g,
         for (int l = 0; l < NV; ++l) {
           const long ii = inst * NV * g.N + l * g.N + p;
           ra[l] = rk_stage == 1 ? q[ii] : (rk_stage == 2 || rk_stage == 3) ? b1[ii] : 0.0;
-          rb[l] = rk_stage >= 2 ? b2[ii] : 0.0;
+          rb[l] = rk_stage >= 2 && b2 ? b2[ii] : 0.0;
         }
       }
     }
@@ -1790,16 +1791,16 @@ __global__ void __launch_bounds__(BX * BY, 512 / (BX * BY)) k_fwd3d_march(Geo g,
             case 1: {
               const double q0 = ra[l];
               if (b1) b1[ii] = q0;
-              b2[ii] = q0 + div6(dt * rl);
+              if (b2) b2[ii] = q0 + div6(dt * rl);
               qn = q0 + dt * rl / 2.0;
               break;
             }
             case 2:
-              b2[ii] = rb[l] + div3(dt * rl);
+              if (b2) b2[ii] = rb[l] + div3(dt * rl);
               qn = ra[l] + dt * rl / 2.0;
               break;
             case 3:
-              b2[ii] = rb[l] + div3(dt * rl);
+              if (b2) b2[ii] = rb[l] + div3(dt * rl);
               qn = ra[l] + dt * rl;
               break;
             default:

@@ -592,9 +592,11 @@ bool march_stages(const mgd_region* r) {
 // One forward RK4 stage on the 3-D z march from the state at `in` to `out` (march_stages):
 // `start` is the state at the start of the step (the input of stage 1, read by stages 2
 // and 3), b2 the RK accumulator.  out must differ from in and start (other blocks still
-// read in).  The step store passes its own slots, so no stage output is written twice.
+// read in).  The step store passes its own slots, so no stage output is written twice;
+// accumulate = false leaves out the RK accumulator (stages 1-3 of a recompute that stops
+// before stage 4).
 void march_stage(mgd_region* r, const double* in, double* out, const double* start, int sidx,
-                 int stage, double dt, double* Gbuf = nullptr) {
+                 int stage, double dt, double* Gbuf = nullptr, bool accumulate = true) {
   check_stage_input(r, in);
   const int na = n_act(r);
   const double* ctrl = nullptr;
@@ -605,7 +607,8 @@ void march_stage(mgd_region* r, const double* in, double* out, const double* sta
   }
   const int ns = (int)r->sources.size();
   RkStage rk{stage, dt, const_cast<double*>(in),
-             stage == 2 || stage == 3 ? const_cast<double*>(start) : nullptr, r->b2.p, nullptr, out};
+             stage == 2 || stage == 3 ? const_cast<double*>(start) : nullptr,
+             accumulate || stage == 4 ? r->b2.p : nullptr, nullptr, out};
   if (r->timing) {
     const size_t k = 2 * (size_t)r->tcount;
     while (r->tev.size() < k + 2) {
@@ -1199,7 +1202,7 @@ void recompute_step(mgd_region* r, int t, double dt) {
     for (int st = 1; st <= 3; ++st)
       march_stage(r, st == 1 ? start : r->window.p + (long)(st - 2) * M,
                   r->window.p + (long)(st - 1) * M, start, 4 * t + st - 1, st, dt,
-                  r->sreuse ? r->sbuf.p + (long)(st - 1) * (long)r->G.n : nullptr);
+                  r->sreuse ? r->sbuf.p + (long)(st - 1) * (long)r->G.n : nullptr, false);
     r->rec_step = t;
     return;
   }
