@@ -849,17 +849,19 @@ int tune_march(mgd_region* r) {
   r->zc_tuned = true;
   const long M = (long)r->ninst * M_of(r);
   if (int rc = ensure_adjoint(r)) return rc;
-  if (r->qs.alloc(M) || r->qy.alloc(M)) return MGD_ERR_HIP;
+  if (r->qs.alloc(M) || r->qy.alloc(M) || r->qs_alt2.alloc(M)) return MGD_ERR_HIP;
   const double dt = 1e-3;
   // the state copies, one forward RK4 step, then (after event `mid`, if given) one adjoint rhs
   auto round_trip = [&](hipEvent_t mid = nullptr) {
     launch_copy(M, r->q.p, r->qs.p, r->stream);
-    launch_copy(M, r->q.p, r->b1.p, r->stream);
     launch_copy(M, r->q.p, r->b2.p, r->stream);
     launch_copy(M, r->q.p, r->lam.p, r->stream);
+    // the stages as the runs issue them: qs -> qy -> qs_alt2 -> qy -> qs, stages 2 and 3
+    // reading the state at the start of the step from qs
+    double* const in[4] = {r->qs.p, r->qy.p, r->qs_alt2.p, r->qy.p};
+    double* const out[4] = {r->qy.p, r->qs_alt2.p, r->qy.p, r->qs.p};
     for (int i = 1; i <= 4; ++i) {
-      RkStage rk{i, dt, (i % 2) ? r->qs.p : r->qy.p, r->b1.p, r->b2.p, nullptr,
-                 (i % 2) ? r->qy.p : r->qs.p};
+      RkStage rk{i, dt, in[i - 1], i == 1 ? nullptr : r->qs.p, r->b2.p, nullptr, out[i - 1]};
       if (r->geo.viscous) launch_stress3d(r->geo, rk.y, r->G.p, r->stream);
       launch_fwd3d_march(r->geo, rk.y, r->G.p, r->fin.kolmogorov, r->rhs.p, &rk, r->stream);
     }
