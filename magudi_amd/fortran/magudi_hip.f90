@@ -355,6 +355,23 @@ module magudi_hip
        real(c_double), intent(out) :: forward_order(*)
        integer(c_int) :: rc
      end function mgd_control_read
+     function mgd_plot3d_write_function(path, n, n_scalars, data)                             &
+          bind(C, name="mgd_plot3d_write_function") result(rc)
+       import :: c_char, c_int, c_double
+       character(kind = c_char), intent(in) :: path(*)
+       integer(c_int), intent(in) :: n(3)
+       integer(c_int), value :: n_scalars
+       real(c_double), intent(in) :: data(*)
+       integer(c_int) :: rc
+     end function mgd_plot3d_write_function
+     ! Grid%update's arrays (0 metrics, 1 jacobian, 2 norm, 3 arc lengths)
+     function mgd_region_get_grid(r, which, host) bind(C, name="mgd_region_get_grid") result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: r
+       integer(c_int), value :: which
+       real(c_double), intent(out) :: host(*)
+       integer(c_int) :: rc
+     end function mgd_region_get_grid
   end interface
 
   public :: mgd_run_forward_checkpointed, mgd_run_adjoint_checkpointed, mgd_state_lincomb,      &
@@ -363,7 +380,8 @@ module magudi_hip
        mgd_adjoint_initial_condition, mgd_update_adjoint_forcing, mgd_functional,               &
        mgd_region_set_solution_limits, mgd_check_solution_limits, mgd_time_step_size, mgd_sync
   public :: mgd_plot3d_detect, mgd_plot3d_read_grid, mgd_plot3d_read_solution,                  &
-       mgd_plot3d_read_function, mgd_plot3d_write_solution, mgd_control_write, mgd_control_read
+       mgd_plot3d_read_function, mgd_plot3d_write_solution, mgd_control_write, mgd_control_read, &
+       mgd_plot3d_write_function, mgd_region_get_grid
   public :: mgd_region_set_option, mgd_group_create, mgd_group_destroy, mgd_group_run_adjoint
   public :: mgd_last_error, mgd_region_create, mgd_region_destroy, mgd_region_add_patch,      &
        mgd_region_set_field, mgd_region_add_source, mgd_region_finalize, mgd_state_put,         &

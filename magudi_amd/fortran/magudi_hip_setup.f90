@@ -95,7 +95,8 @@ contains
        nControlStages)
     ! nInstances: independent copies of the state (the multiple-shooting segments);
     ! loadInitialCondition = .false. leaves the conserved state to the caller;
-    ! nControlStages: stages of the control forcing file (default 4 number_of_timesteps)
+    ! nControlStages: stages of the control forcing file (default 4 number_of_timesteps; 0:
+    ! the file is not read)
     type(t_Region), intent(inout) :: region
     type(t_Solver), intent(inout) :: solver
     type(t_RunSetup), intent(inout) :: run
@@ -292,13 +293,14 @@ contains
     if (solver%saveInterval <= 0) solver%saveInterval = solver%nTimesteps
 
     ! actuator forcing of the forward run (controller_switch): the control forcing file
+    ! (not read with nControlStages = 0)
     run%controllerSwitch = run%enableController .and. getOption("controller_switch", .false.)
-    if (run%controllerSwitch .and. run%nActuatorPoints > 0) then
+    nStages = 4 * solver%nTimesteps
+    if (present(nControlStages)) nStages = nControlStages
+    if (run%controllerSwitch .and. run%nActuatorPoints > 0 .and. nStages > 0) then
        filename = trim(getOption("control_forcing_directory", "")) // trim(run%outputPrefix) //  &
             ".control_forcing_" // trim(run%actuatorName) // ".dat"
        filename = getOption("control_forcing_filename", trim(filename))
-       nStages = 4 * solver%nTimesteps
-       if (present(nControlStages)) nStages = nControlStages
        allocate(run%control(run%nActuatorPoints, nStages))
        call check(mgd_control_read(cstr(filename), int(nStages, c_int),                         &
             int(run%nActuatorPoints, c_int), run%control), "mgd_control_read " // trim(filename))

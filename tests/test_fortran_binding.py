@@ -319,10 +319,14 @@ def test_multiple_shooting_drivers_on_an_example_directory(tmp_path, penalty):
      "time_splitting/segment_length = 4\ntime_splitting/start_timestep = 0\n"
      "time_splitting/penalty_weight = 1.0\ntime_splitting/state_mollifier/enabled = true\n",
      "msforward_hip", "state_mollifier/enabled = true is not built"),
+    ('output_prefix = "X"\n', "compute_norm_hip",
+     "must contain 'enable_adjoint_solver' for compute_norm."),
+    ('enable_adjoint_solver = true\n', "compute_norm_hip",
+     "Required parameter 'time_splitting/number_of_segments' not found!"),
 ])
 def test_multiple_shooting_driver_input_errors(tmp_path, inp, driver, message):
     """The drivers stop with the reference's messages (bin/msforward.f90:127,
-    bin/msadjoint.f90:131-144, src/MSPenaltyImpl.f90:138) on missing or invalid time_splitting input, before any
+    bin/msadjoint.f90:131-144, src/MSPenaltyImpl.f90:138, bin/compute_norm.f90:110-113) on missing or invalid time_splitting input, before any
     device work."""
     exe = os.path.join(FDIR, driver)
     if not os.path.exists(exe):
@@ -331,3 +335,54 @@ def test_multiple_shooting_driver_input_errors(tmp_path, inp, driver, message):
     out = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert out.returncode != 0
     assert message in out.stdout + out.stderr
+
+
+@pytest.mark.gpu
+def test_compute_norm_driver_on_an_example_directory(tmp_path):
+    """compute_norm_hip (bin/compute_norm.f90) on a generated monopole directory with 3
+    segments of 8 steps: the actuator norm file holds H at the patch points (patch order)
+    times the RK4 weight and dt of every stage (collectThermalActuatorNorm,
+    src/ThermalActuatorImpl.f90:710-763), norm_ic.q state_controllability H in every
+    variable, the layout the reference's optimiser reads, and --save_metrics the grid's
+    Jacobian and metrics -- against the oracle's grid arrays, and the mismatch of
+    number_of_timesteps with Nts * Nsplit stops with the reference's message."""
+    import oracle_bind as ob
+    from magudi_amd import cases, plot3d
+    exe = os.path.join(FDIR, "compute_norm_hip")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran drivers not built (needs amdflang at build time)")
+    nsplit, nts, w = 3, 8, 2.5
+    c = cases.acoustic_monopole(n=41, nsteps=nsplit * nts, save_interval=4)
+    c.dt = 0.2
+    extra = (f"time_splitting/number_of_segments = {nsplit}\n"
+             f"time_splitting/segment_length = {nts}\n"
+             "time_splitting/start_timestep = 0\n"
+             f"time_splitting/state_controllability = {w!r}\n")
+    _write_case_directory(tmp_path, c, extra=extra)
+    print(f"compute_norm_hip in {tmp_path}", flush=True)
+    out = subprocess.run([exe, "--save_metrics"], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    O = ob.Region(c)
+    H = O.grid(2)[0]
+    pts = c.patch_points(c.patches_of(cases.ACTUATOR)[0])
+    na = len(pts)
+    norm = plot3d.read_control(str(tmp_path / "Monopole.norm_controlRegion.dat"), na)
+    assert norm.shape == (4 * nsplit * nts, na)
+    wts = [1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0]
+    for s in range(norm.shape[0]):
+        np.testing.assert_allclose(norm[s], H[pts] * (wts[s % 4] * c.dt), rtol=1e-14, atol=0.0)
+    q, aux = plot3d.read_solution(str(tmp_path / "Monopole.norm_ic.q"), c.nd + 2)
+    for v in range(c.nd + 2):
+        np.testing.assert_allclose(q[v], w * H, rtol=1e-14, atol=0.0)
+    rows = [ln.split() for ln in (tmp_path / "Monopole.layout.txt").read_text().splitlines()]
+    assert rows[0] == ["#", "kind", "identifier", "size_in_real64"]
+    assert rows[1] == ["actuator", "controlRegion", str(4 * nsplit * nts * na)]
+    assert rows[2:] == [["ic", str(k), str(c.npoints * (c.nd + 2))] for k in range(1, nsplit)]
+    jac = plot3d.read_function(str(tmp_path / "Monopole.Jacobian.f"))
+    np.testing.assert_allclose(jac[0], O.grid(1)[0], rtol=1e-14, atol=0.0)
+    with open(tmp_path / "magudi.inp", "a") as f:
+        f.write(f"number_of_timesteps = {nsplit * nts + 1}\n")
+    out = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode != 0
+    assert "but compute_norm requires Nts * Nsplit" in out.stdout + out.stderr
