@@ -162,8 +162,8 @@ def cpu_baseline(n=64, nsteps=8, save_interval=4):
 
 def viscous3d_leg(n):
     """One forward and one adjoint rhs evaluation of the 3-D viscous periodic Kolmogorov
-    flow at n^3, HIP-event timed (per-operator kernels: one-pass periodic gradient,
-    LDS-staged flux divergence, line-march derivatives)."""
+    flow at n^3, HIP-event timed (the stress pass and the z marches, after the march-shape
+    autotuner)."""
     from magudi_amd import capi, cases
     from magudi_amd.solver import Region
     c = cases.kolmogorov3d(n=n, nsteps=1)
@@ -194,11 +194,12 @@ class KolmogorovRun:
     stage states of the steps not held (its result, the adjoint state at t0, is dJ/dq0 and
     stays on the device)."""
 
-    def __init__(self, n, nsteps):
+    def __init__(self, n, nsteps, step_store=-1):
         from magudi_amd import cases
         from magudi_amd.solver import Region, Solver
         self.case = cases.kolmogorov3d(n=n, nsteps=nsteps)
         self.R = Region(self.case)
+        self.R.set_step_store(step_store)
         self.R.states[0].initialCondition = self.case.initial_state()
         self.S = Solver(self.R, dt=self.case.dt, nsteps=nsteps,
                         save_interval=self.case.save_interval)
@@ -523,6 +524,21 @@ def monopole_legs(a, extras):
     return out
 
 
+def checkpoint_plans_leg(n, nsteps, t_step, J):
+    """The headline pass with snapshots every nsteps / 8 steps and windows (the window cache
+    holding the last windows free memory allows) in place of the step store: same J, the
+    time against the step store's."""
+    K = KolmogorovRun(n, nsteps, step_store=0)
+    K.one()
+    t, Jw, _, launches, _, _ = K.timed(1)
+    plan = K.R.checkpoint_plan
+    K.R.close()
+    del K
+    return {"step_store_s_per_pass": round(t_step, 3), "windows_s_per_pass": round(t, 3),
+            "windows_plan": plan, "windows_march_launches_per_pass": launches,
+            "same_J": bool(Jw == J), "speedup": round(t / t_step, 3)}
+
+
 def kernels_only(a):
     """HIP-event timing of the headline's dominant kernel and of the state copy of the same
     region (the calibration the counter passes scale by)."""
@@ -645,6 +661,7 @@ def main():
         out["roofline"]["traffic_over_algorithmic"] = round(traffic / (232.0 * M_bytes / 40.0), 3)
         out["roofline"]["traffic_workload"] = "launches without the state copy (--kernels-only)"
     if not a.no_extras:
+        out["checkpoint_plans"] = checkpoint_plans_leg(a.c4_n, a.c4_nsteps, t_step, J)
         out["monopole401"] = monopole_legs(a, extras=True)
         out["fd_sweep"] = fd_sweep_leg(a.n, a.nsteps, a.save_interval, a.batch)
         out["uneven_batch"] = uneven_leg()
