@@ -493,8 +493,20 @@ def monopole_legs(a, extras):
     run_hot_path(Rb, Sb, 1)
     tb, _ = run_hot_path(Rb, Sb, 1)
     run_hot_path(Rb, Sb, 1, True)
-    tbg, _ = run_hot_path(Rb, Sb, 1, True)
+    tbg, (sens_b, _, J_b) = run_hot_path(Rb, Sb, 1, True)
+    plan_b = Rb.checkpoint_plan
+    Rb.close()
     del Rb, Sb
+    # the same batch with snapshots and window recompute in place of the step store
+    Rw = Region(case, n_instances=a.batch)
+    Rw.set_step_store(0)
+    for i in range(a.batch):
+        Rw.states[i].initialCondition = q0
+    Sw = Solver(Rw)
+    run_hot_path(Rw, Sw, 1)
+    tbw, (sens_w, _, J_w) = run_hot_path(Rw, Sw, 1)
+    Rw.close()
+    del Rw, Sw
     # generic per-operator path on one instance, fewer steps, same per-step work
     ns_plain = max(a.save_interval, a.nsteps // 10)
     Rp = Region(case, force_plain=True)
@@ -506,7 +518,12 @@ def monopole_legs(a, extras):
     del Rp, Sp
     out["batched"] = {"instances": a.batch, "value": a.batch * pts / tb, "ms_per_run": tb * 1e3,
                       "graph_value": a.batch * pts / tbg, "graph_ms_per_run": tbg * 1e3,
-                      "speedup_vs_single_fused": (a.batch * pts / min(tb, tbg)) / value}
+                      "speedup_vs_single_fused": (a.batch * pts / min(tb, tbg)) / value,
+                      "checkpoint_plan": plan_b,
+                      "windows_value": a.batch * pts / tbw, "windows_ms_per_run": tbw * 1e3,
+                      "step_store_speedup_vs_windows": tbw / tb,
+                      "same_J_and_Grad0_as_windows": bool(np.array_equal(J_b, J_w) and
+                                                          np.array_equal(sens_b, sens_w))}
     out["plain"] = {"value": case.npoints * ns_plain / tp, "nsteps": ns_plain,
                     "ms_per_run": tp * 1e3,
                     "batched_speedup_vs_plain": (a.batch * pts / tb) / (case.npoints * ns_plain / tp)}
@@ -514,6 +531,7 @@ def monopole_legs(a, extras):
     # a launch with an adjoint stage) instead of every RK stage state resident in HBM
     # (19.8 GB at 401^2 x 960 steps): same results
     Rr = Region(case, tile_shape=R.tile_shape, resident_states=False)
+    Rr.set_step_store(0)
     Rr.states[0].initialCondition = q0
     Sr = Solver(Rr)
     run_hot_path(Rr, Sr, 1)

@@ -307,12 +307,12 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches);
  * depends on the tile shape.
  * MGD_OPT_RESIDENT_STATES (fused 2-D path): 1 keeps every RK stage state of a
  * checkpointed forward run in device memory ((4 nsteps + 1) states, e.g. 19.8 GB for
- * 401^2 x 960 steps) so that the adjoint pass reads them instead of recomputing each
- * window from its snapshot; 0: snapshots every save_interval steps and window recompute
- * (UniformCheckpointer, src/UniformCheckpointerImpl.f90:78-208); -1 (default): resident
- * where the states fit in the device memory left free at the run's preparation less a
- * reserve of max(8 GiB, 1/32 of the device), except for the members of a group
- * launch.  Identical results.
+ * 401^2 x 960 steps) so that the adjoint pass recomputes nothing; 0: the step store
+ * (MGD_OPT_STEP_STORE) or snapshots every save_interval steps and window recompute
+ * (UniformCheckpointer, src/UniformCheckpointerImpl.f90:78-208), each recompute stage
+ * in one launch with an adjoint stage; -1 (default): resident where the states fit in the
+ * device memory left free at the run's preparation less a reserve of max(8 GiB, 1/32 of
+ * the device), except for the members of a group launch (windows).  Identical results.
  * MGD_OPT_WINDOW_CACHE (generic-kernel path, e.g. 3-D regions): the checkpointed forward
  * pass also keeps the stage states of its last n windows (4 save_interval states each) and
  * the adjoint passes read them instead of recomputing those windows; identical results.
@@ -325,13 +325,14 @@ int mgd_last_timings(mgd_region* r, double* out, int* launches);
  * their tiles, 16 (16 x 16 points, 256 threads) or 32 (32 x 16, 512 threads; x sizes that
  * are multiples of 32), 0 (default) autotuned with the z chunk; the forward march and the
  * adjoint marches are tuned apart.  No result depends on either.
- * MGD_OPT_STEP_STORE (generic-kernel path): the checkpointed forward pass keeps the state
+ * MGD_OPT_STEP_STORE (generic-kernel path; fused 2-D path where the stage states are not
+ * resident, except for the members of a group launch): the checkpointed forward pass keeps the state
  * at the start of every step (nsteps states) and, in the memory left (MGD_OPT_WINDOW_CACHE
  * n >= 0: at most n save_interval steps), the three inner stage states of its last steps;
  * the adjoint pass recomputes the inner stage states of the other steps from the stored
  * state, three RK stages per step instead of 4 save_interval - 1 per window from a
- * snapshot, with no window buffer.  -1 (default): where nsteps + 3 states fit in the
- * memory left free less the reserve and MGD_OPT_WINDOW_CACHE is at its default; 0: off
+ * snapshot, with no window buffer.  -1 (default): where nsteps + 3 states (fused path:
+ * nsteps + 6, two recompute buffers) fit in the memory left free less the reserve and MGD_OPT_WINDOW_CACHE is at its default; 0: off
  * (snapshots and windows); 1: on where the states of every step can be allocated.
  * Identical results. */
 #define MGD_OPT_FORCE_PLAIN 1
@@ -345,8 +346,9 @@ int mgd_region_set_option(mgd_region* r, int option, int value);
 
 /* Storage plan of the last checkpointed forward run: plan[0] = 1 step store, plan[1] steps
  * whose inner stage states are held, from step plan[2] on; 0 snapshots and windows,
- * plan[1] windows held, from step plan[2] on; 2 the fused 2-D path (plan[1] stage states
- * held with MGD_OPT_RESIDENT_STATES, else 0: its own window recompute).  No counterpart
+ * plan[1] windows held, from step plan[2] on (the generic kernels); 2 the fused 2-D path
+ * with every stage state resident (plan[1] = 4 nsteps + 1) or with snapshots and its own
+ * window recompute (plan[1] = 0).  No counterpart
  * in the reference (whose UniformCheckpointer writes .q files every save_interval
  * steps). */
 int mgd_region_get_checkpoint_plan(mgd_region* r, int* plan);

@@ -953,6 +953,24 @@ int fused_forward_pass(mgd_region* r, double dt, int nsteps, int K) {
     if (nc > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, r->stream);
     return MGD_OK;
   }
+  if (r->step_mode) {
+    // step store: each stage writes its output where it is kept -- the state at the start
+    // of step s + 1 into the store (the final state into q), the inner stage states of a
+    // held step into the step cache, the others into two slots of `window`
+    r->rec_step = -1;
+    launch_copy(M, r->q.p, r->sstore.p, r->stream);
+    for (int s = 0; s < nsteps; ++s) {
+      const bool held = s >= r->scache_first;
+      double* outs[4];
+      for (int k = 0; k < 3; ++k)
+        outs[k] = held ? r->scache.p + ((long)(s - r->scache_first) * 3 + k) * M
+                       : r->window.p + (long)(k % 2) * M;
+      outs[3] = s + 1 < nsteps ? r->sstore.p + (long)(s + 1) * M : r->q.p;
+      fused_forward_step(r, r->sstore.p + (long)s * M, nullptr, nullptr, r->b2.p, s, dt, nst, true, outs);
+    }
+    if (nc > 0) launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, r->stream);
+    return MGD_OK;
+  }
   const int t0last = ((nsteps + K - 1) / K - 1) * K;
   double* W = r->window.p;
   launch_copy(M, r->q.p, r->snaps.p, r->stream);
@@ -1005,8 +1023,8 @@ int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
     launch_adj_forcing(r->geo, dpatch(r, r->ip_cost), nc, r->qT.p, r->aforce.p, 0, r->stream);
     launch_adj_terminal(r->geo, dpatch(r, r->ip_cost), nc, r->aforce.p, -dt / 6.0, L0, r->stream);
   }
-  auto adj_args = [&](const double* W, int t0w, int t, int i) {
-    const double* qcur = W + (4L * (t - t0w) + (i - 1)) * M;
+  // adjoint stage i of step t about the forward stage state qcur
+  auto adj_args_q = [&](const double* qcur, int t, int i) {
     const int sidx = 4 * t + i - 1;
     AdjStage a;
     std::memset(&a, 0, sizeof a);
@@ -1032,7 +1050,50 @@ int fused_adjoint_pass(mgd_region* r, double dt, int nsteps, int K, bool termina
     }
     return a;
   };
-  if (r->resident) {
+  auto adj_args = [&](const double* W, int t0w, int t, int i) {
+    return adj_args_q(W + (4L * (t - t0w) + (i - 1)) * M, t, i);
+  };
+  if (r->step_mode) {
+    // step store: the stage states of step t are its stored state and its inner states
+    // from the step cache or, for a step not held, recomputed into rbuf[t % 2]; the three
+    // recompute stages of step t - 1 share their launches with the adjoint stages 4, 3, 2
+    // of step t (or follow them), into the other buffer
+    double* const rbuf[2] = {r->window.p, r->window2.p};
+    auto held = [&](int t) { return t >= r->scache_first; };
+    auto rec_args = [&](int t, int i) {
+      double* R = rbuf[t % 2];
+      const double* start = r->sstore.p + (long)t * M;
+      const double* in = i == 1 ? start : R + (long)(i - 2) * M;
+      return fwd_args(r, in, start, r->b2r.p, R + (long)(i - 1) * M, i, dt, 4 * t + i - 1, nst, false);
+    };
+    if (!held(nsteps - 1))
+      for (int i = 1; i <= 3; ++i)
+        launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, rec_args(nsteps - 1, i), r->stream);
+    for (int t = nsteps - 1; t >= 0; --t) {
+      const double* qst[4];
+      qst[0] = r->sstore.p + (long)t * M;
+      for (int k = 1; k < 4; ++k)
+        qst[k] = held(t) ? r->scache.p + ((long)(t - r->scache_first) * 3 + k - 1) * M
+                         : rbuf[t % 2] + (long)(k - 1) * M;
+      const int nrec = t >= 1 && !held(t - 1) ? 3 : 0;
+      int j = 0;
+      for (int i = 4; i >= 1; --i) {
+        const AdjStage a = adj_args_q(qst[i - 1], t, i);
+        if (j < nrec && r->dual) {
+          ++j;
+          launch_dual_stage2d(r->geo, dpatches(r), r->tiling, a, rec_args(t - 1, j), r->stream);
+        } else {
+          launch_adj_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+          if (j < nrec) {
+            ++j;
+            launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, rec_args(t - 1, j), r->stream);
+          }
+        }
+        reduce_chunk(r, r->spart, na, nst, 4 * t + i - 1, true);
+      }
+    }
+    r->rec_step = -1;   // `window` holds the inner states of another step than before
+  } else if (r->resident) {
     for (int t = nsteps - 1; t >= 0; --t)
       for (int i = 4; i >= 1; --i) {
         launch_adj_stage2d(r->geo, dpatches(r), r->tiling, adj_args(r->wall.p, 0, t, i), r->stream);
@@ -1087,8 +1148,8 @@ double* cached_window(mgd_region* r, int lt) {
 size_t storage_budget(mgd_region* r) {
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
-  size_t held = r->window.bytes() + r->snaps.bytes() + r->sstore.bytes() + r->scache.bytes() +
-                r->sbuf.bytes() + r->wall.bytes();
+  size_t held = r->window.bytes() + r->window2.bytes() + r->snaps.bytes() + r->sstore.bytes() +
+                r->scache.bytes() + r->sbuf.bytes() + r->wall.bytes();
   for (int k = 0; k < mgd_region::kMaxCache; ++k) held += r->wcache[k].bytes();
   const size_t reserve = std::max((size_t)8 << 30, tot / 32);
   return fr + held > reserve ? fr + held - reserve : 0;
@@ -1121,38 +1182,47 @@ int plan_window_cache(mgd_region* r, int nsteps, int K, long M) {
   return MGD_OK;
 }
 
-// Storage of a checkpointed run on the generic-kernel path.  Step mode (MGD_OPT_STEP_STORE
-// 1, or -1 with MGD_OPT_WINDOW_CACHE at its default, where the states of every step and
-// three scratch states fit in storage_budget): the state at the start of every step, and
-// the inner stage states of as many of the last steps as the rest of the budget holds
+// Storage of a checkpointed run on the generic-kernel path, and on the fused 2-D path where
+// the stage states are not resident.  Step mode (MGD_OPT_STEP_STORE 1, or -1 with
+// MGD_OPT_WINDOW_CACHE at its default, where the states of every step and the recompute
+// buffers fit in storage_budget): the state at the start of every step, and the inner
+// stage states of as many of the last steps as the rest of the budget holds
 // (MGD_OPT_WINDOW_CACHE n >= 0: at most n save_interval steps); the adjoint pass recomputes
 // three stages per step not held, against 4 save_interval - 1 per window in window mode.
-// Window mode: snapshots every K steps, a window buffer and whole cached windows.
+// The recompute buffer holds three states; the fused path has two (the recompute of one
+// step shares its launches with the adjoint stages of the next).  Window mode (always for
+// the fused members of a group launch): snapshots every K steps, a window buffer and, on
+// the generic kernels, whole cached windows.
 int plan_plain_storage(mgd_region* r, int nsteps, int K, long M) {
   const size_t S = (size_t)M * sizeof(double);
+  const bool fused = fused_ok(r);
+  const size_t nrec = fused ? 6 : 3;   // states of the recompute buffers
   r->rec_step = -1;
   r->step_mode = false;
-  bool step = r->opt_step == 1;
+  const bool allowed = !(fused && r->group_run);
+  bool step = r->opt_step == 1 && allowed;
   size_t budget = 0;
-  if (r->opt_step != 0) {
+  if (r->opt_step != 0 && allowed) {
     budget = storage_budget(r);
-    if (r->opt_step < 0) step = r->opt_cache < 0 && (size_t)(nsteps + 3) * S <= budget;
+    if (r->opt_step < 0) step = r->opt_cache < 0 && (size_t)(nsteps + nrec) * S <= budget;
   }
-  size_t fixed = (size_t)(nsteps + 3) * S;
+  size_t fixed = (size_t)(nsteps + nrec) * S;
   if (step) {
     for (int k = 0; k < mgd_region::kMaxCache; ++k) r->wcache[k].release();
     r->ncache = 0;
     r->snaps.release();
     if (r->window.n != (size_t)3 * M) r->window.release();
+    if (fused && r->window2.n != (size_t)3 * M) r->window2.release();
     if (r->sstore.n != (size_t)nsteps * M) r->sstore.release();
-    if (r->window.alloc((size_t)3 * M) || r->sstore.alloc((size_t)nsteps * M)) {
+    if (r->window.alloc((size_t)3 * M) || (fused && r->window2.alloc((size_t)3 * M)) ||
+        r->sstore.alloc((size_t)nsteps * M)) {
       // taken meanwhile (regions of a group plan one after another): window mode
       (void)hipGetLastError();
       step = false;
     }
   }
   r->sreuse = false;
-  if (step && stress_reuse_path(r->geo, r->fin, (int)r->sources.size()) && r->G.n > 0 &&
+  if (step && !fused && stress_reuse_path(r->geo, r->fin, (int)r->sources.size()) && r->G.n > 0 &&
       fixed + 3 * r->G.bytes() <= budget) {
     if (r->sbuf.alloc(3 * r->G.n) == MGD_OK) {
       r->sreuse = true;
@@ -1198,6 +1268,19 @@ int plan_plain_storage(mgd_region* r, int nsteps, int K, long M) {
 // places.
 void recompute_step(mgd_region* r, int t, double dt) {
   const long M = (long)r->ninst * M_of(r);
+  if (fused_ok(r)) {
+    // fused forward stages 1-3 from the stored state straight into the three slots of
+    // `window` (the accumulator b2r is scratch: no stage 4 follows)
+    const double* start = r->sstore.p + (long)t * M;
+    for (int st = 1; st <= 3; ++st) {
+      const FwdStage a = fwd_args(r, st == 1 ? start : r->window.p + (long)(st - 2) * M, start,
+                                  r->b2r.p, r->window.p + (long)(st - 1) * M, st, dt,
+                                  4 * t + st - 1, 4 * r->ck.nsteps, false);
+      launch_fwd_stage2d(r->geo, dpatches(r), r->tiling, a, r->stream);
+    }
+    r->rec_step = t;
+    return;
+  }
   if (march_stages(r)) {
     // from the stored state straight into the three slots of `window`
     const double* start = r->sstore.p + (long)t * M;
@@ -1429,15 +1512,19 @@ int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, c
   } else {
     const int nsnap = nsteps / K + 1;
     r->step_mode = false;
-    r->sstore.release();
-    r->scache.release();
-    r->sbuf.release();
     // every stage state resident (MGD_OPT_RESIDENT_STATES): asked for, or by default where
-    // they fit in storage_budget (not for members of a group launch, which recompute)
+    // they fit in storage_budget (not for members of a group launch, which recompute);
+    // otherwise the step store or snapshots and windows, as on the generic kernels
     const size_t need = (size_t)(4L * nsteps + 1) * M;
     bool res = r->opt_resident == 1;
     if (r->opt_resident < 0 && !r->group_run) res = need * sizeof(double) <= storage_budget(r);
-    if (!res) r->wall.release();
+    if (res) {
+      r->sstore.release();
+      r->scache.release();
+      r->sbuf.release();
+    } else {
+      r->wall.release();
+    }
     if (res && r->wall.alloc(need)) {
       if (r->opt_resident == 1) {
         set_error("not enough device memory to keep every stage state resident");
@@ -1447,8 +1534,12 @@ int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, c
       res = false;
     }
     r->resident = res;
-    if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M)) return MGD_ERR_HIP;
-    if (int rc = plan_window_cache(r, nsteps, K, M)) return rc;
+    if (res) {
+      if (r->snaps.alloc((size_t)nsnap * M) || r->window.alloc((size_t)4 * K * M)) return MGD_ERR_HIP;
+      if (int rc = plan_window_cache(r, nsteps, K, M)) return rc;
+    } else {
+      if (int rc = plan_plain_storage(r, nsteps, K, M)) return rc;
+    }
   }
   if (fused_ok(r)) {
     if (int rc = prepare_quadrature(r, nst, dt)) return rc;
@@ -1478,8 +1569,9 @@ int prepare_adjoint_ck(mgd_region* r) {
   if (na > 0 && r->grad.alloc((size_t)r->ninst * nst * na)) return MGD_ERR_HIP;
   if (fused_ok(r)) {
     if (int rc = prepare_quadrature(r, nst, r->ck.dt)) return rc;
-    if (r->ly.alloc(M) || r->l0.alloc(M) || r->window2.alloc((size_t)4 * r->ck.K * M) ||
-        r->b2r.alloc(M) || r->qsr.alloc(M))
+    if (r->ly.alloc(M) || r->l0.alloc(M) ||
+        r->window2.alloc((size_t)(r->step_mode ? 3 : 4 * r->ck.K) * M) || r->b2r.alloc(M) ||
+        r->qsr.alloc(M))
       return MGD_ERR_HIP;
   }
   return MGD_OK;
@@ -2311,14 +2403,14 @@ int mgd_region_get_march_shape(mgd_region* r, int* shape) {
 int mgd_region_get_checkpoint_plan(mgd_region* r, int* plan) {
   if (!r || !plan) { set_error("null argument"); return MGD_ERR_ARG; }
   if (!r->ck.valid && r->ck.nsteps == 0) { set_error("no checkpointed forward run prepared"); return MGD_ERR_STATE; }
-  if (fused_ok(r)) {
-    plan[0] = 2;
-    plan[1] = r->resident ? 4 * r->ck.nsteps + 1 : 0;
-    plan[2] = 0;
-  } else if (r->step_mode) {
+  if (r->step_mode) {
     plan[0] = 1;
     plan[1] = r->nscache;
     plan[2] = r->scache_first;
+  } else if (fused_ok(r)) {
+    plan[0] = 2;
+    plan[1] = r->resident ? 4 * r->ck.nsteps + 1 : 0;
+    plan[2] = 0;
   } else {
     plan[0] = 0;
     plan[1] = r->ncache;
@@ -2738,6 +2830,7 @@ int mgd_checkpoint_migrate(mgd_region* r, int timestep, int stage) {
         // the recompute needs the run's actuator forcing and source amplitudes
         MGD_HIP(hipStreamSynchronize(r->stream));
         if (int rc = upload_run_inputs(r)) return rc;
+        if (fused_ok(r) && r->b2r.alloc(M)) return MGD_ERR_HIP;
       }
       src = step_stage_state(r, t, st + 1, dt, rec);
     }

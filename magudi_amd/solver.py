@@ -301,8 +301,9 @@ class Region:
     def checkpoint_plan(self) -> dict:
         """Storage plan of the last checkpointed forward run: {"mode": "step store",
         "held": steps whose inner stage states are kept, "first": the first of them},
-        {"mode": "windows", "held": windows kept, "first": their first step} or
-        {"mode": "fused", "held": resident stage states (0: window recompute)}."""
+        {"mode": "windows", "held": windows kept, "first": their first step} (the generic
+        kernels) or {"mode": "fused", "held": resident stage states (0: window recompute)}
+        (the fused 2-D path without the step store)."""
         out = (C.c_int * 3)()
         check(capi.lib().mgd_region_get_checkpoint_plan(self.handle, out))
         mode = {0: "windows", 1: "step store", 2: "fused"}[out[0]]

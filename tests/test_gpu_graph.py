@@ -56,18 +56,20 @@ def test_adjoint_graph_replays(plain):
 
 
 @pytest.mark.parametrize("plan", ["step store, every step recomputed", "step store, 3 steps held",
-                                  "windows", "fused 2-D windows"])
+                                  "windows", "fused 2-D windows", "fused 2-D step store, 3 steps held"])
 def test_adjoint_graph_replays_with_recompute(plan):
     """Replays of runs that recompute stage states, with the control gradient: the step
     store and snapshots with windows on the generic kernels (a 1-D periodic wave with
-    cost target and actuator), and the fused 2-D window recompute (each recompute stage in
-    one launch with an adjoint stage) -- bit for bit against eager."""
+    cost target and actuator), and the fused 2-D window recompute and step store (each
+    recompute stage in one launch with an adjoint stage) -- bit for bit against eager."""
     from magudi_amd.solver import Region, Solver
     print(f"graph replay: {plan}", flush=True)
     if plan.startswith("fused"):
         c = cases.acoustic_monopole(n=41, nsteps=8, save_interval=3)
         c.dt = 0.2
         R = Region(c, resident_states=False)
+        R.set_step_store(0 if plan.endswith("windows") else 1)
+        R.set_window_cache(1 if plan.endswith("held") else 0)
     else:
         c = cases.periodic_wave_1d(n=256, nsteps=8, save_interval=3)
         R = Region(c)

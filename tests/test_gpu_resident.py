@@ -53,6 +53,8 @@ def test_window_counts_match_resident(nsteps, graph):
     out = []
     for resident in (True, False):
         R = Region(c, resident_states=resident)
+        if not resident:
+            R.set_step_store(0)
         ctl = 1e-2 * np.random.default_rng(nsteps).standard_normal((4 * nsteps, R.n_actuator))
         S = Solver(R, dt=c.dt, nsteps=nsteps, save_interval=c.save_interval)
         for rep in range(2 if graph else 1):
@@ -65,6 +67,48 @@ def test_window_counts_match_resident(nsteps, graph):
                 out.append(res)
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("ninst", [1, 3])
+def test_fused_step_store_matches_resident(ninst):
+    """The fused 2-D path with the step store (MGD_OPT_STEP_STORE): the state at the start
+    of every step kept, the inner stage states of none, of the last save_interval or of
+    every step held, the others recomputed three stages per step in the launches of the
+    adjoint stages -- against the resident stage states and the window recompute, bit for
+    bit, eager and as two graph replays."""
+    from magudi_amd.solver import Region, Solver
+    c = cases.acoustic_monopole(n=41, nsteps=9, save_interval=4)
+    c.dt = 0.2
+    q0 = c.initial_state()
+    plans = [("resident", None, None, {"mode": "fused", "held": 37, "first": 0}),
+             ("windows", 0, 0, {"mode": "fused", "held": 0, "first": 0}),
+             ("step store", 1, 0, {"mode": "step store", "held": 0, "first": 9}),
+             ("step store", 1, 1, {"mode": "step store", "held": 4, "first": 5}),
+             ("step store", 1, -1, {"mode": "step store", "held": 9, "first": 0})]
+    ref = None
+    for name, step, cache, want in plans:
+        print(f"fused plan {name} step={step} cache={cache} ninst={ninst}", flush=True)
+        R = Region(c, n_instances=ninst, resident_states=name == "resident")
+        if step is not None:
+            R.set_step_store(step)
+            R.set_window_cache(cache)
+        na = R.n_actuator
+        rng = np.random.default_rng(19)
+        ctl = np.stack([1e-2 * (k + 1) * rng.standard_normal((4 * c.nsteps, na)) for k in range(ninst)])
+        S = Solver(R, dt=c.dt, nsteps=c.nsteps, save_interval=c.save_interval)
+        for graph in (False, True, True):
+            for i in range(ninst):
+                R.states[i].conservedVariables = q0
+            sens, grad, J = S.runAdjoint(control=ctl, use_graph=graph)
+            lam = np.stack([R.states[i].adjointVariables for i in range(ninst)])
+            res = (sens.copy(), grad.copy(), J.copy(), lam)
+            assert R.checkpoint_plan == want, (name, R.checkpoint_plan)
+            if ref is None:
+                ref = res
+                assert np.abs(lam).max() > 0 and np.all(np.abs(sens) > 0)
+            for a, b in zip(res, ref):
+                assert np.array_equal(a, b), (name, step, cache, graph)
+        R.close()
 
 
 def test_split_passes_and_repeated_adjoint():

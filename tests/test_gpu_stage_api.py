@@ -87,32 +87,42 @@ def test_stagewise_run_matches_whole_run(plain, walled):
         assert np.allclose(sens, sens_run, rtol=1e-12, atol=0.0)
 
 
-@pytest.mark.parametrize("step,cache", [(-1, -1), (0, -1), (0, 0), (1, 0), (1, 1)])
-def test_migrate_serves_every_stage_state(step, cache):
+@pytest.mark.parametrize("step,cache,fused", [(-1, -1, False), (0, -1, False), (0, 0, False),
+                                              (1, 0, False), (1, 1, False), (1, 0, True),
+                                              (1, 1, True), (0, 0, True)])
+def test_migrate_serves_every_stage_state(step, cache, fused):
     """migrateTo in any order, across windows, against the stage states of a run that keeps
     every stage (MGD_OPT_RESIDENT_STATES is not needed: a whole forward run with the
     stage outputs collected by substeps); with the step store (MGD_OPT_STEP_STORE, every
     step's inner states held, none, or those of the last save_interval steps) and with
-    snapshots and windows (every window cached, none)."""
+    snapshots and windows (every window cached, none), on the generic kernels and on the
+    fused 2-D stage kernels (stage states not resident)."""
     from magudi_amd.solver import Region, RK4Integrator, Solver, UniformCheckpointer
     c = _case()
     q0 = c.initial_state()
-    R = Region(c, force_plain=True)
+    R = Region(c, force_plain=True) if not fused else Region(c, resident_states=False)
+    assert (R.tile_shape >= 0) == fused
     R.set_step_store(step)
     R.set_window_cache(cache)
     R.states[0].conservedVariables = q0
     ti = RK4Integrator()
     states = {(0, 4): q0}
     time = 0.0
-    for step in range(1, c.nsteps + 1):
+    for ts in range(1, c.nsteps + 1):
         for i in range(1, 5):
             if i in (2, 4):
                 time = time + c.dt / 2.0
-            ti.substepForward(R, time, c.dt, step, i)
-            states[(step, i)] = R.states[0].conservedVariables
+            ti.substepForward(R, time, c.dt, ts, i)
+            states[(ts, i)] = R.states[0].conservedVariables
     R.states[0].conservedVariables = q0
     S = Solver(R, dt=c.dt, nsteps=c.nsteps, save_interval=c.save_interval)
     S.runForwardCheckpointed()
+    plan = R.checkpoint_plan
+    if step == 1:
+        assert plan == {"mode": "step store", "held": 0 if cache == 0 else c.save_interval,
+                        "first": c.nsteps - (0 if cache == 0 else c.save_interval)}, plan
+    elif fused:
+        assert plan == {"mode": "fused", "held": 0, "first": 0}, plan
     mig = UniformCheckpointer()
     order = list(states)
     np.random.default_rng(9).shuffle(order)
