@@ -392,12 +392,15 @@ int mgd_kernel_timer_read(mgd_region* r, long* launches, long* with_copy, double
 /* Region groups: regions of different sizes run in lock step with one kernel launch per
  * RK stage for all of them (the uneven batches of the multiple-shooting and ensemble
  * workflows; no counterpart in the reference, which runs them as separate jobs).
- * Members: finalized 2-D regions on the fused path, each fitting 16 x 16 tiles, without
- * resident stage states; any number of instances each.  mgd_group_run_adjoint is
- * mgd_run_adjoint of every member (same nsteps and save_interval, own dt[k] and
- * controls[k]); gradients / sensitivities / Js hold one output array per member (or
- * null).  Results equal the members' own runs.  Members keep their streams and tile
- * shapes after the run. */
+ * Members: finalized 2-D regions on the fused path, each fitting 16 x 16 tiles; any
+ * number of instances each.  mgd_group_run_adjoint is mgd_run_adjoint of every member
+ * (same nsteps and save_interval, own dt[k] and controls[k]); gradients / sensitivities /
+ * Js hold one output array per member (or null).  With MGD_OPT_RESIDENT_STATES = 1 on
+ * every member each keeps all its stage states resident (no recompute; an error where
+ * they cannot be allocated); otherwise (the option on some members only is an error)
+ * every member keeps snapshots and the windows are recomputed on a second stream, which
+ * costs little where the members are small.  Results equal the members' own runs.
+ * Members keep their streams and tile shapes after the run. */
 typedef struct mgd_group mgd_group;
 int mgd_group_create(mgd_region* const* regions, int n, mgd_group** group);
 int mgd_group_destroy(mgd_group* group);

@@ -125,6 +125,7 @@ struct mgd_region {
   bool resident = false;                 // this run's choice
   int opt_resident = -1;                 // -1 where every stage state fits, 0 off, 1 on
   bool group_run = false;                // preparing as a member of a group launch
+  bool group_resident = false;           // ... whose members keep every stage state
   mgd::DevBuf wall;
   // the last checkpointed forward pass left the stage states of its last window in
   // `window` (the adjoint pass then starts without recomputing that window)
@@ -1513,11 +1514,12 @@ int prepare_forward_ck(mgd_region* r, double t0, double dt, int nsteps, int K, c
     const int nsnap = nsteps / K + 1;
     r->step_mode = false;
     // every stage state resident (MGD_OPT_RESIDENT_STATES): asked for, or by default where
-    // they fit in storage_budget (not for members of a group launch, which recompute);
-    // otherwise the step store or snapshots and windows, as on the generic kernels
+    // they fit in storage_budget (the members of a group launch: where every member asks
+    // for it); otherwise the step store or snapshots and windows, as on the generic kernels
     const size_t need = (size_t)(4L * nsteps + 1) * M;
     bool res = r->opt_resident == 1;
-    if (r->opt_resident < 0 && !r->group_run) res = need * sizeof(double) <= storage_budget(r);
+    if (r->group_run) res = r->group_resident;
+    else if (r->opt_resident < 0) res = need * sizeof(double) <= storage_budget(r);
     if (res) {
       r->sstore.release();
       r->scache.release();
@@ -1735,20 +1737,26 @@ int group_tables(mgd_group* G, double dt0, const double* dts, int nsteps, int K,
     // forward pass: levels in q, stage states in b1 / qy (fused_forward_step)
     const double* qin[4] = {r->q.p, r->b1.p, r->qy.p, r->b1.p};
     double* qo[4] = {r->b1.p, r->qy.p, r->b1.p, r->q.p};
+    // resident members: every stage state in `wall` (state 4 t + i - 1 the input of stage
+    // i of step t), written by the forward stages and read by the adjoint stages; no
+    // recompute
     double* wbuf[2] = {r->window.p, r->window2.p};
     for (int t = 0; t < nsteps; ++t) {
       const int w = t / K, t0w = w * K, t1w = std::min(nsteps, t0w + K);
-      double* W = wbuf[w % 2];
-      const long base = 4L * (t - t0w);
+      double* W = r->resident ? r->wall.p : wbuf[w % 2];
+      const long base = r->resident ? 4L * t : 4L * (t - t0w);
       double* outs[4];
       for (int i = 0; i < 4; ++i) outs[i] = W + (base + 1 + i) * M;
-      if (t == t1w - 1) outs[3] = r->qsr.p;
+      if (t == t1w - 1 && !r->resident) outs[3] = r->qsr.p;
       for (int i = 1; i <= 4; ++i) {
         const int sidx = 4 * t + i - 1;
+        const double* in_w = i == 1 ? W + base * M : outs[i - 2];
         fwd[(size_t)sidx * n + k] =
-            fwd_args(r, qin[i - 1], r->q.p, r->b2.p, qo[i - 1], i, dt, sidx, nst, true);
-        rec[(size_t)sidx * n + k] = fwd_args(r, i == 1 ? W + base * M : outs[i - 2], W + base * M,
-                                             r->b2r.p, outs[i - 1], i, dt, sidx, nst, false);
+            r->resident ? fwd_args(r, in_w, W + base * M, r->b2.p, outs[i - 1], i, dt, sidx, nst, true)
+                        : fwd_args(r, qin[i - 1], r->q.p, r->b2.p, qo[i - 1], i, dt, sidx, nst, true);
+        rec[(size_t)sidx * n + k] =
+            fwd_args(r, in_w, W + base * M, r->b2r.p, r->resident ? r->qsr.p : outs[i - 1], i, dt,
+                     sidx, nst, false);
         AdjStage a;
         std::memset(&a, 0, sizeof a);
         double* L0 = r->lam.p;
@@ -1799,7 +1807,7 @@ int group_forward_pass(mgd_group* G, int nsteps, int K) {
   const GroupMember* mem = reinterpret_cast<const GroupMember*>(G->dmem.p);
   for (mgd_region* r : G->m) {
     launch_fill(r->ninst, 0.0, r->acc_J.p, G->st);
-    launch_copy((long)r->ninst * M_of(r), r->q.p, r->snaps.p, G->st);
+    launch_copy((long)r->ninst * M_of(r), r->q.p, r->resident ? r->wall.p : r->snaps.p, G->st);
   }
   for (int s = 0; s < nsteps; ++s) {
     for (int i = 1; i <= 4; ++i) {
@@ -1811,11 +1819,14 @@ int group_forward_pass(mgd_group* G, int nsteps, int K) {
     }
     if ((s + 1) % K == 0)
       for (mgd_region* r : G->m) {
+        if (r->resident) continue;
         const long M = (long)r->ninst * M_of(r);
         launch_copy(M, r->q.p, r->snaps.p + (long)((s + 1) / K) * M, G->st);
       }
   }
   for (mgd_region* r : G->m) {
+    const long M = (long)r->ninst * M_of(r);
+    if (r->resident) launch_copy(M, r->wall.p + 4L * nsteps * M, r->q.p, G->st);   // final state
     if (n_cost(r) > 0)
       launch_stage_accumulate(r->ssum.p, nst, r->wts.p, r->acc_J.p, r->ninst, 0, G->st);
     launch_copy((long)r->ninst * M_of(r), r->q.p, r->qT.p, G->st);
@@ -1823,12 +1834,15 @@ int group_forward_pass(mgd_group* G, int nsteps, int K) {
   return MGD_OK;
 }
 
-// fused_adjoint_pass of every member (terminal condition zero), windows recomputed on the
-// group's second stream, one launch per stage for all members.
+// fused_adjoint_pass of every member (terminal condition zero), one launch per stage for
+// all members: the members' resident stage states read in place, or their windows
+// recomputed on the group's second stream.
 int group_adjoint_pass(mgd_group* G, const double* dts, int nsteps, int K) {
   const int n = (int)G->m.size(), nst = 4 * nsteps;
   const GroupMember* mem = reinterpret_cast<const GroupMember*>(G->dmem.p);
-  const int nw = (nsteps + K - 1) / K;
+  const bool resident = G->m[0]->resident;   // the same for every member
+  const int nw = resident ? 1 : (nsteps + K - 1) / K;
+  if (resident) K = nsteps;                  // one window: the resident states
   for (mgd_region* r : G->m) launch_fill(r->ninst, 0.0, r->acc_S.p, G->st);
   MGD_HIP(hipEventRecord(G->ev_fwd, G->st));
   MGD_HIP(hipStreamWaitEvent(G->st_rec, G->ev_fwd, 0));
@@ -1845,8 +1859,10 @@ int group_adjoint_pass(mgd_group* G, const double* dts, int nsteps, int K) {
           return rc;
     return MGD_OK;
   };
-  if (int rc = recompute(nw - 1)) return rc;
-  MGD_HIP(hipEventRecord(G->ev_ready[(nw - 1) % 2], G->st_rec));
+  if (!resident) {
+    if (int rc = recompute(nw - 1)) return rc;
+    MGD_HIP(hipEventRecord(G->ev_ready[(nw - 1) % 2], G->st_rec));
+  }
   for (int k = 0; k < n; ++k) {
     mgd_region* r = G->m[k];
     const long M = (long)r->ninst * M_of(r);
@@ -1865,7 +1881,7 @@ int group_adjoint_pass(mgd_group* G, const double* dts, int nsteps, int K) {
       if (int rc = recompute(w - 1)) return rc;
       MGD_HIP(hipEventRecord(G->ev_ready[(w - 1) % 2], G->st_rec));
     }
-    MGD_HIP(hipStreamWaitEvent(G->st, G->ev_ready[w % 2], 0));
+    if (!resident) MGD_HIP(hipStreamWaitEvent(G->st, G->ev_ready[w % 2], 0));
     const int t0w = w * K, t1w = std::min(nsteps, t0w + K);
     for (int t = t1w - 1; t >= t0w; --t)
       for (int i = 4; i >= 1; --i) {
@@ -1875,7 +1891,7 @@ int group_adjoint_pass(mgd_group* G, const double* dts, int nsteps, int K) {
           return rc;
         for (mgd_region* r : G->m) reduce_chunk(r, r->spart, n_act(r), nst, sidx, true);
       }
-    MGD_HIP(hipEventRecord(G->ev_free[w % 2], G->st));
+    if (!resident) MGD_HIP(hipEventRecord(G->ev_free[w % 2], G->st));
   }
   for (mgd_region* r : G->m)
     if (n_act(r) > 0)
@@ -3178,11 +3194,20 @@ int mgd_group_run_adjoint(mgd_group* G, double t0, const double* dt, int nsteps,
   for (int k = 0; k < n; ++k) {
     mgd_region* r = G->m[k];
     if (!(dt[k] > 0.0)) { set_error("bad time step"); return MGD_ERR_ARG; }
-    if (!fused_ok(r) || r->opt_resident == 1) {
-      set_error("group members must run on the fused 2-D path without resident stage states");
+    if (!fused_ok(r)) {
+      set_error("group members must run on the fused 2-D path");
       return MGD_ERR_UNSUPPORTED;
     }
   }
+  // every member's stage states resident where every member asks for it
+  // (MGD_OPT_RESIDENT_STATES = 1), else every member on snapshots and windows
+  int nres = 0;
+  for (mgd_region* r : G->m) nres += r->opt_resident == 1;
+  if (nres != 0 && nres != n) {
+    set_error("MGD_OPT_RESIDENT_STATES = 1 on some members of a group but not on all");
+    return MGD_ERR_ARG;
+  }
+  const bool gres = nres == n;
   // every member works on the group's streams for the duration of the run
   std::vector<hipStream_t> streams;
   std::vector<int> shapes;
@@ -3196,6 +3221,7 @@ int mgd_group_run_adjoint(mgd_group* G, double t0, const double* dt, int nsteps,
     for (int k = 0; k < n; ++k) {
       mgd_region* r = G->m[k];
       r->group_run = false;
+      r->group_resident = false;
       r->stream = streams[k];
       if (r->tiling.shape != shapes[k]) (void)install_tiling(r, shapes[k]);
     }
@@ -3203,6 +3229,7 @@ int mgd_group_run_adjoint(mgd_group* G, double t0, const double* dt, int nsteps,
   auto run = [&]() -> int {
     for (int k = 0; k < n; ++k) {
       mgd_region* r = G->m[k];
+      r->group_resident = gres;
       if (int rc = prepare_forward_ck(r, t0, dt[k], nsteps, save_interval, controls ? controls[k] : nullptr))
         return rc;
       r->ck.valid = true;

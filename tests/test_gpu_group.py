@@ -59,11 +59,14 @@ def test_group_member_matches_oracle(which):
     assert np.abs(g[0] - go).max() <= 1e-10 * np.abs(go).max() + 1e-300
 
 
-def test_merged_group_equals_single_runs():
+@pytest.mark.parametrize("resident", [True, False])
+def test_merged_group_equals_single_runs(resident):
     """One launch per RK stage for all fused 2-D members (mgd_group_run_adjoint), the 1-D
     member beside them: J, sensitivity and gradient of every member equal its own run
-    bit for bit, with a control on one member and two instances on another; the members'
-    tile shapes are back to their own after the run."""
+    bit for bit, with a control on one member and two instances on another, the members'
+    stage states resident (MGD_OPT_RESIDENT_STATES 1 on every member) or on snapshots and
+    windows (the default); the members' tile shapes are back to their own after the run."""
+    from magudi_amd import capi
     from magudi_amd.group import RegionGroup
     from magudi_amd.solver import Region, Solver
     cs = small_batch()
@@ -71,12 +74,19 @@ def test_merged_group_equals_single_runs():
     G = RegionGroup(cs, n_instances=ninst)
     shapes = [r.tile_shape for r in G.regions]
     assert G._merged_members() == [1, 2, 3]
+    if resident:
+        for k in (1, 2, 3):
+            capi.check(capi.lib().mgd_region_set_option(G.regions[k].handle, capi.OPT_RESIDENT_STATES, 1))
     rng = np.random.default_rng(17)
     na = G.regions[2].n_actuator
     controls = [None, None, 1e-2 * rng.standard_normal((2, 4 * cs[2].nsteps, na)), None]
     for rep in range(2):   # second run reuses the group and its tables
         G.restart()
         out = G.run_adjoint_merged(controls=controls)
+        for k in (1, 2, 3):
+            plan = G.regions[k].checkpoint_plan
+            assert plan == {"mode": "fused", "held": 4 * cs[k].nsteps + 1 if resident else 0,
+                            "first": 0}, (k, plan)
         for k, c in enumerate(cs):
             R = Region(c, n_instances=ninst[k])
             for i in range(R.n_instances):
@@ -86,6 +96,11 @@ def test_merged_group_equals_single_runs():
             assert np.array_equal(J, J1) and np.array_equal(s, s1), (rep, c.name)
             assert (g is None and g1 is None) or np.array_equal(g, g1), (rep, c.name)
     assert [r.tile_shape for r in G.regions] == shapes
+    if resident:
+        # resident on some members but not all: refused
+        capi.check(capi.lib().mgd_region_set_option(G.regions[2].handle, capi.OPT_RESIDENT_STATES, -1))
+        with pytest.raises(capi.MagudiError):
+            G.run_adjoint_merged(controls=controls)
     G.close()
 
 
