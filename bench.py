@@ -59,6 +59,9 @@ def parse():
                     help="only the 3-D viscous rhs leg (its profiling workload)")
     ap.add_argument("--monopole-only", action="store_true",
                     help="only the 401^2 monopole forward + adjoint legs")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched 401^2 monopole forward + adjoint (the step store; "
+                         "its profiling workload)")
     ap.add_argument("--tile-shape", type=int, default=-1,
                     help="tile shape of the fused 2-D kernels (-1: autotune)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -555,6 +558,26 @@ def monopole_legs(a, extras):
     return out
 
 
+def batched_leg(a):
+    """The batched monopole run alone (a.batch instances of 401^2, 960 steps, with the
+    control gradient): one warm-up run, one timed; its storage plan."""
+    from magudi_amd import cases
+    from magudi_amd.solver import Region, Solver
+    case = cases.acoustic_monopole(n=a.n, nsteps=a.nsteps, save_interval=a.save_interval)
+    q0 = case.initial_state()
+    Rb = Region(case, n_instances=a.batch)
+    for i in range(a.batch):
+        Rb.states[i].initialCondition = q0
+    Sb = Solver(Rb)
+    run_hot_path(Rb, Sb, 1)
+    tb, _ = run_hot_path(Rb, Sb, 1)
+    out = {"instances": a.batch, "value": a.batch * case.npoints * a.nsteps / tb,
+           "ms_per_run": tb * 1e3, "checkpoint_plan": Rb.checkpoint_plan,
+           "tile_shape": Rb.tile_shape}
+    Rb.close()
+    return out
+
+
 def checkpoint_plans_leg(n, nsteps, t_step, J):
     """The headline pass with snapshots every nsteps / 8 steps and windows (the window cache
     holding the last windows free memory allows) in place of the step store: same J, the
@@ -604,6 +627,9 @@ def main():
         return 0
     if a.monopole_only:
         print(json.dumps({"monopole401": monopole_legs(a, extras=False)}), flush=True)
+        return 0
+    if a.batched_only:
+        print(json.dumps({"batched": batched_leg(a)}), flush=True)
         return 0
     if a.kernels_only:
         print(json.dumps({"kernels_only": kernels_only(a)}), flush=True)
