@@ -473,14 +473,21 @@ def monopole_legs(a, extras):
     pts = case.npoints * a.nsteps
     value = pts / t_step
     plan = R.checkpoint_plan
-    resident = plan["held"] > 0
-    job_b = 4 * (320 if resident else 448)
+    resident = plan["mode"] == "fused" and plan["held"] > 0
+    step_store = plan["mode"] == "step store"
+    # compulsory bytes per grid-point step: 4 x 320 with every stage state resident, 4 x 448
+    # with windows (each stage recomputed once, 128 B), 4 x 320 + 3 x 128 per step not held
+    # on the step store
+    job_b = (4 * 320 + 3 * 128 * (a.nsteps - plan["held"]) / a.nsteps if step_store
+             else 4 * (320 if resident else 448))
     R.restart()
     fwd_ms, fwd_b = R.time_kernel(capi.KERNEL_FWD_STAGE, reps=50)
     adj_ms, adj_b = R.time_kernel(capi.KERNEL_ADJ_STAGE, reps=50)
     out = {"workload": f"monopole {a.n}x{a.n} forward+adjoint with the control gradient, "
                        f"{a.nsteps} RK4 steps, " +
                        ("every RK stage state resident in HBM" if resident else
+                        f"the state of every step and the stage states of {plan['held']} steps "
+                        f"in HBM, the others recomputed" if step_store else
                         f"HBM snapshots every {a.save_interval} steps with recompute"),
            "checkpoint_plan": plan,
            "value": value, "unit": "grid-pt-steps/s", "ms_per_run": t_step * 1e3,
