@@ -99,6 +99,8 @@ struct mgd_region {
   // stages, stage sums, quadrature weights
   mgd::DevBuf qy, ly, l0, jpart, spart, ssum, wts;
   int chunk = 0;                         // stages per chunk of quadrature terms
+  int wts_nst = -1;                      // stages and dt of the quadrature weights held
+  double wts_dt = 0.0;
   mgd::HostTiling htile;
   mgd::Tiling tiling{};
   mgd::DevBuf d_tile_mask;
@@ -699,9 +701,15 @@ int setup_fused(mgd_region* r) {
 constexpr int kChunkStages = 128;   // stages of quadrature terms held before their reduction
 
 int prepare_quadrature(mgd_region* r, int nst, double dt) {
-  std::vector<double> w((size_t)nst);
-  for (int s = 0; s < nst; ++s) w[s] = kRkNorm[s % 4] * dt;
-  if (upload(r->wts, w)) return MGD_ERR_HIP;
+  // the weights depend on nst and dt only: uploaded again when either changes
+  if (r->wts_nst != nst || r->wts_dt != dt || !r->wts.p) {
+    std::vector<double> w((size_t)nst);
+    for (int s = 0; s < nst; ++s) w[s] = kRkNorm[s % 4] * dt;
+    r->wts_nst = -1;
+    if (upload(r->wts, w)) return MGD_ERR_HIP;
+    r->wts_nst = nst;
+    r->wts_dt = dt;
+  }
   if (r->ssum.alloc((size_t)r->ninst * std::max(nst, 1))) return MGD_ERR_HIP;
   r->chunk = std::max(1, std::min(nst, kChunkStages));
   const int nc = n_cost(r), na = n_act(r);
