@@ -104,6 +104,12 @@ struct mgd_region {
   mgd::HostTiling htile;
   mgd::Tiling tiling{};
   mgd::DevBuf d_tile_mask;
+  // the 16 x 16 tiling of group launches, held beside the region's own (built at the first
+  // group run; the two are swapped for each group run)
+  mgd::HostTiling g_htile;
+  mgd::Tiling g_tiling{};
+  mgd::DevBuf g_tile_mask;
+  bool g_built = false, g_swapped = false;
   bool tiling_ok = false;
   int tile_shape = -1;                   // MGD_OPT_TILE_SHAPE: -1 autotune, else fixed
   bool tuned = false;
@@ -658,25 +664,39 @@ int upload_ints(DevBuf& b, const void* data, size_t bytes) {
 }
 
 // Installs the tiling of one tile shape (host and device views).
-int install_tiling(mgd_region* r, int shape) {
+// A tiling of `shape` for the region: the host partition in H, its patch masks uploaded to
+// `mask`, the launch descriptor in T.
+int make_tiling(mgd_region* r, int shape, HostTiling& Hout, Tiling& T, DevBuf& mask) {
   HostTiling H;
   if (fused2d_make_tiling(r->geo, r->hpatches, shape, &H) != MGD_OK) return MGD_ERR_UNSUPPORTED;
-  if (upload_ints(r->d_tile_mask, H.pmask.data(), H.pmask.size() * sizeof(unsigned)))
-    return MGD_ERR_HIP;
-  r->htile = H;
-  r->tiling.shape = H.shape;
-  r->tiling.nbx = H.nbx;
-  r->tiling.nby = H.nby;
-  r->tiling.nblk = H.nblk;
-  r->tiling.band = H.band;
-  r->tiling.qx = H.qx;
-  r->tiling.lx = H.lx;
-  r->tiling.rx = H.rx;
-  r->tiling.qy = H.qy;
-  r->tiling.ly = H.ly;
-  r->tiling.ry = H.ry;
-  r->tiling.pmask = reinterpret_cast<const unsigned*>(r->d_tile_mask.p);
+  if (upload_ints(mask, H.pmask.data(), H.pmask.size() * sizeof(unsigned))) return MGD_ERR_HIP;
+  Hout = H;
+  T.shape = H.shape;
+  T.nbx = H.nbx;
+  T.nby = H.nby;
+  T.nblk = H.nblk;
+  T.band = H.band;
+  T.qx = H.qx;
+  T.lx = H.lx;
+  T.rx = H.rx;
+  T.qy = H.qy;
+  T.ly = H.ly;
+  T.ry = H.ry;
+  T.pmask = reinterpret_cast<const unsigned*>(mask.p);
   return MGD_OK;
+}
+
+int install_tiling(mgd_region* r, int shape) {
+  return make_tiling(r, shape, r->htile, r->tiling, r->d_tile_mask);
+}
+
+// Exchanges the region's tiling with the group tiling held beside it.
+void swap_tilings(mgd_region* r) {
+  std::swap(r->htile, r->g_htile);
+  std::swap(r->tiling, r->g_tiling);
+  std::swap(r->d_tile_mask.p, r->g_tile_mask.p);
+  std::swap(r->d_tile_mask.n, r->g_tile_mask.n);
+  r->g_swapped = !r->g_swapped;
 }
 
 int setup_fused(mgd_region* r) {
@@ -3231,6 +3251,7 @@ int mgd_group_run_adjoint(mgd_group* G, double t0, const double* dt, int nsteps,
       r->group_run = false;
       r->group_resident = false;
       r->stream = streams[k];
+      if (r->g_swapped) swap_tilings(r);
       if (r->tiling.shape != shapes[k]) (void)install_tiling(r, shapes[k]);
     }
   };
@@ -3242,9 +3263,14 @@ int mgd_group_run_adjoint(mgd_group* G, double t0, const double* dt, int nsteps,
         return rc;
       r->ck.valid = true;
       if (int rc = prepare_adjoint_ck(r)) return rc;
-      if (r->tiling.shape != kGroupShape && install_tiling(r, kGroupShape) != MGD_OK) {
-        set_error("a group member does not fit the 16 x 16 tiles of a group launch");
-        return MGD_ERR_UNSUPPORTED;
+      if (r->tiling.shape != kGroupShape) {
+        // the group tiling, built once per region, in place of the region's own
+        if (!r->g_built && make_tiling(r, kGroupShape, r->g_htile, r->g_tiling, r->g_tile_mask) != MGD_OK) {
+          set_error("a group member does not fit the 16 x 16 tiles of a group launch");
+          return MGD_ERR_UNSUPPORTED;
+        }
+        r->g_built = true;
+        swap_tilings(r);
       }
     }
     const bool want_grad = gradients != nullptr;
