@@ -1590,12 +1590,16 @@ int enqueue_forward_ck(mgd_region* r) {
   return MGD_OK;
 }
 
-// Adjoint pass from the snapshots of the last checkpointed forward pass.
-int prepare_adjoint_ck(mgd_region* r) {
+// Adjoint pass from the snapshots of the last checkpointed forward pass.  inputs_fresh:
+// the run's inputs were uploaded by prepare_forward_ck just before (a group run), so they
+// are not uploaded again.
+int prepare_adjoint_ck(mgd_region* r, bool inputs_fresh = false) {
   if (!r->ck.valid) { set_error("no checkpointed forward run to march back from"); return MGD_ERR_STATE; }
   const int na = n_act(r), nst = 4 * r->ck.nsteps;
   const long M = (long)r->ninst * M_of(r);
-  if (int rc = upload_run_inputs(r)) return rc;
+  if (!inputs_fresh) {
+    if (int rc = upload_run_inputs(r)) return rc;
+  }
   if (na > 0 && r->grad.alloc((size_t)r->ninst * nst * na)) return MGD_ERR_HIP;
   if (fused_ok(r)) {
     if (int rc = prepare_quadrature(r, nst, r->ck.dt)) return rc;
@@ -3262,7 +3266,7 @@ int mgd_group_run_adjoint(mgd_group* G, double t0, const double* dt, int nsteps,
       if (int rc = prepare_forward_ck(r, t0, dt[k], nsteps, save_interval, controls ? controls[k] : nullptr))
         return rc;
       r->ck.valid = true;
-      if (int rc = prepare_adjoint_ck(r)) return rc;
+      if (int rc = prepare_adjoint_ck(r, true)) return rc;
       if (r->tiling.shape != kGroupShape) {
         // the group tiling, built once per region, in place of the region's own
         if (!r->g_built && make_tiling(r, kGroupShape, r->g_htile, r->g_tiling, r->g_tile_mask) != MGD_OK) {
